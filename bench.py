@@ -1,0 +1,177 @@
+"""Benchmark: FP64 AMR multigrid Poisson solve on one MI355X.
+
+Workload (one "step" = one full solve to a 1e-10 relative max-norm residual from a
+zero initial guess, inputs resident in HBM): 3-level AMR, base 256^3, refinement
+ratio 2, centred refined regions covering 25 % then 6 % of the parent domain
+(320^3 and 400^3 fine regions), 64^3 maximum boxes, homogeneous Dirichlet BCs,
+alpha = 0, beta = 1, Gaussian-bump rhs (seed 1002); V-cycles with 2 pre / 2 post
+GSRB sweeps and a BiCGStab bottom solve.
+
+Prints one JSON line.  value = Mcell-relax/s (cells x red-black GSRB iterations
+over every level relaxed, per second of solve wall time).  roofline = the GSRB
+kernel's algorithmic bytes (24 B per cell-relax in FP64) over its HIP-event time
+inside the timed region, against 8 TB/s HBM.  cpu_baseline = the oracle (a C
+restatement of the reference's FP64 path, OpenMP over boxes) timed on a bounded
+sample of the same workload: setup excluded, initial residual + one AMR V-cycle +
+residual.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "chombo_3.2_amd"))
+
+METRIC = "Mcell-relax/s & %HBM roofline; solves/s + p99 latency at 1e-10, MP vs FP64"
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def read_traffic():
+    """HBM bytes per GSRB launch from the committed rocprofv3 PMC summary, if any."""
+    p = os.path.join(ROOT, "profiles", "gsrb_c2_traffic.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            d = json.load(f)
+        return d.get("hbm_bytes_per_launch"), d
+    return None, None
+
+
+def cpu_baseline(hier, rhs, sp, rp):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle, build as build_oracle, _LIB_PATH
+    if not os.path.exists(_LIB_PATH):
+        build_oracle()
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t0 = time.time()
+    o = Oracle(hier)
+    t_setup = time.time() - t0
+    sp1 = sp.copy()
+    sp1[4] = 1  # iterMax = 1: initial residual, one V-cycle, residual
+    Oracle.reset_counters()
+    t0 = time.time()
+    o.solve(rhs, sp1, rp)
+    dt = time.time() - t0
+    cr = Oracle.cell_relax_count()
+    o.close()
+    return dict(value=cr / dt / 1e6, unit="Mcell-relax/s", cores=threads, kind="port",
+                sample=f"same 3-level 256^3 hierarchy and rhs; initial residual + 1 AMR V-cycle + residual "
+                       f"({cr / 1e6:.1f} M cell-relaxations in {dt:.2f} s; oracle setup {t_setup:.1f} s "
+                       f"excluded); OMP threads={threads}")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--base", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only; multi-GPU runs are not part of this build", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import amrmg
+    import hierarchy as H
+
+    hier = H.amr_hierarchy(args.base, fracs=(0.25, 0.06), max_box=64, block=8)
+    cells = [hier.cells(l) for l in range(hier.nlev)]
+    log(f"hierarchy: {[len(l) for l in hier.levels]} boxes, cells {cells} (total {sum(cells)})")
+    t0 = time.time()
+    rhs = H.bump_rhs(hier, 1002)
+    log(f"rhs generated in {time.time() - t0:.1f}s")
+    sp, rp = H.solver_params(eps=1e-10, hang=1e-15, iterMax=100)
+
+    ctx = amrmg.Context(0)
+    t0 = time.time()
+    solver = amrmg.AMRMultiGrid(ctx, hier)
+    solver.setSolverParameters(sp, rp)
+    solver.set_field(solver.RHS, rhs)
+    log(f"solver set up in {time.time() - t0:.1f}s (device bytes ~{solver.device_bytes() / 1e9:.1f} GB)")
+
+    for w in range(args.warmup):
+        hist, info = solver.solve()
+        log(f"warmup {w}: {info} residual {hist[0]:.3e} -> {hist[-1]:.3e}")
+
+    ctx.profile(["gsrb"])
+    ctx.profile_reset()
+    t0 = time.perf_counter()
+    infos = []
+    for s in range(args.steps):
+        hist, info = solver.solve()
+        infos.append((hist, info))
+        log(f"step {s}: {info['iterations']} iterations, rel residual {hist[-1] / hist[0]:.3e}")
+    t1 = time.perf_counter()
+    prof = ctx.profile_read("gsrb")
+    ctx.profile([])
+
+    wall = t1 - t0
+    ms_per_step = wall / args.steps * 1e3
+    cell_relax = prof["bytes"] / 24.0          # one cell-relax = one red + one black sweep, 24 B in FP64
+    value = cell_relax / wall / 1e6
+    avg_launch_ms = prof["ms"] / max(prof["launches"], 1)
+    achieved = prof["bytes"] / (prof["ms"] * 1e-3) / 1e9 if prof["ms"] > 0 else 0.0
+    bytes_per_launch = prof["bytes"] / max(prof["launches"], 1)
+    traffic, _ = read_traffic()
+    final_rel = float(infos[-1][0][-1] / infos[-1][0][0])
+    ok = all(h[-1] <= 1e-10 * h[0] for h, _ in infos)
+
+    cpu = None
+    if not args.no_cpu:
+        log("timing the CPU oracle on a bounded sample ...")
+        cpu = cpu_baseline(hier, rhs, sp, rp)
+        log(f"cpu baseline {cpu['value']:.1f} Mcell-relax/s")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Mcell-relax/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"3-level AMR Poisson, base {args.base}^3, ratio 2, centred refined regions 25% then 6% "
+                        f"of parent, 64^3 max boxes, Dirichlet 0, alpha=0 beta=1, Gaussian-bump rhs seed 1002, "
+                        f"tol 1e-10, V(2,2) + BiCGStab bottom, FP64",
+            "levels": [len(l) for l in hier.levels],
+            "cells": sum(cells),
+        },
+        "solves_per_s": round(args.steps / wall, 4),
+        "iterations": infos[-1][1]["iterations"],
+        "final_rel_residual": final_rel,
+        "converged": ok,
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / PEAK_HBM_GBS, 4),
+            "traffic": traffic,
+            "kernel": "GSRB colour sweep (all boxes of a level per launch)",
+            "avg_launch_ms": round(avg_launch_ms, 5),
+            "algorithmic_bytes_per_launch": round(bytes_per_launch, 1),
+            "launches": prof["launches"],
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    solver.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

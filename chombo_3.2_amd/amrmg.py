@@ -1,0 +1,266 @@
+"""ctypes binding of libamrmg.so (C ABI declared in include/amrmg.h).
+
+Host-side mirror of the reference's driver-level API for the elliptic path:
+``AMRPoissonOpFactory`` + ``AMRMultiGrid`` become one ``AMRMultiGrid`` object
+built from a ``Hierarchy`` (hierarchy.py) and device-resident fields; the level
+operations of ``AMRPoissonOp`` are exposed for parity checks.  Everything runs in
+the HIP library: this module only moves host buffers across the C ABI, and it
+raises if the library is missing (there is no CPU fallback).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libamrmg.so")
+HEADER = os.path.join(os.path.dirname(_HERE), "include", "amrmg.h")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_lib = None
+
+KERNEL_CLASSES = ["gsrb", "residual", "apply", "restrict", "prolong", "average", "ghost", "blas",
+                  "reduce", "cf_interp", "reflux", "copier"]
+
+
+class AMRMGError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise AMRMGError(f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+        L = C.CDLL(LIB_PATH)
+        L.amrmg_last_error.restype = C.c_char_p
+        L.amrmg_ctx_create.argtypes = [C.c_int, C.POINTER(C.c_void_p)]
+        L.amrmg_ctx_destroy.argtypes = [C.c_void_p]
+        L.amrmg_solver_create.argtypes = [C.c_void_p, _ip, _dp, C.c_int, C.POINTER(C.c_void_p)]
+        L.amrmg_solver_destroy.argtypes = [C.c_void_p]
+        L.amrmg_solver_set_params.argtypes = [C.c_void_p, _ip, _dp]
+        L.amrmg_solver_level_cells.restype = C.c_longlong
+        L.amrmg_solver_level_cells.argtypes = [C.c_void_p, C.c_int]
+        L.amrmg_solver_num_mg_depths.argtypes = [C.c_void_p]
+        L.amrmg_solver_mg_cells.restype = C.c_longlong
+        L.amrmg_solver_mg_cells.argtypes = [C.c_void_p, C.c_int]
+        L.amrmg_solver_device_bytes.restype = C.c_longlong
+        L.amrmg_solver_device_bytes.argtypes = [C.c_void_p]
+        L.amrmg_solver_set_field.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.amrmg_solver_get_field.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.amrmg_solver_solve.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _ip]
+        L.amrmg_solver_residual.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.amrmg_solver_operator.argtypes = [C.c_void_p, C.c_int]
+        L.amrmg_level_apply.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.c_int]
+        L.amrmg_level_residual.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, C.c_int]
+        L.amrmg_level_relax.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.c_int]
+        L.amrmg_level_restrict.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp]
+        L.amrmg_level_prolong.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+        L.amrmg_kernel_gsrb.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.c_int]
+        L.amrmg_homog_cf.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.amrmg_quad_cf.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+        L.amrmg_average.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+        L.amrmg_bottom_solve.argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.c_double, C.c_double, _ip]
+        L.amrmg_plan_check.argtypes = [_ip, _dp, C.POINTER(C.c_longlong), C.c_int]
+        L.amrmg_profile_enable.argtypes = [C.c_void_p, C.c_int]
+        L.amrmg_profile_reset.argtypes = [C.c_void_p]
+        L.amrmg_profile_read.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.POINTER(C.c_longlong)]
+        _lib = L
+    return _lib
+
+
+def _chk(rc):
+    if rc != 0:
+        raise AMRMGError(lib().amrmg_last_error().decode())
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_dp)
+
+
+def header_symbols():
+    """Entry points declared in include/amrmg.h."""
+    import re
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(amrmg_[a-z0-9_]+)\s*\(", txt)))
+
+
+def plan_check(hier):
+    """Build and bounds-check every device table of `hier` on the host (no device)."""
+    isp, rsp = hier.spec()
+    st = (C.c_longlong * 8)()
+    _chk(lib().amrmg_plan_check(isp.ctypes.data_as(_ip), rsp.ctypes.data_as(_dp), st, 8))
+    keys = ["levels", "mg_depths", "ghost_patches", "cf_coarse", "cf_fine", "reflux_cells", "tiles", "cells"]
+    return dict(zip(keys, list(st)))
+
+
+class Context:
+    """A HIP device context (one stream at default priority)."""
+
+    def __init__(self, device: int = 0):
+        p = C.c_void_p()
+        _chk(lib().amrmg_ctx_create(device, C.byref(p)))
+        self.p = p
+
+    def close(self):
+        if self.p:
+            lib().amrmg_ctx_destroy(self.p)
+            self.p = None
+
+    def profile(self, classes=None):
+        """Time kernel classes with HIP events (classes: names, None = all, [] = off)."""
+        if classes is None:
+            mask = -1
+        else:
+            mask = 0
+            for c in classes:
+                mask |= 1 << (KERNEL_CLASSES.index(c) if isinstance(c, str) else int(c))
+        _chk(lib().amrmg_profile_enable(self.p, mask))
+
+    def profile_reset(self):
+        _chk(lib().amrmg_profile_reset(self.p))
+
+    def profile_read(self, cls):
+        k = KERNEL_CLASSES.index(cls) if isinstance(cls, str) else int(cls)
+        ms, by, n = C.c_double(), C.c_double(), C.c_longlong()
+        _chk(lib().amrmg_profile_read(self.p, k, C.byref(ms), C.byref(by), C.byref(n)))
+        return dict(ms=ms.value, bytes=by.value, launches=n.value)
+
+
+class AMRMultiGrid:
+    """AMRPoissonOpFactory::define + AMRMultiGrid::define/init over a Hierarchy,
+    with a BiCGStab bottom solver; fields phi/rhs live in HBM."""
+
+    PHI, RHS, RES = 0, 1, 2
+
+    def __init__(self, ctx: Context, hier, precision: int = 0):
+        self.ctx = ctx
+        self.hier = hier
+        isp, rsp = hier.spec()
+        p = C.c_void_p()
+        _chk(lib().amrmg_solver_create(ctx.p, isp.ctypes.data_as(_ip), rsp.ctypes.data_as(_dp), precision,
+                                        C.byref(p)))
+        self.p = p
+        self.ncells = sum(lib().amrmg_solver_level_cells(p, l) for l in range(hier.nlev))
+
+    def close(self):
+        if self.p:
+            lib().amrmg_solver_destroy(self.p)
+            self.p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def mg_depths(self):
+        return lib().amrmg_solver_num_mg_depths(self.p)
+
+    def mg_cells(self, d):
+        return lib().amrmg_solver_mg_cells(self.p, d)
+
+    def device_bytes(self):
+        return lib().amrmg_solver_device_bytes(self.p)
+
+    def setSolverParameters(self, sparams, rparams):
+        sp = np.ascontiguousarray(sparams, dtype=np.int32)
+        rp = np.ascontiguousarray(rparams, dtype=np.float64)
+        _chk(lib().amrmg_solver_set_params(self.p, sp.ctypes.data_as(_ip), rp.ctypes.data_as(_dp)))
+
+    def set_field(self, field, data):
+        data, pd = _d(data)
+        if data.size != self.ncells:
+            raise ValueError("field size does not match the hierarchy")
+        _chk(lib().amrmg_solver_set_field(self.p, field, pd))
+
+    def get_field(self, field):
+        out = np.zeros(self.ncells)
+        _chk(lib().amrmg_solver_get_field(self.p, field, out.ctypes.data_as(_dp)))
+        return out
+
+    def solve(self, zero_phi=True, hist_cap=128):
+        hist = np.zeros(hist_cap)
+        info = np.zeros(3, dtype=np.int32)
+        _chk(lib().amrmg_solver_solve(self.p, int(zero_phi), hist.ctypes.data_as(_dp), hist_cap,
+                                       info.ctypes.data_as(_ip)))
+        it = int(info[0])
+        return hist[:it + 1].copy(), dict(iterations=it, exit_status=int(info[1]), bottom_iters=int(info[2]))
+
+    def computeAMRResidual(self, homogeneous=False):
+        n = C.c_double()
+        _chk(lib().amrmg_solver_residual(self.p, int(homogeneous), C.byref(n)))
+        return self.get_field(self.RES), n.value
+
+    def computeAMROperator(self, homogeneous=False):
+        _chk(lib().amrmg_solver_operator(self.p, int(homogeneous)))
+        return self.get_field(self.RES)
+
+    # ---- level operations ----
+    def level_apply(self, depth, phi, homog=False):
+        phi, pp = _d(phi)
+        out = np.zeros_like(phi)
+        _chk(lib().amrmg_level_apply(self.p, depth, pp, out.ctypes.data_as(_dp), int(homog)))
+        return out
+
+    def level_residual(self, depth, phi, rhs, homog=False):
+        phi, pp = _d(phi)
+        rhs, pr = _d(rhs)
+        out = np.zeros_like(phi)
+        _chk(lib().amrmg_level_residual(self.p, depth, pp, pr, out.ctypes.data_as(_dp), int(homog)))
+        return out
+
+    def level_relax(self, depth, phi, rhs, iters):
+        phi = np.array(phi, dtype=np.float64)
+        rhs, pr = _d(rhs)
+        _chk(lib().amrmg_level_relax(self.p, depth, phi.ctypes.data_as(_dp), pr, iters))
+        return phi
+
+    def level_restrict(self, depth, phi, rhs):
+        phi, pp = _d(phi)
+        rhs, pr = _d(rhs)
+        out = np.zeros(self.mg_cells(depth + 1))
+        _chk(lib().amrmg_level_restrict(self.p, depth, pp, pr, out.ctypes.data_as(_dp)))
+        return out
+
+    def level_prolong(self, depth, phi, ec):
+        phi = np.array(phi, dtype=np.float64)
+        ec, pe = _d(ec)
+        _chk(lib().amrmg_level_prolong(self.p, depth, phi.ctypes.data_as(_dp), pe))
+        return phi
+
+    def kernel_gsrb(self, lev, phi_ghosted, rhs, color):
+        phi = np.array(phi_ghosted, dtype=np.float64)
+        rhs, pr = _d(rhs)
+        _chk(lib().amrmg_kernel_gsrb(self.p, lev, phi.ctypes.data_as(_dp), pr, color))
+        return phi
+
+    def homog_cf(self, lev, phi_ghosted):
+        phi = np.array(phi_ghosted, dtype=np.float64)
+        _chk(lib().amrmg_homog_cf(self.p, lev, phi.ctypes.data_as(_dp)))
+        return phi
+
+    def quad_cf(self, lev, phi_ghosted, coarse):
+        phi = np.array(phi_ghosted, dtype=np.float64)
+        coarse, pc = _d(coarse)
+        _chk(lib().amrmg_quad_cf(self.p, lev, phi.ctypes.data_as(_dp), pc))
+        return phi
+
+    def average(self, lev, fine):
+        fine, pf = _d(fine)
+        out = np.zeros(self.hier.cells(lev) // 8)
+        _chk(lib().amrmg_average(self.p, lev, pf, out.ctypes.data_as(_dp)))
+        return out
+
+    def bottom_solve(self, corr, res, nbottom, metric, eps):
+        corr = np.array(corr, dtype=np.float64)
+        res, pr = _d(res)
+        info = np.zeros(1, dtype=np.int32)
+        _chk(lib().amrmg_bottom_solve(self.p, corr.ctypes.data_as(_dp), pr, nbottom, metric, eps,
+                                       info.ctypes.data_as(_ip)))
+        return corr, int(info[0])

@@ -1,0 +1,627 @@
+// AMRPoissonOp, its factory and AMRMultiGrid on device-resident data.
+// Method-by-method mirror of lib/src/AMRElliptic/AMRPoissonOp.cpp and
+// AMRMultiGrid.H; each method cites the reference lines it follows.
+#include "amrpoissonop.hpp"
+
+#include <cmath>
+#include <stdexcept>
+
+namespace amrmg {
+
+template <class T>
+static T* upload_vec(Context* c, const std::vector<T>& v) {
+  T* d = nullptr;
+  dev_alloc(c, (void**)&d, sizeof(T) * v.size(), "hipMalloc table");
+  dev_h2d(c, d, v.data(), sizeof(T) * v.size(), "copy table");
+  return d;
+}
+
+template <class Real>
+AMRPoissonOp<Real>::~AMRPoissonOp() {
+  if (m_ctx && !m_ctx->dry) (void)hipStreamSynchronize(m_ctx->stream);
+  m_ghost.release();
+  m_resCopier.release();
+  m_revCopier.release();
+  if (m_ctx) {
+    dev_free(m_ctx, d_cfc);
+    dev_free(m_ctx, d_cfg);
+    dev_free(m_ctx, d_cff);
+    dev_free(m_ctx, d_deriv);
+    dev_free(m_ctx, d_rcell);
+    dev_free(m_ctx, d_rc);
+    dev_free(m_ctx, d_rf);
+  }
+}
+
+// Every operand's layout is checked against the layout the operation assumes
+// before anything is launched: slab sizes and box tables come from the field.
+static void need(const void* have, const void* want, const char* what) {
+  if (have != want || have == nullptr) throw std::runtime_error(std::string("level data layout mismatch in ") + what);
+}
+
+// ---- ghost refresh: homogeneousCFInterp (AMRPoissonOp.cpp:1598), exchange
+// (exchangeNoOverlap with the trimmed Copier), BC functor (BCFunc.cpp:58) ----
+template <class Real>
+void AMRPoissonOp<Real>::fillGhosts(T& phi, bool doCF, bool homogBC) {
+  need(phi.lay, m_lay, "fillGhosts");
+  ops::fill_ghosts<Real>(*m_lay, m_ghost, phi.d, doCF ? 1 : 0, homogBC ? 1 : 0, m_dx, m_dxCrse);
+}
+template <class Real>
+void AMRPoissonOp<Real>::homogeneousCFInterp(T& phi) {
+  // CF patches only: exchange and BC rules also run, which only rewrite ghosts
+  // that every later use refreshes again
+  fillGhosts(phi, true, true);
+}
+
+// residual (AMRPoissonOp.cpp:224-236) / residualI (:260-297)
+template <class Real>
+void AMRPoissonOp<Real>::residual(T& lhs, T& phi, const T& rhs, bool homog) {
+  need(lhs.lay, m_lay, "residual"); need(rhs.lay, m_lay, "residual");
+  fillGhosts(phi, homog, homog);
+  ops::residual<Real>(*m_lay, lhs.d, phi.d, rhs.d, m_dx, m_alpha, m_beta);
+}
+template <class Real>
+void AMRPoissonOp<Real>::residualI(T& lhs, T& phi, const T& rhs, bool homog) {
+  need(lhs.lay, m_lay, "residualI"); need(rhs.lay, m_lay, "residualI");
+  fillGhosts(phi, false, homog);
+  ops::residual<Real>(*m_lay, lhs.d, phi.d, rhs.d, m_dx, m_alpha, m_beta);
+}
+// applyOp (:332-340) / applyOpI (:343-385)
+template <class Real>
+void AMRPoissonOp<Real>::applyOp(T& lhs, T& phi, bool homog) {
+  need(lhs.lay, m_lay, "applyOp");
+  fillGhosts(phi, true, homog);
+  ops::apply<Real>(*m_lay, lhs.d, phi.d, m_dx, m_alpha, m_beta);
+}
+template <class Real>
+void AMRPoissonOp<Real>::applyOpI(T& lhs, T& phi, bool homog) {
+  need(lhs.lay, m_lay, "applyOpI");
+  fillGhosts(phi, false, homog);
+  ops::apply<Real>(*m_lay, lhs.d, phi.d, m_dx, m_alpha, m_beta);
+}
+// preCond (:304-329)
+template <class Real>
+void AMRPoissonOp<Real>::preCond(T& phi, const T& rhs) {
+  need(phi.lay, m_lay, "preCond"); need(rhs.lay, m_lay, "preCond");
+  double mult = 1.0 / (m_alpha - 2.0 * 3 * m_beta / (m_dx * m_dx));
+  ops::precond_init<Real>(*m_lay, phi.d, rhs.d, mult);
+  relax(phi, rhs, 2);
+}
+// LevelDataOps on the operands' own layout (LevelDataOps.H:93-294)
+template <class Real>
+void AMRPoissonOp<Real>::assignLocal(T& lhs, const T& rhs) {
+  need(rhs.lay, lhs.lay, "assignLocal");
+  ops::copy_all<Real>(*lhs.lay, lhs.d, rhs.d);
+}
+template <class Real>
+double AMRPoissonOp<Real>::dotProduct(const T& a, const T& b) {
+  need(b.lay, a.lay, "dotProduct");
+  return ops::dot_ordered<Real>(*a.lay, a.d, b.d, 1);
+}
+template <class Real>
+void AMRPoissonOp<Real>::incr(T& lhs, const T& x, double s) {
+  need(x.lay, lhs.lay, "incr");
+  ops::incr<Real>(*lhs.lay, lhs.d, x.d, s);
+}
+template <class Real>
+void AMRPoissonOp<Real>::axby(T& lhs, const T& x, const T& y, double a, double b) {
+  need(x.lay, lhs.lay, "axby"); need(y.lay, lhs.lay, "axby");
+  ops::axby<Real>(*lhs.lay, lhs.d, x.d, y.d, a, b);
+}
+template <class Real>
+void AMRPoissonOp<Real>::scale(T& lhs, double s) {
+  need(lhs.lay, lhs.lay, "scale");
+  ops::scale<Real>(*lhs.lay, lhs.d, s);
+}
+// norm (:558-562 -> BoxLayoutData.cpp:23-92)
+template <class Real>
+double AMRPoissonOp<Real>::norm(const T& x, int ord) {
+  need(x.lay, x.lay, "norm");
+  if (ord == 0) return ops::maxnorm<Real>(*x.lay, x.d, 2);
+  if (ord == 2) return ops::norm2_ordered<Real>(*x.lay, x.d, 3);
+  throw std::runtime_error("AMRPoissonOp::norm: order must be 0 or 2");
+}
+template <class Real>
+void AMRPoissonOp<Real>::setToZero(T& lhs) {
+  need(lhs.lay, lhs.lay, "setToZero");
+  ops::setval<Real>(*lhs.lay, lhs.d, 0.0);
+}
+template <class Real>
+double AMRPoissonOp<Real>::localMaxNorm(const T& x) {
+  need(x.lay, m_lay, "localMaxNorm");
+  return ops::maxnorm<Real>(*m_lay, x.d, 4);
+}
+template <class Real>
+void AMRPoissonOp<Real>::localMaxNormAsync(const T& x, int slot, bool accumulate) {
+  need(x.lay, m_lay, "localMaxNorm");
+  ops::maxnorm_async<Real>(*m_lay, x.d, slot, accumulate ? 1 : 0);
+}
+
+// ---- multigrid ----
+template <class Real>
+void AMRPoissonOp<Real>::createCoarser(T& coarse, const T& fine, bool) {
+  if (!m_mgCoarse) throw std::runtime_error("createCoarser: level not coarsenable");
+  need(fine.lay, m_lay, "createCoarser");
+  coarse.define(m_mgCoarse);
+}
+// relax (:600-632) with levelGSRB (:1216-1281)
+template <class Real>
+void AMRPoissonOp<Real>::levelGSRB(T& phi, const T& rhs) {
+  need(rhs.lay, m_lay, "levelGSRB");
+  for (int pass = 0; pass <= 1; pass++) {
+    fillGhosts(phi, true, true);
+    ops::gsrb<Real>(*m_lay, phi.d, rhs.d, pass, m_dx, m_alpha, m_beta);
+  }
+}
+template <class Real>
+void AMRPoissonOp<Real>::relax(T& e, const T& r, int n) {
+  for (int i = 0; i < n; i++) levelGSRB(e, r);
+}
+// restrictResidual (:651-703)
+template <class Real>
+void AMRPoissonOp<Real>::restrictResidual(T& resC, T& phi, const T& rhs) {
+  need(rhs.lay, m_lay, "restrictResidual"); need(resC.lay, m_mgCoarse, "restrictResidual");
+  fillGhosts(phi, true, true);
+  ops::restrict_res<Real>(*m_lay, *m_mgCoarse, resC.d, phi.d, rhs.d, m_dx, m_alpha, m_beta);
+}
+// prolongIncrement (:706-733)
+template <class Real>
+void AMRPoissonOp<Real>::prolongIncrement(T& phi, const T& c) {
+  need(phi.lay, m_lay, "prolongIncrement"); need(c.lay, m_mgCoarse, "prolongIncrement");
+  ops::prolong<Real>(*m_lay, *m_mgCoarse, phi.d, c.d);
+}
+
+// ---- AMR ----
+template <class Real>
+void AMRPoissonOp<Real>::coarseFineInterp(T& phif, const T& phic) {
+  if (!m_coarserLay) return;
+  need(phif.lay, m_lay, "coarseFineInterp"); need(phic.lay, m_coarserLay, "coarseFineInterp");
+  ops::cf_inhomog<Real>(m_ctx, d_cfc, n_cfc, d_cfg, d_cff, n_cff, d_deriv, phif.d, phic.d, m_dx,
+                        m_refToCoarser * m_dx);
+}
+// AMRResidual (:736-750)
+template <class Real>
+void AMRPoissonOp<Real>::AMRResidual(T& res, T& phiFine, T& phi, T& phiCoarse, const T& rhs, bool homog,
+                                     AMRLevelOp<T>* finerOp) {
+  AMROperator(res, phiFine, phi, phiCoarse, homog, finerOp);
+  axby(res, res, rhs, -1.0, 1.0);
+}
+template <class Real>
+void AMRPoissonOp<Real>::AMRResidualNC(T& res, T& phiFine, T& phi, const T& rhs, bool homog,
+                                       AMRLevelOp<T>* finerOp) {
+  AMROperatorNC(res, phiFine, phi, homog, finerOp);
+  axby(res, res, rhs, -1.0, 1.0);
+}
+// AMRResidualNF (:770-787)
+template <class Real>
+void AMRPoissonOp<Real>::AMRResidualNF(T& res, T& phi, T& phiCoarse, const T& rhs, bool homog) {
+  if (phiCoarse.isDefined()) coarseFineInterp(phi, phiCoarse);
+  residualI(res, phi, rhs, homog);
+}
+// AMROperator (:790-815)
+template <class Real>
+void AMRPoissonOp<Real>::AMROperator(T& LofPhi, T& phiFine, T& phi, T& phiCoarse, bool homog,
+                                     AMRLevelOp<T>* finerOp) {
+  if (phiCoarse.isDefined()) coarseFineInterp(phi, phiCoarse);
+  applyOpI(LofPhi, phi, homog);
+  if (phiFine.isDefined()) reflux(phiFine, phi, LofPhi, static_cast<AMRPoissonOp*>(finerOp));
+}
+template <class Real>
+void AMRPoissonOp<Real>::AMROperatorNC(T& LofPhi, T& phiFine, T& phi, bool homog, AMRLevelOp<T>* finerOp) {
+  applyOpI(LofPhi, phi, homog);
+  if (phiFine.isDefined()) reflux(phiFine, phi, LofPhi, static_cast<AMRPoissonOp*>(finerOp));
+}
+template <class Real>
+void AMRPoissonOp<Real>::AMROperatorNF(T& LofPhi, T& phi, T& phiCoarse, bool homog) {
+  if (phiCoarse.isDefined()) coarseFineInterp(phi, phiCoarse);
+  applyOpI(LofPhi, phi, homog);
+}
+// AMRRestrictS (:877-919)
+template <class Real>
+void AMRPoissonOp<Real>::AMRRestrictS(T& resCoarse, const T& residual, T& correction, T& coarseCorrection,
+                                      T& scratch) {
+  AMRResidualNF(scratch, correction, coarseCorrection, residual, true);
+  need(resCoarse.lay, m_cfLay, "AMRRestrictS");
+  ops::average<Real>(*m_lay, *m_cfLay, resCoarse.d, scratch.d);
+}
+// AMRProlongS (:959-990)
+template <class Real>
+void AMRPoissonOp<Real>::AMRProlongS(T& correction, const T& coarseCorrection, T& temp) {
+  need(temp.lay, m_cfLay, "AMRProlongS"); need(coarseCorrection.lay, m_coarserLay, "AMRProlongS");
+  need(correction.lay, m_lay, "AMRProlongS");
+  ops::motion<Real>(m_revCopier, *m_coarserLay, *m_cfLay, temp.d, coarseCorrection.d, 0);
+  ops::prolong<Real>(*m_lay, *m_cfLay, correction.d, temp.d);
+}
+// AMRUpdateResidual (:1045-1054)
+template <class Real>
+void AMRPoissonOp<Real>::AMRUpdateResidual(T& residual, T& correction, T& coarseCorrection) {
+  AMRResidualNF(residual, correction, coarseCorrection, residual, true);
+}
+template <class Real>
+void AMRPoissonOp<Real>::createCoarsened(T& lhs, const T&, int) {
+  lhs.define(m_cfLay);
+}
+// assignCopier (:486-492): lhs (this level) <- rhs (coarsened finer level)
+template <class Real>
+void AMRPoissonOp<Real>::assignCopier(T& lhs, const T& rhs) {
+  need(lhs.lay, m_lay, "assignCopier"); need(rhs.lay, m_finerCfLay, "assignCopier");
+  ops::motion<Real>(*m_zeroPlan, *m_finerCfLay, *m_lay, lhs.d, rhs.d, 0);
+}
+// zeroCovered (:495-502) -> LevelDataOps::copyToZero
+template <class Real>
+void AMRPoissonOp<Real>::zeroCovered(T& lhs) {
+  need(lhs.lay, m_lay, "zeroCovered");
+  if (!m_zeroPlan) throw std::runtime_error("zeroCovered: no finer level");
+  ops::motion<Real>(*m_zeroPlan, *m_finerCfLay, *m_lay, lhs.d, (const Real*)nullptr, 1);
+}
+// reflux (:1117-1201)
+template <class Real>
+void AMRPoissonOp<Real>::reflux(T& phiFine, T& phi, T& res, AMRPoissonOp* finerOp) {
+  need(phiFine.lay, m_finerLay, "reflux"); need(phi.lay, m_lay, "reflux"); need(res.lay, m_lay, "reflux");
+  finerOp->coarseFineInterp(phiFine, phi);
+  double a_scale = 1.0 / m_dx;
+  ops::reflux<Real>(m_ctx, d_rcell, n_rcell, d_rc, d_rf, res.d, phi.d, phiFine.d, m_beta * 1 / m_dx,
+                    m_beta * m_refToFiner / m_dx, -a_scale);
+}
+
+// ---------------------------------------------------------------------------
+// DeviceHierarchy / factory
+// ---------------------------------------------------------------------------
+void DeviceHierarchy::define(Context* c, const std::vector<DBL>& g, const Box& domain0, double dx0) {
+  ctx = c;
+  grids = g;
+  domains.assign(g.size(), domain0);
+  dxs.assign(g.size(), dx0);
+  for (size_t l = 1; l < g.size(); l++) {
+    domains[l] = refine(domains[l - 1], 2);
+    dxs[l] = dxs[l - 1] / 2;
+  }
+  amrLay.clear();
+  cfLay.clear();
+  mgLay.clear();
+  for (size_t l = 0; l < g.size(); l++) {
+    if (!g[l].sorted()) throw std::runtime_error("boxes of a level must be sorted (Box::operator<)");
+    amrLay.emplace_back(new DeviceLayout(c, g[l], domains[l]));
+    if (l == 0) cfLay.emplace_back(nullptr);
+    else {
+      if (!g[l].coarsenable(2)) throw std::runtime_error("fine level boxes must be coarsenable by 2");
+      cfLay.emplace_back(new DeviceLayout(c, g[l].coarsened(2), domains[l - 1]));
+    }
+  }
+  // MultiGrid::define via MGnewOp: stop when not coarsenable by coarsening*2
+  // (AMRPoissonOp.cpp:1955-1959, s_maxCoarse = 2)
+  for (int depth = 1;; depth++) {
+    int coarsening = 1 << depth;
+    if (!g[0].coarsenable(coarsening * 2)) break;
+    mgLay.emplace_back(new DeviceLayout(c, g[0].coarsened(coarsening), coarsen(domain0, coarsening)));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host validation of device tables: every address a kernel forms from a table
+// entry must lie inside the field slab it indexes.  Violations throw before any
+// launch.
+// ---------------------------------------------------------------------------
+static void require(bool ok, const char* what) {
+  if (!ok) throw std::runtime_error(std::string("device table check failed: ") + what);
+}
+static bool in_grown(const DBL& d, int b, const int lo[3], const int hi[3], int g) {
+  if (b < 0 || b >= d.size()) return false;
+  Box gb = grow(d[b], g);
+  return gb.contains(Box(IV{lo[0], lo[1], lo[2]}, IV{hi[0], hi[1], hi[2]}));
+}
+static void validate_ghost_plan(const ops::GhostPlan& P, const DBL& d) {
+  for (const Patch& p : P.h_patches) {
+    require(in_grown(d, p.dst, p.lo, p.hi, 1), "ghost patch outside its box");
+    if (p.kind == PK_EXCHANGE) require(in_grown(d, p.src, p.lo, p.hi, 0), "exchange source outside its box");
+    else {
+      // the cells read (one and two inward) must stay inside the grown box
+      int lo[3] = {p.lo[0], p.lo[1], p.lo[2]}, hi[3] = {p.hi[0], p.hi[1], p.hi[2]};
+      int s = p.side ? -2 : 2;
+      lo[p.dir] += s;
+      hi[p.dir] += s;
+      require(in_grown(d, p.dst, lo, hi, 1), "ghost rule reads outside its box");
+    }
+  }
+}
+static void validate_motion(const ops::MotionPlan& P, const DBL& src, const DBL& dst) {
+  for (const Motion& m : P.h_items) {
+    require(in_grown(src, m.from, m.lo, m.hi, 1), "copier source region outside its box");
+    require(in_grown(dst, m.to, m.lo, m.hi, 1), "copier destination region outside its box");
+  }
+}
+static void validate_pair(const DeviceLayout& f, const DeviceLayout& c) {
+  require(f.dbl.size() == c.dbl.size(), "coarsened layout must match the fine layout box by box");
+  for (int b = 0; b < f.dbl.size(); b++) require(coarsen(f.dbl[b], 2) == c.dbl[b], "coarsened box mismatch");
+}
+
+template <class Real>
+std::unique_ptr<AMRPoissonOp<Real>> AMRPoissonOpFactory<Real>::AMRnewOp(int l, AMRPoissonOp<Real>* coarserOp) {
+  std::unique_ptr<AMRPoissonOp<Real>> op(new AMRPoissonOp<Real>());
+  op->m_ctx = H->ctx;
+  op->m_lay = H->amrLay[l].get();
+  op->m_domain = H->domains[l];
+  op->m_dx = H->dxs[l];
+  op->m_dxCrse = (l > 0) ? H->dxs[l - 1] : -1.0;
+  op->m_alpha = alpha;
+  op->m_beta = beta;
+  op->m_bc = bc;
+  op->m_mgCoarse = (l == 0 && !H->mgLay.empty()) ? H->mgLay[0].get() : (l > 0 ? H->cfLay[l].get() : nullptr);
+  op->m_ghost = build_ghost_plan(H->grids[l], H->domains[l], bc, l > 0);
+  validate_ghost_plan(op->m_ghost, H->grids[l]);
+  op->m_ghost.upload(H->ctx);
+  if (op->m_mgCoarse) validate_pair(*op->m_lay, *op->m_mgCoarse);
+  if (l > 0) {
+    op->m_coarserLay = H->amrLay[l - 1].get();
+    op->m_cfLay = H->cfLay[l].get();
+    CFPlanHost cf = build_cf_plan(H->grids[l], *H->amrLay[l], H->grids[l - 1], *H->amrLay[l - 1], H->domains[l],
+                                  H->dxs[l], 2);
+    {
+      const long long fs = H->amrLay[l]->slab, cs = H->amrLay[l - 1]->slab;
+      for (const CFCoarse& c : cf.coarse) {
+        require(c.pc >= 0 && c.pc < cs, "CF coarse cell");
+        if (c.std)
+          for (int q = 0; q < 8; q++) require(c.nbr[q] >= 0 && c.nbr[q] < cs, "CF coarse neighbour");
+        else require(c.gen >= 0 && c.gen < (int)cf.gens.size(), "CF stencil index");
+      }
+      for (const CFGen& g : cf.gens)
+        for (int q = 0; q < 5; q++) {
+          require(g.n[q] >= 0 && g.n[q] <= 9, "CF stencil length");
+          for (int t = 0; t < g.n[q]; t++) require(g.off[q][t] >= 0 && g.off[q][t] < cs, "CF stencil cell");
+        }
+      for (const CFFine& f : cf.fine) {
+        require(f.g >= 0 && f.g < fs && f.a1 >= 0 && f.a1 < fs && f.a2 >= 0 && f.a2 < fs, "CF fine cell");
+        require(f.coarse >= 0 && f.coarse < (int)cf.coarse.size(), "CF fine record");
+      }
+    }
+    op->n_cfc = (int)cf.coarse.size();
+    op->n_cfg = (int)cf.gens.size();
+    op->n_cff = (int)cf.fine.size();
+    op->d_cfc = upload_vec(H->ctx, cf.coarse);
+    op->d_cfg = upload_vec(H->ctx, cf.gens);
+    op->d_cff = upload_vec(H->ctx, cf.fine);
+    dev_alloc(H->ctx, (void**)&op->d_deriv, sizeof(Real) * 6 * (size_t)op->n_cfc, "hipMalloc deriv");
+    op->m_resCopier = build_copier(H->grids[l].coarsened(2), H->grids[l - 1]);
+    validate_motion(op->m_resCopier, H->grids[l].coarsened(2), H->grids[l - 1]);
+    op->m_resCopier.upload(H->ctx);
+    op->m_revCopier = build_copier(H->grids[l - 1], H->grids[l].coarsened(2));
+    validate_motion(op->m_revCopier, H->grids[l - 1], H->grids[l].coarsened(2));
+    op->m_revCopier.upload(H->ctx);
+    validate_pair(*op->m_lay, *op->m_cfLay);
+    // the coarser operator refluxes against this level and zeroes what it covers
+    coarserOp->m_finerLay = op->m_lay;
+    coarserOp->m_zeroPlan = &op->m_resCopier;
+    coarserOp->m_finerCfLay = op->m_cfLay;
+    RefluxPlanHost rp = build_reflux_plan(H->grids[l - 1], *H->amrLay[l - 1], H->grids[l], *H->amrLay[l],
+                                          H->domains[l - 1], 2);
+    {
+      const long long fs = H->amrLay[l]->slab, cs = H->amrLay[l - 1]->slab;
+      for (const RefluxCell& c : rp.cells) {
+        require(c.u >= 0 && c.u < cs, "reflux cell");
+        require(c.c0 >= 0 && c.nc >= 0 && c.c0 + c.nc <= (int)rp.coarse.size(), "reflux coarse range");
+        require(c.f0 >= 0 && c.nf >= 0 && c.f0 + c.nf <= (int)rp.fine.size(), "reflux fine range");
+      }
+      for (const RefluxCoarse& x : rp.coarse) require(x.a >= 0 && x.a < cs && x.b >= 0 && x.b < cs, "reflux coarse face");
+      for (const RefluxFine& x : rp.fine)
+        for (int t = 0; t < 4; t++) require(x.a[t] >= 0 && x.a[t] < fs && x.b[t] >= 0 && x.b[t] < fs, "reflux fine face");
+    }
+    coarserOp->n_rcell = (int)rp.cells.size();
+    coarserOp->d_rcell = upload_vec(H->ctx, rp.cells);
+    coarserOp->d_rc = upload_vec(H->ctx, rp.coarse);
+    coarserOp->d_rf = upload_vec(H->ctx, rp.fine);
+  }
+  return op;
+}
+
+template <class Real>
+std::vector<std::unique_ptr<MGLevelOp<LevelData<Real>>>> AMRPoissonOpFactory<Real>::MGnewOps() {
+  std::vector<std::unique_ptr<MGLevelOp<LevelData<Real>>>> out;
+  for (size_t d = 0; d < H->mgLay.size(); d++) {
+    int coarsening = 1 << (d + 1);
+    AMRPoissonOp<Real>* op = new AMRPoissonOp<Real>();
+    op->m_ctx = H->ctx;
+    op->m_lay = H->mgLay[d].get();
+    op->m_domain = coarsen(H->domains[0], coarsening);
+    op->m_dx = H->dxs[0] * coarsening;
+    op->m_dxCrse = -1.0;
+    op->m_alpha = alpha;
+    op->m_beta = beta;
+    op->m_bc = bc;
+    op->m_mgCoarse = (d + 1 < H->mgLay.size()) ? H->mgLay[d + 1].get() : nullptr;
+    op->m_ghost = build_ghost_plan(H->mgLay[d]->dbl, op->m_domain, bc, false);
+    validate_ghost_plan(op->m_ghost, H->mgLay[d]->dbl);
+    op->m_ghost.upload(H->ctx);
+    if (op->m_mgCoarse) validate_pair(*op->m_lay, *op->m_mgCoarse);
+    out.emplace_back(op);
+  }
+  return out;
+}
+
+// ---------------------------------------------------------------------------
+// AMRMultiGrid
+// ---------------------------------------------------------------------------
+template <class Real>
+void AMRMultiGrid<Real>::define(AMRPoissonOpFactory<Real>& factory, LinearSolver<T>* bottom, int numLevels) {
+  m_op.clear();
+  for (int l = 0; l < numLevels; l++) m_op.push_back(factory.AMRnewOp(l, l > 0 ? m_op[l - 1].get() : nullptr));
+  m_bottomSolver = bottom;
+  m_mg0.reset(new MultiGrid<T>());
+  m_mg0->define(m_op[0].get(), factory.MGnewOps(), factory.H->domains[0].npts(), bottom);
+  m_mg0->m_pre = m_pre;
+  m_mg0->m_post = m_post;
+  m_mg0->m_bottom = m_bottom;
+  m_correction.resize(numLevels);
+  m_residual.resize(numLevels);
+  m_resC.resize(numLevels);
+}
+// setSolverParameters (AMRMultiGrid.H:528-553)
+template <class Real>
+void AMRMultiGrid<Real>::setSolverParameters(int pre, int post, int bottom, int numMG, int iterMax, double eps,
+                                             double hang, double normThresh) {
+  m_pre = pre; m_post = post; m_bottom = bottom; m_eps = eps; m_hang = hang; m_normThresh = normThresh;
+  m_iterMax = iterMax; m_numMG = numMG;
+  m_mg0->m_pre = pre; m_mg0->m_post = post; m_mg0->m_bottom = bottom;
+  m_mg0->m_numMG = numMG; m_mg0->m_cycle = numMG;
+  m_bottomSolverEpsCushion = 1.0;
+}
+// init (AMRMultiGrid.H:1212-1249)
+template <class Real>
+void AMRMultiGrid<Real>::init(std::vector<T*>& phi, std::vector<T*>& rhs, int l_max, int l_base) {
+  for (int i = l_base; i <= l_max; i++) {
+    AMRPoissonOp<Real>& op = *m_op[i];
+    m_correction[i].reset(new T());
+    m_residual[i].reset(new T());
+    op.create(*m_correction[i], *phi[i]);
+    op.create(*m_residual[i], *rhs[i]);
+    if (i == 0) m_mg0->init(*phi[i], *rhs[i]);
+    if (i != l_base) {
+      m_resC[i].reset(new T());
+      op.createCoarsened(*m_resC[i], *rhs[i], 2);
+    }
+  }
+}
+template <class Real>
+void AMRMultiGrid<Real>::relax(T& phi, T& R, int depth, int nRelax) {
+  m_op[depth]->relax(phi, R, nRelax);
+}
+// computeAMRResidualLevel (AMRMultiGrid.H:837-892)
+template <class Real>
+void AMRMultiGrid<Real>::computeAMRResidualLevel(std::vector<T*>& resid, std::vector<T*>& phi, std::vector<T*>& rhs,
+                                                 int l_max, int l_base, int ilev, bool homog) {
+  if (l_max != l_base) {
+    if (ilev == l_max) {
+      m_op[l_max]->AMRResidualNF(*resid[l_max], *phi[l_max], *phi[l_max - 1], *rhs[l_max], homog);
+    } else if (ilev == l_base) {
+      if (l_base == 0)
+        m_op[l_base]->AMRResidualNC(*resid[l_base], *phi[l_base + 1], *phi[l_base], *rhs[l_base], homog,
+                                    m_op[l_base + 1].get());
+      else
+        m_op[l_base]->AMRResidual(*resid[l_base], *phi[l_base + 1], *phi[l_base], *phi[l_base - 1], *rhs[l_base],
+                                  homog, m_op[l_base + 1].get());
+    } else {
+      m_op[ilev]->AMRResidual(*resid[ilev], *phi[ilev + 1], *phi[ilev], *phi[ilev - 1], *rhs[ilev], homog,
+                              m_op[ilev + 1].get());
+    }
+  } else {
+    if (l_base == 0) m_op[l_max]->residual(*resid[l_max], *phi[l_max], *rhs[l_max], homog);
+    else m_op[l_max]->AMRResidualNF(*resid[l_max], *phi[l_max], *phi[l_max - 1], *rhs[l_max], homog);
+  }
+}
+// computeAMRResidual (AMRMultiGrid.H:683-736); the level max-norms accumulate on
+// the device and are read once.
+template <class Real>
+double AMRMultiGrid<Real>::computeAMRResidual(std::vector<T*>& resid, std::vector<T*>& phi, std::vector<T*>& rhs,
+                                              int l_max, int l_base, bool homog, bool computeNorm) {
+  for (int ilev = l_base; ilev <= l_max; ilev++) {
+    computeAMRResidualLevel(resid, phi, rhs, l_max, l_base, ilev, homog);
+    if (computeNorm) {
+      if (ilev != l_max) m_op[ilev]->zeroCovered(*resid[ilev]);
+      m_op[ilev]->localMaxNormAsync(*resid[ilev], 0, ilev != l_base);
+    }
+  }
+  if (!computeNorm) return 0.0;
+  Context* ctx = m_op[0]->m_ctx;
+  hip_check(hipMemcpyAsync(ctx->h_scalars, ctx->d_scalars, sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
+            "copy rnorm");
+  ctx->sync();
+  return ctx->h_scalars[0];
+}
+// computeAMROperator (AMRMultiGrid.H:790-833)
+template <class Real>
+void AMRMultiGrid<Real>::computeAMROperator(std::vector<T*>& lphi, std::vector<T*>& phi, int l_max, int l_base,
+                                            bool homog) {
+  for (int ilev = l_base; ilev <= l_max; ilev++) {
+    if (l_max != l_base) {
+      if (ilev == l_max) m_op[l_max]->AMROperatorNF(*lphi[l_max], *phi[l_max], *phi[l_max - 1], homog);
+      else if (ilev == l_base && l_base == 0)
+        m_op[l_base]->AMROperatorNC(*lphi[l_base], *phi[l_base + 1], *phi[l_base], homog, m_op[l_base + 1].get());
+      else
+        m_op[ilev]->AMROperator(*lphi[ilev], *phi[ilev + 1], *phi[ilev], *phi[ilev - 1], homog,
+                                m_op[ilev + 1].get());
+    } else {
+      m_op[l_max]->applyOp(*lphi[l_max], *phi[l_max], homog);
+    }
+  }
+}
+// AMRVCycle (AMRMultiGrid.H:1342-1419)
+template <class Real>
+void AMRMultiGrid<Real>::AMRVCycle(std::vector<T*>& uberCorrection, std::vector<T*>& uberResidual, int ilev,
+                                   int l_max, int l_base) {
+  if (ilev == l_max) {
+    for (int level = l_base; level <= l_max; level++) {
+      m_op[level]->assignLocal(*m_residual[level], *uberResidual[level]);
+      m_op[level]->setToZero(*m_correction[level]);
+    }
+  }
+  if (l_max == l_base) {
+    m_mg0->oneCycle(*uberCorrection[ilev], *uberResidual[ilev]);
+  } else if (ilev == l_base) {
+    m_mg0->oneCycle(*m_correction[ilev], *m_residual[ilev]);
+    m_op[ilev]->incr(*uberCorrection[ilev], *m_correction[ilev], 1.0);
+  } else {
+    relax(*m_correction[ilev], *m_residual[ilev], ilev, m_pre);
+    m_op[ilev]->incr(*uberCorrection[ilev], *m_correction[ilev], 1.0);
+    m_op[ilev - 1]->setToZero(*m_correction[ilev - 1]);
+    std::vector<T*> res(m_residual.size());
+    for (size_t q = 0; q < res.size(); q++) res[q] = m_residual[q].get();
+    computeAMRResidualLevel(res, uberCorrection, uberResidual, l_max, l_base, ilev - 1, true);
+    m_op[ilev]->AMRRestrictS(*m_resC[ilev], *m_residual[ilev], *m_correction[ilev], *m_correction[ilev - 1],
+                             *uberCorrection[ilev]);
+    m_op[ilev - 1]->assignCopier(*m_residual[ilev - 1], *m_resC[ilev]);
+    for (int img = 0; img < m_numMG; img++) AMRVCycle(uberCorrection, uberResidual, ilev - 1, l_max, l_base);
+    m_op[ilev]->AMRProlongS(*m_correction[ilev], *m_correction[ilev - 1], *m_resC[ilev]);
+    m_op[ilev]->AMRUpdateResidual(*m_residual[ilev], *m_correction[ilev], *m_correction[ilev - 1]);
+    T& dCorr = *uberCorrection[ilev];
+    m_op[ilev]->setToZero(dCorr);
+    relax(dCorr, *m_residual[ilev], ilev, m_post);
+    m_op[ilev]->incr(*m_correction[ilev], dCorr, 1.0);
+    m_op[ilev]->assignLocal(*uberCorrection[ilev], *m_correction[ilev]);
+  }
+}
+// solveNoInitResid (AMRMultiGrid.H:935-1124)
+template <class Real>
+void AMRMultiGrid<Real>::solveNoInitResid(std::vector<T*>& phi, std::vector<T*>& uberResidual,
+                                          std::vector<T*>& uberCorrection, std::vector<T*>& rhs, int l_max,
+                                          int l_base, bool zeroPhi) {
+  m_mg0->setBottomSolver(m_bottomSolver);
+  for (int ilev = l_base; ilev <= l_max; ilev++) {
+    m_op[ilev]->setToZero(*uberResidual[ilev]);
+    m_op[ilev]->setToZero(*uberCorrection[ilev]);
+  }
+  if (zeroPhi)
+    for (int ilev = l_base; ilev <= l_max; ilev++) m_op[ilev]->setToZero(*phi[ilev]);
+  m_initial_rnorm = computeAMRResidual(uberResidual, phi, rhs, l_max, l_base, false, true);
+  double rnorm = m_initial_rnorm, norm_last = 2 * m_initial_rnorm;
+  m_bottomSolver->setConvergenceMetrics(m_initial_rnorm, m_bottomSolverEpsCushion * m_eps);
+  int iter = 0;
+  m_history.assign(1, rnorm);
+  bool goNorm = rnorm > m_normThresh, goRedu = rnorm > m_eps * m_initial_rnorm, goIter = iter < m_iterMax;
+  bool goHang = iter < m_imin || rnorm < (1 - m_hang) * norm_last, goMin = iter < m_iterMin;
+  while (goMin || (goIter && goRedu && goHang && goNorm)) {
+    norm_last = rnorm;
+    AMRVCycle(uberCorrection, uberResidual, l_max, l_max, l_base);
+    for (int ilev = l_base; ilev <= l_max; ilev++) {
+      m_op[ilev]->incr(*phi[ilev], *uberCorrection[ilev], 1.0);
+      m_op[ilev]->setToZero(*uberCorrection[ilev]);
+    }
+    rnorm = computeAMRResidual(uberResidual, phi, rhs, l_max, l_base, false, true);
+    iter++;
+    m_history.push_back(rnorm);
+    goNorm = rnorm > m_normThresh;
+    goRedu = rnorm > m_eps * m_initial_rnorm;
+    goIter = iter < m_iterMax;
+    goHang = iter < m_imin || rnorm < (1 - m_hang) * norm_last;
+    goMin = iter < m_iterMin;
+  }
+  m_iterations = iter;
+  m_exitStatus = int(!goRedu) + int(!goIter) * 2 + int(!goHang) * 4 + int(!goNorm) * 8;
+}
+
+template class AMRPoissonOp<double>;
+template class AMRPoissonOp<float>;
+template class AMRPoissonOpFactory<double>;
+template class AMRPoissonOpFactory<float>;
+template class AMRMultiGrid<double>;
+template class AMRMultiGrid<float>;
+
+}  // namespace amrmg

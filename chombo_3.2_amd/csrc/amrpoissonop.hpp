@@ -1,0 +1,159 @@
+// AMRPoissonOp / AMRPoissonOpFactory / AMRMultiGrid on device-resident data.
+//
+// Mirrors lib/src/AMRElliptic/AMRPoissonOp.{H,cpp} (operator alpha*I + beta*Laplacian
+// on one AMR or multigrid level) and AMRMultiGrid.H (Martin-Cartwright AMR V-cycle).
+// Every per-box FORT_* loop of the reference is one kernel launch over the level;
+// Copier exchanges, BCs and coarse-fine interpolation are table-driven kernels.
+#pragma once
+#include <memory>
+#include <vector>
+
+#include "ops.hpp"
+#include "plans.hpp"
+#include "solver.hpp"
+
+namespace amrmg {
+
+template <class Real>
+class AMRPoissonOp : public AMRLevelOp<LevelData<Real>> {
+ public:
+  using T = LevelData<Real>;
+  Context* m_ctx = nullptr;
+  const DeviceLayout* m_lay = nullptr;        // this level
+  const DeviceLayout* m_mgCoarse = nullptr;   // coarsen(this, 2): multigrid child level
+  Box m_domain;
+  double m_dx = 1, m_dxCrse = -1, m_alpha = 0, m_beta = 1;
+  BCSpec m_bc{};
+  ops::GhostPlan m_ghost;
+  // coarser AMR level (QuadCFInterp + restriction/prolongation copiers)
+  const DeviceLayout* m_coarserLay = nullptr;
+  const DeviceLayout* m_cfLay = nullptr;      // coarsen(this level, refToCoarser): resC layout
+  int m_refToCoarser = 2;
+  CFCoarse* d_cfc = nullptr; CFGen* d_cfg = nullptr; CFFine* d_cff = nullptr;
+  int n_cfc = 0, n_cfg = 0, n_cff = 0;
+  Real* d_deriv = nullptr;
+  ops::MotionPlan m_resCopier;                // resC (cfLay) -> coarser level
+  ops::MotionPlan m_revCopier;                // coarser level -> resC (cfLay)
+  // finer AMR level (reflux, zeroCovered)
+  const DeviceLayout* m_finerLay = nullptr;
+  int m_refToFiner = 2;
+  RefluxCell* d_rcell = nullptr; RefluxCoarse* d_rc = nullptr; RefluxFine* d_rf = nullptr;
+  int n_rcell = 0;
+  const ops::MotionPlan* m_zeroPlan = nullptr; // finer op's m_resCopier (cfLay(finer) -> this)
+  const DeviceLayout* m_finerCfLay = nullptr;
+
+  AMRPoissonOp() = default;
+  ~AMRPoissonOp() override;
+
+  // ---- LinearOp ----
+  void residual(T& lhs, T& phi, const T& rhs, bool homogeneous = false) override;
+  void preCond(T& cor, const T& residual) override;
+  void applyOp(T& lhs, T& phi, bool homogeneous = false) override;
+  void create(T& lhs, const T& rhs) override { lhs.define(rhs.lay); }
+  void assignLocal(T& lhs, const T& rhs) override;
+  double dotProduct(const T& a, const T& b) override;
+  void incr(T& lhs, const T& x, double scale) override;
+  void axby(T& lhs, const T& x, const T& y, double a, double b) override;
+  void scale(T& lhs, double s) override;
+  double norm(const T& x, int ord) override;
+  void setToZero(T& lhs) override;
+  double dx() const override { return m_dx; }
+  // ---- MGLevelOp ----
+  void createCoarser(T& coarse, const T& fine, bool ghosted) override;
+  void relax(T& correction, const T& residual, int iterations) override;
+  void restrictResidual(T& resCoarse, T& phiFine, const T& rhsFine) override;
+  void prolongIncrement(T& phiThisLevel, const T& correctCoarse) override;
+  // ---- AMRLevelOp ----
+  int refToCoarser() override { return m_refToCoarser; }
+  void AMRResidual(T& res, T& phiFine, T& phi, T& phiCoarse, const T& rhs, bool homog, AMRLevelOp<T>* finerOp) override;
+  void AMRResidualNC(T& res, T& phiFine, T& phi, const T& rhs, bool homog, AMRLevelOp<T>* finerOp) override;
+  void AMRResidualNF(T& res, T& phi, T& phiCoarse, const T& rhs, bool homog) override;
+  void AMROperator(T& LofPhi, T& phiFine, T& phi, T& phiCoarse, bool homog, AMRLevelOp<T>* finerOp) override;
+  void AMROperatorNC(T& LofPhi, T& phiFine, T& phi, bool homog, AMRLevelOp<T>* finerOp) override;
+  void AMROperatorNF(T& LofPhi, T& phi, T& phiCoarse, bool homog) override;
+  void AMRRestrictS(T& resCoarse, const T& residual, T& correction, T& coarseCorrection, T& scratch) override;
+  void AMRProlongS(T& correction, const T& coarseCorrection, T& temp) override;
+  void AMRUpdateResidual(T& residual, T& correction, T& coarseCorrection) override;
+  void createCoarsened(T& lhs, const T& rhs, int refRat) override;
+  void assignCopier(T& lhs, const T& rhs) override;
+  void zeroCovered(T& lhs) override;
+  double localMaxNorm(const T& x) override;
+
+  // pieces used above and by the C ABI
+  void fillGhosts(T& phi, bool doCF, bool homogBC);
+  void residualI(T& lhs, T& phi, const T& rhs, bool homog);
+  void applyOpI(T& lhs, T& phi, bool homog);
+  void levelGSRB(T& phi, const T& rhs);
+  void homogeneousCFInterp(T& phi);
+  void coarseFineInterp(T& phif, const T& phic);   // QuadCFInterp of this level from the coarser one
+  void reflux(T& phiFine, T& phi, T& res, AMRPoissonOp* finerOp);
+  void localMaxNormAsync(const T& x, int slot, bool accumulate);
+};
+
+// Device layouts of one AMR hierarchy: AMR levels, the multigrid levels of level
+// 0, and the coarsened-fine layouts used for restriction/prolongation between
+// AMR levels.  Shared by the operator factories of every precision.
+struct DeviceHierarchy {
+  Context* ctx = nullptr;
+  std::vector<DBL> grids;
+  std::vector<Box> domains;
+  std::vector<double> dxs;
+  std::vector<std::unique_ptr<DeviceLayout>> amrLay;  // per AMR level
+  std::vector<std::unique_ptr<DeviceLayout>> cfLay;   // coarsen(grids[l], 2); index l >= 1
+  std::vector<std::unique_ptr<DeviceLayout>> mgLay;   // MG levels of AMR level 0, depth >= 1
+  void define(Context* c, const std::vector<DBL>& g, const Box& domain0, double dx0);
+  int nlev() const { return (int)grids.size(); }
+  int mgDepth() const { return 1 + (int)mgLay.size(); }
+};
+
+// AMRPoissonOpFactory::define (AMRPoissonOp.cpp:1857-1920) with AMRnewOp
+// (:1992-2070) and MGnewOp (:1923-1989).
+template <class Real>
+class AMRPoissonOpFactory {
+ public:
+  DeviceHierarchy* H = nullptr;
+  double alpha = 0, beta = 1;
+  BCSpec bc{};
+  void define(DeviceHierarchy* h, double a, double b, const BCSpec& bcs) {
+    H = h; alpha = a; beta = b; bc = bcs;
+  }
+  // operator for AMR level l; the operator of level l-1 must be passed for l > 0
+  std::unique_ptr<AMRPoissonOp<Real>> AMRnewOp(int l, AMRPoissonOp<Real>* coarserOp);
+  std::vector<std::unique_ptr<MGLevelOp<LevelData<Real>>>> MGnewOps();
+};
+
+// ---------------------------------------------------------------------------
+// AMRMultiGrid<T> (AMRMultiGrid.H:307-1419)
+// ---------------------------------------------------------------------------
+template <class Real>
+class AMRMultiGrid {
+ public:
+  using T = LevelData<Real>;
+  double m_eps = 1e-6, m_hang = 1e-15, m_normThresh = 1e-30;
+  int m_imin = 5, m_iterMax = 20, m_iterMin = -1, m_exitStatus = -1;
+  int m_pre = 2, m_post = 2, m_bottom = 2, m_numMG = 1;
+  double m_initial_rnorm = 0;
+  double m_bottomSolverEpsCushion = 1.0;
+  std::vector<std::unique_ptr<AMRPoissonOp<Real>>> m_op;
+  std::unique_ptr<MultiGrid<T>> m_mg0;                   // multigrid of level 0
+  std::vector<std::unique_ptr<T>> m_correction, m_residual, m_resC;
+  LinearSolver<T>* m_bottomSolver = nullptr;
+  std::vector<double> m_history;
+  int m_iterations = 0;
+
+  void define(AMRPoissonOpFactory<Real>& factory, LinearSolver<T>* bottom, int numLevels);
+  void setSolverParameters(int pre, int post, int bottom, int numMG, int iterMax, double eps, double hang,
+                           double normThresh);
+  void init(std::vector<T*>& phi, std::vector<T*>& rhs, int l_max, int l_base);
+  void solveNoInitResid(std::vector<T*>& phi, std::vector<T*>& uberResidual, std::vector<T*>& uberCorrection,
+                        std::vector<T*>& rhs, int l_max, int l_base, bool zeroPhi);
+  void AMRVCycle(std::vector<T*>& uberCorrection, std::vector<T*>& uberResidual, int ilev, int l_max, int l_base);
+  double computeAMRResidual(std::vector<T*>& resid, std::vector<T*>& phi, std::vector<T*>& rhs, int l_max,
+                            int l_base, bool homogeneousBC, bool computeNorm);
+  void computeAMRResidualLevel(std::vector<T*>& resid, std::vector<T*>& phi, std::vector<T*>& rhs, int l_max,
+                               int l_base, int ilev, bool homogeneousBC);
+  void computeAMROperator(std::vector<T*>& lphi, std::vector<T*>& phi, int l_max, int l_base, bool homogeneousBC);
+  void relax(T& phi, T& R, int depth, int nRelax);
+};
+
+}  // namespace amrmg

@@ -1,0 +1,426 @@
+// extern "C" boundary of libamrmg.so (declarations and conventions in include/amrmg.h).
+#include "../../include/amrmg.h"
+
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "amrpoissonop.hpp"
+
+using namespace amrmg;
+
+static thread_local std::string g_err;
+
+extern "C" const char* amrmg_last_error(void) { return g_err.c_str(); }
+
+#define API_BEGIN try {
+#define API_END                    \
+  }                                \
+  catch (const std::exception& e) { \
+    g_err = e.what();              \
+    return -1;                     \
+  }                                \
+  catch (...) {                    \
+    g_err = "unknown error";       \
+    return -1;                     \
+  }                                \
+  return 0;
+
+struct amrmg_ctx {
+  std::unique_ptr<Context> c;
+};
+
+struct amrmg_solver {
+  amrmg_ctx* ctx = nullptr;
+  int precision = 0;
+  DeviceHierarchy H;
+  BCSpec bc{};
+  double alpha = 0, beta = 1;
+  AMRPoissonOpFactory<double> fac;
+  AMRMultiGrid<double> amg;
+  BiCGStabSolver<LevelData<double>> bottom;
+  std::vector<std::unique_ptr<LevelData<double>>> phi, rhs, res, uberRes, uberCorr;
+  int sp[6] = {2, 2, 2, 1, 100, 5};
+  double rp[3] = {1e-10, 1e-15, 1e-30};
+  long long table_bytes = 0;
+
+  int nlev() const { return H.nlev(); }
+  std::vector<LevelData<double>*> ptrs(std::vector<std::unique_ptr<LevelData<double>>>& v) {
+    std::vector<LevelData<double>*> p;
+    for (auto& x : v) p.push_back(x.get());
+    return p;
+  }
+  AMRPoissonOp<double>* mgop(int depth) {
+    if (depth < 0 || depth >= H.mgDepth()) throw std::runtime_error("bad multigrid depth");
+    if (depth == 0) return amg.m_op[0].get();
+    return static_cast<AMRPoissonOp<double>*>(amg.m_mg0->m_op[depth]);
+  }
+  const DeviceLayout* mglay(int depth) { return depth == 0 ? H.amrLay[0].get() : H.mgLay[depth - 1].get(); }
+};
+
+// ---- host <-> device transfers of valid-packed / ghosted-packed data ----
+static void upload_valid(const DeviceLayout& L, double* d, const double* h) {
+  std::vector<double> slab((size_t)L.slab, 0.0);
+  long long o = 0;
+  for (int b = 0; b < L.dbl.size(); b++) {
+    const Box& bx = L.dbl[b];
+    for (int k = bx.lo[2]; k <= bx.hi[2]; k++)
+      for (int j = bx.lo[1]; j <= bx.hi[1]; j++) {
+        long long c = L.index(b, bx.lo[0], j, k);
+        std::memcpy(&slab[(size_t)c], h + o, sizeof(double) * bx.len(0));
+        o += bx.len(0);
+      }
+  }
+  hip_check(hipMemcpyAsync(d, slab.data(), sizeof(double) * L.slab, hipMemcpyHostToDevice, L.ctx->stream), "upload");
+  L.ctx->sync();
+}
+static void download_valid(const DeviceLayout& L, const double* d, double* h) {
+  std::vector<double> slab((size_t)L.slab);
+  L.ctx->sync();
+  hip_check(hipMemcpy(slab.data(), d, sizeof(double) * L.slab, hipMemcpyDeviceToHost), "download");
+  long long o = 0;
+  for (int b = 0; b < L.dbl.size(); b++) {
+    const Box& bx = L.dbl[b];
+    for (int k = bx.lo[2]; k <= bx.hi[2]; k++)
+      for (int j = bx.lo[1]; j <= bx.hi[1]; j++) {
+        long long c = L.index(b, bx.lo[0], j, k);
+        std::memcpy(h + o, &slab[(size_t)c], sizeof(double) * bx.len(0));
+        o += bx.len(0);
+      }
+  }
+}
+static void upload_ghosted(const DeviceLayout& L, double* d, const double* h) {
+  hip_check(hipMemcpyAsync(d, h, sizeof(double) * L.slab, hipMemcpyHostToDevice, L.ctx->stream), "upload");
+  L.ctx->sync();
+}
+static void download_ghosted(const DeviceLayout& L, const double* d, double* h) {
+  L.ctx->sync();
+  hip_check(hipMemcpy(h, d, sizeof(double) * L.slab, hipMemcpyDeviceToHost), "download");
+}
+
+static std::vector<DBL> parse_spec(const int* is, const double* rs, Box& dom, BCSpec& bc, double& dx0, double& a,
+                                   double& b) {
+  int nlev = is[0];
+  if (nlev < 1 || nlev > 16) throw std::runtime_error("nlev out of range");
+  dom = Box(IV{is[1], is[2], is[3]}, IV{is[4], is[5], is[6]});
+  for (int d = 0; d < 3; d++) {
+    bc.type[0][d] = is[7 + d];
+    bc.type[1][d] = is[10 + d];
+    bc.val[0][d] = rs[3 + d];
+    bc.val[1][d] = rs[6 + d];
+    for (int s = 0; s < 2; s++)
+      if (bc.type[s][d] != 0 && bc.type[s][d] != 1) throw std::runtime_error("unknown BC type");
+  }
+  dx0 = rs[0];
+  a = rs[1];
+  b = rs[2];
+  std::vector<DBL> g(nlev);
+  const int* bp = is + 13 + nlev;
+  for (int l = 0; l < nlev; l++) {
+    int n = is[13 + l];
+    if (n <= 0) throw std::runtime_error("a level has no boxes");
+    for (int q = 0; q < n; q++) {
+      g[l].boxes.push_back(Box(IV{bp[0], bp[1], bp[2]}, IV{bp[3], bp[4], bp[5]}));
+      if (g[l].boxes.back().empty()) throw std::runtime_error("empty box");
+      bp += 6;
+    }
+  }
+  return g;
+}
+
+extern "C" {
+
+int amrmg_ctx_create(int device, amrmg_ctx** out) {
+  API_BEGIN
+  amrmg_ctx* c = new amrmg_ctx();
+  c->c.reset(new Context(device));
+  *out = c;
+  API_END
+}
+int amrmg_ctx_destroy(amrmg_ctx* ctx) {
+  API_BEGIN
+  delete ctx;
+  API_END
+}
+
+int amrmg_solver_create(amrmg_ctx* ctx, const int* ispec, const double* rspec, int precision, amrmg_solver** out) {
+  API_BEGIN
+  if (precision != 0) throw std::runtime_error("precision mode not available in this build");
+  std::unique_ptr<amrmg_solver> s(new amrmg_solver());
+  s->ctx = ctx;
+  s->precision = precision;
+  Box dom;
+  double dx0;
+  std::vector<DBL> g = parse_spec(ispec, rspec, dom, s->bc, dx0, s->alpha, s->beta);
+  s->H.define(ctx->c.get(), g, dom, dx0);
+  s->fac.define(&s->H, s->alpha, s->beta, s->bc);
+  s->amg.define(s->fac, &s->bottom, s->nlev());
+  s->amg.setSolverParameters(s->sp[0], s->sp[1], s->sp[2], s->sp[3], s->sp[4], s->rp[0], s->rp[1], s->rp[2]);
+  s->amg.m_imin = s->sp[5];
+  for (int l = 0; l < s->nlev(); l++) {
+    const DeviceLayout* L = s->H.amrLay[l].get();
+    s->phi.emplace_back(new LevelData<double>(L));
+    s->rhs.emplace_back(new LevelData<double>(L));
+    s->res.emplace_back(new LevelData<double>(L));
+    s->uberRes.emplace_back(new LevelData<double>(L));
+    s->uberCorr.emplace_back(new LevelData<double>(L));
+  }
+  auto ph = s->ptrs(s->phi), rh = s->ptrs(s->rhs);
+  s->amg.init(ph, rh, s->nlev() - 1, 0);
+  ctx->c->sync();
+  *out = s.release();
+  API_END
+}
+int amrmg_solver_destroy(amrmg_solver* s) {
+  API_BEGIN
+  delete s;
+  API_END
+}
+int amrmg_solver_set_params(amrmg_solver* s, const int* sp, const double* rp) {
+  API_BEGIN
+  std::memcpy(s->sp, sp, sizeof s->sp);
+  std::memcpy(s->rp, rp, sizeof s->rp);
+  s->amg.setSolverParameters(sp[0], sp[1], sp[2], sp[3], sp[4], rp[0], rp[1], rp[2]);
+  s->amg.m_imin = sp[5];
+  API_END
+}
+long long amrmg_solver_level_cells(amrmg_solver* s, int lev) {
+  if (lev < 0 || lev >= s->nlev()) return -1;
+  return s->H.amrLay[lev]->valid_cells;
+}
+int amrmg_solver_num_mg_depths(amrmg_solver* s) { return s->H.mgDepth(); }
+long long amrmg_solver_mg_cells(amrmg_solver* s, int depth) {
+  if (depth < 0 || depth >= s->H.mgDepth()) return -1;
+  return s->mglay(depth)->valid_cells;
+}
+long long amrmg_solver_device_bytes(amrmg_solver* s) {
+  long long n = 0;
+  for (int l = 0; l < s->nlev(); l++) n += s->H.amrLay[l]->slab;
+  long long fields = 5 + 3;  // phi, rhs, res, uber residual/correction, correction, residual, resC
+  long long total = n * fields * 8;
+  for (auto& m : s->H.mgLay) total += m->slab * 2 * 8;
+  return total;
+}
+
+static std::vector<std::unique_ptr<LevelData<double>>>& field_of(amrmg_solver* s, int field) {
+  if (field == 0) return s->phi;
+  if (field == 1) return s->rhs;
+  if (field == 2) return s->res;
+  throw std::runtime_error("field must be 0 (phi), 1 (rhs) or 2 (residual)");
+}
+int amrmg_solver_set_field(amrmg_solver* s, int field, const double* h) {
+  API_BEGIN
+  auto& f = field_of(s, field);
+  long long o = 0;
+  for (int l = 0; l < s->nlev(); l++) {
+    upload_valid(*s->H.amrLay[l], f[l]->d, h + o);
+    o += s->H.amrLay[l]->valid_cells;
+  }
+  API_END
+}
+int amrmg_solver_get_field(amrmg_solver* s, int field, double* h) {
+  API_BEGIN
+  auto& f = field_of(s, field);
+  long long o = 0;
+  for (int l = 0; l < s->nlev(); l++) {
+    download_valid(*s->H.amrLay[l], f[l]->d, h + o);
+    o += s->H.amrLay[l]->valid_cells;
+  }
+  API_END
+}
+
+int amrmg_solver_solve(amrmg_solver* s, int zero_phi, double* hist, int hist_cap, int* info) {
+  API_BEGIN
+  auto ph = s->ptrs(s->phi), rh = s->ptrs(s->rhs), ur = s->ptrs(s->uberRes), uc = s->ptrs(s->uberCorr);
+  long long b0 = s->bottom.m_total_iters;
+  s->amg.solveNoInitResid(ph, ur, uc, rh, s->nlev() - 1, 0, zero_phi != 0);
+  s->ctx->c->sync();
+  if (hist)
+    for (int q = 0; q < hist_cap && q < (int)s->amg.m_history.size(); q++) hist[q] = s->amg.m_history[q];
+  if (info) {
+    info[0] = s->amg.m_iterations;
+    info[1] = s->amg.m_exitStatus;
+    info[2] = (int)(s->bottom.m_total_iters - b0);
+  }
+  API_END
+}
+int amrmg_solver_residual(amrmg_solver* s, int homog, double* norm_out) {
+  API_BEGIN
+  auto ph = s->ptrs(s->phi), rh = s->ptrs(s->rhs), rs = s->ptrs(s->res);
+  double n = s->amg.computeAMRResidual(rs, ph, rh, s->nlev() - 1, 0, homog != 0, true);
+  if (norm_out) *norm_out = n;
+  API_END
+}
+int amrmg_solver_operator(amrmg_solver* s, int homog) {
+  API_BEGIN
+  auto ph = s->ptrs(s->phi), rs = s->ptrs(s->res);
+  s->amg.computeAMROperator(rs, ph, s->nlev() - 1, 0, homog != 0);
+  s->ctx->c->sync();
+  API_END
+}
+
+int amrmg_level_apply(amrmg_solver* s, int depth, const double* phi, double* out, int homog) {
+  API_BEGIN
+  const DeviceLayout* L = s->mglay(depth);
+  LevelData<double> p(L), o(L);
+  upload_valid(*L, p.d, phi);
+  s->mgop(depth)->applyOp(o, p, homog != 0);
+  download_valid(*L, o.d, out);
+  API_END
+}
+int amrmg_level_residual(amrmg_solver* s, int depth, const double* phi, const double* rhs, double* out,
+                         int homog) {
+  API_BEGIN
+  const DeviceLayout* L = s->mglay(depth);
+  LevelData<double> p(L), r(L), o(L);
+  upload_valid(*L, p.d, phi);
+  upload_valid(*L, r.d, rhs);
+  s->mgop(depth)->residual(o, p, r, homog != 0);
+  download_valid(*L, o.d, out);
+  API_END
+}
+int amrmg_level_relax(amrmg_solver* s, int depth, double* phi, const double* rhs, int iters) {
+  API_BEGIN
+  const DeviceLayout* L = s->mglay(depth);
+  LevelData<double> p(L), r(L);
+  upload_valid(*L, p.d, phi);
+  upload_valid(*L, r.d, rhs);
+  s->mgop(depth)->relax(p, r, iters);
+  download_valid(*L, p.d, phi);
+  API_END
+}
+int amrmg_level_restrict(amrmg_solver* s, int depth, const double* phi, const double* rhs, double* resc) {
+  API_BEGIN
+  if (depth + 1 >= s->H.mgDepth()) throw std::runtime_error("no coarser multigrid level");
+  const DeviceLayout* L = s->mglay(depth);
+  const DeviceLayout* Lc = s->mglay(depth + 1);
+  LevelData<double> p(L), r(L), c(Lc);
+  upload_valid(*L, p.d, phi);
+  upload_valid(*L, r.d, rhs);
+  s->mgop(depth)->restrictResidual(c, p, r);
+  download_valid(*Lc, c.d, resc);
+  API_END
+}
+int amrmg_level_prolong(amrmg_solver* s, int depth, double* phi, const double* ec) {
+  API_BEGIN
+  if (depth + 1 >= s->H.mgDepth()) throw std::runtime_error("no coarser multigrid level");
+  const DeviceLayout* L = s->mglay(depth);
+  const DeviceLayout* Lc = s->mglay(depth + 1);
+  LevelData<double> p(L), c(Lc);
+  upload_valid(*L, p.d, phi);
+  upload_valid(*Lc, c.d, ec);
+  s->mgop(depth)->prolongIncrement(p, c);
+  download_valid(*L, p.d, phi);
+  API_END
+}
+int amrmg_kernel_gsrb(amrmg_solver* s, int lev, double* phi_g, const double* rhs, int color) {
+  API_BEGIN
+  if (lev < 0 || lev >= s->nlev()) throw std::runtime_error("bad level");
+  const DeviceLayout* L = s->H.amrLay[lev].get();
+  LevelData<double> p(L), r(L);
+  upload_ghosted(*L, p.d, phi_g);
+  upload_valid(*L, r.d, rhs);
+  ops::gsrb<double>(*L, p.d, r.d, color, s->H.dxs[lev], s->alpha, s->beta);
+  download_ghosted(*L, p.d, phi_g);
+  API_END
+}
+int amrmg_homog_cf(amrmg_solver* s, int lev, double* phi_g) {
+  API_BEGIN
+  if (lev < 0 || lev >= s->nlev()) throw std::runtime_error("bad level");
+  const DeviceLayout* L = s->H.amrLay[lev].get();
+  LevelData<double> p(L);
+  upload_ghosted(*L, p.d, phi_g);
+  AMRPoissonOp<double>* op = s->amg.m_op[lev].get();
+  ops::fill_ghosts<double>(*L, op->m_ghost, p.d, 2, 1, op->m_dx, op->m_dxCrse);
+  download_ghosted(*L, p.d, phi_g);
+  API_END
+}
+int amrmg_quad_cf(amrmg_solver* s, int lev, double* phi_g, const double* coarse) {
+  API_BEGIN
+  if (lev < 1 || lev >= s->nlev()) throw std::runtime_error("bad level");
+  const DeviceLayout* L = s->H.amrLay[lev].get();
+  const DeviceLayout* Lc = s->H.amrLay[lev - 1].get();
+  LevelData<double> p(L), c(Lc);
+  upload_ghosted(*L, p.d, phi_g);
+  upload_valid(*Lc, c.d, coarse);
+  s->amg.m_op[lev]->coarseFineInterp(p, c);
+  download_ghosted(*L, p.d, phi_g);
+  API_END
+}
+int amrmg_average(amrmg_solver* s, int lev, const double* fine, double* coarse) {
+  API_BEGIN
+  if (lev < 1 || lev >= s->nlev()) throw std::runtime_error("bad level");
+  const DeviceLayout* L = s->H.amrLay[lev].get();
+  const DeviceLayout* Lc = s->H.cfLay[lev].get();
+  LevelData<double> f(L), c(Lc);
+  upload_valid(*L, f.d, fine);
+  ops::average<double>(*L, *Lc, c.d, f.d);
+  download_valid(*Lc, c.d, coarse);
+  API_END
+}
+int amrmg_bottom_solve(amrmg_solver* s, double* corr, const double* res, int nbottom, double metric, double eps,
+                       int* info) {
+  API_BEGIN
+  int d = s->H.mgDepth() - 1;
+  const DeviceLayout* L = s->mglay(d);
+  LevelData<double> c(L), r(L);
+  upload_valid(*L, c.d, corr);
+  upload_valid(*L, r.d, res);
+  AMRPoissonOp<double>* op = s->mgop(d);
+  BiCGStabSolver<LevelData<double>> b;
+  b.define(op, true);
+  b.setConvergenceMetrics(metric, eps);
+  op->relax(c, r, nbottom);
+  b.solve(c, r);
+  download_valid(*L, c.d, corr);
+  if (info) info[0] = b.m_exitStatus;
+  API_END
+}
+
+int amrmg_plan_check(const int* ispec, const double* rspec, long long* stats, int nstats) {
+  API_BEGIN
+  Context dry(0, true);
+  Box dom;
+  double dx0, a, b;
+  BCSpec bc{};
+  std::vector<DBL> g = parse_spec(ispec, rspec, dom, bc, dx0, a, b);
+  DeviceHierarchy H;
+  H.define(&dry, g, dom, dx0);
+  AMRPoissonOpFactory<double> fac;
+  fac.define(&H, a, b, bc);
+  BiCGStabSolver<LevelData<double>> bottom;
+  AMRMultiGrid<double> amg;
+  amg.define(fac, &bottom, H.nlev());
+  long long st[8] = {H.nlev(), H.mgDepth(), 0, 0, 0, 0, 0, 0};
+  for (auto& op : amg.m_op) {
+    st[2] += (long long)op->m_ghost.h_patches.size();
+    st[3] += op->n_cfc;
+    st[4] += op->n_cff;
+    st[5] += op->n_rcell;
+    st[6] += op->m_lay->ntiles;
+  }
+  for (auto& l : H.mgLay) st[6] += l->ntiles;
+  for (int l = 0; l < H.nlev(); l++) st[7] += H.amrLay[l]->valid_cells;
+  for (int q = 0; q < nstats && q < 8; q++) stats[q] = st[q];
+  API_END
+}
+
+int amrmg_profile_enable(amrmg_ctx* ctx, int class_mask) {
+  API_BEGIN
+  profiler_enable(ctx->c.get(), (unsigned)class_mask);
+  API_END
+}
+int amrmg_profile_reset(amrmg_ctx* ctx) {
+  API_BEGIN
+  profiler_reset(ctx->c.get());
+  API_END
+}
+int amrmg_profile_read(amrmg_ctx* ctx, int kind, double* ms, double* bytes, long long* launches) {
+  API_BEGIN
+  if (!profiler_read(ctx->c.get(), kind, ms, bytes, launches)) throw std::runtime_error("profiler off or bad class");
+  API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,167 @@
+// Device-resident data model of the AMR Poisson/Helmholtz path.
+//
+// A level (one DisjointBoxLayout) lives in HBM as a box table plus, per field,
+// one slab holding every box's FAB (valid box grown by one ghost cell, Fortran
+// order, i fastest).  All kernels address a cell through the box table:
+//   slab[ box.off + (i-lo0+1) + (j-lo1+1)*sy + (k-lo2+1)*sz ].
+// Every field of a level shares the same box table, so a kernel over several
+// fields uses one index.  Work over the valid cells of all boxes is enumerated by
+// a tile table (one 256-thread workgroup per tile), so every per-box Fortran call
+// of the reference (FORT_* in AMRPoissonOp.cpp) becomes one launch per level.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "geometry.hpp"
+
+namespace amrmg {
+
+// ---------------- POD tables shared by host and device ----------------
+struct DBox {
+  int lo[3], hi[3];
+  long long off;  // slab index of cell (lo-1, lo-1, lo-1)
+  int sy, sz;     // strides in y and z (x stride is 1)
+};
+
+struct Tile {
+  int box, i0, j0, k0;
+};
+
+// One rectangle of face-ghost cells and the rule that fills it.
+enum PatchKind : int {
+  PK_EXCHANGE = 0,   // copy from the valid cells of box `src` (Copier exchange)
+  PK_DIRICHLET = 1,  // ConstBCFunction type 1: g = 2*v - near
+  PK_NEUMANN = 2,    // ConstBCFunction type 0: g = near (+ isign*dx*v)
+  PK_CFHOMO = 3,     // INTERPHOMO: g = c1*phi(1 in) + c2*phi(2 in)
+  PK_CFHOMOLIN = 4,  // INTERPHOMOLINEAR: g = factor*phi(1 in)
+};
+struct Patch {
+  int kind, dst, src, dir, side;
+  int lo[3], hi[3];
+  double val;
+};
+struct PatchWork {  // a patch split into chunks of at most 256 cells
+  int patch, start;
+};
+
+// Motion item of a Copier between two layouts (same index space)
+struct Motion {
+  int from, to;
+  int lo[3], hi[3];
+};
+struct MotionWork {
+  int item, start;
+};
+
+// Inhomogeneous coarse-fine interpolation (QuadCFInterp), coarse side: one record
+// per (fine box, face, coarse cell under the face).  All offsets index the
+// coarse level's slab.
+enum DerivKind : int { DK_STD = 0, DK_GEN = 1, DK_ZERO = 2 };
+struct CFCoarse {
+  int pc;              // coarse cell
+  int std;             // 1: all derivatives use the centred formulas
+  int nbr[8];          // std: you1-, you1+, you2-, you2+, ur, ul, lr, ll
+  int gen;             // index into the generic pool when !std
+};
+struct CFGen {
+  int kind[5];         // first(you1), second(you1), first(you2), second(you2), mixed
+  int n[5];
+  int off[5][9];
+  double w[5][9];
+};
+// fine side: one record per CF ghost cell
+struct CFFine {
+  long long g;         // ghost cell (fine slab)
+  long long a1, a2;    // one / two cells inward
+  int coarse;          // CFCoarse index
+  int packed;          // FORT_QUADINTERP (1) or interpOnIVS C++ loop (0)
+  double x1, x2;       // tangential offsets, computed on the host as the reference does
+};
+
+// Reflux (LevelFluxRegister + AMRPoissonOp::reflux): one record per coarse cell
+// that the reference's registers touch, with its contributions in reference order.
+struct RefluxCell {
+  long long u;         // output cell in the coarse residual/operator slab
+  int c0, nc;          // coarse flux contributions [c0, c0+nc)
+  int f0, nf;          // fine register contributions [f0, f0+nf)
+};
+struct RefluxCoarse {  // reg += (phi[a]-phi[b])*(beta/dx) * sign
+  long long a, b;
+  double sign;
+};
+struct RefluxFine {    // reg_f = sum_q scale*(phi_f[a_q]-phi_f[b_q])*(beta*2/dx)
+  long long a[4], b[4];
+  double scale;
+};
+
+// ---------------- host-side runtime ----------------
+// Kernel classes timed by the event profiler (bench roofline measurement).
+enum KernelClass : int {
+  KC_GSRB = 0, KC_RESIDUAL, KC_APPLY, KC_RESTRICT, KC_PROLONG, KC_AVERAGE, KC_GHOST, KC_BLAS, KC_REDUCE,
+  KC_CF, KC_REFLUX, KC_MOTION, KC_COUNT
+};
+struct Profiler;
+
+struct Context {
+  int device = 0;
+  bool dry = false;                   // host-only: build and check tables, never touch a device
+  Profiler* prof = nullptr;           // non-null while event profiling is on
+  hipStream_t stream = nullptr;
+  double* d_scalars = nullptr;        // small device scratch for reductions
+  double* h_scalars = nullptr;        // pinned host mirror
+  std::string last_error;
+  explicit Context(int dev, bool dry_run = false);
+  ~Context();
+  void sync();
+};
+
+// device allocation / copy helpers that are no-ops for a dry (host-only) context
+void dev_alloc(Context* c, void** p, size_t bytes, const char* what);
+void dev_free(Context* c, void* p);
+void dev_h2d(Context* c, void* dst, const void* src, size_t bytes, const char* what);
+
+struct DeviceLayout {
+  DBL dbl;
+  Box domain;
+  std::vector<DBox> h_boxes;
+  DBox* d_boxes = nullptr;
+  long long slab = 0;      // elements per field
+  Tile* d_tiles = nullptr;
+  int ntiles = 0;
+  int TX = 64, TY = 4, TZ = 1;
+  long long valid_cells = 0;
+  Context* ctx = nullptr;
+  DeviceLayout(Context* c, const DBL& d, const Box& dom);
+  ~DeviceLayout();
+  DeviceLayout(const DeviceLayout&) = delete;
+  DeviceLayout& operator=(const DeviceLayout&) = delete;
+  long long index(int b, int i, int j, int k) const {
+    const DBox& x = h_boxes[b];
+    return x.off + (i - x.lo[0] + 1) + (long long)(j - x.lo[1] + 1) * x.sy + (long long)(k - x.lo[2] + 1) * x.sz;
+  }
+};
+
+enum Prec : int { PREC_F64 = 0, PREC_F32 = 1 };
+
+// A LevelData<FArrayBox> resident in HBM (ncomp = 1, ghost width 1).
+template <class T>
+struct LevelData {
+  const DeviceLayout* lay = nullptr;
+  T* d = nullptr;
+  LevelData() = default;
+  explicit LevelData(const DeviceLayout* l) { define(l); }
+  ~LevelData() { release(); }
+  LevelData(const LevelData&) = delete;
+  LevelData& operator=(const LevelData&) = delete;
+  void define(const DeviceLayout* l);
+  void release();
+  bool isDefined() const { return d != nullptr; }
+};
+
+void hip_check(hipError_t e, const char* what);
+
+}  // namespace amrmg

@@ -1,0 +1,929 @@
+// HIP kernels for gfx950 (MI355X) of the AMR Poisson/Helmholtz hot path, plus the
+// device runtime (context, layouts, fields).
+//
+// Arithmetic in every FP64 kernel follows the expression order of the reference
+// kernel it replaces (cited per kernel) and the file is compiled with
+// -ffp-contract=off, so an FP64 result is bitwise the reference's result for the
+// same inputs.  Kernels are HBM-bound 7-point stencils and BLAS-1 sweeps; none is
+// a dense contraction, so none uses MFMA.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <stdexcept>
+
+#include "ops.hpp"
+
+namespace amrmg {
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) {
+    throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// runtime
+// ---------------------------------------------------------------------------
+void dev_alloc(Context* c, void** p, size_t bytes, const char* what) {
+  *p = nullptr;
+  if (c->dry || bytes == 0) return;
+  hip_check(hipMalloc(p, bytes), what);
+}
+void dev_free(Context* c, void* p) {
+  if (c->dry || !p) return;
+  (void)hipFree(p);
+}
+void dev_h2d(Context* c, void* dst, const void* src, size_t bytes, const char* what) {
+  if (c->dry || bytes == 0) return;
+  hip_check(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), what);
+}
+
+Context::Context(int dev, bool dry_run) : device(dev), dry(dry_run) {
+  if (dry) return;
+  hip_check(hipSetDevice(dev), "hipSetDevice");
+  hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+  hip_check(hipMalloc(&d_scalars, 64 * sizeof(double)), "hipMalloc scalars");
+  hip_check(hipHostMalloc(&h_scalars, 64 * sizeof(double)), "hipHostMalloc scalars");
+  hip_check(hipMemset(d_scalars, 0, 64 * sizeof(double)), "hipMemset scalars");
+}
+// ---------------------------------------------------------------------------
+// event profiler: start/stop events around each launch of a kernel class
+// ---------------------------------------------------------------------------
+struct Profiler {
+  unsigned mask = ~0u;
+  struct Rec { int kind; hipEvent_t a, b; double bytes; };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> pool;
+  double ms[KC_COUNT] = {0};
+  double bytes[KC_COUNT] = {0};
+  long long count[KC_COUNT] = {0};
+  hipEvent_t get() {
+    if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
+    hipEvent_t e;
+    hip_check(hipEventCreate(&e), "hipEventCreate");
+    return e;
+  }
+  void drain() {
+    for (auto& r : recs) {
+      hip_check(hipEventSynchronize(r.b), "hipEventSynchronize");
+      float t = 0;
+      hip_check(hipEventElapsedTime(&t, r.a, r.b), "hipEventElapsedTime");
+      ms[r.kind] += t; bytes[r.kind] += r.bytes; count[r.kind] += 1;
+      pool.push_back(r.a); pool.push_back(r.b);
+    }
+    recs.clear();
+  }
+  ~Profiler() {
+    for (auto& r : recs) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
+    for (auto e : pool) hipEventDestroy(e);
+  }
+};
+static hipEvent_t prof_begin(Context* c, int kind) {
+  if (!c->prof || !(c->prof->mask & (1u << kind))) return nullptr;
+  hipEvent_t a = c->prof->get();
+  hip_check(hipEventRecord(a, c->stream), "hipEventRecord");
+  return a;
+}
+static void prof_end(Context* c, hipEvent_t a, int kind, double bytes) {
+  if (!c->prof || !a) return;
+  hipEvent_t b = c->prof->get();
+  hip_check(hipEventRecord(b, c->stream), "hipEventRecord");
+  c->prof->recs.push_back(Profiler::Rec{kind, a, b, bytes});
+  if (c->prof->recs.size() > 4096) c->prof->drain();
+}
+void profiler_enable(Context* c, unsigned mask) {
+  if (mask && !c->prof) c->prof = new Profiler();
+  if (mask && c->prof) c->prof->mask = mask;
+  if (!mask && c->prof) { hipStreamSynchronize(c->stream); delete c->prof; c->prof = nullptr; }
+}
+void profiler_reset(Context* c) {
+  if (!c->prof) return;
+  hipStreamSynchronize(c->stream);
+  c->prof->drain();
+  for (int q = 0; q < KC_COUNT; q++) { c->prof->ms[q] = 0; c->prof->bytes[q] = 0; c->prof->count[q] = 0; }
+}
+bool profiler_read(Context* c, int kind, double* ms, double* bytes, long long* count) {
+  if (!c->prof || kind < 0 || kind >= KC_COUNT) return false;
+  hipStreamSynchronize(c->stream);
+  c->prof->drain();
+  *ms = c->prof->ms[kind]; *bytes = c->prof->bytes[kind]; *count = c->prof->count[kind];
+  return true;
+}
+
+Context::~Context() {
+  if (dry) return;
+  if (prof) profiler_enable(this, 0u);
+  if (stream) hipStreamSynchronize(stream);
+  if (d_scalars) hipFree(d_scalars);
+  if (h_scalars) hipHostFree(h_scalars);
+  if (stream) hipStreamDestroy(stream);
+}
+void Context::sync() { hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize"); }
+
+static int pow2_at_least(int v) {
+  int p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+DeviceLayout::DeviceLayout(Context* c, const DBL& d, const Box& dom) : dbl(d), domain(dom), ctx(c) {
+  long long off = 0;
+  int maxnx = 1, maxny = 1;
+  h_boxes.resize(d.size());
+  for (int b = 0; b < d.size(); b++) {
+    const Box& bx = d[b];
+    DBox x;
+    for (int q = 0; q < 3; q++) { x.lo[q] = bx.lo[q]; x.hi[q] = bx.hi[q]; }
+    int nx = bx.len(0) + 2, ny = bx.len(1) + 2, nz = bx.len(2) + 2;
+    x.off = off;
+    x.sy = nx;
+    x.sz = nx * ny;
+    off += (long long)nx * ny * nz;
+    h_boxes[b] = x;
+    maxnx = std::max(maxnx, bx.len(0));
+    maxny = std::max(maxny, bx.len(1));
+    valid_cells += bx.npts();
+  }
+  slab = off;
+  TX = std::min(64, pow2_at_least(maxnx));
+  TY = std::min(256 / TX, pow2_at_least(maxny));
+  TZ = 256 / (TX * TY);
+  std::vector<Tile> tiles;
+  for (int b = 0; b < d.size(); b++) {
+    const Box& bx = d[b];
+    for (int k = bx.lo[2]; k <= bx.hi[2]; k += TZ)
+      for (int j = bx.lo[1]; j <= bx.hi[1]; j += TY)
+        for (int i = bx.lo[0]; i <= bx.hi[0]; i += TX) tiles.push_back(Tile{b, i, j, k});
+  }
+  ntiles = (int)tiles.size();
+  // every tile cell index stays inside its box's grown FAB (threads past the
+  // box's high end return), so tiles need only a valid box index
+  for (const Tile& t : tiles)
+    if (t.box < 0 || t.box >= d.size() || !d[t.box].contains(IV{t.i0, t.j0, t.k0}))
+      throw std::runtime_error("tile outside its box");
+  dev_alloc(c, (void**)&d_boxes, sizeof(DBox) * h_boxes.size(), "hipMalloc boxes");
+  dev_h2d(c, d_boxes, h_boxes.data(), sizeof(DBox) * h_boxes.size(), "copy boxes");
+  dev_alloc(c, (void**)&d_tiles, sizeof(Tile) * tiles.size(), "hipMalloc tiles");
+  dev_h2d(c, d_tiles, tiles.data(), sizeof(Tile) * tiles.size(), "copy tiles");
+}
+DeviceLayout::~DeviceLayout() {
+  dev_free(ctx, d_boxes);
+  dev_free(ctx, d_tiles);
+}
+
+template <class T>
+void LevelData<T>::define(const DeviceLayout* l) {
+  release();
+  lay = l;
+  if (l->slab > 0 && !l->ctx->dry) {
+    hip_check(hipMalloc(&d, sizeof(T) * l->slab), "hipMalloc field");
+    hip_check(hipMemsetAsync(d, 0, sizeof(T) * l->slab, l->ctx->stream), "hipMemset field");
+  }
+}
+template <class T>
+void LevelData<T>::release() {
+  if (d) {
+    (void)hipStreamSynchronize(lay->ctx->stream);
+    (void)hipFree(d);
+  }
+  d = nullptr;
+}
+template struct LevelData<double>;
+template struct LevelData<float>;
+
+namespace ops {
+
+void GhostPlan::upload(Context* ctx) {
+  release();
+  npatch = (int)h_patches.size();
+  std::vector<PatchWork> work;
+  for (int p = 0; p < npatch; p++) {
+    const Patch& x = h_patches[p];
+    long long n = (long long)(x.hi[0] - x.lo[0] + 1) * (x.hi[1] - x.lo[1] + 1) * (x.hi[2] - x.lo[2] + 1);
+    for (long long s = 0; s < n; s += 256) work.push_back(PatchWork{p, (int)s});
+  }
+  nwork = (int)work.size();
+  owner = ctx;
+  dev_alloc(ctx, (void**)&d_patches, sizeof(Patch) * npatch, "hipMalloc patches");
+  dev_h2d(ctx, d_patches, h_patches.data(), sizeof(Patch) * npatch, "copy patches");
+  dev_alloc(ctx, (void**)&d_work, sizeof(PatchWork) * nwork, "hipMalloc patch work");
+  dev_h2d(ctx, d_work, work.data(), sizeof(PatchWork) * nwork, "copy patch work");
+}
+void GhostPlan::release() {
+  if (owner) {
+    dev_free(owner, d_patches);
+    dev_free(owner, d_work);
+  }
+  d_patches = nullptr;
+  d_work = nullptr;
+}
+void MotionPlan::upload(Context* ctx) {
+  release();
+  nitems = (int)h_items.size();
+  std::vector<MotionWork> work;
+  for (int p = 0; p < nitems; p++) {
+    const Motion& x = h_items[p];
+    long long n = (long long)(x.hi[0] - x.lo[0] + 1) * (x.hi[1] - x.lo[1] + 1) * (x.hi[2] - x.lo[2] + 1);
+    for (long long s = 0; s < n; s += 256) work.push_back(MotionWork{p, (int)s});
+  }
+  nwork = (int)work.size();
+  owner = ctx;
+  dev_alloc(ctx, (void**)&d_items, sizeof(Motion) * nitems, "hipMalloc motion");
+  dev_h2d(ctx, d_items, h_items.data(), sizeof(Motion) * nitems, "copy motion");
+  dev_alloc(ctx, (void**)&d_work, sizeof(MotionWork) * nwork, "hipMalloc motion work");
+  dev_h2d(ctx, d_work, work.data(), sizeof(MotionWork) * nwork, "copy motion work");
+}
+void MotionPlan::release() {
+  if (owner) {
+    dev_free(owner, d_items);
+    dev_free(owner, d_work);
+  }
+  d_items = nullptr;
+  d_work = nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// index helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ long long cidx(const DBox& b, int i, int j, int k) {
+  return b.off + (i - b.lo[0] + 1) + (long long)(j - b.lo[1] + 1) * b.sy + (long long)(k - b.lo[2] + 1) * b.sz;
+}
+
+struct TileGeom {
+  int lx, ly;  // log2 TX, log2 TY
+};
+static TileGeom geom(const DeviceLayout& L) {
+  TileGeom g;
+  g.lx = 0;
+  while ((1 << g.lx) < L.TX) g.lx++;
+  g.ly = 0;
+  while ((1 << g.ly) < L.TY) g.ly++;
+  return g;
+}
+// cell of this thread in its tile; false if outside the box
+__device__ __forceinline__ bool tile_cell(const DBox* __restrict__ boxes, const Tile* __restrict__ tiles,
+                                          TileGeom g, DBox& b, int& i, int& j, int& k) {
+  const Tile t = tiles[blockIdx.x];
+  b = boxes[t.box];
+  const int tid = threadIdx.x;
+  i = t.i0 + (tid & ((1 << g.lx) - 1));
+  j = t.j0 + ((tid >> g.lx) & ((1 << g.ly) - 1));
+  k = t.k0 + (tid >> (g.lx + g.ly));
+  return i <= b.hi[0] && j <= b.hi[1] && k <= b.hi[2];
+}
+
+#define LAUNCH_P(kind, bytes, kernel, grid, ...)                                         \
+  do {                                                                                   \
+    if ((grid) > 0) {                                                                    \
+      hipEvent_t ev_ = prof_begin(L.ctx, kind);                                                \
+      hipLaunchKernelGGL(kernel, dim3(grid), dim3(256), 0, L.ctx->stream, __VA_ARGS__);   \
+      hip_check(hipGetLastError(), #kernel);                                             \
+      prof_end(L.ctx, ev_, kind, bytes);                                                 \
+    }                                                                                    \
+  } while (0)
+#define LAUNCH(kernel, grid, ...) LAUNCH_P(KC_BLAS, 0.0, kernel, grid, __VA_ARGS__)
+
+// ---------------------------------------------------------------------------
+// face-ghost fill: Copier exchange (Copier.cpp:1464-1541 with trimEdges),
+// ConstBCFunction (BCFunc.cpp:58-158), homogeneous CF (AMRPoissonOp.cpp:1598-1794,
+// InterpF.ChF:435-494).  All three write disjoint ghost cells and read valid cells
+// only, so one launch performs them all.
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(256) k_fill_ghosts(const DBox* __restrict__ boxes, const Patch* __restrict__ patches,
+                                                     const PatchWork* __restrict__ work, T* __restrict__ phi,
+                                                     int doCF, int homog, double dx, double dxCrse) {
+  const PatchWork w = work[blockIdx.x];
+  const Patch p = patches[w.patch];
+  const bool isCF = (p.kind == PK_CFHOMO || p.kind == PK_CFHOMOLIN);
+  if (isCF && !doCF) return;
+  if (!isCF && doCF == 2) return;  // coarse-fine rules only
+  const int nx = p.hi[0] - p.lo[0] + 1, ny = p.hi[1] - p.lo[1] + 1, nz = p.hi[2] - p.lo[2] + 1;
+  const int q = w.start + threadIdx.x;
+  if (q >= nx * ny * nz) return;
+  const int i = p.lo[0] + q % nx, j = p.lo[1] + (q / nx) % ny, k = p.lo[2] + q / (nx * ny);
+  const DBox bd = boxes[p.dst];
+  const long long g = cidx(bd, i, j, k);
+  if (p.kind == PK_EXCHANGE) {
+    const DBox bs = boxes[p.src];
+    phi[g] = phi[cidx(bs, i, j, k)];
+    return;
+  }
+  const int isign = p.side ? 1 : -1;
+  int a[3] = {i, j, k};
+  a[p.dir] -= isign;  // one cell inward
+  const T nearv = phi[cidx(bd, a[0], a[1], a[2])];
+  if (p.kind == PK_DIRICHLET) {
+    T inhomogVal = T(0.0);
+    if (!homog) inhomogVal = T(p.val);
+    phi[g] = T(2) * inhomogVal - nearv;
+  } else if (p.kind == PK_NEUMANN) {
+    T v = nearv;
+    if (!homog) v += T(double(isign) * dx) * T(p.val);
+    phi[g] = v;
+  } else if (p.kind == PK_CFHOMO) {
+    int b2[3] = {a[0], a[1], a[2]};
+    b2[p.dir] -= isign;  // two cells inward
+    const T c1 = T(2.0 * (dxCrse - dx) / (dxCrse + dx));
+    const T c2 = T(-((dxCrse - dx) / (dxCrse + 3.0 * dx)));
+    const T pb = phi[cidx(bd, b2[0], b2[1], b2[2])];
+    phi[g] = c1 * nearv + c2 * pb;
+  } else {  // PK_CFHOMOLIN
+    const T factor = T(1.0 - 2.0 * dx / (dx + dxCrse));
+    phi[g] = factor * nearv;
+  }
+}
+template <class T>
+void fill_ghosts(const DeviceLayout& L, const GhostPlan& P, T* phi, int doCF, int homog, double dx, double dxCrse) {
+  LAUNCH_P(KC_GHOST, 0.0, k_fill_ghosts<T>, P.nwork, L.d_boxes, P.d_patches, P.d_work, phi, doCF, homog, dx, dxCrse);
+}
+
+// ---------------------------------------------------------------------------
+// GSRBLAPLACIAN (AMRPoissonOpF.ChF:122-198) and GSRBHELMHOLTZ (:53-120): one colour
+// over all boxes; colour by global parity of i+j+k.
+// ---------------------------------------------------------------------------
+template <class T, bool LAP>
+__global__ void __launch_bounds__(256) k_gsrb(const DBox* __restrict__ boxes, const Tile* __restrict__ tiles,
+                                              TileGeom g, T* __restrict__ phi, const T* __restrict__ rhs,
+                                              int color, T dxinv, T lambda, T alpha, T beta) {
+  DBox b;
+  int i, j, k;
+  if (!tile_cell(boxes, tiles, g, b, i, j, k)) return;
+  if ((i + j + k + color) & 1) return;
+  const long long c = cidx(b, i, j, k);
+  const long long sy = b.sy, sz = b.sz;
+  const T s6 = phi[c + 1] + phi[c - 1] + phi[c + sy] + phi[c - sy] + phi[c + sz] + phi[c - sz];
+  if (LAP) {
+    const T lphi = s6 * dxinv;
+    phi[c] = lambda * (rhs[c] - lphi);
+  } else {
+    const T p = phi[c];
+    const T lphi = (s6 - T(6.0) * p) * dxinv;
+    const T helmop = alpha * p + beta * lphi;
+    phi[c] = p + lambda * (helmop - rhs[c]);
+  }
+}
+template <class T>
+void gsrb(const DeviceLayout& L, T* phi, const T* rhs, int color, double dx, double alpha, double beta) {
+  double dxinv = 1.0 / (dx * dx);
+  double sum_b = 0.0;
+  for (int d = 0; d < 3; d++) sum_b = sum_b + 2.0 * dxinv;
+  const bool lap = (alpha == 0.0 && beta == 1.0);
+  double lambda = lap ? -1.0 / sum_b : -1.0 / (alpha - beta * sum_b);
+  if (lap)
+    LAUNCH_P(KC_GSRB, 12.0 * sizeof(T) / 8.0 * L.valid_cells, (k_gsrb<T, true>), L.ntiles, L.d_boxes, L.d_tiles, geom(L), phi, rhs, color, T(dxinv), T(lambda), T(alpha), T(beta));
+  else
+    LAUNCH_P(KC_GSRB, 12.0 * sizeof(T) / 8.0 * L.valid_cells, (k_gsrb<T, false>), L.ntiles, L.d_boxes, L.d_tiles, geom(L), phi, rhs, color, T(dxinv), T(lambda), T(alpha), T(beta));
+}
+
+// OPERATORLAP (AMRPoissonOpF.ChF:287-324)
+template <class T>
+__global__ void __launch_bounds__(256) k_apply(const DBox* __restrict__ boxes, const Tile* __restrict__ tiles,
+                                               TileGeom g, T* __restrict__ lofphi, const T* __restrict__ phi,
+                                               T dxinv, T alpha, T beta) {
+  DBox b;
+  int i, j, k;
+  if (!tile_cell(boxes, tiles, g, b, i, j, k)) return;
+  const long long c = cidx(b, i, j, k);
+  const long long sy = b.sy, sz = b.sz;
+  const T p = phi[c];
+  const T lap = (phi[c + 1] + phi[c - 1] + phi[c + sy] + phi[c - sy] + phi[c + sz] + phi[c - sz] - T(6.0) * p) * dxinv;
+  lofphi[c] = alpha * p + beta * lap;
+}
+template <class T>
+void apply(const DeviceLayout& L, T* lofphi, const T* phi, double dx, double alpha, double beta) {
+  double dxinv = 1.0 / (dx * dx);
+  LAUNCH_P(KC_APPLY, 2.0 * sizeof(T) * L.valid_cells, k_apply<T>, L.ntiles, L.d_boxes, L.d_tiles, geom(L), lofphi, phi, T(dxinv), T(alpha), T(beta));
+}
+
+// OPERATORLAPRES (AMRPoissonOpF.ChF:341-377): r = -alpha*phi - beta*lap + rhs
+template <class T>
+__global__ void __launch_bounds__(256) k_residual(const DBox* __restrict__ boxes, const Tile* __restrict__ tiles,
+                                                  TileGeom g, T* res, const T* __restrict__ phi, const T* rhs,
+                                                  T dxinv, T alpha, T beta) {
+  DBox b;
+  int i, j, k;
+  if (!tile_cell(boxes, tiles, g, b, i, j, k)) return;
+  const long long c = cidx(b, i, j, k);
+  const long long sy = b.sy, sz = b.sz;
+  const T p = phi[c];
+  const T lap = (phi[c + 1] + phi[c - 1] + phi[c + sy] + phi[c - sy] + phi[c + sz] + phi[c - sz] - T(6.0) * p) * dxinv;
+  res[c] = -(alpha * p) - beta * lap + rhs[c];
+}
+template <class T>
+void residual(const DeviceLayout& L, T* res, const T* phi, const T* rhs, double dx, double alpha, double beta) {
+  double dxinv = 1.0 / (dx * dx);
+  LAUNCH_P(KC_RESIDUAL, 3.0 * sizeof(T) * L.valid_cells, k_residual<T>, L.ntiles, L.d_boxes, L.d_tiles, geom(L), res, phi, rhs, T(dxinv), T(alpha), T(beta));
+}
+
+// RESTRICTRES (AMRPoissonOpF.ChF:394-438) after res.setVal(0): one thread per coarse
+// cell accumulates its eight children in the Fortran loop order.
+template <class T>
+__global__ void __launch_bounds__(256) k_restrict(const DBox* __restrict__ cboxes, const Tile* __restrict__ ctiles,
+                                                  TileGeom g, const DBox* __restrict__ fboxes, T* __restrict__ resc,
+                                                  const T* __restrict__ phi, const T* __restrict__ rhs, T dxinv,
+                                                  T alpha, T beta) {
+  DBox cb;
+  int I, J, K;
+  if (!tile_cell(cboxes, ctiles, g, cb, I, J, K)) return;
+  const DBox fb = fboxes[blockIdx.x >= 0 ? ctiles[blockIdx.x].box : 0];
+  const long long sy = fb.sy, sz = fb.sz;
+  const T denom = T(8.0);
+  T acc = T(0.0);
+#pragma unroll
+  for (int kk = 0; kk < 2; kk++)
+#pragma unroll
+    for (int jj = 0; jj < 2; jj++)
+#pragma unroll
+      for (int ii = 0; ii < 2; ii++) {
+        const long long c = cidx(fb, 2 * I + ii, 2 * J + jj, 2 * K + kk);
+        const T p = phi[c];
+        const T lofphi = alpha * p + beta * (phi[c + 1] + phi[c - 1] + phi[c + sy] + phi[c - sy] + phi[c + sz] +
+                                             phi[c - sz] - p * T(2.0) * T(3.0)) * dxinv;
+        acc = acc + (rhs[c] - lofphi) / denom;
+      }
+  resc[cidx(cb, I, J, K)] = acc;
+}
+template <class T>
+void restrict_res(const DeviceLayout& Lf, const DeviceLayout& Lc, T* resc, const T* phi, const T* rhs, double dx,
+                  double alpha, double beta) {
+  const DeviceLayout& L = Lc;
+  double dxinv = 1.0 / (dx * dx);
+  LAUNCH_P(KC_RESTRICT, (2.0 + 1.0 / 8.0) * sizeof(T) * Lf.valid_cells, k_restrict<T>, Lc.ntiles, Lc.d_boxes, Lc.d_tiles, geom(Lc), Lf.d_boxes, resc, phi, rhs, T(dxinv), T(alpha),
+         T(beta));
+}
+
+// PROLONG (AMRPoissonOpF.ChF:453-478), refinement ratio 2
+template <class T>
+__global__ void __launch_bounds__(256) k_prolong(const DBox* __restrict__ fboxes, const Tile* __restrict__ ftiles,
+                                                 TileGeom g, const DBox* __restrict__ cboxes, T* __restrict__ phi,
+                                                 const T* __restrict__ coarse) {
+  DBox fb;
+  int i, j, k;
+  if (!tile_cell(fboxes, ftiles, g, fb, i, j, k)) return;
+  const DBox cb = cboxes[ftiles[blockIdx.x].box];
+  const long long c = cidx(fb, i, j, k);
+  phi[c] = phi[c] + coarse[cidx(cb, i >> 1, j >> 1, k >> 1)];
+}
+template <class T>
+void prolong(const DeviceLayout& Lf, const DeviceLayout& Lc, T* phi, const T* coarse) {
+  const DeviceLayout& L = Lf;
+  LAUNCH_P(KC_PROLONG, (2.0 + 1.0 / 8.0) * sizeof(T) * Lf.valid_cells, k_prolong<T>, Lf.ntiles, Lf.d_boxes, Lf.d_tiles, geom(Lf), Lc.d_boxes, phi, coarse);
+}
+
+// average, refRatio 2 branch (lib/src/AMRTools/AverageF.ChF:40-57)
+template <class T>
+__global__ void __launch_bounds__(256) k_average(const DBox* __restrict__ cboxes, const Tile* __restrict__ ctiles,
+                                                 TileGeom g, const DBox* __restrict__ fboxes, T* __restrict__ coarse,
+                                                 const T* __restrict__ fine) {
+  DBox cb;
+  int I, J, K;
+  if (!tile_cell(cboxes, ctiles, g, cb, I, J, K)) return;
+  const DBox fb = fboxes[ctiles[blockIdx.x].box];
+  const long long c = cidx(fb, 2 * I, 2 * J, 2 * K);
+  const long long sy = fb.sy, sz = fb.sz;
+  const T refScale = T(1.0 / 8.0);
+  coarse[cidx(cb, I, J, K)] = refScale * (fine[c] + fine[c + 1] + fine[c + sy] + fine[c + 1 + sy] + fine[c + sz] +
+                                          fine[c + 1 + sz] + fine[c + sy + sz] + fine[c + 1 + sy + sz]);
+}
+template <class T>
+void average(const DeviceLayout& Lf, const DeviceLayout& Lc, T* coarse, const T* fine) {
+  const DeviceLayout& L = Lc;
+  LAUNCH_P(KC_AVERAGE, (1.0 + 1.0 / 8.0) * sizeof(T) * Lf.valid_cells, k_average<T>, Lc.ntiles, Lc.d_boxes, Lc.d_tiles, geom(Lc), Lf.d_boxes, coarse, fine);
+}
+
+// ---------------------------------------------------------------------------
+// LevelDataOps (LevelDataOps.H:93-294)
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(256) k_incr(const DBox* __restrict__ boxes, const Tile* __restrict__ tiles,
+                                              TileGeom g, T* lhs, const T* x, T scale) {
+  DBox b;
+  int i, j, k;
+  if (!tile_cell(boxes, tiles, g, b, i, j, k)) return;
+  const long long c = cidx(b, i, j, k);
+  lhs[c] = lhs[c] + x[c] * scale;
+}
+template <class T>
+void incr(const DeviceLayout& L, T* lhs, const T* x, double scale) {
+  LAUNCH(k_incr<T>, L.ntiles, L.d_boxes, L.d_tiles, geom(L), lhs, x, T(scale));
+}
+template <class T>
+__global__ void __launch_bounds__(256) k_copy_valid(const DBox* __restrict__ boxes, const Tile* __restrict__ tiles,
+                                                    TileGeom g, T* dst, const T* src) {
+  DBox b;
+  int i, j, k;
+  if (!tile_cell(boxes, tiles, g, b, i, j, k)) return;
+  const long long c = cidx(b, i, j, k);
+  dst[c] = src[c];
+}
+template <class T>
+void copy_valid(const DeviceLayout& L, T* dst, const T* src) {
+  LAUNCH(k_copy_valid<T>, L.ntiles, L.d_boxes, L.d_tiles, geom(L), dst, src);
+}
+
+static int flat_grid(long long n) { return (int)std::min<long long>((n + 255) / 256, 1 << 20); }
+
+template <class T>
+__global__ void __launch_bounds__(256) k_axby(long long n, T* lhs, const T* x, const T* y, T a, T b) {
+  for (long long q = blockIdx.x * 256LL + threadIdx.x; q < n; q += (long long)gridDim.x * 256) {
+    T v = x[q];
+    v = v * a;
+    lhs[q] = v + y[q] * b;
+  }
+}
+template <class T>
+void axby(const DeviceLayout& L, T* lhs, const T* x, const T* y, double a, double b) {
+  LAUNCH(k_axby<T>, flat_grid(L.slab), L.slab, lhs, x, y, T(a), T(b));
+}
+template <class T>
+__global__ void __launch_bounds__(256) k_scale(long long n, T* x, T s) {
+  for (long long q = blockIdx.x * 256LL + threadIdx.x; q < n; q += (long long)gridDim.x * 256) x[q] = x[q] * s;
+}
+template <class T>
+void scale(const DeviceLayout& L, T* x, double s) {
+  LAUNCH(k_scale<T>, flat_grid(L.slab), L.slab, x, T(s));
+}
+template <class T>
+__global__ void __launch_bounds__(256) k_setval(long long n, T* x, T v) {
+  for (long long q = blockIdx.x * 256LL + threadIdx.x; q < n; q += (long long)gridDim.x * 256) x[q] = v;
+}
+template <class T>
+void setval(const DeviceLayout& L, T* x, double v) {
+  LAUNCH(k_setval<T>, flat_grid(L.slab), L.slab, x, T(v));
+}
+template <class T>
+__global__ void __launch_bounds__(256) k_copy_all(long long n, T* dst, const T* src) {
+  for (long long q = blockIdx.x * 256LL + threadIdx.x; q < n; q += (long long)gridDim.x * 256) dst[q] = src[q];
+}
+template <class T>
+void copy_all(const DeviceLayout& L, T* dst, const T* src) {
+  LAUNCH(k_copy_all<T>, flat_grid(L.slab), L.slab, dst, src);
+}
+// preCond (AMRPoissonOp.cpp:304-329): phi = rhs on the valid region, then phi *= mult
+// on the whole FAB.  Ghost cells of phi are never read before the next refresh, so
+// scaling valid cells gives the same valid values.
+template <class T>
+__global__ void __launch_bounds__(256) k_precond_init(const DBox* __restrict__ boxes, const Tile* __restrict__ tiles,
+                                                      TileGeom g, T* phi, const T* rhs, T mult) {
+  DBox b;
+  int i, j, k;
+  if (!tile_cell(boxes, tiles, g, b, i, j, k)) return;
+  const long long c = cidx(b, i, j, k);
+  phi[c] = rhs[c] * mult;
+}
+template <class T>
+void precond_init(const DeviceLayout& L, T* phi, const T* rhs, double mult) {
+  LAUNCH(k_precond_init<T>, L.ntiles, L.d_boxes, L.d_tiles, geom(L), phi, rhs, T(mult));
+}
+__global__ void __launch_bounds__(256) k_d2f(const DBox* __restrict__ boxes, const Tile* __restrict__ tiles,
+                                             TileGeom g, float* dst, const double* src) {
+  DBox b;
+  int i, j, k;
+  if (!tile_cell(boxes, tiles, g, b, i, j, k)) return;
+  const long long c = cidx(b, i, j, k);
+  dst[c] = (float)src[c];
+}
+void convert_valid_d2f(const DeviceLayout& L, float* dst, const double* src) {
+  LAUNCH(k_d2f, L.ntiles, L.d_boxes, L.d_tiles, geom(L), dst, src);
+}
+__global__ void __launch_bounds__(256) k_incr_f2d(const DBox* __restrict__ boxes, const Tile* __restrict__ tiles,
+                                                  TileGeom g, double* lhs, const float* x) {
+  DBox b;
+  int i, j, k;
+  if (!tile_cell(boxes, tiles, g, b, i, j, k)) return;
+  const long long c = cidx(b, i, j, k);
+  lhs[c] = lhs[c] + (double)x[c];
+}
+void incr_f2d(const DeviceLayout& L, double* lhs, const float* x) {
+  LAUNCH(k_incr_f2d, L.ntiles, L.d_boxes, L.d_tiles, geom(L), lhs, x);
+}
+
+// ---------------------------------------------------------------------------
+// Copier motion (copyTo / copyToZero) between two layouts
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(256) k_motion(const Motion* __restrict__ items, const MotionWork* __restrict__ work,
+                                                const DBox* __restrict__ sboxes, const DBox* __restrict__ dboxes,
+                                                T* dst, const T* src, int zero) {
+  const MotionWork w = work[blockIdx.x];
+  const Motion m = items[w.item];
+  const int nx = m.hi[0] - m.lo[0] + 1, ny = m.hi[1] - m.lo[1] + 1, nz = m.hi[2] - m.lo[2] + 1;
+  const int q = w.start + threadIdx.x;
+  if (q >= nx * ny * nz) return;
+  const int i = m.lo[0] + q % nx, j = m.lo[1] + (q / nx) % ny, k = m.lo[2] + q / (nx * ny);
+  const long long dc = cidx(dboxes[m.to], i, j, k);
+  if (zero) dst[dc] = T(0.0);
+  else dst[dc] = src[cidx(sboxes[m.from], i, j, k)];
+}
+template <class T>
+void motion(const MotionPlan& P, const DeviceLayout& Lsrc, const DeviceLayout& Ldst, T* dst, const T* src, int zero) {
+  const DeviceLayout& L = Ldst;
+  LAUNCH_P(KC_MOTION, 0.0, k_motion<T>, P.nwork, P.d_items, P.d_work, Lsrc.d_boxes, Ldst.d_boxes, dst, src, zero);
+}
+
+// ---------------------------------------------------------------------------
+// reductions
+// ---------------------------------------------------------------------------
+__global__ void k_zero_scalar(double* s, int slot) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) s[slot] = 0.0;
+}
+// FArrayBox::norm(p = 0) over valid boxes; max is exact, so any order gives the
+// reference's value.  Non-negative doubles order like their bit patterns.
+template <class T>
+__global__ void __launch_bounds__(256) k_maxnorm(const DBox* __restrict__ boxes, const Tile* __restrict__ tiles,
+                                                 TileGeom g, const T* __restrict__ x, double* out) {
+  DBox b;
+  int i, j, k;
+  double v = 0.0;
+  if (tile_cell(boxes, tiles, g, b, i, j, k)) v = fabs((double)x[cidx(b, i, j, k)]);
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  __shared__ double sm[4];
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = fmax(fmax(sm[0], sm[1]), fmax(sm[2], sm[3]));
+    if (m > 0.0) atomicMax((unsigned long long*)out, (unsigned long long)__double_as_longlong(m));
+  }
+}
+template <class T>
+void maxnorm_async(const DeviceLayout& L, const T* x, int slot, int accumulate) {
+  if (!accumulate) LAUNCH(k_zero_scalar, 1, L.ctx->d_scalars, slot);
+  LAUNCH_P(KC_REDUCE, 1.0 * sizeof(T) * L.valid_cells, k_maxnorm<T>, L.ntiles, L.d_boxes, L.d_tiles, geom(L), x, L.ctx->d_scalars + slot);
+}
+template <class T>
+double maxnorm(const DeviceLayout& L, const T* x, int slot) {
+  maxnorm_async(L, x, slot, 0);
+  hip_check(hipMemcpyAsync(L.ctx->h_scalars + slot, L.ctx->d_scalars + slot, sizeof(double), hipMemcpyDeviceToHost,
+                           L.ctx->stream),
+            "copy scalar");
+  L.ctx->sync();
+  return L.ctx->h_scalars[slot];
+}
+// LevelDataOps::dotProduct: per box a serial sum in Fortran order
+// (FArrayBox::dotProduct, ForAllThisCBNNXC), boxes summed in layout order.
+template <class T>
+__global__ void __launch_bounds__(256) k_dot_ordered(const DBox* __restrict__ boxes, int nbox, const T* a, const T* b,
+                                                     double* partial, double* out) {
+  for (int q = threadIdx.x; q < nbox; q += blockDim.x) {
+    const DBox bx = boxes[q];
+    double dot = 0.0;
+    for (int k = bx.lo[2]; k <= bx.hi[2]; k++)
+      for (int j = bx.lo[1]; j <= bx.hi[1]; j++)
+        for (int i = bx.lo[0]; i <= bx.hi[0]; i++) {
+          const long long c = cidx(bx, i, j, k);
+          dot += (double)a[c] * (double)b[c];
+        }
+    partial[q] = dot;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double val = 0.0;
+    for (int q = 0; q < nbox; q++) val += partial[q];
+    *out = val;
+  }
+}
+// norm(.., 2): per box FArrayBox::sumPow(2) (column sums over rows, then over
+// columns, FArrayBox.cpp:165-230), boxes summed in layout order, then sqrt.
+template <class T>
+__global__ void __launch_bounds__(256) k_sumpow_ordered(const DBox* __restrict__ boxes, int nbox, const T* x,
+                                                        double* out) {
+  __shared__ double tmp[1024];
+  double norm = 0.0;
+  for (int q = 0; q < nbox; q++) {
+    const DBox bx = boxes[q];
+    const int nx = bx.hi[0] - bx.lo[0] + 1;
+    for (int i = threadIdx.x; i < nx; i += blockDim.x) {
+      double t = 0.0;
+      bool first = true;
+      for (int k = bx.lo[2]; k <= bx.hi[2]; k++)
+        for (int j = bx.lo[1]; j <= bx.hi[1]; j++) {
+          const double r = (double)x[cidx(bx, bx.lo[0] + i, j, k)];
+          if (first) { t = r * r; first = false; }
+          else t += r * r;
+        }
+      tmp[i] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double sum = tmp[0];
+      for (int i = 1; i < nx; i++) sum += tmp[i];
+      norm += sum;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = norm;
+}
+static double* scratch_partials(Context* ctx, int n) {
+  static thread_local double* buf = nullptr;
+  static thread_local int cap = 0;
+  static thread_local Context* owner = nullptr;
+  if (n > cap || owner != ctx) {
+    if (buf) { hipStreamSynchronize(ctx->stream); hipFree(buf); }
+    cap = std::max(n, 1024);
+    hip_check(hipMalloc(&buf, sizeof(double) * cap), "hipMalloc partials");
+    owner = ctx;
+  }
+  return buf;
+}
+template <class T>
+double dot_ordered(const DeviceLayout& L, const T* a, const T* b, int slot) {
+  double* part = scratch_partials(L.ctx, (int)L.h_boxes.size());
+  hipLaunchKernelGGL(k_dot_ordered<T>, dim3(1), dim3(256), 0, L.ctx->stream, L.d_boxes, (int)L.h_boxes.size(), a, b,
+                     part, L.ctx->d_scalars + slot);
+  hip_check(hipGetLastError(), "k_dot_ordered");
+  hip_check(hipMemcpyAsync(L.ctx->h_scalars + slot, L.ctx->d_scalars + slot, sizeof(double), hipMemcpyDeviceToHost,
+                           L.ctx->stream),
+            "copy scalar");
+  L.ctx->sync();
+  return L.ctx->h_scalars[slot];
+}
+template <class T>
+double norm2_ordered(const DeviceLayout& L, const T* x, int slot) {
+  for (auto& b : L.h_boxes)
+    if (b.hi[0] - b.lo[0] + 1 > 1024) throw std::runtime_error("norm2_ordered: box wider than 1024");
+  hipLaunchKernelGGL(k_sumpow_ordered<T>, dim3(1), dim3(256), 0, L.ctx->stream, L.d_boxes, (int)L.h_boxes.size(), x,
+                     L.ctx->d_scalars + slot);
+  hip_check(hipGetLastError(), "k_sumpow_ordered");
+  hip_check(hipMemcpyAsync(L.ctx->h_scalars + slot, L.ctx->d_scalars + slot, sizeof(double), hipMemcpyDeviceToHost,
+                           L.ctx->stream),
+            "copy scalar");
+  L.ctx->sync();
+  return std::sqrt(L.ctx->h_scalars[slot]);
+}
+
+// ---------------------------------------------------------------------------
+// QuadCFInterp::coarseFineInterp: derivatives per coarse record
+// (QuadCFStencil.cpp:36-268), then phistar (QuadCFInterp.cpp:570-764,
+// QuadCFInterpF.ChF:50-126) and the quadratic (QuadCFInterp.cpp:775-839,
+// QuadCFInterpF.ChF:9-47) per fine ghost cell.
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ T gen_sum(const CFGen& g, int q, const T* phic) {
+  T s = T(0.0);
+  for (int t = 0; t < g.n[q]; t++) s += T(g.w[q][t]) * phic[g.off[q][t]];
+  return s;
+}
+template <class T>
+__global__ void __launch_bounds__(256) k_cf_coarse(const CFCoarse* __restrict__ recs, int n, const CFGen* __restrict__ gens,
+                                                   const T* __restrict__ phic, T* __restrict__ out, T dx) {
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= n) return;
+  const CFCoarse c = recs[r];
+  T* o = out + 6LL * r;
+  o[0] = phic[c.pc];
+  if (c.std) {
+    const T pcc = phic[c.pc];
+    T lo = phic[c.nbr[0]], hi = phic[c.nbr[1]];
+    o[1] = (hi - lo) / (T(2.0) * dx);
+    o[2] = (hi + lo - T(2.0) * pcc) / (dx * dx);
+    lo = phic[c.nbr[2]];
+    hi = phic[c.nbr[3]];
+    o[3] = (hi - lo) / (T(2.0) * dx);
+    o[4] = (hi + lo - T(2.0) * pcc) / (dx * dx);
+    const T ur = phic[c.nbr[4]], ul = phic[c.nbr[5]], lr = phic[c.nbr[6]], ll = phic[c.nbr[7]];
+    o[5] = (ur + ll - lr - ul) / (T(4.) * dx * dx);
+    return;
+  }
+  const CFGen g = gens[c.gen];
+  // first derivatives divide by dx, second and mixed by dx*dx
+  for (int q = 0; q < 5; q++) {
+    T v;
+    if (g.kind[q] == DK_ZERO) v = T(0.0);
+    else {
+      v = gen_sum(g, q, phic);
+      if (q == 0 || q == 2) v /= dx;
+      else v /= (dx * dx);
+    }
+    o[1 + q] = v;
+  }
+}
+template <class T>
+__global__ void __launch_bounds__(256) k_cf_fine(const CFFine* __restrict__ recs, int n, const T* __restrict__ deriv,
+                                                 T* __restrict__ phif, T h) {
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= n) return;
+  const CFFine f = recs[r];
+  const T* d = deriv + 6LL * f.coarse;
+  const T x1 = T(f.x1), x2 = T(f.x2);
+  const T update1 = x1 * d[1] + T(0.5) * x1 * x1 * d[2];
+  const T update2 = x2 * d[3] + T(0.5) * x2 * x2 * d[4];
+  const T update3 = x1 * x2 * d[5];
+  const T ps = d[0] + update1 + update2 + update3;
+  const T pa = phif[f.a2];
+  const T pb = phif[f.a1];
+  T g;
+  if (f.packed) {
+    const T frac = T(2.0) / (h * h);
+    const T denom = T(15.0);
+    const T mult = frac / denom;
+    const T invh = T(1.0) / h;
+    const T x = T(2.0) * h;
+    const T xsquared = T(4.0) * h * h;
+    const T a = mult * (T(2.0) * ps + T(3.0) * pa - T(5.0) * pb);
+    const T b = (pb - pa) * invh - a * h;
+    g = xsquared * a + b * x + pa;
+  } else {
+    const T a = (T(2.) / h / h) * (T(2.) * ps + pa * T(3.0) - pb * T(5.0)) / T(15.0);
+    const T b = (pb - pa) / h - a * h;
+    const T x = T(2.) * h;
+    g = a * x * x + b * x + pa;
+  }
+  phif[f.g] = g;
+}
+template <class T>
+void cf_inhomog(Context* ctx, const CFCoarse* dc, int ncoarse, const CFGen* dg, const CFFine* df, int nfine,
+                T* deriv, T* phif, const T* phic, double dxf, double dxc) {
+  hipEvent_t ev = prof_begin(ctx, KC_CF);
+  if (ncoarse > 0) {
+    hipLaunchKernelGGL(k_cf_coarse<T>, dim3((ncoarse + 255) / 256), dim3(256), 0, ctx->stream, dc, ncoarse, dg, phic,
+                       deriv, T(dxc));
+    hip_check(hipGetLastError(), "k_cf_coarse");
+  }
+  if (nfine > 0) {
+    hipLaunchKernelGGL(k_cf_fine<T>, dim3((nfine + 255) / 256), dim3(256), 0, ctx->stream, df, nfine, deriv, phif,
+                       T(dxf));
+    hip_check(hipGetLastError(), "k_cf_fine");
+  }
+  prof_end(ctx, ev, KC_CF, 0.0);
+}
+
+// ---------------------------------------------------------------------------
+// reflux: LevelFluxRegister::incrementCoarse/incrementFine/reflux
+// (LevelFluxRegister.cpp:373-670) with NEWGETFLUX (AMRPoissonOpF.ChF:557-586)
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(256) k_reflux(const RefluxCell* __restrict__ cells, int n,
+                                                const RefluxCoarse* __restrict__ rc, const RefluxFine* __restrict__ rf,
+                                                T* u, const T* __restrict__ phic, const T* __restrict__ phif,
+                                                T beta_dx_c, T beta_ref_dx_c, T minus_s) {
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= n) return;
+  const RefluxCell c = cells[r];
+  T val = u[c.u];
+  if (c.nc > 0) {
+    T reg = T(0.0);
+    for (int q = c.c0; q < c.c0 + c.nc; q++) {
+      const RefluxCoarse x = rc[q];
+      const T F = (phic[x.a] - phic[x.b]) * beta_dx_c;
+      reg += F * T(x.sign);
+    }
+    val = val + reg * minus_s;
+  }
+  for (int q = c.f0; q < c.f0 + c.nf; q++) {
+    const RefluxFine x = rf[q];
+    T reg = T(0.0);
+    for (int t = 0; t < 4; t++) {
+      const T F = (phif[x.a[t]] - phif[x.b[t]]) * beta_ref_dx_c;
+      reg = reg + T(x.scale) * F;
+    }
+    val = val + reg * minus_s;
+  }
+  u[c.u] = val;
+}
+template <class T>
+void reflux(Context* ctx, const RefluxCell* cells, int ncells, const RefluxCoarse* rc, const RefluxFine* rf, T* u,
+            const T* phic, const T* phif, double beta_dx_c, double beta_ref_dx_c, double minus_inv_dx) {
+  if (ncells <= 0) return;
+  hipEvent_t ev = prof_begin(ctx, KC_REFLUX);
+  hipLaunchKernelGGL(k_reflux<T>, dim3((ncells + 255) / 256), dim3(256), 0, ctx->stream, cells, ncells, rc, rf, u, phic,
+                     phif, T(beta_dx_c), T(beta_ref_dx_c), T(minus_inv_dx));
+  hip_check(hipGetLastError(), "k_reflux");
+  prof_end(ctx, ev, KC_REFLUX, 0.0);
+}
+
+// ---------------------------------------------------------------------------
+// explicit instantiations
+// ---------------------------------------------------------------------------
+#define INST(T)                                                                                                   \
+  template void fill_ghosts<T>(const DeviceLayout&, const GhostPlan&, T*, int, int, double, double);           \
+  template void gsrb<T>(const DeviceLayout&, T*, const T*, int, double, double, double);                       \
+  template void apply<T>(const DeviceLayout&, T*, const T*, double, double, double);                           \
+  template void residual<T>(const DeviceLayout&, T*, const T*, const T*, double, double, double);              \
+  template void restrict_res<T>(const DeviceLayout&, const DeviceLayout&, T*, const T*, const T*, double, double, \
+                                double);                                                                      \
+  template void prolong<T>(const DeviceLayout&, const DeviceLayout&, T*, const T*);                            \
+  template void average<T>(const DeviceLayout&, const DeviceLayout&, T*, const T*);                            \
+  template void incr<T>(const DeviceLayout&, T*, const T*, double);                                            \
+  template void axby<T>(const DeviceLayout&, T*, const T*, const T*, double, double);                          \
+  template void scale<T>(const DeviceLayout&, T*, double);                                                     \
+  template void setval<T>(const DeviceLayout&, T*, double);                                                    \
+  template void copy_valid<T>(const DeviceLayout&, T*, const T*);                                              \
+  template void copy_all<T>(const DeviceLayout&, T*, const T*);                                                \
+  template void precond_init<T>(const DeviceLayout&, T*, const T*, double);                                    \
+  template void motion<T>(const MotionPlan&, const DeviceLayout&, const DeviceLayout&, T*, const T*, int);     \
+  template double maxnorm<T>(const DeviceLayout&, const T*, int);                                              \
+  template void maxnorm_async<T>(const DeviceLayout&, const T*, int, int);                                      \
+  template double dot_ordered<T>(const DeviceLayout&, const T*, const T*, int);                                \
+  template double norm2_ordered<T>(const DeviceLayout&, const T*, int);                                        \
+  template void cf_inhomog<T>(Context*, const CFCoarse*, int, const CFGen*, const CFFine*, int, T*, T*, const T*, \
+                              double, double);                                                                \
+  template void reflux<T>(Context*, const RefluxCell*, int, const RefluxCoarse*, const RefluxFine*, T*, const T*, \
+                          const T*, double, double, double);
+INST(double)
+INST(float)
+
+}  // namespace ops
+}  // namespace amrmg

@@ -1,0 +1,99 @@
+// Level-wide launchers of the HIP kernels (implemented in kernels.hip).
+//
+// Each launcher replaces a per-box loop of FORT_* calls in the reference
+// (lib/src/AMRElliptic/AMRPoissonOp.cpp) by one launch over the tile table of a
+// level.  All launches go to the context's stream; none synchronises except the
+// reductions that return a host scalar.
+#pragma once
+#include "device.hpp"
+
+namespace amrmg {
+void profiler_enable(Context* c, unsigned class_mask);  // 0 = off
+void profiler_reset(Context* c);
+bool profiler_read(Context* c, int kind, double* ms, double* bytes, long long* count);
+
+namespace ops {
+
+struct GhostPlan {  // face-ghost fill rules of one level (exchange + BC + homogeneous CF)
+  Patch* d_patches = nullptr;
+  PatchWork* d_work = nullptr;
+  int npatch = 0, nwork = 0;
+  Context* owner = nullptr;
+  std::vector<Patch> h_patches;
+  void upload(Context* ctx);
+  void release();
+};
+
+struct MotionPlan {  // Copier between two layouts
+  Motion* d_items = nullptr;
+  MotionWork* d_work = nullptr;
+  int nitems = 0, nwork = 0;
+  Context* owner = nullptr;
+  std::vector<Motion> h_items;
+  void upload(Context* ctx);
+  void release();
+};
+
+template <class T>
+void fill_ghosts(const DeviceLayout& L, const GhostPlan& P, T* phi, int doCF, int homog, double dx,
+                 double dxCrse);
+template <class T>
+void gsrb(const DeviceLayout& L, T* phi, const T* rhs, int color, double dx, double alpha, double beta);
+template <class T>
+void apply(const DeviceLayout& L, T* lofphi, const T* phi, double dx, double alpha, double beta);
+template <class T>
+void residual(const DeviceLayout& L, T* res, const T* phi, const T* rhs, double dx, double alpha,
+              double beta);
+template <class T>
+void restrict_res(const DeviceLayout& Lf, const DeviceLayout& Lc, T* resc, const T* phi, const T* rhs,
+                  double dx, double alpha, double beta);
+template <class T>
+void prolong(const DeviceLayout& Lf, const DeviceLayout& Lc, T* phi, const T* coarse);
+template <class T>
+void average(const DeviceLayout& Lf, const DeviceLayout& Lc, T* coarse, const T* fine);
+
+template <class T>
+void incr(const DeviceLayout& L, T* lhs, const T* x, double scale);
+template <class T>
+void axby(const DeviceLayout& L, T* lhs, const T* x, const T* y, double a, double b);
+template <class T>
+void scale(const DeviceLayout& L, T* x, double s);
+template <class T>
+void setval(const DeviceLayout& L, T* x, double v);
+template <class T>
+void copy_valid(const DeviceLayout& L, T* dst, const T* src);
+template <class T>
+void copy_all(const DeviceLayout& L, T* dst, const T* src);
+template <class T>
+void precond_init(const DeviceLayout& L, T* phi, const T* rhs, double mult);
+// mixed precision: dst (float) <- src (double) on valid cells, and back with incr
+void convert_valid_d2f(const DeviceLayout& L, float* dst, const double* src);
+void incr_f2d(const DeviceLayout& L, double* lhs, const float* x);
+
+template <class T>
+void motion(const MotionPlan& P, const DeviceLayout& Lsrc, const DeviceLayout& Ldst, T* dst, const T* src,
+            int zero);
+
+// reductions; result left in ctx->d_scalars[slot], returned after a stream sync
+template <class T>
+double maxnorm(const DeviceLayout& L, const T* x, int slot);
+template <class T>
+double dot_ordered(const DeviceLayout& L, const T* a, const T* b, int slot);
+template <class T>
+double norm2_ordered(const DeviceLayout& L, const T* x, int slot);
+// asynchronous variant: max-norm into d_scalars[slot] without a sync (value read later)
+template <class T>
+void maxnorm_async(const DeviceLayout& L, const T* x, int slot, int accumulate);
+
+// inhomogeneous coarse-fine interpolation (QuadCFInterp::coarseFineInterp)
+template <class T>
+void cf_inhomog(Context* ctx, const CFCoarse* dc, int ncoarse, const CFGen* dg, const CFFine* df, int nfine,
+                T* deriv_scratch, T* phif, const T* phic, double dxf, double dxc);
+
+// reflux (AMRPoissonOp::reflux after the fine level's CF ghosts are filled)
+template <class T>
+void reflux(Context* ctx, const RefluxCell* cells, int ncells, const RefluxCoarse* rc, const RefluxFine* rf,
+            T* u, const T* phic, const T* phif, double beta_dx_c, double beta_ref_dx_c, double minus_inv_dx);
+
+}  // namespace ops
+}  // namespace amrmg

@@ -1,0 +1,60 @@
+// Host construction of the device tables that replace the reference's per-box
+// metadata: Copier motion plans, BC/exchange/CF ghost rules, QuadCFStencil
+// derivative stencils and LevelFluxRegister reflux lists.  Built once per
+// hierarchy and reused by every solve.
+#pragma once
+#include <vector>
+
+#include "device.hpp"
+#include "ops.hpp"
+
+namespace amrmg {
+
+struct BCSpec {
+  int type[2][3];     // [side][dir]: 1 Dirichlet, 0 Neumann
+  double val[2][3];
+};
+
+// point -> box lookup over a layout (bucketed)
+class BoxFinder {
+ public:
+  explicit BoxFinder(const DBL& d);
+  int find(const IV& p) const;
+  // boxes intersecting region r (each once, ascending index)
+  std::vector<int> overlapping(const Box& r) const;
+
+ private:
+  const DBL* dbl_;
+  Box grid_;
+  int B_ = 1;
+  IV n_{1, 1, 1};
+  std::vector<int> start_, list_;
+  int bucket(int i, int j, int k) const;
+};
+
+// Face-ghost rules of one level: exchange from neighbours, BCs on the domain
+// boundary, homogeneous CF interpolation on coarse-fine faces (CFRegion/CFIVS).
+ops::GhostPlan build_ghost_plan(const DBL& dbl, const Box& domain, const BCSpec& bc, bool withCF);
+
+// Copier from src layout to dst layout (Copier::define with zero ghost).
+ops::MotionPlan build_copier(const DBL& src, const DBL& dst);
+
+// QuadCFInterp tables of a fine level over its coarser level.
+struct CFPlanHost {
+  std::vector<CFCoarse> coarse;
+  std::vector<CFGen> gens;
+  std::vector<CFFine> fine;
+};
+CFPlanHost build_cf_plan(const DBL& fine, const DeviceLayout& fineLay, const DBL& coarse,
+                         const DeviceLayout& coarseLay, const Box& fineDomain, double dxFine, int ref);
+
+// Reflux tables of a coarse level with its finer level.
+struct RefluxPlanHost {
+  std::vector<RefluxCell> cells;
+  std::vector<RefluxCoarse> coarse;
+  std::vector<RefluxFine> fine;
+};
+RefluxPlanHost build_reflux_plan(const DBL& coarse, const DeviceLayout& coarseLay, const DBL& fine,
+                                 const DeviceLayout& fineLay, const Box& coarseDomain, int ref);
+
+}  // namespace amrmg

@@ -1,0 +1,118 @@
+/*
+ * amrmg.h -- C ABI of the MI355X-native AMR multigrid Poisson/Helmholtz library
+ * (libamrmg.so, built from chombo_3.2_amd/csrc).
+ *
+ * The reference (Chombo 3.2) crosses its language boundary once per box per
+ * kernel: C++ calls ChomboFortran subroutines through FORTRAN_NAME/FORT_* with
+ * CHF_FRA/CHF_BOX pointer arguments (lib/src/BoxTools/FORT_PROTO.H:293-447,
+ * lib/util/chfpp/subrout.pm:1049-1068).  This library replaces that layer: each
+ * entry point below runs the reference operation on a whole level (all boxes) on
+ * the GPU, and the solver entry points run the reference's AMRMultiGrid /
+ * MultiGrid / BiCGStab control flow on device-resident levels.
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a negative code on error; the
+ *    message is available from amrmg_last_error() (thread-local).  No C++
+ *    exception crosses this boundary.
+ *  - A hierarchy is described by (ispec, rspec):
+ *      ispec[0]            nlev (AMR levels, refinement ratio 2)
+ *      ispec[1..6]         level-0 domain box lo0 lo1 lo2 hi0 hi1 hi2
+ *      ispec[7..9]         BC type on the lo faces (0 Neumann, 1 Dirichlet)
+ *      ispec[10..12]       BC type on the hi faces
+ *      ispec[13..13+nlev)  number of boxes per level
+ *      then 6 ints per box (lo0 lo1 lo2 hi0 hi1 hi2), level by level, each level
+ *      sorted by Chombo's Box::operator< (as a closed DisjointBoxLayout is)
+ *      rspec = {dx0, alpha, beta, bcval_lo[3], bcval_hi[3]}
+ *    (AMRPoissonOpFactory::define, lib/src/AMRElliptic/AMRPoissonOp.cpp:1857-1903,
+ *     with ConstDiriNeumBC, lib/src/AMRElliptic/BCFunc.cpp:58-158).
+ *  - Host field buffers are "valid-packed": level by level, box by box in layout
+ *    order, the box's valid cells in Fortran order (i fastest).  "Ghosted-packed"
+ *    buffers hold each box grown by one cell instead.
+ *  - solver parameters: sparams = {pre, post, bottom, numMG, iterMax, imin},
+ *    rparams = {eps, hang, normThresh}
+ *    (AMRMultiGrid::setSolverParameters, lib/src/AMRElliptic/AMRMultiGrid.H:528-553).
+ *  - precision: 0 = FP64 throughout; 1 = mixed (FP32 inner V-cycles, FP64 outer
+ *    residual / update / bottom solve).
+ */
+#ifndef AMRMG_H
+#define AMRMG_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct amrmg_ctx amrmg_ctx;
+typedef struct amrmg_solver amrmg_solver;
+
+const char* amrmg_last_error(void);
+
+/* device context (one HIP stream, default priority) */
+int amrmg_ctx_create(int device, amrmg_ctx** out);
+int amrmg_ctx_destroy(amrmg_ctx* ctx);
+
+/* AMRPoissonOpFactory::define + AMRMultiGrid::define + init with a BiCGStab
+   bottom solver (replaces AMRPoissonOp.cpp:1857-2070, AMRMultiGrid.H:1212-1328) */
+int amrmg_solver_create(amrmg_ctx* ctx, const int* ispec, const double* rspec, int precision,
+                        amrmg_solver** out);
+int amrmg_solver_destroy(amrmg_solver* s);
+int amrmg_solver_set_params(amrmg_solver* s, const int* sparams, const double* rparams);
+long long amrmg_solver_level_cells(amrmg_solver* s, int lev);
+int amrmg_solver_num_mg_depths(amrmg_solver* s);
+long long amrmg_solver_mg_cells(amrmg_solver* s, int depth);
+/* bytes of HBM the solver holds (fields + tables) */
+long long amrmg_solver_device_bytes(amrmg_solver* s);
+
+/* field 0 = phi, 1 = rhs, 2 = residual (last computeAMRResidual) */
+int amrmg_solver_set_field(amrmg_solver* s, int field, const double* valid_packed);
+int amrmg_solver_get_field(amrmg_solver* s, int field, double* valid_packed);
+
+/* AMRMultiGrid::solve(phi, rhs, nlev-1, 0, zero_phi) on the device-resident
+   fields.  hist[0..] = max-norm residual history (up to hist_cap entries),
+   info = {iterations, exitStatus, bottom BiCGStab iterations}. */
+int amrmg_solver_solve(amrmg_solver* s, int zero_phi, double* hist, int hist_cap, int* info);
+
+/* AMRMultiGrid::computeAMRResidual(resid, phi, rhs) -> field 2; norm out */
+int amrmg_solver_residual(amrmg_solver* s, int homogeneous, double* norm_out);
+/* AMRMultiGrid::computeAMROperator(lphi, phi) -> field 2 */
+int amrmg_solver_operator(amrmg_solver* s, int homogeneous);
+
+/* ---- level operations (host buffers in/out; one upload, one level-wide
+   launch sequence, one download) ---- */
+/* AMRPoissonOp::applyOp / residual / relax / restrictResidual / prolongIncrement
+   on multigrid depth `depth` of AMR level 0 (depth 0 = level 0) */
+int amrmg_level_apply(amrmg_solver* s, int depth, const double* phi, double* out, int homogeneous);
+int amrmg_level_residual(amrmg_solver* s, int depth, const double* phi, const double* rhs, double* out,
+                         int homogeneous);
+int amrmg_level_relax(amrmg_solver* s, int depth, double* phi, const double* rhs, int iterations);
+int amrmg_level_restrict(amrmg_solver* s, int depth, const double* phi, const double* rhs, double* res_coarse);
+int amrmg_level_prolong(amrmg_solver* s, int depth, double* phi, const double* e_coarse);
+/* FORT_GSRBLAPLACIAN / FORT_GSRBHELMHOLTZ on AMR level lev, ghosts supplied */
+int amrmg_kernel_gsrb(amrmg_solver* s, int lev, double* phi_ghosted, const double* rhs, int color);
+/* AMRPoissonOp::homogeneousCFInterp on AMR level lev (ghosted in/out) */
+int amrmg_homog_cf(amrmg_solver* s, int lev, double* phi_ghosted);
+/* QuadCFInterp::coarseFineInterp(phi[lev], phi[lev-1]) (ghosted in/out) */
+int amrmg_quad_cf(amrmg_solver* s, int lev, double* phi_ghosted, const double* coarse_valid);
+/* FORT_AVERAGE of level lev onto coarsen(grids[lev], 2) */
+int amrmg_average(amrmg_solver* s, int lev, const double* fine, double* coarse);
+/* MultiGrid bottom: relax(nbottom) + BiCGStabSolver::solve on the deepest MG level */
+int amrmg_bottom_solve(amrmg_solver* s, double* corr, const double* res, int nbottom, double metric, double eps,
+                       int* info);
+
+/* Host-only check: builds every device table of a hierarchy (ghost rules,
+   copiers, coarse-fine stencils, reflux lists) without touching a device and
+   verifies each address a kernel will form is inside its field.  stats (up to 8):
+   {levels, multigrid depths, ghost patches, CF coarse records, CF fine records,
+   reflux cells, tiles, valid cells}. */
+int amrmg_plan_check(const int* ispec, const double* rspec, long long* stats, int nstats);
+
+/* ---- event profiler over kernel classes (0 GSRB, 1 residual, 2 applyOp,
+   3 restrict, 4 prolong, 5 average, 6 ghost fill, 7 BLAS-1, 8 reductions,
+   9 CF interpolation, 10 reflux, 11 copier).  class_mask: bit k times class k;
+   -1 times all, 0 turns the profiler off.  Events go on the library's stream. ---- */
+int amrmg_profile_enable(amrmg_ctx* ctx, int class_mask);
+int amrmg_profile_reset(amrmg_ctx* ctx);
+int amrmg_profile_read(amrmg_ctx* ctx, int kernel_class, double* ms, double* bytes, long long* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

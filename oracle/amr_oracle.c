@@ -1,0 +1,1885 @@
+/*
+ * amr_oracle.c -- plain-C restatement of the FP64 path of Chombo 3.2's
+ * AMRPoissonOp under AMRMultiGrid / MultiGrid with a BiCGStab bottom solver.
+ *
+ * TEST INFRASTRUCTURE ONLY (the oracle).  It is loaded by tests/, by the smoke
+ * check and by the CPU-baseline leg of bench.py, and never by the product.
+ *
+ * Every routine names the reference source it restates.  Arithmetic follows the
+ * reference's expression order term by term and the file is compiled with
+ * -ffp-contract=off, so that a restated kernel rounds exactly as the Fortran /
+ * C++ it follows (Fortran evaluates a+b+c as (a+b)+c and never reassociates).
+ * Paths are relative to the Chombo 3.2 source tree.
+ *
+ * Parity pins (see tests/test_oracle_*.py): each kernel is checked against
+ * closed forms (operator exact on quadratics, discrete eigenfunctions, GSRB fixed
+ * point, AVERAGE/PROLONG of polynomials), the reference's own known-answer tests
+ * (lib/test/AMRTools/quadCFInterpTest.cpp quadratic reproduction at 1e-12,
+ * lib/test/AMRElliptic/testAMRPoissonOp.H truncation), dense direct solves of
+ * small systems, conservation of the refluxed composite operator and adjointness.
+ */
+#include "amr_oracle.h"
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+static char g_err[512];
+const char* oracle_last_error(void) { return g_err; }
+static void set_err(const char* m) { snprintf(g_err, sizeof g_err, "%s", m); }
+
+/* ======================================================================= */
+/* Box (lib/src/BoxTools/Box.H, IntVect.H)                                 */
+/* ======================================================================= */
+typedef struct { int lo[3], hi[3]; } Box;
+
+static int b_empty(const Box* b) {
+  for (int d = 0; d < 3; d++) if (b->lo[d] > b->hi[d]) return 1;
+  return 0;
+}
+static long b_npts(const Box* b) {
+  if (b_empty(b)) return 0;
+  return (long)(b->hi[0]-b->lo[0]+1)*(b->hi[1]-b->lo[1]+1)*(b->hi[2]-b->lo[2]+1);
+}
+static int b_len(const Box* b, int d) { return b->hi[d]-b->lo[d]+1; }
+static Box b_grow(Box b, int g) { for (int d = 0; d < 3; d++) { b.lo[d]-=g; b.hi[d]+=g; } return b; }
+static Box b_growd(Box b, int d, int g) { b.lo[d]-=g; b.hi[d]+=g; return b; }
+static Box b_isect(Box a, const Box* b) {
+  for (int d = 0; d < 3; d++) {
+    if (b->lo[d] > a.lo[d]) a.lo[d] = b->lo[d];
+    if (b->hi[d] < a.hi[d]) a.hi[d] = b->hi[d];
+  }
+  return a;
+}
+static int b_intersects(const Box* a, const Box* b) { Box c = b_isect(*a, b); return !b_empty(&c); }
+/* IntVect::coarsen: floor division (IntVect.H:1288-1299) */
+static int fdiv(int a, int r) { return a >= 0 ? a / r : -((-a + r - 1) / r); }
+static Box b_coarsen(Box b, int r) {
+  for (int d = 0; d < 3; d++) { b.lo[d] = fdiv(b.lo[d], r); b.hi[d] = fdiv(b.hi[d], r); }
+  return b;
+}
+static Box b_refine(Box b, int r) {
+  for (int d = 0; d < 3; d++) { b.lo[d] *= r; b.hi[d] = (b.hi[d]+1)*r - 1; }
+  return b;
+}
+static int b_has(const Box* b, int i, int j, int k) {
+  return i >= b->lo[0] && i <= b->hi[0] && j >= b->lo[1] && j <= b->hi[1] &&
+         k >= b->lo[2] && k <= b->hi[2];
+}
+static int b_contains(const Box* a, const Box* b) {
+  for (int d = 0; d < 3; d++) if (b->lo[d] < a->lo[d] || b->hi[d] > a->hi[d]) return 0;
+  return 1;
+}
+static int b_eq(const Box* a, const Box* b) { return memcmp(a, b, sizeof(Box)) == 0; }
+/* adjCellLo / adjCellHi (Box.H) : side 0 = lo, 1 = hi */
+static Box b_adj(Box b, int d, int side, int n) {
+  if (side == 0) { b.hi[d] = b.lo[d]-1; b.lo[d] = b.lo[d]-n; }
+  else           { b.lo[d] = b.hi[d]+1; b.hi[d] = b.hi[d]+n; }
+  return b;
+}
+static Box b_shift(Box b, int d, int s) { b.lo[d] += s; b.hi[d] += s; return b; }
+/* IntVect::lexLT (IntVect.H:237) : dimension 0 most significant */
+static int iv_lexlt(const int* a, const int* b) {
+  for (int d = 0; d < 3; d++) { if (a[d] < b[d]) return 1; if (a[d] > b[d]) return 0; }
+  return 0;
+}
+/* Box::operator< (Box.H:2223) */
+static int b_lt(const Box* a, const Box* b) {
+  if (memcmp(a->lo, b->lo, sizeof a->lo) == 0) return iv_lexlt(a->hi, b->hi);
+  return iv_lexlt(a->lo, b->lo);
+}
+static Box mkbox(int l0, int l1, int l2, int h0, int h1, int h2) {
+  Box b = {{l0,l1,l2},{h0,h1,h2}}; return b;
+}
+
+/* ======================================================================= */
+/* Dense IntVectSet over a bounding box (DenseIntVectSet.cpp)              */
+/* ======================================================================= */
+typedef struct { Box mb; unsigned char* m; } IVS;
+
+static void ivs_init_empty(IVS* s) { s->mb = mkbox(0,0,0,-1,-1,-1); s->m = NULL; }
+static void ivs_free(IVS* s) { free(s->m); ivs_init_empty(s); }
+static void ivs_define(IVS* s, Box b) {
+  free(s->m); s->mb = b; s->m = NULL;
+  if (!b_empty(&b)) { long n = b_npts(&b); s->m = malloc(n); memset(s->m, 1, n); }
+}
+static long ivs_off(const IVS* s, int i, int j, int k) {
+  const Box* b = &s->mb;
+  return ((long)(k-b->lo[2])*b_len(b,1) + (j-b->lo[1]))*b_len(b,0) + (i-b->lo[0]);
+}
+static int ivs_has(const IVS* s, int i, int j, int k) {
+  if (!s->m || !b_has(&s->mb, i, j, k)) return 0;
+  return s->m[ivs_off(s,i,j,k)];
+}
+static void ivs_sub_box(IVS* s, const Box* b) {
+  if (!s->m) return;
+  Box c = b_isect(s->mb, b);
+  if (b_empty(&c)) return;
+  for (int k = c.lo[2]; k <= c.hi[2]; k++)
+    for (int j = c.lo[1]; j <= c.hi[1]; j++)
+      for (int i = c.lo[0]; i <= c.hi[0]; i++) s->m[ivs_off(s,i,j,k)] = 0;
+}
+static void ivs_and_box(IVS* s, const Box* b) {
+  if (!s->m) return;
+  for (int k = s->mb.lo[2]; k <= s->mb.hi[2]; k++)
+    for (int j = s->mb.lo[1]; j <= s->mb.hi[1]; j++)
+      for (int i = s->mb.lo[0]; i <= s->mb.hi[0]; i++)
+        if (!b_has(b, i, j, k)) s->m[ivs_off(s,i,j,k)] = 0;
+}
+static long ivs_numpts(const IVS* s) {
+  if (!s->m) return 0;
+  long n = b_npts(&s->mb), c = 0;
+  for (long q = 0; q < n; q++) c += s->m[q];
+  return c;
+}
+static Box ivs_minbox(const IVS* s) {
+  Box r = mkbox(1<<30,1<<30,1<<30,-(1<<30),-(1<<30),-(1<<30));
+  int any = 0;
+  if (s->m)
+    for (int k = s->mb.lo[2]; k <= s->mb.hi[2]; k++)
+      for (int j = s->mb.lo[1]; j <= s->mb.hi[1]; j++)
+        for (int i = s->mb.lo[0]; i <= s->mb.hi[0]; i++)
+          if (s->m[ivs_off(s,i,j,k)]) {
+            int iv[3] = {i,j,k}; any = 1;
+            for (int d = 0; d < 3; d++) {
+              if (iv[d] < r.lo[d]) r.lo[d] = iv[d];
+              if (iv[d] > r.hi[d]) r.hi[d] = iv[d];
+            }
+          }
+  if (!any) return mkbox(0,0,0,-1,-1,-1);
+  return r;
+}
+static void ivs_copy(IVS* dst, const IVS* src) {
+  ivs_free(dst);
+  dst->mb = src->mb;
+  if (src->m) { long n = b_npts(&src->mb); dst->m = malloc(n); memcpy(dst->m, src->m, n); }
+}
+/* DenseIntVectSet::coarsen */
+static void ivs_coarsen(IVS* dst, const IVS* src, int r) {
+  ivs_free(dst);
+  if (!src->m) return;
+  Box nb = b_coarsen(src->mb, r);
+  dst->mb = nb; long n = b_npts(&nb); dst->m = calloc(n, 1);
+  for (int k = src->mb.lo[2]; k <= src->mb.hi[2]; k++)
+    for (int j = src->mb.lo[1]; j <= src->mb.hi[1]; j++)
+      for (int i = src->mb.lo[0]; i <= src->mb.hi[0]; i++)
+        if (src->m[ivs_off(src,i,j,k)])
+          dst->m[ivs_off(dst, fdiv(i,r), fdiv(j,r), fdiv(k,r))] = 1;
+}
+/* DenseIntVectSet::grow(idir, -1): keep cells whose +-e neighbours are in the set */
+static void ivs_erode(IVS* s, int d) {
+  if (!s->m) return;
+  long n = b_npts(&s->mb);
+  unsigned char* o = malloc(n); memcpy(o, s->m, n);
+  IVS t = { s->mb, o };
+  for (int k = s->mb.lo[2]; k <= s->mb.hi[2]; k++)
+    for (int j = s->mb.lo[1]; j <= s->mb.hi[1]; j++)
+      for (int i = s->mb.lo[0]; i <= s->mb.hi[0]; i++) {
+        int p[3] = {i,j,k}, q[3] = {i,j,k};
+        p[d] -= 1; q[d] += 1;
+        if (!(ivs_has(&t,p[0],p[1],p[2]) && ivs_has(&t,q[0],q[1],q[2])))
+          s->m[ivs_off(s,i,j,k)] = 0;
+      }
+  free(o);
+}
+static int ivs_has_box(const IVS* s, const Box* b) {
+  for (int k = b->lo[2]; k <= b->hi[2]; k++)
+    for (int j = b->lo[1]; j <= b->hi[1]; j++)
+      for (int i = b->lo[0]; i <= b->hi[0]; i++)
+        if (!ivs_has(s,i,j,k)) return 0;
+  return 1;
+}
+static void ivs_sub_ivs(IVS* s, const IVS* o) {
+  if (!s->m || !o->m) return;
+  for (int k = s->mb.lo[2]; k <= s->mb.hi[2]; k++)
+    for (int j = s->mb.lo[1]; j <= s->mb.hi[1]; j++)
+      for (int i = s->mb.lo[0]; i <= s->mb.hi[0]; i++)
+        if (ivs_has(o,i,j,k)) s->m[ivs_off(s,i,j,k)] = 0;
+}
+
+/* ======================================================================= */
+/* FArrayBox (Fortran order, one component; BaseFabImplem.H:291-304)       */
+/* ======================================================================= */
+typedef struct { Box b; long nx, ny; double* d; } Fab;
+
+static void fab_alloc(Fab* f, Box b) {
+  f->b = b; f->nx = b_len(&b,0); f->ny = b_len(&b,1);
+  f->d = calloc((size_t)b_npts(&b), sizeof(double));
+}
+static void fab_free(Fab* f) { free(f->d); f->d = NULL; }
+static inline double* P(const Fab* f, int i, int j, int k) {
+  return f->d + ((long)(k-f->b.lo[2])*f->ny + (j-f->b.lo[1]))*f->nx + (i-f->b.lo[0]);
+}
+static void fab_setval(Fab* f, Box r, double v) {
+  r = b_isect(r, &f->b);
+  for (int k = r.lo[2]; k <= r.hi[2]; k++)
+    for (int j = r.lo[1]; j <= r.hi[1]; j++)
+      for (int i = r.lo[0]; i <= r.hi[0]; i++) *P(f,i,j,k) = v;
+}
+static void fab_copy_region(Fab* dst, const Fab* src, Box r) {
+  for (int k = r.lo[2]; k <= r.hi[2]; k++)
+    for (int j = r.lo[1]; j <= r.hi[1]; j++)
+      for (int i = r.lo[0]; i <= r.hi[0]; i++) *P(dst,i,j,k) = *P(src,i,j,k);
+}
+
+/* DisjointBoxLayout: boxes in Box::operator< order (BoxLayout::close sorts) */
+typedef struct { int n; Box* b; } Layout;
+
+static Layout lay_coarsen(const Layout* L, int r) {
+  Layout c; c.n = L->n; c.b = malloc(sizeof(Box)*(L->n ? L->n : 1));
+  for (int q = 0; q < L->n; q++) c.b[q] = b_coarsen(L->b[q], r);
+  return c;
+}
+static int lay_coarsenable(const Layout* L, int r) {
+  for (int q = 0; q < L->n; q++) {
+    Box c = b_refine(b_coarsen(L->b[q], r), r);
+    if (!b_eq(&c, &L->b[q])) return 0;
+  }
+  return 1;
+}
+static long lay_cells(const Layout* L) { long n = 0; for (int q = 0; q < L->n; q++) n += b_npts(&L->b[q]); return n; }
+
+/* LevelData<FArrayBox> with ghost width g */
+typedef struct { const Layout* L; int g; Fab* f; } LD;
+static LD ld_make(const Layout* L, int g) {
+  LD x; x.L = L; x.g = g; x.f = malloc(sizeof(Fab)*(L->n ? L->n : 1));
+  for (int q = 0; q < L->n; q++) fab_alloc(&x.f[q], b_grow(L->b[q], g));
+  return x;
+}
+static void ld_free(LD* x) {
+  if (!x->f) return;
+  for (int q = 0; q < x->L->n; q++) fab_free(&x->f[q]);
+  free(x->f); x->f = NULL;
+}
+static void ld_from_packed(LD* x, const double* v) {
+  long o = 0;
+  for (int q = 0; q < x->L->n; q++) {
+    const Box* b = &x->L->b[q];
+    for (int k = b->lo[2]; k <= b->hi[2]; k++)
+      for (int j = b->lo[1]; j <= b->hi[1]; j++)
+        for (int i = b->lo[0]; i <= b->hi[0]; i++) *P(&x->f[q],i,j,k) = v[o++];
+  }
+}
+static void ld_to_packed(const LD* x, double* v) {
+  long o = 0;
+  for (int q = 0; q < x->L->n; q++) {
+    const Box* b = &x->L->b[q];
+    for (int k = b->lo[2]; k <= b->hi[2]; k++)
+      for (int j = b->lo[1]; j <= b->hi[1]; j++)
+        for (int i = b->lo[0]; i <= b->hi[0]; i++) v[o++] = *P(&x->f[q],i,j,k);
+  }
+}
+static void ld_from_ghosted(LD* x, const double* v) {
+  long o = 0;
+  for (int q = 0; q < x->L->n; q++) {
+    long n = b_npts(&x->f[q].b);
+    memcpy(x->f[q].d, v + o, sizeof(double)*n); o += n;
+  }
+}
+static void ld_to_ghosted(const LD* x, double* v) {
+  long o = 0;
+  for (int q = 0; q < x->L->n; q++) {
+    long n = b_npts(&x->f[q].b);
+    memcpy(v + o, x->f[q].d, sizeof(double)*n); o += n;
+  }
+}
+
+/* ======================================================================= */
+/* ChF kernels, lib/src/AMRElliptic/AMRPoissonOpF.ChF                       */
+/* ======================================================================= */
+/* GSRBLAPLACIAN (AMRPoissonOpF.ChF:122-198) */
+static void k_gsrb_lap(Fab* phi, const Fab* rhs, const Box* r, double dx, int redBlack) {
+  double dxinv = 1.0/(dx*dx);
+  double sum_b = 0.0;
+  for (int d = 0; d < 3; d++) sum_b = sum_b + 2.0*dxinv;
+  double lambda = -1.0/sum_b;
+  for (int k = r->lo[2]; k <= r->hi[2]; k++)
+    for (int j = r->lo[1]; j <= r->hi[1]; j++) {
+      int imin = r->lo[0];
+      int indtot = imin + j + k;
+      imin = imin + abs((indtot + redBlack) % 2);
+      for (int i = imin; i <= r->hi[0]; i += 2) {
+        double lphi = (*P(phi,i+1,j,k) + *P(phi,i-1,j,k) + *P(phi,i,j+1,k)
+                     + *P(phi,i,j-1,k) + *P(phi,i,j,k+1) + *P(phi,i,j,k-1)) * dxinv;
+        *P(phi,i,j,k) = lambda*(*P(rhs,i,j,k) - lphi);
+      }
+    }
+}
+/* GSRBHELMHOLTZ (AMRPoissonOpF.ChF:53-120) */
+static void k_gsrb_helm(Fab* phi, const Fab* rhs, const Box* r, double dx,
+                        double alpha, double beta, int redBlack) {
+  double dxinv = 1.0/(dx*dx);
+  double sum_b = 0.0;
+  for (int d = 0; d < 3; d++) sum_b = sum_b + 2.0*dxinv;
+  double lambda = -1.0/(alpha - beta*sum_b);
+  for (int k = r->lo[2]; k <= r->hi[2]; k++)
+    for (int j = r->lo[1]; j <= r->hi[1]; j++) {
+      int imin = r->lo[0];
+      int indtot = imin + j + k;
+      imin = imin + abs((indtot + redBlack) % 2);
+      for (int i = imin; i <= r->hi[0]; i += 2) {
+        double p = *P(phi,i,j,k);
+        double lphi = (*P(phi,i+1,j,k) + *P(phi,i-1,j,k) + *P(phi,i,j+1,k)
+                     + *P(phi,i,j-1,k) + *P(phi,i,j,k+1) + *P(phi,i,j,k-1)
+                     - 6.0*p) * dxinv;
+        double helmop = alpha*p + beta*lphi;
+        *P(phi,i,j,k) = p + lambda*(helmop - *P(rhs,i,j,k));
+      }
+    }
+}
+/* OPERATORLAP (AMRPoissonOpF.ChF:287-324) */
+static void k_oplap(Fab* lofphi, const Fab* phi, const Box* r, double dx, double alpha, double beta) {
+  double dxinv = 1.0/(dx*dx);
+  for (int k = r->lo[2]; k <= r->hi[2]; k++)
+    for (int j = r->lo[1]; j <= r->hi[1]; j++)
+      for (int i = r->lo[0]; i <= r->hi[0]; i++) {
+        double p = *P(phi,i,j,k);
+        double lap = (*P(phi,i+1,j,k) + *P(phi,i-1,j,k) + *P(phi,i,j+1,k)
+                    + *P(phi,i,j-1,k) + *P(phi,i,j,k+1) + *P(phi,i,j,k-1)
+                    - 6.0*p) * dxinv;
+        *P(lofphi,i,j,k) = alpha*p + beta*lap;
+      }
+}
+/* OPERATORLAPRES (AMRPoissonOpF.ChF:341-377): r = -alpha*phi - beta*lap + rhs */
+static void k_oplapres(Fab* res, const Fab* phi, const Fab* rhs, const Box* r,
+                       double dx, double alpha, double beta) {
+  double dxinv = 1.0/(dx*dx);
+  for (int k = r->lo[2]; k <= r->hi[2]; k++)
+    for (int j = r->lo[1]; j <= r->hi[1]; j++)
+      for (int i = r->lo[0]; i <= r->hi[0]; i++) {
+        double p = *P(phi,i,j,k);
+        double lap = (*P(phi,i+1,j,k) + *P(phi,i-1,j,k) + *P(phi,i,j+1,k)
+                    + *P(phi,i,j-1,k) + *P(phi,i,j,k+1) + *P(phi,i,j,k-1)
+                    - 6.0*p) * dxinv;
+        *P(res,i,j,k) = -(alpha*p) - beta*lap + *P(rhs,i,j,k);
+      }
+}
+/* RESTRICTRES (AMRPoissonOpF.ChF:394-438) called with CHF_FRA_SHIFT as in
+   AMRPoissonOp::restrictResidual (AMRPoissonOp.cpp:683-700): the coarse index is
+   civ + (i - iv)/2 with iv = region.smallEnd(), civ = coarsen(iv, 2). */
+static void k_restrictres(Fab* res, const Fab* phi, const Fab* rhs, double alpha, double beta,
+                          const Box* r, double dx) {
+  double dxinv = 1.0/(dx*dx);
+  double denom = 8.0;
+  int iv[3] = {r->lo[0], r->lo[1], r->lo[2]};
+  int civ[3] = {fdiv(iv[0],2), fdiv(iv[1],2), fdiv(iv[2],2)};
+  for (int k = r->lo[2]; k <= r->hi[2]; k++)
+    for (int j = r->lo[1]; j <= r->hi[1]; j++)
+      for (int i = r->lo[0]; i <= r->hi[0]; i++) {
+        int ii = civ[0] + (i-iv[0])/2, jj = civ[1] + (j-iv[1])/2, kk = civ[2] + (k-iv[2])/2;
+        double p = *P(phi,i,j,k);
+        double lofphi = alpha*p
+          + beta*(*P(phi,i+1,j,k) + *P(phi,i-1,j,k) + *P(phi,i,j+1,k)
+                + *P(phi,i,j-1,k) + *P(phi,i,j,k+1) + *P(phi,i,j,k-1)
+                - p*2.0*3.0) * dxinv;
+        double* c = P(res,ii,jj,kk);
+        *c = *c + (*P(rhs,i,j,k) - lofphi)/denom;
+      }
+}
+/* PROLONG (AMRPoissonOpF.ChF:453-478) with the shifted indexing of
+   AMRPoissonOp::prolongIncrement / AMRProlongS. */
+static void k_prolong(Fab* phi, const Fab* coarse, const Box* r, int m) {
+  int iv[3] = {r->lo[0], r->lo[1], r->lo[2]};
+  int civ[3] = {fdiv(iv[0],m), fdiv(iv[1],m), fdiv(iv[2],m)};
+  for (int k = r->lo[2]; k <= r->hi[2]; k++)
+    for (int j = r->lo[1]; j <= r->hi[1]; j++)
+      for (int i = r->lo[0]; i <= r->hi[0]; i++) {
+        int ii = civ[0] + (i-iv[0])/m, jj = civ[1] + (j-iv[1])/m, kk = civ[2] + (k-iv[2])/m;
+        *P(phi,i,j,k) = *P(phi,i,j,k) + *P(coarse,ii,jj,kk);
+      }
+}
+/* average, refRatio 2 branch (lib/src/AMRTools/AverageF.ChF:9-60) */
+static void k_average2(Fab* coarse, const Fab* fine, const Box* b) {
+  double refScale = 1.0/8.0;
+  for (int k = b->lo[2]; k <= b->hi[2]; k++)
+    for (int j = b->lo[1]; j <= b->hi[1]; j++)
+      for (int i = b->lo[0]; i <= b->hi[0]; i++) {
+        int a = 2*i, c = 2*j, e = 2*k;
+        *P(coarse,i,j,k) = refScale *
+          (*P(fine,a,c,e) + *P(fine,a+1,c,e) + *P(fine,a,c+1,e) + *P(fine,a+1,c+1,e)
+         + *P(fine,a,c,e+1) + *P(fine,a+1,c,e+1) + *P(fine,a,c+1,e+1) + *P(fine,a+1,c+1,e+1));
+      }
+}
+/* INTERPHOMO (lib/src/AMRTools/InterpF.ChF:435-464) */
+static void k_interphomo(Fab* phi, const Box* r, double x1, double dxCrse, int idir, int ihilo) {
+  double c1 = 2.0*(dxCrse-x1)/(dxCrse+x1);
+  double c2 = -((dxCrse-x1)/(dxCrse+3.0*x1));
+  int s = -ihilo;
+  for (int k = r->lo[2]; k <= r->hi[2]; k++)
+    for (int j = r->lo[1]; j <= r->hi[1]; j++)
+      for (int i = r->lo[0]; i <= r->hi[0]; i++) {
+        int a[3] = {i,j,k}, b[3] = {i,j,k};
+        a[idir] += s; b[idir] += 2*s;
+        double pa = *P(phi,a[0],a[1],a[2]);
+        double pb = *P(phi,b[0],b[1],b[2]);
+        *P(phi,i,j,k) = c1*pa + c2*pb;
+      }
+}
+/* INTERPHOMOLINEAR (InterpF.ChF:466-494) */
+static void k_interphomolinear(Fab* phi, const Box* r, double x1, double dxCrse, int idir, int ihilo) {
+  double factor = 1.0 - 2.0*x1/(x1+dxCrse);
+  int s = -ihilo;
+  for (int k = r->lo[2]; k <= r->hi[2]; k++)
+    for (int j = r->lo[1]; j <= r->hi[1]; j++)
+      for (int i = r->lo[0]; i <= r->hi[0]; i++) {
+        int a[3] = {i,j,k}; a[idir] += s;
+        *P(phi,i,j,k) = factor * *P(phi,a[0],a[1],a[2]);
+      }
+}
+
+/* ======================================================================= */
+/* ConstBCFunction (lib/src/AMRElliptic/BCFunc.cpp:58-158)                 */
+/* ======================================================================= */
+typedef struct { int type[2][3]; double val[2][3]; } BCSpec;
+
+static void apply_bc(Fab* st, const Box* valid, const Box* domain, double dx, int homog,
+                     const BCSpec* bc) {
+  for (int idir = 0; idir < 3; idir++)
+    for (int side = 0; side < 2; side++) {
+      int vend = side ? valid->hi[idir] : valid->lo[idir];
+      int dend = side ? domain->hi[idir] : domain->lo[idir];
+      if (vend != dend) continue;
+      int type = bc->type[side][idir];
+      double value = bc->val[side][idir];
+      int isign = side ? 1 : -1;
+      Box to = b_isect(b_adj(*valid, idir, side, 1), &st->b);
+      if (b_empty(&to)) continue;
+      if (type == 0) {           /* Neumann */
+        for (int k = to.lo[2]; k <= to.hi[2]; k++)
+          for (int j = to.lo[1]; j <= to.hi[1]; j++)
+            for (int i = to.lo[0]; i <= to.hi[0]; i++) {
+              int c[3] = {i,j,k}; c[idir] -= isign;
+              *P(st,i,j,k) = *P(st,c[0],c[1],c[2]);
+            }
+        if (!homog)
+          for (int k = to.lo[2]; k <= to.hi[2]; k++)
+            for (int j = to.lo[1]; j <= to.hi[1]; j++)
+              for (int i = to.lo[0]; i <= to.hi[0]; i++)
+                *P(st,i,j,k) += (double)isign*dx*value;
+      } else {                   /* Dirichlet, linearInterp: 2*v - near */
+        for (int k = to.lo[2]; k <= to.hi[2]; k++)
+          for (int j = to.lo[1]; j <= to.hi[1]; j++)
+            for (int i = to.lo[0]; i <= to.hi[0]; i++) {
+              int c[3] = {i,j,k}; c[idir] -= isign;
+              double inhomogVal = 0.0;
+              if (!homog) inhomogVal = value;
+              double nearVal = *P(st,c[0],c[1],c[2]);
+              *P(st,i,j,k) = 2*inhomogVal - nearVal;
+            }
+      }
+    }
+}
+
+/* ======================================================================= */
+/* Copier motion plans (lib/src/BoxTools/Copier.cpp)                       */
+/* ======================================================================= */
+typedef struct { int from, to; Box region; } Motion;
+typedef struct { int n, cap; Motion* m; } Plan;
+
+static void plan_push(Plan* p, int from, int to, Box r) {
+  if (p->n == p->cap) { p->cap = p->cap ? 2*p->cap : 16; p->m = realloc(p->m, sizeof(Motion)*p->cap); }
+  p->m[p->n].from = from; p->m[p->n].to = to; p->m[p->n].region = r; p->n++;
+}
+static void plan_free(Plan* p) { free(p->m); p->m = NULL; p->n = p->cap = 0; }
+/* MotionItemSorter (Copier.cpp:1547-1563): by toRegion, smallEnd then bigEnd */
+static int motion_cmp(const void* a, const void* b) {
+  const Motion* x = a; const Motion* y = b;
+  if (b_lt(&x->region, &y->region)) return -1;
+  if (b_lt(&y->region, &x->region)) return 1;
+  return 0;
+}
+/* Copier::exchangeDefine + trimEdges (Copier.cpp:1464-1541, 142-177) */
+static Plan plan_exchange(const Layout* L) {
+  Plan p = {0,0,NULL};
+  for (int t = 0; t < L->n; t++) {
+    Box bg = b_grow(L->b[t], 1);
+    for (int f = 0; f < L->n; f++) {
+      if (f == t) continue;
+      Box r = b_isect(L->b[f], &bg);
+      if (b_empty(&r)) continue;
+      int keep = 0;
+      for (int d = 0; d < 3 && !keep; d++) {
+        Box g = b_growd(L->b[t], d, 1);
+        if (b_intersects(&g, &r)) keep = 1;
+      }
+      if (keep) plan_push(&p, f, t, r);
+    }
+  }
+  qsort(p.m, p.n, sizeof(Motion), motion_cmp);
+  return p;
+}
+/* Copier::define(src layout, dest layout, ghost) (Copier.cpp:903-1140): for each
+   dest box grown by ghost, every intersecting src box */
+static Plan plan_copy(const Layout* src, const Layout* dst, int dst_ghost) {
+  Plan p = {0,0,NULL};
+  for (int t = 0; t < dst->n; t++) {
+    Box g = b_grow(dst->b[t], dst_ghost);
+    for (int f = 0; f < src->n; f++) {
+      Box r = b_isect(src->b[f], &g);
+      if (!b_empty(&r)) plan_push(&p, f, t, r);
+    }
+  }
+  qsort(p.m, p.n, sizeof(Motion), motion_cmp);
+  return p;
+}
+static void plan_reverse(Plan* p) {
+  for (int q = 0; q < p->n; q++) { int t = p->m[q].from; p->m[q].from = p->m[q].to; p->m[q].to = t; }
+  qsort(p->m, p->n, sizeof(Motion), motion_cmp);
+}
+static void exec_exchange(LD* x, const Plan* p) {
+  for (int q = 0; q < p->n; q++)
+    fab_copy_region(&x->f[p->m[q].to], &x->f[p->m[q].from], p->m[q].region);
+}
+/* LevelData::copyTo(..., copier) with plain copy */
+static void exec_copy(LD* dst, const LD* src, const Plan* p) {
+  for (int q = 0; q < p->n; q++)
+    fab_copy_region(&dst->f[p->m[q].to], &src->f[p->m[q].from], p->m[q].region);
+}
+
+/* ======================================================================= */
+/* CFRegion / CFIVS (lib/src/AMRTools/CFRegion.cpp, CFIVS.cpp)             */
+/* ======================================================================= */
+typedef struct { int empty, packed; Box packedBox; IVS ivs; } CFIVS;
+
+static void cfivs_define(CFIVS* c, const Box* domain, const Box* box, const Layout* L,
+                         int dir, int side) {
+  ivs_init_empty(&c->ivs);
+  c->empty = 1; c->packed = 0; c->packedBox = mkbox(0,0,0,-1,-1,-1);
+  Box edge = b_isect(b_adj(*box, dir, side, 1), domain);
+  if (b_empty(&edge)) return;
+  ivs_define(&c->ivs, edge);
+  for (int q = 0; q < L->n; q++) ivs_sub_box(&c->ivs, &L->b[q]);
+  /* packIVS */
+  if (ivs_numpts(&c->ivs) == 0) { c->empty = 1; c->packed = 0; return; }
+  c->empty = 0;
+  c->packedBox = ivs_minbox(&c->ivs);
+  c->packed = (b_npts(&c->packedBox) == ivs_numpts(&c->ivs));
+}
+
+/* ======================================================================= */
+/* QuadCFStencil / QuadCFInterp (lib/src/AMRTools/QuadCFStencil.cpp,       */
+/* QuadCFInterp.cpp, QuadCFInterpF.ChF, CFStencil.cpp, DerivStencil.cpp)   */
+/* ======================================================================= */
+#define MAXS 16
+typedef struct { int n; int iv[MAXS][3]; double w[MAXS]; } DSten;
+
+static void dsten_acc(DSten* s, int i, int j, int k, double w) {  /* DerivStencil::accumulate */
+  for (int q = 0; q < s->n; q++)
+    if (s->iv[q][0] == i && s->iv[q][1] == j && s->iv[q][2] == k) { s->w[q] += w; return; }
+  s->iv[s->n][0] = i; s->iv[s->n][1] = j; s->iv[s->n][2] = k; s->w[s->n] = w; s->n++;
+}
+/* addFabToSten: BoxIterator order over the fab box, weight = fab value */
+typedef struct { Box b; double v[27]; } SmallFab;
+static void sfab_init(SmallFab* f, Box b, double v0) { f->b = b; for (int q = 0; q < 27; q++) f->v[q] = v0; }
+static double* sfab(SmallFab* f, int i, int j, int k) {
+  return &f->v[((k-f->b.lo[2])*b_len(&f->b,1) + (j-f->b.lo[1]))*b_len(&f->b,0) + (i-f->b.lo[0])];
+}
+static void add_fab_to_sten(SmallFab* f, DSten* s) {
+  for (int k = f->b.lo[2]; k <= f->b.hi[2]; k++)
+    for (int j = f->b.lo[1]; j <= f->b.hi[1]; j++)
+      for (int i = f->b.lo[0]; i <= f->b.hi[0]; i++) dsten_acc(s, i, j, k, *sfab(f,i,j,k));
+}
+
+typedef struct {
+  int defined_nonempty;        /* coarse IVS non-empty */
+  int packed; Box packedBox;   /* m_baseCFS */
+  IVS fine, coar;              /* m_baseCFS fine / coarse IVS */
+  IVS standard, quadd;
+  Box osbox;
+  unsigned char* dropOrd;      /* over osbox */
+  DSten* mixed;                /* over osbox */
+  DSten* first[3];             /* per derivative direction, over osbox */
+  DSten* second[3];
+  int dir;
+} QCFS;
+
+static long os_off(const QCFS* s, int i, int j, int k) {
+  const Box* b = &s->osbox;
+  return ((long)(k-b->lo[2])*b_len(b,1) + (j-b->lo[1]))*b_len(b,0) + (i-b->lo[0]);
+}
+
+/* QuadCFStencil::buildStencils (QuadCFStencil.cpp:662-918) */
+static void qcfs_build_stencils(QCFS* s, const IVS* allGood) {
+  int vt[2], n = 0;
+  for (int d = 0; d < 3; d++) if (d != s->dir) vt[n++] = d;
+  s->osbox = ivs_minbox(&s->quadd);
+  long no = b_npts(&s->osbox);
+  s->dropOrd = calloc(no, 1);
+  s->mixed = calloc(no, sizeof(DSten));
+  for (int d = 0; d < 3; d++) { s->first[d] = calloc(no, sizeof(DSten)); s->second[d] = calloc(no, sizeof(DSten)); }
+  int t1 = vt[0], t2 = vt[1];
+  /* four 2x2 corner stencils: value -1 at lo corner and far corner, +1 at lo+e_t1, lo+e_t2 */
+  for (int k = s->osbox.lo[2]; k <= s->osbox.hi[2]; k++)
+    for (int j = s->osbox.lo[1]; j <= s->osbox.hi[1]; j++)
+      for (int i = s->osbox.lo[0]; i <= s->osbox.hi[0]; i++) {
+        if (!ivs_has(&s->quadd, i, j, k)) continue;
+        int iv[3] = {i,j,k};
+        /* lo-corner offsets of fabur, fabul, fablr, fabll (see the shifts at
+           QuadCFStencil.cpp:700-722: fabur ends up shifted by -e_t1, fabul unshifted) */
+        int offs[4][2] = {{-1,0},{0,0},{0,-1},{-1,-1}};
+        DSten* ds = &s->mixed[os_off(s,i,j,k)];
+        ds->n = 0;
+        int imixed = 0, hasamixed = 0;
+        for (int c = 0; c < 4; c++) {
+          Box fb; for (int d = 0; d < 3; d++) { fb.lo[d] = iv[d]; fb.hi[d] = iv[d]; }
+          fb.lo[t1] += offs[c][0]; fb.hi[t1] = fb.lo[t1] + 1;
+          fb.lo[t2] += offs[c][1]; fb.hi[t2] = fb.lo[t2] + 1;
+          if (!ivs_has_box(allGood, &fb)) continue;
+          SmallFab f; sfab_init(&f, fb, -1.0);
+          int a[3] = {fb.lo[0], fb.lo[1], fb.lo[2]}, b[3] = {fb.lo[0], fb.lo[1], fb.lo[2]};
+          a[t1] += 1; b[t2] += 1;
+          *sfab(&f, a[0], a[1], a[2]) = 1.0;
+          *sfab(&f, b[0], b[1], b[2]) = 1.0;
+          add_fab_to_sten(&f, ds);
+          hasamixed = 1; imixed++;
+        }
+        if (imixed > 0) { double rn = imixed; for (int q = 0; q < ds->n; q++) ds->w[q] /= rn; }
+        s->dropOrd[os_off(s,i,j,k)] = !hasamixed;
+      }
+  for (int ivec = 0; ivec < 2; ivec++) {
+    int idir = vt[ivec];
+    for (int k = s->osbox.lo[2]; k <= s->osbox.hi[2]; k++)
+      for (int j = s->osbox.lo[1]; j <= s->osbox.hi[1]; j++)
+        for (int i = s->osbox.lo[0]; i <= s->osbox.hi[0]; i++) {
+          if (!ivs_has(&s->quadd, i, j, k)) continue;
+          long o = os_off(s,i,j,k);
+          if (s->dropOrd[o]) continue;
+          int iv[3] = {i,j,k};
+          DSten* sec = &s->second[idir][o];
+          DSten* fir = &s->first[idir][o];
+          Box bsten; for (int d = 0; d < 3; d++) { bsten.lo[d] = iv[d]; bsten.hi[d] = iv[d]; }
+          bsten = b_growd(bsten, idir, 1);
+          int m[3] = {i,j,k}, p[3] = {i,j,k}, p2[3] = {i,j,k}, m2[3] = {i,j,k};
+          m[idir] -= 1; p[idir] += 1; p2[idir] += 2; m2[idir] -= 2;
+          if (ivs_has_box(allGood, &bsten)) {
+            SmallFab f; sfab_init(&f, bsten, 0.0);
+            *sfab(&f,m[0],m[1],m[2]) = 1.0; *sfab(&f,p[0],p[1],p[2]) = 1.0; *sfab(&f,i,j,k) = -2.0;
+            sec->n = 0; add_fab_to_sten(&f, sec);
+            *sfab(&f,m[0],m[1],m[2]) = -0.5; *sfab(&f,p[0],p[1],p[2]) = 0.5; *sfab(&f,i,j,k) = 0.0;
+            fir->n = 0; add_fab_to_sten(&f, fir);
+            continue;
+          }
+          Box bshift = b_shift(bsten, idir, 1);
+          if (ivs_has_box(allGood, &bshift)) {
+            SmallFab f; sfab_init(&f, bshift, 0.0);
+            *sfab(&f,i,j,k) = 1.0; *sfab(&f,p[0],p[1],p[2]) = -2.0; *sfab(&f,p2[0],p2[1],p2[2]) = 1.0;
+            sec->n = 0; add_fab_to_sten(&f, sec);
+            *sfab(&f,i,j,k) = -3.0/2.0; *sfab(&f,p[0],p[1],p[2]) = 4.0/2.0; *sfab(&f,p2[0],p2[1],p2[2]) = -1.0/2.0;
+            fir->n = 0; add_fab_to_sten(&f, fir);
+            continue;
+          }
+          bshift = b_shift(bshift, idir, -2);
+          if (ivs_has_box(allGood, &bshift)) {
+            SmallFab f; sfab_init(&f, bshift, 0.0);
+            *sfab(&f,i,j,k) = 1.0; *sfab(&f,m[0],m[1],m[2]) = -2.0; *sfab(&f,m2[0],m2[1],m2[2]) = 1.0;
+            sec->n = 0; add_fab_to_sten(&f, sec);
+            *sfab(&f,i,j,k) = 3.0/2.0; *sfab(&f,m[0],m[1],m[2]) = -4.0/2.0; *sfab(&f,m2[0],m2[1],m2[2]) = 1.0/2.0;
+            fir->n = 0; add_fab_to_sten(&f, fir);
+            continue;
+          }
+          s->dropOrd[o] = 1;
+          if (ivs_has(allGood, p[0], p[1], p[2])) {
+            Box bf; for (int d = 0; d < 3; d++) { bf.lo[d] = iv[d]; bf.hi[d] = p[d]; }
+            SmallFab f; sfab_init(&f, bf, 0.0);
+            *sfab(&f,i,j,k) = -1.0; *sfab(&f,p[0],p[1],p[2]) = 1.0;
+            fir->n = 0; add_fab_to_sten(&f, fir);
+          } else if (ivs_has(allGood, m[0], m[1], m[2])) {
+            Box bf; for (int d = 0; d < 3; d++) { bf.lo[d] = m[d]; bf.hi[d] = iv[d]; }
+            SmallFab f; sfab_init(&f, bf, 0.0);
+            *sfab(&f,i,j,k) = 1.0; *sfab(&f,m[0],m[1],m[2]) = -1.0;
+            fir->n = 0; add_fab_to_sten(&f, fir);
+          } else {
+            Box bf; for (int d = 0; d < 3; d++) { bf.lo[d] = iv[d]; bf.hi[d] = iv[d]; }
+            SmallFab f; sfab_init(&f, bf, 0.0);
+            fir->n = 0; add_fab_to_sten(&f, fir);
+          }
+        }
+  }
+}
+
+/* CFStencil::define (CFStencil.cpp) + QuadCFStencil::define (QuadCFStencil.cpp:455-660) */
+static void qcfs_define(QCFS* s, const Box* fineDomain, const Box* grid, const Layout* fineL,
+                        const Layout* coarL, int ref, int dir, int side) {
+  memset(s, 0, sizeof *s);
+  ivs_init_empty(&s->fine); ivs_init_empty(&s->coar);
+  ivs_init_empty(&s->standard); ivs_init_empty(&s->quadd);
+  s->dir = dir;
+  Box edge = b_isect(b_adj(*grid, dir, side, 1), fineDomain);
+  if (b_empty(&edge)) return;
+  ivs_define(&s->fine, edge);
+  for (int q = 0; q < fineL->n; q++) ivs_sub_box(&s->fine, &fineL->b[q]);
+  ivs_coarsen(&s->coar, &s->fine, ref);
+  Box cdom = b_coarsen(*fineDomain, ref);
+  Box dib = b_isect(ivs_minbox(&s->coar), &cdom);
+  ivs_and_box(&s->coar, &dib);
+  s->packedBox = ivs_minbox(&s->fine);
+  s->packed = (ivs_numpts(&s->fine) > 0 && ivs_numpts(&s->fine) == b_npts(&s->packedBox));
+  if (ivs_numpts(&s->coar) == 0) return;
+  s->defined_nonempty = 1;
+
+  int vt[2], n = 0;
+  for (int d = 0; d < 3; d++) if (d != dir) vt[n++] = d;
+  Box cgrid = b_coarsen(*grid, ref);
+  Box g2 = b_adj(cgrid, dir, side, 1), g1 = g2;
+  for (int t = 0; t < 2; t++) { g2 = b_growd(g2, vt[t], 2); g1 = b_growd(g1, vt[t], 1); }
+  g2 = b_isect(g2, &cdom); g1 = b_isect(g1, &cdom);
+  if (b_empty(&g2)) return;
+  IVS allGood; ivs_init_empty(&allGood); ivs_define(&allGood, g2);
+  for (int q = 0; q < fineL->n; q++) { Box cb = b_coarsen(fineL->b[q], ref); ivs_sub_box(&allGood, &cb); }
+  /* ivsEdgeOfEarth = allGood minus coarse boxes; allGood -= edgeOfEarth */
+  IVS eoe; ivs_init_empty(&eoe); ivs_copy(&eoe, &allGood);
+  for (int q = 0; q < coarL->n; q++) ivs_sub_box(&eoe, &coarL->b[q]);
+  ivs_sub_ivs(&allGood, &eoe);
+  ivs_free(&eoe);
+  ivs_copy(&s->standard, &allGood);
+  ivs_and_box(&s->standard, &g1);
+  for (int t = 0; t < 2; t++) ivs_erode(&s->standard, vt[t]);
+  ivs_copy(&s->quadd, &s->coar);
+  ivs_sub_ivs(&s->quadd, &s->standard);
+  if (ivs_numpts(&s->quadd) > 0) qcfs_build_stencils(s, &allGood);
+  ivs_free(&allGood);
+}
+static void qcfs_free(QCFS* s) {
+  ivs_free(&s->fine); ivs_free(&s->coar); ivs_free(&s->standard); ivs_free(&s->quadd);
+  free(s->dropOrd); free(s->mixed);
+  for (int d = 0; d < 3; d++) { free(s->first[d]); free(s->second[d]); }
+}
+
+/* QuadCFStencil::computeFirstDerivative (QuadCFStencil.cpp:130-185) */
+static double qcfs_first(const QCFS* s, const Fab* phic, int dd, int i, int j, int k, double dx) {
+  if (ivs_has(&s->standard, i, j, k)) {
+    int lo[3] = {i,j,k}, hi[3] = {i,j,k}; lo[dd] -= 1; hi[dd] += 1;
+    double philo = *P(phic,lo[0],lo[1],lo[2]);
+    double phihi = *P(phic,hi[0],hi[1],hi[2]);
+    return (phihi - philo)/(2.0*dx);
+  }
+  const DSten* ds = &s->first[dd][os_off(s,i,j,k)];
+  double firstd = 0.0; int keepzer = 0;
+  for (int q = 0; q < ds->n; q++) {
+    if (b_has(&phic->b, ds->iv[q][0], ds->iv[q][1], ds->iv[q][2]))
+      firstd += ds->w[q] * *P(phic, ds->iv[q][0], ds->iv[q][1], ds->iv[q][2]);
+    else keepzer = 1;
+  }
+  if (keepzer) firstd = 0.0;
+  firstd /= dx;
+  return firstd;
+}
+/* QuadCFStencil::computeSecondDerivative (QuadCFStencil.cpp:36-120) */
+static double qcfs_second(const QCFS* s, const Fab* phic, int dd, int i, int j, int k, double dx) {
+  if (ivs_has(&s->standard, i, j, k)) {
+    int lo[3] = {i,j,k}, hi[3] = {i,j,k}; lo[dd] -= 1; hi[dd] += 1;
+    double phicc = *P(phic,i,j,k);
+    double philo = *P(phic,lo[0],lo[1],lo[2]);
+    double phihi = *P(phic,hi[0],hi[1],hi[2]);
+    return (phihi + philo - 2.0*phicc)/(dx*dx);
+  }
+  long o = os_off(s,i,j,k);
+  if (s->dropOrd[o]) return 0.0;
+  const DSten* ds = &s->second[dd][o];
+  double secondd = 0.0; int keepzer = 0;
+  for (int q = 0; q < ds->n; q++) {
+    if (b_has(&phic->b, ds->iv[q][0], ds->iv[q][1], ds->iv[q][2]))
+      secondd += ds->w[q] * *P(phic, ds->iv[q][0], ds->iv[q][1], ds->iv[q][2]);
+    else keepzer = 1;
+  }
+  if (keepzer) secondd = 0.0;
+  secondd /= (dx*dx);
+  return secondd;
+}
+/* QuadCFStencil::computeMixedDerivative (QuadCFStencil.cpp:196-268) */
+static double qcfs_mixed(const QCFS* s, const Fab* phic, int i, int j, int k, double dx) {
+  int itran1, itran2;
+  if (s->dir == 0) { itran2 = 1; itran1 = 2; }
+  else if (s->dir == 1) { itran2 = 0; itran1 = 2; }
+  else { itran2 = 0; itran1 = 1; }
+  if (ivs_has(&s->standard, i, j, k)) {
+    int ur[3] = {i,j,k}, ul[3] = {i,j,k}, lr[3] = {i,j,k}, ll[3] = {i,j,k};
+    ur[itran2] += 1; ur[itran1] += 1;
+    ul[itran2] -= 1; ul[itran1] += 1;
+    lr[itran2] += 1; lr[itran1] -= 1;
+    ll[itran2] -= 1; ll[itran1] -= 1;
+    double phiur = *P(phic,ur[0],ur[1],ur[2]);
+    double phiul = *P(phic,ul[0],ul[1],ul[2]);
+    double philr = *P(phic,lr[0],lr[1],lr[2]);
+    double phill = *P(phic,ll[0],ll[1],ll[2]);
+    return (phiur+phill-philr-phiul)/(4.*dx*dx);
+  }
+  long o = os_off(s,i,j,k);
+  if (s->dropOrd[o]) return 0.0;
+  const DSten* ds = &s->mixed[o];
+  double mixedd = 0.0; int keepzer = 0;
+  for (int q = 0; q < ds->n; q++) {
+    if (b_has(&phic->b, ds->iv[q][0], ds->iv[q][1], ds->iv[q][2]))
+      mixedd += ds->w[q] * *P(phic, ds->iv[q][0], ds->iv[q][1], ds->iv[q][2]);
+    else keepzer = 1;
+  }
+  if (keepzer) mixedd = 0.0;
+  mixedd /= (dx*dx);
+  return mixedd;
+}
+
+typedef struct {
+  int level;               /* 0 when there is no coarser level or it is covered */
+  double dxFine; int ref;
+  Layout coarBoxes;        /* coarsen(fine box, ref) grown by 2, one per fine box */
+  LD buf;                  /* coarse buffer on coarBoxes */
+  Plan copier;             /* coarse grids -> coarBoxes */
+  QCFS* st;                /* [fine box][dir][side] */
+  const Layout* fineL;
+} QCFI;
+
+/* QuadCFInterp::define (QuadCFInterp.cpp:397-525) */
+static void qcfi_define(QCFI* q, const Layout* fineL, const Layout* coarL, double dxFine, int ref,
+                        const Box* fineDomain) {
+  memset(q, 0, sizeof *q);
+  q->dxFine = dxFine; q->ref = ref; q->fineL = fineL;
+  long numPts = lay_cells(fineL)/(ref*ref*ref) - lay_cells(coarL);
+  q->level = (numPts == 0) ? 0 : 1;
+  if (!q->level) return;
+  q->coarBoxes.n = fineL->n; q->coarBoxes.b = malloc(sizeof(Box)*fineL->n);
+  for (int b = 0; b < fineL->n; b++) q->coarBoxes.b[b] = b_grow(b_coarsen(fineL->b[b], ref), 2);
+  q->buf = ld_make(&q->coarBoxes, 0);
+  q->copier = plan_copy(coarL, &q->coarBoxes, 0);
+  q->st = calloc((size_t)fineL->n*6, sizeof(QCFS));
+  for (int b = 0; b < fineL->n; b++)
+    for (int d = 0; d < 3; d++)
+      for (int s = 0; s < 2; s++)
+        qcfs_define(&q->st[b*6+d*2+s], fineDomain, &fineL->b[b], fineL, coarL, ref, d, s);
+}
+static void qcfi_free(QCFI* q) {
+  if (!q->level) return;
+  for (int b = 0; b < q->fineL->n*6; b++) qcfs_free(&q->st[b]);
+  free(q->st); ld_free(&q->buf); plan_free(&q->copier); free(q->coarBoxes.b);
+}
+
+/* getPhiStar (QuadCFInterp.cpp:570-764) + interpOnIVS (:775-839) for one face */
+static void qcfi_face(const QCFI* q, Fab* phif, const Fab* phic, const QCFS* s, int side, int idir) {
+  if (!s->defined_nonempty && ivs_numpts(&s->fine) == 0) return;
+  if (ivs_numpts(&s->fine) == 0) return;
+  double dxf = q->dxFine, dxc = q->ref*dxf;
+  int you1, you2;
+  if (idir == 0) { you1 = 1; you2 = 2; } else if (idir == 1) { you1 = 0; you2 = 2; } else { you1 = 0; you2 = 1; }
+  int ihilo = side ? 1 : -1;
+  Box cib = ivs_minbox(&s->coar);
+  Box psb = b_shift(ivs_minbox(&s->fine), idir, ihilo);
+  Fab phistar; fab_alloc(&phistar, psb);
+  /* slopes on the coarse IVS */
+  Fab slope[3], curva[3], mixed;
+  int dd[2] = {you1, you2};
+  for (int t = 0; t < 2; t++) { fab_alloc(&slope[dd[t]], cib); fab_alloc(&curva[dd[t]], cib); }
+  fab_alloc(&mixed, cib);
+  for (int k = cib.lo[2]; k <= cib.hi[2]; k++)
+    for (int j = cib.lo[1]; j <= cib.hi[1]; j++)
+      for (int i = cib.lo[0]; i <= cib.hi[0]; i++) {
+        if (!ivs_has(&s->coar, i, j, k)) continue;
+        *P(&slope[you1],i,j,k) = qcfs_first(s, phic, you1, i, j, k, dxc);
+        *P(&curva[you1],i,j,k) = qcfs_second(s, phic, you1, i, j, k, dxc);
+        *P(&slope[you2],i,j,k) = qcfs_first(s, phic, you2, i, j, k, dxc);
+        *P(&curva[you2],i,j,k) = qcfs_second(s, phic, you2, i, j, k, dxc);
+        *P(&mixed,i,j,k) = qcfs_mixed(s, phic, i, j, k, dxc);
+      }
+  if (s->packed) {
+    /* FORT_PHISTAR (QuadCFInterpF.ChF:50-126) with CHF_*_SHIFT(iv / civ) */
+    int iv[3] = {psb.lo[0], psb.lo[1], psb.lo[2]};
+    int civ[3] = {fdiv(iv[0],q->ref), fdiv(iv[1],q->ref), fdiv(iv[2],q->ref)};
+    const Box* r = &s->packedBox;
+    int nRef = q->ref;
+    double dxcf = nRef*dxf;
+    for (int k = r->lo[2]; k <= r->hi[2]; k++)
+      for (int j = r->lo[1]; j <= r->hi[1]; j++)
+        for (int i = r->lo[0]; i <= r->hi[0]; i++) {
+          int ir[3] = {i-iv[0], j-iv[1], k-iv[2]};
+          int ic[3] = {ir[0]/nRef, ir[1]/nRef, ir[2]/nRef};   /* Fortran truncating division */
+          int cg[3] = {ic[0]+civ[0], ic[1]+civ[1], ic[2]+civ[2]};
+          double xf1 = (ir[you1]+0.5)*dxf;
+          double xc1 = (ic[you1]+0.5)*dxcf;
+          double xf2 = (ir[you2]+0.5)*dxf;
+          double xc2 = (ic[you2]+0.5)*dxcf;
+          double x1 = xf1-xc1, x2 = xf2-xc2;
+          double aa = *P(phic, cg[0], cg[1], cg[2]);
+          double update1 = x1* *P(&slope[you1],cg[0],cg[1],cg[2]) + 0.5*x1*x1* *P(&curva[you1],cg[0],cg[1],cg[2]);
+          double update2 = x2* *P(&slope[you2],cg[0],cg[1],cg[2]) + 0.5*x2*x2* *P(&curva[you2],cg[0],cg[1],cg[2]);
+          double update3 = x1*x2* *P(&mixed,cg[0],cg[1],cg[2]);
+          int st[3] = {i,j,k}; st[idir] += ihilo;
+          *P(&phistar, st[0], st[1], st[2]) = aa+update1+update2+update3;
+        }
+  } else {
+    const Box mb = s->fine.mb;
+    for (int k = mb.lo[2]; k <= mb.hi[2]; k++)
+      for (int j = mb.lo[1]; j <= mb.hi[1]; j++)
+        for (int i = mb.lo[0]; i <= mb.hi[0]; i++) {
+          if (!ivs_has(&s->fine, i, j, k)) continue;
+          int ivf[3] = {i,j,k};
+          int ivc[3] = {fdiv(i,q->ref), fdiv(j,q->ref), fdiv(k,q->ref)};
+          double pc = *P(phic, ivc[0], ivc[1], ivc[2]);
+          int jf = ivf[you1], jc = ivc[you1];
+          double xf = (jf+0.5)*dxf, xc = (jc+0.5)*dxc;
+          double x1 = xf-xc;
+          double update1 = x1* *P(&slope[you1],ivc[0],ivc[1],ivc[2]) + 0.5*x1*x1* *P(&curva[you1],ivc[0],ivc[1],ivc[2]);
+          jf = ivf[you2]; jc = ivc[you2];
+          xf = (jf+0.5)*dxf; xc = (jc+0.5)*dxc;
+          double x2 = xf-xc;
+          double update2 = x2* *P(&slope[you2],ivc[0],ivc[1],ivc[2]) + 0.5*x2*x2* *P(&curva[you2],ivc[0],ivc[1],ivc[2]);
+          double update3 = x1*x2* *P(&mixed,ivc[0],ivc[1],ivc[2]);
+          int st[3] = {i,j,k}; st[idir] += ihilo;
+          *P(&phistar, st[0], st[1], st[2]) = pc+update1+update2+update3;
+        }
+  }
+  /* interpOnIVS */
+  int nref = q->ref;
+  if (!s->packed) {
+    const Box mb = s->fine.mb;
+    for (int k = mb.lo[2]; k <= mb.hi[2]; k++)
+      for (int j = mb.lo[1]; j <= mb.hi[1]; j++)
+        for (int i = mb.lo[0]; i <= mb.hi[0]; i++) {
+          if (!ivs_has(&s->fine, i, j, k)) continue;
+          int a2[3] = {i,j,k}, a1[3] = {i,j,k}, sp[3] = {i,j,k};
+          a2[idir] -= 2*ihilo; a1[idir] -= ihilo; sp[idir] += ihilo;
+          double pa = *P(phif, a2[0], a2[1], a2[2]);
+          double pb = *P(phif, a1[0], a1[1], a1[2]);
+          double ps = *P(&phistar, sp[0], sp[1], sp[2]);
+          double h = dxf;
+          double a = (2./h/h)*(2.*ps + pa*(nref+1.0) -pb*(nref+3.0))/(nref*nref + 4*nref + 3.0);
+          double b = (pb-pa)/h - a*h;
+          double c = pa;
+          double x = 2.*h;
+          *P(phif, i, j, k) = a*x*x + b*x + c;
+        }
+  } else {
+    /* FORT_QUADINTERP (QuadCFInterpF.ChF:9-47) */
+    double h = dxf;
+    double frac = 2.0/(h*h);
+    double denom = nref*nref + 4*nref + 3;
+    double mult = frac/denom;
+    double invh = 1.0/h;
+    double x = 2.0*h;
+    double xsquared = 4.0*h*h;
+    const Box* r = &s->packedBox;
+    for (int k = r->lo[2]; k <= r->hi[2]; k++)
+      for (int j = r->lo[1]; j <= r->hi[1]; j++)
+        for (int i = r->lo[0]; i <= r->hi[0]; i++) {
+          int a2[3] = {i,j,k}, a1[3] = {i,j,k}, sp[3] = {i,j,k};
+          a2[idir] -= 2*ihilo; a1[idir] -= ihilo; sp[idir] += ihilo;
+          double pa = *P(phif, a2[0], a2[1], a2[2]);
+          double pb = *P(phif, a1[0], a1[1], a1[2]);
+          double ps = *P(&phistar, sp[0], sp[1], sp[2]);
+          double a = mult*(2.0*ps + (nref+1)*pa - (nref+3)*pb);
+          double b = (pb-pa)*invh - a*h;
+          *P(phif, i, j, k) = xsquared*a + b*x + pa;
+        }
+  }
+  fab_free(&phistar);
+  for (int t = 0; t < 2; t++) { fab_free(&slope[dd[t]]); fab_free(&curva[dd[t]]); }
+  fab_free(&mixed);
+}
+
+/* QuadCFInterp::coarseFineInterp(LevelData&, const LevelData&) (QuadCFInterp.cpp:846-889) */
+static void qcfi_interp(QCFI* q, LD* phif, const LD* phic) {
+  if (!q->level) return;
+  exec_copy(&q->buf, phic, &q->copier);
+  for (int idir = 0; idir < 3; idir++)
+    for (int b = 0; b < phif->L->n; b++)
+      for (int side = 0; side < 2; side++)
+        qcfi_face(q, &phif->f[b], &q->buf.f[b], &q->st[b*6+idir*2+side], side, idir);
+}
+
+/* ======================================================================= */
+/* LevelFluxRegister (lib/src/AMRTools/LevelFluxRegister.cpp)              */
+/* ======================================================================= */
+typedef struct { int n; Box* b; } BoxList;
+typedef struct {
+  const Layout* coarL; const Layout* fineL;
+  Layout cfine;              /* coarsened fine layout */
+  BoxList* loc;              /* [coarse box][dir][side] */
+  LD coarFlux;               /* on coarse grids */
+  LD fineFlux;               /* on coarsened fine grids, ghost 1 */
+  Plan rev;                  /* fine registers (grown by 1) -> coarse, sorted by toRegion */
+  int nRefine;
+} FluxReg;
+
+static void bl_push(BoxList* l, Box b) { l->b = realloc(l->b, sizeof(Box)*(l->n+1)); l->b[l->n++] = b; }
+
+/* LevelFluxRegister::define (LevelFluxRegister.cpp:101-262), non-periodic */
+static void fr_define(FluxReg* r, const Layout* fineL, const Layout* coarL, int nref) {
+  memset(r, 0, sizeof *r);
+  r->coarL = coarL; r->fineL = fineL; r->nRefine = nref;
+  r->cfine = lay_coarsen(fineL, nref);
+  r->coarFlux = ld_make(coarL, 0);
+  r->fineFlux = ld_make(&r->cfine, 1);
+  r->loc = calloc((size_t)coarL->n*6, sizeof(BoxList));
+  for (int c = 0; c < coarL->n; c++) {
+    const Box* cb = &coarL->b[c];
+    for (int f = 0; f < r->cfine.n; f++) {
+      const Box* fb = &r->cfine.b[f];
+      if (fb->hi[0]+1 < cb->lo[0]) continue;
+      if (fb->lo[0]-1 > cb->hi[0]) break;
+      for (int i = 0; i < 3; i++) {
+        Box lo = b_adj(*fb, i, 0, 1), hi = b_adj(*fb, i, 1, 1);
+        if (b_intersects(cb, &lo)) bl_push(&r->loc[c*6+i*2+0], b_isect(lo, cb));
+        if (b_intersects(cb, &hi)) bl_push(&r->loc[c*6+i*2+1], b_isect(hi, cb));
+      }
+    }
+  }
+  /* m_reverseCopier.ghostDefine(coarsenedFine, coarse, domain, Unit):
+     Copier::define(coarse, coarsenedFine, ghost 1), reversed, sorted */
+  r->rev = plan_copy(coarL, &r->cfine, 1);
+  plan_reverse(&r->rev);
+}
+static void fr_free(FluxReg* r) {
+  if (!r->loc) return;
+  for (int q = 0; q < r->coarL->n*6; q++) free(r->loc[q].b);
+  free(r->loc); ld_free(&r->coarFlux); ld_free(&r->fineFlux); plan_free(&r->rev); free(r->cfine.b);
+  r->loc = NULL;
+}
+
+/* ======================================================================= */
+/* AMRPoissonOp (lib/src/AMRElliptic/AMRPoissonOp.cpp)                     */
+/* ======================================================================= */
+typedef struct Op {
+  Layout L; Box domain; double dx, dxCrse, alpha, beta; BCSpec bc;
+  Plan ex; CFIVS* cf;
+  int hasCoarser, hasFiner, refToCoarser, refToFiner;
+  QCFI qcfi;          /* interpolation from the next coarser AMR level */
+  FluxReg fr;         /* register with the next finer AMR level */
+} Op;
+
+static void op_define(Op* op, const Layout* L, const Box* domain, double dx, double alpha,
+                      double beta, const BCSpec* bc) {
+  memset(op, 0, sizeof *op);
+  op->L.n = L->n; op->L.b = malloc(sizeof(Box)*(L->n ? L->n : 1));
+  memcpy(op->L.b, L->b, sizeof(Box)*L->n);
+  op->domain = *domain; op->dx = dx; op->dxCrse = 2*dx; op->alpha = alpha; op->beta = beta;
+  op->bc = *bc; op->refToCoarser = 2; op->refToFiner = 2;
+  op->ex = plan_exchange(&op->L);
+  op->cf = calloc((size_t)L->n*6, sizeof(CFIVS));
+  for (int b = 0; b < L->n; b++)
+    for (int d = 0; d < 3; d++)
+      for (int s = 0; s < 2; s++) cfivs_define(&op->cf[b*6+d*2+s], domain, &L->b[b], &op->L, d, s);
+}
+static void op_free(Op* op) {
+  for (int b = 0; b < op->L.n*6; b++) ivs_free(&op->cf[b].ivs);
+  free(op->cf); plan_free(&op->ex);
+  if (op->hasCoarser) qcfi_free(&op->qcfi);
+  if (op->hasFiner) fr_free(&op->fr);
+  free(op->L.b);
+}
+
+static void op_exchange(const Op* op, LD* phi) { exec_exchange(phi, &op->ex); }
+static void op_bc(const Op* op, LD* phi, int homog) {
+  for (int b = 0; b < op->L.n; b++) apply_bc(&phi->f[b], &op->L.b[b], &op->domain, op->dx, homog, &op->bc);
+}
+/* homogeneousCFInterp (AMRPoissonOp.cpp:1598-1794) */
+static void op_homog_cf(const Op* op, LD* phi) {
+  for (int b = 0; b < op->L.n; b++)
+    for (int idir = 0; idir < 3; idir++)
+      for (int side = 0; side < 2; side++) {
+        const CFIVS* c = &op->cf[b*6+idir*2+side];
+        Fab* f = &phi->f[b];
+        int ihilo = side ? 1 : -1;
+        if (c->empty) continue;
+        if (c->packed) {
+          if (b_len(&f->b, idir) == 3) k_interphomolinear(f, &c->packedBox, op->dx, op->dxCrse, idir, ihilo);
+          else k_interphomo(f, &c->packedBox, op->dx, op->dxCrse, idir, ihilo);
+        } else {
+          const Box mb = c->ivs.mb;
+          if (b_len(&f->b, idir) == 3) {
+            double factor = 1-2*op->dx/(op->dx+op->dxCrse);
+            for (int k = mb.lo[2]; k <= mb.hi[2]; k++)
+              for (int j = mb.lo[1]; j <= mb.hi[1]; j++)
+                for (int i = mb.lo[0]; i <= mb.hi[0]; i++) {
+                  if (!ivs_has(&c->ivs, i, j, k)) continue;
+                  int a[3] = {i,j,k}; a[idir] -= ihilo;
+                  *P(f,i,j,k) = factor * *P(f,a[0],a[1],a[2]);
+                }
+          } else {
+            double c1 = 2*(op->dxCrse-op->dx)/(op->dxCrse+op->dx);
+            double c2 = -(op->dxCrse-op->dx)/(op->dxCrse+3*op->dx);
+            for (int k = mb.lo[2]; k <= mb.hi[2]; k++)
+              for (int j = mb.lo[1]; j <= mb.hi[1]; j++)
+                for (int i = mb.lo[0]; i <= mb.hi[0]; i++) {
+                  if (!ivs_has(&c->ivs, i, j, k)) continue;
+                  int a[3] = {i,j,k}, bb[3] = {i,j,k};
+                  a[idir] -= 2*ihilo; bb[idir] -= ihilo;
+                  double pa = *P(f,a[0],a[1],a[2]);
+                  double pb = *P(f,bb[0],bb[1],bb[2]);
+                  *P(f,i,j,k) = c1*pb + c2*pa;
+                }
+          }
+        }
+      }
+}
+/* residualI (AMRPoissonOp.cpp:260-297) */
+static void op_residualI(const Op* op, LD* lhs, LD* phi, const LD* rhs, int homog) {
+  op_exchange(op, phi);
+  op_bc(op, phi, homog);
+  #pragma omp parallel for schedule(static)
+  for (int b = 0; b < op->L.n; b++)
+    k_oplapres(&lhs->f[b], &phi->f[b], &rhs->f[b], &op->L.b[b], op->dx, op->alpha, op->beta);
+}
+static void op_residual(const Op* op, LD* lhs, LD* phi, const LD* rhs, int homog) {
+  if (homog) op_homog_cf(op, phi);
+  op_residualI(op, lhs, phi, rhs, homog);
+}
+/* applyOpI (AMRPoissonOp.cpp:343-385) */
+static void op_applyI(const Op* op, LD* lhs, LD* phi, int homog) {
+  op_exchange(op, phi);
+  op_bc(op, phi, homog);
+  #pragma omp parallel for schedule(static)
+  for (int b = 0; b < op->L.n; b++)
+    k_oplap(&lhs->f[b], &phi->f[b], &op->L.b[b], op->dx, op->alpha, op->beta);
+}
+static void op_apply(const Op* op, LD* lhs, LD* phi, int homog) {
+  op_homog_cf(op, phi);
+  op_applyI(op, lhs, phi, homog);
+}
+/* cell-relaxations performed (one red+black iteration of one cell), for timing */
+static long long g_cell_relax = 0;
+long long oracle_cell_relax_count(void) { return g_cell_relax; }
+void oracle_reset_counters(void) { g_cell_relax = 0; }
+
+/* levelGSRB (AMRPoissonOp.cpp:1216-1281) */
+static void op_levelGSRB(const Op* op, LD* phi, const LD* rhs) {
+  g_cell_relax += lay_cells(&op->L);
+  for (int pass = 0; pass <= 1; pass++) {
+    op_homog_cf(op, phi);
+    op_exchange(op, phi);
+    #pragma omp parallel for schedule(static)
+    for (int b = 0; b < op->L.n; b++) {
+      apply_bc(&phi->f[b], &op->L.b[b], &op->domain, op->dx, 1, &op->bc);
+      if (op->alpha == 0.0 && op->beta == 1.0)
+        k_gsrb_lap(&phi->f[b], &rhs->f[b], &op->L.b[b], op->dx, pass);
+      else
+        k_gsrb_helm(&phi->f[b], &rhs->f[b], &op->L.b[b], op->dx, op->alpha, op->beta, pass);
+    }
+  }
+}
+static void op_relax(const Op* op, LD* e, const LD* r, int n) {
+  for (int i = 0; i < n; i++) op_levelGSRB(op, e, r);
+}
+/* restrictResidual (AMRPoissonOp.cpp:651-703) */
+static void op_restrict(const Op* op, LD* resC, LD* phi, const LD* rhs) {
+  op_homog_cf(op, phi);
+  op_exchange(op, phi);
+  op_bc(op, phi, 1);
+  #pragma omp parallel for schedule(static)
+  for (int b = 0; b < op->L.n; b++) {
+    fab_setval(&resC->f[b], resC->f[b].b, 0.0);
+    k_restrictres(&resC->f[b], &phi->f[b], &rhs->f[b], op->alpha, op->beta, &op->L.b[b], op->dx);
+  }
+}
+/* prolongIncrement (AMRPoissonOp.cpp:706-733) */
+static void op_prolong(const Op* op, LD* phi, const LD* coarse) {
+  #pragma omp parallel for schedule(static)
+  for (int b = 0; b < op->L.n; b++) k_prolong(&phi->f[b], &coarse->f[b], &op->L.b[b], 2);
+}
+
+/* ---- LevelDataOps<FArrayBox> (LevelDataOps.H:93-294) ---- */
+static void ldo_setzero(LD* x) {
+  #pragma omp parallel for schedule(static)
+  for (int b = 0; b < x->L->n; b++) memset(x->f[b].d, 0, sizeof(double)*b_npts(&x->f[b].b));
+}
+/* incr: lhs += scale*x on valid boxes (BaseFab::plus) */
+static void ldo_incr(LD* lhs, const LD* x, double scale) {
+  #pragma omp parallel for schedule(static)
+  for (int b = 0; b < lhs->L->n; b++) {
+    const Box* r = &lhs->L->b[b];
+    for (int k = r->lo[2]; k <= r->hi[2]; k++)
+      for (int j = r->lo[1]; j <= r->hi[1]; j++)
+        for (int i = r->lo[0]; i <= r->hi[0]; i++) *P(&lhs->f[b],i,j,k) += *P(&x->f[b],i,j,k)*scale;
+  }
+}
+/* axby on whole FABs: copy x, mult a, plus(y, b) */
+static void ldo_axby(LD* lhs, const LD* x, const LD* y, double a, double bb) {
+  #pragma omp parallel for schedule(static)
+  for (int b = 0; b < lhs->L->n; b++) {
+    Fab* f = &lhs->f[b];
+    Box r = b_isect(f->b, &x->f[b].b);
+    fab_copy_region(f, &x->f[b], r);
+    long n = b_npts(&f->b);
+    for (long q = 0; q < n; q++) f->d[q] *= a;
+    Box ry = b_isect(f->b, &y->f[b].b);
+    for (int k = ry.lo[2]; k <= ry.hi[2]; k++)
+      for (int j = ry.lo[1]; j <= ry.hi[1]; j++)
+        for (int i = ry.lo[0]; i <= ry.hi[0]; i++) *P(f,i,j,k) += *P(&y->f[b],i,j,k)*bb;
+  }
+}
+static void ldo_scale(LD* x, double s) {
+  for (int b = 0; b < x->L->n; b++) { long n = b_npts(&x->f[b].b); for (long q = 0; q < n; q++) x->f[b].d[q] *= s; }
+}
+/* assignLocal: FArrayBox::copy over the intersection of the two FAB boxes */
+static void ldo_assign_local(LD* lhs, const LD* rhs) {
+  #pragma omp parallel for schedule(static)
+  for (int b = 0; b < lhs->L->n; b++) {
+    Box r = b_isect(lhs->f[b].b, &rhs->f[b].b);
+    fab_copy_region(&lhs->f[b], &rhs->f[b], r);
+  }
+}
+/* FArrayBox::dotProduct over the valid box (ForAllThisCBNNXC: serial Fortran order);
+   LevelDataOps::dotProduct sums boxes in layout order */
+static double ldo_dot(const LD* a, const LD* c) {
+  double val = 0.0;
+  for (int b = 0; b < a->L->n; b++) {
+    const Box* r = &a->L->b[b];
+    double dot = 0.0;
+    for (int k = r->lo[2]; k <= r->hi[2]; k++)
+      for (int j = r->lo[1]; j <= r->hi[1]; j++)
+        for (int i = r->lo[0]; i <= r->hi[0]; i++) dot += *P(&a->f[b],i,j,k) * *P(&c->f[b],i,j,k);
+    val += dot;
+  }
+  return val;
+}
+/* FArrayBox::norm(box, 0) : max |x| ; order independent */
+static double fab_maxnorm(const Fab* f, const Box* r) {
+  double m = 0.0;
+  for (int k = r->lo[2]; k <= r->hi[2]; k++)
+    for (int j = r->lo[1]; j <= r->hi[1]; j++)
+      for (int i = r->lo[0]; i <= r->hi[0]; i++) { double a = fabs(*P(f,i,j,k)); if (a > m) m = a; }
+  return m;
+}
+/* FArrayBox::sumPow(box, 2): per-column accumulation over rows, then over columns
+   (FArrayBox.cpp:165-230) */
+static double fab_sumpow2(const Fab* f, const Box* r) {
+  int nx = b_len(r, 0);
+  double* tmp = malloc(sizeof(double)*nx);
+  int first = 1;
+  for (int k = r->lo[2]; k <= r->hi[2]; k++)
+    for (int j = r->lo[1]; j <= r->hi[1]; j++) {
+      const double* row = P(f, r->lo[0], j, k);
+      if (first) { for (int i = 0; i < nx; i++) tmp[i] = row[i]*row[i]; first = 0; }
+      else for (int i = 0; i < nx; i++) tmp[i] += row[i]*row[i];
+    }
+  double sum = tmp[0];
+  for (int i = 1; i < nx; i++) sum += tmp[i];
+  free(tmp);
+  return sum;
+}
+/* norm(BoxLayoutData, interval, p) (BoxLayoutData.cpp:23-92), p = 0 or 2 */
+static double ldo_norm(const LD* x, int p) {
+  double norm = 0;
+  if (p == 0) {
+    for (int b = 0; b < x->L->n; b++) { double m = fab_maxnorm(&x->f[b], &x->L->b[b]); norm = norm > m ? norm : m; }
+    return norm;
+  }
+  for (int b = 0; b < x->L->n; b++) norm += fab_sumpow2(&x->f[b], &x->L->b[b]);
+  return sqrt(norm);
+}
+/* AMRPoissonOp::localMaxNorm */
+static double op_localMaxNorm(const LD* x) {
+  double m = 0;
+  #pragma omp parallel for reduction(max:m) schedule(static)
+  for (int b = 0; b < x->L->n; b++) { double v = fab_maxnorm(&x->f[b], &x->L->b[b]); if (v > m) m = v; }
+  return m;
+}
+/* preCond (AMRPoissonOp.cpp:304-329) */
+static void op_precond(const Op* op, LD* phi, const LD* rhs) {
+  double mult = 1.0/(op->alpha - 2.0*3*op->beta/(op->dx*op->dx));
+  for (int b = 0; b < op->L.n; b++) {
+    Box r = b_isect(phi->f[b].b, &rhs->f[b].b);
+    fab_copy_region(&phi->f[b], &rhs->f[b], r);
+    long n = b_npts(&phi->f[b].b);
+    for (long q = 0; q < n; q++) phi->f[b].d[q] *= mult;
+  }
+  op_relax(op, phi, rhs, 2);
+}
+
+/* reflux (AMRPoissonOp.cpp:1117-1201) + NEWGETFLUX + LevelFluxRegister */
+static void op_reflux(Op* cop, Op* fop, LD* phiFine, LD* phi, LD* res) {
+  FluxReg* r = &cop->fr;
+  ldo_setzero(&r->coarFlux);
+  ldo_setzero(&r->fineFlux);
+  /* incrementCoarse for every coarse box, direction, side */
+  for (int c = 0; c < cop->L.n; c++) {
+    const Fab* cf = &phi->f[c];
+    for (int idir = 0; idir < 3; idir++) {
+      double scale_flux = cop->beta*1/cop->dx;            /* getFlux(..., a_ref = 1) */
+      for (int side = 0; side < 2; side++) {
+        const BoxList* bl = &r->loc[c*6+idir*2+side];
+        double sc = side ? -1.0 : 1.0;                     /* -sign(side)*1 */
+        for (int q = 0; q < bl->n; q++) {
+          const Box* bx = &bl->b[q];
+          for (int k = bx->lo[2]; k <= bx->hi[2]; k++)
+            for (int j = bx->lo[1]; j <= bx->hi[1]; j++)
+              for (int i = bx->lo[0]; i <= bx->hi[0]; i++) {
+                int n0[3] = {i,j,k};
+                if (side == 0) n0[idir] += 1;                /* shiftHalf(-1): node n -> cell n-1 */
+                int m1[3] = {n0[0],n0[1],n0[2]}; m1[idir] -= 1;
+                double F = (*P(cf,n0[0],n0[1],n0[2]) - *P(cf,m1[0],m1[1],m1[2]))*scale_flux;
+                double* reg = P(&r->coarFlux.f[c],i,j,k);
+                *reg += F*sc;
+              }
+        }
+      }
+    }
+  }
+  /* CF ghosts of the finer level from this level */
+  qcfi_interp(&fop->qcfi, phiFine, phi);
+  /* incrementFine for every fine box, direction, side */
+  for (int f = 0; f < phiFine->L->n; f++) {
+    const Box* gb = &phiFine->L->b[f];
+    const Fab* pf = &phiFine->f[f];
+    for (int idir = 0; idir < 3; idir++) {
+      double scale_flux = cop->beta*cop->refToFiner/cop->dx;
+      for (int side = 0; side < 2; side++) {
+        double scale = (side ? 1.0 : -1.0)*1.0/4.0;
+        Box clip = b_refine(r->cfine.b[f], r->nRefine);
+        Box fineBox = b_adj(clip, idir, side, 1);  /* cells outside the box on this side */
+        int iv[3] = {fineBox.lo[0], fineBox.lo[1], fineBox.lo[2]};
+        int civ[3] = {fdiv(iv[0],2), fdiv(iv[1],2), fdiv(iv[2],2)};
+        for (int k = fineBox.lo[2]; k <= fineBox.hi[2]; k++)
+          for (int j = fineBox.lo[1]; j <= fineBox.hi[1]; j++)
+            for (int i = fineBox.lo[0]; i <= fineBox.hi[0]; i++) {
+              int n0[3] = {i,j,k};
+              if (side == 0) n0[idir] += 1;
+              int m1[3] = {n0[0],n0[1],n0[2]}; m1[idir] -= 1;
+              double F = (*P(pf,n0[0],n0[1],n0[2]) - *P(pf,m1[0],m1[1],m1[2]))*scale_flux;
+              int ii = civ[0] + (i-iv[0])/2, jj = civ[1] + (j-iv[1])/2, kk = civ[2] + (k-iv[2])/2;
+              double* cfin = P(&r->fineFlux.f[f], ii, jj, kk);
+              *cfin = *cfin + scale*F;
+            }
+        (void)gb;
+      }
+    }
+  }
+  /* LevelFluxRegister::reflux(res, 1/dx) */
+  double a_scale = 1.0/cop->dx;
+  for (int c = 0; c < cop->L.n; c++) {
+    Fab* u = &res->f[c];
+    const Fab* reg = &r->coarFlux.f[c];
+    Box bx = b_isect(u->b, &reg->b);
+    for (int k = bx.lo[2]; k <= bx.hi[2]; k++)
+      for (int j = bx.lo[1]; j <= bx.hi[1]; j++)
+        for (int i = bx.lo[0]; i <= bx.hi[0]; i++) *P(u,i,j,k) += *P(reg,i,j,k)*(-a_scale);
+  }
+  for (int q = 0; q < r->rev.n; q++) {
+    const Motion* m = &r->rev.m[q];
+    Fab* u = &res->f[m->to];
+    const Fab* src = &r->fineFlux.f[m->from];
+    const Box* bx = &m->region;
+    for (int k = bx->lo[2]; k <= bx->hi[2]; k++)
+      for (int j = bx->lo[1]; j <= bx->hi[1]; j++)
+        for (int i = bx->lo[0]; i <= bx->hi[0]; i++) *P(u,i,j,k) += *P(src,i,j,k)*(-a_scale);
+  }
+}
+
+/* ======================================================================= */
+/* Hierarchy: AMRPoissonOpFactory + AMRMultiGrid + MultiGrid + BiCGStab    */
+/* ======================================================================= */
+struct OHier {
+  int nlev;
+  Layout* grids;      /* per AMR level */
+  Box* domains;
+  double* dx;
+  double alpha, beta;
+  BCSpec bc;
+  Op* op;             /* AMR level ops */
+  /* MultiGrid of AMR level 0 */
+  int mgD;            /* depth count (m_depth) */
+  Layout* mgL;        /* mgL[0] = grids[0] */
+  Op* mgop;           /* mgop[0] aliases op[0] */
+  long bottomCells;
+  /* AMRMultiGrid copiers */
+  Layout* cfL;        /* coarsened fine layouts (resC) per level >= 1 */
+  Plan* resCopier;    /* resC[l] -> grids[l-1] */
+  Plan* revCopier;    /* grids[l-1] -> resC[l] */
+  /* statistics */
+  int bottom_calls;
+};
+
+OHier* oracle_create(const int* is, const double* rs) {
+  OHier* h = calloc(1, sizeof(OHier));
+  int nlev = is[0];
+  if (nlev < 1 || nlev > 8) { set_err("bad nlev"); free(h); return NULL; }
+  h->nlev = nlev;
+  h->grids = calloc(nlev, sizeof(Layout));
+  h->domains = calloc(nlev, sizeof(Box));
+  h->dx = calloc(nlev, sizeof(double));
+  h->domains[0] = mkbox(is[1], is[2], is[3], is[4], is[5], is[6]);
+  for (int d = 0; d < 3; d++) { h->bc.type[0][d] = is[7+d]; h->bc.type[1][d] = is[10+d]; }
+  h->dx[0] = rs[0]; h->alpha = rs[1]; h->beta = rs[2];
+  for (int d = 0; d < 3; d++) { h->bc.val[0][d] = rs[3+d]; h->bc.val[1][d] = rs[6+d]; }
+  const int* bp = is + 13 + nlev;
+  for (int l = 0; l < nlev; l++) {
+    int n = is[13+l];
+    h->grids[l].n = n; h->grids[l].b = malloc(sizeof(Box)*(n ? n : 1));
+    for (int q = 0; q < n; q++) { h->grids[l].b[q] = mkbox(bp[0],bp[1],bp[2],bp[3],bp[4],bp[5]); bp += 6; }
+    for (int q = 1; q < n; q++)
+      if (!b_lt(&h->grids[l].b[q-1], &h->grids[l].b[q])) { set_err("boxes not sorted"); return NULL; }
+    if (l > 0) { h->domains[l] = b_refine(h->domains[l-1], 2); h->dx[l] = h->dx[l-1]/2; }
+  }
+  /* AMRPoissonOpFactory::AMRnewOp */
+  h->op = calloc(nlev, sizeof(Op));
+  for (int l = 0; l < nlev; l++) {
+    Op* op = &h->op[l];
+    op_define(op, &h->grids[l], &h->domains[l], h->dx[l], h->alpha, h->beta, &h->bc);
+    op->dxCrse = (l > 0) ? h->dx[l-1] : -1.0;
+    if (l > 0) {
+      op->hasCoarser = 1; op->refToCoarser = 2;
+      qcfi_define(&op->qcfi, &h->grids[l], &h->grids[l-1], h->dx[l], 2, &h->domains[l]);
+    }
+    if (l < nlev-1) {
+      op->hasFiner = 1; op->refToFiner = 2;
+      fr_define(&op->fr, &h->grids[l+1], &h->grids[l], 2);
+    }
+  }
+  /* MultiGrid::define on level 0 via MGnewOp (AMRPoissonOp.cpp:1923-1989) */
+  int D = 1;
+  while (1) {
+    int coarsening = 1 << D;
+    if (!lay_coarsenable(&h->grids[0], coarsening*2)) break;
+    D++;
+  }
+  h->mgD = D;
+  h->mgL = calloc(D, sizeof(Layout));
+  h->mgop = calloc(D, sizeof(Op));
+  h->mgL[0] = h->grids[0];
+  h->bottomCells = b_npts(&h->domains[0]);
+  for (int d = 1; d < D; d++) {
+    h->mgL[d] = lay_coarsen(&h->grids[0], 1 << d);
+    Box dom = b_coarsen(h->domains[0], 1 << d);
+    op_define(&h->mgop[d], &h->mgL[d], &dom, h->dx[0]*(1 << d), h->alpha, h->beta, &h->bc);
+    h->mgop[d].dxCrse = -1.0;
+    h->bottomCells /= 8;
+  }
+  /* AMRMultiGrid::init copiers */
+  h->cfL = calloc(nlev, sizeof(Layout));
+  h->resCopier = calloc(nlev, sizeof(Plan));
+  h->revCopier = calloc(nlev, sizeof(Plan));
+  for (int l = 1; l < nlev; l++) {
+    h->cfL[l] = lay_coarsen(&h->grids[l], 2);
+    h->resCopier[l] = plan_copy(&h->cfL[l], &h->grids[l-1], 0);
+    h->revCopier[l] = h->resCopier[l];
+    h->revCopier[l].m = malloc(sizeof(Motion)*(h->resCopier[l].n ? h->resCopier[l].n : 1));
+    memcpy(h->revCopier[l].m, h->resCopier[l].m, sizeof(Motion)*h->resCopier[l].n);
+    h->revCopier[l].cap = h->revCopier[l].n;
+    plan_reverse(&h->revCopier[l]);
+  }
+  return h;
+}
+
+void oracle_destroy(OHier* h) {
+  if (!h) return;
+  for (int l = 0; l < h->nlev; l++) op_free(&h->op[l]);
+  for (int d = 1; d < h->mgD; d++) { op_free(&h->mgop[d]); free(h->mgL[d].b); }
+  for (int l = 1; l < h->nlev; l++) { plan_free(&h->resCopier[l]); plan_free(&h->revCopier[l]); free(h->cfL[l].b); }
+  for (int l = 0; l < h->nlev; l++) free(h->grids[l].b);
+  free(h->grids); free(h->domains); free(h->dx); free(h->op); free(h->mgL); free(h->mgop);
+  free(h->cfL); free(h->resCopier); free(h->revCopier); free(h);
+}
+long oracle_level_cells(const OHier* h, int lev) { return lay_cells(&h->grids[lev]); }
+int oracle_num_mg_depths(const OHier* h) { return h->mgD; }
+long oracle_mg_cells(const OHier* h, int d) { return lay_cells(&h->mgL[d]); }
+long oracle_ghosted_cells(const OHier* h, int lev) {
+  long n = 0; for (int q = 0; q < h->grids[lev].n; q++) { Box g = b_grow(h->grids[lev].b[q], 1); n += b_npts(&g); }
+  return n;
+}
+static const Op* mg_op(const OHier* h, int d) { return d == 0 ? &h->op[0] : &h->mgop[d]; }
+
+/* ---- BiCGStabSolver<T>::solve (BiCGStabSolver.H:199-519) ---- */
+typedef struct { double eps, reps, hang, small, metric; int imax, numRestarts, normType; } BiCG;
+
+static int bicgstab_solve(const Op* op, LD* phi, const LD* rhs, const BiCG* p) {
+  LD r = ld_make(rhs->L, rhs->g), r_tilde = ld_make(rhs->L, rhs->g), e = ld_make(phi->L, phi->g),
+     pp = ld_make(rhs->L, rhs->g), p_tilde = ld_make(phi->L, phi->g), s_tilde = ld_make(phi->L, phi->g),
+     t = ld_make(rhs->L, rhs->g), v = ld_make(rhs->L, rhs->g);
+  int recount = 0, status = -1;
+  ldo_setzero(&r);
+  op_residual(op, &r, phi, rhs, 1);
+  ldo_assign_local(&r_tilde, &r);
+  ldo_setzero(&e); ldo_setzero(&p_tilde); ldo_setzero(&s_tilde);
+  int iter = 0;
+  double rho[4] = {0,0,0,0}, norm[2];
+  norm[0] = ldo_norm(&r, p->normType);
+  double initial_norm = norm[0], initial_residual = norm[0];
+  norm[1] = norm[0];
+  double alpha[2] = {0,0}, beta[2] = {0,0}, omega[2] = {0,0};
+  int init = 1, restarts = 0;
+  if (p->metric > 0) initial_norm = p->metric;
+  while ((iter < p->imax && norm[0] > p->eps*norm[1]) && (norm[1] > 0)) {
+    iter++;
+    norm[1] = norm[0]; alpha[1] = alpha[0]; beta[1] = beta[0]; omega[1] = omega[0];
+    rho[3] = rho[2]; rho[2] = rho[1];
+    rho[1] = ldo_dot(&r_tilde, &r);
+    if (rho[1] == 0.0) { ldo_incr(phi, &e, 1.0); status = 2; goto done; }
+    if (init) { ldo_assign_local(&pp, &r); init = 0; }
+    else {
+      beta[1] = (rho[1]/rho[2])*(alpha[1]/omega[1]);
+      ldo_scale(&pp, beta[1]);
+      ldo_incr(&pp, &v, -beta[1]*omega[1]);
+      ldo_incr(&pp, &r, 1.0);
+    }
+    op_precond(op, &p_tilde, &pp);
+    ldo_setzero(&v);
+    op_apply(op, &v, &p_tilde, 1);
+    double m = ldo_dot(&r_tilde, &v);
+    alpha[0] = rho[1]/m;
+    if (fabs(m) > p->small*fabs(rho[1])) {
+      ldo_incr(&r, &v, -alpha[0]);
+      norm[0] = ldo_norm(&r, p->normType);
+      ldo_incr(&e, &p_tilde, alpha[0]);
+    } else { ldo_setzero(&r); norm[0] = 0.0; }
+    if (norm[0] > p->eps*initial_norm && norm[0] > p->reps*initial_residual) {
+      op_precond(op, &s_tilde, &r);
+      ldo_setzero(&t);
+      op_apply(op, &t, &s_tilde, 1);
+      omega[0] = ldo_dot(&t, &r)/ldo_dot(&t, &t);
+      ldo_incr(&e, &s_tilde, omega[0]);
+      ldo_incr(&r, &t, -omega[0]);
+      norm[0] = ldo_norm(&r, p->normType);
+    }
+    if (norm[0] <= p->eps*initial_norm || norm[0] <= p->reps*initial_residual) { status = 1; break; }
+    if (omega[0] == 0.0 || norm[0] > (1-p->hang)*norm[1]) {
+      if (recount == 0) recount = 1;
+      else {
+        recount = 0;
+        ldo_incr(phi, &e, 1.0);
+        if (restarts == p->numRestarts) { status = 3; goto done; }
+        op_residual(op, &r, phi, rhs, 1);
+        norm[0] = ldo_norm(&r, p->normType);
+        rho[1] = 0.0; rho[2] = 0.0; rho[3] = 0.0;
+        alpha[0] = 0; beta[0] = 0; omega[0] = 0;
+        ldo_assign_local(&r_tilde, &r);
+        ldo_setzero(&e);
+        restarts++;
+        init = 1;
+      }
+    }
+  }
+  ldo_incr(phi, &e, 1.0);
+done:
+  ld_free(&r); ld_free(&r_tilde); ld_free(&e); ld_free(&pp); ld_free(&p_tilde);
+  ld_free(&s_tilde); ld_free(&t); ld_free(&v);
+  return status;
+}
+
+/* ---- MultiGrid<T>::cycle (MultiGrid.H:630-696), V/W cycle ---- */
+typedef struct {
+  OHier* h; int pre, post, bottom, cycles;
+  LD* mgres; LD* mgcor;           /* per depth >= 1 */
+  BiCG bic;
+} MGState;
+
+static void mg_cycle(MGState* s, int depth, LD* corr, const LD* res) {
+  OHier* h = s->h;
+  const Op* op = mg_op(h, depth);
+  if (depth == h->mgD-1) {
+    if (h->bottomCells == 1) op_relax(op, corr, res, 1);
+    else {
+      op_relax(op, corr, res, s->bottom);
+      bicgstab_solve(op, corr, res, &s->bic);
+      h->bottom_calls++;
+    }
+    return;
+  }
+  op_relax(op, corr, res, s->pre);
+  op_restrict(op, &s->mgres[depth+1], corr, res);
+  ldo_setzero(&s->mgcor[depth+1]);
+  for (int c = 0; c < s->cycles; c++) mg_cycle(s, depth+1, &s->mgcor[depth+1], &s->mgres[depth+1]);
+  op_prolong(op, corr, &s->mgcor[depth+1]);
+  op_relax(op, corr, res, s->post);
+}
+
+/* ---- AMRMultiGrid ---- */
+typedef struct {
+  OHier* h; MGState mg;
+  LD *corr, *resid, *resC;        /* m_correction, m_residual, m_resC */
+} AMRState;
+
+/* computeAMRResidualLevel (AMRMultiGrid.H:837-892) */
+static void amr_residual_level(AMRState* a, LD* res, LD* phi, const LD* rhs, int lmax, int lbase,
+                               int ilev, int homog) {
+  OHier* h = a->h;
+  if (lmax != lbase) {
+    if (ilev == lmax) {
+      Op* op = &h->op[lmax];               /* AMRResidualNF */
+      qcfi_interp(&op->qcfi, &phi[lmax], &phi[lmax-1]);
+      op_residualI(op, &res[lmax], &phi[lmax], &rhs[lmax], homog);
+    } else {
+      Op* op = &h->op[ilev];                /* AMRResidual / AMRResidualNC */
+      if (ilev > 0) qcfi_interp(&op->qcfi, &phi[ilev], &phi[ilev-1]);
+      op_applyI(op, &res[ilev], &phi[ilev], homog);
+      op_reflux(op, &h->op[ilev+1], &phi[ilev+1], &phi[ilev], &res[ilev]);
+      ldo_axby(&res[ilev], &res[ilev], &rhs[ilev], -1.0, 1.0);
+    }
+  } else {
+    op_residual(&h->op[0], &res[0], &phi[0], &rhs[0], homog);
+  }
+}
+/* zeroCovered -> LevelDataOps::copyToZero over the copier's toRegions */
+static void zero_covered(OHier* h, LD* res, int lev) {
+  const Plan* p = &h->resCopier[lev+1];
+  for (int q = 0; q < p->n; q++) fab_setval(&res->f[p->m[q].to], p->m[q].region, 0.0);
+}
+/* computeAMRResidual (AMRMultiGrid.H:683-736) */
+static double amr_residual(AMRState* a, LD* res, LD* phi, const LD* rhs, int lmax, int lbase, int homog) {
+  double rnorm = 0;
+  for (int ilev = lbase; ilev <= lmax; ilev++) {
+    amr_residual_level(a, res, phi, rhs, lmax, lbase, ilev, homog);
+    double local;
+    if (ilev == lmax) local = op_localMaxNorm(&res[ilev]);
+    else { zero_covered(a->h, &res[ilev], ilev); local = op_localMaxNorm(&res[ilev]); }
+    rnorm = local > rnorm ? local : rnorm;
+  }
+  return rnorm;
+}
+/* AMRRestrictS (AMRPoissonOp.cpp:877-919) */
+static void amr_restrictS(AMRState* a, int l, LD* resC, const LD* residual, LD* correction,
+                          const LD* coarseCorrection, LD* scratch) {
+  OHier* h = a->h; Op* op = &h->op[l];
+  qcfi_interp(&op->qcfi, correction, coarseCorrection);      /* AMRResidualNF */
+  op_residualI(op, scratch, correction, residual, 1);
+  for (int b = 0; b < op->L.n; b++) k_average2(&resC->f[b], &scratch->f[b], &h->cfL[l].b[b]);
+}
+/* AMRProlongS (AMRPoissonOp.cpp:959-990) */
+static void amr_prolongS(AMRState* a, int l, LD* correction, const LD* coarseCorrection, LD* temp) {
+  OHier* h = a->h; Op* op = &h->op[l];
+  exec_copy(temp, coarseCorrection, &h->revCopier[l]);
+  for (int b = 0; b < op->L.n; b++) k_prolong(&correction->f[b], &temp->f[b], &op->L.b[b], 2);
+}
+/* AMRVCycle (AMRMultiGrid.H:1342-1419) */
+static void amr_vcycle(AMRState* a, LD* uberCorr, LD* uberRes, int ilev, int lmax, int lbase) {
+  OHier* h = a->h;
+  if (ilev == lmax)
+    for (int level = lbase; level <= lmax; level++) {
+      ldo_assign_local(&a->resid[level], &uberRes[level]);
+      ldo_setzero(&a->corr[level]);
+    }
+  if (lmax == lbase) { mg_cycle(&a->mg, 0, &uberCorr[ilev], &uberRes[ilev]); return; }
+  if (ilev == lbase) {
+    mg_cycle(&a->mg, 0, &a->corr[ilev], &a->resid[ilev]);
+    ldo_incr(&uberCorr[ilev], &a->corr[ilev], 1.0);
+    return;
+  }
+  Op* op = &h->op[ilev];
+  op_relax(op, &a->corr[ilev], &a->resid[ilev], a->mg.pre);
+  ldo_incr(&uberCorr[ilev], &a->corr[ilev], 1.0);
+  ldo_setzero(&a->corr[ilev-1]);
+  amr_residual_level(a, a->resid, uberCorr, uberRes, lmax, lbase, ilev-1, 1);
+  amr_restrictS(a, ilev, &a->resC[ilev], &a->resid[ilev], &a->corr[ilev], &a->corr[ilev-1], &uberCorr[ilev]);
+  exec_copy(&a->resid[ilev-1], &a->resC[ilev], &h->resCopier[ilev]);
+  for (int img = 0; img < a->mg.cycles; img++) amr_vcycle(a, uberCorr, uberRes, ilev-1, lmax, lbase);
+  amr_prolongS(a, ilev, &a->corr[ilev], &a->corr[ilev-1], &a->resC[ilev]);
+  /* AMRUpdateResidual: AMRResidualNF(res, corr, corrC, res, true) in place */
+  qcfi_interp(&op->qcfi, &a->corr[ilev], &a->corr[ilev-1]);
+  op_residualI(op, &a->resid[ilev], &a->corr[ilev], &a->resid[ilev], 1);
+  LD* dCorr = &uberCorr[ilev];
+  ldo_setzero(dCorr);
+  op_relax(op, dCorr, &a->resid[ilev], a->mg.post);
+  ldo_incr(&a->corr[ilev], dCorr, 1.0);
+  ldo_assign_local(&uberCorr[ilev], &a->corr[ilev]);
+}
+
+static void amr_state_make(AMRState* a, OHier* h, int pre, int post, int bottom, int numMG) {
+  memset(a, 0, sizeof *a);
+  a->h = h;
+  a->corr = calloc(h->nlev, sizeof(LD)); a->resid = calloc(h->nlev, sizeof(LD)); a->resC = calloc(h->nlev, sizeof(LD));
+  for (int l = 0; l < h->nlev; l++) {
+    a->corr[l] = ld_make(&h->grids[l], 1);
+    a->resid[l] = ld_make(&h->grids[l], 0);
+    if (l > 0) a->resC[l] = ld_make(&h->cfL[l], 0);
+  }
+  MGState* m = &a->mg;
+  m->h = h; m->pre = pre; m->post = post; m->bottom = bottom; m->cycles = numMG;
+  m->mgres = calloc(h->mgD, sizeof(LD)); m->mgcor = calloc(h->mgD, sizeof(LD));
+  for (int d = 1; d < h->mgD; d++) { m->mgres[d] = ld_make(&h->mgL[d], 0); m->mgcor[d] = ld_make(&h->mgL[d], 1); }
+  m->bic.eps = 1e-6; m->bic.reps = 1e-12; m->bic.hang = 1e-8; m->bic.small = 1e-30;
+  m->bic.metric = -1.0; m->bic.imax = 80; m->bic.numRestarts = 5; m->bic.normType = 2;
+}
+static void amr_state_free(AMRState* a) {
+  OHier* h = a->h;
+  for (int l = 0; l < h->nlev; l++) { ld_free(&a->corr[l]); ld_free(&a->resid[l]); if (l > 0) ld_free(&a->resC[l]); }
+  for (int d = 1; d < h->mgD; d++) { ld_free(&a->mg.mgres[d]); ld_free(&a->mg.mgcor[d]); }
+  free(a->corr); free(a->resid); free(a->resC); free(a->mg.mgres); free(a->mg.mgcor);
+}
+
+static LD* levels_make(OHier* h, int g) {
+  LD* x = calloc(h->nlev, sizeof(LD));
+  for (int l = 0; l < h->nlev; l++) x[l] = ld_make(&h->grids[l], g);
+  return x;
+}
+static void levels_free(OHier* h, LD* x) { for (int l = 0; l < h->nlev; l++) ld_free(&x[l]); free(x); }
+static void levels_from_packed(OHier* h, LD* x, const double* v) {
+  long o = 0; for (int l = 0; l < h->nlev; l++) { ld_from_packed(&x[l], v + o); o += lay_cells(&h->grids[l]); }
+}
+static void levels_to_packed(OHier* h, const LD* x, double* v) {
+  long o = 0; for (int l = 0; l < h->nlev; l++) { ld_to_packed(&x[l], v + o); o += lay_cells(&h->grids[l]); }
+}
+
+/* AMRMultiGrid::solve -> solveNoInit -> solveNoInitResid (AMRMultiGrid.H:895-1124) */
+int oracle_solve(OHier* h, const int* sp, const double* rp, const double* rhs_in, double* phi_io,
+                 int zero_phi, double* hist, int hist_cap, int* info) {
+  int pre = sp[0], post = sp[1], bottom = sp[2], numMG = sp[3], iterMax = sp[4], imin = sp[5];
+  double eps = rp[0], hang = rp[1], normThresh = rp[2];
+  int lmax = h->nlev-1, lbase = 0;
+  AMRState a; amr_state_make(&a, h, pre, post, bottom, numMG);
+  LD* phi = levels_make(h, 1);
+  LD* rhs = levels_make(h, 0);
+  LD* uberRes = levels_make(h, 0);
+  LD* uberCorr = levels_make(h, 1);
+  levels_from_packed(h, rhs, rhs_in);
+  levels_from_packed(h, phi, phi_io);
+  if (zero_phi) for (int l = 0; l <= lmax; l++) ldo_setzero(&phi[l]);
+  h->bottom_calls = 0;
+  double initial = amr_residual(&a, uberRes, phi, rhs, lmax, lbase, 0);
+  double rnorm = initial, norm_last = 2*initial;
+  a.mg.bic.metric = initial; a.mg.bic.eps = 1.0*eps;     /* setConvergenceMetrics */
+  int iter = 0;
+  if (hist && hist_cap > 0) hist[0] = rnorm;
+  int goNorm = rnorm > normThresh, goRedu = rnorm > eps*initial, goIter = iter < iterMax,
+      goHang = iter < imin || rnorm < (1-hang)*norm_last;
+  while (goIter && goRedu && goHang && goNorm) {
+    norm_last = rnorm;
+    amr_vcycle(&a, uberCorr, uberRes, lmax, lmax, lbase);
+    for (int l = lbase; l <= lmax; l++) { ldo_incr(&phi[l], &uberCorr[l], 1.0); ldo_setzero(&uberCorr[l]); }
+    rnorm = amr_residual(&a, uberRes, phi, rhs, lmax, lbase, 0);
+    iter++;
+    if (hist && iter < hist_cap) hist[iter] = rnorm;
+    goNorm = rnorm > normThresh; goRedu = rnorm > eps*initial; goIter = iter < iterMax;
+    goHang = iter < imin || rnorm < (1-hang)*norm_last;
+  }
+  int exitStatus = (!goRedu) + (!goIter)*2 + (!goHang)*4 + (!goNorm)*8;
+  levels_to_packed(h, phi, phi_io);
+  if (info) { info[0] = iter; info[1] = exitStatus; info[2] = h->bottom_calls; }
+  levels_free(h, phi); levels_free(h, rhs); levels_free(h, uberRes); levels_free(h, uberCorr);
+  amr_state_free(&a);
+  return 0;
+}
+
+double oracle_amr_residual(OHier* h, const double* phi_in, const double* rhs_in, double* res_out, int homog) {
+  AMRState a; amr_state_make(&a, h, 2, 2, 2, 1);
+  LD* phi = levels_make(h, 1); LD* rhs = levels_make(h, 0); LD* res = levels_make(h, 0);
+  levels_from_packed(h, phi, phi_in); levels_from_packed(h, rhs, rhs_in);
+  double n = amr_residual(&a, res, phi, rhs, h->nlev-1, 0, homog);
+  levels_to_packed(h, res, res_out);
+  levels_free(h, phi); levels_free(h, rhs); levels_free(h, res); amr_state_free(&a);
+  return n;
+}
+
+/* computeAMROperator (AMRMultiGrid.H:790-833) */
+int oracle_amr_operator(OHier* h, const double* phi_in, double* out, int homog) {
+  LD* phi = levels_make(h, 1); LD* lph = levels_make(h, 0);
+  levels_from_packed(h, phi, phi_in);
+  int lmax = h->nlev-1, lbase = 0;
+  for (int ilev = lbase; ilev <= lmax; ilev++) {
+    if (lmax != lbase) {
+      Op* op = &h->op[ilev];
+      if (ilev == lmax) {
+        qcfi_interp(&op->qcfi, &phi[lmax], &phi[lmax-1]);
+        op_applyI(op, &lph[lmax], &phi[lmax], homog);
+      } else {
+        if (ilev > 0) qcfi_interp(&op->qcfi, &phi[ilev], &phi[ilev-1]);
+        op_applyI(op, &lph[ilev], &phi[ilev], homog);
+        op_reflux(op, &h->op[ilev+1], &phi[ilev+1], &phi[ilev], &lph[ilev]);
+      }
+    } else op_apply(&h->op[0], &lph[0], &phi[0], homog);
+  }
+  levels_to_packed(h, lph, out);
+  levels_free(h, phi); levels_free(h, lph);
+  return 0;
+}
+
+/* ---- single-level entry points on the MG levels of AMR level 0 ---- */
+int oracle_level_apply(OHier* h, int d, const double* phi_in, double* out, int homog) {
+  if (d < 0 || d >= h->mgD) { set_err("bad depth"); return -1; }
+  const Op* op = mg_op(h, d);
+  LD phi = ld_make(&h->mgL[d], 1), l = ld_make(&h->mgL[d], 0);
+  ld_from_packed(&phi, phi_in);
+  op_apply(op, &l, &phi, homog);
+  ld_to_packed(&l, out);
+  ld_free(&phi); ld_free(&l);
+  return 0;
+}
+int oracle_level_residual(OHier* h, int d, const double* phi_in, const double* rhs_in, double* out, int homog) {
+  if (d < 0 || d >= h->mgD) { set_err("bad depth"); return -1; }
+  const Op* op = mg_op(h, d);
+  LD phi = ld_make(&h->mgL[d], 1), rhs = ld_make(&h->mgL[d], 0), r = ld_make(&h->mgL[d], 0);
+  ld_from_packed(&phi, phi_in); ld_from_packed(&rhs, rhs_in);
+  op_residual(op, &r, &phi, &rhs, homog);
+  ld_to_packed(&r, out);
+  ld_free(&phi); ld_free(&rhs); ld_free(&r);
+  return 0;
+}
+int oracle_level_relax(OHier* h, int d, double* phi_io, const double* rhs_in, int iters) {
+  if (d < 0 || d >= h->mgD) { set_err("bad depth"); return -1; }
+  const Op* op = mg_op(h, d);
+  LD phi = ld_make(&h->mgL[d], 1), rhs = ld_make(&h->mgL[d], 0);
+  ld_from_packed(&phi, phi_io); ld_from_packed(&rhs, rhs_in);
+  op_relax(op, &phi, &rhs, iters);
+  ld_to_packed(&phi, phi_io);
+  ld_free(&phi); ld_free(&rhs);
+  return 0;
+}
+int oracle_level_restrict(OHier* h, int d, const double* phi_in, const double* rhs_in, double* resc) {
+  if (d < 0 || d+1 >= h->mgD) { set_err("bad depth"); return -1; }
+  const Op* op = mg_op(h, d);
+  LD phi = ld_make(&h->mgL[d], 1), rhs = ld_make(&h->mgL[d], 0), rc = ld_make(&h->mgL[d+1], 0);
+  ld_from_packed(&phi, phi_in); ld_from_packed(&rhs, rhs_in);
+  op_restrict(op, &rc, &phi, &rhs);
+  ld_to_packed(&rc, resc);
+  ld_free(&phi); ld_free(&rhs); ld_free(&rc);
+  return 0;
+}
+int oracle_level_prolong(OHier* h, int d, double* phi_io, const double* ec) {
+  if (d < 0 || d+1 >= h->mgD) { set_err("bad depth"); return -1; }
+  const Op* op = mg_op(h, d);
+  LD phi = ld_make(&h->mgL[d], 1), c = ld_make(&h->mgL[d+1], 1);
+  ld_from_packed(&phi, phi_io); ld_from_packed(&c, ec);
+  op_prolong(op, &phi, &c);
+  ld_to_packed(&phi, phi_io);
+  ld_free(&phi); ld_free(&c);
+  return 0;
+}
+int oracle_kernel_gsrb(OHier* h, int lev, double* phi_g, const double* rhs_in, int color) {
+  const Op* op = &h->op[lev];
+  LD phi = ld_make(&h->grids[lev], 1), rhs = ld_make(&h->grids[lev], 0);
+  ld_from_ghosted(&phi, phi_g); ld_from_packed(&rhs, rhs_in);
+  for (int b = 0; b < op->L.n; b++) {
+    if (op->alpha == 0.0 && op->beta == 1.0) k_gsrb_lap(&phi.f[b], &rhs.f[b], &op->L.b[b], op->dx, color);
+    else k_gsrb_helm(&phi.f[b], &rhs.f[b], &op->L.b[b], op->dx, op->alpha, op->beta, color);
+  }
+  ld_to_ghosted(&phi, phi_g);
+  ld_free(&phi); ld_free(&rhs);
+  return 0;
+}
+int oracle_homog_cf(OHier* h, int lev, double* phi_g) {
+  LD phi = ld_make(&h->grids[lev], 1);
+  ld_from_ghosted(&phi, phi_g);
+  op_homog_cf(&h->op[lev], &phi);
+  ld_to_ghosted(&phi, phi_g);
+  ld_free(&phi);
+  return 0;
+}
+int oracle_quad_cf(OHier* h, int lev, double* phi_g, const double* coarse) {
+  if (lev < 1 || lev >= h->nlev) { set_err("bad level"); return -1; }
+  LD phi = ld_make(&h->grids[lev], 1), c = ld_make(&h->grids[lev-1], 1);
+  ld_from_ghosted(&phi, phi_g); ld_from_packed(&c, coarse);
+  qcfi_interp(&h->op[lev].qcfi, &phi, &c);
+  ld_to_ghosted(&phi, phi_g);
+  ld_free(&phi); ld_free(&c);
+  return 0;
+}
+int oracle_average(OHier* h, int lev, const double* fine, double* out) {
+  if (lev < 1 || lev >= h->nlev) { set_err("bad level"); return -1; }
+  LD f = ld_make(&h->grids[lev], 0), c = ld_make(&h->cfL[lev], 0);
+  ld_from_packed(&f, fine);
+  for (int b = 0; b < h->grids[lev].n; b++) k_average2(&c.f[b], &f.f[b], &h->cfL[lev].b[b]);
+  ld_to_packed(&c, out);
+  ld_free(&f); ld_free(&c);
+  return 0;
+}
+int oracle_bottom_solve(OHier* h, double* corr_io, const double* res_in, int nbottom, double metric,
+                        double eps, int* info) {
+  int d = h->mgD-1;
+  const Op* op = mg_op(h, d);
+  LD c = ld_make(&h->mgL[d], 1), r = ld_make(&h->mgL[d], 0);
+  ld_from_packed(&c, corr_io); ld_from_packed(&r, res_in);
+  BiCG b = {eps, 1e-12, 1e-8, 1e-30, metric, 80, 5, 2};
+  op_relax(op, &c, &r, nbottom);
+  int st = bicgstab_solve(op, &c, &r, &b);
+  ld_to_packed(&c, corr_io);
+  if (info) info[0] = st;
+  ld_free(&c); ld_free(&r);
+  return 0;
+}

@@ -1,0 +1,201 @@
+"""ctypes binding of the CPU oracle (oracle/amr_oracle.c).
+
+TEST INFRASTRUCTURE: imported only by tests/, by the smoke check and by the
+CPU-baseline leg of bench.py.  The product never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        L.oracle_create.restype = C.c_void_p
+        L.oracle_create.argtypes = [_ip, _dp]
+        L.oracle_destroy.argtypes = [C.c_void_p]
+        L.oracle_last_error.restype = C.c_char_p
+        L.oracle_level_cells.restype = C.c_long
+        L.oracle_level_cells.argtypes = [C.c_void_p, C.c_int]
+        L.oracle_mg_cells.restype = C.c_long
+        L.oracle_mg_cells.argtypes = [C.c_void_p, C.c_int]
+        L.oracle_ghosted_cells.restype = C.c_long
+        L.oracle_ghosted_cells.argtypes = [C.c_void_p, C.c_int]
+        L.oracle_num_mg_depths.argtypes = [C.c_void_p]
+        L.oracle_solve.argtypes = [C.c_void_p, _ip, _dp, _dp, _dp, C.c_int, _dp, C.c_int, _ip]
+        L.oracle_amr_residual.restype = C.c_double
+        L.oracle_amr_residual.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_int]
+        L.oracle_amr_operator.argtypes = [C.c_void_p, _dp, _dp, C.c_int]
+        L.oracle_level_apply.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.c_int]
+        L.oracle_level_residual.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, C.c_int]
+        L.oracle_level_relax.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.c_int]
+        L.oracle_level_restrict.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp]
+        L.oracle_level_prolong.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+        L.oracle_kernel_gsrb.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.c_int]
+        L.oracle_homog_cf.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.oracle_quad_cf.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+        L.oracle_average.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+        L.oracle_cell_relax_count.restype = C.c_longlong
+        L.oracle_bottom_solve.argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.c_double, C.c_double, _ip]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_dp)
+
+
+class Oracle:
+    """One AMR hierarchy held by the CPU oracle."""
+
+    def __init__(self, hier):
+        self.hier = hier
+        isp, rsp = hier.spec()
+        self._isp, self._rsp = isp, rsp
+        L = lib()
+        self.h = L.oracle_create(isp.ctypes.data_as(_ip), rsp.ctypes.data_as(_dp))
+        if not self.h:
+            raise RuntimeError(L.oracle_last_error().decode())
+
+    def close(self):
+        if self.h:
+            lib().oracle_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def mg_depths(self) -> int:
+        return lib().oracle_num_mg_depths(self.h)
+
+    def mg_cells(self, d: int) -> int:
+        return lib().oracle_mg_cells(self.h, d)
+
+    def solve(self, rhs, sparams, rparams, phi0=None, hist_cap=128):
+        rhs, prhs = _d(rhs)
+        phi = np.zeros_like(rhs) if phi0 is None else np.array(phi0, dtype=np.float64)
+        phi, pphi = _d(phi)
+        hist = np.zeros(hist_cap)
+        info = np.zeros(3, dtype=np.int32)
+        sp = np.ascontiguousarray(sparams, dtype=np.int32)
+        rp = np.ascontiguousarray(rparams, dtype=np.float64)
+        rc = lib().oracle_solve(self.h, sp.ctypes.data_as(_ip), rp.ctypes.data_as(_dp), prhs, pphi,
+                                1 if phi0 is None else 0, hist.ctypes.data_as(_dp), hist_cap,
+                                info.ctypes.data_as(_ip))
+        if rc != 0:
+            raise RuntimeError(lib().oracle_last_error().decode())
+        it = int(info[0])
+        return phi, hist[:it + 1].copy(), dict(iterations=it, exit_status=int(info[1]),
+                                               bottom_calls=int(info[2]))
+
+    def amr_residual(self, phi, rhs, homog=False):
+        phi, pp = _d(phi)
+        rhs, pr = _d(rhs)
+        res = np.zeros_like(rhs)
+        n = lib().oracle_amr_residual(self.h, pp, pr, res.ctypes.data_as(_dp), int(homog))
+        return res, n
+
+    def amr_operator(self, phi, homog=False):
+        phi, pp = _d(phi)
+        out = np.zeros_like(phi)
+        lib().oracle_amr_operator(self.h, pp, out.ctypes.data_as(_dp), int(homog))
+        return out
+
+    def level_apply(self, depth, phi, homog=False):
+        phi, pp = _d(phi)
+        out = np.zeros_like(phi)
+        self._chk(lib().oracle_level_apply(self.h, depth, pp, out.ctypes.data_as(_dp), int(homog)))
+        return out
+
+    def level_residual(self, depth, phi, rhs, homog=False):
+        phi, pp = _d(phi)
+        rhs, pr = _d(rhs)
+        out = np.zeros_like(phi)
+        self._chk(lib().oracle_level_residual(self.h, depth, pp, pr, out.ctypes.data_as(_dp), int(homog)))
+        return out
+
+    def level_relax(self, depth, phi, rhs, iters):
+        phi = np.array(phi, dtype=np.float64)
+        rhs, pr = _d(rhs)
+        self._chk(lib().oracle_level_relax(self.h, depth, phi.ctypes.data_as(_dp), pr, iters))
+        return phi
+
+    def level_restrict(self, depth, phi, rhs):
+        phi, pp = _d(phi)
+        rhs, pr = _d(rhs)
+        out = np.zeros(self.mg_cells(depth + 1))
+        self._chk(lib().oracle_level_restrict(self.h, depth, pp, pr, out.ctypes.data_as(_dp)))
+        return out
+
+    def level_prolong(self, depth, phi, ec):
+        phi = np.array(phi, dtype=np.float64)
+        ec, pe = _d(ec)
+        self._chk(lib().oracle_level_prolong(self.h, depth, phi.ctypes.data_as(_dp), pe))
+        return phi
+
+    def kernel_gsrb(self, lev, phi_ghosted, rhs, color):
+        phi = np.array(phi_ghosted, dtype=np.float64)
+        rhs, pr = _d(rhs)
+        self._chk(lib().oracle_kernel_gsrb(self.h, lev, phi.ctypes.data_as(_dp), pr, color))
+        return phi
+
+    def homog_cf(self, lev, phi_ghosted):
+        phi = np.array(phi_ghosted, dtype=np.float64)
+        self._chk(lib().oracle_homog_cf(self.h, lev, phi.ctypes.data_as(_dp)))
+        return phi
+
+    def quad_cf(self, lev, phi_ghosted, coarse):
+        phi = np.array(phi_ghosted, dtype=np.float64)
+        coarse, pc = _d(coarse)
+        self._chk(lib().oracle_quad_cf(self.h, lev, phi.ctypes.data_as(_dp), pc))
+        return phi
+
+    def average(self, lev, fine):
+        fine, pf = _d(fine)
+        out = np.zeros(self.hier.cells(lev) // 8)
+        self._chk(lib().oracle_average(self.h, lev, pf, out.ctypes.data_as(_dp)))
+        return out
+
+    def bottom_solve(self, corr, res, nbottom, metric, eps):
+        corr = np.array(corr, dtype=np.float64)
+        res, pr = _d(res)
+        info = np.zeros(1, dtype=np.int32)
+        self._chk(lib().oracle_bottom_solve(self.h, corr.ctypes.data_as(_dp), pr, nbottom, metric, eps,
+                                            info.ctypes.data_as(_ip)))
+        return corr, int(info[0])
+
+    @staticmethod
+    def cell_relax_count():
+        return int(lib().oracle_cell_relax_count())
+
+    @staticmethod
+    def reset_counters():
+        lib().oracle_reset_counters()
+
+    @staticmethod
+    def _chk(rc):
+        if rc != 0:
+            raise RuntimeError(lib().oracle_last_error().decode())
