@@ -72,6 +72,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--base", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--iter-max", type=int, default=100,
+                    help="cap on V-cycles per solve (a short cap is for profiling runs only)")
+    ap.add_argument("--copy-mb", type=int, default=2048,
+                    help="size of the plain copy kernel run after the timed region (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only; multi-GPU runs are not part of this build", file=sys.stderr)
@@ -88,7 +92,7 @@ def main():
     t0 = time.time()
     rhs = H.bump_rhs(hier, 1002)
     log(f"rhs generated in {time.time() - t0:.1f}s")
-    sp, rp = H.solver_params(eps=1e-10, hang=1e-15, iterMax=100)
+    sp, rp = H.solver_params(eps=1e-10, hang=1e-15, iterMax=args.iter_max)
 
     ctx = amrmg.Context(0)
     t0 = time.time()
@@ -123,6 +127,12 @@ def main():
     traffic, _ = read_traffic()
     final_rel = float(infos[-1][0][-1] / infos[-1][0][0])
     ok = all(h[-1] <= 1e-10 * h[0] for h, _ in infos)
+    copy_gbs = None
+    if args.copy_mb > 0:
+        nbytes = args.copy_mb << 20
+        ms = ctx.copy_probe(nbytes, 10)
+        copy_gbs = 2 * nbytes / (ms * 1e-3) / 1e9
+        log(f"plain copy kernel ({args.copy_mb} MiB): {ms:.3f} ms, {copy_gbs:.0f} GB/s")
 
     cpu = None
     if not args.no_cpu:
@@ -165,6 +175,7 @@ def main():
             "avg_launch_ms": round(avg_launch_ms, 5),
             "algorithmic_bytes_per_launch": round(bytes_per_launch, 1),
             "launches": prof["launches"],
+            "measured_copy_GBs": round(copy_gbs, 1) if copy_gbs else None,
         },
         "cpu_baseline": cpu,
     }

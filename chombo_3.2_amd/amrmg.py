@@ -65,6 +65,7 @@ def lib():
         L.amrmg_average.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
         L.amrmg_bottom_solve.argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.c_double, C.c_double, _ip]
         L.amrmg_plan_check.argtypes = [_ip, _dp, C.POINTER(C.c_longlong), C.c_int]
+        L.amrmg_copy_probe.argtypes = [C.c_void_p, C.c_longlong, C.c_int, _dp]
         L.amrmg_profile_enable.argtypes = [C.c_void_p, C.c_int]
         L.amrmg_profile_reset.argtypes = [C.c_void_p]
         L.amrmg_profile_read.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.POINTER(C.c_longlong)]
@@ -120,6 +121,12 @@ class Context:
             for c in classes:
                 mask |= 1 << (KERNEL_CLASSES.index(c) if isinstance(c, str) else int(c))
         _chk(lib().amrmg_profile_enable(self.p, mask))
+
+    def copy_probe(self, nbytes, iters=10):
+        """Average ms of one device copy of nbytes (nbytes read + nbytes written)."""
+        ms = C.c_double()
+        _chk(lib().amrmg_copy_probe(self.p, int(nbytes), int(iters), C.byref(ms)))
+        return ms.value
 
     def profile_reset(self):
         _chk(lib().amrmg_profile_reset(self.p))

@@ -379,6 +379,13 @@ int amrmg_bottom_solve(amrmg_solver* s, double* corr, const double* res, int nbo
   API_END
 }
 
+int amrmg_copy_probe(amrmg_ctx* ctx, long long nbytes, int iters, double* ms_out) {
+  API_BEGIN
+  if (nbytes <= 0 || iters <= 0) throw std::runtime_error("copy probe: nbytes and iters must be positive");
+  *ms_out = copy_probe(ctx->c.get(), nbytes, iters);
+  API_END
+}
+
 int amrmg_plan_check(const int* ispec, const double* rspec, long long* stats, int nstats) {
   API_BEGIN
   Context dry(0, true);

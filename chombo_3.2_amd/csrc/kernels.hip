@@ -927,3 +927,37 @@ INST(float)
 
 }  // namespace ops
 }  // namespace amrmg
+
+// ---------------------------------------------------------------------------
+// plain streaming copy of known size: calibrates HBM byte counters (FETCH_SIZE,
+// WRITE_SIZE) and measures the achievable copy bandwidth on the device
+// ---------------------------------------------------------------------------
+namespace amrmg {
+__global__ void __launch_bounds__(256) k_copy_probe(double* __restrict__ dst, const double* __restrict__ src,
+                                                    long long n) {
+  for (long long q = blockIdx.x * 256LL + threadIdx.x; q < n; q += (long long)gridDim.x * 256) dst[q] = src[q];
+}
+double copy_probe(Context* c, long long nbytes, int iters) {
+  long long n = nbytes / 8;
+  double *a = nullptr, *b = nullptr;
+  hip_check(hipMalloc(&a, n * 8), "hipMalloc probe");
+  hip_check(hipMalloc(&b, n * 8), "hipMalloc probe");
+  hip_check(hipMemsetAsync(a, 0, n * 8, c->stream), "memset probe");
+  hipEvent_t e0, e1;
+  hip_check(hipEventCreate(&e0), "event");
+  hip_check(hipEventCreate(&e1), "event");
+  int grid = (int)std::min<long long>((n + 255) / 256, 256 * 1024);
+  hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, c->stream, b, a, n);  // warm
+  hip_check(hipEventRecord(e0, c->stream), "record");
+  for (int i = 0; i < iters; i++) hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, c->stream, b, a, n);
+  hip_check(hipEventRecord(e1, c->stream), "record");
+  hip_check(hipEventSynchronize(e1), "sync");
+  float ms = 0;
+  hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return ms / iters;
+}
+}  // namespace amrmg

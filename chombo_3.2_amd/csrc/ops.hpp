@@ -11,6 +11,8 @@ namespace amrmg {
 void profiler_enable(Context* c, unsigned class_mask);  // 0 = off
 void profiler_reset(Context* c);
 bool profiler_read(Context* c, int kind, double* ms, double* bytes, long long* count);
+// average ms of one copy of nbytes (read nbytes + write nbytes) over iters launches
+double copy_probe(Context* c, long long nbytes, int iters);
 
 namespace ops {
 

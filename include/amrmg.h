@@ -97,6 +97,10 @@ int amrmg_average(amrmg_solver* s, int lev, const double* fine, double* coarse);
 int amrmg_bottom_solve(amrmg_solver* s, double* corr, const double* res, int nbottom, double metric, double eps,
                        int* info);
 
+/* Plain device copy of nbytes (reads nbytes, writes nbytes), iters launches;
+   average ms per launch.  Calibrates HBM byte counters and the copy bandwidth. */
+int amrmg_copy_probe(amrmg_ctx* ctx, long long nbytes, int iters, double* ms_out);
+
 /* Host-only check: builds every device table of a hierarchy (ghost rules,
    copiers, coarse-fine stencils, reflux lists) without touching a device and
    verifies each address a kernel will form is inside its field.  stats (up to 8):
