@@ -12,8 +12,8 @@ over every level relaxed, per second of solve wall time).  roofline = the GSRB
 kernel's algorithmic bytes (24 B per cell-relax in FP64) over its HIP-event time
 inside the timed region, against 8 TB/s HBM.  cpu_baseline = the oracle (a C
 restatement of the reference's FP64 path, OpenMP over boxes) timed on a bounded
-sample of the same workload: setup excluded, initial residual + one AMR V-cycle +
-residual.
+sample of the same workload: setup excluded, initial residual + four AMR V-cycles
+with their residuals.
 """
 import argparse
 import json
@@ -26,20 +26,63 @@ sys.path.insert(0, os.path.join(ROOT, "chombo_3.2_amd"))
 
 METRIC = "Mcell-relax/s & %HBM roofline; solves/s + p99 latency at 1e-10, MP vs FP64"
 PEAK_HBM_GBS = 8000.0
+CPU_VCYCLES = 4
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "gsrb_amr3_256_traffic.json")
+
+
 def read_traffic():
-    """HBM bytes per GSRB launch from the committed rocprofv3 PMC summary, if any."""
-    p = os.path.join(ROOT, "profiles", "gsrb_c2_traffic.json")
-    if os.path.exists(p):
-        with open(p) as f:
-            d = json.load(f)
-        return d.get("hbm_bytes_per_launch"), d
-    return None, None
+    """Measured HBM bytes / algorithmic bytes of the GSRB kernel, from the committed
+    rocprofv3 PMC summary (None when absent)."""
+    if os.path.exists(TRAFFIC_JSON):
+        with open(TRAFFIC_JSON) as f:
+            return json.load(f).get("traffic_over_algorithmic")
+    return None
+
+
+def summarize_pmc(fetch_csv, write_csv, fetch_json, copy_bytes, out_path):
+    """Writes the GSRB traffic summary from two rocprofv3 --pmc runs of this script
+    (FETCH_SIZE, then WRITE_SIZE; both with --steps 1 --warmup 0 --no-cpu).
+
+    Each counter is scaled by what it reads for the plain copy kernel of copy_bytes
+    (reads copy_bytes, writes copy_bytes).  The ratio of scaled GSRB bytes to the
+    GSRB algorithmic bytes of the same run (taken from that run's JSON line) gives
+    the HBM traffic per launch of the benchmark's launch mix."""
+    import csv
+
+    def per_kernel(path):
+        rows = list(csv.DictReader(open(path)))
+        gs = [float(r["Counter_Value"]) * 1024.0 for r in rows if "k_gsrb" in r["Kernel_Name"]]
+        cp = [float(r["Counter_Value"]) * 1024.0 for r in rows if "k_copy_probe" in r["Kernel_Name"]]
+        return gs, cp
+    gf, cf = per_kernel(fetch_csv)
+    gw, cw = per_kernel(write_csv)
+    fscale = copy_bytes / (sum(cf) / len(cf))
+    wscale = copy_bytes / (sum(cw) / len(cw))
+    with open(fetch_json) as f:
+        run = json.loads(f.read().strip().splitlines()[-1])["roofline"]
+    alg_total = run["algorithmic_bytes_per_launch"] * run["launches"]
+    if len(gf) != run["launches"] or len(gw) != run["launches"]:
+        raise SystemExit(f"dispatch count mismatch: {len(gf)} / {len(gw)} vs {run['launches']}")
+    meas_total = sum(gf) * fscale + sum(gw) * wscale
+    ratio = meas_total / alg_total
+    d = {
+        "kernel": "k_gsrb (GSRB colour sweep)",
+        "fetch_scale": round(fscale, 4), "write_scale": round(wscale, 4),
+        "copy_bytes": copy_bytes,
+        "gsrb_dispatches": len(gf),
+        "fetch_bytes_total": round(sum(gf) * fscale), "write_bytes_total": round(sum(gw) * wscale),
+        "algorithmic_bytes_total": round(alg_total),
+        "traffic_over_algorithmic": round(ratio, 4),
+    }
+    with open(out_path, "w") as f:
+        json.dump(d, f, indent=1)
+    print(json.dumps(d))
 
 
 def cpu_baseline(hier, rhs, sp, rp):
@@ -52,7 +95,7 @@ def cpu_baseline(hier, rhs, sp, rp):
     o = Oracle(hier)
     t_setup = time.time() - t0
     sp1 = sp.copy()
-    sp1[4] = 1  # iterMax = 1: initial residual, one V-cycle, residual
+    sp1[4] = CPU_VCYCLES  # initial residual, CPU_VCYCLES AMR V-cycles, residuals
     Oracle.reset_counters()
     t0 = time.time()
     o.solve(rhs, sp1, rp)
@@ -60,7 +103,7 @@ def cpu_baseline(hier, rhs, sp, rp):
     cr = Oracle.cell_relax_count()
     o.close()
     return dict(value=cr / dt / 1e6, unit="Mcell-relax/s", cores=threads, kind="port",
-                sample=f"same 3-level 256^3 hierarchy and rhs; initial residual + 1 AMR V-cycle + residual "
+                sample=f"same 3-level 256^3 hierarchy and rhs; initial residual + {CPU_VCYCLES} AMR V-cycles with residuals "
                        f"({cr / 1e6:.1f} M cell-relaxations in {dt:.2f} s; oracle setup {t_setup:.1f} s "
                        f"excluded); OMP threads={threads}")
 
@@ -76,7 +119,12 @@ def main():
                     help="cap on V-cycles per solve (a short cap is for profiling runs only)")
     ap.add_argument("--copy-mb", type=int, default=2048,
                     help="size of the plain copy kernel run after the timed region (0 = skip)")
+    ap.add_argument("--summarize-pmc", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "FETCH_RUN_JSON"),
+                    help="write the GSRB traffic summary from two rocprofv3 --pmc runs and exit")
     args = ap.parse_args()
+    if args.summarize_pmc:
+        summarize_pmc(*args.summarize_pmc, copy_bytes=args.copy_mb << 20, out_path=TRAFFIC_JSON)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only; multi-GPU runs are not part of this build", file=sys.stderr)
         sys.exit(2)
@@ -124,7 +172,8 @@ def main():
     avg_launch_ms = prof["ms"] / max(prof["launches"], 1)
     achieved = prof["bytes"] / (prof["ms"] * 1e-3) / 1e9 if prof["ms"] > 0 else 0.0
     bytes_per_launch = prof["bytes"] / max(prof["launches"], 1)
-    traffic, _ = read_traffic()
+    ratio = read_traffic()
+    traffic = round(ratio * bytes_per_launch, 1) if ratio else None
     final_rel = float(infos[-1][0][-1] / infos[-1][0][0])
     ok = all(h[-1] <= 1e-10 * h[0] for h, _ in infos)
     copy_gbs = None
@@ -171,6 +220,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / PEAK_HBM_GBS, 4),
             "traffic": traffic,
+            "traffic_over_algorithmic": ratio,
             "kernel": "GSRB colour sweep (all boxes of a level per launch)",
             "avg_launch_ms": round(avg_launch_ms, 5),
             "algorithmic_bytes_per_launch": round(bytes_per_launch, 1),
