@@ -267,7 +267,7 @@ void AMRPoissonOp<Real>::reflux(T& phiFine, T& phi, T& res, AMRPoissonOp* finerO
 // ---------------------------------------------------------------------------
 // DeviceHierarchy / factory
 // ---------------------------------------------------------------------------
-void DeviceHierarchy::define(Context* c, const std::vector<DBL>& g, const Box& domain0, double dx0) {
+void DeviceHierarchy::define(Context* c, const std::vector<DBL>& g, const Box& domain0, double dx0, bool merge) {
   ctx = c;
   grids = g;
   domains.assign(g.size(), domain0);
@@ -281,11 +281,11 @@ void DeviceHierarchy::define(Context* c, const std::vector<DBL>& g, const Box& d
   mgLay.clear();
   for (size_t l = 0; l < g.size(); l++) {
     if (!g[l].sorted()) throw std::runtime_error("boxes of a level must be sorted (Box::operator<)");
-    amrLay.emplace_back(new DeviceLayout(c, g[l], domains[l]));
+    amrLay.emplace_back(new DeviceLayout(c, g[l], domains[l], merge));
     if (l == 0) cfLay.emplace_back(nullptr);
     else {
       if (!g[l].coarsenable(2)) throw std::runtime_error("fine level boxes must be coarsenable by 2");
-      cfLay.emplace_back(new DeviceLayout(c, g[l].coarsened(2), domains[l - 1]));
+      cfLay.emplace_back(new DeviceLayout(c, g[l].coarsened(2), domains[l - 1], merge));
     }
   }
   // MultiGrid::define via MGnewOp: stop when not coarsenable by coarsening*2
@@ -293,7 +293,7 @@ void DeviceHierarchy::define(Context* c, const std::vector<DBL>& g, const Box& d
   for (int depth = 1;; depth++) {
     int coarsening = 1 << depth;
     if (!g[0].coarsenable(coarsening * 2)) break;
-    mgLay.emplace_back(new DeviceLayout(c, g[0].coarsened(coarsening), coarsen(domain0, coarsening)));
+    mgLay.emplace_back(new DeviceLayout(c, g[0].coarsened(coarsening), coarsen(domain0, coarsening), merge));
   }
 }
 
@@ -347,7 +347,7 @@ std::unique_ptr<AMRPoissonOp<Real>> AMRPoissonOpFactory<Real>::AMRnewOp(int l, A
   op->m_beta = beta;
   op->m_bc = bc;
   op->m_mgCoarse = (l == 0 && !H->mgLay.empty()) ? H->mgLay[0].get() : (l > 0 ? H->cfLay[l].get() : nullptr);
-  op->m_ghost = build_ghost_plan(H->grids[l], H->domains[l], bc, l > 0);
+  op->m_ghost = build_ghost_plan(H->grids[l], H->domains[l], bc, l > 0, H->amrLay[l]->merged);
   validate_ghost_plan(op->m_ghost, H->grids[l]);
   op->m_ghost.upload(H->ctx);
   if (op->m_mgCoarse) validate_pair(*op->m_lay, *op->m_mgCoarse);
@@ -428,7 +428,7 @@ std::vector<std::unique_ptr<MGLevelOp<LevelData<Real>>>> AMRPoissonOpFactory<Rea
     op->m_beta = beta;
     op->m_bc = bc;
     op->m_mgCoarse = (d + 1 < H->mgLay.size()) ? H->mgLay[d + 1].get() : nullptr;
-    op->m_ghost = build_ghost_plan(H->mgLay[d]->dbl, op->m_domain, bc, false);
+    op->m_ghost = build_ghost_plan(H->mgLay[d]->dbl, op->m_domain, bc, false, H->mgLay[d]->merged);
     validate_ghost_plan(op->m_ghost, H->mgLay[d]->dbl);
     op->m_ghost.upload(H->ctx);
     if (op->m_mgCoarse) validate_pair(*op->m_lay, *op->m_mgCoarse);

@@ -101,7 +101,8 @@ struct DeviceHierarchy {
   std::vector<std::unique_ptr<DeviceLayout>> amrLay;  // per AMR level
   std::vector<std::unique_ptr<DeviceLayout>> cfLay;   // coarsen(grids[l], 2); index l >= 1
   std::vector<std::unique_ptr<DeviceLayout>> mgLay;   // MG levels of AMR level 0, depth >= 1
-  void define(Context* c, const std::vector<DBL>& g, const Box& domain0, double dx0);
+  // merge: store each level whose boxes tile a rectangle as one array
+  void define(Context* c, const std::vector<DBL>& g, const Box& domain0, double dx0, bool merge = true);
   int nlev() const { return (int)grids.size(); }
   int mgDepth() const { return 1 + (int)mgLay.size(); }
 };

@@ -45,8 +45,27 @@ struct amrmg_solver {
   int sp[6] = {2, 2, 2, 1, 100, 5};
   double rp[3] = {1e-10, 1e-15, 1e-30};
   long long table_bytes = 0;
+  // Per-box storage for the FAB-level entry points (amrmg_kernel_gsrb,
+  // amrmg_homog_cf, amrmg_quad_cf), whose ghost cells are caller inputs like the
+  // ghost cells of the reference's FArrayBoxes; built on first use.
+  std::unique_ptr<DeviceHierarchy> fabH;
+  std::unique_ptr<AMRPoissonOpFactory<double>> fabFac;
+  std::vector<std::unique_ptr<AMRPoissonOp<double>>> fabOps;
 
   int nlev() const { return H.nlev(); }
+  AMRPoissonOp<double>* fabOp(int lev) {
+    if (!fabH) {
+      fabH.reset(new DeviceHierarchy());
+      fabH->define(ctx->c.get(), H.grids, H.domains[0], H.dxs[0], false);
+      fabFac.reset(new AMRPoissonOpFactory<double>());
+      fabFac->define(fabH.get(), alpha, beta, bc);
+    }
+    while ((int)fabOps.size() <= lev) {
+      const int l = (int)fabOps.size();
+      fabOps.emplace_back(fabFac->AMRnewOp(l, l > 0 ? fabOps[l - 1].get() : nullptr));
+    }
+    return fabOps[lev].get();
+  }
   std::vector<LevelData<double>*> ptrs(std::vector<std::unique_ptr<LevelData<double>>>& v) {
     std::vector<LevelData<double>*> p;
     for (auto& x : v) p.push_back(x.get());
@@ -318,7 +337,7 @@ int amrmg_level_prolong(amrmg_solver* s, int depth, double* phi, const double* e
 int amrmg_kernel_gsrb(amrmg_solver* s, int lev, double* phi_g, const double* rhs, int color) {
   API_BEGIN
   if (lev < 0 || lev >= s->nlev()) throw std::runtime_error("bad level");
-  const DeviceLayout* L = s->H.amrLay[lev].get();
+  const DeviceLayout* L = s->fabOp(lev)->m_lay;
   LevelData<double> p(L), r(L);
   upload_ghosted(*L, p.d, phi_g);
   upload_valid(*L, r.d, rhs);
@@ -329,10 +348,10 @@ int amrmg_kernel_gsrb(amrmg_solver* s, int lev, double* phi_g, const double* rhs
 int amrmg_homog_cf(amrmg_solver* s, int lev, double* phi_g) {
   API_BEGIN
   if (lev < 0 || lev >= s->nlev()) throw std::runtime_error("bad level");
-  const DeviceLayout* L = s->H.amrLay[lev].get();
+  AMRPoissonOp<double>* op = s->fabOp(lev);
+  const DeviceLayout* L = op->m_lay;
   LevelData<double> p(L);
   upload_ghosted(*L, p.d, phi_g);
-  AMRPoissonOp<double>* op = s->amg.m_op[lev].get();
   ops::fill_ghosts<double>(*L, op->m_ghost, p.d, 2, 1, op->m_dx, op->m_dxCrse);
   download_ghosted(*L, p.d, phi_g);
   API_END
@@ -340,12 +359,13 @@ int amrmg_homog_cf(amrmg_solver* s, int lev, double* phi_g) {
 int amrmg_quad_cf(amrmg_solver* s, int lev, double* phi_g, const double* coarse) {
   API_BEGIN
   if (lev < 1 || lev >= s->nlev()) throw std::runtime_error("bad level");
-  const DeviceLayout* L = s->H.amrLay[lev].get();
-  const DeviceLayout* Lc = s->H.amrLay[lev - 1].get();
+  AMRPoissonOp<double>* op = s->fabOp(lev);
+  const DeviceLayout* L = op->m_lay;
+  const DeviceLayout* Lc = op->m_coarserLay;
   LevelData<double> p(L), c(Lc);
   upload_ghosted(*L, p.d, phi_g);
   upload_valid(*Lc, c.d, coarse);
-  s->amg.m_op[lev]->coarseFineInterp(p, c);
+  op->coarseFineInterp(p, c);
   download_ghosted(*L, p.d, phi_g);
   API_END
 }

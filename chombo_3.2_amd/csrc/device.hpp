@@ -124,18 +124,32 @@ void dev_alloc(Context* c, void** p, size_t bytes, const char* what);
 void dev_free(Context* c, void* p);
 void dev_h2d(Context* c, void* dst, const void* src, size_t bytes, const char* what);
 
+// Storage of one level's fields.
+//  - per-box: every box owns a ghosted FAB (box grown by 1), FABs back to back,
+//    as the reference's LevelData<FArrayBox> holds them;
+//  - merged: when the boxes tile their bounding rectangle exactly, the level is
+//    one ghosted array over that rectangle.  Every box keeps its own DBox, with
+//    the merged array's strides, so a box's face ghost next to another box is
+//    that box's valid cell (a Copier exchange is the identity) and the ghost
+//    cells outside the rectangle are each the face ghost of exactly one box.
+//    Rows are padded to 16 elements and the array starts so that every row's
+//    first valid cell is 128-byte aligned.  h_boxes[nbox] is the rectangle
+//    itself and the tiles cover it rather than each box.
 struct DeviceLayout {
   DBL dbl;
   Box domain;
-  std::vector<DBox> h_boxes;
+  std::vector<DBox> h_boxes;   // nbox boxes, then (merged) the rectangle
   DBox* d_boxes = nullptr;
-  long long slab = 0;      // elements per field
+  int nbox = 0;
+  bool merged = false;
+  Box rect;                    // bounding rectangle of the boxes
+  long long slab = 0;          // elements per field
   Tile* d_tiles = nullptr;
   int ntiles = 0;
   int TX = 64, TY = 4, TZ = 1;
   long long valid_cells = 0;
   Context* ctx = nullptr;
-  DeviceLayout(Context* c, const DBL& d, const Box& dom);
+  DeviceLayout(Context* c, const DBL& d, const Box& dom, bool merge = true);
   ~DeviceLayout();
   DeviceLayout(const DeviceLayout&) = delete;
   DeviceLayout& operator=(const DeviceLayout&) = delete;

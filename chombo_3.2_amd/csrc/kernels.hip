@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <stdexcept>
@@ -76,8 +77,8 @@ struct Profiler {
     recs.clear();
   }
   ~Profiler() {
-    for (auto& r : recs) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
-    for (auto e : pool) hipEventDestroy(e);
+    for (auto& r : recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+    for (auto e : pool) (void)hipEventDestroy(e);
   }
 };
 static hipEvent_t prof_begin(Context* c, int kind) {
@@ -96,17 +97,17 @@ static void prof_end(Context* c, hipEvent_t a, int kind, double bytes) {
 void profiler_enable(Context* c, unsigned mask) {
   if (mask && !c->prof) c->prof = new Profiler();
   if (mask && c->prof) c->prof->mask = mask;
-  if (!mask && c->prof) { hipStreamSynchronize(c->stream); delete c->prof; c->prof = nullptr; }
+  if (!mask && c->prof) { (void)hipStreamSynchronize(c->stream); delete c->prof; c->prof = nullptr; }
 }
 void profiler_reset(Context* c) {
   if (!c->prof) return;
-  hipStreamSynchronize(c->stream);
+  (void)hipStreamSynchronize(c->stream);
   c->prof->drain();
   for (int q = 0; q < KC_COUNT; q++) { c->prof->ms[q] = 0; c->prof->bytes[q] = 0; c->prof->count[q] = 0; }
 }
 bool profiler_read(Context* c, int kind, double* ms, double* bytes, long long* count) {
   if (!c->prof || kind < 0 || kind >= KC_COUNT) return false;
-  hipStreamSynchronize(c->stream);
+  (void)hipStreamSynchronize(c->stream);
   c->prof->drain();
   *ms = c->prof->ms[kind]; *bytes = c->prof->bytes[kind]; *count = c->prof->count[kind];
   return true;
@@ -115,10 +116,10 @@ bool profiler_read(Context* c, int kind, double* ms, double* bytes, long long* c
 Context::~Context() {
   if (dry) return;
   if (prof) profiler_enable(this, 0u);
-  if (stream) hipStreamSynchronize(stream);
-  if (d_scalars) hipFree(d_scalars);
-  if (h_scalars) hipHostFree(h_scalars);
-  if (stream) hipStreamDestroy(stream);
+  if (stream) (void)hipStreamSynchronize(stream);
+  if (d_scalars) (void)hipFree(d_scalars);
+  if (h_scalars) (void)hipHostFree(h_scalars);
+  if (stream) (void)hipStreamDestroy(stream);
 }
 void Context::sync() { hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize"); }
 
@@ -128,41 +129,85 @@ static int pow2_at_least(int v) {
   return p;
 }
 
-DeviceLayout::DeviceLayout(Context* c, const DBL& d, const Box& dom) : dbl(d), domain(dom), ctx(c) {
-  long long off = 0;
-  int maxnx = 1, maxny = 1;
-  h_boxes.resize(d.size());
-  for (int b = 0; b < d.size(); b++) {
-    const Box& bx = d[b];
-    DBox x;
-    for (int q = 0; q < 3; q++) { x.lo[q] = bx.lo[q]; x.hi[q] = bx.hi[q]; }
-    int nx = bx.len(0) + 2, ny = bx.len(1) + 2, nz = bx.len(2) + 2;
-    x.off = off;
-    x.sy = nx;
-    x.sz = nx * ny;
-    off += (long long)nx * ny * nz;
-    h_boxes[b] = x;
-    maxnx = std::max(maxnx, bx.len(0));
-    maxny = std::max(maxny, bx.len(1));
-    valid_cells += bx.npts();
+DeviceLayout::DeviceLayout(Context* c, const DBL& d, const Box& dom, bool merge) : dbl(d), domain(dom), ctx(c) {
+  nbox = d.size();
+  long long cells = 0;
+  for (int b = 0; b < nbox; b++) {
+    cells += d[b].npts();
+    if (b == 0) rect = d[b];
+    for (int q = 0; q < 3; q++) {
+      rect.lo[q] = std::min(rect.lo[q], d[b].lo[q]);
+      rect.hi[q] = std::max(rect.hi[q], d[b].hi[q]);
+    }
   }
-  slab = off;
+  valid_cells = cells;
+  // disjoint boxes whose volumes add up to the bounding box's tile it exactly
+  merged = merge && nbox > 0 && cells == rect.npts();
+  h_boxes.resize(nbox + (merged ? 1 : 0));
+  int maxnx = 1, maxny = 1;
+  if (merged) {
+    const int sy = (rect.len(0) + 2 + 15) / 16 * 16;
+    const long long sz = (long long)sy * (rect.len(1) + 2);
+    if (sz > INT32_MAX) throw std::runtime_error("merged level plane too large");
+    const long long base = 15;  // slab index of (rect.lo - 1); rect.lo[0] lands on a multiple of 16
+    for (int b = 0; b <= nbox; b++) {
+      const Box& bx = b < nbox ? d[b] : rect;
+      DBox x;
+      for (int q = 0; q < 3; q++) { x.lo[q] = bx.lo[q]; x.hi[q] = bx.hi[q]; }
+      x.sy = sy;
+      x.sz = (int)sz;
+      x.off = base + (bx.lo[0] - rect.lo[0]) + (long long)(bx.lo[1] - rect.lo[1]) * sy +
+              (long long)(bx.lo[2] - rect.lo[2]) * sz;
+      h_boxes[b] = x;
+    }
+    slab = base + sz * (rect.len(2) + 2);
+    maxnx = rect.len(0);
+    maxny = rect.len(1);
+  } else {
+    long long off = 0;
+    for (int b = 0; b < nbox; b++) {
+      const Box& bx = d[b];
+      DBox x;
+      for (int q = 0; q < 3; q++) { x.lo[q] = bx.lo[q]; x.hi[q] = bx.hi[q]; }
+      int nx = bx.len(0) + 2, ny = bx.len(1) + 2, nz = bx.len(2) + 2;
+      x.off = off;
+      x.sy = nx;
+      x.sz = nx * ny;
+      off += (long long)nx * ny * nz;
+      h_boxes[b] = x;
+      maxnx = std::max(maxnx, bx.len(0));
+      maxny = std::max(maxny, bx.len(1));
+    }
+    slab = off;
+  }
   TX = std::min(64, pow2_at_least(maxnx));
   TY = std::min(256 / TX, pow2_at_least(maxny));
   TZ = 256 / (TX * TY);
   std::vector<Tile> tiles;
-  for (int b = 0; b < d.size(); b++) {
-    const Box& bx = d[b];
+  const int tb0 = merged ? nbox : 0, tb1 = merged ? nbox + 1 : nbox;
+  for (int b = tb0; b < tb1; b++) {
+    const DBox& bx = h_boxes[b];
     for (int k = bx.lo[2]; k <= bx.hi[2]; k += TZ)
       for (int j = bx.lo[1]; j <= bx.hi[1]; j += TY)
         for (int i = bx.lo[0]; i <= bx.hi[0]; i += TX) tiles.push_back(Tile{b, i, j, k});
   }
+  if (tiles.size() > (size_t)INT32_MAX) throw std::runtime_error("too many tiles");
   ntiles = (int)tiles.size();
   // every tile cell index stays inside its box's grown FAB (threads past the
   // box's high end return), so tiles need only a valid box index
-  for (const Tile& t : tiles)
-    if (t.box < 0 || t.box >= d.size() || !d[t.box].contains(IV{t.i0, t.j0, t.k0}))
+  for (const Tile& t : tiles) {
+    if (t.box < 0 || t.box >= (int)h_boxes.size()) throw std::runtime_error("tile outside its box");
+    const DBox& x = h_boxes[t.box];
+    if (t.i0 < x.lo[0] || t.i0 > x.hi[0] || t.j0 < x.lo[1] || t.j0 > x.hi[1] || t.k0 < x.lo[2] || t.k0 > x.hi[2])
       throw std::runtime_error("tile outside its box");
+  }
+  // every box's grown region addresses stay inside the slab
+  for (int b = 0; b < (int)h_boxes.size(); b++) {
+    const DBox& x = h_boxes[b];
+    const long long lo = index(b, x.lo[0] - 1, x.lo[1] - 1, x.lo[2] - 1);
+    const long long hi = index(b, x.hi[0] + 1, x.hi[1] + 1, x.hi[2] + 1);
+    if (lo < 0 || hi >= slab) throw std::runtime_error("box storage outside the level slab");
+  }
   dev_alloc(c, (void**)&d_boxes, sizeof(DBox) * h_boxes.size(), "hipMalloc boxes");
   dev_h2d(c, d_boxes, h_boxes.data(), sizeof(DBox) * h_boxes.size(), "copy boxes");
   dev_alloc(c, (void**)&d_tiles, sizeof(Tile) * tiles.size(), "hipMalloc tiles");
@@ -446,10 +491,21 @@ __global__ void __launch_bounds__(256) k_restrict(const DBox* __restrict__ cboxe
       }
   resc[cidx(cb, I, J, K)] = acc;
 }
+// Kernels over two layouts index the second layout's box table with the tile's box
+// index: the layouts must pair box for box (one is the coarsening of the other).
+static void check_paired(const DeviceLayout& Lf, const DeviceLayout& Lc, const char* what) {
+  if (Lf.nbox != Lc.nbox || Lf.merged != Lc.merged || Lf.h_boxes.size() != Lc.h_boxes.size())
+    throw std::runtime_error(std::string(what) + ": fine and coarse layouts do not pair box for box");
+  for (size_t b = 0; b < Lf.h_boxes.size(); b++)
+    for (int q = 0; q < 3; q++)
+      if ((Lf.h_boxes[b].lo[q] >> 1) != Lc.h_boxes[b].lo[q] || (Lf.h_boxes[b].hi[q] >> 1) != Lc.h_boxes[b].hi[q])
+        throw std::runtime_error(std::string(what) + ": coarse box is not the coarsened fine box");
+}
 template <class T>
 void restrict_res(const DeviceLayout& Lf, const DeviceLayout& Lc, T* resc, const T* phi, const T* rhs, double dx,
                   double alpha, double beta) {
   const DeviceLayout& L = Lc;
+  check_paired(Lf, Lc, "restrict_res");
   double dxinv = 1.0 / (dx * dx);
   LAUNCH_P(KC_RESTRICT, (2.0 + 1.0 / 8.0) * sizeof(T) * Lf.valid_cells, k_restrict<T>, Lc.ntiles, Lc.d_boxes, Lc.d_tiles, geom(Lc), Lf.d_boxes, resc, phi, rhs, T(dxinv), T(alpha),
          T(beta));
@@ -470,6 +526,7 @@ __global__ void __launch_bounds__(256) k_prolong(const DBox* __restrict__ fboxes
 template <class T>
 void prolong(const DeviceLayout& Lf, const DeviceLayout& Lc, T* phi, const T* coarse) {
   const DeviceLayout& L = Lf;
+  check_paired(Lf, Lc, "prolong");
   LAUNCH_P(KC_PROLONG, (2.0 + 1.0 / 8.0) * sizeof(T) * Lf.valid_cells, k_prolong<T>, Lf.ntiles, Lf.d_boxes, Lf.d_tiles, geom(Lf), Lc.d_boxes, phi, coarse);
 }
 
@@ -491,6 +548,7 @@ __global__ void __launch_bounds__(256) k_average(const DBox* __restrict__ cboxes
 template <class T>
 void average(const DeviceLayout& Lf, const DeviceLayout& Lc, T* coarse, const T* fine) {
   const DeviceLayout& L = Lc;
+  check_paired(Lf, Lc, "average");
   LAUNCH_P(KC_AVERAGE, (1.0 + 1.0 / 8.0) * sizeof(T) * Lf.valid_cells, k_average<T>, Lc.ntiles, Lc.d_boxes, Lc.d_tiles, geom(Lc), Lf.d_boxes, coarse, fine);
 }
 
@@ -634,11 +692,18 @@ __global__ void k_zero_scalar(double* s, int slot) {
 // reference's value.  Non-negative doubles order like their bit patterns.
 template <class T>
 __global__ void __launch_bounds__(256) k_maxnorm(const DBox* __restrict__ boxes, const Tile* __restrict__ tiles,
-                                                 TileGeom g, const T* __restrict__ x, double* out) {
-  DBox b;
-  int i, j, k;
+                                                 TileGeom g, int ntiles, const T* __restrict__ x, double* out) {
+  // grid-stride over tiles: one atomic per block, a few thousand blocks per launch
   double v = 0.0;
-  if (tile_cell(boxes, tiles, g, b, i, j, k)) v = fabs((double)x[cidx(b, i, j, k)]);
+  const int tid = threadIdx.x;
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const Tile tl = tiles[t];
+    const DBox b = boxes[tl.box];
+    const int i = tl.i0 + (tid & ((1 << g.lx) - 1));
+    const int j = tl.j0 + ((tid >> g.lx) & ((1 << g.ly) - 1));
+    const int k = tl.k0 + (tid >> (g.lx + g.ly));
+    if (i <= b.hi[0] && j <= b.hi[1] && k <= b.hi[2]) v = fmax(v, fabs((double)x[cidx(b, i, j, k)]));
+  }
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
   __shared__ double sm[4];
   if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
@@ -651,7 +716,9 @@ __global__ void __launch_bounds__(256) k_maxnorm(const DBox* __restrict__ boxes,
 template <class T>
 void maxnorm_async(const DeviceLayout& L, const T* x, int slot, int accumulate) {
   if (!accumulate) LAUNCH(k_zero_scalar, 1, L.ctx->d_scalars, slot);
-  LAUNCH_P(KC_REDUCE, 1.0 * sizeof(T) * L.valid_cells, k_maxnorm<T>, L.ntiles, L.d_boxes, L.d_tiles, geom(L), x, L.ctx->d_scalars + slot);
+  const int grid = std::min(L.ntiles, 2048);
+  LAUNCH_P(KC_REDUCE, 1.0 * sizeof(T) * L.valid_cells, k_maxnorm<T>, grid, L.d_boxes, L.d_tiles, geom(L), L.ntiles, x,
+           L.ctx->d_scalars + slot);
 }
 template <class T>
 double maxnorm(const DeviceLayout& L, const T* x, int slot) {
@@ -721,7 +788,7 @@ static double* scratch_partials(Context* ctx, int n) {
   static thread_local int cap = 0;
   static thread_local Context* owner = nullptr;
   if (n > cap || owner != ctx) {
-    if (buf) { hipStreamSynchronize(ctx->stream); hipFree(buf); }
+    if (buf) { (void)hipStreamSynchronize(ctx->stream); (void)hipFree(buf); }
     cap = std::max(n, 1024);
     hip_check(hipMalloc(&buf, sizeof(double) * cap), "hipMalloc partials");
     owner = ctx;
@@ -730,8 +797,8 @@ static double* scratch_partials(Context* ctx, int n) {
 }
 template <class T>
 double dot_ordered(const DeviceLayout& L, const T* a, const T* b, int slot) {
-  double* part = scratch_partials(L.ctx, (int)L.h_boxes.size());
-  hipLaunchKernelGGL(k_dot_ordered<T>, dim3(1), dim3(256), 0, L.ctx->stream, L.d_boxes, (int)L.h_boxes.size(), a, b,
+  double* part = scratch_partials(L.ctx, L.nbox);
+  hipLaunchKernelGGL(k_dot_ordered<T>, dim3(1), dim3(256), 0, L.ctx->stream, L.d_boxes, L.nbox, a, b,
                      part, L.ctx->d_scalars + slot);
   hip_check(hipGetLastError(), "k_dot_ordered");
   hip_check(hipMemcpyAsync(L.ctx->h_scalars + slot, L.ctx->d_scalars + slot, sizeof(double), hipMemcpyDeviceToHost,
@@ -742,9 +809,9 @@ double dot_ordered(const DeviceLayout& L, const T* a, const T* b, int slot) {
 }
 template <class T>
 double norm2_ordered(const DeviceLayout& L, const T* x, int slot) {
-  for (auto& b : L.h_boxes)
-    if (b.hi[0] - b.lo[0] + 1 > 1024) throw std::runtime_error("norm2_ordered: box wider than 1024");
-  hipLaunchKernelGGL(k_sumpow_ordered<T>, dim3(1), dim3(256), 0, L.ctx->stream, L.d_boxes, (int)L.h_boxes.size(), x,
+  for (int q = 0; q < L.nbox; q++)
+    if (L.h_boxes[q].hi[0] - L.h_boxes[q].lo[0] + 1 > 1024) throw std::runtime_error("norm2_ordered: box wider than 1024");
+  hipLaunchKernelGGL(k_sumpow_ordered<T>, dim3(1), dim3(256), 0, L.ctx->stream, L.d_boxes, L.nbox, x,
                      L.ctx->d_scalars + slot);
   hip_check(hipGetLastError(), "k_sumpow_ordered");
   hip_check(hipMemcpyAsync(L.ctx->h_scalars + slot, L.ctx->d_scalars + slot, sizeof(double), hipMemcpyDeviceToHost,

@@ -90,7 +90,9 @@ static Patch make_patch(int kind, int dst, int src, int dir, int side, const Box
 // (AMRPoissonOp.cpp:1598-1671).  Edge and corner ghosts are never read by the
 // 7-point stencil and are not filled.
 // ---------------------------------------------------------------------------
-ops::GhostPlan build_ghost_plan(const DBL& dbl, const Box& domain, const BCSpec& bc, bool withCF) {
+// With merged storage (one array per level) an exchange copies every cell onto
+// itself, so no exchange patch is emitted.
+ops::GhostPlan build_ghost_plan(const DBL& dbl, const Box& domain, const BCSpec& bc, bool withCF, bool merged) {
   ops::GhostPlan P;
   BoxFinder bf(dbl);
   for (int b = 0; b < dbl.size(); b++) {
@@ -109,7 +111,7 @@ ops::GhostPlan build_ghost_plan(const DBL& dbl, const Box& domain, const BCSpec&
         for (int n : bf.overlapping(R)) {
           if (n == b) continue;
           Box r = R & dbl[n];
-          P.h_patches.push_back(make_patch(PK_EXCHANGE, b, n, dir, side, r, 0.0));
+          if (!merged) P.h_patches.push_back(make_patch(PK_EXCHANGE, b, n, dir, side, r, 0.0));
           cf.subtract(dbl[n]);
         }
         if (!withCF || cf.numPts() == 0) continue;

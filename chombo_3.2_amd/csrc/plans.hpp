@@ -34,7 +34,7 @@ class BoxFinder {
 
 // Face-ghost rules of one level: exchange from neighbours, BCs on the domain
 // boundary, homogeneous CF interpolation on coarse-fine faces (CFRegion/CFIVS).
-ops::GhostPlan build_ghost_plan(const DBL& dbl, const Box& domain, const BCSpec& bc, bool withCF);
+ops::GhostPlan build_ghost_plan(const DBL& dbl, const Box& domain, const BCSpec& bc, bool withCF, bool merged);
 
 // Copier from src layout to dst layout (Copier::define with zero ghost).
 ops::MotionPlan build_copier(const DBL& src, const DBL& dst);

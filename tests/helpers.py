@@ -104,3 +104,13 @@ def lshape_qcfi():
     fine = [H.box_refine(b, 2) for b in fine]
     coarse = [((0, 0, 0), (7, 15, 15)), ((8, 0, 0), (15, 7, 15)), ((8, 8, 0), (15, 15, 15))]
     return H.Hierarchy(H.box((0, 0, 0), (15, 15, 15)), [coarse, fine], 2 * 3.14159)
+
+
+def lshape_amr(alpha=0.0, beta=1.0):
+    """Three levels over a 32^3 base whose middle level is L-shaped in x-y, so its
+    boxes do not tile a rectangle and keep per-box storage (levels 0 and 2 do tile
+    one and are stored merged)."""
+    dom = H.box((0, 0, 0), (31, 31, 31))
+    lev1 = [((16, 16, 16), (31, 31, 47)), ((16, 32, 16), (31, 47, 47)), ((32, 16, 16), (47, 31, 47))]
+    lev2 = [((40, 40, 40), (55, 55, 63)), ((40, 40, 64), (55, 55, 87))]
+    return H.Hierarchy(dom, [H.domain_split(dom, 16, 8), lev1, lev2], 1.0 / 32, alpha, beta)

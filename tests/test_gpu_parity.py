@@ -10,8 +10,8 @@ import pytest
 
 import amrmg
 import hierarchy as H
-from helpers import (ghost_dir, ghost_masks, ghosted_from_fn, ghosted_random, lshape_qcfi, small_amr,
-                     small_single, uncovered_mask)
+from helpers import (ghost_dir, ghost_masks, ghosted_from_fn, ghosted_random, lshape_amr, lshape_qcfi,
+                     small_amr, small_single, uncovered_mask)
 from oracle import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -110,9 +110,10 @@ def test_amr_cf_average_bitwise(gpu_ctx):
         np.testing.assert_array_equal(g.average(lev, fine), o.average(lev, fine))
 
 
+@pytest.mark.parametrize("hier", ["amr32", "lshape"])
 @pytest.mark.parametrize("homog", [False, True])
-def test_amr_residual_operator_bitwise(gpu_ctx, homog):
-    h = small_amr()
+def test_amr_residual_operator_bitwise(gpu_ctx, homog, hier):
+    h = small_amr() if hier == "amr32" else lshape_amr()
     g, o = _pair(gpu_ctx, h)
     rng = np.random.default_rng(9)
     phi = rng.standard_normal(h.total_cells())
@@ -142,9 +143,15 @@ def _solve_pair(gpu_ctx, h, rhs, **kw):
     return phi_g, hist_g, info_g, phi_o, hist_o, info_o
 
 
-@pytest.mark.parametrize("case", ["single32", "amr32", "helm_single24"])
+@pytest.mark.parametrize("case", ["single32", "amr32", "helm_single24", "amr_lshape", "helm_lshape"])
 def test_solve_matches_oracle(gpu_ctx, case):
-    if case == "single32":
+    if case == "amr_lshape":
+        h = lshape_amr()
+        rhs = H.bump_rhs(h, 1002)
+    elif case == "helm_lshape":
+        h = lshape_amr(alpha=0.6, beta=1.1)
+        rhs = H.bump_rhs(h, 1003)
+    elif case == "single32":
         h = small_single(n=32, box=16)
         rhs = H.sinusoid_rhs(h, 1001)
     elif case == "amr32":
