@@ -39,6 +39,7 @@ def lib():
         L.amrmg_last_error.restype = C.c_char_p
         L.amrmg_ctx_create.argtypes = [C.c_int, C.POINTER(C.c_void_p)]
         L.amrmg_ctx_destroy.argtypes = [C.c_void_p]
+        L.amrmg_ctx_set_fused_relax.argtypes = [C.c_void_p, C.c_int]
         L.amrmg_solver_create.argtypes = [C.c_void_p, _ip, _dp, C.c_int, C.POINTER(C.c_void_p)]
         L.amrmg_solver_destroy.argtypes = [C.c_void_p]
         L.amrmg_solver_set_params.argtypes = [C.c_void_p, _ip, _dp]
@@ -121,6 +122,11 @@ class Context:
             for c in classes:
                 mask |= 1 << (KERNEL_CLASSES.index(c) if isinstance(c, str) else int(c))
         _chk(lib().amrmg_profile_enable(self.p, mask))
+
+    def set_fused_relax(self, on: bool):
+        """One-launch GSRB iterations on merged levels (True, the default) or the
+        reference's four-launch sequence (False); results are bitwise the same."""
+        _chk(lib().amrmg_ctx_set_fused_relax(self.p, 1 if on else 0))
 
     def copy_probe(self, nbytes, iters=10):
         """Average ms of one device copy of nbytes (nbytes read + nbytes written)."""

@@ -148,6 +148,13 @@ void AMRPoissonOp<Real>::createCoarser(T& coarse, const T& fine, bool) {
 template <class Real>
 void AMRPoissonOp<Real>::levelGSRB(T& phi, const T& rhs) {
   need(rhs.lay, m_lay, "levelGSRB");
+  if (m_fused && m_ctx->fused_relax) {
+    need(phi.lay, m_lay, "levelGSRB");
+    if (!m_relaxTmp.isDefined()) m_relaxTmp.define(m_lay);
+    ops::gsrb_fused<Real>(*m_lay, m_faces, m_relaxTmp.d, phi.d, rhs.d, m_dx, m_alpha, m_beta);
+    std::swap(phi.d, m_relaxTmp.d);
+    return;
+  }
   for (int pass = 0; pass <= 1; pass++) {
     fillGhosts(phi, true, true);
     ops::gsrb<Real>(*m_lay, phi.d, rhs.d, pass, m_dx, m_alpha, m_beta);
@@ -335,6 +342,39 @@ static void validate_pair(const DeviceLayout& f, const DeviceLayout& c) {
   for (int b = 0; b < f.dbl.size(); b++) require(coarsen(f.dbl[b], 2) == c.dbl[b], "coarsened box mismatch");
 }
 
+// The one-launch relaxation applies when the level is stored merged and every
+// ghost rule is the rule of one whole face of its rectangle: homogeneous
+// Dirichlet or Neumann, or quadratic coarse-fine (a coarse-fine face of a box one
+// cell wide uses the linear rule and keeps the two-launch path).
+template <class Real>
+static void setup_fused(AMRPoissonOp<Real>* op) {
+  const DeviceLayout& L = *op->m_lay;
+  op->m_fused = false;
+  if (!L.merged) return;
+  int kind[3][2] = {{-1, -1}, {-1, -1}, {-1, -1}};
+  for (const Patch& p : op->m_ghost.h_patches) {
+    int k;
+    if (p.kind == PK_NEUMANN) k = 0;
+    else if (p.kind == PK_DIRICHLET) k = 1;
+    else if (p.kind == PK_CFHOMO) k = 2;
+    else return;
+    const int face = p.side ? L.rect.hi[p.dir] + 1 : L.rect.lo[p.dir] - 1;
+    if (p.lo[p.dir] != face || p.hi[p.dir] != face) return;
+    if (kind[p.dir][p.side] == -1) kind[p.dir][p.side] = k;
+    else if (kind[p.dir][p.side] != k) return;
+  }
+  for (int d = 0; d < 3; d++)
+    for (int s = 0; s < 2; s++) {
+      if (kind[d][s] < 0) return;
+      op->m_faces.kind[d][s] = kind[d][s];
+    }
+  // the coefficients k_fill_ghosts forms for PK_CFHOMO (AMRPoissonOp.cpp:1774-1775)
+  const double dx = op->m_dx, dxCrse = op->m_dxCrse;
+  op->m_faces.c1 = 2.0 * (dxCrse - dx) / (dxCrse + dx);
+  op->m_faces.c2 = -((dxCrse - dx) / (dxCrse + 3.0 * dx));
+  op->m_fused = true;
+}
+
 template <class Real>
 std::unique_ptr<AMRPoissonOp<Real>> AMRPoissonOpFactory<Real>::AMRnewOp(int l, AMRPoissonOp<Real>* coarserOp) {
   std::unique_ptr<AMRPoissonOp<Real>> op(new AMRPoissonOp<Real>());
@@ -350,6 +390,7 @@ std::unique_ptr<AMRPoissonOp<Real>> AMRPoissonOpFactory<Real>::AMRnewOp(int l, A
   op->m_ghost = build_ghost_plan(H->grids[l], H->domains[l], bc, l > 0, H->amrLay[l]->merged);
   validate_ghost_plan(op->m_ghost, H->grids[l]);
   op->m_ghost.upload(H->ctx);
+  setup_fused(op.get());
   if (op->m_mgCoarse) validate_pair(*op->m_lay, *op->m_mgCoarse);
   if (l > 0) {
     op->m_coarserLay = H->amrLay[l - 1].get();
@@ -431,6 +472,7 @@ std::vector<std::unique_ptr<MGLevelOp<LevelData<Real>>>> AMRPoissonOpFactory<Rea
     op->m_ghost = build_ghost_plan(H->mgLay[d]->dbl, op->m_domain, bc, false, H->mgLay[d]->merged);
     validate_ghost_plan(op->m_ghost, H->mgLay[d]->dbl);
     op->m_ghost.upload(H->ctx);
+    setup_fused(op);
     if (op->m_mgCoarse) validate_pair(*op->m_lay, *op->m_mgCoarse);
     out.emplace_back(op);
   }

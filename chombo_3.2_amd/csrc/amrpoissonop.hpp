@@ -25,6 +25,11 @@ class AMRPoissonOp : public AMRLevelOp<LevelData<Real>> {
   double m_dx = 1, m_dxCrse = -1, m_alpha = 0, m_beta = 1;
   BCSpec m_bc{};
   ops::GhostPlan m_ghost;
+  // one-launch relaxation on merged levels (ops::gsrb_fused); m_relaxTmp is the
+  // second buffer it relaxes into, swapped with the relaxed field afterwards
+  bool m_fused = false;
+  ops::FaceRules m_faces{};
+  LevelData<Real> m_relaxTmp;
   // coarser AMR level (QuadCFInterp + restriction/prolongation copiers)
   const DeviceLayout* m_coarserLay = nullptr;
   const DeviceLayout* m_cfLay = nullptr;      // coarsen(this level, refToCoarser): resC layout

@@ -158,6 +158,11 @@ int amrmg_ctx_create(int device, amrmg_ctx** out) {
   *out = c;
   API_END
 }
+int amrmg_ctx_set_fused_relax(amrmg_ctx* ctx, int on) {
+  API_BEGIN
+  ctx->c->fused_relax = (on != 0);
+  API_END
+}
 int amrmg_ctx_destroy(amrmg_ctx* ctx) {
   API_BEGIN
   delete ctx;

@@ -110,6 +110,7 @@ struct Context {
   int device = 0;
   bool dry = false;                   // host-only: build and check tables, never touch a device
   Profiler* prof = nullptr;           // non-null while event profiling is on
+  bool fused_relax = true;            // one-launch GSRB iterations on merged levels
   hipStream_t stream = nullptr;
   double* d_scalars = nullptr;        // small device scratch for reductions
   double* h_scalars = nullptr;        // pinned host mirror

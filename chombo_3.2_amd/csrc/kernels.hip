@@ -423,6 +423,221 @@ void gsrb(const DeviceLayout& L, T* phi, const T* rhs, int color, double dx, dou
     LAUNCH_P(KC_GSRB, 12.0 * sizeof(T) / 8.0 * L.valid_cells, (k_gsrb<T, false>), L.ntiles, L.d_boxes, L.d_tiles, geom(L), phi, rhs, color, T(dxinv), T(lambda), T(alpha), T(beta));
 }
 
+// ---------------------------------------------------------------------------
+// One whole GSRB iteration in one pass over HBM, on a merged level.
+//
+// The reference's levelGSRB (AMRPoissonOp.cpp:1216-1281) runs, for each colour,
+// a ghost refresh (homogeneousCFInterp, exchange, homogeneous BC) and then
+// FORT_GSRBLAPLACIAN / FORT_GSRBHELMHOLTZ on the cells of that colour.  On a
+// merged level the exchange is the identity and each ghost cell outside the
+// rectangle is the face ghost of exactly one valid cell c, with a value that
+// depends only on c and on the cell one further inward, taken at the moment of
+// the refresh.  Those ghosts are read only when c itself is updated, so this
+// kernel forms them in registers at that moment, from the same values and with
+// the same expressions as k_fill_ghosts; every update therefore sees exactly the
+// operands the two-launch sequence gives it, and the result is bitwise the same.
+//
+// Blocking: a workgroup owns a 64 x 8 column of cells over a chunk of planes
+// and marches up in z.  Step k loads plane k+2 of the old values into one of
+// five LDS plane buffers (the column grown by 2 in x and y), updates the red
+// cells of plane k+1 on the column grown by 1 (the ring is recomputed by the
+// neighbouring workgroups too: every workgroup reads old values only), then the
+// black cells of plane k, and stores plane k-1 (both colours final) to dst.
+// Reads are old values from src, writes go to dst: workgroups never see each
+// other's results, so no ordering between them is needed.  HBM traffic is one
+// read of phi and rhs and one write of phi per cell (24 B in FP64).
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int FX = 64, FY = 8;                 // column of a workgroup (256 threads, 2 cells each)
+constexpr int LW = FX + 4, LH = FY + 4;        // LDS plane: the column grown by 2
+constexpr int LPLANE = LW * LH, NSLOT = 5;
+}
+struct FusedArgs {
+  int lo[3], hi[3];   // the level's rectangle
+  long long off;      // slab index of (lo-1, lo-1, lo-1)
+  int sy, sz;
+  int ntx, nty, kc, nwork;
+  int kind[3][2];     // face rule: 0 Neumann, 1 Dirichlet, 2 homogeneous quadratic CF
+};
+template <class T>
+__device__ __forceinline__ T face_ghost(int kind, T nearv, T farv, T c1, T c2) {
+  if (kind == 1) return T(2) * T(0.0) - nearv;  // PK_DIRICHLET with the homogeneous value
+  if (kind == 2) return c1 * nearv + c2 * farv;  // PK_CFHOMO
+  return nearv;                                  // PK_NEUMANN, homogeneous
+}
+template <class T, bool LAP>
+__device__ __forceinline__ void fused_update(const FusedArgs& a, T (*P)[LPLANE], int sm, int s0, int sp, int x0,
+                                             int y0, int i, int j, int k, T r, T dxinv, T lambda, T alpha, T beta,
+                                             T c1, T c2) {
+  const int c = (j - y0 + 2) * LW + (i - x0 + 2);
+  const T own = P[s0][c];
+  const T xp = (i == a.hi[0]) ? face_ghost(a.kind[0][1], own, P[s0][c - 1], c1, c2) : P[s0][c + 1];
+  const T xm = (i == a.lo[0]) ? face_ghost(a.kind[0][0], own, P[s0][c + 1], c1, c2) : P[s0][c - 1];
+  const T yp = (j == a.hi[1]) ? face_ghost(a.kind[1][1], own, P[s0][c - LW], c1, c2) : P[s0][c + LW];
+  const T ym = (j == a.lo[1]) ? face_ghost(a.kind[1][0], own, P[s0][c + LW], c1, c2) : P[s0][c - LW];
+  const T zp = (k == a.hi[2]) ? face_ghost(a.kind[2][1], own, P[sm][c], c1, c2) : P[sp][c];
+  const T zm = (k == a.lo[2]) ? face_ghost(a.kind[2][0], own, P[sp][c], c1, c2) : P[sm][c];
+  const T s6 = xp + xm + yp + ym + zp + zm;
+  if (LAP) {
+    const T lphi = s6 * dxinv;
+    P[s0][c] = lambda * (r - lphi);
+  } else {
+    const T lphi = (s6 - T(6.0) * own) * dxinv;
+    const T helmop = alpha * own + beta * lphi;
+    P[s0][c] = own + lambda * (helmop - r);
+  }
+}
+// Workgroup barrier for LDS traffic only: waits for this wave's LDS operations,
+// not for its global loads, so the next plane's loads stay in flight across it.
+// Global memory needs no ordering here (src and rhs are read-only, each dst cell
+// is written by one thread).
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+template <class T, bool LAP>
+__global__ void __launch_bounds__(256) k_gsrb_fused(FusedArgs a, T* __restrict__ dst, const T* __restrict__ src,
+                                                    const T* __restrict__ rhs, T dxinv, T lambda, T alpha, T beta,
+                                                    T c1, T c2) {
+  constexpr int NLD = (LPLANE + 255) / 256;                     // plane values per thread
+  constexpr int NRC = (FY + 2) * (FX / 2 + 1);                  // red cells of the column grown by 1
+  constexpr int NRED = (NRC + 255) / 256;
+  __shared__ T P[NSLOT][LPLANE];
+  // consecutive work items on one XCD (workgroups go round-robin over the 8 XCDs),
+  // so neighbouring columns share that XCD's L2 for their overlapping halos
+  const int w = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+  if (w >= a.nwork) return;
+  const int tx = w % a.ntx, ty = (w / a.ntx) % a.nty, ch = w / (a.ntx * a.nty);
+  const int x0 = a.lo[0] + tx * FX, y0 = a.lo[1] + ty * FY;
+  const int k0 = a.lo[2] + ch * a.kc;
+  const int k1 = min(k0 + a.kc, a.hi[2] + 1);
+  const int tid = threadIdx.x;
+  auto slot = [&](int p) { return (p - k0 + 2) % NSLOT; };
+  auto gidx = [&](int i, int j, int k) {
+    return a.off + (i - a.lo[0] + 1) + (long long)(j - a.lo[1] + 1) * a.sy + (long long)(k - a.lo[2] + 1) * a.sz;
+  };
+  // software pipeline: the old values of the next plane and the rhs of the next
+  // step's cells are loaded into registers one step ahead
+  T pphi[NLD], prr[NRED], prb;
+  auto fetch_phi = [&](int p) {
+    const bool zin = p >= a.lo[2] && p <= a.hi[2];
+#pragma unroll
+    for (int u = 0; u < NLD; u++) {
+      const int q = tid + 256 * u;
+      const int i = x0 - 2 + q % LW, j = y0 - 2 + q / LW;
+      const bool in = zin && q < LPLANE && i >= a.lo[0] && i <= a.hi[0] && j >= a.lo[1] && j <= a.hi[1];
+      pphi[u] = in ? src[gidx(i, j, p)] : T(0.0);
+    }
+  };
+  auto put_phi = [&](int p) {
+    T* S = P[slot(p)];
+#pragma unroll
+    for (int u = 0; u < NLD; u++) {
+      const int q = tid + 256 * u;
+      if (q < LPLANE) S[q] = pphi[u];
+    }
+  };
+  // red cells: (i + j + r) even, on the column grown by 1
+  auto red_cell = [&](int q, int r, int& i, int& j) {
+    const int row = q / (FX / 2 + 1), m = q % (FX / 2 + 1);
+    j = y0 - 1 + row;
+    i = x0 - 1 + ((x0 - 1 + j + r) & 1) + 2 * m;
+    return q < NRC && r >= a.lo[2] && r <= a.hi[2] && i >= a.lo[0] && i <= a.hi[0] && j >= a.lo[1] && j <= a.hi[1];
+  };
+  // black cells: (i + j + k) odd, on the column, one per thread
+  auto black_cell = [&](int k, int& i, int& j) {
+    const int row = tid / (FX / 2), m = tid % (FX / 2);
+    j = y0 + row;
+    i = x0 + ((x0 + j + k + 1) & 1) + 2 * m;
+    return k >= k0 && k < k1 && i <= a.hi[0] && j <= a.hi[1];
+  };
+  auto fetch_rhs = [&](int r, int k) {
+#pragma unroll
+    for (int u = 0; u < NRED; u++) {
+      int i, j;
+      prr[u] = red_cell(tid + 256 * u, r, i, j) ? rhs[gidx(i, j, r)] : T(0.0);
+    }
+    int i, j;
+    prb = black_cell(k, i, j) ? rhs[gidx(i, j, k)] : T(0.0);
+  };
+  auto store = [&](int p) {
+    const T* S = P[slot(p)];
+    for (int q = tid; q < FX * FY; q += 256) {
+      const int i = x0 + q % FX, j = y0 + q / FX;
+      if (i <= a.hi[0] && j <= a.hi[1]) dst[gidx(i, j, p)] = S[(j - y0 + 2) * LW + (i - x0 + 2)];
+    }
+  };
+  for (int p = k0 - 2; p < k0; p++) {
+    fetch_phi(p);
+    put_phi(p);
+  }
+  fetch_phi(k0);
+  fetch_rhs(k0 - 1, k0 - 2);
+  // step k: plane k+2 into LDS, red on plane k+1, black on plane k, store plane k-1
+  for (int k = k0 - 2; k < k1; k++) {
+    put_phi(k + 2);
+    T rr[NRED];
+#pragma unroll
+    for (int u = 0; u < NRED; u++) rr[u] = prr[u];
+    const T rb = prb;
+    lds_barrier();
+    if (k + 3 <= k1 + 1) fetch_phi(k + 3);
+    fetch_rhs(k + 2, k + 1);
+    if (k - 1 >= k0) store(k - 1);
+#pragma unroll
+    for (int u = 0; u < NRED; u++) {
+      int i, j;
+      if (red_cell(tid + 256 * u, k + 1, i, j))
+        fused_update<T, LAP>(a, P, slot(k), slot(k + 1), slot(k + 2), x0, y0, i, j, k + 1, rr[u], dxinv, lambda,
+                             alpha, beta, c1, c2);
+    }
+    lds_barrier();
+    int i, j;
+    if (black_cell(k, i, j))
+      fused_update<T, LAP>(a, P, slot(k - 1), slot(k), slot(k + 1), x0, y0, i, j, k, rb, dxinv, lambda, alpha, beta,
+                           c1, c2);
+  }
+  lds_barrier();
+  store(k1 - 1);
+}
+template <class T>
+void gsrb_fused(const DeviceLayout& L, const FaceRules& fr, T* dst, const T* src, const T* rhs, double dx,
+                double alpha, double beta) {
+  if (!L.merged) throw std::runtime_error("gsrb_fused: level is not stored merged");
+  if (dst == src || dst == rhs) throw std::runtime_error("gsrb_fused: dst must be a separate field");
+  const DBox& R = L.h_boxes[L.nbox];
+  FusedArgs a;
+  for (int q = 0; q < 3; q++) { a.lo[q] = R.lo[q]; a.hi[q] = R.hi[q]; }
+  for (int d = 0; d < 3; d++)
+    for (int s = 0; s < 2; s++) {
+      if (fr.kind[d][s] < 0 || fr.kind[d][s] > 2) throw std::runtime_error("gsrb_fused: bad face rule");
+      a.kind[d][s] = fr.kind[d][s];
+    }
+  a.off = R.off;
+  a.sy = R.sy;
+  a.sz = R.sz;
+  a.ntx = (L.rect.len(0) + FX - 1) / FX;
+  a.nty = (L.rect.len(1) + FY - 1) / FY;
+  const int nz = L.rect.len(2);
+  // z-chunks: long enough to amortise the 2-plane prologue, enough of them to fill the chip
+  a.kc = 32;
+  while (a.kc > 4 && (long long)a.ntx * a.nty * ((nz + a.kc - 1) / a.kc) < 2048) a.kc >>= 1;
+  const long long nwork = (long long)a.ntx * a.nty * ((nz + a.kc - 1) / a.kc);
+  if (nwork > INT32_MAX - 8) throw std::runtime_error("gsrb_fused: level too large");
+  a.nwork = (int)nwork;
+  const int grid = (a.nwork + 7) / 8 * 8;
+  double dxinv = 1.0 / (dx * dx);
+  double sum_b = 0.0;
+  for (int d = 0; d < 3; d++) sum_b = sum_b + 2.0 * dxinv;
+  const bool lap = (alpha == 0.0 && beta == 1.0);
+  double lambda = lap ? -1.0 / sum_b : -1.0 / (alpha - beta * sum_b);
+  const T c1 = T(fr.c1), c2 = T(fr.c2);
+  if (lap)
+    LAUNCH_P(KC_GSRB, 24.0 * sizeof(T) / 8.0 * L.valid_cells, (k_gsrb_fused<T, true>), grid, a, dst, src, rhs,
+             T(dxinv), T(lambda), T(alpha), T(beta), c1, c2);
+  else
+    LAUNCH_P(KC_GSRB, 24.0 * sizeof(T) / 8.0 * L.valid_cells, (k_gsrb_fused<T, false>), grid, a, dst, src, rhs,
+             T(dxinv), T(lambda), T(alpha), T(beta), c1, c2);
+}
+
 // OPERATORLAP (AMRPoissonOpF.ChF:287-324)
 template <class T>
 __global__ void __launch_bounds__(256) k_apply(const DBox* __restrict__ boxes, const Tile* __restrict__ tiles,
@@ -967,6 +1182,8 @@ void reflux(Context* ctx, const RefluxCell* cells, int ncells, const RefluxCoars
 #define INST(T)                                                                                                   \
   template void fill_ghosts<T>(const DeviceLayout&, const GhostPlan&, T*, int, int, double, double);           \
   template void gsrb<T>(const DeviceLayout&, T*, const T*, int, double, double, double);                       \
+  template void gsrb_fused<T>(const DeviceLayout&, const FaceRules&, T*, const T*, const T*, double, double,      \
+                              double);                                                                        \
   template void apply<T>(const DeviceLayout&, T*, const T*, double, double, double);                           \
   template void residual<T>(const DeviceLayout&, T*, const T*, const T*, double, double, double);              \
   template void restrict_res<T>(const DeviceLayout&, const DeviceLayout&, T*, const T*, const T*, double, double, \

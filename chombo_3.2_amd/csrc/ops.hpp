@@ -41,6 +41,18 @@ void fill_ghosts(const DeviceLayout& L, const GhostPlan& P, T* phi, int doCF, in
                  double dxCrse);
 template <class T>
 void gsrb(const DeviceLayout& L, T* phi, const T* rhs, int color, double dx, double alpha, double beta);
+
+// Ghost rule of each face of a merged level's rectangle during relaxation
+// (homogeneous): 0 Neumann, 1 Dirichlet, 2 quadratic coarse-fine with c1, c2.
+struct FaceRules {
+  int kind[3][2];
+  double c1 = 0, c2 = 0;
+};
+// One GSRB iteration (red, refresh, black) of a merged level in one launch:
+// dst <- relaxed src.  Bitwise equal to fill_ghosts + gsrb(0) + fill_ghosts + gsrb(1).
+template <class T>
+void gsrb_fused(const DeviceLayout& L, const FaceRules& fr, T* dst, const T* src, const T* rhs, double dx,
+                double alpha, double beta);
 template <class T>
 void apply(const DeviceLayout& L, T* lofphi, const T* phi, double dx, double alpha, double beta);
 template <class T>

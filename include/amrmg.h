@@ -49,6 +49,12 @@ const char* amrmg_last_error(void);
 int amrmg_ctx_create(int device, amrmg_ctx** out);
 int amrmg_ctx_destroy(amrmg_ctx* ctx);
 
+/* Relaxation on levels stored as one rectangle: 1 (default) runs each GSRB
+   iteration as one launch that forms the coarse-fine and boundary ghosts of both
+   colours in registers; 0 runs the reference's sequence (ghost refresh, red,
+   ghost refresh, black) as four launches.  Results are bitwise the same. */
+int amrmg_ctx_set_fused_relax(amrmg_ctx* ctx, int on);
+
 /* AMRPoissonOpFactory::define + AMRMultiGrid::define + init with a BiCGStab
    bottom solver (replaces AMRPoissonOp.cpp:1857-2070, AMRMultiGrid.H:1212-1328) */
 int amrmg_solver_create(amrmg_ctx* ctx, const int* ispec, const double* rspec, int precision,

@@ -143,9 +143,13 @@ def _solve_pair(gpu_ctx, h, rhs, **kw):
     return phi_g, hist_g, info_g, phi_o, hist_o, info_o
 
 
-@pytest.mark.parametrize("case", ["single32", "amr32", "helm_single24", "amr_lshape", "helm_lshape"])
+@pytest.mark.parametrize("case", ["single32", "amr32", "helm_single24", "amr_lshape", "helm_lshape", "amr64"])
 def test_solve_matches_oracle(gpu_ctx, case):
-    if case == "amr_lshape":
+    if case == "amr64":
+        # the benchmark's hierarchy at base 64: 64^3, 80^3 and 100^3 levels, 32^3 boxes
+        h = H.amr_hierarchy(64, fracs=(0.25, 0.06), max_box=32, block=8)
+        rhs = H.bump_rhs(h, 1002)
+    elif case == "amr_lshape":
         h = lshape_amr()
         rhs = H.bump_rhs(h, 1002)
     elif case == "helm_lshape":
@@ -189,3 +193,35 @@ def test_solve_linearity_and_determinism_full_c1(gpu_ctx):
     np.testing.assert_array_equal(g.get_field(g.PHI), 2 * phi1)
     assert hist1[-1] <= 1e-10 * hist1[0]
     assert info1["exit_status"] & 1
+
+
+@pytest.mark.parametrize("case", ["amr64", "helm_amr32", "lshape"])
+def test_fused_relax_matches_four_launch_sequence(gpu_ctx, case):
+    """The one-launch GSRB iteration against the reference's launch sequence
+    (ghost refresh, red, ghost refresh, black), both on the GPU: whole solves
+    must agree bitwise, history and solution."""
+    if case == "amr64":
+        h = H.amr_hierarchy(64, fracs=(0.25, 0.06), max_box=32, block=8)
+    elif case == "helm_amr32":
+        h = small_amr(alpha=0.3, beta=0.8)
+    else:
+        h = lshape_amr()
+    rhs = H.bump_rhs(h, 1005)
+    sp, rp = H.solver_params()
+    out = []
+    try:
+        for fused in (True, False):
+            gpu_ctx.set_fused_relax(fused)
+            g = amrmg.AMRMultiGrid(gpu_ctx, h)
+            g.setSolverParameters(sp, rp)
+            g.set_field(g.RHS, rhs)
+            hist, info = g.solve()
+            out.append((hist, info, g.get_field(g.PHI)))
+            g.close()
+    finally:
+        gpu_ctx.set_fused_relax(True)
+    (h1, i1, p1), (h2, i2, p2) = out
+    assert i1 == i2
+    np.testing.assert_array_equal(h1, h2)
+    np.testing.assert_array_equal(p1, p2)
+    assert h1[-1] <= 1e-10 * h1[0]
