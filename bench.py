@@ -72,7 +72,7 @@ def summarize_pmc(fetch_csv, write_csv, fetch_json, copy_bytes, out_path):
     meas_total = sum(gf) * fscale + sum(gw) * wscale
     ratio = meas_total / alg_total
     d = {
-        "kernel": "k_gsrb (GSRB colour sweep)",
+        "kernel": "GSRB relaxation class (k_gsrb_fused, k_gsrb)",
         "fetch_scale": round(fscale, 4), "write_scale": round(wscale, 4),
         "copy_bytes": copy_bytes,
         "gsrb_dispatches": len(gf),
@@ -117,6 +117,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--iter-max", type=int, default=100,
                     help="cap on V-cycles per solve (a short cap is for profiling runs only)")
+    ap.add_argument("--no-tune", action="store_true", help="skip the relaxation autotuner (default shapes)")
     ap.add_argument("--copy-mb", type=int, default=2048,
                     help="size of the plain copy kernel run after the timed region (0 = skip)")
     ap.add_argument("--summarize-pmc", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "FETCH_RUN_JSON"),
@@ -149,6 +150,13 @@ def main():
     solver.set_field(solver.RHS, rhs)
     log(f"solver set up in {time.time() - t0:.1f}s (device bytes ~{solver.device_bytes() / 1e9:.1f} GB)")
 
+    tune = None
+    if not args.no_tune:
+        t0 = time.time()
+        tune = solver.tune_relax()
+        log(f"relaxation tuned in {time.time() - t0:.2f}s: " +
+            ", ".join(f"{r['kind']}{r['index']} {r['shape'][0]}x{r['shape'][1]}/kc{r['kc']} {r['ms']:.3f}ms"
+                      for r in tune))
     for w in range(args.warmup):
         hist, info = solver.solve()
         log(f"warmup {w}: {info} residual {hist[0]:.3e} -> {hist[-1]:.3e}")
@@ -221,12 +229,14 @@ def main():
             "frac": round(achieved / PEAK_HBM_GBS, 4),
             "traffic": traffic,
             "traffic_over_algorithmic": ratio,
-            "kernel": "GSRB colour sweep (all boxes of a level per launch)",
+            "kernel": "GSRB relaxation class: k_gsrb_fused (one red+black iteration of a whole level per launch)",
             "avg_launch_ms": round(avg_launch_ms, 5),
             "algorithmic_bytes_per_launch": round(bytes_per_launch, 1),
             "launches": prof["launches"],
             "measured_copy_GBs": round(copy_gbs, 1) if copy_gbs else None,
         },
+        "relax_tiles": ({f"{r['kind']}{r['index']}": f"{r['shape'][0]}x{r['shape'][1]}/kc{r['kc']}" for r in tune}
+                        if tune else None),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -50,6 +50,8 @@ def lib():
         L.amrmg_solver_mg_cells.argtypes = [C.c_void_p, C.c_int]
         L.amrmg_solver_device_bytes.restype = C.c_longlong
         L.amrmg_solver_device_bytes.argtypes = [C.c_void_p]
+        L.amrmg_solver_tune_relax.argtypes = [C.c_void_p, _dp, C.c_int, _ip]
+        L.amrmg_solver_set_relax_shape.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.amrmg_solver_set_field.argtypes = [C.c_void_p, C.c_int, _dp]
         L.amrmg_solver_get_field.argtypes = [C.c_void_p, C.c_int, _dp]
         L.amrmg_solver_solve.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _ip]
@@ -180,6 +182,30 @@ class AMRMultiGrid:
 
     def device_bytes(self):
         return lib().amrmg_solver_device_bytes(self.p)
+
+    RELAX_SHAPES = [(64, 8, 256), (32, 8, 256), (64, 4, 256), (32, 16, 256), (128, 4, 256), (64, 16, 256),
+                    (32, 16, 512), (64, 16, 512), (64, 32, 1024), (128, 8, 1024)]
+    RELAX_CHUNKS = [0, 8, 16, 32]
+
+    def tune_relax(self):
+        """Autotunes the one-launch relaxation on every level that uses it; returns one
+        dict per tuned level (kind 'amr' or 'mg', index, cells, chosen shape and
+        z-chunk, ms per iteration, all timings [shape][chunk])."""
+        cap = 64
+        w = 6 + len(self.RELAX_SHAPES) * len(self.RELAX_CHUNKS)
+        tab = np.zeros(cap * w)
+        n = C.c_int()
+        _chk(lib().amrmg_solver_tune_relax(self.p, tab.ctypes.data_as(_dp), cap, C.byref(n)))
+        out = []
+        for r in tab[:min(n.value, cap) * w].reshape(-1, w):
+            out.append(dict(kind="amr" if r[0] == 0 else "mg", index=int(r[1]), cells=int(r[2]),
+                            shape=self.RELAX_SHAPES[int(r[3])], kc=int(r[4]), ms=float(r[5]),
+                            times=r[6:].reshape(len(self.RELAX_SHAPES), len(self.RELAX_CHUNKS)).tolist()))
+        return out
+
+    def set_relax_shape(self, shape: int, kc: int = 0):
+        """Forces tile shape index (into RELAX_SHAPES) and z-chunk (0 = automatic)."""
+        _chk(lib().amrmg_solver_set_relax_shape(self.p, int(shape), int(kc)))
 
     def setSolverParameters(self, sparams, rparams):
         sp = np.ascontiguousarray(sparams, dtype=np.int32)

@@ -151,7 +151,8 @@ void AMRPoissonOp<Real>::levelGSRB(T& phi, const T& rhs) {
   if (m_fused && m_ctx->fused_relax) {
     need(phi.lay, m_lay, "levelGSRB");
     if (!m_relaxTmp.isDefined()) m_relaxTmp.define(m_lay);
-    ops::gsrb_fused<Real>(*m_lay, m_faces, m_relaxTmp.d, phi.d, rhs.d, m_dx, m_alpha, m_beta);
+    ops::gsrb_fused<Real>(*m_lay, m_faces, m_relaxTmp.d, phi.d, rhs.d, m_dx, m_alpha, m_beta, m_relaxShape,
+                          m_relaxKc);
     std::swap(phi.d, m_relaxTmp.d);
     return;
   }
@@ -159,6 +160,17 @@ void AMRPoissonOp<Real>::levelGSRB(T& phi, const T& rhs) {
     fillGhosts(phi, true, true);
     ops::gsrb<Real>(*m_lay, phi.d, rhs.d, pass, m_dx, m_alpha, m_beta);
   }
+}
+template <class Real>
+bool AMRPoissonOp<Real>::tuneRelax(const T& phi, const T& rhs, double* times) {
+  if (!m_fused) return false;
+  need(phi.lay, m_lay, "tuneRelax"); need(rhs.lay, m_lay, "tuneRelax");
+  if (!m_relaxTmp.isDefined()) m_relaxTmp.define(m_lay);
+  const std::pair<int, int> pick =
+      ops::tune_gsrb_fused<Real>(*m_lay, m_faces, m_relaxTmp.d, phi.d, rhs.d, m_dx, m_alpha, m_beta, times);
+  m_relaxShape = pick.first;
+  m_relaxKc = pick.second;
+  return true;
 }
 template <class Real>
 void AMRPoissonOp<Real>::relax(T& e, const T& r, int n) {

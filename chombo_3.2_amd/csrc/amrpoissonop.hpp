@@ -29,6 +29,7 @@ class AMRPoissonOp : public AMRLevelOp<LevelData<Real>> {
   // second buffer it relaxes into, swapped with the relaxed field afterwards
   bool m_fused = false;
   ops::FaceRules m_faces{};
+  int m_relaxShape = 0, m_relaxKc = 0;        // tile shape and z-chunk (0: automatic), see tuneRelax
   LevelData<Real> m_relaxTmp;
   // coarser AMR level (QuadCFInterp + restriction/prolongation copiers)
   const DeviceLayout* m_coarserLay = nullptr;
@@ -93,6 +94,10 @@ class AMRPoissonOp : public AMRLevelOp<LevelData<Real>> {
   void coarseFineInterp(T& phif, const T& phic);   // QuadCFInterp of this level from the coarser one
   void reflux(T& phiFine, T& phi, T& res, AMRPoissonOp* finerOp);
   void localMaxNormAsync(const T& x, int slot, bool accumulate);
+  // Autotuner: times the one-launch relaxation's tile shapes on (phi, rhs) of this
+  // level and keeps the fastest.  No-op (returns false) when the level relaxes
+  // with the four-launch sequence.  times: kFusedShapeCount x 4 ms values.
+  bool tuneRelax(const T& phi, const T& rhs, double* times);
 };
 
 // Device layouts of one AMR hierarchy: AMR levels, the multigrid levels of level

@@ -404,6 +404,40 @@ int amrmg_bottom_solve(amrmg_solver* s, double* corr, const double* res, int nbo
   API_END
 }
 
+int amrmg_solver_tune_relax(amrmg_solver* s, double* table, int cap, int* nrec) {
+  API_BEGIN
+  constexpr int W = 6 + ops::kFusedShapeCount * 4;
+  int n = 0;
+  auto rec = [&](int kind, int idx, AMRPoissonOp<double>* op, LevelData<double>& phi, LevelData<double>& rhs) {
+    double times[ops::kFusedShapeCount * 4];
+    if (!op->tuneRelax(phi, rhs, times)) return;
+    if (table && n < cap) {
+      double* r = table + (size_t)n * W;
+      r[0] = kind;
+      r[1] = idx;
+      r[2] = (double)op->m_lay->valid_cells;
+      r[3] = op->m_relaxShape;
+      r[4] = op->m_relaxKc;
+      r[5] = times[op->m_relaxShape * 4 + (op->m_relaxKc == 0 ? 0 : op->m_relaxKc == 8 ? 1 : op->m_relaxKc == 16 ? 2 : 3)];
+      for (int q = 0; q < ops::kFusedShapeCount * 4; q++) r[6 + q] = times[q];
+    }
+    n++;
+  };
+  for (int l = 0; l < s->nlev(); l++) rec(0, l, s->amg.m_op[l].get(), *s->phi[l], *s->rhs[l]);
+  for (int d = 1; d < s->H.mgDepth(); d++)
+    rec(1, d, s->mgop(d), *s->amg.m_mg0->m_correction[d], *s->amg.m_mg0->m_residual[d]);
+  s->ctx->c->sync();
+  if (nrec) *nrec = n;
+  API_END
+}
+int amrmg_solver_set_relax_shape(amrmg_solver* s, int shape, int kc) {
+  API_BEGIN
+  if (shape < 0 || shape >= ops::kFusedShapeCount || kc < 0 || kc > 1024)
+    throw std::runtime_error("relax shape out of range");
+  for (int l = 0; l < s->nlev(); l++) { s->amg.m_op[l]->m_relaxShape = shape; s->amg.m_op[l]->m_relaxKc = kc; }
+  for (int d = 1; d < s->H.mgDepth(); d++) { s->mgop(d)->m_relaxShape = shape; s->mgop(d)->m_relaxKc = kc; }
+  API_END
+}
 int amrmg_copy_probe(amrmg_ctx* ctx, long long nbytes, int iters, double* ms_out) {
   API_BEGIN
   if (nbytes <= 0 || iters <= 0) throw std::runtime_error("copy probe: nbytes and iters must be positive");

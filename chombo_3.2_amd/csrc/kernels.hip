@@ -447,11 +447,8 @@ void gsrb(const DeviceLayout& L, T* phi, const T* rhs, int color, double dx, dou
 // other's results, so no ordering between them is needed.  HBM traffic is one
 // read of phi and rhs and one write of phi per cell (24 B in FP64).
 // ---------------------------------------------------------------------------
-namespace {
-constexpr int FX = 64, FY = 8;                 // column of a workgroup (256 threads, 2 cells each)
-constexpr int LW = FX + 4, LH = FY + 4;        // LDS plane: the column grown by 2
-constexpr int LPLANE = LW * LH, NSLOT = 5;
-}
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+constexpr int NSLOT = 5;  // LDS plane buffers: planes k-1 .. k+2 in use plus the one being filled
 struct FusedArgs {
   int lo[3], hi[3];   // the level's rectangle
   long long off;      // slab index of (lo-1, lo-1, lo-1)
@@ -461,45 +458,57 @@ struct FusedArgs {
 };
 template <class T>
 __device__ __forceinline__ T face_ghost(int kind, T nearv, T farv, T c1, T c2) {
-  if (kind == 1) return T(2) * T(0.0) - nearv;  // PK_DIRICHLET with the homogeneous value
-  if (kind == 2) return c1 * nearv + c2 * farv;  // PK_CFHOMO
-  return nearv;                                  // PK_NEUMANN, homogeneous
+  // all three rules are formed and one is selected, so no LDS read sits in a branch
+  const T gd = T(2) * T(0.0) - nearv;  // PK_DIRICHLET with the homogeneous value
+  const T gc = c1 * nearv + c2 * farv;  // PK_CFHOMO
+  return kind == 1 ? gd : (kind == 2 ? gc : nearv);  // else PK_NEUMANN, homogeneous
 }
-template <class T, bool LAP>
-__device__ __forceinline__ void fused_update(const FusedArgs& a, T (*P)[LPLANE], int sm, int s0, int sp, int x0,
-                                             int y0, int i, int j, int k, T r, T dxinv, T lambda, T alpha, T beta,
-                                             T c1, T c2) {
-  const int c = (j - y0 + 2) * LW + (i - x0 + 2);
-  const T own = P[s0][c];
-  const T xp = (i == a.hi[0]) ? face_ghost(a.kind[0][1], own, P[s0][c - 1], c1, c2) : P[s0][c + 1];
-  const T xm = (i == a.lo[0]) ? face_ghost(a.kind[0][0], own, P[s0][c + 1], c1, c2) : P[s0][c - 1];
-  const T yp = (j == a.hi[1]) ? face_ghost(a.kind[1][1], own, P[s0][c - LW], c1, c2) : P[s0][c + LW];
-  const T ym = (j == a.lo[1]) ? face_ghost(a.kind[1][0], own, P[s0][c + LW], c1, c2) : P[s0][c - LW];
-  const T zp = (k == a.hi[2]) ? face_ghost(a.kind[2][1], own, P[sm][c], c1, c2) : P[sp][c];
-  const T zm = (k == a.lo[2]) ? face_ghost(a.kind[2][0], own, P[sp][c], c1, c2) : P[sm][c];
+// new value of cell (i,j,k); S0 is its plane, Sm / Sp the planes below / above,
+// c its LDS index, LW the LDS row length
+template <class T, bool LAP, int LW>
+__device__ __forceinline__ T fused_new(const FusedArgs& a, const T* Sm, const T* S0, const T* Sp, int c, int i, int j,
+                                       int k, T r, T dxinv, T lambda, T alpha, T beta, T c1, T c2) {
+  // the seven reads are unconditional (all inside the LDS planes); a face ghost's
+  // inner operand is the opposite neighbour
+  const T own = S0[c];
+  const T vxp = S0[c + 1], vxm = S0[c - 1];
+  const T vyp = S0[c + LW], vym = S0[c - LW];
+  const T vzp = Sp[c], vzm = Sm[c];
+  const T xp = (i == a.hi[0]) ? face_ghost(a.kind[0][1], own, vxm, c1, c2) : vxp;
+  const T xm = (i == a.lo[0]) ? face_ghost(a.kind[0][0], own, vxp, c1, c2) : vxm;
+  const T yp = (j == a.hi[1]) ? face_ghost(a.kind[1][1], own, vym, c1, c2) : vyp;
+  const T ym = (j == a.lo[1]) ? face_ghost(a.kind[1][0], own, vyp, c1, c2) : vym;
+  const T zp = (k == a.hi[2]) ? face_ghost(a.kind[2][1], own, vzm, c1, c2) : vzp;
+  const T zm = (k == a.lo[2]) ? face_ghost(a.kind[2][0], own, vzp, c1, c2) : vzm;
   const T s6 = xp + xm + yp + ym + zp + zm;
   if (LAP) {
     const T lphi = s6 * dxinv;
-    P[s0][c] = lambda * (r - lphi);
+    return lambda * (r - lphi);
   } else {
     const T lphi = (s6 - T(6.0) * own) * dxinv;
     const T helmop = alpha * own + beta * lphi;
-    P[s0][c] = own + lambda * (helmop - r);
+    return own + lambda * (helmop - r);
   }
 }
 // Workgroup barrier for LDS traffic only: waits for this wave's LDS operations,
-// not for its global loads, so the next plane's loads stay in flight across it.
+// not for its global loads, so the next planes' loads stay in flight across it.
 // Global memory needs no ordering here (src and rhs are read-only, each dst cell
 // is written by one thread).
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
-template <class T, bool LAP>
-__global__ void __launch_bounds__(256) k_gsrb_fused(FusedArgs a, T* __restrict__ dst, const T* __restrict__ src,
+// FX x FY: the column of cells of one NT-thread workgroup (FX even)
+template <class T, bool LAP, int FX, int FY, int NT>
+__global__ void __launch_bounds__(NT) k_gsrb_fused(FusedArgs a, T* __restrict__ dst, const T* __restrict__ src,
                                                     const T* __restrict__ rhs, T dxinv, T lambda, T alpha, T beta,
                                                     T c1, T c2) {
-  constexpr int NLD = (LPLANE + 255) / 256;                     // plane values per thread
-  constexpr int NRC = (FY + 2) * (FX / 2 + 1);                  // red cells of the column grown by 1
-  constexpr int NRED = (NRC + 255) / 256;
+  constexpr int LW = FX + 4, LH = FY + 4, LPLANE = LW * LH;  // LDS plane: the column grown by 2
+  constexpr int NLD = (LPLANE + NT - 1) / NT;                   // plane values per thread
+  constexpr int NRC = (FY + 2) * (FX / 2 + 1);                // red cells of the column grown by 1
+  constexpr int NRED = (NRC + NT - 1) / NT;
+  constexpr int NBC = FX * FY / 2;                            // black cells of the column
+  constexpr int NBLK = (NBC + NT - 1) / NT;
+  constexpr int NST = (FX * FY + NT - 1) / NT;
+  static_assert(FX % 2 == 0, "column width must be even");
   __shared__ T P[NSLOT][LPLANE];
   // consecutive work items on one XCD (workgroups go round-robin over the 8 XCDs),
   // so neighbouring columns share that XCD's L2 for their overlapping halos
@@ -514,95 +523,168 @@ __global__ void __launch_bounds__(256) k_gsrb_fused(FusedArgs a, T* __restrict__
   auto gidx = [&](int i, int j, int k) {
     return a.off + (i - a.lo[0] + 1) + (long long)(j - a.lo[1] + 1) * a.sy + (long long)(k - a.lo[2] + 1) * a.sz;
   };
-  // software pipeline: the old values of the next plane and the rhs of the next
-  // step's cells are loaded into registers one step ahead
-  T pphi[NLD], prr[NRED], prb;
-  auto fetch_phi = [&](int p) {
-    const bool zin = p >= a.lo[2] && p <= a.hi[2];
+  auto lidx = [&](int i, int j) { return (j - y0 + 2) * LW + (i - x0 + 2); };
+  // red cell q of plane r (on the column grown by 1, (i + j + r) even); black cell q
+  // of plane k (on the column, (i + j + k) odd)
+  auto red_ij = [&](int q, int r, int& i, int& j) {
+    j = y0 - 1 + q / (FX / 2 + 1);
+    i = x0 - 1 + ((x0 - 1 + j + r) & 1) + 2 * (q % (FX / 2 + 1));
+  };
+  auto black_ij = [&](int q, int k, int& i, int& j) {
+    j = y0 + q / (FX / 2);
+    i = x0 + ((x0 + j + k + 1) & 1) + 2 * (q % (FX / 2));
+  };
+  // software pipeline, two steps deep: the old values of plane k+4 and the rhs of
+  // the cells of step k+2 are loaded into registers during step k, into one of
+  // two register sets used alternately (no register moves between the sets, so
+  // each step waits only for the loads issued two steps before).
+  // Every load is unconditional from an address clamped into the rectangle, and
+  // its value is used as loaded: a value at a position outside the rectangle (or
+  // of a cell that is not updated) is never selected into a result.  A select at
+  // load time would make the wave wait for the load there, ending the pipeline.
+  struct Pre {
+    T phi[NLD], rr[NRED], rb[NBLK];
+  };
+  Pre A, B;
+  auto fetch = [&](Pre& f, int p, int r, int k) {  // plane p; red cells of plane r; black cells of plane k
+    const int pc = clampi(p, a.lo[2], a.hi[2]);
 #pragma unroll
     for (int u = 0; u < NLD; u++) {
-      const int q = tid + 256 * u;
+      const int q = tid + NT * u;
       const int i = x0 - 2 + q % LW, j = y0 - 2 + q / LW;
-      const bool in = zin && q < LPLANE && i >= a.lo[0] && i <= a.hi[0] && j >= a.lo[1] && j <= a.hi[1];
-      pphi[u] = in ? src[gidx(i, j, p)] : T(0.0);
+      f.phi[u] = src[gidx(clampi(i, a.lo[0], a.hi[0]), clampi(j, a.lo[1], a.hi[1]), pc)];
+    }
+    const int rc = clampi(r, a.lo[2], a.hi[2]);
+#pragma unroll
+    for (int u = 0; u < NRED; u++) {
+      int i, j;
+      red_ij(tid + NT * u, r, i, j);
+      f.rr[u] = rhs[gidx(clampi(i, a.lo[0], a.hi[0]), clampi(j, a.lo[1], a.hi[1]), rc)];
+    }
+    const int kc = clampi(k, a.lo[2], a.hi[2]);
+#pragma unroll
+    for (int u = 0; u < NBLK; u++) {
+      int i, j;
+      black_ij(tid + NT * u, k, i, j);
+      f.rb[u] = rhs[gidx(clampi(i, a.lo[0], a.hi[0]), clampi(j, a.lo[1], a.hi[1]), kc)];
     }
   };
-  auto put_phi = [&](int p) {
+  auto put_phi = [&](const Pre& f, int p) {
     T* S = P[slot(p)];
 #pragma unroll
     for (int u = 0; u < NLD; u++) {
-      const int q = tid + 256 * u;
-      if (q < LPLANE) S[q] = pphi[u];
+      const int q = tid + NT * u;
+      if (q < LPLANE) S[q] = f.phi[u];
     }
-  };
-  // red cells: (i + j + r) even, on the column grown by 1
-  auto red_cell = [&](int q, int r, int& i, int& j) {
-    const int row = q / (FX / 2 + 1), m = q % (FX / 2 + 1);
-    j = y0 - 1 + row;
-    i = x0 - 1 + ((x0 - 1 + j + r) & 1) + 2 * m;
-    return q < NRC && r >= a.lo[2] && r <= a.hi[2] && i >= a.lo[0] && i <= a.hi[0] && j >= a.lo[1] && j <= a.hi[1];
-  };
-  // black cells: (i + j + k) odd, on the column, one per thread
-  auto black_cell = [&](int k, int& i, int& j) {
-    const int row = tid / (FX / 2), m = tid % (FX / 2);
-    j = y0 + row;
-    i = x0 + ((x0 + j + k + 1) & 1) + 2 * m;
-    return k >= k0 && k < k1 && i <= a.hi[0] && j <= a.hi[1];
-  };
-  auto fetch_rhs = [&](int r, int k) {
-#pragma unroll
-    for (int u = 0; u < NRED; u++) {
-      int i, j;
-      prr[u] = red_cell(tid + 256 * u, r, i, j) ? rhs[gidx(i, j, r)] : T(0.0);
-    }
-    int i, j;
-    prb = black_cell(k, i, j) ? rhs[gidx(i, j, k)] : T(0.0);
   };
   auto store = [&](int p) {
     const T* S = P[slot(p)];
-    for (int q = tid; q < FX * FY; q += 256) {
+    T v[NST];
+#pragma unroll
+    for (int u = 0; u < NST; u++) {
+      const int q = min(tid + NT * u, FX * FY - 1);
+      v[u] = S[(q / FX + 2) * LW + (q % FX + 2)];
+    }
+#pragma unroll
+    for (int u = 0; u < NST; u++) {
+      const int q = tid + NT * u;
       const int i = x0 + q % FX, j = y0 + q / FX;
-      if (i <= a.hi[0] && j <= a.hi[1]) dst[gidx(i, j, p)] = S[(j - y0 + 2) * LW + (i - x0 + 2)];
+      if (q < FX * FY && i <= a.hi[0] && j <= a.hi[1]) dst[gidx(i, j, p)] = v[u];
     }
   };
-  for (int p = k0 - 2; p < k0; p++) {
-    fetch_phi(p);
-    put_phi(p);
-  }
-  fetch_phi(k0);
-  fetch_rhs(k0 - 1, k0 - 2);
   // step k: plane k+2 into LDS, red on plane k+1, black on plane k, store plane k-1
-  for (int k = k0 - 2; k < k1; k++) {
-    put_phi(k + 2);
-    T rr[NRED];
+  auto step = [&](Pre& f, int k) {
+    put_phi(f, k + 2);
+    T rr[NRED], rb[NBLK];
 #pragma unroll
-    for (int u = 0; u < NRED; u++) rr[u] = prr[u];
-    const T rb = prb;
+    for (int u = 0; u < NRED; u++) rr[u] = f.rr[u];
+#pragma unroll
+    for (int u = 0; u < NBLK; u++) rb[u] = f.rb[u];
     lds_barrier();
-    if (k + 3 <= k1 + 1) fetch_phi(k + 3);
-    fetch_rhs(k + 2, k + 1);
+    fetch(f, k + 4, k + 3, k + 2);
     if (k - 1 >= k0) store(k - 1);
+    const int r = k + 1;
+    if (r >= a.lo[2] && r <= a.hi[2]) {
 #pragma unroll
-    for (int u = 0; u < NRED; u++) {
-      int i, j;
-      if (red_cell(tid + 256 * u, k + 1, i, j))
-        fused_update<T, LAP>(a, P, slot(k), slot(k + 1), slot(k + 2), x0, y0, i, j, k + 1, rr[u], dxinv, lambda,
-                             alpha, beta, c1, c2);
+      for (int u = 0; u < NRED; u++) {
+        const int q = tid + NT * u;
+        int i, j;
+        red_ij(q, r, i, j);
+        if (q < NRC && i >= a.lo[0] && i <= a.hi[0] && j >= a.lo[1] && j <= a.hi[1]) {
+          const int c = lidx(i, j);
+          P[slot(r)][c] = fused_new<T, LAP, LW>(a, P[slot(r - 1)], P[slot(r)], P[slot(r + 1)], c, i, j, r, rr[u],
+                                                dxinv, lambda, alpha, beta, c1, c2);
+        }
+      }
     }
     lds_barrier();
-    int i, j;
-    if (black_cell(k, i, j))
-      fused_update<T, LAP>(a, P, slot(k - 1), slot(k), slot(k + 1), x0, y0, i, j, k, rb, dxinv, lambda, alpha, beta,
-                           c1, c2);
+    if (k >= k0) {
+#pragma unroll
+      for (int u = 0; u < NBLK; u++) {
+        const int q = tid + NT * u;
+        int i, j;
+        black_ij(q, k, i, j);
+        if (q < NBC && i <= a.hi[0] && j <= a.hi[1]) {
+          const int c = lidx(i, j);
+          P[slot(k)][c] = fused_new<T, LAP, LW>(a, P[slot(k - 1)], P[slot(k)], P[slot(k + 1)], c, i, j, k, rb[u],
+                                                dxinv, lambda, alpha, beta, c1, c2);
+        }
+      }
+    }
+  };
+  fetch(A, k0 - 2, k0 - 3, k0 - 4);
+  put_phi(A, k0 - 2);
+  fetch(A, k0 - 1, k0 - 3, k0 - 4);
+  put_phi(A, k0 - 1);
+  fetch(A, k0, k0 - 1, k0 - 2);
+  fetch(B, k0 + 1, k0, k0 - 1);
+  for (int k = k0 - 2; k < k1; k += 2) {
+    step(A, k);
+    if (k + 1 < k1) step(B, k + 1);
   }
   lds_barrier();
   store(k1 - 1);
 }
-template <class T>
-void gsrb_fused(const DeviceLayout& L, const FaceRules& fr, T* dst, const T* src, const T* rhs, double dx,
-                double alpha, double beta) {
+
+// Tile shapes of the one-launch relaxation: column FX x FY, NT threads.
+// All give bitwise the same result; the fastest depends on the level's size.
+const int kFusedShapes[][3] = {{64, 8, 256},  {32, 8, 256},  {64, 4, 256},   {32, 16, 256},
+                               {128, 4, 256}, {64, 16, 256}, {32, 16, 512},  {64, 16, 512},
+                               {64, 32, 1024}, {128, 8, 1024}};
+constexpr int kNumFusedShapes = kFusedShapeCount;
+static_assert(sizeof(kFusedShapes) / sizeof(kFusedShapes[0]) == kNumFusedShapes, "shape table");
+
+template <class T, bool LAP>
+static void launch_fused(Context* ctx, int shape, int grid, const FusedArgs& a, T* dst, const T* src, const T* rhs,
+                         T dxinv, T lambda, T alpha, T beta, T c1, T c2, double bytes) {
+  hipEvent_t ev = prof_begin(ctx, KC_GSRB);
+#define FUSED_CASE(n, X, Y, NT)                                                                               \
+  case n:                                                                                                     \
+    hipLaunchKernelGGL((k_gsrb_fused<T, LAP, X, Y, NT>), dim3(grid), dim3(NT), 0, ctx->stream, a, dst, src, rhs, \
+                       dxinv, lambda, alpha, beta, c1, c2);                                                   \
+    break;
+  switch (shape) {
+    FUSED_CASE(0, 64, 8, 256)
+    FUSED_CASE(1, 32, 8, 256)
+    FUSED_CASE(2, 64, 4, 256)
+    FUSED_CASE(3, 32, 16, 256)
+    FUSED_CASE(4, 128, 4, 256)
+    FUSED_CASE(5, 64, 16, 256)
+    FUSED_CASE(6, 32, 16, 512)
+    FUSED_CASE(7, 64, 16, 512)
+    FUSED_CASE(8, 64, 32, 1024)
+    FUSED_CASE(9, 128, 8, 1024)
+    default: throw std::runtime_error("gsrb_fused: bad tile shape");
+  }
+#undef FUSED_CASE
+  hip_check(hipGetLastError(), "k_gsrb_fused");
+  prof_end(ctx, ev, KC_GSRB, bytes);
+}
+
+static FusedArgs fused_args(const DeviceLayout& L, const FaceRules& fr, int shape, int kc) {
   if (!L.merged) throw std::runtime_error("gsrb_fused: level is not stored merged");
-  if (dst == src || dst == rhs) throw std::runtime_error("gsrb_fused: dst must be a separate field");
+  if (shape < 0 || shape >= kNumFusedShapes) throw std::runtime_error("gsrb_fused: bad tile shape");
+  const int FX = kFusedShapes[shape][0], FY = kFusedShapes[shape][1];
   const DBox& R = L.h_boxes[L.nbox];
   FusedArgs a;
   for (int q = 0; q < 3; q++) { a.lo[q] = R.lo[q]; a.hi[q] = R.hi[q]; }
@@ -617,12 +699,23 @@ void gsrb_fused(const DeviceLayout& L, const FaceRules& fr, T* dst, const T* src
   a.ntx = (L.rect.len(0) + FX - 1) / FX;
   a.nty = (L.rect.len(1) + FY - 1) / FY;
   const int nz = L.rect.len(2);
-  // z-chunks: long enough to amortise the 2-plane prologue, enough of them to fill the chip
-  a.kc = 32;
-  while (a.kc > 4 && (long long)a.ntx * a.nty * ((nz + a.kc - 1) / a.kc) < 2048) a.kc >>= 1;
-  const long long nwork = (long long)a.ntx * a.nty * ((nz + a.kc - 1) / a.kc);
+  if (kc <= 0) {
+    // z-chunks: long enough to amortise the 2-plane prologue, enough of them to fill the chip
+    kc = 32;
+    while (kc > 4 && (long long)a.ntx * a.nty * ((nz + kc - 1) / kc) < 2048) kc >>= 1;
+  }
+  a.kc = kc;
+  const long long nwork = (long long)a.ntx * a.nty * ((nz + kc - 1) / kc);
   if (nwork > INT32_MAX - 8) throw std::runtime_error("gsrb_fused: level too large");
   a.nwork = (int)nwork;
+  return a;
+}
+
+template <class T>
+void gsrb_fused(const DeviceLayout& L, const FaceRules& fr, T* dst, const T* src, const T* rhs, double dx,
+                double alpha, double beta, int shape, int kc) {
+  if (dst == src || dst == rhs) throw std::runtime_error("gsrb_fused: dst must be a separate field");
+  const FusedArgs a = fused_args(L, fr, shape, kc);
   const int grid = (a.nwork + 7) / 8 * 8;
   double dxinv = 1.0 / (dx * dx);
   double sum_b = 0.0;
@@ -630,12 +723,43 @@ void gsrb_fused(const DeviceLayout& L, const FaceRules& fr, T* dst, const T* src
   const bool lap = (alpha == 0.0 && beta == 1.0);
   double lambda = lap ? -1.0 / sum_b : -1.0 / (alpha - beta * sum_b);
   const T c1 = T(fr.c1), c2 = T(fr.c2);
+  const double bytes = 24.0 * sizeof(T) / 8.0 * L.valid_cells;
   if (lap)
-    LAUNCH_P(KC_GSRB, 24.0 * sizeof(T) / 8.0 * L.valid_cells, (k_gsrb_fused<T, true>), grid, a, dst, src, rhs,
-             T(dxinv), T(lambda), T(alpha), T(beta), c1, c2);
+    launch_fused<T, true>(L.ctx, shape, grid, a, dst, src, rhs, T(dxinv), T(lambda), T(alpha), T(beta), c1, c2, bytes);
   else
-    LAUNCH_P(KC_GSRB, 24.0 * sizeof(T) / 8.0 * L.valid_cells, (k_gsrb_fused<T, false>), grid, a, dst, src, rhs,
-             T(dxinv), T(lambda), T(alpha), T(beta), c1, c2);
+    launch_fused<T, false>(L.ctx, shape, grid, a, dst, src, rhs, T(dxinv), T(lambda), T(alpha), T(beta), c1, c2,
+                           bytes);
+}
+
+// Times every tile shape (and z-chunk lengths) of the one-launch relaxation on
+// this level's own fields and returns the fastest as {shape, kc}; times[] (if
+// given, kNumFusedShapes x 4 entries, ms per launch) receives all timings.
+template <class T>
+std::pair<int, int> tune_gsrb_fused(const DeviceLayout& L, const FaceRules& fr, T* dst, const T* src, const T* rhs,
+                                    double dx, double alpha, double beta, double* times) {
+  const int kcs[4] = {0, 8, 16, 32};
+  hipEvent_t e0, e1;
+  hip_check(hipEventCreate(&e0), "event");
+  hip_check(hipEventCreate(&e1), "event");
+  double best = 1e300;
+  std::pair<int, int> pick{0, 0};
+  for (int sh = 0; sh < kNumFusedShapes; sh++)
+    for (int q = 0; q < 4; q++) {
+      gsrb_fused<T>(L, fr, dst, src, rhs, dx, alpha, beta, sh, kcs[q]);  // warm
+      hip_check(hipEventRecord(e0, L.ctx->stream), "record");
+      const int reps = 3;
+      for (int r = 0; r < reps; r++) gsrb_fused<T>(L, fr, dst, src, rhs, dx, alpha, beta, sh, kcs[q]);
+      hip_check(hipEventRecord(e1, L.ctx->stream), "record");
+      hip_check(hipEventSynchronize(e1), "sync");
+      float ms = 0;
+      hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+      const double t = ms / reps;
+      if (times) times[sh * 4 + q] = t;
+      if (t < best) { best = t; pick = {sh, kcs[q]}; }
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return pick;
 }
 
 // OPERATORLAP (AMRPoissonOpF.ChF:287-324)
@@ -1183,7 +1307,9 @@ void reflux(Context* ctx, const RefluxCell* cells, int ncells, const RefluxCoars
   template void fill_ghosts<T>(const DeviceLayout&, const GhostPlan&, T*, int, int, double, double);           \
   template void gsrb<T>(const DeviceLayout&, T*, const T*, int, double, double, double);                       \
   template void gsrb_fused<T>(const DeviceLayout&, const FaceRules&, T*, const T*, const T*, double, double,      \
-                              double);                                                                        \
+                              double, int, int);                                                              \
+  template std::pair<int, int> tune_gsrb_fused<T>(const DeviceLayout&, const FaceRules&, T*, const T*, const T*,  \
+                                                  double, double, double, double*);                          \
   template void apply<T>(const DeviceLayout&, T*, const T*, double, double, double);                           \
   template void residual<T>(const DeviceLayout&, T*, const T*, const T*, double, double, double);              \
   template void restrict_res<T>(const DeviceLayout&, const DeviceLayout&, T*, const T*, const T*, double, double, \

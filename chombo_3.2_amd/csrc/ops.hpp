@@ -5,6 +5,8 @@
 // level.  All launches go to the context's stream; none synchronises except the
 // reductions that return a host scalar.
 #pragma once
+#include <utility>
+
 #include "device.hpp"
 
 namespace amrmg {
@@ -50,9 +52,17 @@ struct FaceRules {
 };
 // One GSRB iteration (red, refresh, black) of a merged level in one launch:
 // dst <- relaxed src.  Bitwise equal to fill_ghosts + gsrb(0) + fill_ghosts + gsrb(1).
+// shape: tile shape index (kFusedShapes in kernels.hip), kc: z-chunk length (0 = automatic)
 template <class T>
 void gsrb_fused(const DeviceLayout& L, const FaceRules& fr, T* dst, const T* src, const T* rhs, double dx,
-                double alpha, double beta);
+                double alpha, double beta, int shape = 0, int kc = 0);
+// Autotuner of the one-launch relaxation: times every tile shape with z-chunk
+// lengths {auto, 8, 16, 32} on the given fields, returns the fastest {shape, kc};
+// times (kFusedShapeCount x 4 entries, ms per launch) receives all timings when not null.
+constexpr int kFusedShapeCount = 10;
+template <class T>
+std::pair<int, int> tune_gsrb_fused(const DeviceLayout& L, const FaceRules& fr, T* dst, const T* src, const T* rhs,
+                                    double dx, double alpha, double beta, double* times);
 template <class T>
 void apply(const DeviceLayout& L, T* lofphi, const T* phi, double dx, double alpha, double beta);
 template <class T>

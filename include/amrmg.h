@@ -103,6 +103,19 @@ int amrmg_average(amrmg_solver* s, int lev, const double* fine, double* coarse);
 int amrmg_bottom_solve(amrmg_solver* s, double* corr, const double* res, int nbottom, double metric, double eps,
                        int* info);
 
+/* Autotuner of the one-launch relaxation: on every level that uses it, times its
+   10 tile shapes (columns 64x8, 32x8, 64x4, 32x16, 128x4, 64x16 of 256 threads;
+   32x16, 64x16 of 512; 64x32, 128x8 of 1024) x z-chunk lengths {auto, 8, 16, 32}
+   on the solver's own fields and keeps the fastest for later solves (results are
+   bitwise the same for every choice).  Writes up to cap records of 46 doubles:
+   {0 AMR level / 1 multigrid depth, index, valid cells, chosen shape, chosen
+   z-chunk, its ms per iteration, then ms of each shape x chunk}; *nrec = number
+   of tuned levels. */
+int amrmg_solver_tune_relax(amrmg_solver* s, double* table, int cap, int* nrec);
+/* Forces tile shape (0..9) and z-chunk (0 = automatic) of the one-launch
+   relaxation on every level. */
+int amrmg_solver_set_relax_shape(amrmg_solver* s, int shape, int kc);
+
 /* Plain device copy of nbytes (reads nbytes, writes nbytes), iters launches;
    average ms per launch.  Calibrates HBM byte counters and the copy bandwidth. */
 int amrmg_copy_probe(amrmg_ctx* ctx, long long nbytes, int iters, double* ms_out);

@@ -225,3 +225,30 @@ def test_fused_relax_matches_four_launch_sequence(gpu_ctx, case):
     np.testing.assert_array_equal(h1, h2)
     np.testing.assert_array_equal(p1, p2)
     assert h1[-1] <= 1e-10 * h1[0]
+
+
+def test_relax_tile_shapes_and_tuner_bitwise(gpu_ctx):
+    """Every tile shape and z-chunk length of the one-launch relaxation, and the
+    autotuner's pick, give the same solve bit for bit."""
+    h = H.amr_hierarchy(64, fracs=(0.25, 0.06), max_box=32, block=8)
+    rhs = H.bump_rhs(h, 1006)
+    sp, rp = H.solver_params()
+    g = amrmg.AMRMultiGrid(gpu_ctx, h)
+    g.setSolverParameters(sp, rp)
+    g.set_field(g.RHS, rhs)
+    ref_hist, ref_info = g.solve()
+    ref_phi = g.get_field(g.PHI)
+    cases = [(s, 0) for s in range(len(g.RELAX_SHAPES))] + [(0, 8), (3, 16), (4, 32)]
+    for shape, kc in cases:
+        g.set_relax_shape(shape, kc)
+        hist, info = g.solve()
+        assert info == ref_info, (shape, kc)
+        np.testing.assert_array_equal(hist, ref_hist)
+        np.testing.assert_array_equal(g.get_field(g.PHI), ref_phi)
+    table = g.tune_relax()
+    assert len(table) == 3 + (g.mg_depths - 1)
+    for r in table:
+        assert r["ms"] > 0 and r["shape"] in g.RELAX_SHAPES
+    hist, info = g.solve()
+    np.testing.assert_array_equal(g.get_field(g.PHI), ref_phi)
+    g.close()
