@@ -108,6 +108,67 @@ def cpu_baseline(hier, rhs, sp, rp):
                        f"excluded); OMP threads={threads}")
 
 
+def mixed_leg(args, ctx, hier, rhs, sp, rp, solver64, fp64_s_per_step):
+    """The same workload solved with FP32 inner V-cycles under the FP64 outer
+    iteration (precision 1): solves/s against the FP64 path and the max-norm
+    difference of the composite solution (cells not covered by a finer level)
+    from the FP64 solution, which the parity tests hold bitwise equal to the
+    oracle's."""
+    import numpy as np
+    import amrmg
+    phi64 = solver64.get_field(solver64.PHI)
+    ms = amrmg.AMRMultiGrid(ctx, hier, precision=1)
+    ms.setSolverParameters(sp, rp)
+    ms.set_field(ms.RHS, rhs)
+    if not args.no_tune:
+        ms.tune_relax()
+    for _ in range(args.warmup):
+        ms.solve()
+    ctx.profile(["gsrb"])
+    ctx.profile_reset()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        hist, info = ms.solve()
+    wall = time.perf_counter() - t0
+    prof = ctx.profile_read("gsrb")
+    ctx.profile([])
+    phi = ms.get_field(ms.PHI)
+    offs = np.cumsum([0] + [hier.cells(l) for l in range(hier.nlev)])
+    mask = np.ones(offs[-1], dtype=bool)
+    for lev in range(hier.nlev - 1):  # covered cells: outside the composite residual
+        cov = [((b[0][0] // 2, b[0][1] // 2, b[0][2] // 2), (b[1][0] // 2, b[1][1] // 2, b[1][2] // 2))
+               for b in hier.levels[lev + 1]]
+        o = offs[lev]
+        for (lo, hi) in hier.levels[lev]:
+            n = [hi[d] - lo[d] + 1 for d in range(3)]
+            cnt = n[0] * n[1] * n[2]
+            I = np.meshgrid(*[np.arange(lo[d], hi[d] + 1) for d in range(3)], indexing="ij")
+            c = np.zeros(n, dtype=bool)
+            for (cl, ch) in cov:
+                m = np.ones(n, dtype=bool)
+                for d in range(3):
+                    m &= (I[d] >= cl[d]) & (I[d] <= ch[d])
+                c |= m
+            mask[o:o + cnt] = ~c.ravel(order="F")
+            o += cnt
+    err = float(np.abs(phi - phi64)[mask].max() / np.abs(phi64)[mask].max())
+    cell_relax = prof["bytes"] / 12.0   # FP32 relaxation: 12 B per cell-relax
+    out = {
+        "dtype": "f32 inner V-cycles / f64 outer residual, update, bottom",
+        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "solves_per_s": round(args.steps / wall, 4),
+        "speedup_vs_fp64": round(fp64_s_per_step / (wall / args.steps), 3),
+        "mcell_relax_per_s": round(cell_relax / wall / 1e6, 3),
+        "relax_achieved_GBs": round(prof["bytes"] / (prof["ms"] * 1e-3) / 1e9, 1) if prof["ms"] > 0 else None,
+        "iterations": info["iterations"],
+        "final_rel_residual": float(hist[-1] / hist[0]),
+        "max_rel_err_vs_fp64_composite": err,
+    }
+    log(f"mixed precision: {out}")
+    ms.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,6 +179,7 @@ def main():
     ap.add_argument("--iter-max", type=int, default=100,
                     help="cap on V-cycles per solve (a short cap is for profiling runs only)")
     ap.add_argument("--no-tune", action="store_true", help="skip the relaxation autotuner (default shapes)")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--copy-mb", type=int, default=2048,
                     help="size of the plain copy kernel run after the timed region (0 = skip)")
     ap.add_argument("--summarize-pmc", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "FETCH_RUN_JSON"),
@@ -184,6 +246,9 @@ def main():
     traffic = round(ratio * bytes_per_launch, 1) if ratio else None
     final_rel = float(infos[-1][0][-1] / infos[-1][0][0])
     ok = all(h[-1] <= 1e-10 * h[0] for h, _ in infos)
+    mixed = None
+    if not args.no_mixed:
+        mixed = mixed_leg(args, ctx, hier, rhs, sp, rp, solver, wall / args.steps)
     copy_gbs = None
     if args.copy_mb > 0:
         nbytes = args.copy_mb << 20
@@ -237,6 +302,7 @@ def main():
         },
         "relax_tiles": ({f"{r['kind']}{r['index']}": f"{r['shape'][0]}x{r['shape'][1]}/kc{r['kc']}" for r in tune}
                         if tune else None),
+        "mixed_precision": mixed,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

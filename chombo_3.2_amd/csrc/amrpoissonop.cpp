@@ -647,13 +647,15 @@ void AMRMultiGrid<Real>::solveNoInitResid(std::vector<T*>& phi, std::vector<T*>&
   m_initial_rnorm = computeAMRResidual(uberResidual, phi, rhs, l_max, l_base, false, true);
   double rnorm = m_initial_rnorm, norm_last = 2 * m_initial_rnorm;
   m_bottomSolver->setConvergenceMetrics(m_initial_rnorm, m_bottomSolverEpsCushion * m_eps);
+  if (m_innerBottomMetrics) m_innerBottomMetrics(m_initial_rnorm, m_bottomSolverEpsCushion * m_eps);
   int iter = 0;
   m_history.assign(1, rnorm);
   bool goNorm = rnorm > m_normThresh, goRedu = rnorm > m_eps * m_initial_rnorm, goIter = iter < m_iterMax;
   bool goHang = iter < m_imin || rnorm < (1 - m_hang) * norm_last, goMin = iter < m_iterMin;
   while (goMin || (goIter && goRedu && goHang && goNorm)) {
     norm_last = rnorm;
-    AMRVCycle(uberCorrection, uberResidual, l_max, l_max, l_base);
+    if (m_innerCycle) m_innerCycle(uberCorrection, uberResidual, l_max, l_base);
+    else AMRVCycle(uberCorrection, uberResidual, l_max, l_max, l_base);
     for (int ilev = l_base; ilev <= l_max; ilev++) {
       m_op[ilev]->incr(*phi[ilev], *uberCorrection[ilev], 1.0);
       m_op[ilev]->setToZero(*uberCorrection[ilev]);

@@ -5,6 +5,7 @@
 // Every per-box FORT_* loop of the reference is one kernel launch over the level;
 // Copier exchanges, BCs and coarse-fine interpolation are table-driven kernels.
 #pragma once
+#include <functional>
 #include <memory>
 #include <vector>
 
@@ -151,6 +152,12 @@ class AMRMultiGrid {
   LinearSolver<T>* m_bottomSolver = nullptr;
   std::vector<double> m_history;
   int m_iterations = 0;
+  // Mixed precision: when set, solveNoInitResid runs this in place of
+  // AMRVCycle(uberCorrection, uberResidual, l_max, l_max, l_base) (an inner
+  // correction cycle in another precision), and m_innerBottomMetrics receives the
+  // bottom solver's convergence metrics.
+  std::function<void(std::vector<T*>&, std::vector<T*>&, int, int)> m_innerCycle;
+  std::function<void(double, double)> m_innerBottomMetrics;
 
   void define(AMRPoissonOpFactory<Real>& factory, LinearSolver<T>* bottom, int numLevels);
   void setSolverParameters(int pre, int post, int bottom, int numMG, int iterMax, double eps, double hang,

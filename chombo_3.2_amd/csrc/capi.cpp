@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "amrpoissonop.hpp"
+#include "mixed.hpp"
 
 using namespace amrmg;
 
@@ -48,6 +49,7 @@ struct amrmg_solver {
   // Per-box storage for the FAB-level entry points (amrmg_kernel_gsrb,
   // amrmg_homog_cf, amrmg_quad_cf), whose ghost cells are caller inputs like the
   // ghost cells of the reference's FArrayBoxes; built on first use.
+  std::unique_ptr<MixedInner> mixed;  // precision 1: FP32 inner cycles
   std::unique_ptr<DeviceHierarchy> fabH;
   std::unique_ptr<AMRPoissonOpFactory<double>> fabFac;
   std::vector<std::unique_ptr<AMRPoissonOp<double>>> fabOps;
@@ -171,7 +173,7 @@ int amrmg_ctx_destroy(amrmg_ctx* ctx) {
 
 int amrmg_solver_create(amrmg_ctx* ctx, const int* ispec, const double* rspec, int precision, amrmg_solver** out) {
   API_BEGIN
-  if (precision != 0) throw std::runtime_error("precision mode not available in this build");
+  if (precision != 0 && precision != 1) throw std::runtime_error("precision must be 0 (FP64) or 1 (mixed)");
   std::unique_ptr<amrmg_solver> s(new amrmg_solver());
   s->ctx = ctx;
   s->precision = precision;
@@ -193,6 +195,11 @@ int amrmg_solver_create(amrmg_ctx* ctx, const int* ispec, const double* rspec, i
   }
   auto ph = s->ptrs(s->phi), rh = s->ptrs(s->rhs);
   s->amg.init(ph, rh, s->nlev() - 1, 0);
+  if (precision == 1) {
+    s->mixed.reset(new MixedInner());
+    s->mixed->define(&s->H, s->alpha, s->beta, s->bc, s->amg);
+    s->mixed->setSolverParameters(s->sp[0], s->sp[1], s->sp[2], s->sp[3], s->sp[4], s->rp[0], s->rp[1], s->rp[2]);
+  }
   ctx->c->sync();
   *out = s.release();
   API_END
@@ -208,6 +215,7 @@ int amrmg_solver_set_params(amrmg_solver* s, const int* sp, const double* rp) {
   std::memcpy(s->rp, rp, sizeof s->rp);
   s->amg.setSolverParameters(sp[0], sp[1], sp[2], sp[3], sp[4], rp[0], rp[1], rp[2]);
   s->amg.m_imin = sp[5];
+  if (s->mixed) s->mixed->setSolverParameters(sp[0], sp[1], sp[2], sp[3], sp[4], rp[0], rp[1], rp[2]);
   API_END
 }
 long long amrmg_solver_level_cells(amrmg_solver* s, int lev) {
@@ -222,9 +230,15 @@ long long amrmg_solver_mg_cells(amrmg_solver* s, int depth) {
 long long amrmg_solver_device_bytes(amrmg_solver* s) {
   long long n = 0;
   for (int l = 0; l < s->nlev(); l++) n += s->H.amrLay[l]->slab;
-  long long fields = 5 + 3;  // phi, rhs, res, uber residual/correction, correction, residual, resC
-  long long total = n * fields * 8;
-  for (auto& m : s->H.mgLay) total += m->slab * 2 * 8;
+  // FP64 per AMR level: phi, rhs, res, uber residual/correction, V-cycle correction,
+  // residual, resC and the relaxation buffer; per multigrid level: correction,
+  // residual, relaxation buffer.  Mixed precision adds the FP32 copies of the
+  // V-cycle fields.
+  long long total = n * 9 * 8;
+  long long mg = 0;
+  for (auto& m : s->H.mgLay) mg += m->slab;
+  total += mg * 3 * 8;
+  if (s->mixed) total += n * 6 * 4 + mg * 3 * 4;
   return total;
 }
 
@@ -258,7 +272,7 @@ int amrmg_solver_get_field(amrmg_solver* s, int field, double* h) {
 int amrmg_solver_solve(amrmg_solver* s, int zero_phi, double* hist, int hist_cap, int* info) {
   API_BEGIN
   auto ph = s->ptrs(s->phi), rh = s->ptrs(s->rhs), ur = s->ptrs(s->uberRes), uc = s->ptrs(s->uberCorr);
-  long long b0 = s->bottom.m_total_iters;
+  long long b0 = s->mixed ? s->mixed->bottom.m_bicg.m_total_iters : s->bottom.m_total_iters;
   s->amg.solveNoInitResid(ph, ur, uc, rh, s->nlev() - 1, 0, zero_phi != 0);
   s->ctx->c->sync();
   if (hist)
@@ -266,7 +280,7 @@ int amrmg_solver_solve(amrmg_solver* s, int zero_phi, double* hist, int hist_cap
   if (info) {
     info[0] = s->amg.m_iterations;
     info[1] = s->amg.m_exitStatus;
-    info[2] = (int)(s->bottom.m_total_iters - b0);
+    info[2] = (int)((s->mixed ? s->mixed->bottom.m_bicg.m_total_iters : s->bottom.m_total_iters) - b0);
   }
   API_END
 }
@@ -408,7 +422,7 @@ int amrmg_solver_tune_relax(amrmg_solver* s, double* table, int cap, int* nrec) 
   API_BEGIN
   constexpr int W = 6 + ops::kFusedShapeCount * 4;
   int n = 0;
-  auto rec = [&](int kind, int idx, AMRPoissonOp<double>* op, LevelData<double>& phi, LevelData<double>& rhs) {
+  auto rec = [&](int kind, int idx, auto* op, auto& phi, auto& rhs) {
     double times[ops::kFusedShapeCount * 4];
     if (!op->tuneRelax(phi, rhs, times)) return;
     if (table && n < cap) {
@@ -423,9 +437,17 @@ int amrmg_solver_tune_relax(amrmg_solver* s, double* table, int cap, int* nrec) 
     }
     n++;
   };
-  for (int l = 0; l < s->nlev(); l++) rec(0, l, s->amg.m_op[l].get(), *s->phi[l], *s->rhs[l]);
-  for (int d = 1; d < s->H.mgDepth(); d++)
-    rec(1, d, s->mgop(d), *s->amg.m_mg0->m_correction[d], *s->amg.m_mg0->m_residual[d]);
+  if (s->mixed) {  // the relaxation of a mixed solve runs in FP32
+    auto& m = *s->mixed;
+    for (int l = 0; l < s->nlev(); l++) rec(0, l, m.amg.m_op[l].get(), *m.corr[l], *m.res[l]);
+    for (int d = 1; d < s->H.mgDepth(); d++)
+      rec(1, d, static_cast<AMRPoissonOp<float>*>(m.amg.m_mg0->m_op[d]), *m.amg.m_mg0->m_correction[d],
+          *m.amg.m_mg0->m_residual[d]);
+  } else {
+    for (int l = 0; l < s->nlev(); l++) rec(0, l, s->amg.m_op[l].get(), *s->phi[l], *s->rhs[l]);
+    for (int d = 1; d < s->H.mgDepth(); d++)
+      rec(1, d, s->mgop(d), *s->amg.m_mg0->m_correction[d], *s->amg.m_mg0->m_residual[d]);
+  }
   s->ctx->c->sync();
   if (nrec) *nrec = n;
   API_END
@@ -436,6 +458,15 @@ int amrmg_solver_set_relax_shape(amrmg_solver* s, int shape, int kc) {
     throw std::runtime_error("relax shape out of range");
   for (int l = 0; l < s->nlev(); l++) { s->amg.m_op[l]->m_relaxShape = shape; s->amg.m_op[l]->m_relaxKc = kc; }
   for (int d = 1; d < s->H.mgDepth(); d++) { s->mgop(d)->m_relaxShape = shape; s->mgop(d)->m_relaxKc = kc; }
+  if (s->mixed) {
+    auto& m = s->mixed->amg;
+    for (int l = 0; l < s->nlev(); l++) { m.m_op[l]->m_relaxShape = shape; m.m_op[l]->m_relaxKc = kc; }
+    for (int d = 1; d < s->H.mgDepth(); d++) {
+      auto* op = static_cast<AMRPoissonOp<float>*>(m.m_mg0->m_op[d]);
+      op->m_relaxShape = shape;
+      op->m_relaxKc = kc;
+    }
+  }
   API_END
 }
 int amrmg_copy_probe(amrmg_ctx* ctx, long long nbytes, int iters, double* ms_out) {

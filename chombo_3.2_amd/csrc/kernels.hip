@@ -997,6 +997,17 @@ __global__ void __launch_bounds__(256) k_incr_f2d(const DBox* __restrict__ boxes
 void incr_f2d(const DeviceLayout& L, double* lhs, const float* x) {
   LAUNCH(k_incr_f2d, L.ntiles, L.d_boxes, L.d_tiles, geom(L), lhs, x);
 }
+__global__ void __launch_bounds__(256) k_f2d(const DBox* __restrict__ boxes, const Tile* __restrict__ tiles,
+                                             TileGeom g, double* dst, const float* src) {
+  DBox b;
+  int i, j, k;
+  if (!tile_cell(boxes, tiles, g, b, i, j, k)) return;
+  const long long c = cidx(b, i, j, k);
+  dst[c] = (double)src[c];
+}
+void convert_valid_f2d(const DeviceLayout& L, double* dst, const float* src) {
+  LAUNCH(k_f2d, L.ntiles, L.d_boxes, L.d_tiles, geom(L), dst, src);
+}
 
 // ---------------------------------------------------------------------------
 // Copier motion (copyTo / copyToZero) between two layouts

@@ -93,6 +93,7 @@ void precond_init(const DeviceLayout& L, T* phi, const T* rhs, double mult);
 // mixed precision: dst (float) <- src (double) on valid cells, and back with incr
 void convert_valid_d2f(const DeviceLayout& L, float* dst, const double* src);
 void incr_f2d(const DeviceLayout& L, double* lhs, const float* x);
+void convert_valid_f2d(const DeviceLayout& L, double* dst, const float* src);
 
 template <class T>
 void motion(const MotionPlan& P, const DeviceLayout& Lsrc, const DeviceLayout& Ldst, T* dst, const T* src,

@@ -1,0 +1,80 @@
+"""Mixed-precision solves (FP32 inner V-cycles under the FP64 outer iteration)
+against the FP64 oracle.
+
+The outer iteration computes the composite residual, its max-norm and the update
+in FP64, so a mixed solve stops on the same FP64 criterion as the reference
+(max-norm residual <= 1e-10 of the initial one).  Its solution then differs from
+the oracle's by the solve tolerance, not by FP32 rounding: the bar below is a
+1e-9 relative max-norm difference, i.e. ten times the 1e-10 solve tolerance.
+It is taken over the composite solution, the cells not covered by a finer level:
+covered coarse cells are outside the composite residual (zeroCovered), so the
+iteration never corrects them and they keep whatever the cycles left there, in
+FP32 for a mixed solve.
+"""
+import numpy as np
+import pytest
+
+import amrmg
+import hierarchy as H
+from helpers import lshape_amr, small_amr, small_single, uncovered_mask
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+MIXED_TOL = 1e-9
+
+
+def _case(name):
+    if name == "amr32":
+        h = small_amr()
+        return h, H.bump_rhs(h, 1002)
+    if name == "lshape":
+        h = lshape_amr()
+        return h, H.bump_rhs(h, 1002)
+    if name == "helm_single24":
+        h = small_single(n=24, box=8, alpha=0.4, beta=1.7)
+        return h, H.sinusoid_rhs(h, 1004, k=(1, 2, 3))
+    h = H.amr_hierarchy(64, fracs=(0.25, 0.06), max_box=32, block=8)
+    return h, H.bump_rhs(h, 1003)
+
+
+@pytest.mark.parametrize("case", ["amr32", "lshape", "helm_single24", "amr64"])
+def test_mixed_precision_solve_matches_fp64_oracle(gpu_ctx, case):
+    h, rhs = _case(case)
+    sp, rp = H.solver_params()
+    g = amrmg.AMRMultiGrid(gpu_ctx, h, precision=1)
+    g.setSolverParameters(sp, rp)
+    g.set_field(g.RHS, rhs)
+    hist, info = g.solve()
+    phi = g.get_field(g.PHI)
+    phi_o, hist_o, info_o = Oracle(h).solve(rhs, sp, rp)
+    m = np.concatenate([uncovered_mask(h, lev) for lev in range(h.nlev)])
+    err = np.abs(phi - phi_o)[m].max() / np.abs(phi_o)[m].max()
+    print(f"{case}: mixed {info['iterations']} it {hist[-1] / hist[0]:.2e}, "
+          f"fp64 {info_o['iterations']} it, composite err {err:.2e}, "
+          f"all cells {np.abs(phi - phi_o).max() / np.abs(phi_o).max():.2e}")
+    # the initial residual is computed in FP64 from the same data: identical
+    assert hist[0] == hist_o[0]
+    assert info["exit_status"] & 1 and hist[-1] <= 1e-10 * hist[0]
+    assert abs(info["iterations"] - info_o["iterations"]) <= 2
+    assert err < MIXED_TOL, err
+    g.close()
+
+
+def test_mixed_precision_tuner_and_determinism(gpu_ctx):
+    """Tuning the FP32 relaxation changes no result; repeated mixed solves are
+    bitwise identical."""
+    h, rhs = _case("amr64")
+    sp, rp = H.solver_params()
+    g = amrmg.AMRMultiGrid(gpu_ctx, h, precision=1)
+    g.setSolverParameters(sp, rp)
+    g.set_field(g.RHS, rhs)
+    h1, i1 = g.solve()
+    p1 = g.get_field(g.PHI)
+    table = g.tune_relax()
+    assert len(table) >= 3
+    h2, i2 = g.solve()
+    assert i1 == i2
+    np.testing.assert_array_equal(h1, h2)
+    np.testing.assert_array_equal(g.get_field(g.PHI), p1)
+    g.close()
