@@ -40,6 +40,7 @@ def lib():
         L.amrmg_ctx_create.argtypes = [C.c_int, C.POINTER(C.c_void_p)]
         L.amrmg_ctx_destroy.argtypes = [C.c_void_p]
         L.amrmg_ctx_set_fused_relax.argtypes = [C.c_void_p, C.c_int]
+        L.amrmg_profile_read_units.argtypes = [C.c_void_p, C.c_int, _dp]
         L.amrmg_solver_create.argtypes = [C.c_void_p, _ip, _dp, C.c_int, C.POINTER(C.c_void_p)]
         L.amrmg_solver_destroy.argtypes = [C.c_void_p]
         L.amrmg_solver_set_params.argtypes = [C.c_void_p, _ip, _dp]
@@ -143,7 +144,9 @@ class Context:
         k = KERNEL_CLASSES.index(cls) if isinstance(cls, str) else int(cls)
         ms, by, n = C.c_double(), C.c_double(), C.c_longlong()
         _chk(lib().amrmg_profile_read(self.p, k, C.byref(ms), C.byref(by), C.byref(n)))
-        return dict(ms=ms.value, bytes=by.value, launches=n.value)
+        u = C.c_double()
+        _chk(lib().amrmg_profile_read_units(self.p, k, C.byref(u)))
+        return dict(ms=ms.value, bytes=by.value, launches=n.value, units=u.value)
 
 
 class AMRMultiGrid:

@@ -172,6 +172,23 @@ bool AMRPoissonOp<Real>::tuneRelax(const T& phi, const T& rhs, double* times) {
   m_relaxKc = pick.second;
   return true;
 }
+// setToZero + relax: on merged levels the first iteration reads no old values
+// (they are all zero) and writes every valid cell, so the zeroing itself is
+// dropped; ghost cells are refilled before any later read, as after any relax.
+template <class Real>
+void AMRPoissonOp<Real>::relaxFromZero(T& e, const T& r, int n) {
+  if (!(m_fused && m_ctx->fused_relax) || n <= 0) {
+    setToZero(e);
+    relax(e, r, n);
+    return;
+  }
+  need(e.lay, m_lay, "relaxFromZero"); need(r.lay, m_lay, "relaxFromZero");
+  if (!m_relaxTmp.isDefined()) m_relaxTmp.define(m_lay);
+  ops::gsrb_fused<Real>(*m_lay, m_faces, m_relaxTmp.d, e.d, r.d, m_dx, m_alpha, m_beta, m_relaxShape, m_relaxKc,
+                        true);
+  std::swap(e.d, m_relaxTmp.d);
+  relax(e, r, n - 1);
+}
 template <class Real>
 void AMRPoissonOp<Real>::relax(T& e, const T& r, int n) {
   for (int i = 0; i < n; i++) levelGSRB(e, r);
@@ -535,8 +552,9 @@ void AMRMultiGrid<Real>::init(std::vector<T*>& phi, std::vector<T*>& rhs, int l_
   }
 }
 template <class Real>
-void AMRMultiGrid<Real>::relax(T& phi, T& R, int depth, int nRelax) {
-  m_op[depth]->relax(phi, R, nRelax);
+void AMRMultiGrid<Real>::relax(T& phi, T& R, int depth, int nRelax, bool fromZero) {
+  if (fromZero) m_op[depth]->relaxFromZero(phi, R, nRelax);
+  else m_op[depth]->relax(phi, R, nRelax);
 }
 // computeAMRResidualLevel (AMRMultiGrid.H:837-892)
 template <class Real>
@@ -600,20 +618,26 @@ void AMRMultiGrid<Real>::computeAMROperator(std::vector<T*>& lphi, std::vector<T
 // AMRVCycle (AMRMultiGrid.H:1342-1419)
 template <class Real>
 void AMRMultiGrid<Real>::AMRVCycle(std::vector<T*>& uberCorrection, std::vector<T*>& uberResidual, int ilev,
-                                   int l_max, int l_base) {
+                                   int l_max, int l_base, bool fromZero) {
   if (ilev == l_max) {
     for (int level = l_base; level <= l_max; level++) {
       m_op[level]->assignLocal(*m_residual[level], *uberResidual[level]);
       m_op[level]->setToZero(*m_correction[level]);
     }
   }
+  // fromZero: the correction this call relaxes first is zero on entry (set at the
+  // top of the cycle or by the finer level just before recursing; only the first
+  // of numMG recursive calls sees it zero)
   if (l_max == l_base) {
-    m_mg0->oneCycle(*uberCorrection[ilev], *uberResidual[ilev]);
+    // uberCorrection is zero on entry: zeroed before the first iteration and after each update
+    if (fromZero) m_mg0->oneCycleFromZero(*uberCorrection[ilev], *uberResidual[ilev]);
+    else m_mg0->oneCycle(*uberCorrection[ilev], *uberResidual[ilev]);
   } else if (ilev == l_base) {
-    m_mg0->oneCycle(*m_correction[ilev], *m_residual[ilev]);
+    if (fromZero) m_mg0->oneCycleFromZero(*m_correction[ilev], *m_residual[ilev]);
+    else m_mg0->oneCycle(*m_correction[ilev], *m_residual[ilev]);
     m_op[ilev]->incr(*uberCorrection[ilev], *m_correction[ilev], 1.0);
   } else {
-    relax(*m_correction[ilev], *m_residual[ilev], ilev, m_pre);
+    relax(*m_correction[ilev], *m_residual[ilev], ilev, m_pre, fromZero);
     m_op[ilev]->incr(*uberCorrection[ilev], *m_correction[ilev], 1.0);
     m_op[ilev - 1]->setToZero(*m_correction[ilev - 1]);
     std::vector<T*> res(m_residual.size());
@@ -622,12 +646,12 @@ void AMRMultiGrid<Real>::AMRVCycle(std::vector<T*>& uberCorrection, std::vector<
     m_op[ilev]->AMRRestrictS(*m_resC[ilev], *m_residual[ilev], *m_correction[ilev], *m_correction[ilev - 1],
                              *uberCorrection[ilev]);
     m_op[ilev - 1]->assignCopier(*m_residual[ilev - 1], *m_resC[ilev]);
-    for (int img = 0; img < m_numMG; img++) AMRVCycle(uberCorrection, uberResidual, ilev - 1, l_max, l_base);
+    for (int img = 0; img < m_numMG; img++)
+      AMRVCycle(uberCorrection, uberResidual, ilev - 1, l_max, l_base, img == 0);
     m_op[ilev]->AMRProlongS(*m_correction[ilev], *m_correction[ilev - 1], *m_resC[ilev]);
     m_op[ilev]->AMRUpdateResidual(*m_residual[ilev], *m_correction[ilev], *m_correction[ilev - 1]);
     T& dCorr = *uberCorrection[ilev];
-    m_op[ilev]->setToZero(dCorr);
-    relax(dCorr, *m_residual[ilev], ilev, m_post);
+    relax(dCorr, *m_residual[ilev], ilev, m_post, true);  // setToZero(dCorr) + relax
     m_op[ilev]->incr(*m_correction[ilev], dCorr, 1.0);
     m_op[ilev]->assignLocal(*uberCorrection[ilev], *m_correction[ilev]);
   }

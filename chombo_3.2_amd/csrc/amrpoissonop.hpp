@@ -68,6 +68,7 @@ class AMRPoissonOp : public AMRLevelOp<LevelData<Real>> {
   // ---- MGLevelOp ----
   void createCoarser(T& coarse, const T& fine, bool ghosted) override;
   void relax(T& correction, const T& residual, int iterations) override;
+  void relaxFromZero(T& correction, const T& residual, int iterations) override;
   void restrictResidual(T& resCoarse, T& phiFine, const T& rhsFine) override;
   void prolongIncrement(T& phiThisLevel, const T& correctCoarse) override;
   // ---- AMRLevelOp ----
@@ -165,13 +166,14 @@ class AMRMultiGrid {
   void init(std::vector<T*>& phi, std::vector<T*>& rhs, int l_max, int l_base);
   void solveNoInitResid(std::vector<T*>& phi, std::vector<T*>& uberResidual, std::vector<T*>& uberCorrection,
                         std::vector<T*>& rhs, int l_max, int l_base, bool zeroPhi);
-  void AMRVCycle(std::vector<T*>& uberCorrection, std::vector<T*>& uberResidual, int ilev, int l_max, int l_base);
+  void AMRVCycle(std::vector<T*>& uberCorrection, std::vector<T*>& uberResidual, int ilev, int l_max, int l_base,
+                 bool fromZero = true);
   double computeAMRResidual(std::vector<T*>& resid, std::vector<T*>& phi, std::vector<T*>& rhs, int l_max,
                             int l_base, bool homogeneousBC, bool computeNorm);
   void computeAMRResidualLevel(std::vector<T*>& resid, std::vector<T*>& phi, std::vector<T*>& rhs, int l_max,
                                int l_base, int ilev, bool homogeneousBC);
   void computeAMROperator(std::vector<T*>& lphi, std::vector<T*>& phi, int l_max, int l_base, bool homogeneousBC);
-  void relax(T& phi, T& R, int depth, int nRelax);
+  void relax(T& phi, T& R, int depth, int nRelax, bool fromZero = false);
 };
 
 }  // namespace amrmg

@@ -519,5 +519,12 @@ int amrmg_profile_read(amrmg_ctx* ctx, int kind, double* ms, double* bytes, long
   if (!profiler_read(ctx->c.get(), kind, ms, bytes, launches)) throw std::runtime_error("profiler off or bad class");
   API_END
 }
+int amrmg_profile_read_units(amrmg_ctx* ctx, int kind, double* units) {
+  API_BEGIN
+  double ms, bytes;
+  long long n;
+  if (!profiler_read(ctx->c.get(), kind, &ms, &bytes, &n, units)) throw std::runtime_error("profiler off or bad class");
+  API_END
+}
 
 }  // extern "C"

@@ -54,11 +54,12 @@ Context::Context(int dev, bool dry_run) : device(dev), dry(dry_run) {
 // ---------------------------------------------------------------------------
 struct Profiler {
   unsigned mask = ~0u;
-  struct Rec { int kind; hipEvent_t a, b; double bytes; };
+  struct Rec { int kind; hipEvent_t a, b; double bytes, units; };
   std::vector<Rec> recs;
   std::vector<hipEvent_t> pool;
   double ms[KC_COUNT] = {0};
   double bytes[KC_COUNT] = {0};
+  double units[KC_COUNT] = {0};   // work units (cell-relaxations for the GSRB class)
   long long count[KC_COUNT] = {0};
   hipEvent_t get() {
     if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
@@ -71,7 +72,7 @@ struct Profiler {
       hip_check(hipEventSynchronize(r.b), "hipEventSynchronize");
       float t = 0;
       hip_check(hipEventElapsedTime(&t, r.a, r.b), "hipEventElapsedTime");
-      ms[r.kind] += t; bytes[r.kind] += r.bytes; count[r.kind] += 1;
+      ms[r.kind] += t; bytes[r.kind] += r.bytes; units[r.kind] += r.units; count[r.kind] += 1;
       pool.push_back(r.a); pool.push_back(r.b);
     }
     recs.clear();
@@ -87,11 +88,11 @@ static hipEvent_t prof_begin(Context* c, int kind) {
   hip_check(hipEventRecord(a, c->stream), "hipEventRecord");
   return a;
 }
-static void prof_end(Context* c, hipEvent_t a, int kind, double bytes) {
+static void prof_end(Context* c, hipEvent_t a, int kind, double bytes, double units = 0.0) {
   if (!c->prof || !a) return;
   hipEvent_t b = c->prof->get();
   hip_check(hipEventRecord(b, c->stream), "hipEventRecord");
-  c->prof->recs.push_back(Profiler::Rec{kind, a, b, bytes});
+  c->prof->recs.push_back(Profiler::Rec{kind, a, b, bytes, units});
   if (c->prof->recs.size() > 4096) c->prof->drain();
 }
 void profiler_enable(Context* c, unsigned mask) {
@@ -103,13 +104,16 @@ void profiler_reset(Context* c) {
   if (!c->prof) return;
   (void)hipStreamSynchronize(c->stream);
   c->prof->drain();
-  for (int q = 0; q < KC_COUNT; q++) { c->prof->ms[q] = 0; c->prof->bytes[q] = 0; c->prof->count[q] = 0; }
+  for (int q = 0; q < KC_COUNT; q++) {
+    c->prof->ms[q] = 0; c->prof->bytes[q] = 0; c->prof->count[q] = 0; c->prof->units[q] = 0;
+  }
 }
-bool profiler_read(Context* c, int kind, double* ms, double* bytes, long long* count) {
+bool profiler_read(Context* c, int kind, double* ms, double* bytes, long long* count, double* units) {
   if (!c->prof || kind < 0 || kind >= KC_COUNT) return false;
   (void)hipStreamSynchronize(c->stream);
   c->prof->drain();
   *ms = c->prof->ms[kind]; *bytes = c->prof->bytes[kind]; *count = c->prof->count[kind];
+  if (units) *units = c->prof->units[kind];
   return true;
 }
 
@@ -319,15 +323,16 @@ __device__ __forceinline__ bool tile_cell(const DBox* __restrict__ boxes, const 
   return i <= b.hi[0] && j <= b.hi[1] && k <= b.hi[2];
 }
 
-#define LAUNCH_P(kind, bytes, kernel, grid, ...)                                         \
+#define LAUNCH_PU(kind, bytes, units, kernel, grid, ...)                                 \
   do {                                                                                   \
     if ((grid) > 0) {                                                                    \
-      hipEvent_t ev_ = prof_begin(L.ctx, kind);                                                \
+      hipEvent_t ev_ = prof_begin(L.ctx, kind);                                          \
       hipLaunchKernelGGL(kernel, dim3(grid), dim3(256), 0, L.ctx->stream, __VA_ARGS__);   \
       hip_check(hipGetLastError(), #kernel);                                             \
-      prof_end(L.ctx, ev_, kind, bytes);                                                 \
+      prof_end(L.ctx, ev_, kind, bytes, units);                                          \
     }                                                                                    \
   } while (0)
+#define LAUNCH_P(kind, bytes, kernel, grid, ...) LAUNCH_PU(kind, bytes, 0.0, kernel, grid, __VA_ARGS__)
 #define LAUNCH(kernel, grid, ...) LAUNCH_P(KC_BLAS, 0.0, kernel, grid, __VA_ARGS__)
 
 // ---------------------------------------------------------------------------
@@ -418,9 +423,9 @@ void gsrb(const DeviceLayout& L, T* phi, const T* rhs, int color, double dx, dou
   const bool lap = (alpha == 0.0 && beta == 1.0);
   double lambda = lap ? -1.0 / sum_b : -1.0 / (alpha - beta * sum_b);
   if (lap)
-    LAUNCH_P(KC_GSRB, 12.0 * sizeof(T) / 8.0 * L.valid_cells, (k_gsrb<T, true>), L.ntiles, L.d_boxes, L.d_tiles, geom(L), phi, rhs, color, T(dxinv), T(lambda), T(alpha), T(beta));
+    LAUNCH_PU(KC_GSRB, 12.0 * sizeof(T) / 8.0 * L.valid_cells, 0.5 * L.valid_cells, (k_gsrb<T, true>), L.ntiles, L.d_boxes, L.d_tiles, geom(L), phi, rhs, color, T(dxinv), T(lambda), T(alpha), T(beta));
   else
-    LAUNCH_P(KC_GSRB, 12.0 * sizeof(T) / 8.0 * L.valid_cells, (k_gsrb<T, false>), L.ntiles, L.d_boxes, L.d_tiles, geom(L), phi, rhs, color, T(dxinv), T(lambda), T(alpha), T(beta));
+    LAUNCH_PU(KC_GSRB, 12.0 * sizeof(T) / 8.0 * L.valid_cells, 0.5 * L.valid_cells, (k_gsrb<T, false>), L.ntiles, L.d_boxes, L.d_tiles, geom(L), phi, rhs, color, T(dxinv), T(lambda), T(alpha), T(beta));
 }
 
 // ---------------------------------------------------------------------------
@@ -455,6 +460,7 @@ struct FusedArgs {
   int sy, sz;
   int ntx, nty, kc, nwork;
   int kind[3][2];     // face rule: 0 Neumann, 1 Dirichlet, 2 homogeneous quadratic CF
+  int zsrc;           // 1: the old values are all zero (src is not read)
 };
 template <class T>
 __device__ __forceinline__ T face_ghost(int kind, T nearv, T farv, T c1, T c2) {
@@ -548,11 +554,16 @@ __global__ void __launch_bounds__(NT) k_gsrb_fused(FusedArgs a, T* __restrict__ 
   Pre A, B;
   auto fetch = [&](Pre& f, int p, int r, int k) {  // plane p; red cells of plane r; black cells of plane k
     const int pc = clampi(p, a.lo[2], a.hi[2]);
+    if (a.zsrc) {  // uniform branch: relaxation of a field known to be zero
 #pragma unroll
-    for (int u = 0; u < NLD; u++) {
-      const int q = tid + NT * u;
-      const int i = x0 - 2 + q % LW, j = y0 - 2 + q / LW;
-      f.phi[u] = src[gidx(clampi(i, a.lo[0], a.hi[0]), clampi(j, a.lo[1], a.hi[1]), pc)];
+      for (int u = 0; u < NLD; u++) f.phi[u] = T(0.0);
+    } else {
+#pragma unroll
+      for (int u = 0; u < NLD; u++) {
+        const int q = tid + NT * u;
+        const int i = x0 - 2 + q % LW, j = y0 - 2 + q / LW;
+        f.phi[u] = src[gidx(clampi(i, a.lo[0], a.hi[0]), clampi(j, a.lo[1], a.hi[1]), pc)];
+      }
     }
     const int rc = clampi(r, a.lo[2], a.hi[2]);
 #pragma unroll
@@ -656,7 +667,7 @@ static_assert(sizeof(kFusedShapes) / sizeof(kFusedShapes[0]) == kNumFusedShapes,
 
 template <class T, bool LAP>
 static void launch_fused(Context* ctx, int shape, int grid, const FusedArgs& a, T* dst, const T* src, const T* rhs,
-                         T dxinv, T lambda, T alpha, T beta, T c1, T c2, double bytes) {
+                         T dxinv, T lambda, T alpha, T beta, T c1, T c2, double bytes, double units) {
   hipEvent_t ev = prof_begin(ctx, KC_GSRB);
 #define FUSED_CASE(n, X, Y, NT)                                                                               \
   case n:                                                                                                     \
@@ -678,10 +689,10 @@ static void launch_fused(Context* ctx, int shape, int grid, const FusedArgs& a, 
   }
 #undef FUSED_CASE
   hip_check(hipGetLastError(), "k_gsrb_fused");
-  prof_end(ctx, ev, KC_GSRB, bytes);
+  prof_end(ctx, ev, KC_GSRB, bytes, units);
 }
 
-static FusedArgs fused_args(const DeviceLayout& L, const FaceRules& fr, int shape, int kc) {
+static FusedArgs fused_args(const DeviceLayout& L, const FaceRules& fr, int shape, int kc, bool zero_src = false) {
   if (!L.merged) throw std::runtime_error("gsrb_fused: level is not stored merged");
   if (shape < 0 || shape >= kNumFusedShapes) throw std::runtime_error("gsrb_fused: bad tile shape");
   const int FX = kFusedShapes[shape][0], FY = kFusedShapes[shape][1];
@@ -696,6 +707,7 @@ static FusedArgs fused_args(const DeviceLayout& L, const FaceRules& fr, int shap
   a.off = R.off;
   a.sy = R.sy;
   a.sz = R.sz;
+  a.zsrc = zero_src ? 1 : 0;
   a.ntx = (L.rect.len(0) + FX - 1) / FX;
   a.nty = (L.rect.len(1) + FY - 1) / FY;
   const int nz = L.rect.len(2);
@@ -713,9 +725,9 @@ static FusedArgs fused_args(const DeviceLayout& L, const FaceRules& fr, int shap
 
 template <class T>
 void gsrb_fused(const DeviceLayout& L, const FaceRules& fr, T* dst, const T* src, const T* rhs, double dx,
-                double alpha, double beta, int shape, int kc) {
+                double alpha, double beta, int shape, int kc, bool zero_src) {
   if (dst == src || dst == rhs) throw std::runtime_error("gsrb_fused: dst must be a separate field");
-  const FusedArgs a = fused_args(L, fr, shape, kc);
+  const FusedArgs a = fused_args(L, fr, shape, kc, zero_src);
   const int grid = (a.nwork + 7) / 8 * 8;
   double dxinv = 1.0 / (dx * dx);
   double sum_b = 0.0;
@@ -723,12 +735,15 @@ void gsrb_fused(const DeviceLayout& L, const FaceRules& fr, T* dst, const T* src
   const bool lap = (alpha == 0.0 && beta == 1.0);
   double lambda = lap ? -1.0 / sum_b : -1.0 / (alpha - beta * sum_b);
   const T c1 = T(fr.c1), c2 = T(fr.c2);
-  const double bytes = 24.0 * sizeof(T) / 8.0 * L.valid_cells;
+  // algorithmic bytes: read phi (not when it is known zero), read rhs, write phi
+  const double bytes = (zero_src ? 16.0 : 24.0) * sizeof(T) / 8.0 * L.valid_cells;
+  const double units = (double)L.valid_cells;  // one cell-relaxation per valid cell
   if (lap)
-    launch_fused<T, true>(L.ctx, shape, grid, a, dst, src, rhs, T(dxinv), T(lambda), T(alpha), T(beta), c1, c2, bytes);
+    launch_fused<T, true>(L.ctx, shape, grid, a, dst, src, rhs, T(dxinv), T(lambda), T(alpha), T(beta), c1, c2, bytes,
+                          units);
   else
     launch_fused<T, false>(L.ctx, shape, grid, a, dst, src, rhs, T(dxinv), T(lambda), T(alpha), T(beta), c1, c2,
-                           bytes);
+                           bytes, units);
 }
 
 // Times every tile shape (and z-chunk lengths) of the one-launch relaxation on
@@ -1318,7 +1333,7 @@ void reflux(Context* ctx, const RefluxCell* cells, int ncells, const RefluxCoars
   template void fill_ghosts<T>(const DeviceLayout&, const GhostPlan&, T*, int, int, double, double);           \
   template void gsrb<T>(const DeviceLayout&, T*, const T*, int, double, double, double);                       \
   template void gsrb_fused<T>(const DeviceLayout&, const FaceRules&, T*, const T*, const T*, double, double,      \
-                              double, int, int);                                                              \
+                              double, int, int, bool);                                                        \
   template std::pair<int, int> tune_gsrb_fused<T>(const DeviceLayout&, const FaceRules&, T*, const T*, const T*,  \
                                                   double, double, double, double*);                          \
   template void apply<T>(const DeviceLayout&, T*, const T*, double, double, double);                           \

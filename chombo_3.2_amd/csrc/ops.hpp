@@ -12,7 +12,7 @@
 namespace amrmg {
 void profiler_enable(Context* c, unsigned class_mask);  // 0 = off
 void profiler_reset(Context* c);
-bool profiler_read(Context* c, int kind, double* ms, double* bytes, long long* count);
+bool profiler_read(Context* c, int kind, double* ms, double* bytes, long long* count, double* units = nullptr);
 // average ms of one copy of nbytes (read nbytes + write nbytes) over iters launches
 double copy_probe(Context* c, long long nbytes, int iters);
 
@@ -52,10 +52,11 @@ struct FaceRules {
 };
 // One GSRB iteration (red, refresh, black) of a merged level in one launch:
 // dst <- relaxed src.  Bitwise equal to fill_ghosts + gsrb(0) + fill_ghosts + gsrb(1).
-// shape: tile shape index (kFusedShapes in kernels.hip), kc: z-chunk length (0 = automatic)
+// shape: tile shape index (kFusedShapes in kernels.hip), kc: z-chunk length (0 = automatic);
+// zero_src: src is known to be zero everywhere and is not read (same result as reading it)
 template <class T>
 void gsrb_fused(const DeviceLayout& L, const FaceRules& fr, T* dst, const T* src, const T* rhs, double dx,
-                double alpha, double beta, int shape = 0, int kc = 0);
+                double alpha, double beta, int shape = 0, int kc = 0, bool zero_src = false);
 // Autotuner of the one-launch relaxation: times every tile shape with z-chunk
 // lengths {auto, 8, 16, 32} on the given fields, returns the fastest {shape, kc};
 // times (kFusedShapeCount x 4 entries, ms per launch) receives all timings when not null.

@@ -41,6 +41,12 @@ class MGLevelOp : public LinearOp<T> {
  public:
   virtual void createCoarser(T& coarse, const T& fine, bool ghosted) = 0;
   virtual void relax(T& correction, const T& residual, int iterations) = 0;
+  // setToZero(correction) followed by relax(correction, residual, iterations); an
+  // operator may produce the same valid values without reading the zeros
+  virtual void relaxFromZero(T& correction, const T& residual, int iterations) {
+    this->setToZero(correction);
+    relax(correction, residual, iterations);
+  }
   virtual void restrictResidual(T& resCoarse, T& phiFine, const T& rhsFine) = 0;
   virtual void prolongIncrement(T& phiThisLevel, const T& correctCoarse) = 0;
 };
@@ -236,22 +242,33 @@ class MultiGrid {
     }
   }
   void oneCycle(T& e, const T& res) { cycle(0, e, res); }
-  // MultiGrid::cycle (MultiGrid.H:630-696), V/W cycles
-  void cycle(int depth, T& correction, const T& residual) {
+  // oneCycle for a caller whose e is zero on entry (the AMR V-cycle's coarsest level)
+  void oneCycleFromZero(T& e, const T& res) { cycle(0, e, res, true); }
+  // MultiGrid::cycle (MultiGrid.H:630-696), V/W cycles.  zero: correction is zero
+  // on entry (as the reference's setToZero before each first coarser cycle makes
+  // it), so its first relaxation need not read it.
+  void cycle(int depth, T& correction, const T& residual, bool zero = false) {
+    auto relax = [&](int n) {
+      if (zero) m_op[depth]->relaxFromZero(correction, residual, n);
+      else m_op[depth]->relax(correction, residual, n);
+      zero = false;
+    };
     if (depth == m_depth - 1) {
       if (m_bottomCells == 1) {
-        m_op[depth]->relax(correction, residual, 1);
+        relax(1);
       } else {
-        m_op[depth]->relax(correction, residual, m_bottom);
+        relax(m_bottom);
         m_bottomSolver->solve(correction, residual);
       }
       return;
     }
     int cycles = m_cycle;
-    m_op[depth]->relax(correction, residual, m_pre);
+    relax(m_pre);
     m_op[depth]->restrictResidual(*m_residual[depth + 1], correction, residual);
-    m_op[depth]->setToZero(*m_correction[depth + 1]);
-    for (int img = 0; img < cycles; img++) cycle(depth + 1, *m_correction[depth + 1], *m_residual[depth + 1]);
+    // setToZero(m_correction[depth + 1]) is folded into the first coarser cycle's
+    // first relaxation (relaxFromZero)
+    for (int img = 0; img < cycles; img++)
+      cycle(depth + 1, *m_correction[depth + 1], *m_residual[depth + 1], img == 0);
     m_op[depth]->prolongIncrement(correction, *m_correction[depth + 1]);
     m_op[depth]->relax(correction, residual, m_post);
   }

@@ -134,6 +134,11 @@ int amrmg_plan_check(const int* ispec, const double* rspec, long long* stats, in
 int amrmg_profile_enable(amrmg_ctx* ctx, int class_mask);
 int amrmg_profile_reset(amrmg_ctx* ctx);
 int amrmg_profile_read(amrmg_ctx* ctx, int kernel_class, double* ms, double* bytes, long long* launches);
+/* work units of a class since the last reset: cell-relaxations (one red + one black
+   update of a cell) for class 0; 0 for the others.  bytes above are algorithmic
+   bytes: a relaxation that starts from a correction known to be zero reads no old
+   values and counts 16 of the 24 bytes per cell-relax in FP64. */
+int amrmg_profile_read_units(amrmg_ctx* ctx, int kernel_class, double* units);
 
 #ifdef __cplusplus
 }

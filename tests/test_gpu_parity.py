@@ -143,8 +143,16 @@ def _solve_pair(gpu_ctx, h, rhs, **kw):
     return phi_g, hist_g, info_g, phi_o, hist_o, info_o
 
 
-@pytest.mark.parametrize("case", ["single32", "amr32", "helm_single24", "amr_lshape", "helm_lshape", "amr64"])
+@pytest.mark.parametrize("case", ["single32", "amr32", "helm_single24", "amr_lshape", "helm_lshape", "amr64",
+                                  "single32_w", "amr32_w"])
 def test_solve_matches_oracle(gpu_ctx, case):
+    kw = {}
+    if case.endswith("_w"):
+        # numMG = 2: W-cycles in MultiGrid and two coarser AMR cycles per level.  With
+        # the AMR hierarchy this iteration does not converge (the oracle's history
+        # grows and it stops on the hang test); the GPU must follow it exactly.
+        kw = dict(numMG=2)
+        case = case[:-2]
     if case == "amr64":
         # the benchmark's hierarchy at base 64: 64^3, 80^3 and 100^3 levels, 32^3 boxes
         h = H.amr_hierarchy(64, fracs=(0.25, 0.06), max_box=32, block=8)
@@ -164,7 +172,7 @@ def test_solve_matches_oracle(gpu_ctx, case):
     else:
         h = small_single(n=24, box=8, alpha=0.4, beta=1.7)
         rhs = H.sinusoid_rhs(h, 1004, k=(1, 2, 3))
-    phi_g, hist_g, info_g, phi_o, hist_o, info_o = _solve_pair(gpu_ctx, h, rhs)
+    phi_g, hist_g, info_g, phi_o, hist_o, info_o = _solve_pair(gpu_ctx, h, rhs, **kw)
     assert info_g["iterations"] == info_o["iterations"]
     assert info_g["exit_status"] == info_o["exit_status"]
     np.testing.assert_allclose(hist_g, hist_o, rtol=1e-9, atol=0)
