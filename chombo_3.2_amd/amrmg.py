@@ -41,6 +41,15 @@ def lib():
         L.amrmg_ctx_destroy.argtypes = [C.c_void_p]
         L.amrmg_ctx_set_fused_relax.argtypes = [C.c_void_p, C.c_int]
         L.amrmg_profile_read_units.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.amrmg_batch_create.argtypes = [C.c_void_p, C.c_int, _ip, _dp, C.c_int, C.POINTER(C.c_void_p)]
+        L.amrmg_batch_destroy.argtypes = [C.c_void_p]
+        L.amrmg_batch_set_params.argtypes = [C.c_void_p, _ip, _dp]
+        L.amrmg_batch_set_rhs.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.amrmg_batch_get_phi.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.amrmg_batch_solve.argtypes = [C.c_void_p, _ip, _ip]
+        L.amrmg_batch_history.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _ip]
+        L.amrmg_batch_tune_relax.argtypes = [C.c_void_p, _dp, C.c_int, _ip]
+        L.amrmg_batch_stats.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
         L.amrmg_solver_create.argtypes = [C.c_void_p, _ip, _dp, C.c_int, C.POINTER(C.c_void_p)]
         L.amrmg_solver_destroy.argtypes = [C.c_void_p]
         L.amrmg_solver_set_params.argtypes = [C.c_void_p, _ip, _dp]
@@ -306,3 +315,78 @@ class AMRMultiGrid:
         _chk(lib().amrmg_bottom_solve(self.p, corr.ctypes.data_as(_dp), pr, nbottom, metric, eps,
                                        info.ctypes.data_as(_ip)))
         return corr, int(info[0])
+
+
+class BatchSolver:
+    """Many independent one-level problems solved together, each V-cycle stage of
+    all of them in one launch (amrmg_batch_*).  Per problem the arithmetic is that
+    of AMRMultiGrid on it alone."""
+
+    def __init__(self, ctx: Context, hiers, precision: int = 0):
+        self.ctx = ctx
+        self.hiers = list(hiers)
+        isps, rsps = [], []
+        for h in self.hiers:
+            if h.nlev != 1:
+                raise ValueError("batched problems have one level")
+            i, r = h.spec()
+            isps.append(np.asarray(i, np.int32))
+            rsps.append(np.asarray(r, np.float64))
+        isp = np.ascontiguousarray(np.concatenate(isps))
+        rsp = np.ascontiguousarray(np.concatenate(rsps))
+        p = C.c_void_p()
+        _chk(lib().amrmg_batch_create(ctx.p, len(self.hiers), isp.ctypes.data_as(_ip), rsp.ctypes.data_as(_dp),
+                                       precision, C.byref(p)))
+        self.p = p
+        self.n = len(self.hiers)
+
+    def close(self):
+        if self.p:
+            lib().amrmg_batch_destroy(self.p)
+            self.p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def setSolverParameters(self, sparams, rparams):
+        sp = np.ascontiguousarray(sparams, dtype=np.int32)
+        rp = np.ascontiguousarray(rparams, dtype=np.float64)
+        _chk(lib().amrmg_batch_set_params(self.p, sp.ctypes.data_as(_ip), rp.ctypes.data_as(_dp)))
+
+    def set_rhs(self, q, data):
+        data, pd = _d(data)
+        if data.size != self.hiers[q].total_cells():
+            raise ValueError("rhs size does not match the problem")
+        _chk(lib().amrmg_batch_set_rhs(self.p, int(q), pd))
+
+    def get_phi(self, q):
+        out = np.zeros(self.hiers[q].total_cells())
+        _chk(lib().amrmg_batch_get_phi(self.p, int(q), out.ctypes.data_as(_dp)))
+        return out
+
+    def solve(self):
+        it = np.zeros(self.n, np.int32)
+        ex = np.zeros(self.n, np.int32)
+        _chk(lib().amrmg_batch_solve(self.p, it.ctypes.data_as(_ip), ex.ctypes.data_as(_ip)))
+        return it, ex
+
+    def history(self, q, cap=256):
+        h = np.zeros(cap)
+        n = C.c_int()
+        _chk(lib().amrmg_batch_history(self.p, int(q), h.ctypes.data_as(_dp), cap, C.byref(n)))
+        return h[:min(n.value, cap)].copy()
+
+    def tune_relax(self):
+        ms = np.zeros(64)
+        n = C.c_int()
+        _chk(lib().amrmg_batch_tune_relax(self.p, ms.ctypes.data_as(_dp), 64, C.byref(n)))
+        return ms[:n.value].copy()
+
+    def stats(self):
+        cr = C.c_double()
+        bi, db = C.c_longlong(), C.c_longlong()
+        _chk(lib().amrmg_batch_stats(self.p, C.byref(cr), C.byref(bi), C.byref(db)))
+        return dict(cell_relax=cr.value, bottom_iters=bi.value, device_bytes=db.value)

@@ -1,0 +1,1018 @@
+// Batched solver (see batch.hpp): kernels that run one V-cycle stage of every
+// problem in one launch, the per-problem bottom BiCGStab as one workgroup per
+// problem, and the host driver of the lockstep outer iteration.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <numeric>
+#include <stdexcept>
+#include <type_traits>
+
+#include "batch.hpp"
+
+namespace amrmg {
+
+namespace {
+
+__device__ __forceinline__ long long ridx(const DBox& b, int i, int j, int k) {
+  return b.off + (i - b.lo[0] + 1) + (long long)(j - b.lo[1] + 1) * b.sy + (long long)(k - b.lo[2] + 1) * b.sz;
+}
+
+// ConstBCFunction ghost value (BCFunc.cpp:58-158) with the expressions of
+// k_fill_ghosts: Dirichlet g = 2v - near, Neumann g = near (+ isign*dx*v)
+template <class T>
+__device__ __forceinline__ T bc_ghost(const BatchCoef& cf, int dir, int side, T nearv, int homog) {
+  const int isign = side ? 1 : -1;
+  if (cf.kind[dir][side] == 1) {
+    T inhomogVal = T(0.0);
+    if (!homog) inhomogVal = T(cf.val[dir][side]);
+    return T(2) * inhomogVal - nearv;
+  }
+  T v = nearv;
+  if (!homog) v += T(double(isign) * cf.dx) * T(cf.val[dir][side]);
+  return v;
+}
+
+// the six neighbours of cell c of rectangle R, with the face ghosts of the
+// rectangle formed from the cell itself (the fill-then-read of the reference)
+template <class T>
+struct Six {
+  T xp, xm, yp, ym, zp, zm;
+};
+template <class T>
+__device__ __forceinline__ Six<T> six(const T* phi, const DBox& R, const BatchCoef& cf, long long c, int i, int j,
+                                      int k, int homog) {
+  const T p = phi[c];
+  const long long sy = R.sy, sz = R.sz;
+  Six<T> s;
+  s.xp = (i == R.hi[0]) ? bc_ghost<T>(cf, 0, 1, p, homog) : phi[c + 1];
+  s.xm = (i == R.lo[0]) ? bc_ghost<T>(cf, 0, 0, p, homog) : phi[c - 1];
+  s.yp = (j == R.hi[1]) ? bc_ghost<T>(cf, 1, 1, p, homog) : phi[c + sy];
+  s.ym = (j == R.lo[1]) ? bc_ghost<T>(cf, 1, 0, p, homog) : phi[c - sy];
+  s.zp = (k == R.hi[2]) ? bc_ghost<T>(cf, 2, 1, p, homog) : phi[c + sz];
+  s.zm = (k == R.lo[2]) ? bc_ghost<T>(cf, 2, 0, p, homog) : phi[c - sz];
+  return s;
+}
+
+// cell of this thread in its tile (64 x 4 x 1 layout, TX = 1 << lx, TY = 1 << ly)
+__device__ __forceinline__ bool btile_cell(const Tile& t, const DBox& R, int lx, int ly, int& i, int& j, int& k) {
+  const int tid = threadIdx.x;
+  i = t.i0 + (tid & ((1 << lx) - 1));
+  j = t.j0 + ((tid >> lx) & ((1 << ly) - 1));
+  k = t.k0 + (tid >> (lx + ly));
+  return i <= R.hi[0] && j <= R.hi[1] && k <= R.hi[2];
+}
+
+// ---------------------------------------------------------------------------
+// one-launch relaxation of every problem present at a depth
+// ---------------------------------------------------------------------------
+template <class T, bool LAP, int FX, int FY, int NT>
+__global__ void __launch_bounds__(NT)
+    k_batch_fused(const ops::FusedArgs* __restrict__ items, const int* __restrict__ wstart, int nitems,
+                  const BatchCoef* __restrict__ coef, const int* __restrict__ active, int zsrc, T* __restrict__ dst,
+                  const T* __restrict__ src, const T* __restrict__ rhs) {
+  const int wg = ops::xcd_contiguous_block();
+  if (wg >= wstart[nitems]) return;
+  int lo = 0, hi = nitems - 1;  // the problem whose work range holds wg
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (wstart[mid] <= wg) lo = mid;
+    else hi = mid - 1;
+  }
+  const int p = lo;
+  if (!active[p] || coef[p].lap != (LAP ? 1 : 0)) return;
+  const BatchCoef cf = coef[p];
+  ops::FusedArgs a = items[p];
+  a.zsrc = zsrc;
+  ops::fused_column<T, LAP, FX, FY, NT>(a, wg - wstart[p], dst, src, rhs, T(cf.dxinv), T(cf.lambda), T(cf.alpha),
+                                        T(cf.beta), T(0.0), T(0.0));
+}
+
+// RESTRICTRES (AMRPoissonOpF.ChF:394-438) after the homogeneous BC refresh: one
+// thread per coarse cell, its eight children in the Fortran loop order
+template <class T>
+__global__ void __launch_bounds__(256)
+    k_batch_restrict(const Tile* __restrict__ ctiles, int lx, int ly, const DBox* __restrict__ crect,
+                     const DBox* __restrict__ frect, const BatchCoef* __restrict__ fcoef,
+                     const int* __restrict__ active, T* __restrict__ resc, const T* __restrict__ phi,
+                     const T* __restrict__ rhs) {
+  const Tile t = ctiles[blockIdx.x];
+  if (!active[t.box]) return;
+  const DBox C = crect[t.box], F = frect[t.box];
+  const BatchCoef cf = fcoef[t.box];
+  int I, J, K;
+  if (!btile_cell(t, C, lx, ly, I, J, K)) return;
+  const T alpha = T(cf.alpha), beta = T(cf.beta), dxinv = T(cf.dxinv);
+  const T denom = T(8.0);
+  T acc = T(0.0);
+#pragma unroll
+  for (int kk = 0; kk < 2; kk++)
+#pragma unroll
+    for (int jj = 0; jj < 2; jj++)
+#pragma unroll
+      for (int ii = 0; ii < 2; ii++) {
+        const int i = 2 * I + ii, j = 2 * J + jj, k = 2 * K + kk;
+        const long long c = ridx(F, i, j, k);
+        const T p = phi[c];
+        const Six<T> s = six<T>(phi, F, cf, c, i, j, k, 1);
+        const T lofphi = alpha * p + beta * (s.xp + s.xm + s.yp + s.ym + s.zp + s.zm - p * T(2.0) * T(3.0)) * dxinv;
+        acc = acc + (rhs[c] - lofphi) / denom;
+      }
+  resc[ridx(C, I, J, K)] = acc;
+}
+
+// PROLONG (AMRPoissonOpF.ChF:453-478)
+template <class T>
+__global__ void __launch_bounds__(256)
+    k_batch_prolong(const Tile* __restrict__ ftiles, int lx, int ly, const DBox* __restrict__ frect,
+                    const DBox* __restrict__ crect, const int* __restrict__ active, T* __restrict__ phi,
+                    const T* __restrict__ coarse) {
+  const Tile t = ftiles[blockIdx.x];
+  if (!active[t.box]) return;
+  const DBox F = frect[t.box], C = crect[t.box];
+  int i, j, k;
+  if (!btile_cell(t, F, lx, ly, i, j, k)) return;
+  const long long c = ridx(F, i, j, k);
+  phi[c] = phi[c] + coarse[ridx(C, i >> 1, j >> 1, k >> 1)];
+}
+
+// OPERATORLAPRES (AMRPoissonOpF.ChF:341-377) with the inhomogeneous BC refresh, the
+// per-problem max-norm of the result, and the result stored in the V-cycle's
+// precision (computeAMRResidual of a single level, AMRMultiGrid.H:683-736)
+template <class TO>
+__global__ void __launch_bounds__(256)
+    k_batch_residual_norm(const Tile* __restrict__ tiles, int lx, int ly, const DBox* __restrict__ rect,
+                          const BatchCoef* __restrict__ coef, const int* __restrict__ active, TO* __restrict__ res,
+                          const double* __restrict__ phi, const double* __restrict__ rhs, double* norms) {
+  const Tile t = tiles[blockIdx.x];
+  if (!active[t.box]) return;
+  const DBox R = rect[t.box];
+  const BatchCoef cf = coef[t.box];
+  int i, j, k;
+  double v = 0.0;
+  if (btile_cell(t, R, lx, ly, i, j, k)) {
+    const long long c = ridx(R, i, j, k);
+    const double p = phi[c];
+    const Six<double> s = six<double>(phi, R, cf, c, i, j, k, 0);
+    const double lap = (s.xp + s.xm + s.yp + s.ym + s.zp + s.zm - 6.0 * p) * cf.dxinv;
+    const double r = -(cf.alpha * p) - cf.beta * lap + rhs[c];
+    res[c] = (TO)r;
+    v = fabs(r);
+  }
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  __shared__ double sm[4];
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double m = fmax(fmax(sm[0], sm[1]), fmax(sm[2], sm[3]));
+    if (m > 0.0) atomicMax((unsigned long long*)(norms + t.box), (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+// phi += correction (incr(phi, uberCorrection, 1.0) of solveNoInitResid; a mixed
+// solve widens the FP32 correction first, as incr_f2d into a zero FP64 field)
+template <class T>
+__global__ void __launch_bounds__(256)
+    k_batch_update(const Tile* __restrict__ tiles, int lx, int ly, const DBox* __restrict__ rect,
+                   const int* __restrict__ active, double* __restrict__ phi, const T* __restrict__ corr) {
+  const Tile t = tiles[blockIdx.x];
+  if (!active[t.box]) return;
+  const DBox R = rect[t.box];
+  int i, j, k;
+  if (!btile_cell(t, R, lx, ly, i, j, k)) return;
+  const long long c = ridx(R, i, j, k);
+  if constexpr (std::is_same<T, double>::value) {
+    phi[c] = phi[c] + corr[c] * 1.0;
+  } else {
+    const double x = 0.0 + (double)corr[c];  // uberCorrection (zero) += widened correction
+    phi[c] = phi[c] + x * 1.0;
+  }
+}
+
+__global__ void k_batch_zero_norms(double* norms, const int* active, int n) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n && active[p]) norms[p] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// bottom of every problem: relax(m_bottom) from zero, then BiCGStabSolver::solve
+// (BiCGStabSolver.H:199-519) in FP64 on the deepest level, one workgroup per problem.
+// Level operations mirror AMRPoissonOp on that level (homogeneous BC) and the
+// reductions keep the reference orders (dot: per box serially in Fortran order,
+// boxes in layout order; norm 2: FArrayBox::sumPow column sums per box).
+// ---------------------------------------------------------------------------
+struct BottomArgs {
+  int imax;
+  double reps, hang, small;
+  int numRestarts;
+  int nbottom;                 // relax iterations before the Krylov solve
+};
+
+struct BottomLevel {
+  BatchCoef cf;
+  int tid, nt;
+  long long ncell;
+  int lo[3], nx, ny;
+  __device__ void cell(long long q, int& i, int& j, int& k) const {
+    i = lo[0] + (int)(q % nx);
+    j = lo[1] + (int)((q / nx) % ny);
+    k = lo[2] + (int)(q / ((long long)nx * ny));
+  }
+  // one GSRB colour (GSRBLAPLACIAN / GSRBHELMHOLTZ) with the homogeneous BC ghosts,
+  // on a field addressed through R
+  template <class U>
+  __device__ void gsrb(const DBox& R, U* phi, const U* rhs, int color) const {
+    const U dxinv = U(cf.dxinv), lambda = U(cf.lambda), alpha = U(cf.alpha), beta = U(cf.beta);
+    for (long long q = tid; q < ncell; q += nt) {
+      int i, j, k;
+      cell(q, i, j, k);
+      if ((i + j + k + color) & 1) continue;
+      const long long c = ridx(R, i, j, k);
+      const Six<U> s = six<U>(phi, R, cf, c, i, j, k, 1);
+      const U s6 = s.xp + s.xm + s.yp + s.ym + s.zp + s.zm;
+      if (cf.lap) {
+        const U lphi = s6 * dxinv;
+        phi[c] = lambda * (rhs[c] - lphi);
+      } else {
+        const U p = phi[c];
+        const U lphi = (s6 - U(6.0) * p) * dxinv;
+        const U helmop = alpha * p + beta * lphi;
+        phi[c] = p + lambda * (helmop - rhs[c]);
+      }
+    }
+    __syncthreads();
+  }
+  template <class U>
+  __device__ void relax(const DBox& R, U* phi, const U* rhs, int n) const {
+    for (int it = 0; it < n; it++) {
+      gsrb(R, phi, rhs, 0);
+      gsrb(R, phi, rhs, 1);
+    }
+  }
+};
+
+__device__ double block_sum_serial_boxes(double* sh, const DBox* boxes, int nbox, const double* a, const double* b,
+                                         int tid) {
+  // FArrayBox::dotProduct per box (ForAllThisCBNNXC, Fortran order), boxes in order
+  for (int q = tid; q < nbox; q += blockDim.x) {
+    const DBox bx = boxes[q];
+    double dot = 0.0;
+    for (int k = bx.lo[2]; k <= bx.hi[2]; k++)
+      for (int j = bx.lo[1]; j <= bx.hi[1]; j++)
+        for (int i = bx.lo[0]; i <= bx.hi[0]; i++) {
+          const long long c = ridx(bx, i, j, k);
+          dot += a[c] * b[c];
+        }
+    sh[q] = dot;
+  }
+  __syncthreads();
+  __shared__ double res;
+  if (tid == 0) {
+    double val = 0.0;
+    for (int q = 0; q < nbox; q++) val += sh[q];
+    res = val;
+  }
+  __syncthreads();
+  const double r = res;
+  __syncthreads();
+  return r;
+}
+
+__device__ double block_norm2_boxes(double* sh, const DBox* boxes, int nbox, const double* x, int tid) {
+  // FArrayBox::sumPow(2) per box: column sums over (k, j), columns summed in i
+  // order; boxes summed in order; then sqrt (norm(.., 2))
+  __shared__ double normAcc;
+  if (tid == 0) normAcc = 0.0;
+  __syncthreads();
+  for (int q = 0; q < nbox; q++) {
+    const DBox bx = boxes[q];
+    const int nx = bx.hi[0] - bx.lo[0] + 1;
+    for (int i = tid; i < nx; i += blockDim.x) {
+      double t = 0.0;
+      bool first = true;
+      for (int k = bx.lo[2]; k <= bx.hi[2]; k++)
+        for (int j = bx.lo[1]; j <= bx.hi[1]; j++) {
+          const double r = x[ridx(bx, bx.lo[0] + i, j, k)];
+          if (first) { t = r * r; first = false; }
+          else t += r * r;
+        }
+      sh[i] = t;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double sum = sh[0];
+      for (int i = 1; i < nx; i++) sum += sh[i];
+      normAcc += sum;
+    }
+    __syncthreads();
+  }
+  const double n = normAcc;
+  __syncthreads();
+  return std::sqrt(n);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256)
+    k_batch_bottom(const BatchBottom* __restrict__ bots, const BatchCoef* __restrict__ bcoef,
+                   const DBox* __restrict__ bboxes, T* const* __restrict__ corr, const T* const* __restrict__ res,
+                   double* __restrict__ scratch, const long long* __restrict__ soff, const double* __restrict__ metric,
+                   const int* __restrict__ active, int* __restrict__ stats, BottomArgs ba) {
+  const int p = blockIdx.x;
+  if (!active[p]) return;
+  __shared__ double sh[1024];
+  const BatchBottom bt = bots[p];
+  const DBox R = bt.rect;  // the level's fields in its depth's slab
+  DBox Rs = R;             // the same shape in this problem's scratch vectors
+  Rs.off = 15;
+  BottomLevel L;
+  L.cf = bcoef[p];
+  L.tid = threadIdx.x;
+  L.nt = blockDim.x;
+  for (int q = 0; q < 3; q++) L.lo[q] = R.lo[q];
+  L.nx = R.hi[0] - R.lo[0] + 1;
+  L.ny = R.hi[1] - R.lo[1] + 1;
+  L.ncell = bt.cells;
+  const DBox* boxes = bboxes + bt.box0;  // addressed like Rs
+  const int nbox = bt.nbox;
+  const int tid = threadIdx.x;
+  T* e32 = corr[p];
+  const T* r32 = res[p];
+  // MultiGrid::cycle at the bottom: the correction is zero on entry
+  for (long long q = tid; q < L.ncell; q += L.nt) {
+    int i, j, k;
+    L.cell(q, i, j, k);
+    e32[ridx(R, i, j, k)] = T(0.0);
+  }
+  __syncthreads();
+  L.relax(R, e32, r32, ba.nbottom);
+  if (L.ncell == 1) return;  // m_bottomCells == 1: relax only
+  const long long vs = 15 + (long long)Rs.sz * (R.hi[2] - R.lo[2] + 3);  // elements per scratch vector
+  double* base = scratch + soff[p];
+  double *x = base, *b = base + vs, *r = base + 2 * vs, *r_tilde = base + 3 * vs, *e = base + 4 * vs,
+         *pp = base + 5 * vs, *p_tilde = base + 6 * vs, *s_tilde = base + 7 * vs, *t = base + 8 * vs,
+         *v = base + 9 * vs;
+  auto each = [&](auto fn) {
+    for (long long q = tid; q < L.ncell; q += L.nt) {
+      int i, j, k;
+      L.cell(q, i, j, k);
+      fn(ridx(Rs, i, j, k), i, j, k);
+    }
+    __syncthreads();
+  };
+  // PromotedBottomSolver: widen the correction and the residual
+  for (long long q = tid; q < L.ncell; q += L.nt) {
+    int i, j, k;
+    L.cell(q, i, j, k);
+    x[ridx(Rs, i, j, k)] = (double)e32[ridx(R, i, j, k)];
+    b[ridx(Rs, i, j, k)] = (double)r32[ridx(R, i, j, k)];
+  }
+  __syncthreads();
+  const BatchCoef cf = L.cf;
+  // residual / applyOp of AMRPoissonOp on this level, homogeneous BC
+  auto residual = [&](double* out, const double* phi, const double* rhs) {
+    each([&](long long c, int i, int j, int k) {
+      const double pv = phi[c];
+      const Six<double> s6v = six<double>(phi, Rs, cf, c, i, j, k, 1);
+      const double lap = (s6v.xp + s6v.xm + s6v.yp + s6v.ym + s6v.zp + s6v.zm - 6.0 * pv) * cf.dxinv;
+      out[c] = -(cf.alpha * pv) - cf.beta * lap + rhs[c];
+    });
+  };
+  auto applyOp = [&](double* out, const double* phi) {
+    each([&](long long c, int i, int j, int k) {
+      const double pv = phi[c];
+      const Six<double> s6v = six<double>(phi, Rs, cf, c, i, j, k, 1);
+      const double lap = (s6v.xp + s6v.xm + s6v.yp + s6v.ym + s6v.zp + s6v.zm - 6.0 * pv) * cf.dxinv;
+      out[c] = cf.alpha * pv + cf.beta * lap;
+    });
+  };
+  // preCond (AMRPoissonOp.cpp:304-329): phi = rhs * mult, then relax(phi, rhs, 2)
+  const double mult = 1.0 / (cf.alpha - 2.0 * 3 * cf.beta / (cf.dx * cf.dx));
+  auto preCond = [&](double* cor, const double* rv) {
+    each([&](long long c, int, int, int) { cor[c] = rv[c] * mult; });
+    L.relax(Rs, cor, rv, 2);
+  };
+  auto incr = [&](double* lhs, const double* xv, double s) {
+    each([&](long long c, int, int, int) { lhs[c] = lhs[c] + xv[c] * s; });
+  };
+  auto assign = [&](double* lhs, const double* rv) { each([&](long long c, int, int, int) { lhs[c] = rv[c]; }); };
+  auto zero = [&](double* lhs) { each([&](long long c, int, int, int) { lhs[c] = 0.0; }); };
+  auto scale = [&](double* lhs, double s) { each([&](long long c, int, int, int) { lhs[c] = lhs[c] * s; }); };
+  auto dot = [&](const double* a, const double* bb) { return block_sum_serial_boxes(sh, boxes, nbox, a, bb, tid); };
+  auto norm = [&](const double* a) { return block_norm2_boxes(sh, boxes, nbox, a, tid); };
+
+  const double eps = metric[2 * p + 1];
+  const double convergenceMetric = metric[2 * p];
+  int recount = 0;
+  zero(r);
+  residual(r, x, b);
+  assign(r_tilde, r);
+  zero(e);
+  zero(p_tilde);
+  zero(s_tilde);
+  int iter = 0;
+  double rho[4] = {0, 0, 0, 0}, nrm[2];
+  nrm[0] = norm(r);
+  double initial_norm = nrm[0];
+  const double initial_residual = nrm[0];
+  nrm[1] = nrm[0];
+  double alpha[2] = {0, 0}, beta[2] = {0, 0}, omega[2] = {0, 0};
+  bool init = true;
+  int restarts = 0;
+  bool done = false;
+  if (convergenceMetric > 0) initial_norm = convergenceMetric;
+  while (!done && (iter < ba.imax && nrm[0] > eps * nrm[1]) && (nrm[1] > 0)) {
+    iter++;
+    nrm[1] = nrm[0]; alpha[1] = alpha[0]; beta[1] = beta[0]; omega[1] = omega[0];
+    rho[3] = rho[2]; rho[2] = rho[1];
+    rho[1] = dot(r_tilde, r);
+    if (rho[1] == 0.0) {  // exit status 2
+      incr(x, e, 1.0);
+      done = true;
+      break;
+    }
+    if (init) {
+      assign(pp, r);
+      init = false;
+    } else {
+      beta[1] = (rho[1] / rho[2]) * (alpha[1] / omega[1]);
+      scale(pp, beta[1]);
+      incr(pp, v, -beta[1] * omega[1]);
+      incr(pp, r, 1.0);
+    }
+    preCond(p_tilde, pp);
+    zero(v);
+    applyOp(v, p_tilde);
+    const double m = dot(r_tilde, v);
+    alpha[0] = rho[1] / m;
+    if (fabs(m) > ba.small * fabs(rho[1])) {
+      incr(r, v, -alpha[0]);
+      nrm[0] = norm(r);
+      incr(e, p_tilde, alpha[0]);
+    } else {
+      zero(r);
+      nrm[0] = 0.0;
+    }
+    if (nrm[0] > eps * initial_norm && nrm[0] > ba.reps * initial_residual) {
+      preCond(s_tilde, r);
+      zero(t);
+      applyOp(t, s_tilde);
+      omega[0] = dot(t, r) / dot(t, t);
+      incr(e, s_tilde, omega[0]);
+      incr(r, t, -omega[0]);
+      nrm[0] = norm(r);
+    }
+    if (nrm[0] <= eps * initial_norm || nrm[0] <= ba.reps * initial_residual) break;  // exit status 1
+    if (omega[0] == 0.0 || nrm[0] > (1 - ba.hang) * nrm[1]) {
+      if (recount == 0) {
+        recount = 1;
+      } else {
+        recount = 0;
+        incr(x, e, 1.0);
+        if (restarts == ba.numRestarts) {  // exit status 3
+          done = true;
+          break;
+        }
+        residual(r, x, b);
+        nrm[0] = norm(r);
+        rho[1] = 0.0; rho[2] = 0.0; rho[3] = 0.0;
+        alpha[0] = 0; beta[0] = 0; omega[0] = 0;
+        assign(r_tilde, r);
+        zero(e);
+        restarts++;
+        init = true;
+      }
+    }
+  }
+  if (!done) incr(x, e, 1.0);
+  if (tid == 0) stats[p] += iter;
+  // round the correction back into the V-cycle's precision
+  for (long long q = tid; q < L.ncell; q += L.nt) {
+    int i, j, k;
+    L.cell(q, i, j, k);
+    e32[ridx(R, i, j, k)] = (T)x[ridx(Rs, i, j, k)];
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+void BatchSolver::alloc(void** p, size_t bytes) {
+  dev_alloc(m_ctx, p, bytes, "hipMalloc batch");
+  if (*p) hip_check(hipMemsetAsync(*p, 0, bytes, m_ctx->stream), "hipMemset batch");
+  m_bytes += (long long)bytes;
+}
+
+template <class V>
+static V* upload(Context* c, const std::vector<V>& h, long long* bytes) {
+  if (h.empty()) return nullptr;
+  void* p = nullptr;
+  dev_alloc(c, &p, sizeof(V) * h.size(), "hipMalloc batch table");
+  dev_h2d(c, p, h.data(), sizeof(V) * h.size(), "copy batch table");
+  *bytes += (long long)(sizeof(V) * h.size());
+  return (V*)p;
+}
+
+static int mg_depths(const DBL& g) {
+  int n = 1;
+  for (int depth = 1;; depth++) {  // DeviceHierarchy::define / MGnewOp's coarsening rule
+    if (!g.coarsenable((1 << depth) * 2)) break;
+    n++;
+  }
+  return n;
+}
+
+BatchSolver::BatchSolver(Context* ctx, std::vector<BatchProblemSpec> problems, int precision)
+    : m_ctx(ctx), m_prec(precision), m_np((int)problems.size()) {
+  if (precision != 0 && precision != 1) throw std::runtime_error("batch: precision must be 0 or 1");
+  if (m_np <= 0) throw std::runtime_error("batch: no problems");
+  std::vector<int> nd(m_np);
+  for (int q = 0; q < m_np; q++) {
+    const BatchProblemSpec& P = problems[q];
+    if (P.boxes.size() == 0) throw std::runtime_error("batch: a problem has no boxes");
+    if (!P.boxes.sorted()) throw std::runtime_error("batch: boxes must be sorted (Box::operator<)");
+    if (P.boxes.numCells() != P.domain.npts()) throw std::runtime_error("batch: boxes must tile the domain");
+    for (auto& b : P.boxes.boxes)
+      if (!P.domain.contains(b.lo) || !P.domain.contains(b.hi)) throw std::runtime_error("batch: box outside domain");
+    nd[q] = mg_depths(P.boxes);
+  }
+  // slots: problems by depth count, descending (stable)
+  std::vector<int> order(m_np);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return nd[a] > nd[b]; });
+  m_slot.assign(m_np, 0);
+  for (int s = 0; s < m_np; s++) {
+    m_prob.push_back(problems[order[s]]);
+    m_ndepth.push_back(nd[order[s]]);
+    m_slot[order[s]] = s;
+  }
+  const int D = m_ndepth[0];
+  m_depth.resize(D);
+  for (int d = 0; d < D; d++) {
+    Depth& L = m_depth[d];
+    while (L.np < m_np && m_ndepth[L.np] > d) L.np++;
+    long long off = 0;
+    L.tileStart.assign(1, 0);
+    L.wStart.assign(1, 0);
+    L.cellStart.assign(1, 0);
+    std::vector<Tile> tiles;
+    for (int p = 0; p < L.np; p++) {
+      const BatchProblemSpec& P = m_prob[p];
+      const Box r = coarsen(P.domain, 1 << d);
+      DBox x;
+      for (int q = 0; q < 3; q++) { x.lo[q] = r.lo[q]; x.hi[q] = r.hi[q]; }
+      x.sy = (r.len(0) + 2 + 15) / 16 * 16;
+      const long long sz = (long long)x.sy * (r.len(1) + 2);
+      if (sz > INT32_MAX) throw std::runtime_error("batch: problem plane too large");
+      x.sz = (int)sz;
+      x.off = off + 15;  // first valid cell of every row 128-byte aligned
+      off += ((15 + sz * (r.len(2) + 2)) + 15) / 16 * 16;
+      L.h_rect.push_back(x);
+      BatchCoef c{};
+      c.dx = P.dx * (double)(1 << d);
+      c.dxinv = 1.0 / (c.dx * c.dx);
+      double sum_b = 0.0;
+      for (int q = 0; q < 3; q++) sum_b = sum_b + 2.0 * c.dxinv;
+      c.lap = (P.alpha == 0.0 && P.beta == 1.0) ? 1 : 0;
+      c.lambda = c.lap ? -1.0 / sum_b : -1.0 / (P.alpha - P.beta * sum_b);
+      c.alpha = P.alpha;
+      c.beta = P.beta;
+      for (int dir = 0; dir < 3; dir++)
+        for (int side = 0; side < 2; side++) {
+          c.kind[dir][side] = P.bc.type[side][dir];
+          c.val[dir][side] = P.bc.val[side][dir];
+        }
+      L.h_coef.push_back(c);
+      for (int k = r.lo[2]; k <= r.hi[2]; k++)
+        for (int j = r.lo[1]; j <= r.hi[1]; j += 4)
+          for (int i = r.lo[0]; i <= r.hi[0]; i += 64) tiles.push_back(Tile{p, i, j, k});
+      L.tileStart.push_back((int)tiles.size());
+      L.cellStart.push_back(L.cellStart.back() + r.npts());
+    }
+    L.slab = off;
+    L.cellsPresent = L.cellStart.back();
+    L.d_rect = upload(m_ctx, L.h_rect, &m_bytes);
+    L.d_coef = upload(m_ctx, L.h_coef, &m_bytes);
+    L.d_tiles = upload(m_ctx, tiles, &m_bytes);
+    // tile addresses stay inside each problem's array (threads past the rectangle return)
+    for (const Tile& t : tiles) {
+      const DBox& x = L.h_rect[t.box];
+      if (t.i0 < x.lo[0] || t.i0 > x.hi[0] || t.j0 < x.lo[1] || t.j0 > x.hi[1] || t.k0 < x.lo[2] || t.k0 > x.hi[2])
+        throw std::runtime_error("batch: tile outside its problem");
+    }
+    const size_t es = m_prec ? sizeof(float) : sizeof(double);
+    alloc(&L.corr, es * L.slab);
+    alloc(&L.res, es * L.slab);
+    alloc(&L.tmp, es * L.slab);
+  }
+  alloc((void**)&m_phi, sizeof(double) * m_depth[0].slab);
+  alloc((void**)&m_rhs, sizeof(double) * m_depth[0].slab);
+  alloc((void**)&m_norm, sizeof(double) * m_np);
+  alloc((void**)&d_active, sizeof(int) * m_np);
+  alloc((void**)&d_metric, sizeof(double) * 2 * m_np);
+  alloc((void**)&d_bstats, sizeof(int) * m_np);
+  if (!m_ctx->dry) hip_check(hipHostMalloc((void**)&h_norm, sizeof(double) * m_np), "hipHostMalloc norms");
+  h_active.assign(m_np, 0);
+  // bottoms
+  std::vector<BatchCoef> bcoef;
+  std::vector<void*> corrp, resp;
+  long long soff = 0;
+  for (int p = 0; p < m_np; p++) {
+    const int db = m_ndepth[p] - 1;
+    const Depth& L = m_depth[db];
+    BatchBottom b{};
+    b.rect = L.h_rect[p];
+    b.depth = db;
+    b.cells = L.cellStart[p + 1] - L.cellStart[p];
+    b.box0 = (int)h_bbox.size();
+    const DBL g = m_prob[p].boxes.coarsened(1 << db);
+    b.nbox = g.size();
+    if (b.nbox > 1024) throw std::runtime_error("batch: too many bottom boxes");
+    for (auto& bx : g.boxes) {  // addressed in the problem's scratch vectors (rectangle at offset 15)
+      if (bx.len(0) > 1024) throw std::runtime_error("batch: bottom box wider than 1024");
+      DBox x = b.rect;
+      for (int q = 0; q < 3; q++) { x.lo[q] = bx.lo[q]; x.hi[q] = bx.hi[q]; }
+      x.off = 15 + (bx.lo[0] - b.rect.lo[0]) + (long long)(bx.lo[1] - b.rect.lo[1]) * b.rect.sy +
+              (long long)(bx.lo[2] - b.rect.lo[2]) * b.rect.sz;
+      h_bbox.push_back(x);
+    }
+    h_bottom.push_back(b);
+    bcoef.push_back(L.h_coef[p]);
+    const size_t es = m_prec ? sizeof(float) : sizeof(double);
+    corrp.push_back(L.corr);
+    resp.push_back(L.res);
+    (void)es;
+    // scratch: 10 FP64 vectors of the same ghosted shape, each addressed with the
+    // bottom rectangle's own offsets
+    const long long vs = 15 + (long long)b.rect.sz * (b.rect.hi[2] - b.rect.lo[2] + 3);
+    h_bscratchOff.push_back(soff);
+    soff += 10 * vs;
+  }
+  d_bottom = upload(m_ctx, h_bottom, &m_bytes);
+  d_bbox = upload(m_ctx, h_bbox, &m_bytes);
+  m_bcoef = upload(m_ctx, bcoef, &m_bytes);
+  d_corrPtr = (void**)upload(m_ctx, corrp, &m_bytes);
+  d_resPtr = (void**)upload(m_ctx, resp, &m_bytes);
+  d_bscratchOff = upload(m_ctx, h_bscratchOff, &m_bytes);
+  alloc((void**)&d_bscratch, sizeof(double) * soff);
+  for (int d = 0; d < D; d++) setRelaxShape(d, 0, 0);
+  m_iter.assign(m_np, 0);
+  m_exit.assign(m_np, -1);
+  m_hist.assign(m_np, {});
+  if (!m_ctx->dry) m_ctx->sync();
+}
+
+BatchSolver::~BatchSolver() {
+  if (m_ctx->dry) return;
+  (void)hipStreamSynchronize(m_ctx->stream);
+  auto f = [&](void* p) { if (p) (void)hipFree(p); };
+  for (auto& L : m_depth) {
+    f(L.d_rect); f(L.d_coef); f(L.d_tiles); f(L.d_items); f(L.d_wStart); f(L.corr); f(L.res); f(L.tmp);
+  }
+  f(m_phi); f(m_rhs); f(m_norm); f(d_active); f(d_metric); f(d_bstats); f(d_bottom); f(d_bbox); f(m_bcoef);
+  f(d_corrPtr); f(d_resPtr); f(d_bscratchOff); f(d_bscratch);
+  if (h_norm) (void)hipHostFree(h_norm);
+}
+
+long long BatchSolver::cells(int q) const { return m_prob[m_slot[q]].domain.npts(); }
+
+void BatchSolver::setRelaxShape(int d, int shape, int kc) {
+  Depth& L = m_depth[d];
+  if (L.d_items) { dev_free(m_ctx, L.d_items); L.d_items = nullptr; }
+  if (L.d_wStart) { dev_free(m_ctx, L.d_wStart); L.d_wStart = nullptr; }
+  if (shape < 0 || shape >= ops::kFusedShapeCount) throw std::runtime_error("batch: bad tile shape");
+  const int FX = ops::kFusedShapeTable[shape][0], FY = ops::kFusedShapeTable[shape][1];
+  L.shape = shape;
+  L.kc = kc;
+  L.h_items.clear();
+  L.wStart.assign(1, 0);
+  long long cols = 0;
+  for (int p = 0; p < L.np; p++) {
+    const DBox& R = L.h_rect[p];
+    cols += (long long)((R.hi[0] - R.lo[0] + FX) / FX) * ((R.hi[1] - R.lo[1] + FY) / FY);
+  }
+  int k = kc;
+  if (k <= 0) {  // long enough chunks to amortise the prologue, enough work items to fill the chip
+    k = 32;
+    while (k > 4) {
+      long long n = 0;
+      for (int p = 0; p < L.np; p++) {
+        const DBox& R = L.h_rect[p];
+        n += (long long)((R.hi[0] - R.lo[0] + FX) / FX) * ((R.hi[1] - R.lo[1] + FY) / FY) *
+             ((R.hi[2] - R.lo[2] + k) / k);
+      }
+      if (n >= 2048) break;
+      k >>= 1;
+    }
+  }
+  for (int p = 0; p < L.np; p++) {
+    const DBox& R = L.h_rect[p];
+    ops::FusedArgs a{};
+    for (int q = 0; q < 3; q++) { a.lo[q] = R.lo[q]; a.hi[q] = R.hi[q]; }
+    a.off = R.off;
+    a.sy = R.sy;
+    a.sz = R.sz;
+    a.ntx = (R.hi[0] - R.lo[0] + FX) / FX;
+    a.nty = (R.hi[1] - R.lo[1] + FY) / FY;
+    a.kc = k;
+    const long long nw = (long long)a.ntx * a.nty * ((R.hi[2] - R.lo[2] + k) / k);
+    if (nw + L.wStart.back() > INT32_MAX - 8) throw std::runtime_error("batch: too many work items");
+    a.nwork = (int)nw;
+    for (int dir = 0; dir < 3; dir++)
+      for (int side = 0; side < 2; side++) a.kind[dir][side] = L.h_coef[p].kind[dir][side];
+    a.zsrc = 0;
+    L.h_items.push_back(a);
+    L.wStart.push_back(L.wStart.back() + a.nwork);
+  }
+  (void)cols;
+  L.d_items = upload(m_ctx, L.h_items, &m_bytes);
+  L.d_wStart = upload(m_ctx, L.wStart, &m_bytes);
+}
+
+void BatchSolver::setSolverParameters(int pre, int post, int bottom, int numMG, int iterMax, int imin, double eps,
+                                      double hang, double normThresh) {
+  if (numMG != 1) throw std::runtime_error("batch: V-cycles only (numMG = 1)");
+  m_pre = pre; m_post = post; m_nbottom = bottom; m_numMG = numMG;
+  m_iterMax = iterMax; m_imin = imin; m_eps = eps; m_hang = hang; m_normThresh = normThresh;
+}
+
+void BatchSolver::setRhs(int q, const double* h) {
+  const int p = m_slot.at(q);
+  const Depth& L = m_depth[0];
+  const DBox& R = L.h_rect[p];
+  const BatchProblemSpec& P = m_prob[p];
+  // valid-packed: boxes in layout order, Fortran order within a box
+  const long long n = 15 + (long long)R.sz * (R.hi[2] - R.lo[2] + 3);  // the problem's array
+  std::vector<double> buf((size_t)n, 0.0);
+  const long long base = R.off - 15;
+  long long o = 0;
+  for (auto& b : P.boxes.boxes)
+    for (int k = b.lo[2]; k <= b.hi[2]; k++)
+      for (int j = b.lo[1]; j <= b.hi[1]; j++)
+        for (int i = b.lo[0]; i <= b.hi[0]; i++) {
+          const long long c = R.off + (i - R.lo[0] + 1) + (long long)(j - R.lo[1] + 1) * R.sy +
+                              (long long)(k - R.lo[2] + 1) * R.sz - base;
+          buf[(size_t)c] = h[o++];
+        }
+  hip_check(hipMemcpy(m_rhs + base, buf.data(), sizeof(double) * n, hipMemcpyHostToDevice), "batch rhs upload");
+}
+
+void BatchSolver::getPhi(int q, double* h) const {
+  const int p = m_slot.at(q);
+  const Depth& L = m_depth[0];
+  const DBox& R = L.h_rect[p];
+  const BatchProblemSpec& P = m_prob[p];
+  const long long base = R.off - 15;
+  const long long n = 15 + (long long)R.sz * (R.hi[2] - R.lo[2] + 3);
+  std::vector<double> buf((size_t)n);
+  m_ctx->sync();
+  hip_check(hipMemcpy(buf.data(), m_phi + base, sizeof(double) * n, hipMemcpyDeviceToHost), "batch phi download");
+  long long o = 0;
+  for (auto& b : P.boxes.boxes)
+    for (int k = b.lo[2]; k <= b.hi[2]; k++)
+      for (int j = b.lo[1]; j <= b.hi[1]; j++)
+        for (int i = b.lo[0]; i <= b.hi[0]; i++) {
+          const long long c = R.off + (i - R.lo[0] + 1) + (long long)(j - R.lo[1] + 1) * R.sy +
+                              (long long)(k - R.lo[2] + 1) * R.sz - base;
+          h[o++] = buf[(size_t)c];
+        }
+}
+
+void BatchSolver::uploadActive() {
+  hip_check(hipMemcpyAsync(d_active, h_active.data(), sizeof(int) * m_np, hipMemcpyHostToDevice, m_ctx->stream),
+            "batch active flags");
+}
+
+#define BATCH_LAUNCH(kernel, grid, block, ...)                                                   \
+  do {                                                                                           \
+    if ((grid) > 0) {                                                                            \
+      hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, m_ctx->stream, __VA_ARGS__);        \
+      hip_check(hipGetLastError(), #kernel);                                                     \
+    }                                                                                            \
+  } while (0)
+
+template <class T, bool LAP>
+static void launch_batch_fused(Context* ctx, int shape, int grid, const ops::FusedArgs* items, const int* wstart,
+                               int n, const BatchCoef* coef, const int* active, int zsrc, T* dst, const T* src,
+                               const T* rhs) {
+#define BF_CASE(q, X, Y, NT)                                                                                   \
+  case q:                                                                                                      \
+    hipLaunchKernelGGL((k_batch_fused<T, LAP, X, Y, NT>), dim3(grid), dim3(NT), 0, ctx->stream, items, wstart, n, \
+                       coef, active, zsrc, dst, src, rhs);                                                     \
+    break;
+  switch (shape) {
+    BF_CASE(0, 64, 8, 256)
+    BF_CASE(1, 32, 8, 256)
+    BF_CASE(2, 64, 4, 256)
+    BF_CASE(3, 32, 16, 256)
+    BF_CASE(4, 128, 4, 256)
+    BF_CASE(5, 64, 16, 256)
+    BF_CASE(6, 32, 16, 512)
+    BF_CASE(7, 64, 16, 512)
+    BF_CASE(8, 64, 32, 1024)
+    BF_CASE(9, 128, 8, 1024)
+    default: throw std::runtime_error("batch: bad tile shape");
+  }
+#undef BF_CASE
+  hip_check(hipGetLastError(), "k_batch_fused");
+}
+
+// relaxations of depth d for its first np problems; the first from a zero
+// correction if fromZero.  Iterations alternate corr -> tmp -> corr (no buffer
+// swap: the problems of the depth that are not relaxed keep their data in corr);
+// an odd count ends with a copy back.
+template <class T>
+__global__ void __launch_bounds__(256)
+    k_batch_copy(const Tile* __restrict__ tiles, int lx, int ly, const DBox* __restrict__ rect,
+                 const int* __restrict__ active, T* __restrict__ dst, const T* __restrict__ src) {
+  const Tile t = tiles[blockIdx.x];
+  if (!active[t.box]) return;
+  const DBox R = rect[t.box];
+  int i, j, k;
+  if (!btile_cell(t, R, lx, ly, i, j, k)) return;
+  const long long c = ridx(R, i, j, k);
+  dst[c] = src[c];
+}
+template <class T>
+void BatchSolver::relaxDepth(int d, int np, int iters, bool fromZero) {
+  Depth& L = m_depth[d];
+  if (np <= 0 || iters <= 0) return;
+  const int nw = L.wStart[np];
+  const int grid = (nw + 7) / 8 * 8;
+  bool anyLap = false, anyHelm = false;
+  for (int p = 0; p < np; p++) (L.h_coef[p].lap ? anyLap : anyHelm) = true;
+  for (int it = 0; it < iters; it++) {
+    const int zsrc = (fromZero && it == 0) ? 1 : 0;
+    T* src = (T*)((it & 1) ? L.tmp : L.corr);
+    T* dst = (T*)((it & 1) ? L.corr : L.tmp);
+    if (anyLap)
+      launch_batch_fused<T, true>(m_ctx, L.shape, grid, L.d_items, L.d_wStart, np, L.d_coef, d_active, zsrc, dst, src,
+                                  (const T*)L.res);
+    if (anyHelm)
+      launch_batch_fused<T, false>(m_ctx, L.shape, grid, L.d_items, L.d_wStart, np, L.d_coef, d_active, zsrc, dst,
+                                   src, (const T*)L.res);
+    for (int p = 0; p < np; p++) m_cellRelax += h_active[p] ? (double)(L.cellStart[p + 1] - L.cellStart[p]) : 0.0;
+  }
+  if (iters & 1)
+    BATCH_LAUNCH(k_batch_copy<T>, L.tileStart[np], 256, L.d_tiles, 6, 2, L.d_rect, d_active, (T*)L.corr,
+                 (const T*)L.tmp);
+}
+
+// MultiGrid::cycle in lockstep over the problems: descent (relax, restrict) at
+// each depth for the problems that go deeper, every problem's bottom in one
+// launch, then ascent (prolong, relax)
+template <class T>
+void BatchSolver::vcycle() {
+  const int D = (int)m_depth.size();
+  for (int d = 0; d + 1 < D; d++) {
+    const int np = m_depth[d + 1].np;  // problems with a depth below d
+    relaxDepth<T>(d, np, m_pre, true);
+    Depth& F = m_depth[d];
+    Depth& C = m_depth[d + 1];
+    BATCH_LAUNCH(k_batch_restrict<T>, C.tileStart[np], 256, C.d_tiles, 6, 2, C.d_rect, F.d_rect, F.d_coef, d_active,
+                 (T*)C.res, (const T*)F.corr, (const T*)F.res);
+  }
+  BottomArgs ba{80, 1e-12, 1e-8, 1e-30, 5, m_nbottom};
+  BATCH_LAUNCH(k_batch_bottom<T>, m_np, 256, d_bottom, m_bcoef, d_bbox, (T* const*)d_corrPtr,
+               (const T* const*)d_resPtr, d_bscratch, d_bscratchOff, d_metric, d_active, d_bstats, ba);
+  for (int d = D - 2; d >= 0; d--) {
+    const int np = m_depth[d + 1].np;
+    Depth& F = m_depth[d];
+    Depth& C = m_depth[d + 1];
+    BATCH_LAUNCH(k_batch_prolong<T>, F.tileStart[np], 256, F.d_tiles, 6, 2, F.d_rect, C.d_rect, d_active, (T*)F.corr,
+                 (const T*)C.corr);
+    relaxDepth<T>(d, np, m_post, false);
+  }
+}
+
+template <class T>
+void BatchSolver::residualNorm(bool homog) {
+  (void)homog;
+  Depth& L = m_depth[0];
+  BATCH_LAUNCH(k_batch_zero_norms, (m_np + 255) / 256, 256, m_norm, d_active, m_np);
+  BATCH_LAUNCH(k_batch_residual_norm<T>, L.tileStart[m_np], 256, L.d_tiles, 6, 2, L.d_rect, L.d_coef, d_active,
+               (T*)L.res, (const double*)m_phi, (const double*)m_rhs, m_norm);
+  hip_check(hipMemcpyAsync(h_norm, m_norm, sizeof(double) * m_np, hipMemcpyDeviceToHost, m_ctx->stream),
+            "batch norms");
+  m_ctx->sync();
+}
+
+template <class T>
+void BatchSolver::update() {
+  Depth& L = m_depth[0];
+  BATCH_LAUNCH(k_batch_update<T>, L.tileStart[m_np], 256, L.d_tiles, 6, 2, L.d_rect, d_active, m_phi,
+               (const T*)L.corr);
+}
+
+// solveNoInitResid (AMRMultiGrid.H:935-1124) for every problem at once: each
+// problem keeps its own history and convergence tests and leaves the lockstep
+// when they stop it
+template <class T>
+void BatchSolver::runSolve() {
+  std::fill(h_active.begin(), h_active.end(), 1);
+  uploadActive();
+  hip_check(hipMemsetAsync(m_phi, 0, sizeof(double) * m_depth[0].slab, m_ctx->stream), "batch phi zero");
+  hip_check(hipMemsetAsync(d_bstats, 0, sizeof(int) * m_np, m_ctx->stream), "batch stats");
+  m_cellRelax = 0;
+  residualNorm<T>(false);
+  std::vector<double> r0(m_np), rnorm(m_np), normLast(m_np);
+  std::vector<double> metric(2 * m_np);
+  for (int p = 0; p < m_np; p++) {
+    r0[p] = rnorm[p] = h_norm[p];
+    normLast[p] = 2 * r0[p];
+    m_iter[p] = 0;
+    m_hist[p].assign(1, r0[p]);
+    metric[2 * p] = r0[p];
+    metric[2 * p + 1] = 1.0 * m_eps;  // m_bottomSolverEpsCushion * m_eps
+  }
+  hip_check(hipMemcpyAsync(d_metric, metric.data(), sizeof(double) * 2 * m_np, hipMemcpyHostToDevice, m_ctx->stream),
+            "batch metrics");
+  auto keepGoing = [&](int p, bool& goNorm, bool& goRedu, bool& goIter, bool& goHang) {
+    goNorm = rnorm[p] > m_normThresh;
+    goRedu = rnorm[p] > m_eps * r0[p];
+    goIter = m_iter[p] < m_iterMax;
+    goHang = m_iter[p] < m_imin || rnorm[p] < (1 - m_hang) * normLast[p];
+    const bool goMin = m_iter[p] < m_iterMin;
+    return goMin || (goIter && goRedu && goHang && goNorm);
+  };
+  int nactive = 0;
+  for (int p = 0; p < m_np; p++) {
+    bool a, b, c, d;
+    h_active[p] = keepGoing(p, a, b, c, d) ? 1 : 0;
+    if (!h_active[p]) m_exit[p] = int(!b) + int(!c) * 2 + int(!d) * 4 + int(!a) * 8;
+    nactive += h_active[p];
+  }
+  while (nactive > 0) {
+    uploadActive();
+    for (int p = 0; p < m_np; p++)
+      if (h_active[p]) normLast[p] = rnorm[p];
+    vcycle<T>();
+    update<T>();
+    residualNorm<T>(false);
+    nactive = 0;
+    for (int p = 0; p < m_np; p++) {
+      if (!h_active[p]) continue;
+      rnorm[p] = h_norm[p];
+      m_iter[p]++;
+      m_hist[p].push_back(rnorm[p]);
+      bool goNorm, goRedu, goIter, goHang;
+      if (keepGoing(p, goNorm, goRedu, goIter, goHang)) {
+        nactive++;
+      } else {
+        h_active[p] = 0;
+        m_exit[p] = int(!goRedu) + int(!goIter) * 2 + int(!goHang) * 4 + int(!goNorm) * 8;
+      }
+    }
+  }
+  std::vector<int> st(m_np);
+  hip_check(hipMemcpy(st.data(), d_bstats, sizeof(int) * m_np, hipMemcpyDeviceToHost), "batch stats");
+  m_bottomIters = 0;
+  for (int v : st) m_bottomIters += v;
+}
+
+void BatchSolver::solve() {
+  if (m_prec) runSolve<float>();
+  else runSolve<double>();
+}
+
+std::vector<double> BatchSolver::tuneRelax() {
+  std::vector<double> best(m_depth.size(), 0.0);
+  hipEvent_t e0, e1;
+  hip_check(hipEventCreate(&e0), "event");
+  hip_check(hipEventCreate(&e1), "event");
+  std::fill(h_active.begin(), h_active.end(), 1);
+  uploadActive();
+  const int kcs[4] = {0, 8, 16, 32};
+  for (size_t d = 0; d < m_depth.size(); d++) {
+    const int np = m_depth[d].np;
+    double bt = 1e300;
+    int bs = 0, bk = 0;
+    for (int sh = 0; sh < ops::kFusedShapeCount; sh++)
+      for (int q = 0; q < 4; q++) {
+        setRelaxShape((int)d, sh, kcs[q]);
+        auto run = [&]() {
+          if (m_prec) relaxDepth<float>((int)d, np, 1, false);
+          else relaxDepth<double>((int)d, np, 1, false);
+        };
+        run();
+        hip_check(hipEventRecord(e0, m_ctx->stream), "record");
+        for (int r = 0; r < 3; r++) run();
+        hip_check(hipEventRecord(e1, m_ctx->stream), "record");
+        hip_check(hipEventSynchronize(e1), "sync");
+        float ms = 0;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+        if (ms / 3 < bt) { bt = ms / 3; bs = sh; bk = kcs[q]; }
+      }
+    setRelaxShape((int)d, bs, bk);
+    best[d] = bt;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  m_cellRelax = 0;
+  return best;
+}
+
+}  // namespace amrmg

@@ -9,6 +9,7 @@
 
 #include "amrpoissonop.hpp"
 #include "mixed.hpp"
+#include "batch.hpp"
 
 using namespace amrmg;
 
@@ -31,6 +32,10 @@ extern "C" const char* amrmg_last_error(void) { return g_err.c_str(); }
 
 struct amrmg_ctx {
   std::unique_ptr<Context> c;
+};
+struct amrmg_batch {
+  amrmg_ctx* ctx = nullptr;
+  std::unique_ptr<BatchSolver> solver;
 };
 
 struct amrmg_solver {
@@ -469,6 +474,85 @@ int amrmg_solver_set_relax_shape(amrmg_solver* s, int shape, int kc) {
   }
   API_END
 }
+// ---- batched single-level solves ----
+int amrmg_batch_create(amrmg_ctx* ctx, int nprob, const int* ispecs, const double* rspecs, int precision,
+                       amrmg_batch** out) {
+  API_BEGIN
+  if (nprob <= 0 || nprob > (1 << 20)) throw std::runtime_error("batch: bad problem count");
+  std::vector<BatchProblemSpec> probs;
+  const int* is = ispecs;
+  for (int q = 0; q < nprob; q++) {
+    if (is[0] != 1) throw std::runtime_error("batch: every problem must have one level");
+    BatchProblemSpec P;
+    double a, b, dx0;
+    std::vector<DBL> g = parse_spec(is, rspecs + 9 * q, P.domain, P.bc, dx0, a, b);
+    P.boxes = g[0];
+    P.dx = dx0;
+    P.alpha = a;
+    P.beta = b;
+    probs.push_back(P);
+    is += 14 + 6 * is[13];
+  }
+  std::unique_ptr<amrmg_batch> bt(new amrmg_batch());
+  bt->ctx = ctx;
+  bt->solver.reset(new BatchSolver(ctx->c.get(), std::move(probs), precision));
+  *out = bt.release();
+  API_END
+}
+int amrmg_batch_destroy(amrmg_batch* b) {
+  API_BEGIN
+  delete b;
+  API_END
+}
+int amrmg_batch_set_params(amrmg_batch* b, const int* sp, const double* rp) {
+  API_BEGIN
+  b->solver->setSolverParameters(sp[0], sp[1], sp[2], sp[3], sp[4], sp[5], rp[0], rp[1], rp[2]);
+  API_END
+}
+int amrmg_batch_set_rhs(amrmg_batch* b, int q, const double* valid_packed) {
+  API_BEGIN
+  if (q < 0 || q >= b->solver->numProblems()) throw std::runtime_error("batch: bad problem index");
+  b->solver->setRhs(q, valid_packed);
+  API_END
+}
+int amrmg_batch_get_phi(amrmg_batch* b, int q, double* valid_packed) {
+  API_BEGIN
+  if (q < 0 || q >= b->solver->numProblems()) throw std::runtime_error("batch: bad problem index");
+  b->solver->getPhi(q, valid_packed);
+  API_END
+}
+int amrmg_batch_solve(amrmg_batch* b, int* iterations, int* exit_status) {
+  API_BEGIN
+  b->solver->solve();
+  for (int q = 0; q < b->solver->numProblems(); q++) {
+    if (iterations) iterations[q] = b->solver->iterations(q);
+    if (exit_status) exit_status[q] = b->solver->exitStatus(q);
+  }
+  API_END
+}
+int amrmg_batch_history(amrmg_batch* b, int q, double* hist, int cap, int* n) {
+  API_BEGIN
+  if (q < 0 || q >= b->solver->numProblems()) throw std::runtime_error("batch: bad problem index");
+  const std::vector<double>& h = b->solver->history(q);
+  for (int i = 0; i < cap && i < (int)h.size(); i++) hist[i] = h[i];
+  if (n) *n = (int)h.size();
+  API_END
+}
+int amrmg_batch_tune_relax(amrmg_batch* b, double* ms, int cap, int* ndepth) {
+  API_BEGIN
+  std::vector<double> t = b->solver->tuneRelax();
+  for (int i = 0; i < cap && i < (int)t.size(); i++) ms[i] = t[i];
+  if (ndepth) *ndepth = (int)t.size();
+  API_END
+}
+int amrmg_batch_stats(amrmg_batch* b, double* cell_relax, long long* bottom_iters, long long* device_bytes) {
+  API_BEGIN
+  if (cell_relax) *cell_relax = b->solver->cellRelaxations();
+  if (bottom_iters) *bottom_iters = b->solver->bottomIterations();
+  if (device_bytes) *device_bytes = b->solver->deviceBytes();
+  API_END
+}
+
 int amrmg_copy_probe(amrmg_ctx* ctx, long long nbytes, int iters, double* ms_out) {
   API_BEGIN
   if (nbytes <= 0 || iters <= 0) throw std::runtime_error("copy probe: nbytes and iters must be positive");

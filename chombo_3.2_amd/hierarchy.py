@@ -259,3 +259,25 @@ def solver_params(**kw):
     ip = np.asarray([p["pre"], p["post"], p["bottom"], p["numMG"], p["iterMax"], p["imin"]], np.int32)
     rp = np.asarray([p["eps"], p["hang"], p["normThresh"]], np.float64)
     return ip, rp
+
+
+# ---- batched single-level problems ----
+def batch_problems(count: int = 512, seed: int = 1004, n_min: int = 16, n_max: int = 128):
+    """Independent single-level Helmholtz problems: edge n = 8*round(exp(U(ln n_min,
+    ln n_max))/8), domainSplit with 64^3 maximum boxes (blocking factor 8),
+    alpha ~ U(0,1), beta ~ U(0.5,2), homogeneous Dirichlet, rhs =
+    sin(2 pi k1 x) sin(2 pi k2 y) sin(2 pi k3 z) + N(0, 1e-3) with k_d ~ U{1,2,3}.
+    Returns a list of (Hierarchy, rhs)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for q in range(count):
+        n = int(8 * round(math.exp(rng.uniform(math.log(n_min), math.log(n_max))) / 8))
+        n = max(8, min(n_max, n))
+        alpha = float(rng.uniform(0.0, 1.0))
+        beta = float(rng.uniform(0.5, 2.0))
+        k = tuple(int(v) for v in rng.integers(1, 4, size=3))
+        dom = box((0, 0, 0), (n - 1, n - 1, n - 1))
+        h = Hierarchy(dom, [domain_split(dom, 64, 8)], 1.0 / n, alpha, beta)
+        rhs = sinusoid_rhs(h, seed * 100003 + q, k=k)
+        out.append((h, rhs))
+    return out

@@ -116,6 +116,31 @@ int amrmg_solver_tune_relax(amrmg_solver* s, double* table, int cap, int* nrec);
    relaxation on every level. */
 int amrmg_solver_set_relax_shape(amrmg_solver* s, int shape, int kc);
 
+/* ---- batched single-level solves: many independent problems, each V-cycle stage
+   of all of them in one launch (relaxation, restriction, prolongation, residual
+   with per-problem max-norm, and every problem's bottom BiCGStab as one workgroup
+   each).  Per problem the arithmetic is that of amrmg_solver_solve on it alone
+   (AMRMultiGrid::solve with one level).  No counterpart in the reference, which
+   solves one hierarchy at a time. ---- */
+typedef struct amrmg_batch amrmg_batch;
+/* nprob problems, each a one-level (ispec, rspec) as for amrmg_solver_create,
+   ispecs back to back (14 + 6*nbox ints each), rspecs 9 doubles each; each
+   problem's boxes must tile its domain.  precision: 0 FP64, 1 mixed. */
+int amrmg_batch_create(amrmg_ctx* ctx, int nprob, const int* ispecs, const double* rspecs, int precision,
+                       amrmg_batch** out);
+int amrmg_batch_destroy(amrmg_batch* b);
+/* sparams / rparams as for amrmg_solver_set_params (numMG must be 1) */
+int amrmg_batch_set_params(amrmg_batch* b, const int* sparams, const double* rparams);
+int amrmg_batch_set_rhs(amrmg_batch* b, int problem, const double* valid_packed);
+int amrmg_batch_get_phi(amrmg_batch* b, int problem, double* valid_packed);
+/* solves every problem from phi = 0; per-problem iteration counts and exit status */
+int amrmg_batch_solve(amrmg_batch* b, int* iterations, int* exit_status);
+int amrmg_batch_history(amrmg_batch* b, int problem, double* hist, int cap, int* n);
+/* autotunes the relaxation tile shape per multigrid depth; ms per depth */
+int amrmg_batch_tune_relax(amrmg_batch* b, double* ms, int cap, int* ndepth);
+/* cell-relaxations and bottom BiCGStab iterations of the last solve; HBM bytes held */
+int amrmg_batch_stats(amrmg_batch* b, double* cell_relax, long long* bottom_iters, long long* device_bytes);
+
 /* Plain device copy of nbytes (reads nbytes, writes nbytes), iters launches;
    average ms per launch.  Calibrates HBM byte counters and the copy bandwidth. */
 int amrmg_copy_probe(amrmg_ctx* ctx, long long nbytes, int iters, double* ms_out);

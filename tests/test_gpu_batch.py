@@ -1,0 +1,108 @@
+"""Batched single-level solves (amrmg_batch_*) against solving each problem alone.
+
+In FP64 every stage of the batched V-cycle evaluates, per problem, the same
+expressions on the same operands as the single-problem path (the relaxation, the
+residual-restriction and prolongation with their boundary ghosts, the bottom
+BiCGStab with the reference's reduction orders), so each problem's history and
+solution must be bitwise those of AMRMultiGrid on it alone, which the parity
+tests hold bitwise equal to the oracle.  The mixed-precision batch must land
+within 1e-9 relative of the FP64 solution (ten times the solve tolerance).
+"""
+import numpy as np
+import pytest
+
+import amrmg
+import hierarchy as H
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(n, alpha, beta, bc=(1, 1, 1), seed=0, k=(1, 2, 1)):
+    dom = H.box((0, 0, 0), (n - 1, n - 1, n - 1))
+    h = H.Hierarchy(dom, [H.domain_split(dom, 64, 8)], 1.0 / n, alpha, beta, bc_type_lo=bc, bc_type_hi=(1, 1, 1))
+    return h, H.sinusoid_rhs(h, 2000 + seed, k=k)
+
+
+def _edge_set():
+    """Power-of-two single boxes, non-power-of-two boxes (deeper bottoms), a
+    64 + 8 split (bottom of 8 boxes), Laplacian and Helmholtz, Neumann faces."""
+    specs = [(16, 0.0, 1.0), (24, 0.3, 1.2), (32, 0.0, 1.0), (40, 0.7, 0.6), (72, 0.5, 1.5), (64, 0.0, 1.0),
+             (48, 0.9, 1.9), (8, 0.2, 0.7)]
+    out = [_problem(n, a, b, seed=q) for q, (n, a, b) in enumerate(specs)]
+    out.append(_problem(32, 0.6, 1.1, bc=(0, 1, 0), seed=20, k=(1, 1, 2)))
+    return out
+
+
+def _single(gpu_ctx, h, rhs, sp, rp):
+    g = amrmg.AMRMultiGrid(gpu_ctx, h)
+    g.setSolverParameters(sp, rp)
+    g.set_field(g.RHS, rhs)
+    hist, info = g.solve()
+    phi = g.get_field(g.PHI)
+    g.close()
+    return hist, info, phi
+
+
+def _batch(gpu_ctx, probs, sp, rp, precision=0, tune=False):
+    b = amrmg.BatchSolver(gpu_ctx, [h for h, _ in probs], precision)
+    b.setSolverParameters(sp, rp)
+    for q, (_, rhs) in enumerate(probs):
+        b.set_rhs(q, rhs)
+    if tune:
+        b.tune_relax()
+    it, ex = b.solve()
+    out = [(b.history(q), int(it[q]), int(ex[q]), b.get_phi(q)) for q in range(len(probs))]
+    b.close()
+    return out
+
+
+def test_batch_fp64_bitwise_equals_single_solves(gpu_ctx):
+    probs = _edge_set()
+    sp, rp = H.solver_params()
+    res = _batch(gpu_ctx, probs, sp, rp)
+    for q, ((h, rhs), (hist, it, ex, phi)) in enumerate(zip(probs, res)):
+        hs, info, phis = _single(gpu_ctx, h, rhs, sp, rp)
+        assert it == info["iterations"] and ex == info["exit_status"], q
+        np.testing.assert_array_equal(hist, hs)
+        np.testing.assert_array_equal(phi, phis)
+    # and against the oracle directly on two of them
+    for q in (1, 4):
+        h, rhs = probs[q]
+        phi_o, hist_o, info_o = Oracle(h).solve(rhs, sp, rp)
+        np.testing.assert_array_equal(res[q][3], phi_o)
+
+
+def test_batch_tuned_shapes_bitwise(gpu_ctx):
+    probs = _edge_set()[:6]
+    sp, rp = H.solver_params()
+    a = _batch(gpu_ctx, probs, sp, rp)
+    b = _batch(gpu_ctx, probs, sp, rp, tune=True)
+    for (h1, i1, e1, p1), (h2, i2, e2, p2) in zip(a, b):
+        assert (i1, e1) == (i2, e2)
+        np.testing.assert_array_equal(h1, h2)
+        np.testing.assert_array_equal(p1, p2)
+
+
+def test_batch_mixed_precision_within_tolerance(gpu_ctx):
+    probs = _edge_set()
+    sp, rp = H.solver_params()
+    ref = _batch(gpu_ctx, probs, sp, rp)
+    mp = _batch(gpu_ctx, probs, sp, rp, precision=1)
+    for q, ((h1, i1, e1, p1), (h2, i2, e2, p2)) in enumerate(zip(ref, mp)):
+        assert e2 & 1 and h2[-1] <= 1e-10 * h2[0], q
+        assert abs(i1 - i2) <= 2, q
+        err = np.abs(p2 - p1).max() / np.abs(p1).max()
+        assert err < 1e-9, (q, err)
+
+
+def test_batch_c4_sample_bitwise(gpu_ctx):
+    """48 problems of the 512-problem workload's distribution (seed 1004)."""
+    probs = H.batch_problems(48, seed=1004)
+    sp, rp = H.solver_params()
+    res = _batch(gpu_ctx, probs, sp, rp)
+    for q in range(0, 48, 3):
+        h, rhs = probs[q]
+        hs, info, phis = _single(gpu_ctx, h, rhs, sp, rp)
+        assert res[q][1] == info["iterations"], q
+        np.testing.assert_array_equal(res[q][3], phis)
