@@ -169,6 +169,175 @@ def mixed_leg(args, ctx, hier, rhs, sp, rp, solver64, fp64_s_per_step):
     return out
 
 
+def batch_leg(args, ctx):
+    """The batched workload: independent single-level Helmholtz problems of edge
+    16..128 (log-uniform), solved (a) one after another with the plain FP64 solver,
+    each solve timed alone with its inputs already in HBM, (b) together with the
+    batched FP64 solver and (c) together with the batched mixed-precision solver,
+    one launch per V-cycle stage for all problems."""
+    import numpy as np
+    import amrmg
+    import hierarchy as H
+    t0 = time.time()
+    probs = H.batch_problems(args.batch, seed=1004)
+    cells = sum(h.total_cells() for h, _ in probs)
+    log(f"batch: {len(probs)} problems, {cells / 1e6:.1f} M cells (generated in {time.time() - t0:.1f}s)")
+    sp, rp = H.solver_params(eps=1e-10, hang=1e-15, iterMax=100)
+    # (a) plain FP64 path: one solver per edge size, coefficients and rhs set per problem (untimed)
+    solvers = {}
+    t_plain, it_plain, phi_plain = 0.0, [], {}
+    for q, (h, rhs) in enumerate(probs):
+        n = h.domain[1][0] + 1
+        if n not in solvers:
+            s = amrmg.AMRMultiGrid(ctx, h)
+            s.setSolverParameters(sp, rp)
+            s.set_field(s.RHS, rhs)
+            s.solve()  # warm-up of this size
+            solvers[n] = s
+        s = solvers[n]
+        s.set_alpha_beta(h.alpha, h.beta)
+        s.set_field(s.RHS, rhs)
+        t1 = time.perf_counter()
+        _, info = s.solve()
+        t_plain += time.perf_counter() - t1
+        it_plain.append(info["iterations"])
+        if q % 16 == 0:
+            phi_plain[q] = s.get_field(s.PHI)
+    for s in solvers.values():
+        s.close()
+    log(f"plain FP64 one-by-one: {t_plain:.3f}s for {len(probs)} solves")
+
+    def batched(prec):
+        b = amrmg.BatchSolver(ctx, [h for h, _ in probs], prec)
+        b.setSolverParameters(sp, rp)
+        for q, (_, rhs) in enumerate(probs):
+            b.set_rhs(q, rhs)
+        if not args.no_tune:
+            b.tune_relax()
+        for _ in range(args.warmup):
+            b.solve()
+        ctx.profile(["gsrb", "residual"])
+        ctx.profile_reset()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            it, ex = b.solve()
+        wall = (time.perf_counter() - t1) / args.steps
+        rel = ctx.profile_read("gsrb")
+        res = ctx.profile_read("residual")
+        ctx.profile([])
+        phis = {q: b.get_phi(q) for q in phi_plain}
+        hists = [b.history(q) for q in range(len(probs))]
+        st = b.stats()
+        b.close()
+        return wall, it, ex, rel, res, phis, hists, st
+
+    w64, it64, ex64, rel64, _, phi64, hist64, st64 = batched(0)
+    bitwise = all(np.array_equal(phi64[q], phi_plain[q]) for q in phi_plain)
+    log(f"batched FP64: {w64 * 1e3:.1f} ms per batch; bitwise equal to plain on the {len(phi_plain)} sampled: {bitwise}")
+    wmp, itmp, exmp, relmp, resmp, phimp, histmp, stmp = batched(1)
+    err = max(float(np.abs(phimp[q] - phi64[q]).max() / np.abs(phi64[q]).max()) for q in phi64)
+    conv = all(h[-1] <= 1e-10 * h[0] for h in histmp)
+    out = {
+        "workload": f"{len(probs)} single-level Helmholtz problems, edge 16..128 log-uniform (seed 1004), "
+                    "64^3 max boxes, alpha~U(0,1), beta~U(0.5,2), Dirichlet 0, sinusoid+N(0,1e-3) rhs, tol 1e-10",
+        "problems": len(probs),
+        "cells": cells,
+        "plain_fp64_solves_per_s": round(len(probs) / t_plain, 2),
+        "batched_fp64_solves_per_s": round(len(probs) / w64, 2),
+        "batched_mixed_solves_per_s": round(len(probs) / wmp, 2),
+        "speedup_batched_mixed_vs_plain_fp64": round(t_plain / wmp, 2),
+        "speedup_batched_fp64_vs_plain_fp64": round(t_plain / w64, 2),
+        "ms_per_batch_fp64": round(w64 * 1e3, 3),
+        "ms_per_batch_mixed": round(wmp * 1e3, 3),
+        "batched_fp64_bitwise_equal_plain_sampled": bitwise,
+        "mixed_converged_all": conv,
+        "max_rel_err_mixed_vs_fp64_sampled": err,
+        "iterations_mean_plain_fp64": round(float(np.mean(it_plain)), 3),
+        "iterations_mean_mixed": round(float(np.mean(itmp)), 3),
+        "relax_mixed_achieved_GBs": round(relmp["bytes"] / (relmp["ms"] * 1e-3) / 1e9, 1) if relmp["ms"] else None,
+        "relax_mixed_frac": round(relmp["bytes"] / (relmp["ms"] * 1e-3) / 1e9 / PEAK_HBM_GBS, 4) if relmp["ms"] else None,
+        "relax_fp64_achieved_GBs": round(rel64["bytes"] / (rel64["ms"] * 1e-3) / 1e9, 1) if rel64["ms"] else None,
+        "residual_mixed_achieved_GBs": round(resmp["bytes"] / (resmp["ms"] * 1e-3) / 1e9, 1) if resmp["ms"] else None,
+        "mcell_relax_per_s_mixed": round(stmp["cell_relax"] / wmp / 1e6, 1),
+    }
+    log(f"batched: {out}")
+    return out
+
+
+def serve_leg(args):
+    """Serving: 1-2-level AMR Poisson requests (bases 32..256^3) through the
+    continuous-admission request loop.  Capacity is measured closed-loop (all
+    requests submitted at once); then the same requests arrive as a Poisson
+    process at 0.8x that capacity (seeded) and latency is arrival to result."""
+    import threading
+    import numpy as np
+    import amrmg
+    import hierarchy as H
+    t0 = time.time()
+    reqs = H.serve_requests(args.serve, seed=1005)
+    log(f"serve: {len(reqs)} requests, mean {np.mean([h.total_cells() for h, _ in reqs]) / 1e6:.2f} M cells "
+        f"(generated in {time.time() - t0:.1f}s)")
+    sp, rp = H.solver_params(eps=1e-10, hang=1e-15, iterMax=100)
+    budget = int(args.serve_budget_gb * (1 << 30))
+
+    def run(arrivals):
+        eng = amrmg.ServeEngine(0, slots=args.serve_slots, budget_bytes=budget, sparams=sp, rparams=rp)
+        # warm the resident hierarchies (one request per distinct hierarchy)
+        seen = {}
+        for h, rhs in reqs:
+            if id(h) not in seen:
+                seen[id(h)] = eng.submit(h, rhs)
+        for i in seen.values():
+            eng.wait(i, want_phi=False)
+        ids = [None] * len(reqs)
+
+        def producer():
+            start = time.perf_counter()
+            for q, (h, rhs) in enumerate(reqs):
+                if arrivals is not None:
+                    delay = arrivals[q] - (time.perf_counter() - start)
+                    if delay > 0:
+                        time.sleep(delay)
+                ids[q] = eng.submit(h, rhs)
+        th = threading.Thread(target=producer)
+        t1 = time.perf_counter()
+        th.start()
+        th.join()
+        res = [eng.wait(i, want_phi=False) for i in ids]
+        wall = time.perf_counter() - t1
+        st = eng.stats()
+        eng.close()
+        return res, wall, st
+
+    res, wall, st = run(None)
+    cap = len(reqs) / wall
+    log(f"serve closed loop: {cap:.2f} solves/s, stats {st}")
+    rate = 0.8 * cap
+    rng = np.random.default_rng(1005)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(reqs)))
+    res2, wall2, st2 = run(arrivals)
+    lat = np.array([r["latency"] for r in res2]) * 1e3
+    out = {
+        "workload": f"{len(reqs)} requests: 1-2-level AMR Poisson, base U{{32,64,128,256}}^3, refined fraction "
+                    "U(0.05,0.25), Gaussian-bump rhs, tol 1e-10, Poisson arrivals at 0.8x closed-loop capacity "
+                    "(seed 1005)",
+        "slots": args.serve_slots,
+        "hbm_budget_gb": args.serve_budget_gb,
+        "closed_loop_solves_per_s": round(cap, 3),
+        "arrival_rate_per_s": round(rate, 3),
+        "open_loop_solves_per_s": round(len(reqs) / wall2, 3),
+        "latency_ms_p50": round(float(np.percentile(lat, 50)), 2),
+        "latency_ms_p99": round(float(np.percentile(lat, 99)), 2),
+        "latency_ms_max": round(float(lat.max()), 2),
+        "converged_all": all(r["exit_status"] & 1 for r in res2),
+        "max_running": st2["max_running"],
+        "hierarchies_created": st2["created"],
+        "max_hbm_gb": round(st2["max_bytes"] / (1 << 30), 2),
+    }
+    log(f"serve: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -180,6 +349,10 @@ def main():
                     help="cap on V-cycles per solve (a short cap is for profiling runs only)")
     ap.add_argument("--no-tune", action="store_true", help="skip the relaxation autotuner (default shapes)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--batch", type=int, default=512, help="problems in the batched leg (0 = skip)")
+    ap.add_argument("--serve", type=int, default=600, help="requests in the serving leg (0 = skip)")
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-budget-gb", type=float, default=96.0)
     ap.add_argument("--copy-mb", type=int, default=2048,
                     help="size of the plain copy kernel run after the timed region (0 = skip)")
     ap.add_argument("--summarize-pmc", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "FETCH_RUN_JSON"),
@@ -249,6 +422,14 @@ def main():
     mixed = None
     if not args.no_mixed:
         mixed = mixed_leg(args, ctx, hier, rhs, sp, rp, solver, wall / args.steps)
+    batch = None
+    if args.batch > 0:
+        solver.close()
+        batch = batch_leg(args, ctx)
+    serve = None
+    if args.serve > 0:
+        solver.close()
+        serve = serve_leg(args)
     copy_gbs = None
     if args.copy_mb > 0:
         nbytes = args.copy_mb << 20
@@ -303,6 +484,8 @@ def main():
         "relax_tiles": ({f"{r['kind']}{r['index']}": f"{r['shape'][0]}x{r['shape'][1]}/kc{r['kc']}" for r in tune}
                         if tune else None),
         "mixed_precision": mixed,
+        "batched": batch,
+        "serving": serve,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

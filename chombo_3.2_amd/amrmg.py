@@ -41,6 +41,13 @@ def lib():
         L.amrmg_ctx_destroy.argtypes = [C.c_void_p]
         L.amrmg_ctx_set_fused_relax.argtypes = [C.c_void_p, C.c_int]
         L.amrmg_profile_read_units.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.amrmg_solver_set_alpha_beta.argtypes = [C.c_void_p, C.c_double, C.c_double]
+        L.amrmg_serve_create.argtypes = [C.c_int, C.c_int, C.c_longlong, C.c_int, C.POINTER(C.c_void_p)]
+        L.amrmg_serve_set_params.argtypes = [C.c_void_p, _ip, _dp]
+        L.amrmg_serve_submit.argtypes = [C.c_void_p, _ip, _dp, _dp, C.POINTER(C.c_longlong)]
+        L.amrmg_serve_wait.argtypes = [C.c_void_p, C.c_longlong, _dp, _dp, _ip]
+        L.amrmg_serve_stats.argtypes = [C.c_void_p, C.POINTER(C.c_longlong), C.c_int]
+        L.amrmg_serve_destroy.argtypes = [C.c_void_p]
         L.amrmg_batch_create.argtypes = [C.c_void_p, C.c_int, _ip, _dp, C.c_int, C.POINTER(C.c_void_p)]
         L.amrmg_batch_destroy.argtypes = [C.c_void_p]
         L.amrmg_batch_set_params.argtypes = [C.c_void_p, _ip, _dp]
@@ -219,6 +226,10 @@ class AMRMultiGrid:
         """Forces tile shape index (into RELAX_SHAPES) and z-chunk (0 = automatic)."""
         _chk(lib().amrmg_solver_set_relax_shape(self.p, int(shape), int(kc)))
 
+    def set_alpha_beta(self, alpha, beta):
+        """AMRPoissonOp::setAlphaAndBeta on every operator of the solver."""
+        _chk(lib().amrmg_solver_set_alpha_beta(self.p, float(alpha), float(beta)))
+
     def setSolverParameters(self, sparams, rparams):
         sp = np.ascontiguousarray(sparams, dtype=np.int32)
         rp = np.ascontiguousarray(rparams, dtype=np.float64)
@@ -390,3 +401,58 @@ class BatchSolver:
         bi, db = C.c_longlong(), C.c_longlong()
         _chk(lib().amrmg_batch_stats(self.p, C.byref(cr), C.byref(bi), C.byref(db)))
         return dict(cell_relax=cr.value, bottom_iters=bi.value, device_bytes=db.value)
+
+
+class ServeEngine:
+    """Request loop with continuous admission (amrmg_serve_*): `slots` requests run
+    at once, each on its own HIP stream, admitted FIFO as slots and the HBM budget
+    allow; hierarchies stay device-resident for later requests with the same spec."""
+
+    def __init__(self, device: int = 0, slots: int = 4, budget_bytes: int = 64 << 30, precision: int = 0,
+                 sparams=None, rparams=None):
+        p = C.c_void_p()
+        _chk(lib().amrmg_serve_create(int(device), int(slots), int(budget_bytes), int(precision), C.byref(p)))
+        self.p = p
+        self._size = {}
+        if sparams is not None:
+            sp = np.ascontiguousarray(sparams, dtype=np.int32)
+            rp = np.ascontiguousarray(rparams, dtype=np.float64)
+            _chk(lib().amrmg_serve_set_params(self.p, sp.ctypes.data_as(_ip), rp.ctypes.data_as(_dp)))
+
+    def submit(self, hier, rhs):
+        isp, rsp = hier.spec()
+        isp = np.ascontiguousarray(isp, np.int32)
+        rsp = np.ascontiguousarray(rsp, np.float64)
+        rhs, pr = _d(rhs)
+        if rhs.size != hier.total_cells():
+            raise ValueError("rhs size does not match the hierarchy")
+        i = C.c_longlong()
+        _chk(lib().amrmg_serve_submit(self.p, isp.ctypes.data_as(_ip), rsp.ctypes.data_as(_dp), pr, C.byref(i)))
+        self._size[i.value] = rhs.size
+        return i.value
+
+    def wait(self, rid, want_phi=True):
+        n = self._size.pop(rid)
+        phi = np.zeros(n) if want_phi else None
+        t = np.zeros(3)
+        info = np.zeros(2, np.int32)
+        _chk(lib().amrmg_serve_wait(self.p, int(rid), phi.ctypes.data_as(_dp) if want_phi else None,
+                                     t.ctypes.data_as(_dp), info.ctypes.data_as(_ip)))
+        return dict(phi=phi, t_arrival=t[0], t_admit=t[1], t_done=t[2], latency=t[2] - t[0],
+                    iterations=int(info[0]), exit_status=int(info[1]))
+
+    def stats(self):
+        v = (C.c_longlong * 5)()
+        _chk(lib().amrmg_serve_stats(self.p, v, 5))
+        return dict(max_bytes=v[0], created=v[1], reused=v[2], evicted=v[3], max_running=v[4])
+
+    def close(self):
+        if self.p:
+            lib().amrmg_serve_destroy(self.p)
+            self.p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -844,17 +844,22 @@ void BatchSolver::relaxDepth(int d, int np, int iters, bool fromZero) {
   const int grid = (nw + 7) / 8 * 8;
   bool anyLap = false, anyHelm = false;
   for (int p = 0; p < np; p++) (L.h_coef[p].lap ? anyLap : anyHelm) = true;
+  double cells = 0;
+  for (int p = 0; p < np; p++) cells += h_active[p] ? (double)(L.cellStart[p + 1] - L.cellStart[p]) : 0.0;
   for (int it = 0; it < iters; it++) {
     const int zsrc = (fromZero && it == 0) ? 1 : 0;
     T* src = (T*)((it & 1) ? L.tmp : L.corr);
     T* dst = (T*)((it & 1) ? L.corr : L.tmp);
+    hipEvent_t ev = profile_begin(m_ctx, KC_GSRB);
     if (anyLap)
       launch_batch_fused<T, true>(m_ctx, L.shape, grid, L.d_items, L.d_wStart, np, L.d_coef, d_active, zsrc, dst, src,
                                   (const T*)L.res);
     if (anyHelm)
       launch_batch_fused<T, false>(m_ctx, L.shape, grid, L.d_items, L.d_wStart, np, L.d_coef, d_active, zsrc, dst,
                                    src, (const T*)L.res);
-    for (int p = 0; p < np; p++) m_cellRelax += h_active[p] ? (double)(L.cellStart[p + 1] - L.cellStart[p]) : 0.0;
+    // algorithmic bytes: read phi (not when it is known zero), read rhs, write phi
+    profile_end(m_ctx, ev, KC_GSRB, (zsrc ? 16.0 : 24.0) * sizeof(T) / 8.0 * cells, cells);
+    m_cellRelax += cells;
   }
   if (iters & 1)
     BATCH_LAUNCH(k_batch_copy<T>, L.tileStart[np], 256, L.d_tiles, 6, 2, L.d_rect, d_active, (T*)L.corr,
@@ -893,8 +898,12 @@ void BatchSolver::residualNorm(bool homog) {
   (void)homog;
   Depth& L = m_depth[0];
   BATCH_LAUNCH(k_batch_zero_norms, (m_np + 255) / 256, 256, m_norm, d_active, m_np);
+  double cells = 0;
+  for (int p = 0; p < m_np; p++) cells += h_active[p] ? (double)(L.cellStart[p + 1] - L.cellStart[p]) : 0.0;
+  hipEvent_t ev = profile_begin(m_ctx, KC_RESIDUAL);
   BATCH_LAUNCH(k_batch_residual_norm<T>, L.tileStart[m_np], 256, L.d_tiles, 6, 2, L.d_rect, L.d_coef, d_active,
                (T*)L.res, (const double*)m_phi, (const double*)m_rhs, m_norm);
+  profile_end(m_ctx, ev, KC_RESIDUAL, (16.0 + sizeof(T)) * cells, cells);
   hip_check(hipMemcpyAsync(h_norm, m_norm, sizeof(double) * m_np, hipMemcpyDeviceToHost, m_ctx->stream),
             "batch norms");
   m_ctx->sync();

@@ -1,8 +1,15 @@
 // extern "C" boundary of libamrmg.so (declarations and conventions in include/amrmg.h).
 #include "../../include/amrmg.h"
 
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
 #include <cstring>
+#include <deque>
 #include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -51,10 +58,10 @@ struct amrmg_solver {
   int sp[6] = {2, 2, 2, 1, 100, 5};
   double rp[3] = {1e-10, 1e-15, 1e-30};
   long long table_bytes = 0;
+  std::unique_ptr<MixedInner> mixed;  // precision 1: FP32 inner cycles
   // Per-box storage for the FAB-level entry points (amrmg_kernel_gsrb,
   // amrmg_homog_cf, amrmg_quad_cf), whose ghost cells are caller inputs like the
   // ghost cells of the reference's FArrayBoxes; built on first use.
-  std::unique_ptr<MixedInner> mixed;  // precision 1: FP32 inner cycles
   std::unique_ptr<DeviceHierarchy> fabH;
   std::unique_ptr<AMRPoissonOpFactory<double>> fabFac;
   std::vector<std::unique_ptr<AMRPoissonOp<double>>> fabOps;
@@ -221,6 +228,25 @@ int amrmg_solver_set_params(amrmg_solver* s, const int* sp, const double* rp) {
   s->amg.setSolverParameters(sp[0], sp[1], sp[2], sp[3], sp[4], rp[0], rp[1], rp[2]);
   s->amg.m_imin = sp[5];
   if (s->mixed) s->mixed->setSolverParameters(sp[0], sp[1], sp[2], sp[3], sp[4], rp[0], rp[1], rp[2]);
+  API_END
+}
+int amrmg_solver_set_alpha_beta(amrmg_solver* s, double alpha, double beta) {
+  API_BEGIN
+  s->alpha = alpha;
+  s->beta = beta;
+  auto set = [&](auto* op) {
+    op->m_alpha = alpha;
+    op->m_beta = beta;
+  };
+  for (auto& op : s->amg.m_op) set(op.get());
+  for (int d = 1; d < s->H.mgDepth(); d++) set(s->mgop(d));
+  if (s->mixed) {
+    for (auto& op : s->mixed->amg.m_op) set(op.get());
+    for (int d = 1; d < s->H.mgDepth(); d++) set(static_cast<AMRPoissonOp<float>*>(s->mixed->amg.m_mg0->m_op[d]));
+  }
+  s->fabOps.clear();
+  s->fabFac.reset();
+  s->fabH.reset();
   API_END
 }
 long long amrmg_solver_level_cells(amrmg_solver* s, int lev) {
@@ -550,6 +576,232 @@ int amrmg_batch_stats(amrmg_batch* b, double* cell_relax, long long* bottom_iter
   if (cell_relax) *cell_relax = b->solver->cellRelaxations();
   if (bottom_iters) *bottom_iters = b->solver->bottomIterations();
   if (device_bytes) *device_bytes = b->solver->deviceBytes();
+  API_END
+}
+
+// ---- serving: a request loop over device-resident hierarchies ----
+namespace {
+double serve_now(const std::chrono::steady_clock::time_point& t0) {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+// HBM footprint estimate of a solver for a spec (fields as amrmg_solver_device_bytes
+// counts them, a 1.1 ghost factor, multigrid levels as N0/7)
+long long serve_footprint(const std::vector<DBL>& g, int precision) {
+  double n = 0;
+  for (auto& d : g) n += (double)d.numCells();
+  const double n0 = (double)g[0].numCells();
+  double b = 1.1 * 8.0 * (9.0 * n + 3.0 * n0 / 7.0);
+  if (precision == 1) b += 1.1 * 4.0 * (6.0 * n + 3.0 * n0 / 7.0);
+  return (long long)b + (1LL << 20);
+}
+struct ServeInstance {
+  amrmg_ctx* ctx = nullptr;  // its own stream and reduction scratch
+  amrmg_solver* s = nullptr;
+  std::string key;
+  long long bytes = 0;
+  double lastUse = 0;
+  ~ServeInstance() {
+    if (s) amrmg_solver_destroy(s);
+    if (ctx) amrmg_ctx_destroy(ctx);
+  }
+};
+struct ServeReq {
+  long long id = 0;
+  std::vector<int> ispec;
+  std::vector<double> rspec, rhs, phi, hist;
+  std::string key;
+  long long bytes = 0;
+  double tArrival = 0, tAdmit = 0, tDone = 0;
+  int iters = 0, exitStatus = -1;
+  bool done = false, taken = false;
+  std::string error;
+};
+}  // namespace
+
+struct amrmg_serve {
+  int device = 0, precision = 0;
+  long long budget = 0;
+  int sp[6] = {2, 2, 2, 1, 100, 5};
+  double rp[3] = {1e-10, 1e-15, 1e-30};
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<ServeReq*> queue;                              // waiting for admission, FIFO
+  std::unordered_map<long long, std::unique_ptr<ServeReq>> reqs;
+  std::vector<std::unique_ptr<ServeInstance>> idle;        // device-resident, not in use
+  long long bytesInUse = 0, bytesIdle = 0, maxBytes = 0;
+  long long created = 0, reused = 0, evicted = 0;
+  int running = 0, maxRunning = 0;
+  bool stop = false;
+  long long nextId = 1;
+  std::vector<std::thread> workers;
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+
+  void worker() {
+    for (;;) {
+      ServeReq* r = nullptr;
+      std::unique_ptr<ServeInstance> inst;
+      std::vector<std::unique_ptr<ServeInstance>> evict;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        // admission: the head request, once its footprint fits beside the running ones
+        cv.wait(lk, [&] { return stop || (!queue.empty() && bytesInUse + queue.front()->bytes <= budget); });
+        if (stop) return;
+        r = queue.front();
+        queue.pop_front();
+        for (size_t q = 0; q < idle.size(); q++)
+          if (idle[q]->key == r->key) {
+            inst = std::move(idle[q]);
+            idle.erase(idle.begin() + q);
+            bytesIdle -= inst->bytes;
+            reused++;
+            break;
+          }
+        // make room: evict the least recently used idle hierarchies
+        while (bytesInUse + r->bytes + bytesIdle > budget && !idle.empty()) {
+          auto lru = std::min_element(idle.begin(), idle.end(),
+                                      [](const auto& a, const auto& b) { return a->lastUse < b->lastUse; });
+          bytesIdle -= (*lru)->bytes;
+          evict.push_back(std::move(*lru));
+          idle.erase(lru);
+          evicted++;
+        }
+        bytesInUse += r->bytes;
+        maxBytes = std::max(maxBytes, bytesInUse + bytesIdle);
+        running++;
+        maxRunning = std::max(maxRunning, running);
+        r->tAdmit = serve_now(t0);
+      }
+      evict.clear();
+      try {
+        if (!inst) {
+          inst.reset(new ServeInstance());
+          inst->key = r->key;
+          inst->bytes = r->bytes;
+          if (amrmg_ctx_create(device, &inst->ctx) || amrmg_solver_create(inst->ctx, r->ispec.data(), r->rspec.data(),
+                                                                          precision, &inst->s))
+            throw std::runtime_error(amrmg_last_error());
+          std::lock_guard<std::mutex> lk(mu);
+          created++;
+        }
+        double hist[256];
+        int info[3];
+        if (amrmg_solver_set_params(inst->s, sp, rp) || amrmg_solver_set_field(inst->s, 1, r->rhs.data()) ||
+            amrmg_solver_solve(inst->s, 1, hist, 256, info))
+          throw std::runtime_error(amrmg_last_error());
+        r->phi.resize(r->rhs.size());
+        if (amrmg_solver_get_field(inst->s, 0, r->phi.data())) throw std::runtime_error(amrmg_last_error());
+        r->iters = info[0];
+        r->exitStatus = info[1];
+        r->hist.assign(hist, hist + std::min(256, info[0] + 1));
+      } catch (const std::exception& e) {
+        r->error = e.what();
+        inst.reset();
+      }
+      std::lock_guard<std::mutex> lk(mu);
+      r->tDone = serve_now(t0);
+      r->done = true;
+      r->rhs.clear();
+      r->rhs.shrink_to_fit();
+      bytesInUse -= r->bytes;
+      running--;
+      if (inst) {
+        inst->lastUse = r->tDone;
+        bytesIdle += inst->bytes;
+        idle.push_back(std::move(inst));
+      }
+      cv.notify_all();
+    }
+  }
+};
+
+int amrmg_serve_create(int device, int nslots, long long hbm_budget, int precision, amrmg_serve** out) {
+  API_BEGIN
+  if (nslots <= 0 || nslots > 64) throw std::runtime_error("serve: slots must be in 1..64");
+  if (hbm_budget <= 0) throw std::runtime_error("serve: HBM budget must be positive");
+  if (precision != 0 && precision != 1) throw std::runtime_error("serve: precision must be 0 or 1");
+  std::unique_ptr<amrmg_serve> e(new amrmg_serve());
+  e->device = device;
+  e->precision = precision;
+  e->budget = hbm_budget;
+  for (int q = 0; q < nslots; q++) e->workers.emplace_back([p = e.get()] { p->worker(); });
+  *out = e.release();
+  API_END
+}
+int amrmg_serve_set_params(amrmg_serve* e, const int* sp, const double* rp) {
+  API_BEGIN
+  std::lock_guard<std::mutex> lk(e->mu);
+  std::memcpy(e->sp, sp, sizeof e->sp);
+  std::memcpy(e->rp, rp, sizeof e->rp);
+  API_END
+}
+int amrmg_serve_submit(amrmg_serve* e, const int* ispec, const double* rspec, const double* rhs, long long* id) {
+  API_BEGIN
+  Box dom;
+  BCSpec bc;
+  double dx0, a, b;
+  std::vector<DBL> g = parse_spec(ispec, rspec, dom, bc, dx0, a, b);
+  int nint = 13 + g.size();
+  long long ncell = 0;
+  for (auto& d : g) {
+    nint += 6 * d.size();
+    ncell += d.numCells();
+  }
+  std::unique_ptr<ServeReq> r(new ServeReq());
+  r->ispec.assign(ispec, ispec + nint);
+  r->rspec.assign(rspec, rspec + 9);
+  r->rhs.assign(rhs, rhs + ncell);
+  r->key.assign((const char*)ispec, sizeof(int) * nint);
+  r->key.append((const char*)rspec, sizeof(double) * 9);
+  r->bytes = serve_footprint(g, e->precision);
+  if (r->bytes > e->budget) throw std::runtime_error("serve: request larger than the HBM budget");
+  std::lock_guard<std::mutex> lk(e->mu);
+  r->id = e->nextId++;
+  r->tArrival = serve_now(e->t0);
+  *id = r->id;
+  e->queue.push_back(r.get());
+  e->reqs[r->id] = std::move(r);
+  e->cv.notify_all();
+  API_END
+}
+// blocks until request id is done; phi (valid-packed, may be null), times =
+// {arrival, admission, completion} in seconds since the engine started, info =
+// {iterations, exit status}; the request is then forgotten
+int amrmg_serve_wait(amrmg_serve* e, long long id, double* phi, double* times, int* info) {
+  API_BEGIN
+  std::unique_lock<std::mutex> lk(e->mu);
+  auto it = e->reqs.find(id);
+  if (it == e->reqs.end()) throw std::runtime_error("serve: unknown request id");
+  ServeReq* r = it->second.get();
+  e->cv.wait(lk, [&] { return r->done; });
+  if (!r->error.empty()) {
+    std::string m = r->error;
+    e->reqs.erase(it);
+    throw std::runtime_error("serve: request failed: " + m);
+  }
+  if (phi) std::memcpy(phi, r->phi.data(), sizeof(double) * r->phi.size());
+  if (times) { times[0] = r->tArrival; times[1] = r->tAdmit; times[2] = r->tDone; }
+  if (info) { info[0] = r->iters; info[1] = r->exitStatus; }
+  e->reqs.erase(it);
+  API_END
+}
+// {max bytes in use (running + idle hierarchies), hierarchies created, reused,
+//  evicted, most requests running at once}
+int amrmg_serve_stats(amrmg_serve* e, long long* stats, int n) {
+  API_BEGIN
+  std::lock_guard<std::mutex> lk(e->mu);
+  const long long v[5] = {e->maxBytes, e->created, e->reused, e->evicted, e->maxRunning};
+  for (int q = 0; q < n && q < 5; q++) stats[q] = v[q];
+  API_END
+}
+int amrmg_serve_destroy(amrmg_serve* e) {
+  API_BEGIN
+  {
+    std::lock_guard<std::mutex> lk(e->mu);
+    e->stop = true;
+    e->cv.notify_all();
+  }
+  for (auto& t : e->workers) t.join();
+  delete e;
   API_END
 }
 

@@ -96,6 +96,10 @@ static void prof_end(Context* c, hipEvent_t a, int kind, double bytes, double un
   c->prof->recs.push_back(Profiler::Rec{kind, a, b, bytes, units});
   if (c->prof->recs.size() > 4096) c->prof->drain();
 }
+hipEvent_t profile_begin(Context* c, int kind) { return prof_begin(c, kind); }
+void profile_end(Context* c, hipEvent_t a, int kind, double bytes, double units) {
+  prof_end(c, a, kind, bytes, units);
+}
 void profiler_enable(Context* c, unsigned mask) {
   if (mask && !c->prof) c->prof = new Profiler();
   if (mask && c->prof) c->prof->mask = mask;

@@ -12,6 +12,9 @@
 namespace amrmg {
 void profiler_enable(Context* c, unsigned class_mask);  // 0 = off
 void profiler_reset(Context* c);
+// event pair around one launch of a kernel class (no-op when that class is not timed)
+hipEvent_t profile_begin(Context* c, int kernel_class);
+void profile_end(Context* c, hipEvent_t begin, int kernel_class, double bytes, double units = 0.0);
 bool profiler_read(Context* c, int kind, double* ms, double* bytes, long long* count, double* units = nullptr);
 // average ms of one copy of nbytes (read nbytes + write nbytes) over iters launches
 double copy_probe(Context* c, long long nbytes, int iters);

@@ -281,3 +281,56 @@ def batch_problems(count: int = 512, seed: int = 1004, n_min: int = 16, n_max: i
         rhs = sinusoid_rhs(h, seed * 100003 + q, k=k)
         out.append((h, rhs))
     return out
+
+
+# ---- serving workload ----
+def bump_rhs_separable(h: Hierarchy, seed: int, nbumps: int = 3) -> np.ndarray:
+    """The Gaussian-bump rhs of bump_rhs (same bump parameters for a seed), each bump
+    evaluated as a product of three one-dimensional exponentials per box, which is
+    fast enough to generate thousands of requests; values agree with bump_rhs to
+    rounding, not bitwise."""
+    rng = np.random.default_rng(seed)
+    cs = rng.uniform(0.4, 0.6, size=(nbumps, 3))
+    ws = rng.uniform(5e-3, 2e-2, size=nbumps)
+    ss = rng.uniform(1.0, 3.0, size=nbumps)
+    parts = []
+    for lev in range(h.nlev):
+        dx = h.dx(lev)
+        for (l, hh) in h.levels[lev]:
+            xs = [(np.arange(l[d], hh[d] + 1) + 0.5) * dx for d in range(3)]
+            v = np.zeros((len(xs[0]), len(xs[1]), len(xs[2])))
+            for c, w, sv in zip(cs, ws, ss):
+                e = [np.exp(-((xs[d] - c[d]) ** 2) / w) for d in range(3)]
+                v += sv * e[0][:, None, None] * e[1][None, :, None] * e[2][None, None, :]
+            parts.append(v.ravel(order="F"))
+    return np.concatenate(parts)
+
+
+def serve_requests(count: int = 2000, seed: int = 1005, bases=(32, 64, 128, 256)):
+    """Requests of 1-2-level AMR Poisson solves: base n ~ U{bases}, levels ~ U{1,2};
+    the refined region covers a fraction f_r ~ U(0.05, 0.25) of the base domain's
+    volume as a centred cube of side 8*round(n f_r^(1/3) / 8) (ratio 2, 64^3 maximum
+    boxes, blocking factor 8).  Requests with the same hierarchy share one bump rhs
+    (bump_rhs_separable, seeded by the hierarchy).  Returns a list of (Hierarchy, rhs)."""
+    rng = np.random.default_rng(seed)
+    cache = {}
+    out = []
+    for q in range(count):
+        n = int(rng.choice(bases))
+        nlev = int(rng.integers(1, 3))
+        side = 0
+        if nlev == 2:
+            fr = float(rng.uniform(0.05, 0.25))
+            side = max(8, 8 * int(round(n * fr ** (1.0 / 3.0) / 8)))
+        key = (n, side)
+        if key not in cache:
+            dom = box((0, 0, 0), (n - 1, n - 1, n - 1))
+            levels = [domain_split(dom, 64, 8)]
+            if side:
+                lo = 8 * (((n - side) // 2) // 8)
+                reg = box((lo, lo, lo), (lo + side - 1,) * 3)
+                levels.append(domain_split(box_refine(reg, 2), 64, 8))
+            h = Hierarchy(dom, levels, 1.0 / n)
+            cache[key] = (h, bump_rhs_separable(h, seed * 1009 + n * 7 + side))
+        out.append(cache[key])
+    return out

@@ -61,6 +61,10 @@ int amrmg_solver_create(amrmg_ctx* ctx, const int* ispec, const double* rspec, i
                         amrmg_solver** out);
 int amrmg_solver_destroy(amrmg_solver* s);
 int amrmg_solver_set_params(amrmg_solver* s, const int* sparams, const double* rparams);
+/* AMRPoissonOp::setAlphaAndBeta (AMRPoissonOp.cpp:1103-1108, unit aCoef and bCoef) on
+   every operator of the solver (AMR levels and multigrid levels): the same as defining the solver
+   with these coefficients. */
+int amrmg_solver_set_alpha_beta(amrmg_solver* s, double alpha, double beta);
 long long amrmg_solver_level_cells(amrmg_solver* s, int lev);
 int amrmg_solver_num_mg_depths(amrmg_solver* s);
 long long amrmg_solver_mg_cells(amrmg_solver* s, int depth);
@@ -140,6 +144,26 @@ int amrmg_batch_history(amrmg_batch* b, int problem, double* hist, int cap, int*
 int amrmg_batch_tune_relax(amrmg_batch* b, double* ms, int cap, int* ndepth);
 /* cell-relaxations and bottom BiCGStab iterations of the last solve; HBM bytes held */
 int amrmg_batch_stats(amrmg_batch* b, double* cell_relax, long long* bottom_iters, long long* device_bytes);
+
+/* ---- serving: a request loop with continuous admission.  nslots worker threads
+   each run one request at a time on its own HIP stream, so the solves of
+   different requests overlap on the device.  A request is admitted (FIFO) as soon
+   as a slot is free and its HBM footprint estimate fits in the budget beside the
+   running ones; hierarchies stay device-resident after a solve and are reused by
+   later requests with the same spec (least recently used ones are evicted to make
+   room).  No counterpart in the reference. ---- */
+typedef struct amrmg_serve amrmg_serve;
+int amrmg_serve_create(int device, int nslots, long long hbm_budget_bytes, int precision, amrmg_serve** out);
+int amrmg_serve_set_params(amrmg_serve* e, const int* sparams, const double* rparams);
+/* copies the spec and the valid-packed rhs; the arrival time is the call's time */
+int amrmg_serve_submit(amrmg_serve* e, const int* ispec, const double* rspec, const double* rhs, long long* id);
+/* blocks until request id is solved; phi valid-packed (may be null); times =
+   {arrival, admission, completion} in s since the engine started; info =
+   {iterations, exit status} */
+int amrmg_serve_wait(amrmg_serve* e, long long id, double* phi, double* times, int* info);
+/* {max HBM bytes held, hierarchies created, reused, evicted, max requests running at once} */
+int amrmg_serve_stats(amrmg_serve* e, long long* stats, int n);
+int amrmg_serve_destroy(amrmg_serve* e);
 
 /* Plain device copy of nbytes (reads nbytes, writes nbytes), iters launches;
    average ms per launch.  Calibrates HBM byte counters and the copy bandwidth. */
