@@ -104,6 +104,11 @@ void AMRPoissonOp<Real>::incr(T& lhs, const T& x, double s) {
   ops::incr<Real>(*lhs.lay, lhs.d, x.d, s);
 }
 template <class Real>
+void AMRPoissonOp<Real>::incrAssign(T& lhs, T& x) {
+  need(x.lay, lhs.lay, "incrAssign");
+  ops::incr_assign<Real>(*lhs.lay, lhs.d, x.d);
+}
+template <class Real>
 void AMRPoissonOp<Real>::axby(T& lhs, const T& x, const T& y, double a, double b) {
   need(x.lay, lhs.lay, "axby"); need(y.lay, lhs.lay, "axby");
   ops::axby<Real>(*lhs.lay, lhs.d, x.d, y.d, a, b);
@@ -619,8 +624,18 @@ void AMRMultiGrid<Real>::computeAMROperator(std::vector<T*>& lphi, std::vector<T
 template <class Real>
 void AMRMultiGrid<Real>::AMRVCycle(std::vector<T*>& uberCorrection, std::vector<T*>& uberResidual, int ilev,
                                    int l_max, int l_base, bool fromZero) {
-  if (ilev == l_max) {
+  if (ilev == l_max && l_max > l_base) {
+    // (with one level the cycle works on uberCorrection / uberResidual directly and
+    // the reference's copies below are never read)
     for (int level = l_base; level <= l_max; level++) {
+      if (level == l_max) {
+        // uberResidual[l_max] is read only by this copy until the next
+        // computeAMRResidual rewrites it, so the buffers trade places instead;
+        // m_correction[l_max] is first written by its relaxation from zero
+        need(m_residual[level]->lay, uberResidual[level]->lay, "AMRVCycle");
+        std::swap(m_residual[level]->d, uberResidual[level]->d);
+        continue;
+      }
       m_op[level]->assignLocal(*m_residual[level], *uberResidual[level]);
       m_op[level]->setToZero(*m_correction[level]);
     }
@@ -652,8 +667,9 @@ void AMRMultiGrid<Real>::AMRVCycle(std::vector<T*>& uberCorrection, std::vector<
     m_op[ilev]->AMRUpdateResidual(*m_residual[ilev], *m_correction[ilev], *m_correction[ilev - 1]);
     T& dCorr = *uberCorrection[ilev];
     relax(dCorr, *m_residual[ilev], ilev, m_post, true);  // setToZero(dCorr) + relax
-    m_op[ilev]->incr(*m_correction[ilev], dCorr, 1.0);
-    m_op[ilev]->assignLocal(*uberCorrection[ilev], *m_correction[ilev]);
+    // incr(m_correction, dCorr, 1.0) then assignLocal(uberCorrection, m_correction),
+    // in one pass over the valid cells (every later read of a ghost refills it)
+    m_op[ilev]->incrAssign(*m_correction[ilev], dCorr);
   }
 }
 // solveNoInitResid (AMRMultiGrid.H:935-1124)

@@ -60,6 +60,8 @@ class AMRPoissonOp : public AMRLevelOp<LevelData<Real>> {
   void assignLocal(T& lhs, const T& rhs) override;
   double dotProduct(const T& a, const T& b) override;
   void incr(T& lhs, const T& x, double scale) override;
+  // lhs = lhs + x * 1.0 then x = lhs on the valid cells (incr + assignLocal fused)
+  void incrAssign(T& lhs, T& x);
   void axby(T& lhs, const T& x, const T& y, double a, double b) override;
   void scale(T& lhs, double s) override;
   double norm(const T& x, int ord) override;

@@ -12,6 +12,8 @@
 
 #include "batch.hpp"
 
+#include <cstring>
+
 namespace amrmg {
 
 namespace {
@@ -208,6 +210,8 @@ struct BottomArgs {
   double reps, hang, small;
   int numRestarts;
   int nbottom;                 // relax iterations before the Krylov solve
+  int relaxFirst;              // 1: zero the correction and relax nbottom times first (the
+                               // whole bottom of MultiGrid::cycle); 0: start from the correction
 };
 
 struct BottomLevel {
@@ -339,15 +343,17 @@ __global__ void __launch_bounds__(256)
   const int tid = threadIdx.x;
   T* e32 = corr[p];
   const T* r32 = res[p];
-  // MultiGrid::cycle at the bottom: the correction is zero on entry
-  for (long long q = tid; q < L.ncell; q += L.nt) {
-    int i, j, k;
-    L.cell(q, i, j, k);
-    e32[ridx(R, i, j, k)] = T(0.0);
+  if (ba.relaxFirst) {
+    // MultiGrid::cycle at the bottom: the correction is zero on entry
+    for (long long q = tid; q < L.ncell; q += L.nt) {
+      int i, j, k;
+      L.cell(q, i, j, k);
+      e32[ridx(R, i, j, k)] = T(0.0);
+    }
+    __syncthreads();
+    L.relax(R, e32, r32, ba.nbottom);
+    if (L.ncell == 1) return;  // m_bottomCells == 1: relax only
   }
-  __syncthreads();
-  L.relax(R, e32, r32, ba.nbottom);
-  if (L.ncell == 1) return;  // m_bottomCells == 1: relax only
   const long long vs = 15 + (long long)Rs.sz * (R.hi[2] - R.lo[2] + 3);  // elements per scratch vector
   double* base = scratch + soff[p];
   double *x = base, *b = base + vs, *r = base + 2 * vs, *r_tilde = base + 3 * vs, *e = base + 4 * vs,
@@ -880,7 +886,7 @@ void BatchSolver::vcycle() {
     BATCH_LAUNCH(k_batch_restrict<T>, C.tileStart[np], 256, C.d_tiles, 6, 2, C.d_rect, F.d_rect, F.d_coef, d_active,
                  (T*)C.res, (const T*)F.corr, (const T*)F.res);
   }
-  BottomArgs ba{80, 1e-12, 1e-8, 1e-30, 5, m_nbottom};
+  BottomArgs ba{80, 1e-12, 1e-8, 1e-30, 5, m_nbottom, 1};
   BATCH_LAUNCH(k_batch_bottom<T>, m_np, 256, d_bottom, m_bcoef, d_bbox, (T* const*)d_corrPtr,
                (const T* const*)d_resPtr, d_bscratch, d_bscratchOff, d_metric, d_active, d_bstats, ba);
   for (int d = D - 2; d >= 0; d--) {
@@ -1023,5 +1029,151 @@ std::vector<double> BatchSolver::tuneRelax() {
   m_cellRelax = 0;
   return best;
 }
+
+// ---------------------------------------------------------------------------
+// DeviceBottomSolver: the bottom BiCGStab of one hierarchy's multigrid as one
+// workgroup on the device (the batched bottom kernel on a one-entry table)
+// ---------------------------------------------------------------------------
+template <class T>
+DeviceBottomSolver<T>::~DeviceBottomSolver() {
+  release();
+}
+template <class T>
+void DeviceBottomSolver<T>::release() {
+  if (m_ctx->dry) return;
+  if (m_ctx->stream) (void)hipStreamSynchronize(m_ctx->stream);
+  void* ps[] = {d_bottom, d_bcoef, d_bbox, d_ptrs, d_scratch, d_soff, d_metric, d_active, d_stats};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+  if (h_stage) (void)hipHostFree(h_stage);
+  d_bottom = nullptr; d_bcoef = nullptr; d_bbox = nullptr; d_ptrs = nullptr; d_scratch = nullptr;
+  d_soff = nullptr; d_metric = nullptr; d_active = nullptr; d_stats = nullptr; h_stage = nullptr;
+}
+// the operator's coefficients as the bottom kernel reads them (face rules are set by define)
+template <class T>
+static BatchCoef bottom_coef(const AMRPoissonOp<T>* ap) {
+  BatchCoef c{};
+  c.dx = ap->m_dx;
+  c.dxinv = 1.0 / (c.dx * c.dx);
+  double sum_b = 0.0;
+  for (int q = 0; q < 3; q++) sum_b = sum_b + 2.0 * c.dxinv;
+  c.lap = (ap->m_alpha == 0.0 && ap->m_beta == 1.0) ? 1 : 0;
+  c.lambda = c.lap ? -1.0 / sum_b : -1.0 / (ap->m_alpha - ap->m_beta * sum_b);
+  c.alpha = ap->m_alpha;
+  c.beta = ap->m_beta;
+  return c;
+}
+template <class T>
+void DeviceBottomSolver<T>::define(LinearOp<LevelData<T>>* op, bool homogeneous) {
+  m_fallback.define(op, homogeneous);
+  if (m_device && op == m_boundOp && homogeneous) return;  // MultiGrid re-binds its bottom at every solve
+  m_device = false;
+  m_boundOp = op;
+  auto* ap = dynamic_cast<AMRPoissonOp<T>*>(op);
+  if (!homogeneous || !ap || !ap->m_lay->merged || m_ctx->dry) return;
+  release();
+  const DeviceLayout& L = *ap->m_lay;
+  m_lay = &L;
+  BatchBottom b{};
+  b.rect = L.h_boxes[L.nbox];
+  b.cells = L.valid_cells;
+  b.box0 = 0;
+  b.nbox = L.nbox;
+  if (b.nbox > 1024) return;
+  std::vector<DBox> boxes;
+  for (int q = 0; q < L.nbox; q++) {
+    const DBox& x0 = L.h_boxes[q];
+    if (x0.hi[0] - x0.lo[0] + 1 > 1024) return;
+    DBox x = x0;  // addressed in the scratch vectors (rectangle at offset 15)
+    x.off = 15 + (x0.lo[0] - b.rect.lo[0]) + (long long)(x0.lo[1] - b.rect.lo[1]) * b.rect.sy +
+            (long long)(x0.lo[2] - b.rect.lo[2]) * b.rect.sz;
+    boxes.push_back(x);
+  }
+  BatchCoef c = bottom_coef(ap);
+  // the bottom of a merged multigrid level is a whole (coarsened) domain: every
+  // face is a domain face with the operator's BC type
+  for (int dir = 0; dir < 3; dir++)
+    for (int side = 0; side < 2; side++) {
+      const int face = side ? b.rect.hi[dir] : b.rect.lo[dir];
+      const int dface = side ? ap->m_domain.hi[dir] : ap->m_domain.lo[dir];
+      if (face != dface) return;  // a level with coarse-fine faces keeps the host-driven solver
+      c.kind[dir][side] = ap->m_bc.type[side][dir];
+      c.val[dir][side] = ap->m_bc.val[side][dir];
+    }
+  const long long vs = 15 + (long long)b.rect.sz * (b.rect.hi[2] - b.rect.lo[2] + 3);
+  m_ap = ap;
+  m_coef = c;
+  long long bytes = 0;
+  d_bottom = upload(m_ctx, std::vector<BatchBottom>{b}, &bytes);
+  d_bcoef = upload(m_ctx, std::vector<BatchCoef>{c}, &bytes);
+  d_bbox = upload(m_ctx, boxes, &bytes);
+  d_soff = upload(m_ctx, std::vector<long long>{0}, &bytes);
+  d_active = upload(m_ctx, std::vector<int>{1}, &bytes);
+  d_stats = upload(m_ctx, std::vector<int>{0}, &bytes);
+  hip_check(hipMalloc(&d_scratch, sizeof(double) * 10 * vs), "hipMalloc bottom scratch");
+  hip_check(hipMalloc(&d_metric, sizeof(double) * 2), "hipMalloc bottom metric");
+  hip_check(hipMalloc(&d_ptrs, sizeof(void*) * 2), "hipMalloc bottom pointers");
+  hip_check(hipHostMalloc(&h_stage, 64 + sizeof(BatchCoef)), "hipHostMalloc bottom staging");
+  m_device = true;
+  setConvergenceMetrics(m_metric, m_eps);
+}
+template <class T>
+void DeviceBottomSolver<T>::setConvergenceMetrics(double metric, double tol) {
+  m_metric = metric;
+  m_eps = tol;
+  m_fallback.setConvergenceMetrics(metric, tol);
+  if (!m_device) return;
+  double* h = (double*)h_stage;
+  (void)hipStreamSynchronize(m_ctx->stream);  // the staging buffer may still feed an earlier copy
+  h[0] = metric;
+  h[1] = tol;
+  hip_check(hipMemcpyAsync(d_metric, h, sizeof(double) * 2, hipMemcpyHostToDevice, m_ctx->stream), "bottom metric");
+}
+template <class T>
+void DeviceBottomSolver<T>::solve(LevelData<T>& phi, const LevelData<T>& rhs) {
+  if (!m_device) {
+    m_fallback.solve(phi, rhs);
+    return;
+  }
+  if (phi.lay != m_lay || rhs.lay != m_lay) throw std::runtime_error("DeviceBottomSolver: layout mismatch");
+  // the fields' buffers move (the relaxation swaps them), so their addresses go with
+  // each launch; the operator's coefficients may have been reset since define
+  void** h = (void**)((char*)h_stage + 32);
+  (void)hipStreamSynchronize(m_ctx->stream);
+  h[0] = phi.d;
+  h[1] = (void*)rhs.d;
+  hip_check(hipMemcpyAsync(d_ptrs, h, sizeof(void*) * 2, hipMemcpyHostToDevice, m_ctx->stream), "bottom pointers");
+  const auto* ap = static_cast<const AMRPoissonOp<T>*>(m_ap);
+  if (ap->m_alpha != m_coef.alpha || ap->m_beta != m_coef.beta || ap->m_dx != m_coef.dx) {
+    BatchCoef c = bottom_coef(ap);
+    for (int dir = 0; dir < 3; dir++)
+      for (int side = 0; side < 2; side++) {
+        c.kind[dir][side] = m_coef.kind[dir][side];
+        c.val[dir][side] = m_coef.val[dir][side];
+      }
+    m_coef = c;
+    std::memcpy((char*)h_stage + 64, &m_coef, sizeof(BatchCoef));
+    hip_check(hipMemcpyAsync(d_bcoef, (char*)h_stage + 64, sizeof(BatchCoef), hipMemcpyHostToDevice, m_ctx->stream),
+              "bottom coefficients");
+  }
+  BottomArgs ba{m_fallback.m_imax, m_fallback.m_reps, m_fallback.m_hang, m_fallback.m_small,
+                m_fallback.m_numRestarts, 0, 0};
+  hipLaunchKernelGGL(k_batch_bottom<T>, dim3(1), dim3(256), 0, m_ctx->stream, (const BatchBottom*)d_bottom,
+                     (const BatchCoef*)d_bcoef, (const DBox*)d_bbox, (T* const*)d_ptrs,
+                     (const T* const*)((void**)d_ptrs + 1), d_scratch, (const long long*)d_soff,
+                     (const double*)d_metric, (const int*)d_active, (int*)d_stats, ba);
+  hip_check(hipGetLastError(), "k_batch_bottom");
+}
+template <class T>
+long long DeviceBottomSolver<T>::totalIterations() {
+  if (!m_device) return m_fallback.m_total_iters;
+  int v = 0;
+  hip_check(hipMemcpyAsync(h_stage, d_stats, sizeof(int), hipMemcpyDeviceToHost, m_ctx->stream), "bottom stats");
+  m_ctx->sync();
+  std::memcpy(&v, h_stage, sizeof(int));
+  return v;
+}
+template class DeviceBottomSolver<double>;
+template class DeviceBottomSolver<float>;
 
 }  // namespace amrmg

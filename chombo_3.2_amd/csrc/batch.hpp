@@ -20,6 +20,7 @@
 #include <memory>
 #include <vector>
 
+#include "amrpoissonop.hpp"
 #include "fused.hpp"
 #include "ops.hpp"
 #include "plans.hpp"
@@ -47,6 +48,40 @@ struct BatchBottom {
   int box0, nbox;   // its boxes in the bottom box table
   int depth;
   long long cells;
+};
+
+// The bottom solve of one hierarchy's multigrid (BiCGStabSolver::solve on the
+// deepest level, homogeneous) run on the device as one workgroup: the control flow,
+// level operations and reduction orders of BiCGStabSolver, with no host round
+// trip per iteration.  With T = float it widens the correction and the residual
+// and solves in FP64, as PromotedBottomSolver does.  Levels that are not stored
+// merged over a whole domain use the host-driven BiCGStabSolver (FP64 only).
+template <class T>
+class DeviceBottomSolver : public LinearSolver<LevelData<T>> {
+ public:
+  explicit DeviceBottomSolver(Context* c) : m_ctx(c) {}
+  ~DeviceBottomSolver() override;
+  DeviceBottomSolver(const DeviceBottomSolver&) = delete;
+  DeviceBottomSolver& operator=(const DeviceBottomSolver&) = delete;
+  void define(LinearOp<LevelData<T>>* op, bool homogeneous) override;
+  void solve(LevelData<T>& phi, const LevelData<T>& rhs) override;
+  void setConvergenceMetrics(double metric, double tol) override;
+  long long totalIterations();
+  bool onDevice() const { return m_device; }
+  BiCGStabSolver<LevelData<T>> m_fallback;   // parameters (m_imax, m_reps, ...) are read from here
+
+ private:
+  Context* m_ctx;
+  bool m_device = false;
+  const DeviceLayout* m_lay = nullptr;
+  const void* m_boundOp = nullptr;
+  const void* m_ap = nullptr;     // the bound AMRPoissonOp<T>
+  BatchCoef m_coef{};             // coefficients uploaded for it
+  double m_metric = -1.0, m_eps = 1e-6;
+  void *d_bottom = nullptr, *d_bcoef = nullptr, *d_bbox = nullptr, *d_ptrs = nullptr, *d_soff = nullptr;
+  void *d_active = nullptr, *d_stats = nullptr, *h_stage = nullptr;
+  double *d_scratch = nullptr, *d_metric = nullptr;
+  void release();
 };
 
 class BatchSolver {

@@ -53,7 +53,7 @@ struct amrmg_solver {
   double alpha = 0, beta = 1;
   AMRPoissonOpFactory<double> fac;
   AMRMultiGrid<double> amg;
-  BiCGStabSolver<LevelData<double>> bottom;
+  std::unique_ptr<DeviceBottomSolver<double>> bottom;
   std::vector<std::unique_ptr<LevelData<double>>> phi, rhs, res, uberRes, uberCorr;
   int sp[6] = {2, 2, 2, 1, 100, 5};
   double rp[3] = {1e-10, 1e-15, 1e-30};
@@ -194,7 +194,8 @@ int amrmg_solver_create(amrmg_ctx* ctx, const int* ispec, const double* rspec, i
   std::vector<DBL> g = parse_spec(ispec, rspec, dom, s->bc, dx0, s->alpha, s->beta);
   s->H.define(ctx->c.get(), g, dom, dx0);
   s->fac.define(&s->H, s->alpha, s->beta, s->bc);
-  s->amg.define(s->fac, &s->bottom, s->nlev());
+  s->bottom.reset(new DeviceBottomSolver<double>(ctx->c.get()));
+  s->amg.define(s->fac, s->bottom.get(), s->nlev());
   s->amg.setSolverParameters(s->sp[0], s->sp[1], s->sp[2], s->sp[3], s->sp[4], s->rp[0], s->rp[1], s->rp[2]);
   s->amg.m_imin = s->sp[5];
   for (int l = 0; l < s->nlev(); l++) {
@@ -303,7 +304,7 @@ int amrmg_solver_get_field(amrmg_solver* s, int field, double* h) {
 int amrmg_solver_solve(amrmg_solver* s, int zero_phi, double* hist, int hist_cap, int* info) {
   API_BEGIN
   auto ph = s->ptrs(s->phi), rh = s->ptrs(s->rhs), ur = s->ptrs(s->uberRes), uc = s->ptrs(s->uberCorr);
-  long long b0 = s->mixed ? s->mixed->bottom.m_bicg.m_total_iters : s->bottom.m_total_iters;
+  long long b0 = s->mixed ? s->mixed->bottomIterations() : s->bottom->totalIterations();
   s->amg.solveNoInitResid(ph, ur, uc, rh, s->nlev() - 1, 0, zero_phi != 0);
   s->ctx->c->sync();
   if (hist)
@@ -311,7 +312,7 @@ int amrmg_solver_solve(amrmg_solver* s, int zero_phi, double* hist, int hist_cap
   if (info) {
     info[0] = s->amg.m_iterations;
     info[1] = s->amg.m_exitStatus;
-    info[2] = (int)((s->mixed ? s->mixed->bottom.m_bicg.m_total_iters : s->bottom.m_total_iters) - b0);
+    info[2] = (int)((s->mixed ? s->mixed->bottomIterations() : s->bottom->totalIterations()) - b0);
   }
   API_END
 }

@@ -727,6 +727,21 @@ void incr(const DeviceLayout& L, T* lhs, const T* x, double scale) {
   LAUNCH(k_incr<T>, L.ntiles, L.d_boxes, L.d_tiles, geom(L), lhs, x, T(scale));
 }
 template <class T>
+__global__ void __launch_bounds__(256) k_incr_assign(const DBox* __restrict__ boxes, const Tile* __restrict__ tiles,
+                                                     TileGeom g, T* lhs, T* x) {
+  DBox b;
+  int i, j, k;
+  if (!tile_cell(boxes, tiles, g, b, i, j, k)) return;
+  const long long c = cidx(b, i, j, k);
+  const T v = lhs[c] + x[c] * T(1.0);
+  lhs[c] = v;
+  x[c] = v;
+}
+template <class T>
+void incr_assign(const DeviceLayout& L, T* lhs, T* x) {
+  LAUNCH(k_incr_assign<T>, L.ntiles, L.d_boxes, L.d_tiles, geom(L), lhs, x);
+}
+template <class T>
 __global__ void __launch_bounds__(256) k_copy_valid(const DBox* __restrict__ boxes, const Tile* __restrict__ tiles,
                                                     TileGeom g, T* dst, const T* src) {
   DBox b;
@@ -1147,6 +1162,7 @@ void reflux(Context* ctx, const RefluxCell* cells, int ncells, const RefluxCoars
   template void prolong<T>(const DeviceLayout&, const DeviceLayout&, T*, const T*);                            \
   template void average<T>(const DeviceLayout&, const DeviceLayout&, T*, const T*);                            \
   template void incr<T>(const DeviceLayout&, T*, const T*, double);                                            \
+  template void incr_assign<T>(const DeviceLayout&, T*, T*);                                                   \
   template void axby<T>(const DeviceLayout&, T*, const T*, const T*, double, double);                          \
   template void scale<T>(const DeviceLayout&, T*, double);                                                     \
   template void setval<T>(const DeviceLayout&, T*, double);                                                    \

@@ -13,6 +13,7 @@
 // addition beside its FP64 path.
 #pragma once
 #include "amrpoissonop.hpp"
+#include "batch.hpp"
 
 namespace amrmg {
 
@@ -53,7 +54,9 @@ class PromotedBottomSolver : public LinearSolver<LevelData<float>> {
 struct MixedInner {
   AMRPoissonOpFactory<float> fac;
   AMRMultiGrid<float> amg;
-  PromotedBottomSolver bottom;
+  PromotedBottomSolver bottom;                       // host-driven FP64 BiCGStab
+  std::unique_ptr<DeviceBottomSolver<float>> devBottom;  // the same on the device, when the level allows
+  LinearSolver<LevelData<float>>* active = nullptr;
   std::vector<std::unique_ptr<LevelData<float>>> res, corr;
 
   // amg64 must be defined and initialised on the same DeviceHierarchy
@@ -62,6 +65,14 @@ struct MixedInner {
     MGLevelOp<LevelData<double>>* bottomOp64 = amg64.m_mg0->m_op.back();
     bottom.bind(bottomOp64, H->mgDepth() > 1 ? H->mgLay.back().get() : H->amrLay[0].get());
     amg.define(fac, &bottom, H->nlev());
+    active = &bottom;
+    devBottom.reset(new DeviceBottomSolver<float>(H->ctx));
+    devBottom->define(amg.m_mg0->m_op.back(), true);
+    if (devBottom->onDevice()) {
+      active = devBottom.get();
+      amg.m_bottomSolver = active;
+      amg.m_mg0->setBottomSolver(active);
+    }
     for (int l = 0; l < H->nlev(); l++) {
       res.emplace_back(new LevelData<float>(H->amrLay[l].get()));
       corr.emplace_back(new LevelData<float>(H->amrLay[l].get()));
@@ -71,7 +82,10 @@ struct MixedInner {
     amg.init(c, r, H->nlev() - 1, 0);
     amg64.m_innerCycle = [this](std::vector<LevelData<double>*>& uc, std::vector<LevelData<double>*>& ur, int l_max,
                                 int l_base) { cycle(uc, ur, l_max, l_base); };
-    amg64.m_innerBottomMetrics = [this](double metric, double tol) { bottom.setConvergenceMetrics(metric, tol); };
+    amg64.m_innerBottomMetrics = [this](double metric, double tol) { active->setConvergenceMetrics(metric, tol); };
+  }
+  long long bottomIterations() {
+    return active == &bottom ? bottom.m_bicg.m_total_iters : devBottom->totalIterations();
   }
   void setSolverParameters(int pre, int post, int nbottom, int numMG, int iterMax, double eps, double hang,
                            double normThresh) {

@@ -82,6 +82,9 @@ void average(const DeviceLayout& Lf, const DeviceLayout& Lc, T* coarse, const T*
 
 template <class T>
 void incr(const DeviceLayout& L, T* lhs, const T* x, double scale);
+// lhs = lhs + x * 1.0; x = lhs (valid cells)
+template <class T>
+void incr_assign(const DeviceLayout& L, T* lhs, T* x);
 template <class T>
 void axby(const DeviceLayout& L, T* lhs, const T* x, const T* y, double a, double b);
 template <class T>

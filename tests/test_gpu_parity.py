@@ -260,3 +260,29 @@ def test_relax_tile_shapes_and_tuner_bitwise(gpu_ctx):
     hist, info = g.solve()
     np.testing.assert_array_equal(g.get_field(g.PHI), ref_phi)
     g.close()
+
+
+def test_set_alpha_beta_equals_fresh_solver(gpu_ctx):
+    """AMRPoissonOp::setAlphaAndBeta on a solver that has already solved gives the
+    same solve, bit for bit, as a solver defined with the new coefficients
+    (operators at every level, the relaxation and the device bottom solver)."""
+    h1 = small_single(n=32, box=16, alpha=0.4, beta=1.3)
+    h2 = small_single(n=32, box=16, alpha=0.9, beta=0.6)
+    rhs = H.sinusoid_rhs(h1, 1010)
+    sp, rp = H.solver_params()
+    g = amrmg.AMRMultiGrid(gpu_ctx, h1)
+    g.setSolverParameters(sp, rp)
+    g.set_field(g.RHS, rhs)
+    g.solve()
+    g.set_alpha_beta(0.9, 0.6)
+    hist_a, info_a = g.solve()
+    phi_a = g.get_field(g.PHI)
+    g.close()
+    f = amrmg.AMRMultiGrid(gpu_ctx, h2)
+    f.setSolverParameters(sp, rp)
+    f.set_field(f.RHS, rhs)
+    hist_b, info_b = f.solve()
+    assert info_a == info_b
+    np.testing.assert_array_equal(hist_a, hist_b)
+    np.testing.assert_array_equal(phi_a, f.get_field(f.PHI))
+    f.close()
