@@ -214,209 +214,495 @@ struct BottomArgs {
                                // whole bottom of MultiGrid::cycle); 0: start from the correction
 };
 
+// A walk over the cells of a box of extents (ea, eb, *), a fastest, in steps of
+// `step` cells: the position (a, b, c) and its element offset are advanced with
+// adds and compares only (no division per cell).
+struct Walk {
+  int a, b, c, off;
+  int da, db, dc, doff, ea, eb, ca, cb;
+  __device__ void init(int n, int step, int ea_, int eb_, int base, int s0, int s1, int s2) {
+    ea = ea_;
+    eb = eb_;
+    int r = n / ea;
+    a = n - r * ea;
+    b = r % eb;
+    c = r / eb;
+    r = step / ea;
+    da = step - r * ea;
+    db = r % eb;
+    dc = r / eb;
+    off = base + a * s0 + b * s1 + c * s2;
+    doff = da * s0 + db * s1 + dc * s2;
+    ca = s1 - ea * s0;
+    cb = s2 - eb * s1;
+  }
+  __device__ void next() {
+    a += da;
+    b += db;
+    c += dc;
+    off += doff;
+    if (a >= ea) { a -= ea; b++; off += ca; }
+    if (b >= eb) { b -= eb; c++; off += cb; }
+  }
+};
+
+// The bottom level of one problem, swept by one workgroup.  Fields are addressed
+// by offsets from the base of the level's ghosted rectangle R (R.off = 0).  A sweep
+// gives each thread U cells per pass and issues all their loads (at always valid
+// offsets) before any store, so the loads of a pass are in flight together; the
+// cells of a pass are distinct, and cells of one GSRB colour read only the other
+// colour, so the results are those of a cell-by-cell loop.
 struct BottomLevel {
   BatchCoef cf;
+  DBox R;
   int tid, nt;
-  long long ncell;
-  int lo[3], nx, ny;
-  __device__ void cell(long long q, int& i, int& j, int& k) const {
-    i = lo[0] + (int)(q % nx);
-    j = lo[1] + (int)((q / nx) % ny);
-    k = lo[2] + (int)(q / ((long long)nx * ny));
+  int ncell;
+  int nx, ny, nz;
+  int off0;  // offset of cell lo
+  template <class V>
+  struct Nb {
+    V p, xp, xm, yp, ym, zp, zm, r;
+  };
+  // the six neighbours with the homogeneous BC ghosts of the rectangle formed
+  // from the cell itself (the fill-then-read of the reference)
+  template <class V>
+  __device__ void bc(Nb<V>& s, int i, int j, int k) const {
+    if (i == R.hi[0]) s.xp = bc_ghost<V>(cf, 0, 1, s.p, 1);
+    if (i == R.lo[0]) s.xm = bc_ghost<V>(cf, 0, 0, s.p, 1);
+    if (j == R.hi[1]) s.yp = bc_ghost<V>(cf, 1, 1, s.p, 1);
+    if (j == R.lo[1]) s.ym = bc_ghost<V>(cf, 1, 0, s.p, 1);
+    if (k == R.hi[2]) s.zp = bc_ghost<V>(cf, 2, 1, s.p, 1);
+    if (k == R.lo[2]) s.zm = bc_ghost<V>(cf, 2, 0, s.p, 1);
   }
-  // one GSRB colour (GSRBLAPLACIAN / GSRBHELMHOLTZ) with the homogeneous BC ghosts,
-  // on a field addressed through R
-  template <class U>
-  __device__ void gsrb(const DBox& R, U* phi, const U* rhs, int color) const {
-    const U dxinv = U(cf.dxinv), lambda = U(cf.lambda), alpha = U(cf.alpha), beta = U(cf.beta);
-    for (long long q = tid; q < ncell; q += nt) {
-      int i, j, k;
-      cell(q, i, j, k);
-      if ((i + j + k + color) & 1) continue;
-      const long long c = ridx(R, i, j, k);
-      const Six<U> s = six<U>(phi, R, cf, c, i, j, k, 1);
-      const U s6 = s.xp + s.xm + s.yp + s.ym + s.zp + s.zm;
-      if (cf.lap) {
-        const U lphi = s6 * dxinv;
-        phi[c] = lambda * (rhs[c] - lphi);
-      } else {
-        const U p = phi[c];
-        const U lphi = (s6 - U(6.0) * p) * dxinv;
-        const U helmop = alpha * p + beta * lphi;
-        phi[c] = p + lambda * (helmop - rhs[c]);
+  template <class V>
+  __device__ Nb<V> load_nb(const V* phi, const V* rhs, int c) const {
+    Nb<V> s;
+    s.p = phi[c];
+    s.xp = phi[c + 1];
+    s.xm = phi[c - 1];
+    s.yp = phi[c + R.sy];
+    s.ym = phi[c - R.sy];
+    s.zp = phi[c + R.sz];
+    s.zm = phi[c - R.sz];
+    s.r = rhs ? rhs[c] : V(0);
+    return s;
+  }
+  // one GSRB colour (GSRBLAPLACIAN / GSRBHELMHOLTZ), cells of the colour walked
+  // over half-rows of (nx + 1) / 2 candidates; with src, phi is not read: every
+  // value of phi is taken as src * mult (the preconditioner's phi = rhs * mult
+  // folded into its first colour), and the cells of the other colour are written
+  // with that value
+  template <int U, class V>
+  __device__ void gsrb(V* phi, const V* rhs, int color, const V* src = nullptr, V mult = V(0)) const {
+    const V dxinv = V(cf.dxinv), lambda = V(cf.lambda), alpha = V(cf.alpha), beta = V(cf.beta);
+    const int nxh = (nx + 1) / 2, tot = nxh * ny * nz;
+    const int par0 = R.lo[0] + R.lo[1] + R.lo[2] + color;
+    Walk w;
+    w.init(tid, nt, nxh, ny, off0, 2, R.sy, R.sz);
+    for (int q0 = tid; q0 < tot; q0 += U * nt) {
+      Nb<V> s[U];
+      int c[U], ii[U], jj[U], kk[U];
+      bool ok[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int par = (par0 + w.b + w.c) & 1;
+        ii[u] = R.lo[0] + par + 2 * w.a;
+        jj[u] = R.lo[1] + w.b;
+        kk[u] = R.lo[2] + w.c;
+        ok[u] = w.c < nz && ii[u] <= R.hi[0];
+        c[u] = ok[u] ? w.off + par : off0;
+        s[u] = load_nb<V>(src ? src : phi, rhs, c[u]);
+        w.next();
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        if (!ok[u]) continue;
+        Nb<V> v = s[u];
+        if (src) {
+          v.p = v.p * mult; v.xp = v.xp * mult; v.xm = v.xm * mult; v.yp = v.yp * mult;
+          v.ym = v.ym * mult; v.zp = v.zp * mult; v.zm = v.zm * mult;
+        }
+        bc(v, ii[u], jj[u], kk[u]);
+        const V s6 = v.xp + v.xm + v.yp + v.ym + v.zp + v.zm;
+        V out;
+        if (cf.lap) {
+          const V lphi = s6 * dxinv;
+          out = lambda * (v.r - lphi);
+        } else {
+          const V lphi = (s6 - V(6.0) * v.p) * dxinv;
+          const V helmop = alpha * v.p + beta * lphi;
+          out = v.p + lambda * (helmop - v.r);
+        }
+        phi[c[u]] = out;
+      }
+    }
+    if (src) {
+      // the other colour: phi = src * mult
+      const int par1 = par0 + 1;
+      w.init(tid, nt, nxh, ny, off0, 2, R.sy, R.sz);
+      for (int q0 = tid; q0 < tot; q0 += U * nt) {
+        V s[U];
+        int c[U];
+        bool ok[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const int par = (par1 + w.b + w.c) & 1;
+          ok[u] = w.c < nz && par + 2 * w.a < nx;
+          c[u] = ok[u] ? w.off + par : off0;
+          s[u] = src[c[u]];
+          w.next();
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+          if (ok[u]) phi[c[u]] = s[u] * mult;
       }
     }
     __syncthreads();
   }
-  template <class U>
-  __device__ void relax(const DBox& R, U* phi, const U* rhs, int n) const {
+  template <int U, class V>
+  __device__ void relax(V* phi, const V* rhs, int n) const {
     for (int it = 0; it < n; it++) {
-      gsrb(R, phi, rhs, 0);
-      gsrb(R, phi, rhs, 1);
+      gsrb<U>(phi, rhs, 0);
+      gsrb<U>(phi, rhs, 1);
     }
+  }
+  // every valid cell: ld(c) (loads only) for U cells, then st(c, i, j, k, loaded)
+  template <int U, class Ld, class St>
+  __device__ void sweep(Ld ld, St st) const {
+    using V = decltype(ld(0));
+    Walk w;
+    w.init(tid, nt, nx, ny, off0, 1, R.sy, R.sz);
+    for (int q0 = tid; q0 < ncell; q0 += U * nt) {
+      V v[U];
+      int c[U], ii[U], jj[U], kk[U];
+      bool ok[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        ok[u] = w.c < nz;
+        c[u] = ok[u] ? w.off : off0;
+        ii[u] = R.lo[0] + w.a;
+        jj[u] = R.lo[1] + w.b;
+        kk[u] = R.lo[2] + w.c;
+        v[u] = ld(c[u]);
+        w.next();
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        if (ok[u]) st(c[u], ii[u], jj[u], kk[u], v[u]);
+    }
+    __syncthreads();
   }
 };
 
-__device__ double block_sum_serial_boxes(double* sh, const DBox* boxes, int nbox, const double* a, const double* b,
-                                         int tid) {
-  // FArrayBox::dotProduct per box (ForAllThisCBNNXC, Fortran order), boxes in order
-  for (int q = tid; q < nbox; q += blockDim.x) {
-    const DBox bx = boxes[q];
-    double dot = 0.0;
-    for (int k = bx.lo[2]; k <= bx.hi[2]; k++)
-      for (int j = bx.lo[1]; j <= bx.hi[1]; j++)
-        for (int i = bx.lo[0]; i <= bx.hi[0]; i++) {
-          const long long c = ridx(bx, i, j, k);
-          dot += a[c] * b[c];
-        }
-    sh[q] = dot;
+// One chain of a reduction: len terms at the offsets of a walk over extents
+// (ea, eb, *) with strides (s0, s1, s2) from base.
+struct Chain {
+  int len, base, ea, eb, s0, s1, s2;
+};
+
+// Serial sums of nch chains (nch <= blockDim.x): chain c's terms (x[o] * y[o] at
+// its offsets o, or x[o] * x[o] with SQUARE) are added in order (with firstAssign
+// the first term is assigned, otherwise added to 0.0), exactly as a serial loop
+// adds them.  The terms of a stage are formed by the whole workgroup into LDS (the
+// blockDim.x / nch threads of chain c walk it, kFillSteps terms each per stage, all
+// loads issued together; term n of the stage at c * cst + n, cst odd so that the
+// chains start in different banks); then the owner of each chain (thread c) adds
+// its part of the stage in order.  Leaves chain c's sum in out[c].  The stage
+// holds kFillSteps * blockDim.x + nch terms.
+constexpr int kFillSteps = 14;
+template <bool SQUARE, class Geo, class Ptrs>
+__device__ void chain_sums(double* stage, double* out, int nch, int maxlen, bool firstAssign, Geo geo, Ptrs ptrs) {
+  constexpr int K = 16, S = kFillSteps;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int per = nt / nch;  // walkers per chain
+  const int seg = S * per;   // terms per chain and stage
+  const int cst = seg | 1;
+  const int myc = tid % nch;
+  const bool filler = tid < per * nch;
+  const Chain g = geo(myc);
+  const double *px, *py;
+  ptrs(myc, px, py);
+  Walk w;
+  w.init(tid / nch, per, g.ea, g.eb, g.base, g.s0, g.s1, g.s2);
+  int n = tid / nch;  // term index of the walk
+  const int ownlen = tid < nch ? geo(tid).len : 0;
+  double acc = 0.0;
+  for (int c0 = 0; c0 < maxlen; c0 += seg) {
+    if (filler) {
+      double x[S], y[S];
+      int slot[S];
+#pragma unroll
+      for (int u = 0; u < S; u++) {
+        const int o = n < g.len ? w.off : g.base;
+        x[u] = px[o];
+        if (!SQUARE) y[u] = py[o];
+        slot[u] = myc * cst + (n - c0);  // terms past the chain's end land in slots nobody reads
+        w.next();
+        n += per;
+      }
+#pragma unroll
+      for (int u = 0; u < S; u++) stage[slot[u]] = SQUARE ? x[u] * x[u] : x[u] * y[u];
+    }
+    __syncthreads();
+    if (tid < nch) {
+      const double* st = stage + tid * cst;
+      const int lim = min(seg, ownlen - c0);
+      int m = 0;
+      if (firstAssign && c0 == 0 && lim > 0) acc = st[m++];
+      for (; m + 2 * K <= lim; m += 2 * K) {
+        double a[K], b[K];
+#pragma unroll
+        for (int u = 0; u < K; u++) a[u] = st[m + u];
+#pragma unroll
+        for (int u = 0; u < K; u++) b[u] = st[m + K + u];
+#pragma unroll
+        for (int u = 0; u < K; u++) acc += a[u];
+#pragma unroll
+        for (int u = 0; u < K; u++) acc += b[u];
+      }
+      for (; m < lim; m++) acc += st[m];
+    }
+    __syncthreads();
   }
+  if (tid < nch) out[tid] = acc;
   __syncthreads();
-  __shared__ double res;
-  if (tid == 0) {
-    double val = 0.0;
-    for (int q = 0; q < nbox; q++) val += sh[q];
-    res = val;
-  }
-  __syncthreads();
-  const double r = res;
-  __syncthreads();
-  return r;
 }
 
-__device__ double block_norm2_boxes(double* sh, const DBox* boxes, int nbox, const double* x, int tid) {
-  // FArrayBox::sumPow(2) per box: column sums over (k, j), columns summed in i
-  // order; boxes summed in order; then sqrt (norm(.., 2))
-  __shared__ double normAcc;
-  if (tid == 0) normAcc = 0.0;
-  __syncthreads();
-  for (int q = 0; q < nbox; q++) {
-    const DBox bx = boxes[q];
-    const int nx = bx.hi[0] - bx.lo[0] + 1;
-    for (int i = tid; i < nx; i += blockDim.x) {
-      double t = 0.0;
-      bool first = true;
-      for (int k = bx.lo[2]; k <= bx.hi[2]; k++)
-        for (int j = bx.lo[1]; j <= bx.hi[1]; j++) {
-          const double r = x[ridx(bx, bx.lo[0] + i, j, k)];
-          if (first) { t = r * r; first = false; }
-          else t += r * r;
-        }
-      sh[i] = t;
+// the bottom level's boxes (offsets from the rectangle's base), the first
+// kBoxCache of them in LDS
+constexpr int kBoxCache = 64;
+struct BoxSet {
+  const DBox* g;    // all boxes (global)
+  const DBox* s;    // the first kBoxCache of them (LDS)
+  int n;
+  short2* colChain; // norm chains: (box, column) per chain (LDS)
+  int ncol;         // number of norm chains
+  int maxCells, maxCol;
+  __device__ DBox get(int q) const { return q < kBoxCache ? s[q] : g[q]; }
+};
+
+// FArrayBox::dotProduct per box (ForAllThisCBNNXC: Fortran order, serial from
+// 0.0), the box results added in layout order from 0.0.  NPAIR dot products
+// (a[m], b[m]) are formed in one pass, each with its own chains.
+template <int NPAIR>
+__device__ void block_dots(double* stage, double* sh, const BoxSet& B, const double* const (&a)[NPAIR],
+                           const double* const (&b)[NPAIR], double (&res)[NPAIR]) {
+  chain_sums<false>(
+      stage, sh, NPAIR * B.n, B.maxCells, false,
+      [&](int c) {
+        const DBox bx = B.get(c % B.n);
+        const int nx = bx.hi[0] - bx.lo[0] + 1, ny = bx.hi[1] - bx.lo[1] + 1, nz = bx.hi[2] - bx.lo[2] + 1;
+        return Chain{nx * ny * nz, (int)bx.off + 1 + bx.sy + bx.sz, nx, ny, 1, bx.sy, bx.sz};
+      },
+      [&](int c, const double*& x, const double*& y) {
+        const int m = c / B.n;
+        x = a[0];
+        y = b[0];
+#pragma unroll
+        for (int q = 1; q < NPAIR; q++)
+          if (m == q) { x = a[q]; y = b[q]; }
+      });
+  __shared__ double sres[NPAIR];
+  if (threadIdx.x == 0) {
+    for (int m = 0; m < NPAIR; m++) {
+      double val = 0.0;
+      for (int q = 0; q < B.n; q++) val += sh[m * B.n + q];
+      sres[m] = val;
     }
-    __syncthreads();
-    if (tid == 0) {
-      double sum = sh[0];
-      for (int i = 1; i < nx; i++) sum += sh[i];
-      normAcc += sum;
-    }
-    __syncthreads();
   }
+  __syncthreads();
+  for (int m = 0; m < NPAIR; m++) res[m] = sres[m];
+  __syncthreads();
+}
+
+// FArrayBox::sumPow(2) per box: column sums over (k, j) (the first square assigned),
+// the columns added in i order; boxes added in order from 0.0; then sqrt
+// (norm(.., 2)).  Each (box, column) is one chain.
+__device__ double block_norm2_boxes(double* stage, double* sh, const BoxSet& B, const double* x) {
+  chain_sums<true>(
+      stage, sh, B.ncol, B.maxCol, true,
+      [&](int c) {
+        const short2 qc = B.colChain[c];
+        const DBox bx = B.get(qc.x);
+        const int ny = bx.hi[1] - bx.lo[1] + 1, nz = bx.hi[2] - bx.lo[2] + 1;
+        return Chain{ny * nz, (int)bx.off + 1 + qc.y + bx.sy + bx.sz, ny, nz, bx.sy, bx.sz, 0};
+      },
+      [&](int, const double*& px, const double*& py) {
+        px = x;
+        py = x;
+      });
+  __shared__ double normAcc;
+  if (threadIdx.x == 0) {
+    double acc = 0.0;
+    int c = 0;
+    for (int q = 0; q < B.n; q++) {
+      const DBox bx = B.get(q);
+      const int nx = bx.hi[0] - bx.lo[0] + 1;
+      double sum = sh[c];
+      for (int i = 1; i < nx; i++) sum += sh[c + i];
+      c += nx;
+      acc += sum;
+    }
+    normAcc = acc;
+  }
+  __syncthreads();
   const double n = normAcc;
   __syncthreads();
   return std::sqrt(n);
 }
 
+constexpr int kBottomThreads = 512;  // one workgroup per bottom solve
+constexpr int kStage = kFillSteps * kBottomThreads + kBottomThreads;
+constexpr int kUS = 4, kUE = 8;      // cells per thread and pass: stencil / elementwise sweeps
 template <class T>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kBottomThreads)
     k_batch_bottom(const BatchBottom* __restrict__ bots, const BatchCoef* __restrict__ bcoef,
                    const DBox* __restrict__ bboxes, T* const* __restrict__ corr, const T* const* __restrict__ res,
                    double* __restrict__ scratch, const long long* __restrict__ soff, const double* __restrict__ metric,
                    const int* __restrict__ active, int* __restrict__ stats, BottomArgs ba) {
   const int p = blockIdx.x;
   if (!active[p]) return;
-  __shared__ double sh[1024];
+  __shared__ double sh[kBottomThreads];
+  __shared__ double stage[kStage];
+  __shared__ DBox sbox[kBoxCache];
+  __shared__ short2 colChain[kBottomThreads];
+  __shared__ int smax[2];
   const BatchBottom bt = bots[p];
-  const DBox R = bt.rect;  // the level's fields in its depth's slab
-  DBox Rs = R;             // the same shape in this problem's scratch vectors
-  Rs.off = 15;
+  const int tid = threadIdx.x;
+  // fields are addressed from the base of the level's ghosted rectangle
   BottomLevel L;
   L.cf = bcoef[p];
-  L.tid = threadIdx.x;
+  L.R = bt.rect;
+  L.R.off = 0;
+  L.tid = tid;
   L.nt = blockDim.x;
-  for (int q = 0; q < 3; q++) L.lo[q] = R.lo[q];
-  L.nx = R.hi[0] - R.lo[0] + 1;
-  L.ny = R.hi[1] - R.lo[1] + 1;
-  L.ncell = bt.cells;
-  const DBox* boxes = bboxes + bt.box0;  // addressed like Rs
-  const int nbox = bt.nbox;
-  const int tid = threadIdx.x;
-  T* e32 = corr[p];
-  const T* r32 = res[p];
+  L.nx = L.R.hi[0] - L.R.lo[0] + 1;
+  L.ny = L.R.hi[1] - L.R.lo[1] + 1;
+  L.nz = L.R.hi[2] - L.R.lo[2] + 1;
+  L.ncell = L.nx * L.ny * L.nz;
+  L.off0 = 1 + L.R.sy + L.R.sz;
+  BoxSet B;
+  B.g = bboxes + bt.box0;
+  B.s = sbox;
+  B.n = bt.nbox;
+  B.colChain = colChain;
+  for (int q = tid; q < min(B.n, kBoxCache); q += blockDim.x) sbox[q] = B.g[q];
+  if (tid == 0) {
+    int c = 0, mc = 0, mcol = 0;
+    for (int q = 0; q < B.n; q++) {
+      const DBox bx = B.g[q];
+      const int nx = bx.hi[0] - bx.lo[0] + 1, nyz = (bx.hi[1] - bx.lo[1] + 1) * (bx.hi[2] - bx.lo[2] + 1);
+      for (int i = 0; i < nx; i++) colChain[c++] = make_short2((short)q, (short)i);
+      mc = max(mc, nx * nyz);
+      mcol = max(mcol, nyz);
+    }
+    smax[0] = mc;
+    smax[1] = mcol;
+  }
+  __syncthreads();
+  B.maxCells = smax[0];
+  B.maxCol = smax[1];
+  B.ncol = 0;
+  for (int q = 0; q < B.n; q++) B.ncol += B.get(q).hi[0] - B.get(q).lo[0] + 1;
+  T* e32 = corr[p] + bt.rect.off;
+  const T* r32 = res[p] + bt.rect.off;
   if (ba.relaxFirst) {
     // MultiGrid::cycle at the bottom: the correction is zero on entry
-    for (long long q = tid; q < L.ncell; q += L.nt) {
-      int i, j, k;
-      L.cell(q, i, j, k);
-      e32[ridx(R, i, j, k)] = T(0.0);
-    }
-    __syncthreads();
-    L.relax(R, e32, r32, ba.nbottom);
+    L.sweep<kUE>([&](int) { return 0; }, [&](int c, int, int, int, int) { e32[c] = T(0.0); });
+    L.relax<kUS>(e32, r32, ba.nbottom);
     if (L.ncell == 1) return;  // m_bottomCells == 1: relax only
   }
-  const long long vs = 15 + (long long)Rs.sz * (R.hi[2] - R.lo[2] + 3);  // elements per scratch vector
-  double* base = scratch + soff[p];
+  const long long vs = 15 + (long long)L.R.sz * (L.nz + 2);  // elements per scratch vector
+  double* base = scratch + soff[p] + 15;                      // rectangle base of the first
   double *x = base, *b = base + vs, *r = base + 2 * vs, *r_tilde = base + 3 * vs, *e = base + 4 * vs,
          *pp = base + 5 * vs, *p_tilde = base + 6 * vs, *s_tilde = base + 7 * vs, *t = base + 8 * vs,
          *v = base + 9 * vs;
-  auto each = [&](auto fn) {
-    for (long long q = tid; q < L.ncell; q += L.nt) {
-      int i, j, k;
-      L.cell(q, i, j, k);
-      fn(ridx(Rs, i, j, k), i, j, k);
-    }
-    __syncthreads();
+  struct D2 {
+    double a, b;
   };
   // PromotedBottomSolver: widen the correction and the residual
-  for (long long q = tid; q < L.ncell; q += L.nt) {
-    int i, j, k;
-    L.cell(q, i, j, k);
-    x[ridx(Rs, i, j, k)] = (double)e32[ridx(R, i, j, k)];
-    b[ridx(Rs, i, j, k)] = (double)r32[ridx(R, i, j, k)];
-  }
-  __syncthreads();
+  L.sweep<kUE>(
+      [&](int c) { return D2{(double)e32[c], (double)r32[c]}; },
+      [&](int c, int, int, int, D2 w) {
+        x[c] = w.a;
+        b[c] = w.b;
+      });
   const BatchCoef cf = L.cf;
   // residual / applyOp of AMRPoissonOp on this level, homogeneous BC
   auto residual = [&](double* out, const double* phi, const double* rhs) {
-    each([&](long long c, int i, int j, int k) {
-      const double pv = phi[c];
-      const Six<double> s6v = six<double>(phi, Rs, cf, c, i, j, k, 1);
-      const double lap = (s6v.xp + s6v.xm + s6v.yp + s6v.ym + s6v.zp + s6v.zm - 6.0 * pv) * cf.dxinv;
-      out[c] = -(cf.alpha * pv) - cf.beta * lap + rhs[c];
-    });
+    L.sweep<kUS>(
+        [&](int c) { return L.load_nb<double>(phi, rhs, c); },
+        [&](int c, int i, int j, int k, BottomLevel::Nb<double> s) {
+          L.bc(s, i, j, k);
+          const double lap = (s.xp + s.xm + s.yp + s.ym + s.zp + s.zm - 6.0 * s.p) * cf.dxinv;
+          out[c] = -(cf.alpha * s.p) - cf.beta * lap + s.r;
+        });
   };
+  // zero(out) followed by applyOp(out, phi): applyOp writes every valid cell
   auto applyOp = [&](double* out, const double* phi) {
-    each([&](long long c, int i, int j, int k) {
-      const double pv = phi[c];
-      const Six<double> s6v = six<double>(phi, Rs, cf, c, i, j, k, 1);
-      const double lap = (s6v.xp + s6v.xm + s6v.yp + s6v.ym + s6v.zp + s6v.zm - 6.0 * pv) * cf.dxinv;
-      out[c] = cf.alpha * pv + cf.beta * lap;
-    });
+    L.sweep<kUS>(
+        [&](int c) { return L.load_nb<double>(phi, nullptr, c); },
+        [&](int c, int i, int j, int k, BottomLevel::Nb<double> s) {
+          L.bc(s, i, j, k);
+          const double lap = (s.xp + s.xm + s.yp + s.ym + s.zp + s.zm - 6.0 * s.p) * cf.dxinv;
+          out[c] = cf.alpha * s.p + cf.beta * lap;
+        });
   };
-  // preCond (AMRPoissonOp.cpp:304-329): phi = rhs * mult, then relax(phi, rhs, 2)
+  // preCond (AMRPoissonOp.cpp:304-329): phi = rhs * mult, then relax(phi, rhs, 2);
+  // the scaling is folded into the first colour
   const double mult = 1.0 / (cf.alpha - 2.0 * 3 * cf.beta / (cf.dx * cf.dx));
   auto preCond = [&](double* cor, const double* rv) {
-    each([&](long long c, int, int, int) { cor[c] = rv[c] * mult; });
-    L.relax(Rs, cor, rv, 2);
+    L.gsrb<kUS>(cor, rv, 0, rv, mult);
+    L.gsrb<kUS>(cor, rv, 1);
+    L.relax<kUS>(cor, rv, 1);
   };
+  // lhs += xv * s
   auto incr = [&](double* lhs, const double* xv, double s) {
-    each([&](long long c, int, int, int) { lhs[c] = lhs[c] + xv[c] * s; });
+    L.sweep<kUE>(
+        [&](int c) { return D2{lhs[c], xv[c]}; },
+        [&](int c, int, int, int, D2 w) { lhs[c] = w.a + w.b * s; });
   };
-  auto assign = [&](double* lhs, const double* rv) { each([&](long long c, int, int, int) { lhs[c] = rv[c]; }); };
-  auto zero = [&](double* lhs) { each([&](long long c, int, int, int) { lhs[c] = 0.0; }); };
-  auto scale = [&](double* lhs, double s) { each([&](long long c, int, int, int) { lhs[c] = lhs[c] * s; }); };
-  auto dot = [&](const double* a, const double* bb) { return block_sum_serial_boxes(sh, boxes, nbox, a, bb, tid); };
-  auto norm = [&](const double* a) { return block_norm2_boxes(sh, boxes, nbox, a, tid); };
+  // lhs1 += x1 * s1 and lhs2 += x2 * s2 in one sweep
+  auto incr2 = [&](double* l1, const double* x1, double s1, double* l2, const double* x2, double s2) {
+    struct D4 {
+      double a, b, c, d;
+    };
+    L.sweep<kUS>(
+        [&](int c) { return D4{l1[c], x1[c], l2[c], x2[c]}; },
+        [&](int c, int, int, int, D4 w) {
+          l1[c] = w.a + w.b * s1;
+          l2[c] = w.c + w.d * s2;
+        });
+  };
+  auto assign = [&](double* lhs, const double* rv) {
+    L.sweep<kUE>(
+        [&](int c) { return rv[c]; }, [&](int c, int, int, int, double w) { lhs[c] = w; });
+  };
+  // setToZero of several vectors in one sweep
+  auto zero3 = [&](double* z1, double* z2, double* z3) {
+    L.sweep<kUE>(
+        [&](int) { return 0; },
+        [&](int c, int, int, int, int) {
+          z1[c] = 0.0;
+          z2[c] = 0.0;
+          z3[c] = 0.0;
+        });
+  };
+  auto dot = [&](const double* a1, const double* b1) {
+    const double* const aa[1] = {a1};
+    const double* const bb[1] = {b1};
+    double rr[1];
+    block_dots<1>(stage, sh, B, aa, bb, rr);
+    return rr[0];
+  };
+  auto norm = [&](const double* a) { return block_norm2_boxes(stage, sh, B, a); };
 
   const double eps = metric[2 * p + 1];
   const double convergenceMetric = metric[2 * p];
   int recount = 0;
-  zero(r);
-  residual(r, x, b);
+  residual(r, x, b);  // setToZero(r) then residual: every valid cell is written
   assign(r_tilde, r);
-  zero(e);
-  zero(p_tilde);
-  zero(s_tilde);
+  zero3(e, p_tilde, s_tilde);
   int iter = 0;
   double rho[4] = {0, 0, 0, 0}, nrm[2];
   nrm[0] = norm(r);
@@ -442,31 +728,40 @@ __global__ void __launch_bounds__(256)
       assign(pp, r);
       init = false;
     } else {
+      // scale(p, beta); incr(p, v, -beta*omega); incr(p, r, 1.0) in one sweep
       beta[1] = (rho[1] / rho[2]) * (alpha[1] / omega[1]);
-      scale(pp, beta[1]);
-      incr(pp, v, -beta[1] * omega[1]);
-      incr(pp, r, 1.0);
+      const double bw = -beta[1] * omega[1], bt1 = beta[1];
+      struct D3 {
+        double p, v, r;
+      };
+      L.sweep<kUS>(
+          [&](int c) { return D3{pp[c], v[c], r[c]}; },
+          [&](int c, int, int, int, D3 w) {
+            const double s1 = w.p * bt1;
+            const double s2 = s1 + w.v * bw;
+            pp[c] = s2 + w.r * 1.0;
+          });
     }
     preCond(p_tilde, pp);
-    zero(v);
     applyOp(v, p_tilde);
     const double m = dot(r_tilde, v);
     alpha[0] = rho[1] / m;
     if (fabs(m) > ba.small * fabs(rho[1])) {
-      incr(r, v, -alpha[0]);
+      incr2(r, v, -alpha[0], e, p_tilde, alpha[0]);
       nrm[0] = norm(r);
-      incr(e, p_tilde, alpha[0]);
     } else {
-      zero(r);
+      L.sweep<kUE>([&](int) { return 0; }, [&](int c, int, int, int, int) { r[c] = 0.0; });
       nrm[0] = 0.0;
     }
     if (nrm[0] > eps * initial_norm && nrm[0] > ba.reps * initial_residual) {
       preCond(s_tilde, r);
-      zero(t);
       applyOp(t, s_tilde);
-      omega[0] = dot(t, r) / dot(t, t);
-      incr(e, s_tilde, omega[0]);
-      incr(r, t, -omega[0]);
+      const double* const aa[2] = {t, t};
+      const double* const bb[2] = {r, t};
+      double tt[2];
+      block_dots<2>(stage, sh, B, aa, bb, tt);
+      omega[0] = tt[0] / tt[1];
+      incr2(e, s_tilde, omega[0], r, t, -omega[0]);
       nrm[0] = norm(r);
     }
     if (nrm[0] <= eps * initial_norm || nrm[0] <= ba.reps * initial_residual) break;  // exit status 1
@@ -485,7 +780,7 @@ __global__ void __launch_bounds__(256)
         rho[1] = 0.0; rho[2] = 0.0; rho[3] = 0.0;
         alpha[0] = 0; beta[0] = 0; omega[0] = 0;
         assign(r_tilde, r);
-        zero(e);
+        L.sweep<kUE>([&](int) { return 0; }, [&](int c, int, int, int, int) { e[c] = 0.0; });
         restarts++;
         init = true;
       }
@@ -494,11 +789,7 @@ __global__ void __launch_bounds__(256)
   if (!done) incr(x, e, 1.0);
   if (tid == 0) stats[p] += iter;
   // round the correction back into the V-cycle's precision
-  for (long long q = tid; q < L.ncell; q += L.nt) {
-    int i, j, k;
-    L.cell(q, i, j, k);
-    e32[ridx(R, i, j, k)] = (T)x[ridx(Rs, i, j, k)];
-  }
+  L.sweep<kUE>([&](int c) { return x[c]; }, [&](int c, int, int, int, double w) { e32[c] = (T)w; });
 }
 
 }  // namespace
@@ -636,12 +927,16 @@ BatchSolver::BatchSolver(Context* ctx, std::vector<BatchProblemSpec> problems, i
     b.box0 = (int)h_bbox.size();
     const DBL g = m_prob[p].boxes.coarsened(1 << db);
     b.nbox = g.size();
-    if (b.nbox > 1024) throw std::runtime_error("batch: too many bottom boxes");
+    // the bottom's reductions give each chain (box of a dot product, column of a
+    // norm) one thread: two dot products share a pass
+    if (2 * b.nbox > kBottomThreads) throw std::runtime_error("batch: too many bottom boxes");
+    long long widths = 0;
+    for (auto& bx : g.boxes) widths += bx.len(0);
+    if (widths > kBottomThreads) throw std::runtime_error("batch: the bottom boxes' widths add up to too many columns");
     for (auto& bx : g.boxes) {  // addressed in the problem's scratch vectors (rectangle at offset 15)
-      if (bx.len(0) > 1024) throw std::runtime_error("batch: bottom box wider than 1024");
       DBox x = b.rect;
       for (int q = 0; q < 3; q++) { x.lo[q] = bx.lo[q]; x.hi[q] = bx.hi[q]; }
-      x.off = 15 + (bx.lo[0] - b.rect.lo[0]) + (long long)(bx.lo[1] - b.rect.lo[1]) * b.rect.sy +
+      x.off = (bx.lo[0] - b.rect.lo[0]) + (long long)(bx.lo[1] - b.rect.lo[1]) * b.rect.sy +
               (long long)(bx.lo[2] - b.rect.lo[2]) * b.rect.sz;
       h_bbox.push_back(x);
     }
@@ -887,7 +1182,7 @@ void BatchSolver::vcycle() {
                  (T*)C.res, (const T*)F.corr, (const T*)F.res);
   }
   BottomArgs ba{80, 1e-12, 1e-8, 1e-30, 5, m_nbottom, 1};
-  BATCH_LAUNCH(k_batch_bottom<T>, m_np, 256, d_bottom, m_bcoef, d_bbox, (T* const*)d_corrPtr,
+  BATCH_LAUNCH(k_batch_bottom<T>, m_np, kBottomThreads, d_bottom, m_bcoef, d_bbox, (T* const*)d_corrPtr,
                (const T* const*)d_resPtr, d_bscratch, d_bscratchOff, d_metric, d_active, d_bstats, ba);
   for (int d = D - 2; d >= 0; d--) {
     const int np = m_depth[d + 1].np;
@@ -1079,13 +1374,17 @@ void DeviceBottomSolver<T>::define(LinearOp<LevelData<T>>* op, bool homogeneous)
   b.cells = L.valid_cells;
   b.box0 = 0;
   b.nbox = L.nbox;
-  if (b.nbox > 1024) return;
+  // one thread per reduction chain (two dot products share a pass; a norm has a
+  // chain per box column): otherwise the host-driven solve
+  if (2 * b.nbox > kBottomThreads) return;
+  long long widths = 0;
+  for (int q = 0; q < L.nbox; q++) widths += L.h_boxes[q].hi[0] - L.h_boxes[q].lo[0] + 1;
+  if (widths > kBottomThreads) return;
   std::vector<DBox> boxes;
   for (int q = 0; q < L.nbox; q++) {
     const DBox& x0 = L.h_boxes[q];
-    if (x0.hi[0] - x0.lo[0] + 1 > 1024) return;
     DBox x = x0;  // addressed in the scratch vectors (rectangle at offset 15)
-    x.off = 15 + (x0.lo[0] - b.rect.lo[0]) + (long long)(x0.lo[1] - b.rect.lo[1]) * b.rect.sy +
+    x.off = (x0.lo[0] - b.rect.lo[0]) + (long long)(x0.lo[1] - b.rect.lo[1]) * b.rect.sy +
             (long long)(x0.lo[2] - b.rect.lo[2]) * b.rect.sz;
     boxes.push_back(x);
   }
@@ -1158,7 +1457,7 @@ void DeviceBottomSolver<T>::solve(LevelData<T>& phi, const LevelData<T>& rhs) {
   }
   BottomArgs ba{m_fallback.m_imax, m_fallback.m_reps, m_fallback.m_hang, m_fallback.m_small,
                 m_fallback.m_numRestarts, 0, 0};
-  hipLaunchKernelGGL(k_batch_bottom<T>, dim3(1), dim3(256), 0, m_ctx->stream, (const BatchBottom*)d_bottom,
+  hipLaunchKernelGGL(k_batch_bottom<T>, dim3(1), dim3(kBottomThreads), 0, m_ctx->stream, (const BatchBottom*)d_bottom,
                      (const BatchCoef*)d_bcoef, (const DBox*)d_bbox, (T* const*)d_ptrs,
                      (const T* const*)((void**)d_ptrs + 1), d_scratch, (const long long*)d_soff,
                      (const double*)d_metric, (const int*)d_active, (int*)d_stats, ba);
