@@ -142,26 +142,48 @@ __global__ void __launch_bounds__(256)
 
 // OPERATORLAPRES (AMRPoissonOpF.ChF:341-377) with the inhomogeneous BC refresh, the
 // per-problem max-norm of the result, and the result stored in the V-cycle's
-// precision (computeAMRResidual of a single level, AMRMultiGrid.H:683-736)
+// precision (computeAMRResidual of a single level, AMRMultiGrid.H:683-736),
+// marching in z over a column tile of 64 x 4 cells and kz planes: the z neighbours
+// come from the previous and next planes already in registers; workgroups of
+// neighbouring tiles run on one XCD.
 template <class TO>
 __global__ void __launch_bounds__(256)
-    k_batch_residual_norm(const Tile* __restrict__ tiles, int lx, int ly, const DBox* __restrict__ rect,
-                          const BatchCoef* __restrict__ coef, const int* __restrict__ active, TO* __restrict__ res,
-                          const double* __restrict__ phi, const double* __restrict__ rhs, double* norms) {
-  const Tile t = tiles[blockIdx.x];
+    k_batch_residual_norm_z(const Tile* __restrict__ ctiles, int ntiles, int kz, const DBox* __restrict__ rect,
+                            const BatchCoef* __restrict__ coef, const int* __restrict__ active,
+                            TO* __restrict__ res, const double* __restrict__ phi, const double* __restrict__ rhs,
+                            double* norms) {
+  const int wg = ops::xcd_contiguous_block();
+  if (wg >= ntiles) return;
+  const Tile t = ctiles[wg];
   if (!active[t.box]) return;
   const DBox R = rect[t.box];
   const BatchCoef cf = coef[t.box];
-  int i, j, k;
+  const int i = t.i0 + (threadIdx.x & 63), j = t.j0 + (threadIdx.x >> 6);
+  const bool in = i <= R.hi[0] && j <= R.hi[1];
+  const int ic = min(i, R.hi[0]), jc = min(j, R.hi[1]);
+  const int kend = min(t.k0 + kz - 1, R.hi[2]);
+  const long long sy = R.sy, sz = R.sz;
+  long long c = ridx(R, ic, jc, t.k0);
+  double pm = phi[c - sz], p = phi[c];
   double v = 0.0;
-  if (btile_cell(t, R, lx, ly, i, j, k)) {
-    const long long c = ridx(R, i, j, k);
-    const double p = phi[c];
-    const Six<double> s = six<double>(phi, R, cf, c, i, j, k, 0);
-    const double lap = (s.xp + s.xm + s.yp + s.ym + s.zp + s.zm - 6.0 * p) * cf.dxinv;
-    const double r = -(cf.alpha * p) - cf.beta * lap + rhs[c];
-    res[c] = (TO)r;
-    v = fabs(r);
+  for (int k = t.k0; k <= kend; k++, c += sz) {
+    const double pz = phi[c + sz];  // plane k + 1 <= hi + 1 (the ghost plane)
+    double xp = phi[c + 1], xm = phi[c - 1], yp = phi[c + sy], ym = phi[c - sy];
+    const double rh = rhs[c];
+    if (ic == R.hi[0]) xp = bc_ghost<double>(cf, 0, 1, p, 0);
+    if (ic == R.lo[0]) xm = bc_ghost<double>(cf, 0, 0, p, 0);
+    if (jc == R.hi[1]) yp = bc_ghost<double>(cf, 1, 1, p, 0);
+    if (jc == R.lo[1]) ym = bc_ghost<double>(cf, 1, 0, p, 0);
+    const double zp = (k == R.hi[2]) ? bc_ghost<double>(cf, 2, 1, p, 0) : pz;
+    const double zm = (k == R.lo[2]) ? bc_ghost<double>(cf, 2, 0, p, 0) : pm;
+    const double lap = (xp + xm + yp + ym + zp + zm - 6.0 * p) * cf.dxinv;
+    const double r = -(cf.alpha * p) - cf.beta * lap + rh;
+    if (in) {
+      res[c] = (TO)r;
+      v = fmax(v, fabs(r));
+    }
+    pm = p;
+    p = pz;
   }
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
   __shared__ double sm[4];
@@ -174,22 +196,43 @@ __global__ void __launch_bounds__(256)
 }
 
 // phi += correction (incr(phi, uberCorrection, 1.0) of solveNoInitResid; a mixed
-// solve widens the FP32 correction first, as incr_f2d into a zero FP64 field)
+// solve widens the FP32 correction first, as incr_f2d into a zero FP64 field), over a
+// column tile of 64 x 4 cells and kz planes
 template <class T>
 __global__ void __launch_bounds__(256)
-    k_batch_update(const Tile* __restrict__ tiles, int lx, int ly, const DBox* __restrict__ rect,
-                   const int* __restrict__ active, double* __restrict__ phi, const T* __restrict__ corr) {
-  const Tile t = tiles[blockIdx.x];
+    k_batch_update_z(const Tile* __restrict__ ctiles, int ntiles, int kz, const DBox* __restrict__ rect,
+                     const int* __restrict__ active, double* __restrict__ phi, const T* __restrict__ corr) {
+  const int wg = ops::xcd_contiguous_block();
+  if (wg >= ntiles) return;
+  const Tile t = ctiles[wg];
   if (!active[t.box]) return;
   const DBox R = rect[t.box];
-  int i, j, k;
-  if (!btile_cell(t, R, lx, ly, i, j, k)) return;
-  const long long c = ridx(R, i, j, k);
-  if constexpr (std::is_same<T, double>::value) {
-    phi[c] = phi[c] + corr[c] * 1.0;
-  } else {
-    const double x = 0.0 + (double)corr[c];  // uberCorrection (zero) += widened correction
-    phi[c] = phi[c] + x * 1.0;
+  const int i = t.i0 + (threadIdx.x & 63), j = t.j0 + (threadIdx.x >> 6);
+  if (i > R.hi[0] || j > R.hi[1]) return;
+  const int kend = min(t.k0 + kz - 1, R.hi[2]);
+  long long c = ridx(R, i, j, t.k0);
+  const int n = kend - t.k0 + 1;
+  constexpr int U = 4;
+  for (int k0 = 0; k0 < n; k0 += U, c += U * (long long)R.sz) {
+    double pv[U];
+    T cv[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const long long cc = c + (long long)min(u, n - 1 - k0) * R.sz;
+      pv[u] = phi[cc];
+      cv[u] = corr[cc];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (k0 + u >= n) break;
+      const long long cc = c + (long long)u * R.sz;
+      if constexpr (std::is_same<T, double>::value) {
+        phi[cc] = pv[u] + cv[u] * 1.0;
+      } else {
+        const double x = 0.0 + (double)cv[u];  // uberCorrection (zero) += widened correction
+        phi[cc] = pv[u] + x * 1.0;
+      }
+    }
   }
 }
 
@@ -551,6 +594,7 @@ __device__ double block_norm2_boxes(double* stage, double* sh, const BoxSet& B, 
   return std::sqrt(n);
 }
 
+constexpr int kColumnPlanes = 8;     // planes of a column tile
 constexpr int kBottomThreads = 512;  // one workgroup per bottom solve
 constexpr int kStage = kFillSteps * kBottomThreads + kBottomThreads;
 constexpr int kUS = 4, kUE = 8;      // cells per thread and pass: stencil / elementwise sweeps
@@ -855,7 +899,7 @@ BatchSolver::BatchSolver(Context* ctx, std::vector<BatchProblemSpec> problems, i
     L.tileStart.assign(1, 0);
     L.wStart.assign(1, 0);
     L.cellStart.assign(1, 0);
-    std::vector<Tile> tiles;
+    std::vector<Tile> tiles, ctiles;
     for (int p = 0; p < L.np; p++) {
       const BatchProblemSpec& P = m_prob[p];
       const Box r = coarsen(P.domain, 1 << d);
@@ -887,6 +931,13 @@ BatchSolver::BatchSolver(Context* ctx, std::vector<BatchProblemSpec> problems, i
         for (int j = r.lo[1]; j <= r.hi[1]; j += 4)
           for (int i = r.lo[0]; i <= r.hi[0]; i += 64) tiles.push_back(Tile{p, i, j, k});
       L.tileStart.push_back((int)tiles.size());
+      if (d == 0) {
+        if (L.ctileStart.empty()) L.ctileStart.push_back(0);
+        for (int k = r.lo[2]; k <= r.hi[2]; k += kColumnPlanes)
+          for (int j = r.lo[1]; j <= r.hi[1]; j += 4)
+            for (int i = r.lo[0]; i <= r.hi[0]; i += 64) ctiles.push_back(Tile{p, i, j, k});
+        L.ctileStart.push_back((int)ctiles.size());
+      }
       L.cellStart.push_back(L.cellStart.back() + r.npts());
     }
     L.slab = off;
@@ -894,8 +945,10 @@ BatchSolver::BatchSolver(Context* ctx, std::vector<BatchProblemSpec> problems, i
     L.d_rect = upload(m_ctx, L.h_rect, &m_bytes);
     L.d_coef = upload(m_ctx, L.h_coef, &m_bytes);
     L.d_tiles = upload(m_ctx, tiles, &m_bytes);
+    L.d_ctiles = upload(m_ctx, ctiles, &m_bytes);
     // tile addresses stay inside each problem's array (threads past the rectangle return)
-    for (const Tile& t : tiles) {
+    for (const std::vector<Tile>* ts : {&tiles, &ctiles})
+    for (const Tile& t : *ts) {
       const DBox& x = L.h_rect[t.box];
       if (t.i0 < x.lo[0] || t.i0 > x.hi[0] || t.j0 < x.lo[1] || t.j0 > x.hi[1] || t.k0 < x.lo[2] || t.k0 > x.hi[2])
         throw std::runtime_error("batch: tile outside its problem");
@@ -971,7 +1024,7 @@ BatchSolver::~BatchSolver() {
   (void)hipStreamSynchronize(m_ctx->stream);
   auto f = [&](void* p) { if (p) (void)hipFree(p); };
   for (auto& L : m_depth) {
-    f(L.d_rect); f(L.d_coef); f(L.d_tiles); f(L.d_items); f(L.d_wStart); f(L.corr); f(L.res); f(L.tmp);
+    f(L.d_rect); f(L.d_coef); f(L.d_tiles); f(L.d_ctiles); f(L.d_items); f(L.d_wStart); f(L.corr); f(L.res); f(L.tmp);
   }
   f(m_phi); f(m_rhs); f(m_norm); f(d_active); f(d_metric); f(d_bstats); f(d_bottom); f(d_bbox); f(m_bcoef);
   f(d_corrPtr); f(d_resPtr); f(d_bscratchOff); f(d_bscratch);
@@ -1202,8 +1255,9 @@ void BatchSolver::residualNorm(bool homog) {
   double cells = 0;
   for (int p = 0; p < m_np; p++) cells += h_active[p] ? (double)(L.cellStart[p + 1] - L.cellStart[p]) : 0.0;
   hipEvent_t ev = profile_begin(m_ctx, KC_RESIDUAL);
-  BATCH_LAUNCH(k_batch_residual_norm<T>, L.tileStart[m_np], 256, L.d_tiles, 6, 2, L.d_rect, L.d_coef, d_active,
-               (T*)L.res, (const double*)m_phi, (const double*)m_rhs, m_norm);
+  const int nct = L.ctileStart[m_np];
+  BATCH_LAUNCH(k_batch_residual_norm_z<T>, (nct + 7) / 8 * 8, 256, L.d_ctiles, nct, kColumnPlanes, L.d_rect,
+               L.d_coef, d_active, (T*)L.res, (const double*)m_phi, (const double*)m_rhs, m_norm);
   profile_end(m_ctx, ev, KC_RESIDUAL, (16.0 + sizeof(T)) * cells, cells);
   hip_check(hipMemcpyAsync(h_norm, m_norm, sizeof(double) * m_np, hipMemcpyDeviceToHost, m_ctx->stream),
             "batch norms");
@@ -1213,8 +1267,9 @@ void BatchSolver::residualNorm(bool homog) {
 template <class T>
 void BatchSolver::update() {
   Depth& L = m_depth[0];
-  BATCH_LAUNCH(k_batch_update<T>, L.tileStart[m_np], 256, L.d_tiles, 6, 2, L.d_rect, d_active, m_phi,
-               (const T*)L.corr);
+  const int nct = L.ctileStart[m_np];
+  BATCH_LAUNCH(k_batch_update_z<T>, (nct + 7) / 8 * 8, 256, L.d_ctiles, nct, kColumnPlanes, L.d_rect, d_active,
+               m_phi, (const T*)L.corr);
 }
 
 // solveNoInitResid (AMRMultiGrid.H:935-1124) for every problem at once: each

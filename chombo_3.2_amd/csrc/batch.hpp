@@ -127,6 +127,9 @@ class BatchSolver {
     std::vector<int> tileStart;                  // per problem, prefix; [np] = total
     Tile* d_tiles = nullptr;
     int lx = 6, ly = 2;
+    // column tiles (64 x 4 x kColumnPlanes cells) of the finest depth
+    std::vector<int> ctileStart;
+    Tile* d_ctiles = nullptr;
     // fused relaxation work items
     std::vector<ops::FusedArgs> h_items;
     ops::FusedArgs* d_items = nullptr;
