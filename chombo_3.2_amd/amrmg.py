@@ -57,6 +57,7 @@ def lib():
         L.amrmg_batch_history.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _ip]
         L.amrmg_batch_tune_relax.argtypes = [C.c_void_p, _dp, C.c_int, _ip]
         L.amrmg_batch_stats.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+        L.amrmg_batch_set_graph.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_longlong)]
         L.amrmg_solver_create.argtypes = [C.c_void_p, _ip, _dp, C.c_int, C.POINTER(C.c_void_p)]
         L.amrmg_solver_destroy.argtypes = [C.c_void_p]
         L.amrmg_solver_set_params.argtypes = [C.c_void_p, _ip, _dp]
@@ -401,6 +402,14 @@ class BatchSolver:
         bi, db = C.c_longlong(), C.c_longlong()
         _chk(lib().amrmg_batch_stats(self.p, C.byref(cr), C.byref(bi), C.byref(db)))
         return dict(cell_relax=cr.value, bottom_iters=bi.value, device_bytes=db.value)
+
+    def set_graph(self, on=None):
+        """V-cycles replayed from one captured HIP graph (True, the default) or
+        launched directly (False); None leaves the setting.  Returns the number of
+        graph replays so far."""
+        n = C.c_longlong()
+        _chk(lib().amrmg_batch_set_graph(self.p, -1 if on is None else int(bool(on)), C.byref(n)))
+        return n.value
 
 
 class ServeEngine:

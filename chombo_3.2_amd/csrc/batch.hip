@@ -1021,6 +1021,7 @@ BatchSolver::BatchSolver(Context* ctx, std::vector<BatchProblemSpec> problems, i
 
 BatchSolver::~BatchSolver() {
   if (m_ctx->dry) return;
+  dropGraph();
   (void)hipStreamSynchronize(m_ctx->stream);
   auto f = [&](void* p) { if (p) (void)hipFree(p); };
   for (auto& L : m_depth) {
@@ -1034,6 +1035,7 @@ BatchSolver::~BatchSolver() {
 long long BatchSolver::cells(int q) const { return m_prob[m_slot[q]].domain.npts(); }
 
 void BatchSolver::setRelaxShape(int d, int shape, int kc) {
+  dropGraph();  // the captured cycle holds the old work tables and shape
   Depth& L = m_depth[d];
   if (L.d_items) { dev_free(m_ctx, L.d_items); L.d_items = nullptr; }
   if (L.d_wStart) { dev_free(m_ctx, L.d_wStart); L.d_wStart = nullptr; }
@@ -1089,6 +1091,7 @@ void BatchSolver::setRelaxShape(int d, int shape, int kc) {
 void BatchSolver::setSolverParameters(int pre, int post, int bottom, int numMG, int iterMax, int imin, double eps,
                                       double hang, double normThresh) {
   if (numMG != 1) throw std::runtime_error("batch: V-cycles only (numMG = 1)");
+  dropGraph();  // the captured cycle holds the relaxation counts
   m_pre = pre; m_post = post; m_nbottom = bottom; m_numMG = numMG;
   m_iterMax = iterMax; m_imin = imin; m_eps = eps; m_hang = hang; m_normThresh = normThresh;
 }
@@ -1213,7 +1216,6 @@ void BatchSolver::relaxDepth(int d, int np, int iters, bool fromZero) {
                                    src, (const T*)L.res);
     // algorithmic bytes: read phi (not when it is known zero), read rhs, write phi
     profile_end(m_ctx, ev, KC_GSRB, (zsrc ? 16.0 : 24.0) * sizeof(T) / 8.0 * cells, cells);
-    m_cellRelax += cells;
   }
   if (iters & 1)
     BATCH_LAUNCH(k_batch_copy<T>, L.tileStart[np], 256, L.d_tiles, 6, 2, L.d_rect, d_active, (T*)L.corr,
@@ -1245,6 +1247,63 @@ void BatchSolver::vcycle() {
                  (const T*)C.corr);
     relaxDepth<T>(d, np, m_post, false);
   }
+}
+
+// cell-relaxations of one V-cycle (pre and post relaxation at every depth above
+// the bottom, for the active problems that go below it)
+double BatchSolver::cycleUnits() const {
+  double units = 0;
+  for (size_t d = 0; d + 1 < m_depth.size(); d++) {
+    const Depth& L = m_depth[d];
+    const int np = m_depth[d + 1].np;
+    double cells = 0;
+    for (int p = 0; p < np; p++) cells += h_active[p] ? (double)(L.cellStart[p + 1] - L.cellStart[p]) : 0.0;
+    units += (m_pre + m_post) * cells;
+  }
+  return units;
+}
+
+// One V-cycle.  With the graph enabled (and the profiler off), the cycle's launches
+// are captured once into a HIP graph and every later cycle replays it: the launch
+// sequence, grids and arguments are the same for every cycle of every solve (the
+// problems that have left the lockstep are masked on the device), so the replay
+// runs exactly the captured work.  The cycle holds no memset or copy.
+template <class T>
+void BatchSolver::cycle() {
+  if (!m_useGraph || m_ctx->prof) {
+    vcycle<T>();
+    return;
+  }
+  if (!m_graph) {
+    hipGraph_t g = nullptr;
+    hip_check(hipStreamBeginCapture(m_ctx->stream, hipStreamCaptureModeThreadLocal), "batch: begin capture");
+    try {
+      vcycle<T>();
+    } catch (...) {
+      (void)hipStreamEndCapture(m_ctx->stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    hip_check(hipStreamEndCapture(m_ctx->stream, &g), "batch: end capture");
+    const hipError_t e = hipGraphInstantiate(&m_graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    hip_check(e, "batch: graph instantiate");
+  }
+  hip_check(hipGraphLaunch(m_graph, m_ctx->stream), "batch: graph launch");
+  m_graphLaunches++;
+}
+
+void BatchSolver::dropGraph() {
+  if (m_graph) {
+    (void)hipStreamSynchronize(m_ctx->stream);
+    (void)hipGraphExecDestroy(m_graph);
+    m_graph = nullptr;
+  }
+}
+
+void BatchSolver::setGraph(bool on) {
+  if (!on) dropGraph();
+  m_useGraph = on;
 }
 
 template <class T>
@@ -1314,7 +1373,8 @@ void BatchSolver::runSolve() {
     uploadActive();
     for (int p = 0; p < m_np; p++)
       if (h_active[p]) normLast[p] = rnorm[p];
-    vcycle<T>();
+    cycle<T>();
+    m_cellRelax += cycleUnits();
     update<T>();
     residualNorm<T>(false);
     nactive = 0;
@@ -1344,6 +1404,7 @@ void BatchSolver::solve() {
 }
 
 std::vector<double> BatchSolver::tuneRelax() {
+  dropGraph();
   std::vector<double> best(m_depth.size(), 0.0);
   hipEvent_t e0, e1;
   hip_check(hipEventCreate(&e0), "event");

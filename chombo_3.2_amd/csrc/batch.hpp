@@ -114,6 +114,11 @@ class BatchSolver {
   long long cells(int q) const;
   // profiler work units: cell-relaxations of the last solve
   double cellRelaxations() const { return m_cellRelax; }
+  // V-cycles replayed from one captured HIP graph (default on; the graph is not
+  // used while the context's profiler is on, so per-kernel timing stays exact)
+  void setGraph(bool on);
+  bool graph() const { return m_useGraph; }
+  long long graphLaunches() const { return m_graphLaunches; }
 
  private:
   struct Depth {
@@ -180,6 +185,12 @@ class BatchSolver {
 
   void alloc(void** p, size_t bytes);
   template <class T> void vcycle();
+  template <class T> void cycle();          // vcycle, through the captured graph when enabled
+  double cycleUnits() const;                // cell-relaxations of one V-cycle of the active problems
+  void dropGraph();
+  bool m_useGraph = true;
+  hipGraphExec_t m_graph = nullptr;
+  long long m_graphLaunches = 0;
   template <class T> void relaxDepth(int d, int np, int iters, bool fromZero);
   template <class T> void residualNorm(bool homog);
   template <class T> void update();

@@ -579,6 +579,12 @@ int amrmg_batch_stats(amrmg_batch* b, double* cell_relax, long long* bottom_iter
   if (device_bytes) *device_bytes = b->solver->deviceBytes();
   API_END
 }
+int amrmg_batch_set_graph(amrmg_batch* b, int on, long long* launches) {
+  API_BEGIN
+  if (on >= 0) b->solver->setGraph(on != 0);
+  if (launches) *launches = b->solver->graphLaunches();
+  API_END
+}
 
 // ---- serving: a request loop over device-resident hierarchies ----
 namespace {

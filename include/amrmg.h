@@ -144,6 +144,11 @@ int amrmg_batch_history(amrmg_batch* b, int problem, double* hist, int cap, int*
 int amrmg_batch_tune_relax(amrmg_batch* b, double* ms, int cap, int* ndepth);
 /* cell-relaxations and bottom BiCGStab iterations of the last solve; HBM bytes held */
 int amrmg_batch_stats(amrmg_batch* b, double* cell_relax, long long* bottom_iters, long long* device_bytes);
+/* No counterpart in the reference (its V-cycles are host loops of kernel calls).
+   on = 1: every V-cycle after the first is replayed from one captured HIP graph
+   (the default); 0: direct launches; -1: leave as is.  The graph is not used while
+   the context's profiler is on.  launches: graph replays so far. */
+int amrmg_batch_set_graph(amrmg_batch* b, int on, long long* launches);
 
 /* ---- serving: a request loop with continuous admission.  nslots worker threads
    each run one request at a time on its own HIP stream, so the solves of

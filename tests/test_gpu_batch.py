@@ -106,3 +106,32 @@ def test_batch_c4_sample_bitwise(gpu_ctx):
         hs, info, phis = _single(gpu_ctx, h, rhs, sp, rp)
         assert res[q][1] == info["iterations"], q
         np.testing.assert_array_equal(res[q][3], phis)
+
+
+def test_batch_graph_replays_bitwise_equal_direct_launches(gpu_ctx):
+    """V-cycles replayed from the captured HIP graph give bitwise the history and
+    solution of direct launches, over two solves in a row (every V-cycle after the
+    capture is a replay, with the eager update, residual and norm read-back between
+    replays), in FP64 and in mixed precision; the replay count shows the graph ran."""
+    probs = _edge_set()[:7]
+    sp, rp = H.solver_params()
+    gpu_ctx.profile([])  # the graph is bypassed while the profiler is on
+    for precision in (0, 1):
+        b = amrmg.BatchSolver(gpu_ctx, [h for h, _ in probs], precision)
+        b.setSolverParameters(sp, rp)
+        for q, (_, rhs) in enumerate(probs):
+            b.set_rhs(q, rhs)
+        b.set_graph(False)
+        it0, ex0 = b.solve()
+        direct = [(b.history(q), b.get_phi(q)) for q in range(len(probs))]
+        assert b.set_graph(None) == 0
+        b.set_graph(True)
+        for rep in range(2):
+            it, ex = b.solve()
+            np.testing.assert_array_equal(it, it0)
+            np.testing.assert_array_equal(ex, ex0)
+            for q in range(len(probs)):
+                np.testing.assert_array_equal(b.history(q), direct[q][0], err_msg=f"prec {precision} rep {rep} q {q}")
+                np.testing.assert_array_equal(b.get_phi(q), direct[q][1], err_msg=f"prec {precision} rep {rep} q {q}")
+        assert b.set_graph(None) >= 2 * int(np.max(it0)) - 2
+        b.close()
