@@ -208,7 +208,7 @@ def batch_leg(args, ctx):
     log(f"plain FP64 one-by-one: {t_plain:.3f}s for {len(probs)} solves")
 
     def batched(prec):
-        b = amrmg.BatchSolver(ctx, [h for h, _ in probs], prec)
+        b = amrmg.BatchSolver(ctx, [h for h, _ in probs], prec, lanes=args.batch_lanes)
         b.setSolverParameters(sp, rp)
         for q, (_, rhs) in enumerate(probs):
             b.set_rhs(q, rhs)
@@ -264,6 +264,7 @@ def batch_leg(args, ctx):
         "residual_mixed_achieved_GBs": round(resmp["bytes"] / (resmp["ms"] * 1e-3) / 1e9, 1) if resmp["ms"] else None,
         "mcell_relax_per_s_mixed": round(stmp["cell_relax"] / wmp / 1e6, 1),
         "vcycles_from_hip_graph": {"fp64": rep64, "mixed": repmp},
+        "lanes": args.batch_lanes,
     }
     log(f"batched: {out}")
     return out
@@ -355,6 +356,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the relaxation autotuner (default shapes)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--batch", type=int, default=512, help="problems in the batched leg (0 = skip)")
+    ap.add_argument("--batch-lanes", type=int, default=2, choices=(1, 2),
+                    help="2: the problems with large multigrid bottoms lockstep on a second stream")
     ap.add_argument("--serve", type=int, default=600, help="requests in the serving leg (0 = skip)")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-budget-gb", type=float, default=96.0)

@@ -49,6 +49,9 @@ def lib():
         L.amrmg_serve_stats.argtypes = [C.c_void_p, C.POINTER(C.c_longlong), C.c_int]
         L.amrmg_serve_destroy.argtypes = [C.c_void_p]
         L.amrmg_batch_create.argtypes = [C.c_void_p, C.c_int, _ip, _dp, C.c_int, C.POINTER(C.c_void_p)]
+        L.amrmg_batch_create_lanes.argtypes = [C.c_void_p, C.c_int, _ip, _dp, C.c_int, C.c_int,
+                                               C.POINTER(C.c_void_p)]
+        L.amrmg_batch_lanes.argtypes = [C.c_void_p, _ip, _ip]
         L.amrmg_batch_destroy.argtypes = [C.c_void_p]
         L.amrmg_batch_set_params.argtypes = [C.c_void_p, _ip, _dp]
         L.amrmg_batch_set_rhs.argtypes = [C.c_void_p, C.c_int, _dp]
@@ -332,9 +335,10 @@ class AMRMultiGrid:
 class BatchSolver:
     """Many independent one-level problems solved together, each V-cycle stage of
     all of them in one launch (amrmg_batch_*).  Per problem the arithmetic is that
-    of AMRMultiGrid on it alone."""
+    of AMRMultiGrid on it alone.  lanes=2 runs the problems with large multigrid
+    bottoms in a lockstep of their own on a second stream (same results)."""
 
-    def __init__(self, ctx: Context, hiers, precision: int = 0):
+    def __init__(self, ctx: Context, hiers, precision: int = 0, lanes: int = 1):
         self.ctx = ctx
         self.hiers = list(hiers)
         isps, rsps = [], []
@@ -347,10 +351,17 @@ class BatchSolver:
         isp = np.ascontiguousarray(np.concatenate(isps))
         rsp = np.ascontiguousarray(np.concatenate(rsps))
         p = C.c_void_p()
-        _chk(lib().amrmg_batch_create(ctx.p, len(self.hiers), isp.ctypes.data_as(_ip), rsp.ctypes.data_as(_dp),
-                                       precision, C.byref(p)))
+        _chk(lib().amrmg_batch_create_lanes(ctx.p, len(self.hiers), isp.ctypes.data_as(_ip),
+                                             rsp.ctypes.data_as(_dp), precision, lanes, C.byref(p)))
         self.p = p
         self.n = len(self.hiers)
+
+    def lanes(self):
+        """(number of lanes, lane of each problem)"""
+        n = C.c_int()
+        per = np.zeros(self.n, np.int32)
+        _chk(lib().amrmg_batch_lanes(self.p, C.byref(n), per.ctypes.data_as(_ip)))
+        return n.value, per
 
     def close(self):
         if self.p:

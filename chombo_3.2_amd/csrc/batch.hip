@@ -866,6 +866,19 @@ static int mg_depths(const DBL& g) {
   return n;
 }
 
+long long batchBottomCells(const BatchProblemSpec& P) {
+  const int nd = mg_depths(P.boxes);
+  return coarsen(P.domain, 1 << (nd - 1)).npts();
+}
+
+void BatchSolver::setContext(Context* c) {
+  if (c == m_ctx) return;
+  if (c->device != m_ctx->device) throw std::runtime_error("batch: context on another device");
+  dropGraph();
+  (void)hipStreamSynchronize(m_ctx->stream);
+  m_ctx = c;
+}
+
 BatchSolver::BatchSolver(Context* ctx, std::vector<BatchProblemSpec> problems, int precision)
     : m_ctx(ctx), m_prec(precision), m_np((int)problems.size()) {
   if (precision != 0 && precision != 1) throw std::runtime_error("batch: precision must be 0 or 1");

@@ -84,6 +84,9 @@ class DeviceBottomSolver : public LinearSolver<LevelData<T>> {
   void release();
 };
 
+// valid cells of a problem's multigrid bottom (its deepest coarsening)
+long long batchBottomCells(const BatchProblemSpec& P);
+
 class BatchSolver {
  public:
   BatchSolver(Context* ctx, std::vector<BatchProblemSpec> problems, int precision);
@@ -117,6 +120,9 @@ class BatchSolver {
   // V-cycles replayed from one captured HIP graph (default on; the graph is not
   // used while the context's profiler is on, so per-kernel timing stays exact)
   void setGraph(bool on);
+  // run later launches on another context's stream (same device)
+  void setContext(Context* c);
+  Context* context() const { return m_ctx; }
   bool graph() const { return m_useGraph; }
   long long graphLaunches() const { return m_graphLaunches; }
 

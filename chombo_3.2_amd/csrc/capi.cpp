@@ -40,9 +40,24 @@ extern "C" const char* amrmg_last_error(void) { return g_err.c_str(); }
 struct amrmg_ctx {
   std::unique_ptr<Context> c;
 };
+// A batch runs in one or more lanes: each lane is a BatchSolver over a subset of the
+// problems, and lanes after the first run on streams of their own, so their lockstep
+// iterations overlap on the device (problem results do not depend on the lanes).
 struct amrmg_batch {
   amrmg_ctx* ctx = nullptr;
-  std::unique_ptr<BatchSolver> solver;
+  struct Lane {
+    std::unique_ptr<Context> own;  // the stream of a lane after the first
+    std::unique_ptr<BatchSolver> solver;
+    std::vector<int> probs;        // caller indices of the lane's problems
+  };
+  std::vector<Lane> lanes;
+  std::vector<std::pair<int, int>> at;  // caller index -> (lane, index in the lane)
+  int nprob = 0;
+  BatchSolver& solver(int q, int& local) {
+    if (q < 0 || q >= nprob) throw std::runtime_error("batch: bad problem index");
+    local = at[q].second;
+    return *lanes[at[q].first].solver;
+  }
 };
 
 struct amrmg_solver {
@@ -504,8 +519,13 @@ int amrmg_solver_set_relax_shape(amrmg_solver* s, int shape, int kc) {
 // ---- batched single-level solves ----
 int amrmg_batch_create(amrmg_ctx* ctx, int nprob, const int* ispecs, const double* rspecs, int precision,
                        amrmg_batch** out) {
+  return amrmg_batch_create_lanes(ctx, nprob, ispecs, rspecs, precision, 1, out);
+}
+int amrmg_batch_create_lanes(amrmg_ctx* ctx, int nprob, const int* ispecs, const double* rspecs, int precision,
+                             int lanes, amrmg_batch** out) {
   API_BEGIN
   if (nprob <= 0 || nprob > (1 << 20)) throw std::runtime_error("batch: bad problem count");
+  if (lanes != 1 && lanes != 2) throw std::runtime_error("batch: lanes must be 1 or 2");
   std::vector<BatchProblemSpec> probs;
   const int* is = ispecs;
   for (int q = 0; q < nprob; q++) {
@@ -522,7 +542,34 @@ int amrmg_batch_create(amrmg_ctx* ctx, int nprob, const int* ispecs, const doubl
   }
   std::unique_ptr<amrmg_batch> bt(new amrmg_batch());
   bt->ctx = ctx;
-  bt->solver.reset(new BatchSolver(ctx->c.get(), std::move(probs), precision));
+  bt->nprob = nprob;
+  // two lanes: the problems whose bottom holds at least 16^3 cells (their
+  // BiCGStab bottoms are the long pole of every V-cycle) lockstep on their own
+  // stream, so the other problems' cycles fill the device meanwhile
+  constexpr long long kBigBottom = 4096;
+  std::vector<int> lane(nprob, 0);
+  int nbig = 0;
+  if (lanes == 2)
+    for (int q = 0; q < nprob; q++)
+      if (batchBottomCells(probs[q]) >= kBigBottom) { lane[q] = 1; nbig++; }
+  const int nl = (nbig > 0 && nbig < nprob) ? 2 : 1;
+  if (nl == 1) std::fill(lane.begin(), lane.end(), 0);
+  bt->lanes.resize(nl);
+  bt->at.resize(nprob);
+  std::vector<std::vector<BatchProblemSpec>> lp(nl);
+  for (int q = 0; q < nprob; q++) {
+    bt->at[q] = {lane[q], (int)lp[lane[q]].size()};
+    lp[lane[q]].push_back(probs[q]);
+    bt->lanes[lane[q]].probs.push_back(q);
+  }
+  for (int l = 0; l < nl; l++) {
+    Context* c = ctx->c.get();
+    if (l > 0) {
+      bt->lanes[l].own.reset(new Context(ctx->c->device, ctx->c->dry));
+      c = bt->lanes[l].own.get();
+    }
+    bt->lanes[l].solver.reset(new BatchSolver(c, std::move(lp[l]), precision));
+  }
   *out = bt.release();
   API_END
 }
@@ -533,56 +580,123 @@ int amrmg_batch_destroy(amrmg_batch* b) {
 }
 int amrmg_batch_set_params(amrmg_batch* b, const int* sp, const double* rp) {
   API_BEGIN
-  b->solver->setSolverParameters(sp[0], sp[1], sp[2], sp[3], sp[4], sp[5], rp[0], rp[1], rp[2]);
+  for (auto& l : b->lanes) l.solver->setSolverParameters(sp[0], sp[1], sp[2], sp[3], sp[4], sp[5], rp[0], rp[1], rp[2]);
   API_END
 }
 int amrmg_batch_set_rhs(amrmg_batch* b, int q, const double* valid_packed) {
   API_BEGIN
-  if (q < 0 || q >= b->solver->numProblems()) throw std::runtime_error("batch: bad problem index");
-  b->solver->setRhs(q, valid_packed);
+  int k;
+  BatchSolver& s = b->solver(q, k);
+  s.setRhs(k, valid_packed);
   API_END
 }
 int amrmg_batch_get_phi(amrmg_batch* b, int q, double* valid_packed) {
   API_BEGIN
-  if (q < 0 || q >= b->solver->numProblems()) throw std::runtime_error("batch: bad problem index");
-  b->solver->getPhi(q, valid_packed);
+  int k;
+  BatchSolver& s = b->solver(q, k);
+  s.getPhi(k, valid_packed);
   API_END
 }
 int amrmg_batch_solve(amrmg_batch* b, int* iterations, int* exit_status) {
   API_BEGIN
-  b->solver->solve();
-  for (int q = 0; q < b->solver->numProblems(); q++) {
-    if (iterations) iterations[q] = b->solver->iterations(q);
-    if (exit_status) exit_status[q] = b->solver->exitStatus(q);
+  Context* main = b->ctx->c.get();
+  if (b->lanes.size() == 1) {
+    b->lanes[0].solver->solve();
+  } else if (main->prof) {
+    // with the event profiler on, the lanes run one after another on the profiled
+    // stream, so every kernel is timed alone
+    for (auto& l : b->lanes) {
+      Context* own = l.solver->context();
+      l.solver->setContext(main);
+      try {
+        l.solver->solve();
+      } catch (...) {
+        l.solver->setContext(own);
+        throw;
+      }
+      l.solver->setContext(own);
+    }
+  } else {
+    // each lane after the first is driven by a thread of its own on its own stream
+    std::vector<std::string> err(b->lanes.size());
+    std::vector<std::thread> th;
+    for (size_t l = 1; l < b->lanes.size(); l++)
+      th.emplace_back([&, l] {
+        try {
+          hip_check(hipSetDevice(main->device), "hipSetDevice");
+          b->lanes[l].solver->solve();
+        } catch (const std::exception& e) {
+          err[l] = e.what();
+        }
+      });
+    try {
+      b->lanes[0].solver->solve();
+    } catch (const std::exception& e) {
+      err[0] = e.what();
+    }
+    for (auto& t : th) t.join();
+    for (auto& e : err)
+      if (!e.empty()) throw std::runtime_error(e);
+  }
+  for (int q = 0; q < b->nprob; q++) {
+    int k;
+    BatchSolver& s = b->solver(q, k);
+    if (iterations) iterations[q] = s.iterations(k);
+    if (exit_status) exit_status[q] = s.exitStatus(k);
   }
   API_END
 }
 int amrmg_batch_history(amrmg_batch* b, int q, double* hist, int cap, int* n) {
   API_BEGIN
-  if (q < 0 || q >= b->solver->numProblems()) throw std::runtime_error("batch: bad problem index");
-  const std::vector<double>& h = b->solver->history(q);
+  int k;
+  BatchSolver& s = b->solver(q, k);
+  const std::vector<double>& h = s.history(k);
   for (int i = 0; i < cap && i < (int)h.size(); i++) hist[i] = h[i];
   if (n) *n = (int)h.size();
   API_END
 }
 int amrmg_batch_tune_relax(amrmg_batch* b, double* ms, int cap, int* ndepth) {
   API_BEGIN
-  std::vector<double> t = b->solver->tuneRelax();
+  // every lane is tuned; the reported times are the per-depth sums over the lanes
+  std::vector<double> t;
+  for (auto& l : b->lanes) {
+    std::vector<double> tl = l.solver->tuneRelax();
+    if (tl.size() > t.size()) t.resize(tl.size(), 0.0);
+    for (size_t i = 0; i < tl.size(); i++) t[i] += tl[i];
+  }
   for (int i = 0; i < cap && i < (int)t.size(); i++) ms[i] = t[i];
   if (ndepth) *ndepth = (int)t.size();
   API_END
 }
 int amrmg_batch_stats(amrmg_batch* b, double* cell_relax, long long* bottom_iters, long long* device_bytes) {
   API_BEGIN
-  if (cell_relax) *cell_relax = b->solver->cellRelaxations();
-  if (bottom_iters) *bottom_iters = b->solver->bottomIterations();
-  if (device_bytes) *device_bytes = b->solver->deviceBytes();
+  double cr = 0;
+  long long bi = 0, db = 0;
+  for (auto& l : b->lanes) {
+    cr += l.solver->cellRelaxations();
+    bi += l.solver->bottomIterations();
+    db += l.solver->deviceBytes();
+  }
+  if (cell_relax) *cell_relax = cr;
+  if (bottom_iters) *bottom_iters = bi;
+  if (device_bytes) *device_bytes = db;
   API_END
 }
 int amrmg_batch_set_graph(amrmg_batch* b, int on, long long* launches) {
   API_BEGIN
-  if (on >= 0) b->solver->setGraph(on != 0);
-  if (launches) *launches = b->solver->graphLaunches();
+  long long n = 0;
+  for (auto& l : b->lanes) {
+    if (on >= 0) l.solver->setGraph(on != 0);
+    n += l.solver->graphLaunches();
+  }
+  if (launches) *launches = n;
+  API_END
+}
+int amrmg_batch_lanes(amrmg_batch* b, int* lanes, int* lane_of_problem) {
+  API_BEGIN
+  if (lanes) *lanes = (int)b->lanes.size();
+  if (lane_of_problem)
+    for (int q = 0; q < b->nprob; q++) lane_of_problem[q] = b->at[q].first;
   API_END
 }
 

@@ -135,3 +135,30 @@ def test_batch_graph_replays_bitwise_equal_direct_launches(gpu_ctx):
                 np.testing.assert_array_equal(b.get_phi(q), direct[q][1], err_msg=f"prec {precision} rep {rep} q {q}")
         assert b.set_graph(None) >= 2 * int(np.max(it0)) - 2
         b.close()
+
+
+def test_batch_two_lanes_bitwise_equal_one_lane(gpu_ctx):
+    """lanes=2 puts the problems with bottoms of at least 16^3 cells (edge 72 here:
+    an 18^3 bottom) in a lockstep of their own on a second stream, run concurrently
+    by a second host thread; every problem's history and solution are bitwise those
+    of the single lane, in FP64 and mixed precision, concurrent (profiler off) and
+    one lane after the other (profiler on)."""
+    probs = _edge_set()
+    sp, rp = H.solver_params()
+    for precision in (0, 1):
+        ref = _batch(gpu_ctx, probs, sp, rp, precision=precision)
+        b = amrmg.BatchSolver(gpu_ctx, [h for h, _ in probs], precision, lanes=2)
+        nl, per = b.lanes()
+        assert nl == 2 and per[4] == 1 and int(per.sum()) == 1, per  # only the edge-72 problem
+        b.setSolverParameters(sp, rp)
+        for q, (_, rhs) in enumerate(probs):
+            b.set_rhs(q, rhs)
+        for profiled in (False, True, False):
+            gpu_ctx.profile(["gsrb"] if profiled else [])
+            it, ex = b.solve()
+            for q in range(len(probs)):
+                assert it[q] == ref[q][1] and ex[q] == ref[q][2], (precision, profiled, q)
+                np.testing.assert_array_equal(b.history(q), ref[q][0])
+                np.testing.assert_array_equal(b.get_phi(q), ref[q][3])
+        gpu_ctx.profile([])
+        b.close()
