@@ -356,8 +356,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the relaxation autotuner (default shapes)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--batch", type=int, default=512, help="problems in the batched leg (0 = skip)")
-    ap.add_argument("--batch-lanes", type=int, default=2, choices=(1, 2),
-                    help="2: the problems with large multigrid bottoms lockstep on a second stream")
+    ap.add_argument("--batch-lanes", type=int, default=3, choices=(1, 2, 3),
+                    help="2 or 3: the problems with large multigrid bottoms lockstep on further streams")
     ap.add_argument("--serve", type=int, default=600, help="requests in the serving leg (0 = skip)")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-budget-gb", type=float, default=96.0)

@@ -525,7 +525,7 @@ int amrmg_batch_create_lanes(amrmg_ctx* ctx, int nprob, const int* ispecs, const
                              int lanes, amrmg_batch** out) {
   API_BEGIN
   if (nprob <= 0 || nprob > (1 << 20)) throw std::runtime_error("batch: bad problem count");
-  if (lanes != 1 && lanes != 2) throw std::runtime_error("batch: lanes must be 1 or 2");
+  if (lanes < 1 || lanes > 3) throw std::runtime_error("batch: lanes must be 1, 2 or 3");
   std::vector<BatchProblemSpec> probs;
   const int* is = ispecs;
   for (int q = 0; q < nprob; q++) {
@@ -543,17 +543,24 @@ int amrmg_batch_create_lanes(amrmg_ctx* ctx, int nprob, const int* ispecs, const
   std::unique_ptr<amrmg_batch> bt(new amrmg_batch());
   bt->ctx = ctx;
   bt->nprob = nprob;
-  // two lanes: the problems whose bottom holds at least 16^3 cells (their
-  // BiCGStab bottoms are the long pole of every V-cycle) lockstep on their own
-  // stream, so the other problems' cycles fill the device meanwhile
-  constexpr long long kBigBottom = 4096;
-  std::vector<int> lane(nprob, 0);
-  int nbig = 0;
-  if (lanes == 2)
-    for (int q = 0; q < nprob; q++)
-      if (batchBottomCells(probs[q]) >= kBigBottom) { lane[q] = 1; nbig++; }
-  const int nl = (nbig > 0 && nbig < nprob) ? 2 : 1;
-  if (nl == 1) std::fill(lane.begin(), lane.end(), 0);
+  // lanes by the size of the multigrid bottom: the problems whose bottom holds at
+  // least 16^3 cells (their BiCGStab bottoms are the long pole of every V-cycle)
+  // lockstep on a stream of their own, so the other problems' cycles fill the
+  // device meanwhile; with three lanes the bottoms of at least 27^3 cells get a
+  // third.  Empty lanes are dropped.
+  const long long kBottomLane[2] = {4096, 19683};
+  std::vector<int> want(nprob, 0);
+  for (int q = 0; q < nprob; q++) {
+    const long long bc = lanes > 1 ? batchBottomCells(probs[q]) : 0;
+    if (bc >= kBottomLane[0]) want[q] = 1;
+    if (lanes > 2 && bc >= kBottomLane[1]) want[q] = 2;
+  }
+  std::vector<int> remap(3, -1);
+  int nl = 0;
+  for (int l = 0; l < 3; l++)
+    if (std::count(want.begin(), want.end(), l) > 0) remap[l] = nl++;
+  std::vector<int> lane(nprob);
+  for (int q = 0; q < nprob; q++) lane[q] = remap[want[q]];
   bt->lanes.resize(nl);
   bt->at.resize(nprob);
   std::vector<std::vector<BatchProblemSpec>> lp(nl);

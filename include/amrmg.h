@@ -144,10 +144,11 @@ int amrmg_batch_history(amrmg_batch* b, int problem, double* hist, int cap, int*
 int amrmg_batch_tune_relax(amrmg_batch* b, double* ms, int cap, int* ndepth);
 /* cell-relaxations and bottom BiCGStab iterations of the last solve; HBM bytes held */
 int amrmg_batch_stats(amrmg_batch* b, double* cell_relax, long long* bottom_iters, long long* device_bytes);
-/* No counterpart in the reference.  As amrmg_batch_create, with lanes = 1 or 2.  With
-   2, the problems whose multigrid bottom holds at least 16^3 cells lockstep on a
+/* No counterpart in the reference.  As amrmg_batch_create, with lanes = 1, 2 or 3.
+   With 2, the problems whose multigrid bottom holds at least 16^3 cells lockstep on a
    stream of their own (a lane), so the other problems' V-cycles overlap their long
-   bottom solves.  Each problem's result is the same in either case. */
+   bottom solves; with 3, bottoms of at least 27^3 cells get a third lane.  Each
+   problem's result is the same in every case. */
 int amrmg_batch_create_lanes(amrmg_ctx* ctx, int nproblems, const int* ispecs, const double* rspecs, int precision,
                              int lanes, amrmg_batch** out);
 /* number of lanes and, per problem, its lane */

@@ -162,3 +162,25 @@ def test_batch_two_lanes_bitwise_equal_one_lane(gpu_ctx):
                 np.testing.assert_array_equal(b.get_phi(q), ref[q][3])
         gpu_ctx.profile([])
         b.close()
+
+
+def test_batch_three_lanes_bitwise_equal_one_lane(gpu_ctx):
+    """lanes=3: edge 120 (64 + 56 boxes, a 30^3 bottom) in the third lane, edge 72
+    (an 18^3 bottom) in the second, the rest in the first; bitwise equal to one lane."""
+    probs = [_problem(120, 0.4, 1.3, seed=31), _problem(72, 0.5, 1.5, seed=32), _problem(24, 0.3, 1.2, seed=33),
+             _problem(16, 0.0, 1.0, seed=34)]
+    sp, rp = H.solver_params()
+    gpu_ctx.profile([])
+    ref = _batch(gpu_ctx, probs, sp, rp)
+    b = amrmg.BatchSolver(gpu_ctx, [h for h, _ in probs], 0, lanes=3)
+    nl, per = b.lanes()
+    assert nl == 3 and list(per) == [2, 1, 0, 0], per
+    b.setSolverParameters(sp, rp)
+    for q, (_, rhs) in enumerate(probs):
+        b.set_rhs(q, rhs)
+    it, ex = b.solve()
+    for q in range(len(probs)):
+        assert it[q] == ref[q][1] and ex[q] == ref[q][2], q
+        np.testing.assert_array_equal(b.history(q), ref[q][0])
+        np.testing.assert_array_equal(b.get_phi(q), ref[q][3])
+    b.close()
