@@ -148,6 +148,11 @@ struct DeviceLayout {
   Tile* d_tiles = nullptr;
   int ntiles = 0;
   int TX = 64, TY = 4, TZ = 1;
+  // merged levels: column tiles of 64 x 4 cells and kColumnPlanes planes over the
+  // rectangle, for the stencil kernels that march in z
+  static constexpr int kColumnPlanes = 8;
+  Tile* d_ctiles = nullptr;
+  int nctiles = 0;
   long long valid_cells = 0;
   Context* ctx = nullptr;
   DeviceLayout(Context* c, const DBL& d, const Box& dom, bool merge = true);

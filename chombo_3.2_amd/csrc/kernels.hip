@@ -221,10 +221,22 @@ DeviceLayout::DeviceLayout(Context* c, const DBL& d, const Box& dom, bool merge)
   dev_h2d(c, d_boxes, h_boxes.data(), sizeof(DBox) * h_boxes.size(), "copy boxes");
   dev_alloc(c, (void**)&d_tiles, sizeof(Tile) * tiles.size(), "hipMalloc tiles");
   dev_h2d(c, d_tiles, tiles.data(), sizeof(Tile) * tiles.size(), "copy tiles");
+  if (merged) {
+    std::vector<Tile> ct;
+    const DBox& r = h_boxes[nbox];
+    for (int k = r.lo[2]; k <= r.hi[2]; k += kColumnPlanes)
+      for (int j = r.lo[1]; j <= r.hi[1]; j += 4)
+        for (int i = r.lo[0]; i <= r.hi[0]; i += 64) ct.push_back(Tile{nbox, i, j, k});
+    if (ct.size() > (size_t)INT32_MAX - 8) throw std::runtime_error("too many column tiles");
+    nctiles = (int)ct.size();
+    dev_alloc(c, (void**)&d_ctiles, sizeof(Tile) * ct.size(), "hipMalloc column tiles");
+    dev_h2d(c, d_ctiles, ct.data(), sizeof(Tile) * ct.size(), "copy column tiles");
+  }
 }
 DeviceLayout::~DeviceLayout() {
   dev_free(ctx, d_boxes);
   dev_free(ctx, d_tiles);
+  dev_free(ctx, d_ctiles);
 }
 
 template <class T>
@@ -595,9 +607,38 @@ __global__ void __launch_bounds__(256) k_apply(const DBox* __restrict__ boxes, c
   const T lap = (phi[c + 1] + phi[c - 1] + phi[c + sy] + phi[c - sy] + phi[c + sz] + phi[c - sz] - T(6.0) * p) * dxinv;
   lofphi[c] = alpha * p + beta * lap;
 }
+// the same over a merged level's column tiles, marching in z: the z neighbours come
+// from the planes already in registers, so each plane of phi is read once
+template <class T>
+__global__ void __launch_bounds__(256) k_apply_z(const DBox* __restrict__ boxes, const Tile* __restrict__ ctiles,
+                                                 int nct, int kz, T* __restrict__ lofphi, const T* __restrict__ phi,
+                                                 T dxinv, T alpha, T beta) {
+  const int w = xcd_contiguous_block();
+  if (w >= nct) return;
+  const Tile t = ctiles[w];
+  const DBox b = boxes[t.box];
+  const int i = t.i0 + (threadIdx.x & 63), j = t.j0 + (threadIdx.x >> 6);
+  if (i > b.hi[0] || j > b.hi[1]) return;
+  const int kend = min(t.k0 + kz - 1, b.hi[2]);
+  const long long sy = b.sy, sz = b.sz;
+  long long c = cidx(b, i, j, t.k0);
+  T pm = phi[c - sz], p = phi[c];
+  for (int k = t.k0; k <= kend; k++, c += sz) {
+    const T pz = phi[c + sz];
+    const T lap = (phi[c + 1] + phi[c - 1] + phi[c + sy] + phi[c - sy] + pz + pm - T(6.0) * p) * dxinv;
+    lofphi[c] = alpha * p + beta * lap;
+    pm = p;
+    p = pz;
+  }
+}
 template <class T>
 void apply(const DeviceLayout& L, T* lofphi, const T* phi, double dx, double alpha, double beta) {
   double dxinv = 1.0 / (dx * dx);
+  if (L.merged && L.nctiles > 0) {
+    LAUNCH_P(KC_APPLY, 2.0 * sizeof(T) * L.valid_cells, k_apply_z<T>, (L.nctiles + 7) / 8 * 8, L.d_boxes, L.d_ctiles,
+             L.nctiles, DeviceLayout::kColumnPlanes, lofphi, phi, T(dxinv), T(alpha), T(beta));
+    return;
+  }
   LAUNCH_P(KC_APPLY, 2.0 * sizeof(T) * L.valid_cells, k_apply<T>, L.ntiles, L.d_boxes, L.d_tiles, geom(L), lofphi, phi, T(dxinv), T(alpha), T(beta));
 }
 
@@ -615,9 +656,37 @@ __global__ void __launch_bounds__(256) k_residual(const DBox* __restrict__ boxes
   const T lap = (phi[c + 1] + phi[c - 1] + phi[c + sy] + phi[c - sy] + phi[c + sz] + phi[c - sz] - T(6.0) * p) * dxinv;
   res[c] = -(alpha * p) - beta * lap + rhs[c];
 }
+// the same over a merged level's column tiles, marching in z (as k_apply_z)
+template <class T>
+__global__ void __launch_bounds__(256) k_residual_z(const DBox* __restrict__ boxes, const Tile* __restrict__ ctiles,
+                                                    int nct, int kz, T* res, const T* __restrict__ phi, const T* rhs,
+                                                    T dxinv, T alpha, T beta) {
+  const int w = xcd_contiguous_block();
+  if (w >= nct) return;
+  const Tile t = ctiles[w];
+  const DBox b = boxes[t.box];
+  const int i = t.i0 + (threadIdx.x & 63), j = t.j0 + (threadIdx.x >> 6);
+  if (i > b.hi[0] || j > b.hi[1]) return;
+  const int kend = min(t.k0 + kz - 1, b.hi[2]);
+  const long long sy = b.sy, sz = b.sz;
+  long long c = cidx(b, i, j, t.k0);
+  T pm = phi[c - sz], p = phi[c];
+  for (int k = t.k0; k <= kend; k++, c += sz) {
+    const T pz = phi[c + sz];
+    const T lap = (phi[c + 1] + phi[c - 1] + phi[c + sy] + phi[c - sy] + pz + pm - T(6.0) * p) * dxinv;
+    res[c] = -(alpha * p) - beta * lap + rhs[c];
+    pm = p;
+    p = pz;
+  }
+}
 template <class T>
 void residual(const DeviceLayout& L, T* res, const T* phi, const T* rhs, double dx, double alpha, double beta) {
   double dxinv = 1.0 / (dx * dx);
+  if (L.merged && L.nctiles > 0) {
+    LAUNCH_P(KC_RESIDUAL, 3.0 * sizeof(T) * L.valid_cells, k_residual_z<T>, (L.nctiles + 7) / 8 * 8, L.d_boxes,
+             L.d_ctiles, L.nctiles, DeviceLayout::kColumnPlanes, res, phi, rhs, T(dxinv), T(alpha), T(beta));
+    return;
+  }
   LAUNCH_P(KC_RESIDUAL, 3.0 * sizeof(T) * L.valid_cells, k_residual<T>, L.ntiles, L.d_boxes, L.d_tiles, geom(L), res, phi, rhs, T(dxinv), T(alpha), T(beta));
 }
 
