@@ -164,6 +164,22 @@ class Hierarchy:
                 parts.append(np.asarray(fn(x, y, z, l), dtype=np.float64))
         return np.concatenate(parts) if parts else np.zeros(0)
 
+    def pack_faces(self, fn, d: int, lev: int = 0) -> np.ndarray:
+        """Evaluate fn(x, y, z) on the faces normal to d of every box of level lev
+        (the box grown by one face in d), per box in Fortran order: the face-packed
+        layout of a FluxBox component (variable coefficients)."""
+        dx = self.dx(lev)
+        parts = []
+        for (lo, hi) in self.levels[lev]:
+            ax = []
+            for q in range(3):
+                n = hi[q] - lo[q] + 1 + (1 if q == d else 0)
+                idx = np.arange(lo[q], lo[q] + n, dtype=np.float64)
+                ax.append(idx * dx if q == d else (idx + 0.5) * dx)
+            z, y, x = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+            parts.append(np.asarray(fn(x, y, z), dtype=np.float64).ravel())
+        return np.concatenate(parts) if parts else np.zeros(0)
+
     def unpack_level(self, data: np.ndarray, lev: int):
         """List of (box, 3-D array indexed [i,j,k]) for level lev of valid-packed data."""
         offs = self.level_offsets()
@@ -247,6 +263,26 @@ def bump_rhs(h: Hierarchy, seed: int, nbumps: int = 3) -> np.ndarray:
         return v
 
     return h.pack(f)
+
+
+def vc_coefficients(h: "Hierarchy", seed: int = 7, contrast: float = 4.0):
+    """Smooth positive variable coefficients on level 0 (VCAMRPoissonOp2):
+    a(x) = 1 + 0.5 sin(2 pi x) cos(2 pi y) sin(2 pi z) (cells) and
+    b(x) = 1 + (contrast - 1) * (0.5 + 0.5 sin(2 pi (x + phase)) cos(2 pi y) cos(pi z))
+    (faces), phases from the seed.  Returns (a, [b0, b1, b2])."""
+    rng = np.random.default_rng(seed)
+    ph = rng.uniform(0, 1, size=3)
+    a = h.pack(lambda x, y, z, l: 1.0 + 0.5 * np.sin(2 * math.pi * (x + ph[0])) * np.cos(2 * math.pi * y)
+               * np.sin(2 * math.pi * z))
+
+    def bf(x, y, z):
+        return 1.0 + (contrast - 1.0) * (0.5 + 0.5 * np.sin(2 * math.pi * (x + ph[1])) * np.cos(2 * math.pi * (y + ph[2]))
+                                         * np.cos(math.pi * z))
+
+    if h.nlev != 1:
+        raise ValueError("variable coefficients: one level")
+    a = a[:h.cells(0)]
+    return a, [h.pack_faces(bf, d) for d in range(3)]
 
 
 DEFAULT_SOLVER = dict(pre=2, post=2, bottom=2, numMG=1, iterMax=100, imin=5,

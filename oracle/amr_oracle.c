@@ -1044,6 +1044,10 @@ typedef struct Op {
   int hasCoarser, hasFiner, refToCoarser, refToFiner;
   QCFI qcfi;          /* interpolation from the next coarser AMR level */
   FluxReg fr;         /* register with the next finer AMR level */
+  /* VCAMRPoissonOp2 (variable coefficients): L phi = alpha a phi - beta div(b grad phi) */
+  int vc;
+  LD va, vlam;        /* aCoef and the relaxation coefficient lambda, valid cells */
+  Fab* vb[3];         /* bCoef: per box, the faces normal to d (box grown by one face in d) */
 } Op;
 
 static void op_define(Op* op, const Layout* L, const Box* domain, double dx, double alpha,
@@ -1060,6 +1064,10 @@ static void op_define(Op* op, const Layout* L, const Box* domain, double dx, dou
       for (int s = 0; s < 2; s++) cfivs_define(&op->cf[b*6+d*2+s], domain, &L->b[b], &op->L, d, s);
 }
 static void op_free(Op* op) {
+  if (op->vc) {
+    ld_free(&op->va); ld_free(&op->vlam);
+    for (int d = 0; d < 3; d++) { for (int b = 0; b < op->L.n; b++) fab_free(&op->vb[d][b]); free(op->vb[d]); }
+  }
   for (int b = 0; b < op->L.n*6; b++) ivs_free(&op->cf[b].ivs);
   free(op->cf); plan_free(&op->ex);
   if (op->hasCoarser) qcfi_free(&op->qcfi);
@@ -1111,13 +1119,154 @@ static void op_homog_cf(const Op* op, LD* phi) {
         }
       }
 }
-/* residualI (AMRPoissonOp.cpp:260-297) */
+/* ---- VCAMRPoissonOp2 kernels (lib/src/AMRElliptic/VCAMRPoissonOpF.ChF) ---- */
+/* the operator at (i,j,k): alpha*a*phi - beta*(sum of face fluxes)*dxinv, the six
+   flux terms in the order of the CHF_DTERM expansion (VCAMRPoissonOpF.ChF:236-254) */
+static inline double vc_lofphi(const Fab* phi, const Fab* a, Fab* const* b, int i, int j, int k,
+                               double alpha, double beta, double dxinv) {
+  const double p = *P(phi,i,j,k);
+  const double s = *P(&b[0][0],i+1,j,k) * (*P(phi,i+1,j,k) - p)
+                 - *P(&b[0][0],i,j,k) * (p - *P(phi,i-1,j,k))
+                 + *P(&b[1][0],i,j+1,k) * (*P(phi,i,j+1,k) - p)
+                 - *P(&b[1][0],i,j,k) * (p - *P(phi,i,j-1,k))
+                 + *P(&b[2][0],i,j,k+1) * (*P(phi,i,j,k+1) - p)
+                 - *P(&b[2][0],i,j,k) * (p - *P(phi,i,j,k-1));
+  return alpha * *P(a,i,j,k) * p - beta * s * dxinv;
+}
+/* the three face arrays of box q */
+#define VCB(op, q) ((Fab* const[3]){&(op)->vb[0][q], &(op)->vb[1][q], &(op)->vb[2][q]})
+/* VCCOMPUTEOP (VCAMRPoissonOpF.ChF:197-265) */
+static void k_vcop(Fab* lofphi, const Fab* phi, const Fab* a, Fab* const* b, const Box* r, double dx,
+                   double alpha, double beta) {
+  const double dxinv = 1.0/(dx*dx);
+  for (int k = r->lo[2]; k <= r->hi[2]; k++)
+    for (int j = r->lo[1]; j <= r->hi[1]; j++)
+      for (int i = r->lo[0]; i <= r->hi[0]; i++)
+        *P(lofphi,i,j,k) = vc_lofphi(phi, a, b, i, j, k, alpha, beta, dxinv);
+}
+/* VCCOMPUTERES (VCAMRPoissonOpF.ChF:311-382): res = rhs - L(phi) */
+static void k_vcres(Fab* res, const Fab* phi, const Fab* rhs, const Fab* a, Fab* const* b, const Box* r,
+                    double dx, double alpha, double beta) {
+  const double dxinv = 1.0/(dx*dx);
+  for (int k = r->lo[2]; k <= r->hi[2]; k++)
+    for (int j = r->lo[1]; j <= r->hi[1]; j++)
+      for (int i = r->lo[0]; i <= r->hi[0]; i++)
+        *P(res,i,j,k) = *P(rhs,i,j,k) - vc_lofphi(phi, a, b, i, j, k, alpha, beta, dxinv);
+}
+/* GSRBHELMHOLTZVC (VCAMRPoissonOpF.ChF:57-155): phi -= lambda*(L(phi) - rhs) on one colour */
+static void k_vcgsrb(Fab* phi, const Fab* rhs, const Fab* a, Fab* const* b, const Fab* lam, const Box* r,
+                     double dx, double alpha, double beta, int redBlack) {
+  const double dxinv = 1.0/(dx*dx);
+  for (int k = r->lo[2]; k <= r->hi[2]; k++)
+    for (int j = r->lo[1]; j <= r->hi[1]; j++) {
+      int imin = r->lo[0];
+      int indtot = imin + j + k;
+      imin = imin + abs((indtot + redBlack) % 2);
+      for (int i = imin; i <= r->hi[0]; i += 2) {
+        const double lofphi = vc_lofphi(phi, a, b, i, j, k, alpha, beta, dxinv);
+        *P(phi,i,j,k) = *P(phi,i,j,k) - *P(lam,i,j,k) * (lofphi - *P(rhs,i,j,k));
+      }
+    }
+}
+/* RESTRICTRESVC (VCAMRPoissonOpF.ChF:422-496): the coarse residual accumulates
+   (rhs - L(phi))/8 over the fine region in Fortran order (res zeroed first) */
+static void k_vcrestrictres(Fab* res, const Fab* phi, const Fab* rhs, const Fab* a, Fab* const* b,
+                            const Box* r, double dx, double alpha, double beta) {
+  const double dxinv = 1.0/(dx*dx);
+  const double denom = 2*2*2;
+  for (int k = r->lo[2]; k <= r->hi[2]; k++)
+    for (int j = r->lo[1]; j <= r->hi[1]; j++)
+      for (int i = r->lo[0]; i <= r->hi[0]; i++) {
+        const double lofphi = vc_lofphi(phi, a, b, i, j, k, alpha, beta, dxinv);
+        double* c = P(res, fdiv(r->lo[0], 2) + (i - r->lo[0])/2, fdiv(r->lo[1], 2) + (j - r->lo[1])/2,
+                      fdiv(r->lo[2], 2) + (k - r->lo[2])/2);
+        *c = *c + (*P(rhs,i,j,k) - lofphi) / denom;
+      }
+}
+/* resetLambda (VCAMRPoissonOp2.cpp:283-318) with SUMFACES (VCAMRPoissonOpF.ChF:498-525):
+   lambda = 1 / (alpha*a + sum_d scale*beta*(b_d(i+e_d) + b_d(i))) */
+static void k_vclambda(Fab* lam, const Fab* a, Fab* const* b, const Box* r, double dx, double alpha,
+                       double beta) {
+  const double scale = 1.0/(dx*dx);
+  for (int k = r->lo[2]; k <= r->hi[2]; k++)
+    for (int j = r->lo[1]; j <= r->hi[1]; j++)
+      for (int i = r->lo[0]; i <= r->hi[0]; i++) *P(lam,i,j,k) = *P(a,i,j,k) * alpha;
+  for (int d = 0; d < 3; d++) {
+    const int e0 = d == 0, e1 = d == 1, e2 = d == 2;
+    for (int k = r->lo[2]; k <= r->hi[2]; k++)
+      for (int j = r->lo[1]; j <= r->hi[1]; j++)
+        for (int i = r->lo[0]; i <= r->hi[0]; i++) {
+          const double sumVal = *P(b[d],i+e0,j+e1,k+e2) + *P(b[d],i,j,k);
+          *P(lam,i,j,k) = *P(lam,i,j,k) + scale * beta * sumVal;
+        }
+  }
+  for (int k = r->lo[2]; k <= r->hi[2]; k++)
+    for (int j = r->lo[1]; j <= r->hi[1]; j++)
+      for (int i = r->lo[0]; i <= r->hi[0]; i++) *P(lam,i,j,k) = 1.0 / *P(lam,i,j,k);
+}
+/* CoarseAverage::averageToCoarse -> average (AverageF.ChF:9-162): the r^3 children
+   summed with i fastest; refRatio 2 and 4 are unrolled sums multiplied by refScale,
+   other ratios a loop from zero times refScale */
+static void k_vcavg_cells(Fab* coarse, const Fab* fine, const Box* cb, int r) {
+  const double refScale = 1.0/(double)(r*r*r);
+  for (int K = cb->lo[2]; K <= cb->hi[2]; K++)
+    for (int J = cb->lo[1]; J <= cb->hi[1]; J++)
+      for (int I = cb->lo[0]; I <= cb->hi[0]; I++) {
+        double sum = 0.0;
+        int first = 1;
+        for (int kk = 0; kk < r; kk++)
+          for (int jj = 0; jj < r; jj++)
+            for (int ii = 0; ii < r; ii++) {
+              const double f = *P(fine, I*r+ii, J*r+jj, K*r+kk);
+              if (first && r <= 4) { sum = f; first = 0; }
+              else sum = sum + f;
+            }
+        *P(coarse,I,J,K) = (r <= 4) ? refScale * sum : sum * refScale;
+      }
+}
+/* CoarseAverageFace::averageGridData -> AVERAGEFACE (AverageFaceF.ChF): each coarse
+   face sums the r^2 fine faces on it (i fastest) from zero, times (1/r)^2 */
+static void k_vcavg_faces(Fab* coarse, const Fab* fine, const Box* cfb, int dir, int r) {
+  const double ref_scale = (1.0/r)*(1.0/r);
+  const int n0 = dir == 0 ? 1 : r, n1 = dir == 1 ? 1 : r, n2 = dir == 2 ? 1 : r;
+  for (int K = cfb->lo[2]; K <= cfb->hi[2]; K++)
+    for (int J = cfb->lo[1]; J <= cfb->hi[1]; J++)
+      for (int I = cfb->lo[0]; I <= cfb->hi[0]; I++) {
+        double crseSum = 0;
+        for (int kk = 0; kk < n2; kk++)
+          for (int jj = 0; jj < n1; jj++)
+            for (int ii = 0; ii < n0; ii++) crseSum = crseSum + *P(fine, I*r+ii, J*r+jj, K*r+kk);
+        *P(coarse,I,J,K) = ref_scale*crseSum;
+      }
+}
+static Box vc_facebox(Box b, int d) { b.hi[d] += 1; return b; }
+/* define the VC data of op (coefficients given) and its lambda */
+static void op_vc_alloc(Op* op) {
+  op->vc = 1;
+  op->va = ld_make(&op->L, 0);
+  op->vlam = ld_make(&op->L, 0);
+  for (int d = 0; d < 3; d++) {
+    op->vb[d] = calloc(op->L.n ? op->L.n : 1, sizeof(Fab));
+    for (int q = 0; q < op->L.n; q++) fab_alloc(&op->vb[d][q], vc_facebox(op->L.b[q], d));
+  }
+}
+static void op_vc_lambda(Op* op) {
+  for (int q = 0; q < op->L.n; q++)
+    k_vclambda(&op->vlam.f[q], &op->va.f[q], VCB(op, q), &op->L.b[q], op->dx, op->alpha, op->beta);
+}
+
+/* residualI (AMRPoissonOp.cpp:260-297; VCAMRPoissonOp2.cpp:30-87) */
 static void op_residualI(const Op* op, LD* lhs, LD* phi, const LD* rhs, int homog) {
   op_exchange(op, phi);
   op_bc(op, phi, homog);
   #pragma omp parallel for schedule(static)
-  for (int b = 0; b < op->L.n; b++)
-    k_oplapres(&lhs->f[b], &phi->f[b], &rhs->f[b], &op->L.b[b], op->dx, op->alpha, op->beta);
+  for (int b = 0; b < op->L.n; b++) {
+    if (op->vc)
+      k_vcres(&lhs->f[b], &phi->f[b], &rhs->f[b], &op->va.f[b], VCB(op, b), &op->L.b[b], op->dx, op->alpha,
+              op->beta);
+    else
+      k_oplapres(&lhs->f[b], &phi->f[b], &rhs->f[b], &op->L.b[b], op->dx, op->alpha, op->beta);
+  }
 }
 static void op_residual(const Op* op, LD* lhs, LD* phi, const LD* rhs, int homog) {
   if (homog) op_homog_cf(op, phi);
@@ -1128,8 +1277,12 @@ static void op_applyI(const Op* op, LD* lhs, LD* phi, int homog) {
   op_exchange(op, phi);
   op_bc(op, phi, homog);
   #pragma omp parallel for schedule(static)
-  for (int b = 0; b < op->L.n; b++)
-    k_oplap(&lhs->f[b], &phi->f[b], &op->L.b[b], op->dx, op->alpha, op->beta);
+  for (int b = 0; b < op->L.n; b++) {
+    if (op->vc)
+      k_vcop(&lhs->f[b], &phi->f[b], &op->va.f[b], VCB(op, b), &op->L.b[b], op->dx, op->alpha, op->beta);
+    else
+      k_oplap(&lhs->f[b], &phi->f[b], &op->L.b[b], op->dx, op->alpha, op->beta);
+  }
 }
 static void op_apply(const Op* op, LD* lhs, LD* phi, int homog) {
   op_homog_cf(op, phi);
@@ -1149,7 +1302,10 @@ static void op_levelGSRB(const Op* op, LD* phi, const LD* rhs) {
     #pragma omp parallel for schedule(static)
     for (int b = 0; b < op->L.n; b++) {
       apply_bc(&phi->f[b], &op->L.b[b], &op->domain, op->dx, 1, &op->bc);
-      if (op->alpha == 0.0 && op->beta == 1.0)
+      if (op->vc)  /* VCAMRPoissonOp2::levelGSRB (VCAMRPoissonOp2.cpp:547-626) */
+        k_vcgsrb(&phi->f[b], &rhs->f[b], &op->va.f[b], VCB(op, b), &op->vlam.f[b], &op->L.b[b], op->dx,
+                 op->alpha, op->beta, pass);
+      else if (op->alpha == 0.0 && op->beta == 1.0)
         k_gsrb_lap(&phi->f[b], &rhs->f[b], &op->L.b[b], op->dx, pass);
       else
         k_gsrb_helm(&phi->f[b], &rhs->f[b], &op->L.b[b], op->dx, op->alpha, op->beta, pass);
@@ -1167,7 +1323,11 @@ static void op_restrict(const Op* op, LD* resC, LD* phi, const LD* rhs) {
   #pragma omp parallel for schedule(static)
   for (int b = 0; b < op->L.n; b++) {
     fab_setval(&resC->f[b], resC->f[b].b, 0.0);
-    k_restrictres(&resC->f[b], &phi->f[b], &rhs->f[b], op->alpha, op->beta, &op->L.b[b], op->dx);
+    if (op->vc)  /* VCAMRPoissonOp2::restrictResidual (VCAMRPoissonOp2.cpp:195-256) */
+      k_vcrestrictres(&resC->f[b], &phi->f[b], &rhs->f[b], &op->va.f[b], VCB(op, b), &op->L.b[b], op->dx,
+                      op->alpha, op->beta);
+    else
+      k_restrictres(&resC->f[b], &phi->f[b], &rhs->f[b], op->alpha, op->beta, &op->L.b[b], op->dx);
   }
 }
 /* prolongIncrement (AMRPoissonOp.cpp:706-733) */
@@ -1275,6 +1435,18 @@ static double op_localMaxNorm(const LD* x) {
 }
 /* preCond (AMRPoissonOp.cpp:304-329) */
 static void op_precond(const Op* op, LD* phi, const LD* rhs) {
+  if (op->vc) {  /* VCAMRPoissonOp2::preCond (VCAMRPoissonOp2.cpp:93-127): phi = rhs * lambda */
+    for (int b = 0; b < op->L.n; b++) {
+      Box r = b_isect(phi->f[b].b, &rhs->f[b].b);
+      fab_copy_region(&phi->f[b], &rhs->f[b], r);
+      const Box* v = &op->L.b[b];
+      for (int k = v->lo[2]; k <= v->hi[2]; k++)
+        for (int j = v->lo[1]; j <= v->hi[1]; j++)
+          for (int i = v->lo[0]; i <= v->hi[0]; i++) *P(&phi->f[b],i,j,k) = *P(&phi->f[b],i,j,k) * *P(&op->vlam.f[b],i,j,k);
+    }
+    op_relax(op, phi, rhs, 2);
+    return;
+  }
   double mult = 1.0/(op->alpha - 2.0*3*op->beta/(op->dx*op->dx));
   for (int b = 0; b < op->L.n; b++) {
     Box r = b_isect(phi->f[b].b, &rhs->f[b].b);
@@ -1467,6 +1639,53 @@ void oracle_destroy(OHier* h) {
   for (int l = 0; l < h->nlev; l++) free(h->grids[l].b);
   free(h->grids); free(h->domains); free(h->dx); free(h->op); free(h->mgL); free(h->mgop);
   free(h->cfL); free(h->resCopier); free(h->revCopier); free(h);
+}
+/* VCAMRPoissonOp2Factory::define + MGnewOp (VCAMRPoissonOp2.cpp:868-1070) on a
+   one-level hierarchy: the given coefficients on level 0, and on MG depth d the
+   coefficients averaged from level 0 with ratio 2^d (CoarseAverage, CoarseAverageFace,
+   arithmetic), each op with its lambda. */
+static const Op* mg_op(const OHier* h, int d);
+int oracle_vc_set(OHier* h, const double* a, const double* b0, const double* b1, const double* b2) {
+  if (h->nlev != 1) { set_err("variable coefficients: one level only"); return -1; }
+  Op* op0 = &h->op[0];
+  if (!op0->vc) op_vc_alloc(op0);
+  ld_from_packed(&op0->va, a);
+  const double* bs[3] = {b0, b1, b2};
+  for (int d = 0; d < 3; d++) {
+    long o = 0;
+    for (int q = 0; q < op0->L.n; q++) {
+      Fab* f = &op0->vb[d][q];
+      long n = b_npts(&f->b);
+      memcpy(f->d, bs[d] + o, sizeof(double)*n); o += n;
+    }
+  }
+  op_vc_lambda(op0);
+  for (int dd = 1; dd < h->mgD; dd++) {
+    Op* op = &h->mgop[dd];
+    if (!op->vc) op_vc_alloc(op);
+    const int r = 1 << dd;
+    for (int q = 0; q < op->L.n; q++) {
+      k_vcavg_cells(&op->va.f[q], &op0->va.f[q], &op->L.b[q], r);
+      for (int d = 0; d < 3; d++) {
+        Box cfb = vc_facebox(op->L.b[q], d);
+        k_vcavg_faces(&op->vb[d][q], &op0->vb[d][q], &cfb, d, r);
+      }
+    }
+    op_vc_lambda(op);
+  }
+  return 0;
+}
+long oracle_face_cells(const OHier* h, int lev, int dir) {
+  long n = 0;
+  for (int q = 0; q < h->grids[lev].n; q++) { Box f = vc_facebox(h->grids[lev].b[q], dir); n += b_npts(&f); }
+  return n;
+}
+/* the lambda of MG depth d, valid-packed (a check of the coefficient coarsening) */
+int oracle_vc_lambda(OHier* h, int depth, double* out) {
+  const Op* op = mg_op(h, depth);
+  if (!op->vc) { set_err("no variable coefficients"); return -1; }
+  ld_to_packed(&op->vlam, out);
+  return 0;
 }
 long oracle_level_cells(const OHier* h, int lev) { return lay_cells(&h->grids[lev]); }
 int oracle_num_mg_depths(const OHier* h) { return h->mgD; }

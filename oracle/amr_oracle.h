@@ -85,6 +85,14 @@ long oracle_ghosted_cells(const OHier* h, int lev);
 int oracle_bottom_solve(OHier* h, double* corr, const double* res, int nbottom,
                         double metric, double eps, int* info);
 
+/* VCAMRPoissonOp2 on a one-level hierarchy: L phi = alpha a phi - beta div(b grad phi).
+   a valid-packed; b_d per box in layout order, the box's faces normal to d (the box
+   grown by one in d) in Fortran order.  Every later call on h uses the variable-
+   coefficient operator (MG depths get the coefficients averaged from level 0). */
+int  oracle_vc_set(OHier* h, const double* a, const double* b0, const double* b1, const double* b2);
+long oracle_face_cells(const OHier* h, int lev, int dir);
+int  oracle_vc_lambda(OHier* h, int depth, double* lambda_out);
+
 /* number of cell-relaxations (one red+black GSRB iteration of one cell) since reset */
 long long oracle_cell_relax_count(void);
 void oracle_reset_counters(void);

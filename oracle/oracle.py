@@ -55,6 +55,10 @@ def lib():
         L.oracle_average.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
         L.oracle_cell_relax_count.restype = C.c_longlong
         L.oracle_bottom_solve.argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.c_double, C.c_double, _ip]
+        L.oracle_vc_set.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+        L.oracle_face_cells.restype = C.c_long
+        L.oracle_face_cells.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.oracle_vc_lambda.argtypes = [C.c_void_p, C.c_int, _dp]
         _lib = L
     return _lib
 
@@ -186,6 +190,22 @@ class Oracle:
         self._chk(lib().oracle_bottom_solve(self.h, corr.ctypes.data_as(_dp), pr, nbottom, metric, eps,
                                             info.ctypes.data_as(_ip)))
         return corr, int(info[0])
+
+    def vc_set(self, a, b):
+        """Variable coefficients (VCAMRPoissonOp2) on a one-level hierarchy: a valid-packed,
+        b = [b0, b1, b2], each per box its faces normal to d in Fortran order."""
+        a, pa = _d(a)
+        bs = [_d(x) for x in b]
+        self._vc = (a, bs)  # keep the buffers alive for the call
+        self._chk(lib().oracle_vc_set(self.h, pa, bs[0][1], bs[1][1], bs[2][1]))
+
+    def face_cells(self, lev, d):
+        return lib().oracle_face_cells(self.h, lev, d)
+
+    def vc_lambda(self, depth):
+        out = np.zeros(self.mg_cells(depth))
+        self._chk(lib().oracle_vc_lambda(self.h, depth, out.ctypes.data_as(_dp)))
+        return out
 
     @staticmethod
     def cell_relax_count():
