@@ -42,6 +42,7 @@ def lib():
         L.amrmg_ctx_set_fused_relax.argtypes = [C.c_void_p, C.c_int]
         L.amrmg_profile_read_units.argtypes = [C.c_void_p, C.c_int, _dp]
         L.amrmg_solver_set_alpha_beta.argtypes = [C.c_void_p, C.c_double, C.c_double]
+        L.amrmg_solver_set_vc_coefs.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
         L.amrmg_serve_create.argtypes = [C.c_int, C.c_int, C.c_longlong, C.c_int, C.POINTER(C.c_void_p)]
         L.amrmg_serve_set_params.argtypes = [C.c_void_p, _ip, _dp]
         L.amrmg_serve_submit.argtypes = [C.c_void_p, _ip, _dp, _dp, C.POINTER(C.c_longlong)]
@@ -233,6 +234,22 @@ class AMRMultiGrid:
     def set_alpha_beta(self, alpha, beta):
         """AMRPoissonOp::setAlphaAndBeta on every operator of the solver."""
         _chk(lib().amrmg_solver_set_alpha_beta(self.p, float(alpha), float(beta)))
+
+    def set_vc_coefs(self, a, b):
+        """Variable coefficients (VCAMRPoissonOp2Factory::define on a one-level
+        hierarchy): a valid-packed, b = [b0, b1, b2], each per box its faces normal
+        to d (the box grown by one face in d) in Fortran order.  a=None returns to
+        constant coefficients."""
+        if a is None:
+            _chk(lib().amrmg_solver_set_vc_coefs(self.p, None, None, None, None))
+            return
+        a, pa = _d(a)
+        if a.size != self.ncells:
+            raise ValueError("aCoef size does not match the hierarchy")
+        bs = [_d(x) for x in b]
+        if len(bs) != 3:
+            raise ValueError("bCoef needs three directions")
+        _chk(lib().amrmg_solver_set_vc_coefs(self.p, pa, bs[0][1], bs[1][1], bs[2][1]))
 
     def setSolverParameters(self, sparams, rparams):
         sp = np.ascontiguousarray(sparams, dtype=np.int32)

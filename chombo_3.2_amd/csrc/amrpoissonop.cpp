@@ -58,31 +58,40 @@ template <class Real>
 void AMRPoissonOp<Real>::residual(T& lhs, T& phi, const T& rhs, bool homog) {
   need(lhs.lay, m_lay, "residual"); need(rhs.lay, m_lay, "residual");
   fillGhosts(phi, homog, homog);
-  ops::residual<Real>(*m_lay, lhs.d, phi.d, rhs.d, m_dx, m_alpha, m_beta);
+  if (m_vc) vc::residual<Real>(*m_lay, *m_vc, lhs.d, phi.d, rhs.d, m_dx, m_alpha, m_beta);
+  else ops::residual<Real>(*m_lay, lhs.d, phi.d, rhs.d, m_dx, m_alpha, m_beta);
 }
 template <class Real>
 void AMRPoissonOp<Real>::residualI(T& lhs, T& phi, const T& rhs, bool homog) {
   need(lhs.lay, m_lay, "residualI"); need(rhs.lay, m_lay, "residualI");
   fillGhosts(phi, false, homog);
-  ops::residual<Real>(*m_lay, lhs.d, phi.d, rhs.d, m_dx, m_alpha, m_beta);
+  if (m_vc) vc::residual<Real>(*m_lay, *m_vc, lhs.d, phi.d, rhs.d, m_dx, m_alpha, m_beta);
+  else ops::residual<Real>(*m_lay, lhs.d, phi.d, rhs.d, m_dx, m_alpha, m_beta);
 }
 // applyOp (:332-340) / applyOpI (:343-385)
 template <class Real>
 void AMRPoissonOp<Real>::applyOp(T& lhs, T& phi, bool homog) {
   need(lhs.lay, m_lay, "applyOp");
   fillGhosts(phi, true, homog);
-  ops::apply<Real>(*m_lay, lhs.d, phi.d, m_dx, m_alpha, m_beta);
+  if (m_vc) vc::apply<Real>(*m_lay, *m_vc, lhs.d, phi.d, m_dx, m_alpha, m_beta);
+  else ops::apply<Real>(*m_lay, lhs.d, phi.d, m_dx, m_alpha, m_beta);
 }
 template <class Real>
 void AMRPoissonOp<Real>::applyOpI(T& lhs, T& phi, bool homog) {
   need(lhs.lay, m_lay, "applyOpI");
   fillGhosts(phi, false, homog);
-  ops::apply<Real>(*m_lay, lhs.d, phi.d, m_dx, m_alpha, m_beta);
+  if (m_vc) vc::apply<Real>(*m_lay, *m_vc, lhs.d, phi.d, m_dx, m_alpha, m_beta);
+  else ops::apply<Real>(*m_lay, lhs.d, phi.d, m_dx, m_alpha, m_beta);
 }
 // preCond (:304-329)
 template <class Real>
 void AMRPoissonOp<Real>::preCond(T& phi, const T& rhs) {
   need(phi.lay, m_lay, "preCond"); need(rhs.lay, m_lay, "preCond");
+  if (m_vc) {  // VCAMRPoissonOp2::preCond (VCAMRPoissonOp2.cpp:93-127): phi = rhs * lambda, then relax
+    vc::precond_init<Real>(*m_lay, *m_vc, phi.d, rhs.d);
+    relax(phi, rhs, 2);
+    return;
+  }
   double mult = 1.0 / (m_alpha - 2.0 * 3 * m_beta / (m_dx * m_dx));
   ops::precond_init<Real>(*m_lay, phi.d, rhs.d, mult);
   relax(phi, rhs, 2);
@@ -142,6 +151,19 @@ void AMRPoissonOp<Real>::localMaxNormAsync(const T& x, int slot, bool accumulate
   ops::maxnorm_async<Real>(*m_lay, x.d, slot, accumulate ? 1 : 0);
 }
 
+// ---- variable coefficients ----
+template <class Real>
+void AMRPoissonOp<Real>::setVC(std::shared_ptr<VCCoefs<Real>> c) {
+  if (c && c->lay != m_lay) throw std::runtime_error("setVC: coefficients belong to another level");
+  m_vc = std::move(c);
+  if (m_vc) resetLambda();
+}
+// computeLambda (VCAMRPoissonOp2.cpp:283-318)
+template <class Real>
+void AMRPoissonOp<Real>::resetLambda() {
+  if (m_vc) vc::lambda<Real>(*m_vc, m_dx, m_alpha, m_beta);
+}
+
 // ---- multigrid ----
 template <class Real>
 void AMRPoissonOp<Real>::createCoarser(T& coarse, const T& fine, bool) {
@@ -153,6 +175,14 @@ void AMRPoissonOp<Real>::createCoarser(T& coarse, const T& fine, bool) {
 template <class Real>
 void AMRPoissonOp<Real>::levelGSRB(T& phi, const T& rhs) {
   need(rhs.lay, m_lay, "levelGSRB");
+  if (m_vc) {  // VCAMRPoissonOp2::levelGSRB (VCAMRPoissonOp2.cpp:547-626)
+    need(phi.lay, m_lay, "levelGSRB");
+    for (int pass = 0; pass <= 1; pass++) {
+      fillGhosts(phi, true, true);
+      vc::gsrb<Real>(*m_lay, *m_vc, phi.d, rhs.d, pass, m_dx, m_alpha, m_beta);
+    }
+    return;
+  }
   if (m_fused && m_ctx->fused_relax) {
     need(phi.lay, m_lay, "levelGSRB");
     if (!m_relaxTmp.isDefined()) m_relaxTmp.define(m_lay);
@@ -168,7 +198,7 @@ void AMRPoissonOp<Real>::levelGSRB(T& phi, const T& rhs) {
 }
 template <class Real>
 bool AMRPoissonOp<Real>::tuneRelax(const T& phi, const T& rhs, double* times) {
-  if (!m_fused) return false;
+  if (!m_fused || m_vc) return false;
   need(phi.lay, m_lay, "tuneRelax"); need(rhs.lay, m_lay, "tuneRelax");
   if (!m_relaxTmp.isDefined()) m_relaxTmp.define(m_lay);
   const std::pair<int, int> pick =
@@ -182,7 +212,7 @@ bool AMRPoissonOp<Real>::tuneRelax(const T& phi, const T& rhs, double* times) {
 // dropped; ghost cells are refilled before any later read, as after any relax.
 template <class Real>
 void AMRPoissonOp<Real>::relaxFromZero(T& e, const T& r, int n) {
-  if (!(m_fused && m_ctx->fused_relax) || n <= 0) {
+  if (!(m_fused && m_ctx->fused_relax) || m_vc || n <= 0) {
     setToZero(e);
     relax(e, r, n);
     return;
@@ -203,7 +233,10 @@ template <class Real>
 void AMRPoissonOp<Real>::restrictResidual(T& resC, T& phi, const T& rhs) {
   need(rhs.lay, m_lay, "restrictResidual"); need(resC.lay, m_mgCoarse, "restrictResidual");
   fillGhosts(phi, true, true);
-  ops::restrict_res<Real>(*m_lay, *m_mgCoarse, resC.d, phi.d, rhs.d, m_dx, m_alpha, m_beta);
+  if (m_vc)  // VCAMRPoissonOp2::restrictResidual (VCAMRPoissonOp2.cpp:195-256)
+    vc::restrict_res<Real>(*m_lay, *m_mgCoarse, *m_vc, resC.d, phi.d, rhs.d, m_dx, m_alpha, m_beta);
+  else
+    ops::restrict_res<Real>(*m_lay, *m_mgCoarse, resC.d, phi.d, rhs.d, m_dx, m_alpha, m_beta);
 }
 // prolongIncrement (:706-733)
 template <class Real>
@@ -299,6 +332,8 @@ void AMRPoissonOp<Real>::zeroCovered(T& lhs) {
 template <class Real>
 void AMRPoissonOp<Real>::reflux(T& phiFine, T& phi, T& res, AMRPoissonOp* finerOp) {
   need(phiFine.lay, m_finerLay, "reflux"); need(phi.lay, m_lay, "reflux"); need(res.lay, m_lay, "reflux");
+  if (m_vc || finerOp->m_vc)
+    throw std::runtime_error("reflux: variable coefficients are supported on one-level hierarchies only");
   finerOp->coarseFineInterp(phiFine, phi);
   double a_scale = 1.0 / m_dx;
   ops::reflux<Real>(m_ctx, d_rcell, n_rcell, d_rc, d_rf, res.d, phi.d, phiFine.d, m_beta * 1 / m_dx,

@@ -12,6 +12,7 @@
 #include "ops.hpp"
 #include "plans.hpp"
 #include "solver.hpp"
+#include "vc.hpp"
 
 namespace amrmg {
 
@@ -48,6 +49,12 @@ class AMRPoissonOp : public AMRLevelOp<LevelData<Real>> {
   int n_rcell = 0;
   const ops::MotionPlan* m_zeroPlan = nullptr; // finer op's m_resCopier (cfLay(finer) -> this)
   const DeviceLayout* m_finerCfLay = nullptr;
+  // Variable coefficients (VCAMRPoissonOp2, one level): when set, the operator is
+  // alpha a phi - beta div(b grad phi) with these coefficients and relaxes with
+  // GSRBHELMHOLTZVC; the one-launch constant-coefficient relaxation is not used.
+  std::shared_ptr<VCCoefs<Real>> m_vc;
+  void setVC(std::shared_ptr<VCCoefs<Real>> c);   // also computes lambda (computeLambda)
+  void resetLambda();                              // after alpha, beta or the coefficients change
 
   AMRPoissonOp() = default;
   ~AMRPoissonOp() override;

@@ -1490,11 +1490,12 @@ static BatchCoef bottom_coef(const AMRPoissonOp<T>* ap) {
 template <class T>
 void DeviceBottomSolver<T>::define(LinearOp<LevelData<T>>* op, bool homogeneous) {
   m_fallback.define(op, homogeneous);
-  if (m_device && op == m_boundOp && homogeneous) return;  // MultiGrid re-binds its bottom at every solve
+  auto* ap = dynamic_cast<AMRPoissonOp<T>*>(op);
+  const bool vc = ap && ap->m_vc;  // variable coefficients: the host-driven solve
+  if (m_device && op == m_boundOp && homogeneous && !vc) return;  // MultiGrid re-binds its bottom at every solve
   m_device = false;
   m_boundOp = op;
-  auto* ap = dynamic_cast<AMRPoissonOp<T>*>(op);
-  if (!homogeneous || !ap || !ap->m_lay->merged || m_ctx->dry) return;
+  if (!homogeneous || !ap || vc || !ap->m_lay->merged || m_ctx->dry) return;
   release();
   const DeviceLayout& L = *ap->m_lay;
   m_lay = &L;
@@ -1559,6 +1560,8 @@ void DeviceBottomSolver<T>::setConvergenceMetrics(double metric, double tol) {
 }
 template <class T>
 void DeviceBottomSolver<T>::solve(LevelData<T>& phi, const LevelData<T>& rhs) {
+  if (m_device && static_cast<const AMRPoissonOp<T>*>(m_ap)->m_vc)
+    throw std::runtime_error("DeviceBottomSolver: variable coefficients set after define");
   if (!m_device) {
     m_fallback.solve(phi, rhs);
     return;

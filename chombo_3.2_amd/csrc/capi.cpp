@@ -74,6 +74,9 @@ struct amrmg_solver {
   double rp[3] = {1e-10, 1e-15, 1e-30};
   long long table_bytes = 0;
   std::unique_ptr<MixedInner> mixed;  // precision 1: FP32 inner cycles
+  // variable coefficients per multigrid depth (empty: constant coefficients)
+  std::vector<std::shared_ptr<VCCoefs<double>>> vc;
+  std::vector<std::shared_ptr<VCCoefs<float>>> vc32;
   // Per-box storage for the FAB-level entry points (amrmg_kernel_gsrb,
   // amrmg_homog_cf, amrmg_quad_cf), whose ghost cells are caller inputs like the
   // ghost cells of the reference's FArrayBoxes; built on first use.
@@ -106,6 +109,10 @@ struct amrmg_solver {
     return static_cast<AMRPoissonOp<double>*>(amg.m_mg0->m_op[depth]);
   }
   const DeviceLayout* mglay(int depth) { return depth == 0 ? H.amrLay[0].get() : H.mgLay[depth - 1].get(); }
+  AMRPoissonOp<float>* mixedop(int depth) {
+    if (depth == 0) return mixed->amg.m_op[0].get();
+    return static_cast<AMRPoissonOp<float>*>(mixed->amg.m_mg0->m_op[depth]);
+  }
 };
 
 // ---- host <-> device transfers of valid-packed / ghosted-packed data ----
@@ -263,6 +270,48 @@ int amrmg_solver_set_alpha_beta(amrmg_solver* s, double alpha, double beta) {
   s->fabOps.clear();
   s->fabFac.reset();
   s->fabH.reset();
+  for (int d = 0; d < (int)s->vc.size(); d++) s->mgop(d)->resetLambda();
+  if (s->mixed)
+    for (int d = 0; d < (int)s->vc32.size(); d++) s->mixedop(d)->resetLambda();
+  API_END
+}
+int amrmg_solver_set_vc_coefs(amrmg_solver* s, const double* a, const double* b0, const double* b1,
+                              const double* b2) {
+  API_BEGIN
+  const int nd = s->H.mgDepth();
+  Context* c = s->ctx->c.get();
+  if (!a) {  // back to constant coefficients
+    for (int d = 0; d < nd; d++) s->mgop(d)->setVC(nullptr);
+    if (s->mixed)
+      for (int d = 0; d < nd; d++) s->mixedop(d)->setVC(nullptr);
+    s->vc.clear();
+    s->vc32.clear();
+  } else {
+    if (s->nlev() != 1) throw std::runtime_error("variable coefficients: one-level solvers only");
+    if (!b0 || !b1 || !b2) throw std::runtime_error("variable coefficients: bCoef arrays missing");
+    std::vector<std::shared_ptr<VCCoefs<double>>> v;
+    for (int d = 0; d < nd; d++) v.emplace_back(new VCCoefs<double>(s->mglay(d)));
+    v[0]->setCells(a);
+    const double* bs[3] = {b0, b1, b2};
+    for (int q = 0; q < 3; q++) v[0]->setFaces(q, bs[q]);
+    for (int d = 1; d < nd; d++) vc::average_coefs<double>(*v[0], *v[d], 1 << d);
+    for (int d = 0; d < nd; d++) s->mgop(d)->setVC(v[d]);
+    s->vc = v;
+    s->vc32.clear();
+    if (s->mixed) {
+      for (int d = 0; d < nd; d++) {
+        std::shared_ptr<VCCoefs<float>> f(new VCCoefs<float>(s->mglay(d)));
+        vc::narrow(*v[d], *f);
+        s->mixedop(d)->setVC(f);
+        s->vc32.push_back(f);
+      }
+    }
+  }
+  // the bottom of the multigrid: the device solver for constant coefficients, the
+  // host-driven BiCGStab otherwise (both bind again here)
+  s->amg.m_mg0->setBottomSolver(s->bottom.get());
+  if (s->mixed) s->mixed->rebindBottom();
+  c->sync();
   API_END
 }
 long long amrmg_solver_level_cells(amrmg_solver* s, int lev) {
@@ -286,6 +335,8 @@ long long amrmg_solver_device_bytes(amrmg_solver* s) {
   for (auto& m : s->H.mgLay) mg += m->slab;
   total += mg * 3 * 8;
   if (s->mixed) total += n * 6 * 4 + mg * 3 * 4;
+  for (auto& v : s->vc) total += v->bytes();
+  for (auto& v : s->vc32) total += v->bytes();
   return total;
 }
 

@@ -84,6 +84,14 @@ struct MixedInner {
                                 int l_base) { cycle(uc, ur, l_max, l_base); };
     amg64.m_innerBottomMetrics = [this](double metric, double tol) { active->setConvergenceMetrics(metric, tol); };
   }
+  // bind the bottom again after the operators' coefficients change: the device
+  // solver when it applies to the FP32 bottom level, the promoted BiCGStab otherwise
+  void rebindBottom() {
+    devBottom->define(amg.m_mg0->m_op.back(), true);
+    active = devBottom->onDevice() ? (LinearSolver<LevelData<float>>*)devBottom.get() : &bottom;
+    amg.m_bottomSolver = active;
+    amg.m_mg0->setBottomSolver(active);
+  }
   long long bottomIterations() {
     return active == &bottom ? bottom.m_bicg.m_total_iters : devBottom->totalIterations();
   }

@@ -65,6 +65,20 @@ int amrmg_solver_set_params(amrmg_solver* s, const int* sparams, const double* r
    every operator of the solver (AMR levels and multigrid levels): the same as defining the solver
    with these coefficients. */
 int amrmg_solver_set_alpha_beta(amrmg_solver* s, double alpha, double beta);
+/* Variable coefficients: the solver's operator becomes VCAMRPoissonOp2's
+   alpha a(x) phi - beta div(b(x) grad phi) (VCAMRPoissonOp2Factory::define,
+   VCAMRPoissonOp2.cpp:868-940, with MGnewOp's arithmetic coefficient averaging,
+   :991-1070, and computeLambda, :283-318).  One-level solvers only.  a is
+   valid-packed; b_d (d = 0, 1, 2) holds, box after box, the faces normal to d of
+   the box grown by one face in d, in Fortran order (the packing of a FluxBox
+   component).  On a merged level, faces shared by two boxes must carry the same
+   value in both.  alpha and beta are the solver's (amrmg_solver_set_alpha_beta
+   changes them and recomputes lambda).  a = NULL returns to constant
+   coefficients.  With precision 1 the inner FP32 cycles use the coefficients
+   rounded to FP32 (averaged in FP64 first).  The bottom solve is the
+   host-driven BiCGStab. */
+int amrmg_solver_set_vc_coefs(amrmg_solver* s, const double* a, const double* b0, const double* b1,
+                              const double* b2);
 long long amrmg_solver_level_cells(amrmg_solver* s, int lev);
 int amrmg_solver_num_mg_depths(amrmg_solver* s);
 long long amrmg_solver_mg_cells(amrmg_solver* s, int depth);

@@ -1,0 +1,188 @@
+"""Variable-coefficient operator (VCAMRPoissonOp2) on the HIP path against the oracle.
+
+The kernels restate GSRBHELMHOLTZVC, VCCOMPUTEOP, VCCOMPUTERES, RESTRICTRESVC and
+SUMFACES in the reference's expression order (no FMA contraction), and the
+multigrid coefficients are averaged in CoarseAverage / CoarseAverageFace order,
+so level operations at every depth and whole FP64 solves are compared bitwise.
+Mixed-precision solves are held to 1e-9 relative of the FP64 oracle's solution
+(ten times the 1e-10 solve tolerance), as for constant coefficients.
+"""
+import numpy as np
+import pytest
+
+import amrmg
+import hierarchy as H
+from helpers import small_amr, small_single
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "vc32": dict(n=32, box=16, alpha=0.5, beta=1.0),
+    "vc32_negidx": dict(n=32, box=16, lo=-16, alpha=0.0, beta=1.0),
+    "vc40_ragged": dict(n=40, box=16, alpha=1.5, beta=0.8),
+    "vc16_neumann": dict(n=16, box=8, bc=(0, 0, 0), alpha=1.0, beta=1.0),
+}
+
+
+def _pair(gpu_ctx, name, seed=5, precision=0):
+    h = small_single(**CASES[name])
+    a, b = H.vc_coefficients(h, seed=seed)
+    g = amrmg.AMRMultiGrid(gpu_ctx, h, precision=precision)
+    g.set_vc_coefs(a, b)
+    o = Oracle(h)
+    o.vc_set(a, b)
+    return h, g, o
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_vc_level_kernels_bitwise(gpu_ctx, case):
+    print(f"vc level kernels {case}", flush=True)
+    h, g, o = _pair(gpu_ctx, case)
+    rng = np.random.default_rng(17)
+    assert g.mg_depths == o.mg_depths
+    for d in range(o.mg_depths):
+        n = o.mg_cells(d)
+        phi, rhs = rng.standard_normal(n), rng.standard_normal(n)
+        for homog in (False, True):
+            np.testing.assert_array_equal(g.level_apply(d, phi, homog), o.level_apply(d, phi, homog))
+            np.testing.assert_array_equal(g.level_residual(d, phi, rhs, homog), o.level_residual(d, phi, rhs, homog))
+        for iters in (1, 2):
+            np.testing.assert_array_equal(g.level_relax(d, phi, rhs, iters), o.level_relax(d, phi, rhs, iters))
+        if d + 1 < o.mg_depths:
+            np.testing.assert_array_equal(g.level_restrict(d, phi, rhs), o.level_restrict(d, phi, rhs))
+    g.close()
+
+
+def test_vc_bottom_solve_bitwise(gpu_ctx):
+    print("vc bottom solve", flush=True)
+    h, g, o = _pair(gpu_ctx, "vc32")
+    d = o.mg_depths - 1
+    rng = np.random.default_rng(3)
+    res = rng.standard_normal(o.mg_cells(d))
+    c0 = np.zeros_like(res)
+    cg, sg = g.bottom_solve(c0, res, 2, 1.0, 1e-10)
+    co, so = o.bottom_solve(c0, res, 2, 1.0, 1e-10)
+    assert sg == so
+    np.testing.assert_array_equal(cg, co)
+    g.close()
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_vc_solve_matches_oracle_bitwise(gpu_ctx, case):
+    print(f"vc solve {case}", flush=True)
+    h, g, o = _pair(gpu_ctx, case, seed=8)
+    rhs = H.sinusoid_rhs(h, 1005, k=(1, 2, 1))
+    sp, rp = H.solver_params()
+    g.setSolverParameters(sp, rp)
+    g.set_field(g.RHS, rhs)
+    hist_g, info_g = g.solve()
+    phi_g = g.get_field(g.PHI)
+    phi_o, hist_o, info_o = o.solve(rhs, sp, rp)
+    assert info_g["iterations"] == info_o["iterations"]
+    assert info_g["exit_status"] == info_o["exit_status"]
+    assert info_g["exit_status"] & 1
+    np.testing.assert_array_equal(hist_g, hist_o)
+    np.testing.assert_array_equal(phi_g, phi_o)
+    g.close()
+
+
+@pytest.mark.parametrize("case", ["vc32", "vc40_ragged"])
+def test_vc_mixed_precision_solve(gpu_ctx, case):
+    print(f"vc mixed {case}", flush=True)
+    h, g, o = _pair(gpu_ctx, case, seed=8, precision=1)
+    rhs = H.sinusoid_rhs(h, 1005, k=(1, 2, 1))
+    sp, rp = H.solver_params()
+    g.setSolverParameters(sp, rp)
+    g.set_field(g.RHS, rhs)
+    hist, info = g.solve()
+    phi = g.get_field(g.PHI)
+    phi_o, hist_o, info_o = o.solve(rhs, sp, rp)
+    err = np.abs(phi - phi_o).max() / np.abs(phi_o).max()
+    print(f"{case}: mixed {info['iterations']} it, fp64 {info_o['iterations']} it, err {err:.2e}")
+    assert hist[0] == hist_o[0]
+    assert info["exit_status"] & 1 and hist[-1] <= 1e-10 * hist[0]
+    assert abs(info["iterations"] - info_o["iterations"]) <= 2
+    assert err < 1e-9, err
+    g.close()
+
+
+def test_vc_unit_coefficients_match_constant_solver(gpu_ctx):
+    """a = b = 1 with beta -> -beta is the constant-coefficient operator: the two
+    GPU solves agree to rounding (their expression orders differ)."""
+    print("vc unit coefficients", flush=True)
+    hc = small_single(n=32, box=16, alpha=0.6, beta=1.7)
+    hv = small_single(n=32, box=16, alpha=0.6, beta=-1.7)
+    rhs = H.sinusoid_rhs(hc, 1006)
+    sp, rp = H.solver_params()
+    c = amrmg.AMRMultiGrid(gpu_ctx, hc)
+    v = amrmg.AMRMultiGrid(gpu_ctx, hv)
+    o = Oracle(hv)
+    v.set_vc_coefs(np.ones(hv.cells(0)), [np.ones(o.face_cells(0, d)) for d in range(3)])
+    out = []
+    for s in (c, v):
+        s.setSolverParameters(sp, rp)
+        s.set_field(s.RHS, rhs)
+        s.solve()
+        out.append(s.get_field(s.PHI))
+        s.close()
+    assert np.abs(out[0] - out[1]).max() < 1e-9 * np.abs(out[0]).max()
+
+
+def test_vc_clear_and_alpha_beta_equal_fresh_solvers(gpu_ctx):
+    """Coefficients set on a solver that has solved, then alpha and beta changed
+    (lambda recomputed at every depth), give the fresh solver's solve bit for
+    bit; clearing them returns to the constant-coefficient solve (device bottom
+    solver included)."""
+    print("vc reset", flush=True)
+    h1 = small_single(n=32, box=16, alpha=0.4, beta=1.3)
+    h2 = small_single(n=32, box=16, alpha=0.9, beta=0.6)
+    a, b = H.vc_coefficients(h1, seed=12)
+    rhs = H.sinusoid_rhs(h1, 1010)
+    sp, rp = H.solver_params()
+
+    def run(s):
+        s.setSolverParameters(sp, rp)
+        s.set_field(s.RHS, rhs)
+        hist, info = s.solve()
+        return hist, info, s.get_field(s.PHI)
+
+    g = amrmg.AMRMultiGrid(gpu_ctx, h1)
+    base = run(g)
+    g.set_vc_coefs(a, b)
+    run(g)
+    g.set_alpha_beta(0.9, 0.6)
+    got = run(g)
+    f = amrmg.AMRMultiGrid(gpu_ctx, h2)
+    f.set_vc_coefs(a, b)
+    ref = run(f)
+    f.close()
+    assert got[1] == ref[1]
+    np.testing.assert_array_equal(got[0], ref[0])
+    np.testing.assert_array_equal(got[2], ref[2])
+    g.set_alpha_beta(0.4, 1.3)
+    g.set_vc_coefs(None, None)
+    back = run(g)
+    assert back[1] == base[1]
+    np.testing.assert_array_equal(back[2], base[2])
+    g.close()
+
+
+def test_vc_rejects_bad_input(gpu_ctx):
+    print("vc errors", flush=True)
+    h = small_single(n=32, box=16)
+    g = amrmg.AMRMultiGrid(gpu_ctx, h)
+    a, b = H.vc_coefficients(h)
+    # box 0 is [0, 15]^3 and box 1 starts at z = 16: face (0, 0, 16) normal to z is
+    # the last plane of box 0's z faces and the first of box 1's
+    assert h.levels[0][0] == ((0, 0, 0), (15, 15, 15)) and h.levels[0][1][0] == (0, 0, 16)
+    bad = b[2].copy()
+    bad[16 * 16 * 16] += 1.0
+    with pytest.raises(amrmg.AMRMGError, match="shared by two boxes"):
+        g.set_vc_coefs(a, [b[0], b[1], bad])
+    g.close()
+    ha = small_amr()
+    ga = amrmg.AMRMultiGrid(gpu_ctx, ha)
+    with pytest.raises(amrmg.AMRMGError, match="one-level"):
+        ga.set_vc_coefs(np.ones(ga.ncells), [np.ones(1)] * 3)
+    ga.close()
