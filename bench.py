@@ -169,6 +169,54 @@ def mixed_leg(args, ctx, hier, rhs, sp, rp, solver64, fp64_s_per_step):
     return out
 
 
+def vc_leg(args, ctx):
+    """Variable coefficients (VCAMRPoissonOp2): one level of base^3 cells in 64^3
+    boxes, -div(b grad phi) = rhs with a smooth positive face coefficient b of
+    contrast 4, solved to 1e-10 in FP64 and in mixed precision.  The GSRB class's
+    algorithmic bytes are 64 B per cell-relax in FP64 (phi read and written, rhs,
+    aCoef, lambda and the three face arrays read once per red-black iteration)."""
+    import amrmg
+    import hierarchy as H
+    t0 = time.time()
+    h = H.single_level(args.base, 64, 8, alpha=0.0, beta=1.0)
+    a, b = H.vc_coefficients(h, seed=7, contrast=4.0)
+    rhs = H.sinusoid_rhs(h, 1007, k=(1, 2, 1))
+    log(f"vc: {len(h.levels[0])} boxes, {h.cells(0)} cells, coefficients in {time.time() - t0:.1f}s")
+    sp, rp = H.solver_params(eps=1e-10, hang=1e-15, iterMax=args.iter_max)
+    out = {"workload": f"one level {args.base}^3, 64^3 boxes, Dirichlet 0, alpha=0 beta=1, "
+                       f"b = 1 + 3(0.5 + 0.5 sin cos cos) on faces, a smooth, tol 1e-10"}
+    for prec in (0, 1):
+        s = amrmg.AMRMultiGrid(ctx, h, precision=prec)
+        s.setSolverParameters(sp, rp)
+        s.set_vc_coefs(a, b)
+        s.set_field(s.RHS, rhs)
+        for _ in range(args.warmup):
+            s.solve()
+        ctx.profile(["gsrb"])
+        ctx.profile_reset()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            hist, info = s.solve()
+        wall = time.perf_counter() - t1
+        prof = ctx.profile_read("gsrb")
+        ctx.profile([])
+        key = "fp64" if prec == 0 else "mixed"
+        out[key] = {
+            "ms_per_step": round(wall / args.steps * 1e3, 3),
+            "solves_per_s": round(args.steps / wall, 4),
+            "iterations": info["iterations"],
+            "final_rel_residual": float(hist[-1] / hist[0]),
+            "mcell_relax_per_s": round(prof["units"] / wall / 1e6, 3),
+            "relax_achieved_GBs": round(prof["bytes"] / (prof["ms"] * 1e-3) / 1e9, 1) if prof["ms"] > 0 else None,
+            "relax_avg_launch_ms": round(prof["ms"] / max(prof["launches"], 1), 5),
+            "relax_share_of_solve": round(prof["ms"] * 1e-3 / wall, 3),
+        }
+        log(f"vc {key}: {out[key]}")
+        s.close()
+    out["mixed_speedup"] = round(out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"], 3)
+    return out
+
+
 def batch_leg(args, ctx):
     """The batched workload: independent single-level Helmholtz problems of edge
     16..128 (log-uniform), solved (a) one after another with the plain FP64 solver,
@@ -355,6 +403,7 @@ def main():
                     help="cap on V-cycles per solve (a short cap is for profiling runs only)")
     ap.add_argument("--no-tune", action="store_true", help="skip the relaxation autotuner (default shapes)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--no-vc", action="store_true", help="skip the variable-coefficient leg")
     ap.add_argument("--batch", type=int, default=512, help="problems in the batched leg (0 = skip)")
     ap.add_argument("--batch-lanes", type=int, default=3, choices=(1, 2, 3),
                     help="2 or 3: the problems with large multigrid bottoms lockstep on further streams")
@@ -430,6 +479,10 @@ def main():
     mixed = None
     if not args.no_mixed:
         mixed = mixed_leg(args, ctx, hier, rhs, sp, rp, solver, wall / args.steps)
+    vcout = None
+    if not args.no_vc:
+        solver.close()
+        vcout = vc_leg(args, ctx)
     batch = None
     if args.batch > 0:
         solver.close()
@@ -492,6 +545,7 @@ def main():
         "relax_tiles": ({f"{r['kind']}{r['index']}": f"{r['shape'][0]}x{r['shape'][1]}/kc{r['kc']}" for r in tune}
                         if tune else None),
         "mixed_precision": mixed,
+        "variable_coefficients": vcout,
         "batched": batch,
         "serving": serve,
         "cpu_baseline": cpu,

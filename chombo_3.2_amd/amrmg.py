@@ -235,6 +235,10 @@ class AMRMultiGrid:
         """AMRPoissonOp::setAlphaAndBeta on every operator of the solver."""
         _chk(lib().amrmg_solver_set_alpha_beta(self.p, float(alpha), float(beta)))
 
+    def set_vc_relax_shape(self, shape):
+        """Tile shape of the one-launch variable-coefficient relaxation (-1 automatic, 0..3)."""
+        _chk(lib().amrmg_solver_set_vc_relax_shape(self.p, int(shape)))
+
     def set_vc_coefs(self, a, b):
         """Variable coefficients (VCAMRPoissonOp2Factory::define on a one-level
         hierarchy): a valid-packed, b = [b0, b1, b2], each per box its faces normal

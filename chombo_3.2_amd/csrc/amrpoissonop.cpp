@@ -177,6 +177,12 @@ void AMRPoissonOp<Real>::levelGSRB(T& phi, const T& rhs) {
   need(rhs.lay, m_lay, "levelGSRB");
   if (m_vc) {  // VCAMRPoissonOp2::levelGSRB (VCAMRPoissonOp2.cpp:547-626)
     need(phi.lay, m_lay, "levelGSRB");
+    if (m_fused && m_ctx->fused_relax) {
+      if (!m_relaxTmp.isDefined()) m_relaxTmp.define(m_lay);
+      vc::gsrb_fused<Real>(*m_lay, m_faces, *m_vc, m_relaxTmp.d, phi.d, rhs.d, m_dx, m_alpha, m_beta, m_vcShape);
+      std::swap(phi.d, m_relaxTmp.d);
+      return;
+    }
     for (int pass = 0; pass <= 1; pass++) {
       fillGhosts(phi, true, true);
       vc::gsrb<Real>(*m_lay, *m_vc, phi.d, rhs.d, pass, m_dx, m_alpha, m_beta);

@@ -53,6 +53,7 @@ class AMRPoissonOp : public AMRLevelOp<LevelData<Real>> {
   // alpha a phi - beta div(b grad phi) with these coefficients and relaxes with
   // GSRBHELMHOLTZVC; the one-launch constant-coefficient relaxation is not used.
   std::shared_ptr<VCCoefs<Real>> m_vc;
+  int m_vcShape = -1;                              // tile shape of the one-launch VC relaxation (-1 automatic)
   void setVC(std::shared_ptr<VCCoefs<Real>> c);   // also computes lambda (computeLambda)
   void resetLambda();                              // after alpha, beta or the coefficients change
 

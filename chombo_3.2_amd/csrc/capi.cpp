@@ -567,6 +567,14 @@ int amrmg_solver_set_relax_shape(amrmg_solver* s, int shape, int kc) {
   }
   API_END
 }
+int amrmg_solver_set_vc_relax_shape(amrmg_solver* s, int shape) {
+  API_BEGIN
+  if (shape < -1 || shape > 3) throw std::runtime_error("variable-coefficient relax shape out of range");
+  for (int d = 0; d < s->H.mgDepth(); d++) s->mgop(d)->m_vcShape = shape;
+  if (s->mixed)
+    for (int d = 0; d < s->H.mgDepth(); d++) s->mixedop(d)->m_vcShape = shape;
+  API_END
+}
 // ---- batched single-level solves ----
 int amrmg_batch_create(amrmg_ctx* ctx, int nprob, const int* ispecs, const double* rspecs, int precision,
                        amrmg_batch** out) {

@@ -1036,6 +1036,40 @@ __global__ void __launch_bounds__(256) k_sumpow_ordered(const DBox* __restrict__
   }
   if (threadIdx.x == 0) *out = norm;
 }
+// The same sums with every (box, column) chain in parallel, for levels whose
+// nbox x (widest box) chains fit in LDS: each chain runs in its thread in the
+// order above, then one thread adds the column sums box by box in the same order.
+constexpr int kSumpowChains = 4096;
+template <class T>
+__global__ void __launch_bounds__(256) k_sumpow_chains(const DBox* __restrict__ boxes, int nbox, int W, const T* x,
+                                                       double* out) {
+  __shared__ double tmp[kSumpowChains];
+  for (int p = threadIdx.x; p < nbox * W; p += blockDim.x) {
+    const int q = p / W, i = p % W;
+    const DBox bx = boxes[q];
+    if (i >= bx.hi[0] - bx.lo[0] + 1) continue;
+    double t = 0.0;
+    bool first = true;
+    for (int k = bx.lo[2]; k <= bx.hi[2]; k++)
+      for (int j = bx.lo[1]; j <= bx.hi[1]; j++) {
+        const double r = (double)x[cidx(bx, bx.lo[0] + i, j, k)];
+        if (first) { t = r * r; first = false; }
+        else t += r * r;
+      }
+    tmp[p] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double norm = 0.0;
+    for (int q = 0; q < nbox; q++) {
+      const int nx = boxes[q].hi[0] - boxes[q].lo[0] + 1;
+      double sum = tmp[q * W];
+      for (int i = 1; i < nx; i++) sum += tmp[q * W + i];
+      norm += sum;
+    }
+    *out = norm;
+  }
+}
 static double* scratch_partials(Context* ctx, int n) {
   static thread_local double* buf = nullptr;
   static thread_local int cap = 0;
@@ -1062,11 +1096,20 @@ double dot_ordered(const DeviceLayout& L, const T* a, const T* b, int slot) {
 }
 template <class T>
 double norm2_ordered(const DeviceLayout& L, const T* x, int slot) {
-  for (int q = 0; q < L.nbox; q++)
+  int W = 0;
+  for (int q = 0; q < L.nbox; q++) {
+    W = std::max(W, L.h_boxes[q].hi[0] - L.h_boxes[q].lo[0] + 1);
     if (L.h_boxes[q].hi[0] - L.h_boxes[q].lo[0] + 1 > 1024) throw std::runtime_error("norm2_ordered: box wider than 1024");
-  hipLaunchKernelGGL(k_sumpow_ordered<T>, dim3(1), dim3(256), 0, L.ctx->stream, L.d_boxes, L.nbox, x,
-                     L.ctx->d_scalars + slot);
-  hip_check(hipGetLastError(), "k_sumpow_ordered");
+  }
+  if ((long long)L.nbox * W <= kSumpowChains) {
+    hipLaunchKernelGGL(k_sumpow_chains<T>, dim3(1), dim3(256), 0, L.ctx->stream, L.d_boxes, L.nbox, W, x,
+                       L.ctx->d_scalars + slot);
+    hip_check(hipGetLastError(), "k_sumpow_chains");
+  } else {
+    hipLaunchKernelGGL(k_sumpow_ordered<T>, dim3(1), dim3(256), 0, L.ctx->stream, L.d_boxes, L.nbox, x,
+                       L.ctx->d_scalars + slot);
+    hip_check(hipGetLastError(), "k_sumpow_ordered");
+  }
   hip_check(hipMemcpyAsync(L.ctx->h_scalars + slot, L.ctx->d_scalars + slot, sizeof(double), hipMemcpyDeviceToHost,
                            L.ctx->stream),
             "copy scalar");

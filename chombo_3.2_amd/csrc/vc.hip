@@ -11,6 +11,7 @@
 #include <stdexcept>
 #include <string>
 
+#include "fused.hpp"
 #include "ops.hpp"
 #include "vc.hpp"
 
@@ -328,6 +329,221 @@ __global__ void __launch_bounds__(256) k_vc_avg_faces(const DBox* __restrict__ c
   }
 }
 
+// ---------------------------------------------------------------------------
+// One whole GSRB iteration of a merged level in one launch, variable coefficients.
+//
+// The scheme of the constant-coefficient one-launch relaxation (ops::fused_column,
+// whose equivalence with the reference's ghost refresh / red / ghost refresh /
+// black sequence is derived at gsrb_fused in kernels.hip): a workgroup marches up
+// a column of FX x FY cells; five LDS planes hold phi over the column grown by 2;
+// step k loads plane k+2, updates the red cells of plane k+1 on the column grown
+// by 1, the black cells of plane k on the column, and stores plane k-1.  Face
+// ghosts outside the rectangle are formed in registers at the moment of the
+// update, from the operands the ghost refresh would use.  The update is
+// GSRBHELMHOLTZVC's, with lambda formed in registers by resetLambda's expression
+// (SUMFACES: the operations and order of k_vc_lambda, hence the same value), so
+// the lambda array is not read.  Per updated cell the kernel reads rhs, aCoef and
+// the cell's six faces from global memory, two steps ahead of their use.
+// ---------------------------------------------------------------------------
+template <class T>
+struct VCFused {
+  const T* a;
+  const T* b0;
+  const T* b1;
+  const T* b2;
+  long long foff[3];  // face index of the rectangle's low corner, per direction
+  int fsy[3], fsz[3];
+};
+template <class T>
+struct VCCell {  // operands of one update
+  T r, a, b0m, b0p, b1m, b1p, b2m, b2p;
+};
+// operands of cell (i, j, k), index clamped into the rectangle (a cell that is not
+// updated still loads from a valid address, and its values are never used)
+template <class T>
+__device__ __forceinline__ void vc_fetch_cell(VCCell<T>& q, const ops::FusedArgs& a, const VCFused<T>& v,
+                                              const T* __restrict__ rhs, int i, int j, int k) {
+  const int di = ops::clampi(i, a.lo[0], a.hi[0]) - a.lo[0];
+  const int dj = ops::clampi(j, a.lo[1], a.hi[1]) - a.lo[1];
+  const int dk = ops::clampi(k, a.lo[2], a.hi[2]) - a.lo[2];
+  const long long c = a.off + (di + 1) + (long long)(dj + 1) * a.sy + (long long)(dk + 1) * a.sz;
+  const long long x0 = v.foff[0] + di + (long long)dj * v.fsy[0] + (long long)dk * v.fsz[0];
+  const long long x1 = v.foff[1] + di + (long long)dj * v.fsy[1] + (long long)dk * v.fsz[1];
+  const long long x2 = v.foff[2] + di + (long long)dj * v.fsy[2] + (long long)dk * v.fsz[2];
+  q.r = rhs[c];
+  q.a = v.a[c];
+  q.b0m = v.b0[x0];
+  q.b0p = v.b0[x0 + 1];
+  q.b1m = v.b1[x1];
+  q.b1p = v.b1[x1 + v.fsy[1]];
+  q.b2m = v.b2[x2];
+  q.b2p = v.b2[x2 + v.fsz[2]];
+}
+template <class T, int LW>
+__device__ __forceinline__ T vc_fused_new(const ops::FusedArgs& a, const T* Sm, const T* S0, const T* Sp, int c,
+                                          int i, int j, int k, const VCCell<T>& q, T dxinv, T scale, T alpha,
+                                          T beta, T c1, T c2) {
+  const T own = S0[c];
+  const T vxp = S0[c + 1], vxm = S0[c - 1];
+  const T vyp = S0[c + LW], vym = S0[c - LW];
+  const T vzp = Sp[c], vzm = Sm[c];
+  const T xp = (i == a.hi[0]) ? ops::face_ghost(a.kind[0][1], own, vxm, c1, c2) : vxp;
+  const T xm = (i == a.lo[0]) ? ops::face_ghost(a.kind[0][0], own, vxp, c1, c2) : vxm;
+  const T yp = (j == a.hi[1]) ? ops::face_ghost(a.kind[1][1], own, vym, c1, c2) : vyp;
+  const T ym = (j == a.lo[1]) ? ops::face_ghost(a.kind[1][0], own, vyp, c1, c2) : vym;
+  const T zp = (k == a.hi[2]) ? ops::face_ghost(a.kind[2][1], own, vzm, c1, c2) : vzp;
+  const T zm = (k == a.lo[2]) ? ops::face_ghost(a.kind[2][0], own, vzp, c1, c2) : vzm;
+  // lambda (k_vc_lambda's expression)
+  T l = q.a * alpha;
+  T sumVal = q.b0p + q.b0m;
+  l = l + scale * beta * sumVal;
+  sumVal = q.b1p + q.b1m;
+  l = l + scale * beta * sumVal;
+  sumVal = q.b2p + q.b2m;
+  l = l + scale * beta * sumVal;
+  const T lam = T(1.0) / l;
+  // GSRBHELMHOLTZVC (vc_lofphi's expression)
+  const T s = q.b0p * (xp - own) - q.b0m * (own - xm) + q.b1p * (yp - own) - q.b1m * (own - ym) +
+              q.b2p * (zp - own) - q.b2m * (own - zm);
+  const T lofphi = alpha * q.a * own - beta * s * dxinv;
+  return own - lam * (lofphi - q.r);
+}
+template <class T, int FX, int FY, int NT>
+__global__ void __launch_bounds__(NT) k_vc_gsrb_fused(ops::FusedArgs a, VCFused<T> v, T* __restrict__ dst,
+                                                      const T* __restrict__ src, const T* __restrict__ rhs, T dxinv,
+                                                      T scale, T alpha, T beta, T c1, T c2) {
+  constexpr int LW = FX + 4, LH = FY + 4, LPLANE = LW * LH;
+  constexpr int NLD = (LPLANE + NT - 1) / NT;
+  constexpr int NRC = (FY + 2) * (FX / 2 + 1);
+  constexpr int NRED = (NRC + NT - 1) / NT;
+  constexpr int NBC = FX * FY / 2;
+  constexpr int NBLK = (NBC + NT - 1) / NT;
+  constexpr int NST = (FX * FY + NT - 1) / NT;
+  static_assert(FX % 2 == 0, "column width must be even");
+  __shared__ T P[ops::NSLOT][LPLANE];
+  const int w = ops::xcd_contiguous_block();
+  if (w >= a.nwork) return;
+  const int tx = w % a.ntx, ty = (w / a.ntx) % a.nty, ch = w / (a.ntx * a.nty);
+  const int x0 = a.lo[0] + tx * FX, y0 = a.lo[1] + ty * FY;
+  const int k0 = a.lo[2] + ch * a.kc;
+  const int k1 = min(k0 + a.kc, a.hi[2] + 1);
+  const int tid = threadIdx.x;
+  auto slot = [&](int p) { return (p - k0 + 2) % ops::NSLOT; };
+  auto gidx = [&](int i, int j, int k) {
+    return a.off + (i - a.lo[0] + 1) + (long long)(j - a.lo[1] + 1) * a.sy + (long long)(k - a.lo[2] + 1) * a.sz;
+  };
+  auto lidx = [&](int i, int j) { return (j - y0 + 2) * LW + (i - x0 + 2); };
+  auto red_ij = [&](int q, int r, int& i, int& j) {
+    j = y0 - 1 + q / (FX / 2 + 1);
+    i = x0 - 1 + ((x0 - 1 + j + r) & 1) + 2 * (q % (FX / 2 + 1));
+  };
+  auto black_ij = [&](int q, int k, int& i, int& j) {
+    j = y0 + q / (FX / 2);
+    i = x0 + ((x0 + j + k + 1) & 1) + 2 * (q % (FX / 2));
+  };
+  struct Pre {
+    T phi[NLD];
+    VCCell<T> rr[NRED], rb[NBLK];
+  };
+  Pre A, B;
+  auto fetch = [&](Pre& f, int p, int r, int k) {  // plane p; red cells of plane r; black cells of plane k
+    const int pc = ops::clampi(p, a.lo[2], a.hi[2]);
+#pragma unroll
+    for (int u = 0; u < NLD; u++) {
+      const int q = tid + NT * u;
+      const int i = x0 - 2 + q % LW, j = y0 - 2 + q / LW;
+      f.phi[u] = src[gidx(ops::clampi(i, a.lo[0], a.hi[0]), ops::clampi(j, a.lo[1], a.hi[1]), pc)];
+    }
+#pragma unroll
+    for (int u = 0; u < NRED; u++) {
+      int i, j;
+      red_ij(tid + NT * u, r, i, j);
+      vc_fetch_cell(f.rr[u], a, v, rhs, i, j, r);
+    }
+#pragma unroll
+    for (int u = 0; u < NBLK; u++) {
+      int i, j;
+      black_ij(tid + NT * u, k, i, j);
+      vc_fetch_cell(f.rb[u], a, v, rhs, i, j, k);
+    }
+  };
+  auto put_phi = [&](const Pre& f, int p) {
+    T* S = P[slot(p)];
+#pragma unroll
+    for (int u = 0; u < NLD; u++) {
+      const int q = tid + NT * u;
+      if (q < LPLANE) S[q] = f.phi[u];
+    }
+  };
+  auto store = [&](int p) {
+    const T* S = P[slot(p)];
+    T val[NST];
+#pragma unroll
+    for (int u = 0; u < NST; u++) {
+      const int q = min(tid + NT * u, FX * FY - 1);
+      val[u] = S[(q / FX + 2) * LW + (q % FX + 2)];
+    }
+#pragma unroll
+    for (int u = 0; u < NST; u++) {
+      const int q = tid + NT * u;
+      const int i = x0 + q % FX, j = y0 + q / FX;
+      if (q < FX * FY && i <= a.hi[0] && j <= a.hi[1]) dst[gidx(i, j, p)] = val[u];
+    }
+  };
+  // step k consumes the set f (plane k+2, red cells of plane k+1, black cells of
+  // plane k) and then refills it for step k+2, so the loads have step k+1 to land
+  // and the operands are held in one place (no copies between sets)
+  auto step = [&](Pre& f, int k) {
+    put_phi(f, k + 2);
+    ops::lds_barrier();
+    if (k - 1 >= k0) store(k - 1);
+    const int r = k + 1;
+    if (r >= a.lo[2] && r <= a.hi[2]) {
+#pragma unroll
+      for (int u = 0; u < NRED; u++) {
+        const int q = tid + NT * u;
+        int i, j;
+        red_ij(q, r, i, j);
+        if (q < NRC && i >= a.lo[0] && i <= a.hi[0] && j >= a.lo[1] && j <= a.hi[1]) {
+          const int c = lidx(i, j);
+          P[slot(r)][c] = vc_fused_new<T, LW>(a, P[slot(r - 1)], P[slot(r)], P[slot(r + 1)], c, i, j, r, f.rr[u],
+                                              dxinv, scale, alpha, beta, c1, c2);
+        }
+      }
+    }
+    ops::lds_barrier();
+    if (k >= k0) {
+#pragma unroll
+      for (int u = 0; u < NBLK; u++) {
+        const int q = tid + NT * u;
+        int i, j;
+        black_ij(q, k, i, j);
+        if (q < NBC && i <= a.hi[0] && j <= a.hi[1]) {
+          const int c = lidx(i, j);
+          P[slot(k)][c] = vc_fused_new<T, LW>(a, P[slot(k - 1)], P[slot(k)], P[slot(k + 1)], c, i, j, k, f.rb[u],
+                                              dxinv, scale, alpha, beta, c1, c2);
+        }
+      }
+    }
+    fetch(f, k + 4, k + 3, k + 2);
+  };
+  fetch(A, k0 - 2, k0 - 3, k0 - 4);
+  put_phi(A, k0 - 2);
+  fetch(A, k0 - 1, k0 - 3, k0 - 4);
+  put_phi(A, k0 - 1);
+  fetch(A, k0, k0 - 1, k0 - 2);
+  fetch(B, k0 + 1, k0, k0 - 1);
+  for (int k = k0 - 2; k < k1; k += 2) {
+    step(A, k);
+    if (k + 1 < k1) step(B, k + 1);
+  }
+  ops::lds_barrier();
+  store(k1 - 1);
+}
+// tile shapes of the one-launch VC relaxation {FX, FY, NT}
+constexpr int kVCShapes[][3] = {{64, 8, 256}, {32, 8, 256}, {64, 4, 256}, {64, 16, 512}};
+constexpr int kVCShapeCount = 4;
+
 #define VC_LAUNCH(L, kind, bytes, units, kernel, grid, ...)                                \
   do {                                                                                     \
     if ((grid) > 0 && !(L).ctx->dry) {                                                     \
@@ -358,11 +574,13 @@ void check_ratio(const DeviceLayout& Lf, const DeviceLayout& Lc, int r, const ch
 
 namespace vc {
 
-// bytes per cell moved by one launch (FP64: phi, rhs, a, lambda, three face arrays
-// read, phi written; the second face of each pair is the next cell's first)
+// algorithmic bytes of one red-black iteration: phi read and written, rhs, aCoef
+// and the three face arrays read once (7 values per cell: the high face of a cell
+// is the next cell's low face, and lambda is a function of aCoef and bCoef); a
+// colour launch is half an iteration
 template <class T>
 static double gsrb_bytes(const DeviceLayout& L) {
-  return 8.0 * sizeof(T) * L.valid_cells;
+  return 3.5 * sizeof(T) * L.valid_cells;
 }
 
 template <class T>
@@ -372,6 +590,72 @@ void gsrb(const DeviceLayout& L, const VCCoefs<T>& C, T* phi, const T* rhs, int 
   const double dxinv = 1.0 / (dx * dx);
   VC_LAUNCH(L, KC_GSRB, gsrb_bytes<T>(L), 0.5 * L.valid_cells, k_vc_gsrb<T>, L.ntiles, L.d_boxes, L.d_tiles, geom(L),
             args(C), phi, rhs, color, T(dxinv), T(alpha), T(beta));
+}
+template <class T>
+void gsrb_fused(const DeviceLayout& L, const ops::FaceRules& fr, const VCCoefs<T>& C, T* dst, const T* src,
+                const T* rhs, double dx, double alpha, double beta, int shape) {
+  need_coefs(L, C, "vc::gsrb_fused");
+  if (!L.merged) throw std::runtime_error("vc::gsrb_fused: level is not stored merged");
+  if (dst == src || dst == rhs) throw std::runtime_error("vc::gsrb_fused: dst must be a separate field");
+  const DBox& R = L.h_boxes[L.nbox];
+  // automatic: 64-wide columns where the level is that wide, 4 rows in FP64 (fewer
+  // registers: more waves in flight), 8 in FP32; 32 x 8 on narrower levels
+  if (shape < 0) shape = (R.hi[0] - R.lo[0] + 1 >= 64) ? (sizeof(T) == 8 ? 2 : 0) : 1;
+  if (shape >= kVCShapeCount) throw std::runtime_error("vc::gsrb_fused: bad tile shape");
+  const int FX = kVCShapes[shape][0], FY = kVCShapes[shape][1];
+  ops::FusedArgs a;
+  for (int q = 0; q < 3; q++) {
+    a.lo[q] = R.lo[q];
+    a.hi[q] = R.hi[q];
+  }
+  for (int d = 0; d < 3; d++)
+    for (int s = 0; s < 2; s++) {
+      if (fr.kind[d][s] < 0 || fr.kind[d][s] > 2) throw std::runtime_error("vc::gsrb_fused: bad face rule");
+      a.kind[d][s] = fr.kind[d][s];
+    }
+  a.off = R.off;
+  a.sy = R.sy;
+  a.sz = R.sz;
+  a.zsrc = 0;
+  a.ntx = (R.hi[0] - R.lo[0] + 1 + FX - 1) / FX;
+  a.nty = (R.hi[1] - R.lo[1] + 1 + FY - 1) / FY;
+  const int nz = R.hi[2] - R.lo[2] + 1;
+  int kc = 32;
+  while (kc > 4 && (long long)a.ntx * a.nty * ((nz + kc - 1) / kc) < 2048) kc >>= 1;
+  a.kc = kc;
+  const long long nwork = (long long)a.ntx * a.nty * ((nz + kc - 1) / kc);
+  if (nwork > INT32_MAX - 8) throw std::runtime_error("vc::gsrb_fused: level too large");
+  a.nwork = (int)nwork;
+  VCFused<T> v;
+  v.a = C.a.d;
+  v.b0 = C.b[0];
+  v.b1 = C.b[1];
+  v.b2 = C.b[2];
+  for (int d = 0; d < 3; d++) {
+    const DBox& f = C.h_fbox[d][L.nbox];
+    v.foff[d] = f.off;
+    v.fsy[d] = f.sy;
+    v.fsz[d] = f.sz;
+  }
+  const int grid = (a.nwork + 7) / 8 * 8;
+  const double dxinv = 1.0 / (dx * dx), scale = 1.0 / (dx * dx);
+  const double bytes = 2.0 * gsrb_bytes<T>(L), units = (double)L.valid_cells;
+  if (L.ctx->dry) return;
+  hipEvent_t ev = profile_begin(L.ctx, KC_GSRB);
+#define VC_FUSED_CASE(n, X, Y, NT)                                                                             \
+  case n:                                                                                                      \
+    hipLaunchKernelGGL((k_vc_gsrb_fused<T, X, Y, NT>), dim3(grid), dim3(NT), 0, L.ctx->stream, a, v, dst, src, rhs, \
+                       T(dxinv), T(scale), T(alpha), T(beta), T(fr.c1), T(fr.c2));                              \
+    break;
+  switch (shape) {
+    VC_FUSED_CASE(0, 64, 8, 256)
+    VC_FUSED_CASE(1, 32, 8, 256)
+    VC_FUSED_CASE(2, 64, 4, 256)
+    VC_FUSED_CASE(3, 64, 16, 512)
+  }
+#undef VC_FUSED_CASE
+  hip_check(hipGetLastError(), "k_vc_gsrb_fused");
+  profile_end(L.ctx, ev, KC_GSRB, bytes, units);
 }
 template <class T>
 void apply(const DeviceLayout& L, const VCCoefs<T>& C, T* lofphi, const T* phi, double dx, double alpha,
@@ -439,6 +723,8 @@ void narrow(const VCCoefs<double>& src, VCCoefs<float>& dst) {
 
 #define VC_INST(T)                                                                                                  \
   template void gsrb<T>(const DeviceLayout&, const VCCoefs<T>&, T*, const T*, int, double, double, double);        \
+  template void gsrb_fused<T>(const DeviceLayout&, const ops::FaceRules&, const VCCoefs<T>&, T*, const T*,          \
+                              const T*, double, double, double, int);                                               \
   template void apply<T>(const DeviceLayout&, const VCCoefs<T>&, T*, const T*, double, double, double);            \
   template void residual<T>(const DeviceLayout&, const VCCoefs<T>&, T*, const T*, const T*, double, double, double); \
   template void restrict_res<T>(const DeviceLayout&, const DeviceLayout&, const VCCoefs<T>&, T*, const T*,          \

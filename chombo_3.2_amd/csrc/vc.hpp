@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "device.hpp"
+#include "ops.hpp"
 
 namespace amrmg {
 
@@ -44,6 +45,12 @@ namespace vc {
 template <class T>
 void gsrb(const DeviceLayout& L, const VCCoefs<T>& C, T* phi, const T* rhs, int color, double dx, double alpha,
           double beta);
+// One GSRB iteration (red, refresh, black) of a merged level in one launch:
+// dst <- relaxed src, bitwise equal to the ghost refresh + gsrb(0) + ghost refresh
+// + gsrb(1) sequence.  shape: tile shape (-1 automatic, 0..3).
+template <class T>
+void gsrb_fused(const DeviceLayout& L, const ops::FaceRules& fr, const VCCoefs<T>& C, T* dst, const T* src,
+                const T* rhs, double dx, double alpha, double beta, int shape = -1);
 // VCCOMPUTEOP (VCAMRPoissonOpF.ChF:197-265)
 template <class T>
 void apply(const DeviceLayout& L, const VCCoefs<T>& C, T* lofphi, const T* phi, double dx, double alpha,

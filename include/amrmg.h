@@ -133,6 +133,11 @@ int amrmg_solver_tune_relax(amrmg_solver* s, double* table, int cap, int* nrec);
 /* Forces tile shape (0..9) and z-chunk (0 = automatic) of the one-launch
    relaxation on every level. */
 int amrmg_solver_set_relax_shape(amrmg_solver* s, int shape, int kc);
+/* Tile shape of the one-launch variable-coefficient relaxation on every multigrid
+   depth: -1 automatic, 0 64x8, 1 32x8, 2 64x4 (256 threads), 3 64x16 (512
+   threads); results are bitwise the same for every choice.  No counterpart in the
+   reference (its GSRBHELMHOLTZVC runs per box and colour). */
+int amrmg_solver_set_vc_relax_shape(amrmg_solver* s, int shape);
 
 /* ---- batched single-level solves: many independent problems, each V-cycle stage
    of all of them in one launch (relaxation, restriction, prolongation, residual

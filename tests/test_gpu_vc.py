@@ -168,6 +168,45 @@ def test_vc_clear_and_alpha_beta_equal_fresh_solvers(gpu_ctx):
     g.close()
 
 
+@pytest.mark.parametrize("case", ["vc40_ragged", "vc16_neumann"])
+def test_vc_fused_relax_shapes_match_two_colour_launches(gpu_ctx, case):
+    """The one-launch VC relaxation (every tile shape) against the reference's
+    sequence (ghost refresh, red, ghost refresh, black), both on the GPU: level
+    relaxations at every depth and whole solves agree bitwise."""
+    print(f"vc fused shapes {case}", flush=True)
+    h = small_single(**CASES[case])
+    a, b = H.vc_coefficients(h, seed=21)
+    rhs = H.sinusoid_rhs(h, 1008, k=(2, 1, 1))
+    sp, rp = H.solver_params()
+    rng = np.random.default_rng(23)
+    g = amrmg.AMRMultiGrid(gpu_ctx, h)
+    g.set_vc_coefs(a, b)
+    depths = g.mg_depths
+    fields = [(rng.standard_normal(g.mg_cells(d)), rng.standard_normal(g.mg_cells(d))) for d in range(depths)]
+
+    def run(shape, fused):
+        gpu_ctx.set_fused_relax(fused)
+        g.set_vc_relax_shape(shape)
+        lv = [g.level_relax(d, p, r, 2) for d, (p, r) in enumerate(fields)]
+        g.setSolverParameters(sp, rp)
+        g.set_field(g.RHS, rhs)
+        hist, info = g.solve()
+        return lv, hist, info, g.get_field(g.PHI)
+
+    try:
+        ref = run(-1, False)
+        for shape in (-1, 0, 1, 2, 3):
+            got = run(shape, True)
+            for d in range(depths):
+                np.testing.assert_array_equal(got[0][d], ref[0][d])
+            assert got[2] == ref[2]
+            np.testing.assert_array_equal(got[1], ref[1])
+            np.testing.assert_array_equal(got[3], ref[3])
+    finally:
+        gpu_ctx.set_fused_relax(True)
+    g.close()
+
+
 def test_vc_rejects_bad_input(gpu_ctx):
     print("vc errors", flush=True)
     h = small_single(n=32, box=16)
