@@ -266,10 +266,11 @@ def bump_rhs(h: Hierarchy, seed: int, nbumps: int = 3) -> np.ndarray:
 
 
 def vc_coefficients(h: "Hierarchy", seed: int = 7, contrast: float = 4.0):
-    """Smooth positive variable coefficients on level 0 (VCAMRPoissonOp2):
-    a(x) = 1 + 0.5 sin(2 pi x) cos(2 pi y) sin(2 pi z) (cells) and
-    b(x) = 1 + (contrast - 1) * (0.5 + 0.5 sin(2 pi (x + phase)) cos(2 pi y) cos(pi z))
-    (faces), phases from the seed.  Returns (a, [b0, b1, b2])."""
+    """Smooth positive variable coefficients on every level (VCAMRPoissonOp2):
+    a(x) = 1 + 0.5 sin(2 pi (x + phase)) cos(2 pi y) sin(2 pi z) (cells) and
+    b(x) = 1 + (contrast - 1) * (0.5 + 0.5 sin(2 pi (x + phase)) cos(2 pi (y + phase)) cos(pi z))
+    (faces), phases from the seed.  Returns (a valid-packed over all levels,
+    [b0, b1, b2] each face-packed level after level)."""
     rng = np.random.default_rng(seed)
     ph = rng.uniform(0, 1, size=3)
     a = h.pack(lambda x, y, z, l: 1.0 + 0.5 * np.sin(2 * math.pi * (x + ph[0])) * np.cos(2 * math.pi * y)
@@ -279,10 +280,7 @@ def vc_coefficients(h: "Hierarchy", seed: int = 7, contrast: float = 4.0):
         return 1.0 + (contrast - 1.0) * (0.5 + 0.5 * np.sin(2 * math.pi * (x + ph[1])) * np.cos(2 * math.pi * (y + ph[2]))
                                          * np.cos(math.pi * z))
 
-    if h.nlev != 1:
-        raise ValueError("variable coefficients: one level")
-    a = a[:h.cells(0)]
-    return a, [h.pack_faces(bf, d) for d in range(3)]
+    return a, [np.concatenate([h.pack_faces(bf, d, lev) for lev in range(h.nlev)]) for d in range(3)]
 
 
 DEFAULT_SOLVER = dict(pre=2, post=2, bottom=2, numMG=1, iterMax=100, imin=5,

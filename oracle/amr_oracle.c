@@ -1479,6 +1479,8 @@ static void op_reflux(Op* cop, Op* fop, LD* phiFine, LD* phi, LD* res) {
                 if (side == 0) n0[idir] += 1;                /* shiftHalf(-1): node n -> cell n-1 */
                 int m1[3] = {n0[0],n0[1],n0[2]}; m1[idir] -= 1;
                 double F = (*P(cf,n0[0],n0[1],n0[2]) - *P(cf,m1[0],m1[1],m1[2]))*scale_flux;
+                /* VCAMRPoissonOp2::getFlux (VCAMRPoissonOp2.cpp:787-837): -b(face) * gradphi */
+                if (cop->vc) F = -*P(&cop->vb[idir][c],n0[0],n0[1],n0[2]) * F;
                 double* reg = P(&r->coarFlux.f[c],i,j,k);
                 *reg += F*sc;
               }
@@ -1507,6 +1509,7 @@ static void op_reflux(Op* cop, Op* fop, LD* phiFine, LD* phi, LD* res) {
               if (side == 0) n0[idir] += 1;
               int m1[3] = {n0[0],n0[1],n0[2]}; m1[idir] -= 1;
               double F = (*P(pf,n0[0],n0[1],n0[2]) - *P(pf,m1[0],m1[1],m1[2]))*scale_flux;
+              if (fop->vc) F = -*P(&fop->vb[idir][f],n0[0],n0[1],n0[2]) * F;
               int ii = civ[0] + (i-iv[0])/2, jj = civ[1] + (j-iv[1])/2, kk = civ[2] + (k-iv[2])/2;
               double* cfin = P(&r->fineFlux.f[f], ii, jj, kk);
               *cfin = *cfin + scale*F;
@@ -1646,20 +1649,23 @@ void oracle_destroy(OHier* h) {
    arithmetic), each op with its lambda. */
 static const Op* mg_op(const OHier* h, int d);
 int oracle_vc_set(OHier* h, const double* a, const double* b0, const double* b1, const double* b2) {
-  if (h->nlev != 1) { set_err("variable coefficients: one level only"); return -1; }
-  Op* op0 = &h->op[0];
-  if (!op0->vc) op_vc_alloc(op0);
-  ld_from_packed(&op0->va, a);
   const double* bs[3] = {b0, b1, b2};
-  for (int d = 0; d < 3; d++) {
-    long o = 0;
-    for (int q = 0; q < op0->L.n; q++) {
-      Fab* f = &op0->vb[d][q];
-      long n = b_npts(&f->b);
-      memcpy(f->d, bs[d] + o, sizeof(double)*n); o += n;
-    }
+  long ao = 0, bo[3] = {0, 0, 0};
+  /* every AMR level: its own coefficients (VCAMRPoissonOp2Factory::AMRnewOp, :1074-1171) */
+  for (int l = 0; l < h->nlev; l++) {
+    Op* op = &h->op[l];
+    if (!op->vc) op_vc_alloc(op);
+    ld_from_packed(&op->va, a + ao);
+    ao += lay_cells(&op->L);
+    for (int d = 0; d < 3; d++)
+      for (int q = 0; q < op->L.n; q++) {
+        Fab* f = &op->vb[d][q];
+        long n = b_npts(&f->b);
+        memcpy(f->d, bs[d] + bo[d], sizeof(double)*n); bo[d] += n;
+      }
+    op_vc_lambda(op);
   }
-  op_vc_lambda(op0);
+  Op* op0 = &h->op[0];
   for (int dd = 1; dd < h->mgD; dd++) {
     Op* op = &h->mgop[dd];
     if (!op->vc) op_vc_alloc(op);

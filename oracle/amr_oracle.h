@@ -85,10 +85,11 @@ long oracle_ghosted_cells(const OHier* h, int lev);
 int oracle_bottom_solve(OHier* h, double* corr, const double* res, int nbottom,
                         double metric, double eps, int* info);
 
-/* VCAMRPoissonOp2 on a one-level hierarchy: L phi = alpha a phi - beta div(b grad phi).
-   a valid-packed; b_d per box in layout order, the box's faces normal to d (the box
-   grown by one in d) in Fortran order.  Every later call on h uses the variable-
-   coefficient operator (MG depths get the coefficients averaged from level 0). */
+/* VCAMRPoissonOp2: L phi = alpha a phi - beta div(b grad phi) on every AMR level.
+   a valid-packed over all levels; b_d level after level, per box in layout order,
+   the box's faces normal to d (the box grown by one in d) in Fortran order.  Every
+   later call on h uses the variable-coefficient operator (MG depths of level 0 get
+   the coefficients averaged from level 0; reflux uses -b * gradient fluxes). */
 int  oracle_vc_set(OHier* h, const double* a, const double* b0, const double* b1, const double* b2);
 long oracle_face_cells(const OHier* h, int lev, int dir);
 int  oracle_vc_lambda(OHier* h, int depth, double* lambda_out);

@@ -192,8 +192,9 @@ class Oracle:
         return corr, int(info[0])
 
     def vc_set(self, a, b):
-        """Variable coefficients (VCAMRPoissonOp2) on a one-level hierarchy: a valid-packed,
-        b = [b0, b1, b2], each per box its faces normal to d in Fortran order."""
+        """Variable coefficients (VCAMRPoissonOp2) on every level: a valid-packed over all
+        levels, b = [b0, b1, b2], each level after level, per box its faces normal to d
+        in Fortran order."""
         a, pa = _d(a)
         bs = [_d(x) for x in b]
         self._vc = (a, bs)  # keep the buffers alive for the call

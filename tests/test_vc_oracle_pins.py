@@ -14,7 +14,13 @@ independently of its code:
   extrapolation), and the multigrid solve equals a sparse direct solve;
 * coefficient coarsening: arithmetic averages of linear a and b are their centre
   values, so the lambda of every multigrid depth has a closed form;
-* algebra: GSRB fixed point, restriction = mean of fine residuals.
+* algebra: GSRB fixed point, restriction = mean of fine residuals;
+* AMR (two levels, refluxing with -b * gradient fluxes): with b_d depending on
+  the normal coordinate only and quadratic phi every fine face flux is exact and
+  so is its average, so the composite operator is exact on uncovered interior
+  cells; the refluxed composite operator is conservative (its integral is the
+  b-weighted boundary flux); the AMR multigrid solve equals a dense solve of the
+  column-probed composite system.
 """
 import numpy as np
 import pytest
@@ -22,7 +28,7 @@ import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
 import hierarchy as H
-from helpers import small_single
+from helpers import small_amr, small_single, uncovered_mask
 from oracle import Oracle
 
 
@@ -229,3 +235,111 @@ def test_gsrb_fixed_point_and_restriction_mean():
     hc = H.Hierarchy(H.box((-4,) * 3, (3,) * 3), [[H.box_coarsen(bx, 2) for bx in h.levels[0]]], 1 / 8)
     mean = r.reshape(2, 8, 2, 8, 2, 8, order="F").mean(axis=(0, 2, 4))
     assert np.abs(hc.to_global(rc, 0) - mean).max() < 1e-12 * np.abs(mean).max()
+
+
+def _amr_interior(h, lev, width=1):
+    dom = h.level_domain(lev)
+    parts = []
+    for (l, hh) in h.levels[lev]:
+        I = np.meshgrid(*[np.arange(l[d], hh[d] + 1) for d in range(3)], indexing="ij")
+        m = np.ones(I[0].shape, bool)
+        for d in range(3):
+            m &= (I[d] >= dom[0][d] + width) & (I[d] <= dom[1][d] - width)
+        parts.append(m.ravel(order="F"))
+    return np.concatenate(parts)
+
+
+def _amr_faces(h, fns):
+    return [np.concatenate([h.pack_faces(fns[d], d, lev) for lev in range(h.nlev)]) for d in range(3)]
+
+
+@pytest.mark.parametrize("alpha,beta", [(0.0, 1.0), (0.4, 1.3)])
+def test_amr_composite_operator_exact_for_normal_linear_b(alpha, beta):
+    h = small_amr(alpha=alpha, beta=beta)
+    o = Oracle(h)
+    c = [(1.0, 0.5), (1.5, -0.4), (0.8, 0.3)]
+    fns = [lambda x, y, z: c[0][0] + c[0][1] * x, lambda x, y, z: c[1][0] + c[1][1] * y,
+           lambda x, y, z: c[2][0] + c[2][1] * z]
+    a = h.pack(lambda x, y, z, l: 1.0 + 0.25 * x - 0.5 * y * z)
+    o.vc_set(a, _amr_faces(h, fns))
+    q = lambda x, y, z, l: 1 + 0.3 * x + x * x + 2 * y * y + 0.5 * z * z + 0.7 * x * y - 0.4 * y * z + 0.25 * x * z
+    phi = h.pack(q)
+    X, Y, Z = (h.pack(lambda x, y, z, l, t=t: [x, y, z][t]) for t in range(3))
+    grad = [0.3 + 2 * X + 0.7 * Y + 0.25 * Z, 4 * Y + 0.7 * X - 0.4 * Z, Z - 0.4 * Y + 0.25 * X]
+    hess = [2.0, 4.0, 1.0]
+    coord = [X, Y, Z]
+    div = sum(c[d][1] * grad[d] + (c[d][0] + c[d][1] * coord[d]) * hess[d] for d in range(3))
+    expect = alpha * a * phi - beta * div
+    L = o.amr_operator(phi, homog=False)
+    offs = h.level_offsets()
+    checked = 0
+    for lev in range(h.nlev):
+        m = uncovered_mask(h, lev) & _amr_interior(h, lev)
+        sl = slice(offs[lev], offs[lev + 1])
+        assert np.abs(L[sl] - expect[sl])[m].max() < 1e-7 * np.abs(expect).max(), lev
+        checked += m.sum()
+    assert checked > 0
+    # the coarse cells next to the fine level see the fine level's b through the
+    # refluxed fluxes: other fine coefficients change them
+    o2 = Oracle(h)
+    o2.vc_set(a, [np.concatenate([h.pack_faces(fns[d], d, 0)] + [np.ones(h.pack_faces(fns[d], d, l).size)
+                                                                 for l in range(1, h.nlev)]) for d in range(3)])
+    L2 = o2.amr_operator(phi, homog=False)
+    m0 = uncovered_mask(h, 0) & _amr_interior(h, 0)
+    assert np.abs(L2 - L)[offs[0]:offs[1]][m0].max() > 1e-3 * np.abs(expect).max()
+
+
+def test_amr_composite_operator_is_conservative():
+    h = small_amr(alpha=0.0, beta=1.0)
+    o = Oracle(h)
+    fb = lambda x, y, z: 1.0 + 0.5 * np.sin(3 * x) * np.cos(2 * y) + 0.25 * z
+    a = h.pack(lambda x, y, z, l: 1.0 + 0.0 * x)
+    o.vc_set(a, _amr_faces(h, [fb, fb, fb]))
+    rng = np.random.default_rng(9)
+    phi = rng.standard_normal(h.total_cells())
+    L = o.amr_operator(phi, homog=True)
+    offs = h.level_offsets()
+    total = 0.0
+    for lev in range(h.nlev):
+        m = uncovered_mask(h, lev)
+        total += L[offs[lev]:offs[lev + 1]][m].sum() * h.dx(lev) ** 3
+    # level 0 alone touches the boundary: face term -beta * b * (ghost - near) * dx with
+    # ghost = -near, i.e. +2 beta b near dx per boundary face
+    g0 = h.to_global(phi[offs[0]:offs[1]], 0)
+    n = g0.shape[0]
+    dx = h.dx(0)
+    cen = (np.arange(n) + 0.5) * dx
+    flux = 0.0
+    for d in range(3):
+        for side, pos in ((0, 0.0), (1, 1.0)):
+            near = np.take(g0, 0 if side == 0 else -1, axis=d)
+            t = [cen, cen, cen]
+            t[d] = np.array([pos])
+            G = np.meshgrid(*t, indexing="ij")
+            bf = np.squeeze(fb(*G), axis=d)
+            flux += (2.0 * bf * near).sum() * dx
+    assert abs(total - flux) < 1e-9 * np.abs(L).max()
+
+
+def test_amr_solve_matches_dense_solve_of_composite_system():
+    coarse = H.domain_split(H.box((0, 0, 0), (7, 7, 7)), 4, 4)
+    fine = H.domain_split(H.box((4, 4, 4), (11, 11, 11)), 4, 4)
+    h = H.Hierarchy(H.box((0, 0, 0), (7, 7, 7)), [coarse, fine], 1.0 / 8, 0.5, 1.0)
+    o = Oracle(h)
+    a, b = H.vc_coefficients(h, seed=4)
+    o.vc_set(a, b)
+    N = h.total_cells()
+    A = np.zeros((N, N))
+    e = np.zeros(N)
+    for col in range(N):
+        e[:] = 0
+        e[col] = 1
+        A[:, col] = o.amr_operator(e, homog=True)
+    keep = np.concatenate([uncovered_mask(h, l) for l in range(h.nlev)])
+    rhs = H.bump_rhs(h, 1002)
+    rhs_k = np.where(keep, rhs, 0.0)
+    sp_, rp = H.solver_params(eps=1e-12)
+    phi, hist, info = o.solve(rhs_k, sp_, rp)
+    assert info["exit_status"] & 1
+    x = np.linalg.solve(A[np.ix_(keep, keep)], rhs_k[keep])
+    assert np.abs(phi[keep] - x).max() / np.abs(x).max() < 1e-9
