@@ -30,6 +30,8 @@ AMRPoissonOp<Real>::~AMRPoissonOp() {
     dev_free(m_ctx, d_rcell);
     dev_free(m_ctx, d_rc);
     dev_free(m_ctx, d_rf);
+    dev_free(m_ctx, d_vcRcFace);
+    dev_free(m_ctx, d_vcRfFace);
   }
 }
 
@@ -157,6 +159,43 @@ void AMRPoissonOp<Real>::setVC(std::shared_ptr<VCCoefs<Real>> c) {
   if (c && c->lay != m_lay) throw std::runtime_error("setVC: coefficients belong to another level");
   m_vc = std::move(c);
   if (m_vc) resetLambda();
+}
+// face indices of the reflux contributions in this level's and the finer level's
+// face arrays (the arrays' layout depends only on the level layouts)
+template <class Real>
+void AMRPoissonOp<Real>::setupVCReflux(const VCCoefs<Real>& fine) {
+  if (!m_vc) throw std::runtime_error("setupVCReflux: no coefficients on this level");
+  auto idx = [](const VCCoefs<Real>& C, int box, int d, const int* iv) {
+    const DBox& f = C.h_fbox[d][box];
+    return f.off + (iv[0] - f.lo[0]) + (long long)(iv[1] - f.lo[1]) * f.sy + (long long)(iv[2] - f.lo[2]) * f.sz;
+  };
+  auto check = [](const VCCoefs<Real>& C, int box, int d, const int* iv) {
+    if (box < 0 || box >= C.lay->nbox) throw std::runtime_error("reflux face: bad box");
+    const DBox& f = C.h_fbox[d][box];
+    for (int q = 0; q < 3; q++)
+      if (iv[q] < f.lo[q] || iv[q] > f.hi[q] + (q == d ? 1 : 0))
+        throw std::runtime_error("reflux face outside its box's faces");
+  };
+  std::vector<VCFaceIdx> cf(m_rcFace.size()), ff(m_rfFace.size());
+  for (size_t r = 0; r < m_rcFace.size(); r++) {
+    const RefluxFace& x = m_rcFace[r];
+    check(*m_vc, x.box, x.dir, x.iv[0]);
+    cf[r].dir = x.dir;
+    for (int t = 0; t < 4; t++) cf[r].x[t] = idx(*m_vc, x.box, x.dir, x.iv[0]);
+  }
+  for (size_t r = 0; r < m_rfFace.size(); r++) {
+    const RefluxFace& x = m_rfFace[r];
+    ff[r].dir = x.dir;
+    for (int t = 0; t < 4; t++) {
+      check(fine, x.box, x.dir, x.iv[t]);
+      ff[r].x[t] = idx(fine, x.box, x.dir, x.iv[t]);
+    }
+  }
+  dev_free(m_ctx, d_vcRcFace);
+  dev_free(m_ctx, d_vcRfFace);
+  d_vcRcFace = cf.empty() ? nullptr : upload_vec(m_ctx, cf);
+  d_vcRfFace = ff.empty() ? nullptr : upload_vec(m_ctx, ff);
+  m_vcRefluxFine = &fine;
 }
 // computeLambda (VCAMRPoissonOp2.cpp:283-318)
 template <class Real>
@@ -338,10 +377,15 @@ void AMRPoissonOp<Real>::zeroCovered(T& lhs) {
 template <class Real>
 void AMRPoissonOp<Real>::reflux(T& phiFine, T& phi, T& res, AMRPoissonOp* finerOp) {
   need(phiFine.lay, m_finerLay, "reflux"); need(phi.lay, m_lay, "reflux"); need(res.lay, m_lay, "reflux");
-  if (m_vc || finerOp->m_vc)
-    throw std::runtime_error("reflux: variable coefficients are supported on one-level hierarchies only");
+  if (!m_vc != !finerOp->m_vc) throw std::runtime_error("reflux: variable coefficients on one level of two");
   finerOp->coarseFineInterp(phiFine, phi);
   double a_scale = 1.0 / m_dx;
+  if (m_vc) {  // VCAMRPoissonOp2::reflux (VCAMRPoissonOp2.cpp:339-430) with getFlux (:787-837)
+    if (!d_vcRcFace || m_vcRefluxFine != finerOp->m_vc.get()) setupVCReflux(*finerOp->m_vc);
+    vc::reflux<Real>(m_ctx, d_rcell, n_rcell, d_rc, d_rf, d_vcRcFace, d_vcRfFace, res.d, phi.d, phiFine.d, *m_vc,
+                     *finerOp->m_vc, m_beta * 1 / m_dx, m_beta * m_refToFiner / m_dx, -a_scale);
+    return;
+  }
   ops::reflux<Real>(m_ctx, d_rcell, n_rcell, d_rc, d_rf, res.d, phi.d, phiFine.d, m_beta * 1 / m_dx,
                     m_beta * m_refToFiner / m_dx, -a_scale);
 }
@@ -522,6 +566,8 @@ std::unique_ptr<AMRPoissonOp<Real>> AMRPoissonOpFactory<Real>::AMRnewOp(int l, A
         for (int t = 0; t < 4; t++) require(x.a[t] >= 0 && x.a[t] < fs && x.b[t] >= 0 && x.b[t] < fs, "reflux fine face");
     }
     coarserOp->n_rcell = (int)rp.cells.size();
+    coarserOp->m_rcFace = rp.coarseFace;
+    coarserOp->m_rfFace = rp.fineFace;
     coarserOp->d_rcell = upload_vec(H->ctx, rp.cells);
     coarserOp->d_rc = upload_vec(H->ctx, rp.coarse);
     coarserOp->d_rf = upload_vec(H->ctx, rp.fine);

@@ -54,6 +54,12 @@ class AMRPoissonOp : public AMRLevelOp<LevelData<Real>> {
   // GSRBHELMHOLTZVC; the one-launch constant-coefficient relaxation is not used.
   std::shared_ptr<VCCoefs<Real>> m_vc;
   int m_vcShape = -1;                              // tile shape of the one-launch VC relaxation (-1 automatic)
+  // reflux with variable coefficients: each contribution's face in the face arrays
+  std::vector<RefluxFace> m_rcFace, m_rfFace;
+  VCFaceIdx* d_vcRcFace = nullptr;
+  VCFaceIdx* d_vcRfFace = nullptr;
+  const VCCoefs<Real>* m_vcRefluxFine = nullptr;   // the finer coefficients the tables were built for
+  void setupVCReflux(const VCCoefs<Real>& fine);
   void setVC(std::shared_ptr<VCCoefs<Real>> c);   // also computes lambda (computeLambda)
   void resetLambda();                              // after alpha, beta or the coefficients change
 

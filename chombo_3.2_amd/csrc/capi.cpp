@@ -270,41 +270,66 @@ int amrmg_solver_set_alpha_beta(amrmg_solver* s, double alpha, double beta) {
   s->fabOps.clear();
   s->fabFac.reset();
   s->fabH.reset();
-  for (int d = 0; d < (int)s->vc.size(); d++) s->mgop(d)->resetLambda();
-  if (s->mixed)
-    for (int d = 0; d < (int)s->vc32.size(); d++) s->mixedop(d)->resetLambda();
+  if (!s->vc.empty()) {
+    for (auto& op : s->amg.m_op) op->resetLambda();
+    for (int d = 1; d < s->H.mgDepth(); d++) s->mgop(d)->resetLambda();
+    if (s->mixed) {
+      for (auto& op : s->mixed->amg.m_op) op->resetLambda();
+      for (int d = 1; d < s->H.mgDepth(); d++) s->mixedop(d)->resetLambda();
+    }
+  }
   API_END
 }
 int amrmg_solver_set_vc_coefs(amrmg_solver* s, const double* a, const double* b0, const double* b1,
                               const double* b2) {
   API_BEGIN
-  const int nd = s->H.mgDepth();
+  // the operators that carry coefficients: AMR levels 0..nlev-1, then the multigrid
+  // depths 1.. of level 0 (depth 0 is AMR level 0)
+  const int nd = s->H.mgDepth(), nl = s->nlev();
+  std::vector<AMRPoissonOp<double>*> ops;
+  std::vector<AMRPoissonOp<float>*> ops32;
+  std::vector<const DeviceLayout*> lays;
+  for (int l = 0; l < nl; l++) {
+    ops.push_back(s->amg.m_op[l].get());
+    if (s->mixed) ops32.push_back(s->mixed->amg.m_op[l].get());
+    lays.push_back(s->H.amrLay[l].get());
+  }
+  for (int d = 1; d < nd; d++) {
+    ops.push_back(s->mgop(d));
+    if (s->mixed) ops32.push_back(s->mixedop(d));
+    lays.push_back(s->mglay(d));
+  }
   Context* c = s->ctx->c.get();
   if (!a) {  // back to constant coefficients
-    for (int d = 0; d < nd; d++) s->mgop(d)->setVC(nullptr);
-    if (s->mixed)
-      for (int d = 0; d < nd; d++) s->mixedop(d)->setVC(nullptr);
+    for (auto* op : ops) op->setVC(nullptr);
+    for (auto* op : ops32) op->setVC(nullptr);
     s->vc.clear();
     s->vc32.clear();
   } else {
-    if (s->nlev() != 1) throw std::runtime_error("variable coefficients: one-level solvers only");
     if (!b0 || !b1 || !b2) throw std::runtime_error("variable coefficients: bCoef arrays missing");
     std::vector<std::shared_ptr<VCCoefs<double>>> v;
-    for (int d = 0; d < nd; d++) v.emplace_back(new VCCoefs<double>(s->mglay(d)));
-    v[0]->setCells(a);
+    for (const DeviceLayout* L : lays) v.emplace_back(new VCCoefs<double>(L));
+    // every AMR level its own coefficients (VCAMRPoissonOp2Factory::define, AMRnewOp)
     const double* bs[3] = {b0, b1, b2};
-    for (int q = 0; q < 3; q++) v[0]->setFaces(q, bs[q]);
-    for (int d = 1; d < nd; d++) vc::average_coefs<double>(*v[0], *v[d], 1 << d);
-    for (int d = 0; d < nd; d++) s->mgop(d)->setVC(v[d]);
+    long long ao = 0, bo[3] = {0, 0, 0};
+    for (int l = 0; l < nl; l++) {
+      v[l]->setCells(a + ao);
+      ao += lays[l]->valid_cells;
+      for (int q = 0; q < 3; q++) {
+        v[l]->setFaces(q, bs[q] + bo[q]);
+        bo[q] += v[l]->packedFaces(q);
+      }
+    }
+    // multigrid depths: level 0's coefficients averaged with ratio 2^d (MGnewOp)
+    for (int d = 1; d < nd; d++) vc::average_coefs<double>(*v[0], *v[nl + d - 1], 1 << d);
+    for (size_t q = 0; q < ops.size(); q++) ops[q]->setVC(v[q]);
     s->vc = v;
     s->vc32.clear();
-    if (s->mixed) {
-      for (int d = 0; d < nd; d++) {
-        std::shared_ptr<VCCoefs<float>> f(new VCCoefs<float>(s->mglay(d)));
-        vc::narrow(*v[d], *f);
-        s->mixedop(d)->setVC(f);
-        s->vc32.push_back(f);
-      }
+    for (size_t q = 0; q < ops32.size(); q++) {
+      std::shared_ptr<VCCoefs<float>> f(new VCCoefs<float>(lays[q]));
+      vc::narrow(*v[q], *f);
+      ops32[q]->setVC(f);
+      s->vc32.push_back(f);
     }
   }
   // the bottom of the multigrid: the device solver for constant coefficients, the

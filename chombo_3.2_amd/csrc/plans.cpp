@@ -408,6 +408,7 @@ RefluxPlanHost build_reflux_plan(const DBL& coarse, const DeviceLayout& coarseLa
   struct CellLists {
     std::vector<RefluxCoarse> c;
     std::vector<RefluxFine> f;
+    std::vector<RefluxFace> cface, fface;
   };
   std::map<std::pair<int, long long>, CellLists> cells;  // (coarse box, cell index) -> contributions
   auto covered = [&](const IV& p) { return cff.find(p) >= 0; };
@@ -440,7 +441,10 @@ RefluxPlanHost build_reflux_plan(const DBL& coarse, const DeviceLayout& coarseLa
                 x.a = coarseLay.index(C, n[0], n[1], n[2]);
                 x.b = coarseLay.index(C, m1[0], m1[1], m1[2]);
                 x.sign = side ? -1.0 : 1.0;
-                cells[{C, coarseLay.index(C, i, j, k)}].c.push_back(x);
+                RefluxFace rf{C, idir, {{n[0], n[1], n[2]}}};
+                CellLists& cl = cells[{C, coarseLay.index(C, i, j, k)}];
+                cl.c.push_back(x);
+                cl.cface.push_back(rf);
               }
   }
   // fine registers copied through the reverse ghost copier, in its sorted order
@@ -465,6 +469,7 @@ RefluxPlanHost build_reflux_plan(const DBL& coarse, const DeviceLayout& coarseLa
           }
           if (nout != 1) continue;  // interior or edge/corner ghost: register holds zero
           RefluxFine x;
+          RefluxFace xf{it.F, dir, {}};
           x.scale = (side ? 1.0 : -1.0) * 1.0 / 4.0;
           // fine cells just outside fgrid under c, INCREMENTFINE loop order
           Box fcell = refine(pointBox(c), ref);
@@ -481,10 +486,13 @@ RefluxPlanHost build_reflux_plan(const DBL& coarse, const DeviceLayout& coarseLa
                 m1[dir] -= 1;
                 x.a[t] = fineLay.index(it.F, n[0], n[1], n[2]);
                 x.b[t] = fineLay.index(it.F, m1[0], m1[1], m1[2]);
+                for (int q3 = 0; q3 < 3; q3++) xf.iv[t][q3] = n[q3];
                 t++;
               }
           if (t != 4) throw std::runtime_error("reflux: fine face must have 4 cells");
-          cells[{it.C, coarseLay.index(it.C, i, j, k)}].f.push_back(x);
+          CellLists& cl = cells[{it.C, coarseLay.index(it.C, i, j, k)}];
+          cl.f.push_back(x);
+          cl.fface.push_back(xf);
         }
   }
   for (auto& kv : cells) {
@@ -496,6 +504,8 @@ RefluxPlanHost build_reflux_plan(const DBL& coarse, const DeviceLayout& coarseLa
     rc.nf = (int)kv.second.f.size();
     out.coarse.insert(out.coarse.end(), kv.second.c.begin(), kv.second.c.end());
     out.fine.insert(out.fine.end(), kv.second.f.begin(), kv.second.f.end());
+    out.coarseFace.insert(out.coarseFace.end(), kv.second.cface.begin(), kv.second.cface.end());
+    out.fineFace.insert(out.fineFace.end(), kv.second.fface.begin(), kv.second.fface.end());
     out.cells.push_back(rc);
   }
   return out;

@@ -48,11 +48,19 @@ struct CFPlanHost {
 CFPlanHost build_cf_plan(const DBL& fine, const DeviceLayout& fineLay, const DBL& coarse,
                          const DeviceLayout& coarseLay, const Box& fineDomain, double dxFine, int ref);
 
+// The face of a reflux contribution (variable coefficients weight each flux by
+// bCoef there): box, direction and the face index (the low face of cell iv) of
+// each of its fluxes (one for a coarse record, four for a fine one).
+struct RefluxFace {
+  int box, dir;
+  int iv[4][3];
+};
 // Reflux tables of a coarse level with its finer level.
 struct RefluxPlanHost {
   std::vector<RefluxCell> cells;
   std::vector<RefluxCoarse> coarse;
   std::vector<RefluxFine> fine;
+  std::vector<RefluxFace> coarseFace, fineFace;  // beside coarse / fine, record for record
 };
 RefluxPlanHost build_reflux_plan(const DBL& coarse, const DeviceLayout& coarseLay, const DBL& fine,
                                  const DeviceLayout& fineLay, const Box& coarseDomain, int ref);

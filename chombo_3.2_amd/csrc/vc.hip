@@ -709,6 +709,57 @@ void average_coefs(const VCCoefs<T>& F, VCCoefs<T>& C, int r) {
               F.d_fbox[d], C.b[d], F.b[d], d, r, T(ref_scale));
 }
 
+template <class T>
+struct VCB3 {
+  const T* b[3];
+};
+template <class T>
+__global__ void __launch_bounds__(256) k_vc_reflux(const RefluxCell* __restrict__ cells, int n,
+                                                   const RefluxCoarse* __restrict__ rc, const RefluxFine* __restrict__ rf,
+                                                   const VCFaceIdx* __restrict__ fc, const VCFaceIdx* __restrict__ ff,
+                                                   T* u, const T* __restrict__ phic, const T* __restrict__ phif,
+                                                   VCB3<T> bc, VCB3<T> bf, T beta_dx_c, T beta_ref_dx_c, T minus_s) {
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= n) return;
+  const RefluxCell c = cells[r];
+  T val = u[c.u];
+  if (c.nc > 0) {
+    T reg = T(0.0);
+    for (int q = c.c0; q < c.c0 + c.nc; q++) {
+      const RefluxCoarse x = rc[q];
+      const VCFaceIdx f = fc[q];
+      T F = (phic[x.a] - phic[x.b]) * beta_dx_c;
+      F = -bc.b[f.dir][f.x[0]] * F;
+      reg += F * T(x.sign);
+    }
+    val = val + reg * minus_s;
+  }
+  for (int q = c.f0; q < c.f0 + c.nf; q++) {
+    const RefluxFine x = rf[q];
+    const VCFaceIdx f = ff[q];
+    T reg = T(0.0);
+    for (int t = 0; t < 4; t++) {
+      T F = (phif[x.a[t]] - phif[x.b[t]]) * beta_ref_dx_c;
+      F = -bf.b[f.dir][f.x[t]] * F;
+      reg = reg + T(x.scale) * F;
+    }
+    val = val + reg * minus_s;
+  }
+  u[c.u] = val;
+}
+template <class T>
+void reflux(Context* ctx, const RefluxCell* cells, int ncells, const RefluxCoarse* rc, const RefluxFine* rf,
+            const VCFaceIdx* fc, const VCFaceIdx* ff, T* u, const T* phic, const T* phif, const VCCoefs<T>& C,
+            const VCCoefs<T>& F, double beta_dx_c, double beta_ref_dx_c, double minus_inv_dx) {
+  if (ncells <= 0 || ctx->dry) return;
+  VCB3<T> bc{{C.b[0], C.b[1], C.b[2]}}, bf{{F.b[0], F.b[1], F.b[2]}};
+  hipEvent_t ev = profile_begin(ctx, KC_REFLUX);
+  hipLaunchKernelGGL(k_vc_reflux<T>, dim3((ncells + 255) / 256), dim3(256), 0, ctx->stream, cells, ncells, rc, rf, fc,
+                     ff, u, phic, phif, bc, bf, T(beta_dx_c), T(beta_ref_dx_c), T(minus_inv_dx));
+  hip_check(hipGetLastError(), "k_vc_reflux");
+  profile_end(ctx, ev, KC_REFLUX, 0.0);
+}
+
 __global__ void __launch_bounds__(256) k_vc_narrow(long long n, float* __restrict__ dst, const double* __restrict__ src) {
   for (long long q = blockIdx.x * 256LL + threadIdx.x; q < n; q += (long long)gridDim.x * 256) dst[q] = (float)src[q];
 }
@@ -731,7 +782,10 @@ void narrow(const VCCoefs<double>& src, VCCoefs<float>& dst) {
                                 const T*, double, double, double);                                                  \
   template void lambda<T>(VCCoefs<T>&, double, double, double);                                                     \
   template void precond_init<T>(const DeviceLayout&, const VCCoefs<T>&, T*, const T*);                              \
-  template void average_coefs<T>(const VCCoefs<T>&, VCCoefs<T>&, int);
+  template void average_coefs<T>(const VCCoefs<T>&, VCCoefs<T>&, int);                                             \
+  template void reflux<T>(Context*, const RefluxCell*, int, const RefluxCoarse*, const RefluxFine*, const VCFaceIdx*, \
+                          const VCFaceIdx*, T*, const T*, const T*, const VCCoefs<T>&, const VCCoefs<T>&, double,     \
+                          double, double);
 VC_INST(double)
 VC_INST(float)
 

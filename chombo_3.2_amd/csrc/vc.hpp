@@ -18,6 +18,13 @@
 
 namespace amrmg {
 
+// faces of one reflux contribution in a level's face arrays (x[0] for a coarse
+// record, x[0..3] for a fine one)
+struct VCFaceIdx {
+  long long x[4];
+  int dir, pad;
+};
+
 template <class Real>
 struct VCCoefs {
   Context* ctx = nullptr;
@@ -73,6 +80,13 @@ void precond_init(const DeviceLayout& L, const VCCoefs<T>& C, T* phi, const T* r
 // coefficients F averaged with ratio r (CoarseAverage / CoarseAverageFace, arithmetic)
 template <class T>
 void average_coefs(const VCCoefs<T>& F, VCCoefs<T>& C, int r);
+// VCAMRPoissonOp2::reflux (VCAMRPoissonOp2.cpp:339-430): the refluxed residual with
+// fluxes -b * gradient (getFlux, :787-837), from the tables of ops::reflux and the
+// faces of their records
+template <class T>
+void reflux(Context* ctx, const RefluxCell* cells, int ncells, const RefluxCoarse* rc, const RefluxFine* rf,
+            const VCFaceIdx* fc, const VCFaceIdx* ff, T* u, const T* phic, const T* phif, const VCCoefs<T>& C,
+            const VCCoefs<T>& F, double beta_dx_c, double beta_ref_dx_c, double minus_inv_dx);
 // FP32 copy of FP64 coefficients on the same level (a and bCoef; lambda is recomputed)
 void narrow(const VCCoefs<double>& src, VCCoefs<float>& dst);
 }  // namespace vc

@@ -68,11 +68,13 @@ int amrmg_solver_set_alpha_beta(amrmg_solver* s, double alpha, double beta);
 /* Variable coefficients: the solver's operator becomes VCAMRPoissonOp2's
    alpha a(x) phi - beta div(b(x) grad phi) (VCAMRPoissonOp2Factory::define,
    VCAMRPoissonOp2.cpp:868-940, with MGnewOp's arithmetic coefficient averaging,
-   :991-1070, and computeLambda, :283-318).  One-level solvers only.  a is
-   valid-packed; b_d (d = 0, 1, 2) holds, box after box, the faces normal to d of
-   the box grown by one face in d, in Fortran order (the packing of a FluxBox
-   component).  On a merged level, faces shared by two boxes must carry the same
-   value in both.  alpha and beta are the solver's (amrmg_solver_set_alpha_beta
+   :991-1070, AMRnewOp, :1074-1171, computeLambda, :283-318, and reflux with
+   -b * gradient fluxes, :339-430, 787-837).  a is valid-packed over all levels;
+   b_d (d = 0, 1, 2) holds, level after level and box after box, the faces normal
+   to d of the box grown by one face in d, in Fortran order (the packing of a
+   FluxBox component).  On a merged level, faces shared by two boxes must carry
+   the same value in both.  Multigrid depths of level 0 use level 0's coefficients
+   averaged arithmetically.  alpha and beta are the solver's (amrmg_solver_set_alpha_beta
    changes them and recomputes lambda).  a = NULL returns to constant
    coefficients.  With precision 1 the inner FP32 cycles use the coefficients
    rounded to FP32 (averaged in FP64 first).  The bottom solve is the

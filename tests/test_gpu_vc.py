@@ -12,7 +12,7 @@ import pytest
 
 import amrmg
 import hierarchy as H
-from helpers import small_amr, small_single
+from helpers import lshape_amr, small_amr, small_single, uncovered_mask
 from oracle import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -207,6 +207,94 @@ def test_vc_fused_relax_shapes_match_two_colour_launches(gpu_ctx, case):
     g.close()
 
 
+def _amr(name, alpha=0.0, beta=1.0):
+    if name == "amr32":
+        return small_amr(alpha=alpha, beta=beta)
+    if name == "lshape":
+        return lshape_amr(alpha=alpha, beta=beta)
+    return H.amr_hierarchy(64, fracs=(0.25, 0.06), max_box=32, block=8, alpha=alpha, beta=beta)
+
+
+@pytest.mark.parametrize("hier", ["amr32", "lshape"])
+@pytest.mark.parametrize("homog", [False, True])
+def test_vc_amr_residual_operator_bitwise(gpu_ctx, homog, hier):
+    """Composite residual and operator with variable coefficients on every level
+    (coarse-fine interpolation, b-weighted refluxing) against the oracle."""
+    print(f"vc amr residual {hier} homog={homog}", flush=True)
+    h = _amr(hier, 0.3, 1.1)
+    a, b = H.vc_coefficients(h, seed=31)
+    g = amrmg.AMRMultiGrid(gpu_ctx, h)
+    g.set_vc_coefs(a, b)
+    o = Oracle(h)
+    o.vc_set(a, b)
+    rng = np.random.default_rng(9)
+    phi = rng.standard_normal(h.total_cells())
+    rhs = rng.standard_normal(h.total_cells())
+    g.set_field(g.PHI, phi)
+    g.set_field(g.RHS, rhs)
+    rg, ng = g.computeAMRResidual(homog)
+    ro, no = o.amr_residual(phi, rhs, homog)
+    assert ng == no
+    np.testing.assert_array_equal(rg, ro)
+    lg = g.computeAMROperator(homog)
+    lo = o.amr_operator(phi, homog)
+    offs = h.level_offsets()
+    for lev in range(h.nlev):
+        m = uncovered_mask(h, lev)
+        np.testing.assert_array_equal(lg[offs[lev]:offs[lev + 1]][m], lo[offs[lev]:offs[lev + 1]][m])
+    g.close()
+
+
+@pytest.mark.parametrize("hier", ["amr32", "lshape", "amr64"])
+def test_vc_amr_solve_bitwise(gpu_ctx, hier):
+    print(f"vc amr solve {hier}", flush=True)
+    h = _amr(hier)
+    a, b = H.vc_coefficients(h, seed=33)
+    rhs = H.bump_rhs(h, 1011)
+    sp, rp = H.solver_params()
+    g = amrmg.AMRMultiGrid(gpu_ctx, h)
+    g.set_vc_coefs(a, b)
+    g.setSolverParameters(sp, rp)
+    g.set_field(g.RHS, rhs)
+    hist_g, info_g = g.solve()
+    phi_g = g.get_field(g.PHI)
+    o = Oracle(h)
+    o.vc_set(a, b)
+    phi_o, hist_o, info_o = o.solve(rhs, sp, rp)
+    print(f"{hier}: {info_g}, {hist_g[-1] / hist_g[0]:.2e}")
+    assert info_g["iterations"] == info_o["iterations"]
+    assert info_g["exit_status"] == info_o["exit_status"]
+    assert info_g["exit_status"] & 1
+    np.testing.assert_array_equal(hist_g, hist_o)
+    np.testing.assert_array_equal(phi_g, phi_o)
+    g.close()
+
+
+def test_vc_amr_mixed_precision_solve(gpu_ctx):
+    print("vc amr mixed", flush=True)
+    h = _amr("amr32")
+    a, b = H.vc_coefficients(h, seed=33)
+    rhs = H.bump_rhs(h, 1011)
+    sp, rp = H.solver_params()
+    g = amrmg.AMRMultiGrid(gpu_ctx, h, precision=1)
+    g.set_vc_coefs(a, b)
+    g.setSolverParameters(sp, rp)
+    g.set_field(g.RHS, rhs)
+    hist, info = g.solve()
+    phi = g.get_field(g.PHI)
+    o = Oracle(h)
+    o.vc_set(a, b)
+    phi_o, hist_o, info_o = o.solve(rhs, sp, rp)
+    m = np.concatenate([uncovered_mask(h, lev) for lev in range(h.nlev)])
+    err = np.abs(phi - phi_o)[m].max() / np.abs(phi_o)[m].max()
+    print(f"amr32 mixed: {info['iterations']} it, fp64 {info_o['iterations']} it, composite err {err:.2e}")
+    assert hist[0] == hist_o[0]
+    assert info["exit_status"] & 1 and hist[-1] <= 1e-10 * hist[0]
+    assert abs(info["iterations"] - info_o["iterations"]) <= 2
+    assert err < 1e-9, err
+    g.close()
+
+
 def test_vc_rejects_bad_input(gpu_ctx):
     print("vc errors", flush=True)
     h = small_single(n=32, box=16)
@@ -220,8 +308,3 @@ def test_vc_rejects_bad_input(gpu_ctx):
     with pytest.raises(amrmg.AMRMGError, match="shared by two boxes"):
         g.set_vc_coefs(a, [b[0], b[1], bad])
     g.close()
-    ha = small_amr()
-    ga = amrmg.AMRMultiGrid(gpu_ctx, ha)
-    with pytest.raises(amrmg.AMRMGError, match="one-level"):
-        ga.set_vc_coefs(np.ones(ga.ncells), [np.ones(1)] * 3)
-    ga.close()
