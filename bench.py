@@ -169,22 +169,22 @@ def mixed_leg(args, ctx, hier, rhs, sp, rp, solver64, fp64_s_per_step):
     return out
 
 
-def vc_leg(args, ctx):
-    """Variable coefficients (VCAMRPoissonOp2): one level of base^3 cells in 64^3
-    boxes, -div(b grad phi) = rhs with a smooth positive face coefficient b of
-    contrast 4, solved to 1e-10 in FP64 and in mixed precision.  The GSRB class's
-    algorithmic bytes are 64 B per cell-relax in FP64 (phi read and written, rhs,
-    aCoef, lambda and the three face arrays read once per red-black iteration)."""
+def vc_leg(args, ctx, h, rhs):
+    """Variable coefficients (VCAMRPoissonOp2) on the benchmark's AMR hierarchy:
+    -div(b grad phi) = rhs with a smooth positive face coefficient b of contrast 4
+    on every level, solved to 1e-10 in FP64 and in mixed precision.  The GSRB
+    class's algorithmic bytes are 56 B per cell-relax in FP64 (phi read and
+    written, rhs, aCoef and the three face arrays read once per red-black
+    iteration; lambda is a function of aCoef and bCoef)."""
     import amrmg
     import hierarchy as H
     t0 = time.time()
-    h = H.single_level(args.base, 64, 8, alpha=0.0, beta=1.0)
     a, b = H.vc_coefficients(h, seed=7, contrast=4.0)
-    rhs = H.sinusoid_rhs(h, 1007, k=(1, 2, 1))
-    log(f"vc: {len(h.levels[0])} boxes, {h.cells(0)} cells, coefficients in {time.time() - t0:.1f}s")
+    log(f"vc: coefficients for {h.total_cells()} cells in {time.time() - t0:.1f}s")
     sp, rp = H.solver_params(eps=1e-10, hang=1e-15, iterMax=args.iter_max)
-    out = {"workload": f"one level {args.base}^3, 64^3 boxes, Dirichlet 0, alpha=0 beta=1, "
-                       f"b = 1 + 3(0.5 + 0.5 sin cos cos) on faces, a smooth, tol 1e-10"}
+    out = {"workload": f"the 3-level AMR hierarchy above (base {args.base}^3), Dirichlet 0, alpha=0 beta=1, "
+                       f"b = 1 + 3(0.5 + 0.5 sin cos cos) on faces of every level, a smooth, Gaussian-bump rhs, "
+                       f"tol 1e-10"}
     for prec in (0, 1):
         s = amrmg.AMRMultiGrid(ctx, h, precision=prec)
         s.setSolverParameters(sp, rp)
@@ -482,7 +482,7 @@ def main():
     vcout = None
     if not args.no_vc:
         solver.close()
-        vcout = vc_leg(args, ctx)
+        vcout = vc_leg(args, ctx, hier, rhs)
     batch = None
     if args.batch > 0:
         solver.close()

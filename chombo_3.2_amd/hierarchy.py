@@ -267,20 +267,32 @@ def bump_rhs(h: Hierarchy, seed: int, nbumps: int = 3) -> np.ndarray:
 
 def vc_coefficients(h: "Hierarchy", seed: int = 7, contrast: float = 4.0):
     """Smooth positive variable coefficients on every level (VCAMRPoissonOp2):
-    a(x) = 1 + 0.5 sin(2 pi (x + phase)) cos(2 pi y) sin(2 pi z) (cells) and
-    b(x) = 1 + (contrast - 1) * (0.5 + 0.5 sin(2 pi (x + phase)) cos(2 pi (y + phase)) cos(pi z))
-    (faces), phases from the seed.  Returns (a valid-packed over all levels,
-    [b0, b1, b2] each face-packed level after level)."""
+    a(x) = 1 + 0.5 sin(2 pi (x + p0)) cos(2 pi y) sin(2 pi z) at cell centres and
+    b(x) = 1 + (contrast - 1) * (0.5 + 0.5 sin(2 pi (x + p1)) cos(2 pi (y + p2)) cos(pi z))
+    at face centres, phases p from the seed.  Both are products of one-dimensional
+    factors, evaluated per box as outer products.  Returns (a valid-packed over all
+    levels, [b0, b1, b2] each face-packed level after level)."""
     rng = np.random.default_rng(seed)
     ph = rng.uniform(0, 1, size=3)
-    a = h.pack(lambda x, y, z, l: 1.0 + 0.5 * np.sin(2 * math.pi * (x + ph[0])) * np.cos(2 * math.pi * y)
-               * np.sin(2 * math.pi * z))
+    fa = (lambda x: np.sin(2 * math.pi * (x + ph[0])), lambda y: np.cos(2 * math.pi * y),
+          lambda z: np.sin(2 * math.pi * z))
+    fb = (lambda x: np.sin(2 * math.pi * (x + ph[1])), lambda y: np.cos(2 * math.pi * (y + ph[2])),
+          lambda z: np.cos(math.pi * z))
 
-    def bf(x, y, z):
-        return 1.0 + (contrast - 1.0) * (0.5 + 0.5 * np.sin(2 * math.pi * (x + ph[1])) * np.cos(2 * math.pi * (y + ph[2]))
-                                         * np.cos(math.pi * z))
+    def prod(fs, axes):
+        return np.multiply.outer(np.multiply.outer(fs[0](axes[0]), fs[1](axes[1])), fs[2](axes[2]))
 
-    return a, [np.concatenate([h.pack_faces(bf, d, lev) for lev in range(h.nlev)]) for d in range(3)]
+    a_parts, b_parts = [], [[], [], []]
+    for lev in range(h.nlev):
+        dx = h.dx(lev)
+        for (lo, hi) in h.levels[lev]:
+            cen = [(np.arange(lo[q], hi[q] + 1) + 0.5) * dx for q in range(3)]
+            a_parts.append((1.0 + 0.5 * prod(fa, cen)).ravel(order="F"))
+            for d in range(3):
+                ax = list(cen)
+                ax[d] = np.arange(lo[d], hi[d] + 2) * dx
+                b_parts[d].append((1.0 + (contrast - 1.0) * (0.5 + 0.5 * prod(fb, ax))).ravel(order="F"))
+    return np.concatenate(a_parts), [np.concatenate(p) for p in b_parts]
 
 
 DEFAULT_SOLVER = dict(pre=2, post=2, bottom=2, numMG=1, iterMax=100, imin=5,
