@@ -43,6 +43,10 @@ def lib():
         L.amrmg_profile_read_units.argtypes = [C.c_void_p, C.c_int, _dp]
         L.amrmg_solver_set_alpha_beta.argtypes = [C.c_void_p, C.c_double, C.c_double]
         L.amrmg_solver_set_vc_coefs.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+        L.amrmg_heat_create.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]
+        L.amrmg_heat_set_source.argtypes = [C.c_void_p, _dp]
+        L.amrmg_heat_step.argtypes = [C.c_void_p, C.c_double, C.c_int, _ip]
+        L.amrmg_heat_destroy.argtypes = [C.c_void_p]
         L.amrmg_serve_create.argtypes = [C.c_int, C.c_int, C.c_longlong, C.c_int, C.POINTER(C.c_void_p)]
         L.amrmg_serve_set_params.argtypes = [C.c_void_p, _ip, _dp]
         L.amrmg_serve_submit.argtypes = [C.c_void_p, _ip, _dp, _dp, C.POINTER(C.c_longlong)]
@@ -351,6 +355,44 @@ class AMRMultiGrid:
         _chk(lib().amrmg_bottom_solve(self.p, corr.ctypes.data_as(_dp), pr, nbottom, metric, eps,
                                        info.ctypes.data_as(_ip)))
         return corr, int(info[0])
+
+
+class HeatSolver:
+    """BackwardEuler (scheme "be", BackwardEuler.cpp) or AMRTGA (scheme "tga",
+    AMRTGA.H) over an AMRMultiGrid: d phi/dt = L phi + S with the solver's operator
+    as L.  step() advances the solver's PHI field by dt in place."""
+
+    SCHEMES = {"be": 0, "tga": 1}
+
+    def __init__(self, solver: "AMRMultiGrid", scheme: str = "tga"):
+        self.solver = solver
+        p = C.c_void_p()
+        _chk(lib().amrmg_heat_create(solver.p, self.SCHEMES[scheme], C.byref(p)))
+        self.p = p
+
+    def close(self):
+        if self.p:
+            lib().amrmg_heat_destroy(self.p)
+            self.p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_source(self, src):
+        src, ps = _d(src)
+        if src.size != self.solver.ncells:
+            raise ValueError("source size does not match the hierarchy")
+        _chk(lib().amrmg_heat_set_source(self.p, ps))
+
+    def step(self, dt, zero_phi=False):
+        """One step; returns ({iterations, exit_status} of each solve)."""
+        info = np.zeros(4, dtype=np.int32)
+        _chk(lib().amrmg_heat_step(self.p, float(dt), 1 if zero_phi else 0, info.ctypes.data_as(_ip)))
+        return (dict(iterations=int(info[0]), exit_status=int(info[1])),
+                dict(iterations=int(info[2]), exit_status=int(info[3])))
 
 
 class BatchSolver:

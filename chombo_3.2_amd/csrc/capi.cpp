@@ -600,6 +600,128 @@ int amrmg_solver_set_vc_relax_shape(amrmg_solver* s, int shape) {
     for (int d = 0; d < s->H.mgDepth(); d++) s->mixedop(d)->m_vcShape = shape;
   API_END
 }
+// ---- heat-equation steppers (BackwardEuler, AMRTGA) over a solver ----
+// d phi/dt = L phi + S with the solver's operator as L: every step resets alpha and
+// beta of all the solver's operators, as the reference's steppers do.
+struct amrmg_heat {
+  amrmg_solver* s = nullptr;
+  int scheme = 0;                                   // 0 BackwardEuler, 1 AMRTGA
+  double mu1 = 0, mu2 = 0, mu3 = 0, mu4 = 0;
+  std::vector<std::unique_ptr<LevelData<double>>> src, old, rhst, srct;
+  AMRPoissonOp<double>* op(int l) { return s->amg.m_op[l].get(); }
+  // VCAMRPoissonOp2::diagonalScale / divideByIdentityCoef; no-ops for AMRPoissonOp
+  void aScale(int l, LevelData<double>& x, bool divide) {
+    if (op(l)->m_vc) vc::scale_by_a<double>(*x.lay, *op(l)->m_vc, x.d, divide);
+  }
+  void setAlphaBeta(double alpha, double beta) {
+    if (amrmg_solver_set_alpha_beta(s, alpha, beta) != 0) throw std::runtime_error(g_err);
+  }
+  // AMRMultiGrid::solveNoInit (zeroPhi as given) on the solver's phi with rhs rhst
+  void solve(bool zeroPhi, int* info) {
+    auto ph = s->ptrs(s->phi), rh = s->ptrs(rhst), ur = s->ptrs(s->uberRes), uc = s->ptrs(s->uberCorr);
+    s->amg.solveNoInitResid(ph, ur, uc, rh, s->nlev() - 1, 0, zeroPhi);
+    info[0] = s->amg.m_iterations;
+    info[1] = s->amg.m_exitStatus;
+  }
+};
+
+int amrmg_heat_create(amrmg_solver* s, int scheme, amrmg_heat** out) {
+  API_BEGIN
+  if (scheme != 0 && scheme != 1) throw std::runtime_error("heat scheme must be 0 (backward Euler) or 1 (TGA)");
+  std::unique_ptr<amrmg_heat> h(new amrmg_heat());
+  h->s = s;
+  h->scheme = scheme;
+  // AMRTGA::setMu (AMRTGA.H:368-384)
+  const double tgaEpsilon = 1.e-12;
+  const double a = 2.0 - std::sqrt(2.0) - tgaEpsilon;
+  h->mu1 = (a - std::sqrt(a * a - 4.0 * a + 2.0)) / 2.0;
+  h->mu2 = (a + std::sqrt(a * a - 4.0 * a + 2.0)) / 2.0;
+  h->mu3 = (1.0 - a);
+  h->mu4 = 0.5 - a;
+  for (int l = 0; l < s->nlev(); l++) {
+    const DeviceLayout* L = s->H.amrLay[l].get();
+    h->src.emplace_back(new LevelData<double>(L));
+    h->old.emplace_back(new LevelData<double>(L));
+    h->rhst.emplace_back(new LevelData<double>(L));
+    h->srct.emplace_back(new LevelData<double>(L));
+    ops::setval<double>(*L, h->src[l]->d, 0.0);
+  }
+  s->ctx->c->sync();
+  *out = h.release();
+  API_END
+}
+int amrmg_heat_destroy(amrmg_heat* h) {
+  API_BEGIN
+  delete h;
+  API_END
+}
+int amrmg_heat_set_source(amrmg_heat* h, const double* valid_packed) {
+  API_BEGIN
+  long long o = 0;
+  for (int l = 0; l < h->s->nlev(); l++) {
+    const DeviceLayout& L = *h->s->H.amrLay[l];
+    upload_valid(L, h->src[l]->d, valid_packed + o);
+    o += L.valid_cells;
+  }
+  API_END
+}
+int amrmg_heat_step(amrmg_heat* h, double dt, int zero_phi, int* info) {
+  API_BEGIN
+  amrmg_solver* s = h->s;
+  const int nl = s->nlev();
+  int i1[2] = {0, 0}, i2[2] = {0, 0};
+  for (int l = 0; l < nl; l++) h->op(l)->assignLocal(*h->old[l], *s->phi[l]);
+  if (h->scheme == 0) {
+    // BackwardEuler::createEulerRHS (BackwardEuler.cpp:196-233): rhs = a phi^n + dt S
+    for (int l = 0; l < nl; l++) {
+      AMRPoissonOp<double>* op = h->op(l);
+      op->setToZero(*h->rhst[l]);
+      op->incr(*h->rhst[l], *h->old[l], 1.0);
+      h->aScale(l, *h->rhst[l], false);
+      op->setToZero(*h->srct[l]);
+      op->incr(*h->srct[l], *h->src[l], 1.0);
+      op->scale(*h->srct[l], dt);
+      op->incr(*h->rhst[l], *h->srct[l], 1.0);
+    }
+    // oneStep: phi^{n+1} starts from phi^n; solveHelm (:236-254): alpha 1, beta -dt
+    h->setAlphaBeta(1.0, -dt);
+    h->solve(zero_phi != 0, i1);
+  } else {
+    // AMRTGA::oneStep (AMRTGA.H:420-510)
+    for (int l = 0; l < nl; l++) {
+      h->op(l)->setToZero(*h->srct[l]);
+      h->op(l)->incr(*h->srct[l], *h->src[l], 1.0);
+      h->aScale(l, *h->srct[l], true);
+    }
+    auto srct = s->ptrs(h->srct), rhst = s->ptrs(h->rhst), ph = s->ptrs(s->phi), old = s->ptrs(h->old);
+    h->setAlphaBeta(1.0, h->mu4 * dt);                     // applyHelm (:514-537)
+    s->amg.computeAMROperator(rhst, srct, nl - 1, 0, true);
+    for (int l = 0; l < nl; l++) h->op(l)->scale(*h->rhst[l], dt);
+    h->setAlphaBeta(1.0, h->mu3 * dt);
+    s->amg.computeAMROperator(ph, old, nl - 1, 0, false);
+    for (int l = 0; l < nl; l++) {
+      h->op(l)->incr(*h->rhst[l], *s->phi[l], 1.0);
+      h->op(l)->assignLocal(*s->phi[l], *h->old[l]);
+    }
+    h->setAlphaBeta(1.0, -h->mu2 * dt);                    // solveHelm (:540-560)
+    h->solve(false, i1);
+    for (int l = 0; l < nl; l++) {
+      h->op(l)->assignLocal(*h->rhst[l], *s->phi[l]);
+      h->aScale(l, *h->rhst[l], false);
+      h->op(l)->assignLocal(*s->phi[l], *h->old[l]);
+    }
+    h->setAlphaBeta(1.0, -h->mu1 * dt);
+    h->solve(false, i2);
+  }
+  s->ctx->c->sync();
+  if (info) {
+    info[0] = i1[0];
+    info[1] = i1[1];
+    info[2] = i2[0];
+    info[3] = i2[1];
+  }
+  API_END
+}
 // ---- batched single-level solves ----
 int amrmg_batch_create(amrmg_ctx* ctx, int nprob, const int* ispecs, const double* rspecs, int precision,
                        amrmg_batch** out) {

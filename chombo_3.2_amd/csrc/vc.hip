@@ -710,6 +710,21 @@ void average_coefs(const VCCoefs<T>& F, VCCoefs<T>& C, int r) {
 }
 
 template <class T>
+__global__ void __launch_bounds__(256) k_vc_scale_a(const DBox* __restrict__ boxes, const Tile* __restrict__ tiles,
+                                                    Geom g, T* x, const T* __restrict__ a, int divide) {
+  DBox b;
+  int box, i, j, k;
+  if (!tile_cell(boxes, tiles, g, box, b, i, j, k)) return;
+  const long long c = cidx(b, i, j, k);
+  x[c] = divide ? x[c] / a[c] : x[c] * a[c];
+}
+template <class T>
+void scale_by_a(const DeviceLayout& L, const VCCoefs<T>& C, T* x, bool divide) {
+  need_coefs(L, C, "vc::scale_by_a");
+  VC_LAUNCH(L, KC_BLAS, 0.0, 0.0, k_vc_scale_a<T>, L.ntiles, L.d_boxes, L.d_tiles, geom(L), x, C.a.d, divide ? 1 : 0);
+}
+
+template <class T>
 struct VCB3 {
   const T* b[3];
 };
@@ -783,6 +798,7 @@ void narrow(const VCCoefs<double>& src, VCCoefs<float>& dst) {
   template void lambda<T>(VCCoefs<T>&, double, double, double);                                                     \
   template void precond_init<T>(const DeviceLayout&, const VCCoefs<T>&, T*, const T*);                              \
   template void average_coefs<T>(const VCCoefs<T>&, VCCoefs<T>&, int);                                             \
+  template void scale_by_a<T>(const DeviceLayout&, const VCCoefs<T>&, T*, bool);                                    \
   template void reflux<T>(Context*, const RefluxCell*, int, const RefluxCoarse*, const RefluxFine*, const VCFaceIdx*, \
                           const VCFaceIdx*, T*, const T*, const T*, const VCCoefs<T>&, const VCCoefs<T>&, double,     \
                           double, double);

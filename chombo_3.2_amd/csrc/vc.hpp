@@ -87,6 +87,10 @@ template <class T>
 void reflux(Context* ctx, const RefluxCell* cells, int ncells, const RefluxCoarse* rc, const RefluxFine* rf,
             const VCFaceIdx* fc, const VCFaceIdx* ff, T* u, const T* phic, const T* phif, const VCCoefs<T>& C,
             const VCCoefs<T>& F, double beta_dx_c, double beta_ref_dx_c, double minus_inv_dx);
+// diagonalScale / divideByIdentityCoef (VCAMRPoissonOp2.H:120-140): x *= aCoef,
+// resp. x /= aCoef on valid cells
+template <class T>
+void scale_by_a(const DeviceLayout& L, const VCCoefs<T>& C, T* x, bool divide);
 // FP32 copy of FP64 coefficients on the same level (a and bCoef; lambda is recomputed)
 void narrow(const VCCoefs<double>& src, VCCoefs<float>& dst);
 }  // namespace vc

@@ -147,6 +147,22 @@ int amrmg_solver_set_vc_relax_shape(amrmg_solver* s, int shape);
    each).  Per problem the arithmetic is that of amrmg_solver_solve on it alone
    (AMRMultiGrid::solve with one level).  No counterpart in the reference, which
    solves one hierarchy at a time. ---- */
+/* Heat-equation steppers, d phi/dt = L phi + S with the solver's operator as L
+   (AMRPoissonOp: alpha I + beta Laplacian; VCAMRPoissonOp2: alpha a - beta div(b grad)).
+   scheme 0: BackwardEuler (BackwardEuler.cpp:47-254): rhs = a phi^n + dt S, then
+   alpha = 1, beta = -dt and AMRMultiGrid::solve from phi^n (zero_phi = 0) or zero.
+   scheme 1: AMRTGA (AMRTGA.H:204-563): the two-stage TGA step with its mu1..mu4
+   (setMu), two solves from phi^n.  The step reads phi^n from the solver's field 0
+   and leaves phi^{n+1} there; S is set with amrmg_heat_set_source (valid-packed,
+   zero by default).  Like the reference's steppers, a step resets alpha and beta of
+   every operator of the solver.  info = {iterations, exitStatus} of the first solve,
+   then of the second (TGA; zero for backward Euler). */
+typedef struct amrmg_heat amrmg_heat;
+int amrmg_heat_create(amrmg_solver* s, int scheme, amrmg_heat** out);
+int amrmg_heat_set_source(amrmg_heat* h, const double* valid_packed);
+int amrmg_heat_step(amrmg_heat* h, double dt, int zero_phi, int* info);
+int amrmg_heat_destroy(amrmg_heat* h);
+
 typedef struct amrmg_batch amrmg_batch;
 /* nprob problems, each a one-level (ispec, rspec) as for amrmg_solver_create,
    ispecs back to back (14 + 6*nbox ints each), rspecs 9 doubles each; each

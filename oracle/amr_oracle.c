@@ -1681,6 +1681,20 @@ int oracle_vc_set(OHier* h, const double* a, const double* b0, const double* b1,
   }
   return 0;
 }
+/* setAlphaAndBeta (AMRPoissonOp.cpp:1103-1108, VCAMRPoissonOp2.cpp:258-266) on every
+   operator of the hierarchy (AMRMultiGrid::getAllOperators), lambda recomputed */
+int oracle_set_alpha_beta(OHier* h, double alpha, double beta) {
+  h->alpha = alpha; h->beta = beta;
+  for (int l = 0; l < h->nlev; l++) {
+    h->op[l].alpha = alpha; h->op[l].beta = beta;
+    if (h->op[l].vc) op_vc_lambda(&h->op[l]);
+  }
+  for (int d = 1; d < h->mgD; d++) {
+    h->mgop[d].alpha = alpha; h->mgop[d].beta = beta;
+    if (h->mgop[d].vc) op_vc_lambda(&h->mgop[d]);
+  }
+  return 0;
+}
 long oracle_face_cells(const OHier* h, int lev, int dir) {
   long n = 0;
   for (int q = 0; q < h->grids[lev].n; q++) { Box f = vc_facebox(h->grids[lev].b[q], dir); n += b_npts(&f); }

@@ -92,6 +92,8 @@ int oracle_bottom_solve(OHier* h, double* corr, const double* res, int nbottom,
    the coefficients averaged from level 0; reflux uses -b * gradient fluxes). */
 int  oracle_vc_set(OHier* h, const double* a, const double* b0, const double* b1, const double* b2);
 long oracle_face_cells(const OHier* h, int lev, int dir);
+/* setAlphaAndBeta on every operator (AMR levels and MG depths), lambda recomputed */
+int  oracle_set_alpha_beta(OHier* h, double alpha, double beta);
 int  oracle_vc_lambda(OHier* h, int depth, double* lambda_out);
 
 /* number of cell-relaxations (one red+black GSRB iteration of one cell) since reset */

@@ -59,6 +59,7 @@ def lib():
         L.oracle_face_cells.restype = C.c_long
         L.oracle_face_cells.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.oracle_vc_lambda.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.oracle_set_alpha_beta.argtypes = [C.c_void_p, C.c_double, C.c_double]
         _lib = L
     return _lib
 
@@ -202,6 +203,59 @@ class Oracle:
 
     def face_cells(self, lev, d):
         return lib().oracle_face_cells(self.h, lev, d)
+
+    def set_alpha_beta(self, alpha, beta):
+        """setAlphaAndBeta on every operator (AMRMultiGrid::getAllOperators)."""
+        self._chk(lib().oracle_set_alpha_beta(self.h, float(alpha), float(beta)))
+
+    # ---- heat solvers over the hierarchy's AMRMultiGrid (valid-packed fields) ----
+    def _acoef(self):
+        return self._vc[0] if getattr(self, "_vc", None) is not None else None
+
+    def backward_euler_step(self, phi_old, src, dt, sp, rp, zero_phi=True):
+        """BackwardEuler::oneStep (BackwardEuler.cpp:145-188) with createEulerRHS
+        (:196-233) and solveHelm (:236-254): rhs = a phi^n + dt src (a = 1 for
+        AMRPoissonOp, aCoef for VCAMRPoissonOp2), then alpha = 1, beta = -dt and
+        AMRMultiGrid::solve from phi^n (or zero)."""
+        a = self._acoef()
+        rhs = 0.0 + np.asarray(phi_old, dtype=np.float64)      # setVal(0); += phiOld
+        if a is not None:
+            rhs = rhs * a                                      # diagonalScale
+        rhs = rhs + (0.0 + np.asarray(src, dtype=np.float64)) * dt   # scaleRho = dt * rho; += scaleRho
+        self.set_alpha_beta(1.0, -dt)
+        return self.solve(rhs, sp, rp, phi0=None if zero_phi else phi_old)
+
+    @staticmethod
+    def tga_mu():
+        """AMRTGA::setMu (AMRTGA.H:368-384)."""
+        eps = 1.e-12
+        a = 2.0 - np.sqrt(2.0) - eps
+        r = np.sqrt(a * a - 4.0 * a + 2.0)
+        return (a - r) / 2.0, (a + r) / 2.0, (1.0 - a), 0.5 - a
+
+    def tga_step(self, phi_old, src, dt, sp, rp):
+        """AMRTGA::oneStep (AMRTGA.H:420-510) with applyHelm (:514-537) and solveHelm
+        (:540-560).  Returns phi^{n+1} and the two solves' (history, info)."""
+        mu1, mu2, mu3, mu4 = self.tga_mu()
+        a = self._acoef()
+        phi_old = np.asarray(phi_old, dtype=np.float64)
+        srct = 0.0 + np.asarray(src, dtype=np.float64) * 1.0   # setToZero; incr(src, 1)
+        if a is not None:
+            srct = srct / a                                    # divideByIdentityCoef
+        self.set_alpha_beta(1.0, mu4 * dt)
+        rhst = self.amr_operator(srct, homog=True)             # (I + mu4 dt L) S/a
+        rhst = rhst * dt                                       # scale(dt)
+        self.set_alpha_beta(1.0, mu3 * dt)
+        phi_new = self.amr_operator(phi_old, homog=False)      # (I + mu3 dt L) phi^n
+        rhst = rhst + phi_new * 1.0                            # incr
+        self.set_alpha_beta(1.0, -mu2 * dt)
+        phi_new, h1, i1 = self.solve(rhst, sp, rp, phi0=phi_old)    # solveNoInit, zeroPhi false
+        rhst = phi_new.copy()                                  # assignLocal
+        if a is not None:
+            rhst = rhst * a                                    # diagonalScale
+        self.set_alpha_beta(1.0, -mu1 * dt)
+        phi_new, h2, i2 = self.solve(rhst, sp, rp, phi0=phi_old)
+        return phi_new, (h1, i1), (h2, i2)
 
     def vc_lambda(self, depth):
         out = np.zeros(self.mg_cells(depth))
