@@ -217,6 +217,39 @@ def vc_leg(args, ctx, h, rhs):
     return out
 
 
+def heat_leg(args, ctx, h, rhs):
+    """Time stepping over the benchmark's AMR hierarchy: d phi/dt = Laplacian(phi) + S
+    with the Gaussian bump as S, from phi = 0, dt = 1e-3, AMRTGA (two Helmholtz
+    solves per step from phi^n) and BackwardEuler (one solve per step from phi^n),
+    FP64, each solve to 1e-10.  Steps after the first are warm-started, which is the
+    use the device-resident hierarchy serves."""
+    import amrmg
+    import hierarchy as H
+    sp, rp = H.solver_params(eps=1e-10, hang=1e-15, iterMax=args.iter_max)
+    out = {"workload": f"the 3-level AMR hierarchy above (base {args.base}^3), dphi/dt = lap(phi) + S, "
+                       f"S = Gaussian bump, phi0 = 0, dt = 1e-3, each solve to 1e-10, FP64"}
+    nsteps = max(args.steps, 2)
+    for scheme in ("tga", "be"):
+        s = amrmg.AMRMultiGrid(ctx, h)
+        s.setSolverParameters(sp, rp)
+        s.set_field(s.PHI, 0.0 * rhs)
+        heat = amrmg.HeatSolver(s, scheme)
+        heat.set_source(rhs)
+        heat.step(1e-3)  # warm-up step (not timed)
+        its = []
+        t0 = time.perf_counter()
+        for _ in range(nsteps):
+            i1, i2 = heat.step(1e-3)
+            its.append((i1["iterations"], i2["iterations"]) if scheme == "tga" else (i1["iterations"],))
+        wall = time.perf_counter() - t0
+        out[scheme] = {"ms_per_step": round(wall / nsteps * 1e3, 3), "steps_per_s": round(nsteps / wall, 4),
+                       "solve_iterations_per_step": its[-1]}
+        log(f"heat {scheme}: {out[scheme]}")
+        heat.close()
+        s.close()
+    return out
+
+
 def batch_leg(args, ctx):
     """The batched workload: independent single-level Helmholtz problems of edge
     16..128 (log-uniform), solved (a) one after another with the plain FP64 solver,
@@ -404,6 +437,7 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the relaxation autotuner (default shapes)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--no-vc", action="store_true", help="skip the variable-coefficient leg")
+    ap.add_argument("--no-heat", action="store_true", help="skip the heat-stepping leg")
     ap.add_argument("--batch", type=int, default=512, help="problems in the batched leg (0 = skip)")
     ap.add_argument("--batch-lanes", type=int, default=3, choices=(1, 2, 3),
                     help="2 or 3: the problems with large multigrid bottoms lockstep on further streams")
@@ -483,6 +517,10 @@ def main():
     if not args.no_vc:
         solver.close()
         vcout = vc_leg(args, ctx, hier, rhs)
+    heat = None
+    if not args.no_heat:
+        solver.close()
+        heat = heat_leg(args, ctx, hier, rhs)
     batch = None
     if args.batch > 0:
         solver.close()
@@ -546,6 +584,7 @@ def main():
                         if tune else None),
         "mixed_precision": mixed,
         "variable_coefficients": vcout,
+        "heat": heat,
         "batched": batch,
         "serving": serve,
         "cpu_baseline": cpu,
