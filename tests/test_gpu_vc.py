@@ -270,6 +270,71 @@ def test_vc_amr_solve_bitwise(gpu_ctx, hier):
     g.close()
 
 
+@pytest.mark.parametrize("variant", ["wcycle", "neumann_mixed_faces", "negidx"])
+def test_vc_amr_solve_variants_bitwise(gpu_ctx, variant):
+    """W-cycles (numMG = 2), Neumann faces on the low sides (a Helmholtz problem,
+    alpha > 0, so it stays definite), and an AMR hierarchy at negative indices."""
+    print(f"vc amr variant {variant}", flush=True)
+    kw = {}
+    if variant == "wcycle":
+        h = _amr("amr32")
+        kw = dict(numMG=2)
+    elif variant == "neumann_mixed_faces":
+        h = small_amr(alpha=0.8, beta=1.0)
+        h.bc_type_lo = (0, 0, 0)
+    else:
+        base = H.box((-32, -32, -32), (-1, -1, -1))
+        lev1 = H.domain_split(H.box((-48, -48, -48), (-17, -17, -17)), 16, 4)
+        h = H.Hierarchy(base, [H.domain_split(base, 16, 4), lev1], 1.0 / 32, 0.2, 1.0)
+    a, b = H.vc_coefficients(h, seed=35)
+    rhs = H.sinusoid_rhs(h, 1013) if variant == "negidx" else H.bump_rhs(h, 1013)
+    sp, rp = H.solver_params(**kw)
+    g = amrmg.AMRMultiGrid(gpu_ctx, h)
+    g.set_vc_coefs(a, b)
+    g.setSolverParameters(sp, rp)
+    g.set_field(g.RHS, rhs)
+    hist_g, info_g = g.solve()
+    phi_g = g.get_field(g.PHI)
+    o = Oracle(h)
+    o.vc_set(a, b)
+    phi_o, hist_o, info_o = o.solve(rhs, sp, rp)
+    print(f"{variant}: {info_g}")
+    if variant != "wcycle":
+        # (numMG = 2 on an AMR hierarchy does not converge, as with constant
+        # coefficients: the oracle stops on the hang test and the GPU must follow it)
+        assert info_g["exit_status"] & 1
+    assert info_g["iterations"] == info_o["iterations"]
+    assert info_g["exit_status"] == info_o["exit_status"]
+    np.testing.assert_array_equal(hist_g, hist_o)
+    np.testing.assert_array_equal(phi_g, phi_o)
+    g.close()
+
+
+def test_vc_full_size_linearity_and_determinism(gpu_ctx):
+    """The benchmark's 3-level hierarchy (113.5 M cells) with variable coefficients:
+    the solve converges to 1e-10, repeats bitwise, and solving with 2 rhs gives
+    exactly 2 phi (scaling by 2 is exact in every operation of the cycle)."""
+    print("vc full size", flush=True)
+    h = H.amr_hierarchy(256, fracs=(0.25, 0.06), max_box=64, block=8)
+    a, b = H.vc_coefficients(h, seed=7)
+    rhs = H.bump_rhs(h, 1002)
+    sp, rp = H.solver_params()
+    g = amrmg.AMRMultiGrid(gpu_ctx, h)
+    g.set_vc_coefs(a, b)
+    g.setSolverParameters(sp, rp)
+    g.set_field(g.RHS, rhs)
+    hist1, info1 = g.solve()
+    phi1 = g.get_field(g.PHI)
+    assert info1["exit_status"] & 1 and hist1[-1] <= 1e-10 * hist1[0]
+    hist2, _ = g.solve()
+    np.testing.assert_array_equal(hist2, hist1)
+    np.testing.assert_array_equal(g.get_field(g.PHI), phi1)
+    g.set_field(g.RHS, 2 * rhs)
+    g.solve()
+    np.testing.assert_array_equal(g.get_field(g.PHI), 2 * phi1)
+    g.close()
+
+
 def test_vc_amr_mixed_precision_solve(gpu_ctx):
     print("vc amr mixed", flush=True)
     h = _amr("amr32")
