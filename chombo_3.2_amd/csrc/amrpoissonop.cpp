@@ -302,14 +302,36 @@ void AMRPoissonOp<Real>::coarseFineInterp(T& phif, const T& phic) {
 template <class Real>
 void AMRPoissonOp<Real>::AMRResidual(T& res, T& phiFine, T& phi, T& phiCoarse, const T& rhs, bool homog,
                                      AMRLevelOp<T>* finerOp) {
+  if (!m_vc && phiFine.isDefined()) {
+    if (phiCoarse.isDefined()) coarseFineInterp(phi, phiCoarse);
+    residualFused(res, phiFine, phi, rhs, homog, static_cast<AMRPoissonOp*>(finerOp));
+    return;
+  }
   AMROperator(res, phiFine, phi, phiCoarse, homog, finerOp);
   axby(res, res, rhs, -1.0, 1.0);
 }
 template <class Real>
 void AMRPoissonOp<Real>::AMRResidualNC(T& res, T& phiFine, T& phi, const T& rhs, bool homog,
                                        AMRLevelOp<T>* finerOp) {
+  if (!m_vc && phiFine.isDefined()) {
+    residualFused(res, phiFine, phi, rhs, homog, static_cast<AMRPoissonOp*>(finerOp));
+    return;
+  }
   AMROperatorNC(res, phiFine, phi, homog, finerOp);
   axby(res, res, rhs, -1.0, 1.0);
+}
+// applyOpI + reflux + axby(res, res, rhs, -1, 1) of AMRResidual / AMRResidualNC in two
+// launches (valid cells, then the refluxed cells), bitwise the same values
+template <class Real>
+void AMRPoissonOp<Real>::residualFused(T& res, T& phiFine, T& phi, const T& rhs, bool homog, AMRPoissonOp* finerOp) {
+  need(res.lay, m_lay, "AMRResidual"); need(rhs.lay, m_lay, "AMRResidual");
+  need(phiFine.lay, m_finerLay, "AMRResidual"); need(phi.lay, m_lay, "AMRResidual");
+  fillGhosts(phi, false, homog);
+  ops::apply_residual<Real>(*m_lay, res.d, phi.d, rhs.d, m_dx, m_alpha, m_beta);
+  finerOp->coarseFineInterp(phiFine, phi);
+  const double a_scale = 1.0 / m_dx;
+  ops::reflux_residual<Real>(m_ctx, d_rcell, n_rcell, d_rc, d_rf, res.d, phi.d, phiFine.d, rhs.d, m_beta * 1 / m_dx,
+                             m_beta * m_refToFiner / m_dx, -a_scale, m_dx, m_alpha, m_beta);
 }
 // AMRResidualNF (:770-787)
 template <class Real>
@@ -339,8 +361,18 @@ void AMRPoissonOp<Real>::AMROperatorNF(T& LofPhi, T& phi, T& phiCoarse, bool hom
 template <class Real>
 void AMRPoissonOp<Real>::AMRRestrictS(T& resCoarse, const T& residual, T& correction, T& coarseCorrection,
                                       T& scratch) {
-  AMRResidualNF(scratch, correction, coarseCorrection, residual, true);
   need(resCoarse.lay, m_cfLay, "AMRRestrictS");
+  if (!m_vc) {
+    // AMRResidualNF (coarse-fine interpolation, homogeneous ghost refresh,
+    // OPERATORLAPRES) and the average in one pass: scratch is only scratch space
+    // here and is not written
+    need(correction.lay, m_lay, "AMRRestrictS"); need(residual.lay, m_lay, "AMRRestrictS");
+    if (coarseCorrection.isDefined()) coarseFineInterp(correction, coarseCorrection);
+    fillGhosts(correction, false, true);
+    ops::residual_average<Real>(*m_lay, *m_cfLay, resCoarse.d, correction.d, residual.d, m_dx, m_alpha, m_beta);
+    return;
+  }
+  AMRResidualNF(scratch, correction, coarseCorrection, residual, true);
   ops::average<Real>(*m_lay, *m_cfLay, resCoarse.d, scratch.d);
 }
 // AMRProlongS (:959-990)

@@ -111,6 +111,7 @@ class AMRPoissonOp : public AMRLevelOp<LevelData<Real>> {
   void homogeneousCFInterp(T& phi);
   void coarseFineInterp(T& phif, const T& phic);   // QuadCFInterp of this level from the coarser one
   void reflux(T& phiFine, T& phi, T& res, AMRPoissonOp* finerOp);
+  void residualFused(T& res, T& phiFine, T& phi, const T& rhs, bool homog, AMRPoissonOp* finerOp);
   void localMaxNormAsync(const T& x, int slot, bool accumulate);
   // Autotuner: times the one-launch relaxation's tile shapes on (phi, rhs) of this
   // level and keeps the fastest.  No-op (returns false) when the level relaxes

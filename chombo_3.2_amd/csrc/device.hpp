@@ -88,6 +88,7 @@ struct RefluxCell {
   long long u;         // output cell in the coarse residual/operator slab
   int c0, nc;          // coarse flux contributions [c0, c0+nc)
   int f0, nf;          // fine register contributions [f0, f0+nf)
+  int sy, sz;          // strides of the cell's box (to re-form its 7-point stencil)
 };
 struct RefluxCoarse {  // reg += (phi[a]-phi[b])*(beta/dx) * sign
   long long a, b;

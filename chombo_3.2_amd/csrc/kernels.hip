@@ -642,6 +642,63 @@ void apply(const DeviceLayout& L, T* lofphi, const T* phi, double dx, double alp
   LAUNCH_P(KC_APPLY, 2.0 * sizeof(T) * L.valid_cells, k_apply<T>, L.ntiles, L.d_boxes, L.d_tiles, geom(L), lofphi, phi, T(dxinv), T(alpha), T(beta));
 }
 
+// AMRResidual's operator and axby in one pass: res = (-1) * (alpha*phi + beta*lap) +
+// rhs * 1, OPERATORLAP's expression followed by LevelDataOps::axby's, on valid cells
+// (the cells that reflux corrects are rewritten by k_reflux_residual)
+template <class T>
+__device__ __forceinline__ T apply_axby(T lofphi, T r) {
+  T v = lofphi;
+  v = v * T(-1.0);
+  return v + r * T(1.0);
+}
+template <class T>
+__global__ void __launch_bounds__(256) k_apply_residual(const DBox* __restrict__ boxes, const Tile* __restrict__ tiles,
+                                                        TileGeom g, T* __restrict__ res, const T* __restrict__ phi,
+                                                        const T* __restrict__ rhs, T dxinv, T alpha, T beta) {
+  DBox b;
+  int i, j, k;
+  if (!tile_cell(boxes, tiles, g, b, i, j, k)) return;
+  const long long c = cidx(b, i, j, k);
+  const long long sy = b.sy, sz = b.sz;
+  const T p = phi[c];
+  const T lap = (phi[c + 1] + phi[c - 1] + phi[c + sy] + phi[c - sy] + phi[c + sz] + phi[c - sz] - T(6.0) * p) * dxinv;
+  res[c] = apply_axby(alpha * p + beta * lap, rhs[c]);
+}
+template <class T>
+__global__ void __launch_bounds__(256) k_apply_residual_z(const DBox* __restrict__ boxes,
+                                                          const Tile* __restrict__ ctiles, int nct, int kz,
+                                                          T* __restrict__ res, const T* __restrict__ phi,
+                                                          const T* __restrict__ rhs, T dxinv, T alpha, T beta) {
+  const int w = xcd_contiguous_block();
+  if (w >= nct) return;
+  const Tile t = ctiles[w];
+  const DBox b = boxes[t.box];
+  const int i = t.i0 + (threadIdx.x & 63), j = t.j0 + (threadIdx.x >> 6);
+  if (i > b.hi[0] || j > b.hi[1]) return;
+  const int kend = min(t.k0 + kz - 1, b.hi[2]);
+  const long long sy = b.sy, sz = b.sz;
+  long long c = cidx(b, i, j, t.k0);
+  T pm = phi[c - sz], p = phi[c];
+  for (int k = t.k0; k <= kend; k++, c += sz) {
+    const T pz = phi[c + sz];
+    const T lap = (phi[c + 1] + phi[c - 1] + phi[c + sy] + phi[c - sy] + pz + pm - T(6.0) * p) * dxinv;
+    res[c] = apply_axby(alpha * p + beta * lap, rhs[c]);
+    pm = p;
+    p = pz;
+  }
+}
+template <class T>
+void apply_residual(const DeviceLayout& L, T* res, const T* phi, const T* rhs, double dx, double alpha, double beta) {
+  double dxinv = 1.0 / (dx * dx);
+  if (L.merged && L.nctiles > 0) {
+    LAUNCH_P(KC_APPLY, 3.0 * sizeof(T) * L.valid_cells, k_apply_residual_z<T>, (L.nctiles + 7) / 8 * 8, L.d_boxes,
+             L.d_ctiles, L.nctiles, DeviceLayout::kColumnPlanes, res, phi, rhs, T(dxinv), T(alpha), T(beta));
+    return;
+  }
+  LAUNCH_P(KC_APPLY, 3.0 * sizeof(T) * L.valid_cells, k_apply_residual<T>, L.ntiles, L.d_boxes, L.d_tiles, geom(L), res,
+           phi, rhs, T(dxinv), T(alpha), T(beta));
+}
+
 // OPERATORLAPRES (AMRPoissonOpF.ChF:341-377): r = -alpha*phi - beta*lap + rhs
 template <class T>
 __global__ void __launch_bounds__(256) k_residual(const DBox* __restrict__ boxes, const Tile* __restrict__ tiles,
@@ -777,6 +834,43 @@ void average(const DeviceLayout& Lf, const DeviceLayout& Lc, T* coarse, const T*
   const DeviceLayout& L = Lc;
   check_paired(Lf, Lc, "average");
   LAUNCH_P(KC_AVERAGE, (1.0 + 1.0 / 8.0) * sizeof(T) * Lf.valid_cells, k_average<T>, Lc.ntiles, Lc.d_boxes, Lc.d_tiles, geom(Lc), Lf.d_boxes, coarse, fine);
+}
+
+// AMRRestrictS's residual and average in one pass: OPERATORLAPRES on the eight fine
+// cells of each coarse cell of coarsen(fine, 2), then AVERAGE's refRatio-2 sum, with
+// the residuals in registers.  The same expressions in the same order as k_residual
+// followed by k_average, so the coarse values are bitwise theirs; the fine residual
+// itself is not stored.
+template <class T>
+__global__ void __launch_bounds__(256) k_residual_average(const DBox* __restrict__ cboxes,
+                                                          const Tile* __restrict__ ctiles, TileGeom g,
+                                                          const DBox* __restrict__ fboxes, T* __restrict__ coarse,
+                                                          const T* __restrict__ phi, const T* __restrict__ rhs,
+                                                          T dxinv, T alpha, T beta) {
+  DBox cb;
+  int I, J, K;
+  if (!tile_cell(cboxes, ctiles, g, cb, I, J, K)) return;
+  const DBox fb = fboxes[ctiles[blockIdx.x].box];
+  const long long c = cidx(fb, 2 * I, 2 * J, 2 * K);
+  const long long sy = fb.sy, sz = fb.sz;
+  auto res = [&](long long q) {
+    const T p = phi[q];
+    const T lap = (phi[q + 1] + phi[q - 1] + phi[q + sy] + phi[q - sy] + phi[q + sz] + phi[q - sz] - T(6.0) * p) * dxinv;
+    return -(alpha * p) - beta * lap + rhs[q];
+  };
+  const T r0 = res(c), r1 = res(c + 1), r2 = res(c + sy), r3 = res(c + 1 + sy);
+  const T r4 = res(c + sz), r5 = res(c + 1 + sz), r6 = res(c + sy + sz), r7 = res(c + 1 + sy + sz);
+  const T refScale = T(1.0 / 8.0);
+  coarse[cidx(cb, I, J, K)] = refScale * (r0 + r1 + r2 + r3 + r4 + r5 + r6 + r7);
+}
+template <class T>
+void residual_average(const DeviceLayout& Lf, const DeviceLayout& Lc, T* coarse, const T* phi, const T* rhs,
+                      double dx, double alpha, double beta) {
+  const DeviceLayout& L = Lc;
+  check_paired(Lf, Lc, "residual_average");
+  double dxinv = 1.0 / (dx * dx);
+  LAUNCH_P(KC_RESTRICT, (2.0 + 1.0 / 8.0) * sizeof(T) * Lf.valid_cells, k_residual_average<T>, Lc.ntiles, Lc.d_boxes,
+           Lc.d_tiles, geom(Lc), Lf.d_boxes, coarse, phi, rhs, T(dxinv), T(alpha), T(beta));
 }
 
 // ---------------------------------------------------------------------------
@@ -1246,6 +1340,56 @@ __global__ void __launch_bounds__(256) k_reflux(const RefluxCell* __restrict__ c
   }
   u[c.u] = val;
 }
+// the refluxed cells of AMRResidual: the operator value re-formed from phi (bitwise
+// k_apply's), the reflux corrections of k_reflux in its order, then axby's expression
+template <class T>
+__global__ void __launch_bounds__(256) k_reflux_residual(const RefluxCell* __restrict__ cells, int n,
+                                                         const RefluxCoarse* __restrict__ rc,
+                                                         const RefluxFine* __restrict__ rf, T* res,
+                                                         const T* __restrict__ phic, const T* __restrict__ phif,
+                                                         const T* __restrict__ rhs, T beta_dx_c, T beta_ref_dx_c,
+                                                         T minus_s, T dxinv, T alpha, T beta) {
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= n) return;
+  const RefluxCell c = cells[r];
+  const long long q = c.u, sy = c.sy, sz = c.sz;
+  const T p = phic[q];
+  const T lap = (phic[q + 1] + phic[q - 1] + phic[q + sy] + phic[q - sy] + phic[q + sz] + phic[q - sz] - T(6.0) * p) *
+                dxinv;
+  T val = alpha * p + beta * lap;
+  if (c.nc > 0) {
+    T reg = T(0.0);
+    for (int t = c.c0; t < c.c0 + c.nc; t++) {
+      const RefluxCoarse x = rc[t];
+      const T F = (phic[x.a] - phic[x.b]) * beta_dx_c;
+      reg += F * T(x.sign);
+    }
+    val = val + reg * minus_s;
+  }
+  for (int t = c.f0; t < c.f0 + c.nf; t++) {
+    const RefluxFine x = rf[t];
+    T reg = T(0.0);
+    for (int m = 0; m < 4; m++) {
+      const T F = (phif[x.a[m]] - phif[x.b[m]]) * beta_ref_dx_c;
+      reg = reg + T(x.scale) * F;
+    }
+    val = val + reg * minus_s;
+  }
+  res[q] = apply_axby(val, rhs[q]);
+}
+template <class T>
+void reflux_residual(Context* ctx, const RefluxCell* cells, int ncells, const RefluxCoarse* rc, const RefluxFine* rf,
+                     T* res, const T* phic, const T* phif, const T* rhs, double beta_dx_c, double beta_ref_dx_c,
+                     double minus_inv_dx, double dx, double alpha, double beta) {
+  if (ncells <= 0) return;
+  const double dxinv = 1.0 / (dx * dx);
+  hipEvent_t ev = prof_begin(ctx, KC_REFLUX);
+  hipLaunchKernelGGL(k_reflux_residual<T>, dim3((ncells + 255) / 256), dim3(256), 0, ctx->stream, cells, ncells, rc,
+                     rf, res, phic, phif, rhs, T(beta_dx_c), T(beta_ref_dx_c), T(minus_inv_dx), T(dxinv), T(alpha),
+                     T(beta));
+  hip_check(hipGetLastError(), "k_reflux_residual");
+  prof_end(ctx, ev, KC_REFLUX, 0.0);
+}
 template <class T>
 void reflux(Context* ctx, const RefluxCell* cells, int ncells, const RefluxCoarse* rc, const RefluxFine* rf, T* u,
             const T* phic, const T* phif, double beta_dx_c, double beta_ref_dx_c, double minus_inv_dx) {
@@ -1273,6 +1417,11 @@ void reflux(Context* ctx, const RefluxCell* cells, int ncells, const RefluxCoars
                                 double);                                                                      \
   template void prolong<T>(const DeviceLayout&, const DeviceLayout&, T*, const T*);                            \
   template void average<T>(const DeviceLayout&, const DeviceLayout&, T*, const T*);                            \
+  template void residual_average<T>(const DeviceLayout&, const DeviceLayout&, T*, const T*, const T*, double,     \
+                                    double, double);                                                          \
+  template void apply_residual<T>(const DeviceLayout&, T*, const T*, const T*, double, double, double);        \
+  template void reflux_residual<T>(Context*, const RefluxCell*, int, const RefluxCoarse*, const RefluxFine*, T*, \
+                                   const T*, const T*, const T*, double, double, double, double, double, double); \
   template void incr<T>(const DeviceLayout&, T*, const T*, double);                                            \
   template void incr_assign<T>(const DeviceLayout&, T*, T*);                                                   \
   template void axby<T>(const DeviceLayout&, T*, const T*, const T*, double, double);                          \

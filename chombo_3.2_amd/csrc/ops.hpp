@@ -79,6 +79,11 @@ template <class T>
 void prolong(const DeviceLayout& Lf, const DeviceLayout& Lc, T* phi, const T* coarse);
 template <class T>
 void average(const DeviceLayout& Lf, const DeviceLayout& Lc, T* coarse, const T* fine);
+// coarse = AVERAGE(rhs - L(phi)) over coarsen(fine, 2) without storing the fine
+// residual (bitwise residual followed by average); phi's ghosts must be filled
+template <class T>
+void residual_average(const DeviceLayout& Lf, const DeviceLayout& Lc, T* coarse, const T* phi, const T* rhs,
+                      double dx, double alpha, double beta);
 
 template <class T>
 void incr(const DeviceLayout& L, T* lhs, const T* x, double scale);
@@ -122,6 +127,15 @@ template <class T>
 void cf_inhomog(Context* ctx, const CFCoarse* dc, int ncoarse, const CFGen* dg, const CFFine* df, int nfine,
                 T* deriv_scratch, T* phif, const T* phic, double dxf, double dxc);
 
+// AMRResidual's AMROperator + axby(res, res, rhs, -1, 1) in two launches: every valid
+// cell from its stencil, then the refluxed cells again with their corrections;
+// bitwise the operator, reflux and axby launches
+template <class T>
+void apply_residual(const DeviceLayout& L, T* res, const T* phi, const T* rhs, double dx, double alpha, double beta);
+template <class T>
+void reflux_residual(Context* ctx, const RefluxCell* cells, int ncells, const RefluxCoarse* rc, const RefluxFine* rf,
+                     T* res, const T* phic, const T* phif, const T* rhs, double beta_dx_c, double beta_ref_dx_c,
+                     double minus_inv_dx, double dx, double alpha, double beta);
 // reflux (AMRPoissonOp::reflux after the fine level's CF ghosts are filled)
 template <class T>
 void reflux(Context* ctx, const RefluxCell* cells, int ncells, const RefluxCoarse* rc, const RefluxFine* rf,

@@ -498,6 +498,8 @@ RefluxPlanHost build_reflux_plan(const DBL& coarse, const DeviceLayout& coarseLa
   for (auto& kv : cells) {
     RefluxCell rc;
     rc.u = kv.first.second;
+    rc.sy = coarseLay.h_boxes[kv.first.first].sy;
+    rc.sz = coarseLay.h_boxes[kv.first.first].sz;
     rc.c0 = (int)out.coarse.size();
     rc.nc = (int)kv.second.c.size();
     rc.f0 = (int)out.fine.size();
