@@ -813,11 +813,14 @@ void AMRMultiGrid<Real>::solveNoInitResid(std::vector<T*>& phi, std::vector<T*>&
   bool goHang = iter < m_imin || rnorm < (1 - m_hang) * norm_last, goMin = iter < m_iterMin;
   while (goMin || (goIter && goRedu && goHang && goNorm)) {
     norm_last = rnorm;
-    if (m_innerCycle) m_innerCycle(uberCorrection, uberResidual, l_max, l_base);
-    else AMRVCycle(uberCorrection, uberResidual, l_max, l_max, l_base);
-    for (int ilev = l_base; ilev <= l_max; ilev++) {
-      m_op[ilev]->incr(*phi[ilev], *uberCorrection[ilev], 1.0);
-      m_op[ilev]->setToZero(*uberCorrection[ilev]);
+    if (m_innerCycleUpdate) {
+      m_innerCycleUpdate(phi, uberResidual, l_max, l_base);  // uberCorrection stays zero
+    } else {
+      AMRVCycle(uberCorrection, uberResidual, l_max, l_max, l_base);
+      for (int ilev = l_base; ilev <= l_max; ilev++) {
+        m_op[ilev]->incr(*phi[ilev], *uberCorrection[ilev], 1.0);
+        m_op[ilev]->setToZero(*uberCorrection[ilev]);
+      }
     }
     rnorm = computeAMRResidual(uberResidual, phi, rhs, l_max, l_base, false, true);
     iter++;

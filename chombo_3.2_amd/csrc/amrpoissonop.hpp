@@ -171,10 +171,12 @@ class AMRMultiGrid {
   std::vector<double> m_history;
   int m_iterations = 0;
   // Mixed precision: when set, solveNoInitResid runs this in place of
-  // AMRVCycle(uberCorrection, uberResidual, l_max, l_max, l_base) (an inner
-  // correction cycle in another precision), and m_innerBottomMetrics receives the
-  // bottom solver's convergence metrics.
-  std::function<void(std::vector<T*>&, std::vector<T*>&, int, int)> m_innerCycle;
+  // AMRVCycle(uberCorrection, uberResidual, l_max, l_max, l_base) followed by
+  // phi += uberCorrection and uberCorrection = 0: an inner correction cycle in another
+  // precision, given phi and uberResidual, that adds its correction c to phi as
+  // phi + (0 + c) * 1 (those two steps' arithmetic with uberCorrection zero); and
+  // m_innerBottomMetrics receives the bottom solver's convergence metrics.
+  std::function<void(std::vector<T*>&, std::vector<T*>&, int, int)> m_innerCycleUpdate;
   std::function<void(double, double)> m_innerBottomMetrics;
 
   void define(AMRPoissonOpFactory<Real>& factory, LinearSolver<T>* bottom, int numLevels);

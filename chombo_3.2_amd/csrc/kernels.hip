@@ -994,6 +994,19 @@ __global__ void __launch_bounds__(256) k_incr_f2d(const DBox* __restrict__ boxes
 void incr_f2d(const DeviceLayout& L, double* lhs, const float* x) {
   LAUNCH(k_incr_f2d, L.ntiles, L.d_boxes, L.d_tiles, geom(L), lhs, x);
 }
+// phi + (0 + (double)x) * 1: k_incr_f2d on a zero field followed by k_incr(phi, ., 1)
+__global__ void __launch_bounds__(256) k_update_f2d(const DBox* __restrict__ boxes, const Tile* __restrict__ tiles,
+                                                    TileGeom g, double* phi, const float* x) {
+  DBox b;
+  int i, j, k;
+  if (!tile_cell(boxes, tiles, g, b, i, j, k)) return;
+  const long long c = cidx(b, i, j, k);
+  const double u = 0.0 + (double)x[c];
+  phi[c] = phi[c] + u * 1.0;
+}
+void update_f2d(const DeviceLayout& L, double* phi, const float* x) {
+  LAUNCH(k_update_f2d, L.ntiles, L.d_boxes, L.d_tiles, geom(L), phi, x);
+}
 __global__ void __launch_bounds__(256) k_f2d(const DBox* __restrict__ boxes, const Tile* __restrict__ tiles,
                                              TileGeom g, double* dst, const float* src) {
   DBox b;

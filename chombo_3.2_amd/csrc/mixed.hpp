@@ -80,8 +80,8 @@ struct MixedInner {
     std::vector<LevelData<float>*> c, r;
     for (int l = 0; l < H->nlev(); l++) { c.push_back(corr[l].get()); r.push_back(res[l].get()); }
     amg.init(c, r, H->nlev() - 1, 0);
-    amg64.m_innerCycle = [this](std::vector<LevelData<double>*>& uc, std::vector<LevelData<double>*>& ur, int l_max,
-                                int l_base) { cycle(uc, ur, l_max, l_base); };
+    amg64.m_innerCycleUpdate = [this](std::vector<LevelData<double>*>& phi, std::vector<LevelData<double>*>& ur,
+                                      int l_max, int l_base) { cycle(phi, ur, l_max, l_base); };
     amg64.m_innerBottomMetrics = [this](double metric, double tol) { active->setConvergenceMetrics(metric, tol); };
   }
   // bind the bottom again after the operators' coefficients change: the device
@@ -99,8 +99,9 @@ struct MixedInner {
                            double normThresh) {
     amg.setSolverParameters(pre, post, nbottom, numMG, iterMax, eps, hang, normThresh);
   }
-  // uberCorrection (zero on entry) += the FP32 correction cycle's result for uberResidual
-  void cycle(std::vector<LevelData<double>*>& uc, std::vector<LevelData<double>*>& ur, int l_max, int l_base) {
+  // phi += the FP32 correction cycle's result for uberResidual, with the arithmetic of
+  // uberCorrection (zero) += widened correction, phi += uberCorrection * 1
+  void cycle(std::vector<LevelData<double>*>& phi, std::vector<LevelData<double>*>& ur, int l_max, int l_base) {
     std::vector<LevelData<float>*> c, r;
     for (size_t l = 0; l < corr.size(); l++) { c.push_back(corr[l].get()); r.push_back(res[l].get()); }
     for (int l = l_base; l <= l_max; l++) {
@@ -108,7 +109,7 @@ struct MixedInner {
       ops::setval<float>(*c[l]->lay, c[l]->d, 0.0);
     }
     amg.AMRVCycle(c, r, l_max, l_max, l_base);
-    for (int l = l_base; l <= l_max; l++) ops::incr_f2d(*uc[l]->lay, uc[l]->d, c[l]->d);
+    for (int l = l_base; l <= l_max; l++) ops::update_f2d(*phi[l]->lay, phi[l]->d, c[l]->d);
   }
 };
 

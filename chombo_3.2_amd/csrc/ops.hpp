@@ -106,6 +106,8 @@ void precond_init(const DeviceLayout& L, T* phi, const T* rhs, double mult);
 void convert_valid_d2f(const DeviceLayout& L, float* dst, const double* src);
 void incr_f2d(const DeviceLayout& L, double* lhs, const float* x);
 void convert_valid_f2d(const DeviceLayout& L, double* dst, const float* src);
+// phi += widened x, as incr_f2d into a zero field followed by incr(phi, field, 1)
+void update_f2d(const DeviceLayout& L, double* phi, const float* x);
 
 template <class T>
 void motion(const MotionPlan& P, const DeviceLayout& Lsrc, const DeviceLayout& Ldst, T* dst, const T* src,
