@@ -479,6 +479,8 @@ static void validate_ghost_plan(const ops::GhostPlan& P, const DBL& d) {
       lo[p.dir] += s;
       hi[p.dir] += s;
       require(in_grown(d, p.dst, lo, hi, 1), "ghost rule reads outside its box");
+      // quadratic Dirichlet reads two valid cells: the box must be two cells wide there
+      if (p.kind == PK_DIRICHLET2) require(in_grown(d, p.dst, lo, hi, 0), "quadratic Dirichlet face of a box one cell wide");
     }
   }
 }
@@ -495,8 +497,9 @@ static void validate_pair(const DeviceLayout& f, const DeviceLayout& c) {
 
 // The one-launch relaxation applies when the level is stored merged and every
 // ghost rule is the rule of one whole face of its rectangle: homogeneous
-// Dirichlet or Neumann, or quadratic coarse-fine (a coarse-fine face of a box one
-// cell wide uses the linear rule and keeps the two-launch path).
+// Dirichlet (linear) or Neumann, or quadratic coarse-fine (a coarse-fine face of a
+// box one cell wide uses the linear rule and keeps the two-launch path; so does a
+// quadratic Dirichlet face, whose rule the one-launch kernel does not form).
 template <class Real>
 static void setup_fused(AMRPoissonOp<Real>* op) {
   const DeviceLayout& L = *op->m_lay;

@@ -936,6 +936,8 @@ BatchSolver::BatchSolver(Context* ctx, std::vector<BatchProblemSpec> problems, i
       c.beta = P.beta;
       for (int dir = 0; dir < 3; dir++)
         for (int side = 0; side < 2; side++) {
+          if (P.bc.type[side][dir] > 1)
+            throw std::runtime_error("batch: quadratic Dirichlet faces are not supported (use the single solver)");
           c.kind[dir][side] = P.bc.type[side][dir];
           c.val[dir][side] = P.bc.val[side][dir];
         }
@@ -1526,6 +1528,7 @@ void DeviceBottomSolver<T>::define(LinearOp<LevelData<T>>* op, bool homogeneous)
       const int face = side ? b.rect.hi[dir] : b.rect.lo[dir];
       const int dface = side ? ap->m_domain.hi[dir] : ap->m_domain.lo[dir];
       if (face != dface) return;  // a level with coarse-fine faces keeps the host-driven solver
+      if (ap->m_bc.type[side][dir] > 1) return;  // quadratic Dirichlet faces: the host-driven solver
       c.kind[dir][side] = ap->m_bc.type[side][dir];
       c.val[dir][side] = ap->m_bc.val[side][dir];
     }

@@ -166,7 +166,7 @@ static std::vector<DBL> parse_spec(const int* is, const double* rs, Box& dom, BC
     bc.val[0][d] = rs[3 + d];
     bc.val[1][d] = rs[6 + d];
     for (int s = 0; s < 2; s++)
-      if (bc.type[s][d] != 0 && bc.type[s][d] != 1) throw std::runtime_error("unknown BC type");
+      if (bc.type[s][d] < 0 || bc.type[s][d] > 2) throw std::runtime_error("unknown BC type");
   }
   dx0 = rs[0];
   a = rs[1];

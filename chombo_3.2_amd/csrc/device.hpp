@@ -38,6 +38,7 @@ enum PatchKind : int {
   PK_NEUMANN = 2,    // ConstBCFunction type 0: g = near (+ isign*dx*v)
   PK_CFHOMO = 3,     // INTERPHOMO: g = c1*phi(1 in) + c2*phi(2 in)
   PK_CFHOMOLIN = 4,  // INTERPHOMOLINEAR: g = factor*phi(1 in)
+  PK_DIRICHLET2 = 5, // DiriBC order 2 (quadraticInterp): g = (8/3)*v + (1/3)*phi(2 in) - 2*phi(1 in)
 };
 struct Patch {
   int kind, dst, src, dir, side;

@@ -386,6 +386,13 @@ __global__ void __launch_bounds__(256) k_fill_ghosts(const DBox* __restrict__ bo
     T inhomogVal = T(0.0);
     if (!homog) inhomogVal = T(p.val);
     phi[g] = T(2) * inhomogVal - nearv;
+  } else if (p.kind == PK_DIRICHLET2) {  // DiriBC order 2 (BCFunc.cpp:34-40, 291-354)
+    int b2[3] = {a[0], a[1], a[2]};
+    b2[p.dir] -= isign;  // two cells inward
+    T inhomogVal = T(0.0);
+    if (!homog) inhomogVal = T(p.val);
+    const T farv = phi[cidx(bd, b2[0], b2[1], b2[2])];
+    phi[g] = T(8.0 / 3.0) * inhomogVal + T(1.0 / 3.0) * farv - T(2) * nearv;
   } else if (p.kind == PK_NEUMANN) {
     T v = nearv;
     if (!homog) v += T(double(isign) * dx) * T(p.val);

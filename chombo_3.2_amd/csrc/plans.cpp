@@ -104,7 +104,8 @@ ops::GhostPlan build_ghost_plan(const DBL& dbl, const Box& domain, const BCSpec&
         int dend = side ? domain.hi[dir] : domain.lo[dir];
         if (vend == dend) {
           int t = bc.type[side][dir];
-          P.h_patches.push_back(make_patch(t == 1 ? PK_DIRICHLET : PK_NEUMANN, b, -1, dir, side, R, bc.val[side][dir]));
+          const int kind = t == 1 ? PK_DIRICHLET : (t == 2 ? PK_DIRICHLET2 : PK_NEUMANN);
+          P.h_patches.push_back(make_patch(kind, b, -1, dir, side, R, bc.val[side][dir]));
           continue;
         }
         IVS cf(R & domain);

@@ -11,7 +11,7 @@
 namespace amrmg {
 
 struct BCSpec {
-  int type[2][3];     // [side][dir]: 1 Dirichlet, 0 Neumann
+  int type[2][3];     // [side][dir]: 0 Neumann, 1 Dirichlet (linear), 2 Dirichlet (quadratic, DiriBC order 2)
   double val[2][3];
 };
 

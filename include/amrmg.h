@@ -40,6 +40,9 @@
 extern "C" {
 #endif
 
+/* BC type codes in ispec: 0 Neumann (ConstBCFunction type 0), 1 Dirichlet with the
+   linear rule 2v - near (ConstBCFunction type 1), 2 Dirichlet with DiriBC's order-2
+   rule (8/3)v + (1/3)far - 2near (BCFunc.cpp:34-40, 291-354); values are constant per face. */
 typedef struct amrmg_ctx amrmg_ctx;
 typedef struct amrmg_solver amrmg_solver;
 

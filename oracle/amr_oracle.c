@@ -459,6 +459,18 @@ static void apply_bc(Fab* st, const Box* valid, const Box* domain, double dx, in
             for (int j = to.lo[1]; j <= to.hi[1]; j++)
               for (int i = to.lo[0]; i <= to.hi[0]; i++)
                 *P(st,i,j,k) += (double)isign*dx*value;
+      } else if (type == 2) {    /* DiriBC order 2, quadraticInterp (BCFunc.cpp:34-40, 291-354) */
+        for (int k = to.lo[2]; k <= to.hi[2]; k++)
+          for (int j = to.lo[1]; j <= to.hi[1]; j++)
+            for (int i = to.lo[0]; i <= to.hi[0]; i++) {
+              int c[3] = {i,j,k}; c[idir] -= isign;
+              int f[3] = {c[0],c[1],c[2]}; f[idir] -= isign;
+              double inhomogVal = 0.0;
+              if (!homog) inhomogVal = value;
+              double nearVal = *P(st,c[0],c[1],c[2]);
+              double farVal = *P(st,f[0],f[1],f[2]);
+              *P(st,i,j,k) = (8.0/3.0)*inhomogVal + (1.0/3.0)*(farVal) - 2*(nearVal);
+            }
       } else {                   /* Dirichlet, linearInterp: 2*v - near */
         for (int k = to.lo[2]; k <= to.hi[2]; k++)
           for (int j = to.lo[1]; j <= to.hi[1]; j++)
