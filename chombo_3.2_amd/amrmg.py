@@ -43,6 +43,7 @@ def lib():
         L.amrmg_profile_read_units.argtypes = [C.c_void_p, C.c_int, _dp]
         L.amrmg_solver_set_alpha_beta.argtypes = [C.c_void_p, C.c_double, C.c_double]
         L.amrmg_solver_set_vc_coefs.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+        L.amrmg_level_norm.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _dp]
         L.amrmg_heat_create.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]
         L.amrmg_heat_set_source.argtypes = [C.c_void_p, _dp]
         L.amrmg_heat_step.argtypes = [C.c_void_p, C.c_double, C.c_int, _ip]
@@ -298,6 +299,13 @@ class AMRMultiGrid:
         out = np.zeros_like(phi)
         _chk(lib().amrmg_level_apply(self.p, depth, pp, out.ctypes.data_as(_dp), int(homog)))
         return out
+
+    def level_norm(self, depth, x, p):
+        """norm(BoxLayoutData, interval, p) on multigrid depth `depth` (p = 0, 1, 2)."""
+        x, px = _d(x)
+        out = np.zeros(1)
+        _chk(lib().amrmg_level_norm(self.p, int(depth), px, int(p), out.ctypes.data_as(_dp)))
+        return float(out[0])
 
     def level_residual(self, depth, phi, rhs, homog=False):
         phi, pp = _d(phi)

@@ -134,8 +134,9 @@ template <class Real>
 double AMRPoissonOp<Real>::norm(const T& x, int ord) {
   need(x.lay, x.lay, "norm");
   if (ord == 0) return ops::maxnorm<Real>(*x.lay, x.d, 2);
+  if (ord == 1) return ops::norm1_ordered<Real>(*x.lay, x.d, 3);
   if (ord == 2) return ops::norm2_ordered<Real>(*x.lay, x.d, 3);
-  throw std::runtime_error("AMRPoissonOp::norm: order must be 0 or 2");
+  throw std::runtime_error("AMRPoissonOp::norm: order must be 0, 1 or 2");
 }
 template <class Real>
 void AMRPoissonOp<Real>::setToZero(T& lhs) {

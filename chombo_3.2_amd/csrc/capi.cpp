@@ -431,6 +431,14 @@ int amrmg_level_apply(amrmg_solver* s, int depth, const double* phi, double* out
   download_valid(*L, o.d, out);
   API_END
 }
+int amrmg_level_norm(amrmg_solver* s, int depth, const double* x, int p, double* out) {
+  API_BEGIN
+  const DeviceLayout* L = s->mglay(depth);
+  LevelData<double> v(L);
+  upload_valid(*L, v.d, x);
+  *out = s->mgop(depth)->norm(v, p);
+  API_END
+}
 int amrmg_level_residual(amrmg_solver* s, int depth, const double* phi, const double* rhs, double* out,
                          int homog) {
   API_BEGIN

@@ -1121,7 +1121,14 @@ __global__ void __launch_bounds__(256) k_dot_ordered(const DBox* __restrict__ bo
 }
 // norm(.., 2): per box FArrayBox::sumPow(2) (column sums over rows, then over
 // columns, FArrayBox.cpp:165-230), boxes summed in layout order, then sqrt.
-template <class T>
+// ABS: norm(.., 1), FArrayBox::norm(1) (FArrayBox.cpp:110-141): the same column
+// sums of |x|, no root.
+template <bool ABS, class T>
+__device__ __forceinline__ double sumpow_term(T v) {
+  const double r = (double)v;
+  return ABS ? fabs(r) : r * r;
+}
+template <class T, bool ABS = false>
 __global__ void __launch_bounds__(256) k_sumpow_ordered(const DBox* __restrict__ boxes, int nbox, const T* x,
                                                         double* out) {
   __shared__ double tmp[1024];
@@ -1134,9 +1141,9 @@ __global__ void __launch_bounds__(256) k_sumpow_ordered(const DBox* __restrict__
       bool first = true;
       for (int k = bx.lo[2]; k <= bx.hi[2]; k++)
         for (int j = bx.lo[1]; j <= bx.hi[1]; j++) {
-          const double r = (double)x[cidx(bx, bx.lo[0] + i, j, k)];
-          if (first) { t = r * r; first = false; }
-          else t += r * r;
+          const double r = sumpow_term<ABS>(x[cidx(bx, bx.lo[0] + i, j, k)]);
+          if (first) { t = r; first = false; }
+          else t += r;
         }
       tmp[i] = t;
     }
@@ -1154,7 +1161,7 @@ __global__ void __launch_bounds__(256) k_sumpow_ordered(const DBox* __restrict__
 // nbox x (widest box) chains fit in LDS: each chain runs in its thread in the
 // order above, then one thread adds the column sums box by box in the same order.
 constexpr int kSumpowChains = 4096;
-template <class T>
+template <class T, bool ABS = false>
 __global__ void __launch_bounds__(256) k_sumpow_chains(const DBox* __restrict__ boxes, int nbox, int W, const T* x,
                                                        double* out) {
   __shared__ double tmp[kSumpowChains];
@@ -1166,9 +1173,9 @@ __global__ void __launch_bounds__(256) k_sumpow_chains(const DBox* __restrict__ 
     bool first = true;
     for (int k = bx.lo[2]; k <= bx.hi[2]; k++)
       for (int j = bx.lo[1]; j <= bx.hi[1]; j++) {
-        const double r = (double)x[cidx(bx, bx.lo[0] + i, j, k)];
-        if (first) { t = r * r; first = false; }
-        else t += r * r;
+        const double r = sumpow_term<ABS>(x[cidx(bx, bx.lo[0] + i, j, k)]);
+        if (first) { t = r; first = false; }
+        else t += r;
       }
     tmp[p] = t;
   }
@@ -1208,19 +1215,19 @@ double dot_ordered(const DeviceLayout& L, const T* a, const T* b, int slot) {
   L.ctx->sync();
   return L.ctx->h_scalars[slot];
 }
-template <class T>
-double norm2_ordered(const DeviceLayout& L, const T* x, int slot) {
+template <class T, bool ABS>
+static double sum_ordered(const DeviceLayout& L, const T* x, int slot) {
   int W = 0;
   for (int q = 0; q < L.nbox; q++) {
     W = std::max(W, L.h_boxes[q].hi[0] - L.h_boxes[q].lo[0] + 1);
-    if (L.h_boxes[q].hi[0] - L.h_boxes[q].lo[0] + 1 > 1024) throw std::runtime_error("norm2_ordered: box wider than 1024");
+    if (L.h_boxes[q].hi[0] - L.h_boxes[q].lo[0] + 1 > 1024) throw std::runtime_error("norm: box wider than 1024");
   }
   if ((long long)L.nbox * W <= kSumpowChains) {
-    hipLaunchKernelGGL(k_sumpow_chains<T>, dim3(1), dim3(256), 0, L.ctx->stream, L.d_boxes, L.nbox, W, x,
+    hipLaunchKernelGGL((k_sumpow_chains<T, ABS>), dim3(1), dim3(256), 0, L.ctx->stream, L.d_boxes, L.nbox, W, x,
                        L.ctx->d_scalars + slot);
     hip_check(hipGetLastError(), "k_sumpow_chains");
   } else {
-    hipLaunchKernelGGL(k_sumpow_ordered<T>, dim3(1), dim3(256), 0, L.ctx->stream, L.d_boxes, L.nbox, x,
+    hipLaunchKernelGGL((k_sumpow_ordered<T, ABS>), dim3(1), dim3(256), 0, L.ctx->stream, L.d_boxes, L.nbox, x,
                        L.ctx->d_scalars + slot);
     hip_check(hipGetLastError(), "k_sumpow_ordered");
   }
@@ -1228,7 +1235,15 @@ double norm2_ordered(const DeviceLayout& L, const T* x, int slot) {
                            L.ctx->stream),
             "copy scalar");
   L.ctx->sync();
-  return std::sqrt(L.ctx->h_scalars[slot]);
+  return L.ctx->h_scalars[slot];
+}
+template <class T>
+double norm2_ordered(const DeviceLayout& L, const T* x, int slot) {
+  return std::sqrt(sum_ordered<T, false>(L, x, slot));
+}
+template <class T>
+double norm1_ordered(const DeviceLayout& L, const T* x, int slot) {
+  return sum_ordered<T, true>(L, x, slot);
 }
 
 // ---------------------------------------------------------------------------
@@ -1455,6 +1470,7 @@ void reflux(Context* ctx, const RefluxCell* cells, int ncells, const RefluxCoars
   template void maxnorm_async<T>(const DeviceLayout&, const T*, int, int);                                      \
   template double dot_ordered<T>(const DeviceLayout&, const T*, const T*, int);                                \
   template double norm2_ordered<T>(const DeviceLayout&, const T*, int);                                        \
+  template double norm1_ordered<T>(const DeviceLayout&, const T*, int);                                        \
   template void cf_inhomog<T>(Context*, const CFCoarse*, int, const CFGen*, const CFFine*, int, T*, T*, const T*, \
                               double, double);                                                                \
   template void reflux<T>(Context*, const RefluxCell*, int, const RefluxCoarse*, const RefluxFine*, T*, const T*, \

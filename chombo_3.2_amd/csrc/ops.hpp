@@ -120,6 +120,9 @@ template <class T>
 double dot_ordered(const DeviceLayout& L, const T* a, const T* b, int slot);
 template <class T>
 double norm2_ordered(const DeviceLayout& L, const T* x, int slot);
+// norm(.., 1): the same per-box column sums of |x|, no root
+template <class T>
+double norm1_ordered(const DeviceLayout& L, const T* x, int slot);
 // asynchronous variant: max-norm into d_scalars[slot] without a sync (value read later)
 template <class T>
 void maxnorm_async(const DeviceLayout& L, const T* x, int slot, int accumulate);

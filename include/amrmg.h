@@ -109,6 +109,10 @@ int amrmg_solver_operator(amrmg_solver* s, int homogeneous);
 /* AMRPoissonOp::applyOp / residual / relax / restrictResidual / prolongIncrement
    on multigrid depth `depth` of AMR level 0 (depth 0 = level 0) */
 int amrmg_level_apply(amrmg_solver* s, int depth, const double* phi, double* out, int homogeneous);
+/* norm(BoxLayoutData, interval, p) (BoxLayoutData.cpp:23-92) through AMRPoissonOp::norm
+   (AMRPoissonOp.cpp:558-562) on MG depth `depth` of level 0: p = 0 max |x|, 1 sum |x|
+   (FArrayBox::norm(1) column sums), 2 sqrt(sumPow(2)); x valid-packed */
+int amrmg_level_norm(amrmg_solver* s, int depth, const double* x, int p, double* out);
 int amrmg_level_residual(amrmg_solver* s, int depth, const double* phi, const double* rhs, double* out,
                          int homogeneous);
 int amrmg_level_relax(amrmg_solver* s, int depth, double* phi, const double* rhs, int iterations);

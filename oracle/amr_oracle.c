@@ -1428,11 +1428,32 @@ static double fab_sumpow2(const Fab* f, const Box* r) {
   free(tmp);
   return sum;
 }
-/* norm(BoxLayoutData, interval, p) (BoxLayoutData.cpp:23-92), p = 0 or 2 */
+/* FArrayBox::norm(box, 1) (FArrayBox.cpp:110-141): per-column sums of |x| over rows,
+   then over columns */
+static double fab_sumabs(const Fab* f, const Box* r) {
+  int nx = b_len(r, 0);
+  double* tmp = malloc(sizeof(double)*nx);
+  int first = 1;
+  for (int k = r->lo[2]; k <= r->hi[2]; k++)
+    for (int j = r->lo[1]; j <= r->hi[1]; j++) {
+      const double* row = P(f, r->lo[0], j, k);
+      if (first) { for (int i = 0; i < nx; i++) tmp[i] = fabs(row[i]); first = 0; }
+      else for (int i = 0; i < nx; i++) tmp[i] += fabs(row[i]);
+    }
+  double sum = tmp[0];
+  for (int i = 1; i < nx; i++) sum += tmp[i];
+  free(tmp);
+  return sum;
+}
+/* norm(BoxLayoutData, interval, p) (BoxLayoutData.cpp:23-92), p = 0, 1 or 2 */
 static double ldo_norm(const LD* x, int p) {
   double norm = 0;
   if (p == 0) {
     for (int b = 0; b < x->L->n; b++) { double m = fab_maxnorm(&x->f[b], &x->L->b[b]); norm = norm > m ? norm : m; }
+    return norm;
+  }
+  if (p == 1) {
+    for (int b = 0; b < x->L->n; b++) norm += fab_sumabs(&x->f[b], &x->L->b[b]);
     return norm;
   }
   for (int b = 0; b < x->L->n; b++) norm += fab_sumpow2(&x->f[b], &x->L->b[b]);
@@ -1720,6 +1741,17 @@ int oracle_vc_lambda(OHier* h, int depth, double* out) {
   return 0;
 }
 long oracle_level_cells(const OHier* h, int lev) { return lay_cells(&h->grids[lev]); }
+/* norm(BoxLayoutData, interval, p) on MG depth d of level 0 (p = 0, 1, 2) */
+int oracle_level_norm(OHier* h, int d, const double* x_in, int p, double* out) {
+  if (d < 0 || d >= h->mgD) { set_err("bad depth"); return -1; }
+  if (p < 0 || p > 2) { set_err("norm order must be 0, 1 or 2"); return -1; }
+  const Op* op = mg_op(h, d);
+  LD x = ld_make(&op->L, 0);
+  ld_from_packed(&x, x_in);
+  *out = ldo_norm(&x, p);
+  ld_free(&x);
+  return 0;
+}
 int oracle_num_mg_depths(const OHier* h) { return h->mgD; }
 long oracle_mg_cells(const OHier* h, int d) { return lay_cells(&h->mgL[d]); }
 long oracle_ghosted_cells(const OHier* h, int lev) {

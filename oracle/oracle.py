@@ -60,6 +60,7 @@ def lib():
         L.oracle_face_cells.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.oracle_vc_lambda.argtypes = [C.c_void_p, C.c_int, _dp]
         L.oracle_set_alpha_beta.argtypes = [C.c_void_p, C.c_double, C.c_double]
+        L.oracle_level_norm.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _dp]
         _lib = L
     return _lib
 
@@ -203,6 +204,12 @@ class Oracle:
 
     def face_cells(self, lev, d):
         return lib().oracle_face_cells(self.h, lev, d)
+
+    def level_norm(self, depth, x, p):
+        x, px = _d(x)
+        out = np.zeros(1)
+        self._chk(lib().oracle_level_norm(self.h, depth, px, int(p), out.ctypes.data_as(_dp)))
+        return float(out[0])
 
     def set_alpha_beta(self, alpha, beta):
         """setAlphaAndBeta on every operator (AMRMultiGrid::getAllOperators)."""

@@ -61,6 +61,19 @@ def test_gsrb_kernel_bitwise(gpu_ctx, case):
         np.testing.assert_array_equal(g.kernel_gsrb(0, phi_g, rhs, color), o.kernel_gsrb(0, phi_g, rhs, color))
 
 
+@pytest.mark.parametrize("case", ["lap32_negidx", "ragged40"])
+def test_level_norms_bitwise(gpu_ctx, case):
+    """norm(BoxLayoutData, p), p = 0, 1, 2, at every multigrid depth: the ordered
+    per-box column sums give the oracle's value bit for bit."""
+    h = small_single(**SINGLE_CASES[case])
+    g, o = _pair(gpu_ctx, h)
+    rng = np.random.default_rng(13)
+    for d in range(o.mg_depths):
+        x = rng.standard_normal(o.mg_cells(d))
+        for p in (0, 1, 2):
+            assert g.level_norm(d, x, p) == o.level_norm(d, x, p), (d, p)
+
+
 def test_bottom_solve_bitwise(gpu_ctx):
     h = small_single(n=64, box=32)
     g, o = _pair(gpu_ctx, h)

@@ -361,3 +361,16 @@ def test_bottom_bicgstab_solves_bottom_system():
     assert status == 1
     r = o.level_residual(d, corr, res, homog=True)
     assert np.abs(r).max() < 1e-9 * np.abs(res).max()
+
+
+def test_level_norms_match_numpy():
+    """norm(BoxLayoutData, p) for p = 0, 1, 2 equals numpy's max |x|, sum |x| and
+    sqrt(sum x^2) to rounding, on every multigrid depth (box-ordered sums)."""
+    h = small_single(n=32, box=16, lo=-8)
+    o = Oracle(h)
+    rng = np.random.default_rng(12)
+    for d in range(o.mg_depths):
+        x = rng.standard_normal(o.mg_cells(d))
+        assert o.level_norm(d, x, 0) == np.abs(x).max()
+        assert abs(o.level_norm(d, x, 1) - np.abs(x).sum()) < 1e-12 * np.abs(x).sum()
+        assert abs(o.level_norm(d, x, 2) - np.sqrt((x * x).sum())) < 1e-12 * np.sqrt((x * x).sum())
