@@ -244,9 +244,9 @@ class MultiGrid {
   void oneCycle(T& e, const T& res) { cycle(0, e, res); }
   // oneCycle for a caller whose e is zero on entry (the AMR V-cycle's coarsest level)
   void oneCycleFromZero(T& e, const T& res) { cycle(0, e, res, true); }
-  // MultiGrid::cycle (MultiGrid.H:630-696), V/W cycles.  zero: correction is zero
-  // on entry (as the reference's setToZero before each first coarser cycle makes
-  // it), so its first relaxation need not read it.
+  // MultiGrid::cycle (MultiGrid.H:630-696): V/W cycles (m_cycle > 0) and F cycles
+  // (m_cycle < 0).  zero: correction is zero on entry (as the reference's setToZero
+  // before each coarser cycle makes it), so its first relaxation need not read it.
   void cycle(int depth, T& correction, const T& residual, bool zero = false) {
     auto relax = [&](int n) {
       if (zero) m_op[depth]->relaxFromZero(correction, residual, n);
@@ -263,6 +263,28 @@ class MultiGrid {
       return;
     }
     int cycles = m_cycle;
+    if (cycles < 0) {
+      // F cycle: coarse correction first, then -m_cycle V-cycles (the reference sets
+      // m_cycle = 1 around them and restores it)
+      cycles = -cycles;
+      if (zero) {  // restrictResidual reads the correction
+        m_op[depth]->setToZero(correction);
+        zero = false;
+      }
+      m_op[depth]->restrictResidual(*m_residual[depth + 1], correction, residual);
+      cycle(depth + 1, *m_correction[depth + 1], *m_residual[depth + 1], true);
+      m_op[depth]->prolongIncrement(correction, *m_correction[depth + 1]);
+      m_op[depth]->relax(correction, residual, m_pre);
+      for (int img = 0; img < cycles; img++) {
+        m_op[depth]->restrictResidual(*m_residual[depth + 1], correction, residual);
+        m_cycle = 1;
+        cycle(depth + 1, *m_correction[depth + 1], *m_residual[depth + 1], true);
+        m_cycle = -cycles;
+        m_op[depth]->prolongIncrement(correction, *m_correction[depth + 1]);
+      }
+      m_op[depth]->relax(correction, residual, m_post);
+      return;
+    }
     relax(m_pre);
     m_op[depth]->restrictResidual(*m_residual[depth + 1], correction, residual);
     // setToZero(m_correction[depth + 1]) is folded into the first coarser cycle's

@@ -1856,6 +1856,24 @@ static void mg_cycle(MGState* s, int depth, LD* corr, const LD* res) {
     }
     return;
   }
+  if (s->cycles < 0) {  /* F cycle (MultiGrid.H:655-681) */
+    int cycles = -s->cycles;
+    op_restrict(op, &s->mgres[depth+1], corr, res);
+    ldo_setzero(&s->mgcor[depth+1]);
+    mg_cycle(s, depth+1, &s->mgcor[depth+1], &s->mgres[depth+1]);
+    op_prolong(op, corr, &s->mgcor[depth+1]);
+    op_relax(op, corr, res, s->pre);
+    for (int img = 0; img < cycles; img++) {
+      op_restrict(op, &s->mgres[depth+1], corr, res);
+      ldo_setzero(&s->mgcor[depth+1]);
+      s->cycles = 1;                 /* "hack to get a V-cycle" */
+      mg_cycle(s, depth+1, &s->mgcor[depth+1], &s->mgres[depth+1]);
+      s->cycles = -cycles;
+      op_prolong(op, corr, &s->mgcor[depth+1]);
+    }
+    op_relax(op, corr, res, s->post);
+    return;
+  }
   op_relax(op, corr, res, s->pre);
   op_restrict(op, &s->mgres[depth+1], corr, res);
   ldo_setzero(&s->mgcor[depth+1]);

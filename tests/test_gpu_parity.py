@@ -195,6 +195,19 @@ def test_solve_matches_oracle(gpu_ctx, case):
     np.testing.assert_array_equal(phi_g, phi_o)
 
 
+@pytest.mark.parametrize("numMG,case", [(-1, "lap32"), (-2, "helm24"), (-1, "ragged40")])
+def test_fcycle_solve_bitwise(gpu_ctx, numMG, case):
+    """F cycles (MultiGrid::cycle with m_cycle = numMG < 0) on one level, device
+    bottom solver included: history, iterations and solution bitwise the oracle's."""
+    h = small_single(**SINGLE_CASES[case])
+    rhs = H.sinusoid_rhs(h, 1021, k=(1, 2, 1))
+    phi_g, hist_g, info_g, phi_o, hist_o, info_o = _solve_pair(gpu_ctx, h, rhs, numMG=numMG)
+    assert info_g["exit_status"] & 1
+    assert info_g["iterations"] == info_o["iterations"] and info_g["exit_status"] == info_o["exit_status"]
+    np.testing.assert_array_equal(hist_g, hist_o)
+    np.testing.assert_array_equal(phi_g, phi_o)
+
+
 def test_solve_linearity_and_determinism_full_c1(gpu_ctx):
     """Size-independent properties at the 64^3 / 32^3-box workload: solving with
     2*rhs gives exactly 2*phi (scaling by 2 is exact), and repeated solves are

@@ -374,3 +374,21 @@ def test_level_norms_match_numpy():
         assert o.level_norm(d, x, 0) == np.abs(x).max()
         assert abs(o.level_norm(d, x, 1) - np.abs(x).sum()) < 1e-12 * np.abs(x).sum()
         assert abs(o.level_norm(d, x, 2) - np.sqrt((x * x).sum())) < 1e-12 * np.sqrt((x * x).sum())
+
+
+@pytest.mark.parametrize("numMG", [-1, -2])
+def test_fcycle_solve_matches_sparse_direct_solve(numMG):
+    """F cycles (MultiGrid::cycle with m_cycle < 0) on one level: the solve converges
+    to the sparse direct solution, in no more outer iterations than V-cycles."""
+    n = 16
+    h = small_single(n=n, box=8)
+    o = Oracle(h)
+    rhs = H.sinusoid_rhs(h, 1001)
+    sp_, rp = H.solver_params(eps=1e-12, numMG=numMG)
+    phi, hist, info = o.solve(rhs, sp_, rp)
+    assert info["exit_status"] & 1
+    x = spla.spsolve(_assemble_dirichlet(n).tocsc(), h.to_global(rhs, 0).ravel(order="F")).reshape((n, n, n),
+                                                                                                  order="F")
+    assert np.abs(h.to_global(phi, 0) - x).max() / np.abs(x).max() < 1e-10
+    _, hist_v, info_v = o.solve(rhs, *H.solver_params(eps=1e-12))
+    assert info["iterations"] <= info_v["iterations"]
