@@ -44,6 +44,7 @@ def lib():
         L.amrmg_solver_set_alpha_beta.argtypes = [C.c_void_p, C.c_double, C.c_double]
         L.amrmg_solver_set_vc_coefs.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
         L.amrmg_level_norm.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _dp]
+        L.amrmg_solver_set_relax_mode.argtypes = [C.c_void_p, C.c_int]
         L.amrmg_heat_create.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]
         L.amrmg_heat_set_source.argtypes = [C.c_void_p, _dp]
         L.amrmg_heat_step.argtypes = [C.c_void_p, C.c_double, C.c_int, _ip]
@@ -239,6 +240,10 @@ class AMRMultiGrid:
     def set_alpha_beta(self, alpha, beta):
         """AMRPoissonOp::setAlphaAndBeta on every operator of the solver."""
         _chk(lib().amrmg_solver_set_alpha_beta(self.p, float(alpha), float(beta)))
+
+    def set_relax_mode(self, mode):
+        """AMRPoissonOp::s_relaxMode for every operator: 1 levelGSRB, 4 levelJacobi."""
+        _chk(lib().amrmg_solver_set_relax_mode(self.p, int(mode)))
 
     def set_vc_relax_shape(self, shape):
         """Tile shape of the one-launch variable-coefficient relaxation (-1 automatic, 0..3)."""

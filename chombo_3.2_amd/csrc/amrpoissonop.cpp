@@ -258,7 +258,7 @@ bool AMRPoissonOp<Real>::tuneRelax(const T& phi, const T& rhs, double* times) {
 // dropped; ghost cells are refilled before any later read, as after any relax.
 template <class Real>
 void AMRPoissonOp<Real>::relaxFromZero(T& e, const T& r, int n) {
-  if (!(m_fused && m_ctx->fused_relax) || m_vc || n <= 0) {
+  if (!(m_fused && m_ctx->fused_relax) || m_vc || m_relaxMode != 1 || n <= 0) {
     setToZero(e);
     relax(e, r, n);
     return;
@@ -272,7 +272,25 @@ void AMRPoissonOp<Real>::relaxFromZero(T& e, const T& r, int n) {
 }
 template <class Real>
 void AMRPoissonOp<Real>::relax(T& e, const T& r, int n) {
-  for (int i = 0; i < n; i++) levelGSRB(e, r);
+  for (int i = 0; i < n; i++) {
+    if (m_relaxMode == 4) levelJacobi(e, r);
+    else levelGSRB(e, r);
+  }
+}
+// levelJacobi (AMRPoissonOp.cpp:1579-1596; VCAMRPoissonOp2.cpp:762-786): the homogeneous
+// residual, times 0.666 / (alpha - 2 D beta / dx^2), or times lambda and 0.5 with
+// variable coefficients, added to phi
+template <class Real>
+void AMRPoissonOp<Real>::levelJacobi(T& phi, const T& rhs) {
+  if (!m_jacobiTmp.isDefined()) m_jacobiTmp.define(m_lay);
+  residual(m_jacobiTmp, phi, rhs, true);
+  if (m_vc) {
+    vc::scale_by_lambda<Real>(*m_lay, *m_vc, m_jacobiTmp.d);
+    incr(phi, m_jacobiTmp, 0.5);
+  } else {
+    const double weight = m_alpha - 2.0 * 3 * m_beta / (m_dx * m_dx);
+    incr(phi, m_jacobiTmp, 0.666 / weight);
+  }
 }
 // restrictResidual (:651-703)
 template <class Real>

@@ -1493,7 +1493,8 @@ template <class T>
 void DeviceBottomSolver<T>::define(LinearOp<LevelData<T>>* op, bool homogeneous) {
   m_fallback.define(op, homogeneous);
   auto* ap = dynamic_cast<AMRPoissonOp<T>*>(op);
-  const bool vc = ap && ap->m_vc;  // variable coefficients: the host-driven solve
+  // variable coefficients or Jacobi relaxation: the host-driven solve
+  const bool vc = ap && (ap->m_vc || ap->m_relaxMode != 1);
   if (m_device && op == m_boundOp && homogeneous && !vc) return;  // MultiGrid re-binds its bottom at every solve
   m_device = false;
   m_boundOp = op;
@@ -1563,8 +1564,9 @@ void DeviceBottomSolver<T>::setConvergenceMetrics(double metric, double tol) {
 }
 template <class T>
 void DeviceBottomSolver<T>::solve(LevelData<T>& phi, const LevelData<T>& rhs) {
-  if (m_device && static_cast<const AMRPoissonOp<T>*>(m_ap)->m_vc)
-    throw std::runtime_error("DeviceBottomSolver: variable coefficients set after define");
+  if (m_device && (static_cast<const AMRPoissonOp<T>*>(m_ap)->m_vc ||
+                   static_cast<const AMRPoissonOp<T>*>(m_ap)->m_relaxMode != 1))
+    throw std::runtime_error("DeviceBottomSolver: operator changed after define");
   if (!m_device) {
     m_fallback.solve(phi, rhs);
     return;

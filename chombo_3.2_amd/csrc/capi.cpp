@@ -600,6 +600,19 @@ int amrmg_solver_set_relax_shape(amrmg_solver* s, int shape, int kc) {
   }
   API_END
 }
+int amrmg_solver_set_relax_mode(amrmg_solver* s, int mode) {
+  API_BEGIN
+  if (mode != 1 && mode != 4) throw std::runtime_error("relax mode must be 1 (levelGSRB) or 4 (levelJacobi)");
+  for (auto& op : s->amg.m_op) op->m_relaxMode = mode;
+  for (int d = 1; d < s->H.mgDepth(); d++) s->mgop(d)->m_relaxMode = mode;
+  if (s->mixed) {
+    for (auto& op : s->mixed->amg.m_op) op->m_relaxMode = mode;
+    for (int d = 1; d < s->H.mgDepth(); d++) s->mixedop(d)->m_relaxMode = mode;
+  }
+  s->amg.m_mg0->setBottomSolver(s->bottom.get());
+  if (s->mixed) s->mixed->rebindBottom();
+  API_END
+}
 int amrmg_solver_set_vc_relax_shape(amrmg_solver* s, int shape) {
   API_BEGIN
   if (shape < -1 || shape > 3) throw std::runtime_error("variable-coefficient relax shape out of range");

@@ -723,6 +723,11 @@ void scale_by_a(const DeviceLayout& L, const VCCoefs<T>& C, T* x, bool divide) {
   need_coefs(L, C, "vc::scale_by_a");
   VC_LAUNCH(L, KC_BLAS, 0.0, 0.0, k_vc_scale_a<T>, L.ntiles, L.d_boxes, L.d_tiles, geom(L), x, C.a.d, divide ? 1 : 0);
 }
+template <class T>
+void scale_by_lambda(const DeviceLayout& L, const VCCoefs<T>& C, T* x) {
+  need_coefs(L, C, "vc::scale_by_lambda");
+  VC_LAUNCH(L, KC_BLAS, 0.0, 0.0, k_vc_scale_a<T>, L.ntiles, L.d_boxes, L.d_tiles, geom(L), x, C.lambda.d, 0);
+}
 
 template <class T>
 struct VCB3 {
@@ -799,6 +804,7 @@ void narrow(const VCCoefs<double>& src, VCCoefs<float>& dst) {
   template void precond_init<T>(const DeviceLayout&, const VCCoefs<T>&, T*, const T*);                              \
   template void average_coefs<T>(const VCCoefs<T>&, VCCoefs<T>&, int);                                             \
   template void scale_by_a<T>(const DeviceLayout&, const VCCoefs<T>&, T*, bool);                                    \
+  template void scale_by_lambda<T>(const DeviceLayout&, const VCCoefs<T>&, T*);                                     \
   template void reflux<T>(Context*, const RefluxCell*, int, const RefluxCoarse*, const RefluxFine*, const VCFaceIdx*, \
                           const VCFaceIdx*, T*, const T*, const T*, const VCCoefs<T>&, const VCCoefs<T>&, double,     \
                           double, double);

@@ -91,6 +91,9 @@ void reflux(Context* ctx, const RefluxCell* cells, int ncells, const RefluxCoars
 // resp. x /= aCoef on valid cells
 template <class T>
 void scale_by_a(const DeviceLayout& L, const VCCoefs<T>& C, T* x, bool divide);
+// x *= lambda on valid cells (VCAMRPoissonOp2::levelJacobi)
+template <class T>
+void scale_by_lambda(const DeviceLayout& L, const VCCoefs<T>& C, T* x);
 // FP32 copy of FP64 coefficients on the same level (a and bCoef; lambda is recomputed)
 void narrow(const VCCoefs<double>& src, VCCoefs<float>& dst);
 }  // namespace vc

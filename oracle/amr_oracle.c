@@ -1060,6 +1060,7 @@ typedef struct Op {
   int vc;
   LD va, vlam;        /* aCoef and the relaxation coefficient lambda, valid cells */
   Fab* vb[3];         /* bCoef: per box, the faces normal to d (box grown by one face in d) */
+  int relaxMode;      /* AMRPoissonOp::s_relaxMode: 1 levelGSRB, 4 levelJacobi */
 } Op;
 
 static void op_define(Op* op, const Layout* L, const Box* domain, double dx, double alpha,
@@ -1068,7 +1069,7 @@ static void op_define(Op* op, const Layout* L, const Box* domain, double dx, dou
   op->L.n = L->n; op->L.b = malloc(sizeof(Box)*(L->n ? L->n : 1));
   memcpy(op->L.b, L->b, sizeof(Box)*L->n);
   op->domain = *domain; op->dx = dx; op->dxCrse = 2*dx; op->alpha = alpha; op->beta = beta;
-  op->bc = *bc; op->refToCoarser = 2; op->refToFiner = 2;
+  op->bc = *bc; op->refToCoarser = 2; op->refToFiner = 2; op->relaxMode = 1;
   op->ex = plan_exchange(&op->L);
   op->cf = calloc((size_t)L->n*6, sizeof(CFIVS));
   for (int b = 0; b < L->n; b++)
@@ -1324,8 +1325,34 @@ static void op_levelGSRB(const Op* op, LD* phi, const LD* rhs) {
     }
   }
 }
+static void ldo_incr(LD* lhs, const LD* x, double scale);
+/* levelJacobi (AMRPoissonOp.cpp:1579-1596; VCAMRPoissonOp2.cpp:762-786): the
+   homogeneous residual, weighted, added to phi */
+static void op_levelJacobi(const Op* op, LD* phi, const LD* rhs) {
+  g_cell_relax += lay_cells(&op->L);
+  LD resid = ld_make(&op->L, 0);
+  op_residual(op, &resid, phi, rhs, 1);
+  if (op->vc) {
+    for (int b = 0; b < op->L.n; b++) {
+      const Box* v = &op->L.b[b];
+      for (int k = v->lo[2]; k <= v->hi[2]; k++)
+        for (int j = v->lo[1]; j <= v->hi[1]; j++)
+          for (int i = v->lo[0]; i <= v->hi[0]; i++)
+            *P(&resid.f[b],i,j,k) = *P(&resid.f[b],i,j,k) * *P(&op->vlam.f[b],i,j,k);
+    }
+    ldo_incr(phi, &resid, 0.5);
+  } else {
+    double weight = op->alpha - 2.0*3 * op->beta / (op->dx*op->dx);
+    ldo_incr(phi, &resid, 0.666/weight);
+  }
+  ld_free(&resid);
+}
+/* relax (AMRPoissonOp.cpp:600-632): s_relaxMode 1 (levelGSRB) or 4 (levelJacobi) */
 static void op_relax(const Op* op, LD* e, const LD* r, int n) {
-  for (int i = 0; i < n; i++) op_levelGSRB(op, e, r);
+  for (int i = 0; i < n; i++) {
+    if (op->relaxMode == 4) op_levelJacobi(op, e, r);
+    else op_levelGSRB(op, e, r);
+  }
 }
 /* restrictResidual (AMRPoissonOp.cpp:651-703) */
 static void op_restrict(const Op* op, LD* resC, LD* phi, const LD* rhs) {
@@ -1712,6 +1739,13 @@ int oracle_vc_set(OHier* h, const double* a, const double* b0, const double* b1,
     }
     op_vc_lambda(op);
   }
+  return 0;
+}
+/* AMRPoissonOp::s_relaxMode (AMRPoissonOp.cpp:31, 600-632) for every operator: 1 or 4 */
+int oracle_set_relax_mode(OHier* h, int mode) {
+  if (mode != 1 && mode != 4) { set_err("relax mode must be 1 (levelGSRB) or 4 (levelJacobi)"); return -1; }
+  for (int l = 0; l < h->nlev; l++) h->op[l].relaxMode = mode;
+  for (int d = 1; d < h->mgD; d++) h->mgop[d].relaxMode = mode;
   return 0;
 }
 /* setAlphaAndBeta (AMRPoissonOp.cpp:1103-1108, VCAMRPoissonOp2.cpp:258-266) on every

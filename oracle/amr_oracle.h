@@ -94,6 +94,8 @@ int  oracle_vc_set(OHier* h, const double* a, const double* b0, const double* b1
 long oracle_face_cells(const OHier* h, int lev, int dir);
 /* norm(BoxLayoutData, interval, p) (BoxLayoutData.cpp:23-92) on MG depth d, p = 0, 1, 2 */
 int  oracle_level_norm(OHier* h, int d, const double* x, int p, double* out);
+/* AMRPoissonOp::s_relaxMode for every operator: 1 levelGSRB (default), 4 levelJacobi */
+int  oracle_set_relax_mode(OHier* h, int mode);
 /* setAlphaAndBeta on every operator (AMR levels and MG depths), lambda recomputed */
 int  oracle_set_alpha_beta(OHier* h, double alpha, double beta);
 int  oracle_vc_lambda(OHier* h, int depth, double* lambda_out);

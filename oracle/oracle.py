@@ -61,6 +61,7 @@ def lib():
         L.oracle_vc_lambda.argtypes = [C.c_void_p, C.c_int, _dp]
         L.oracle_set_alpha_beta.argtypes = [C.c_void_p, C.c_double, C.c_double]
         L.oracle_level_norm.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _dp]
+        L.oracle_set_relax_mode.argtypes = [C.c_void_p, C.c_int]
         _lib = L
     return _lib
 
@@ -210,6 +211,10 @@ class Oracle:
         out = np.zeros(1)
         self._chk(lib().oracle_level_norm(self.h, depth, px, int(p), out.ctypes.data_as(_dp)))
         return float(out[0])
+
+    def set_relax_mode(self, mode):
+        """AMRPoissonOp::s_relaxMode: 1 levelGSRB, 4 levelJacobi."""
+        self._chk(lib().oracle_set_relax_mode(self.h, int(mode)))
 
     def set_alpha_beta(self, alpha, beta):
         """setAlphaAndBeta on every operator (AMRMultiGrid::getAllOperators)."""

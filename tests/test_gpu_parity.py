@@ -208,6 +208,37 @@ def test_fcycle_solve_bitwise(gpu_ctx, numMG, case):
     np.testing.assert_array_equal(phi_g, phi_o)
 
 
+@pytest.mark.parametrize("case", ["helm24", "amr32"])
+def test_jacobi_relax_mode_bitwise(gpu_ctx, case):
+    """s_relaxMode 4 (levelJacobi) everywhere: level relaxation at every depth and
+    whole solves (host-driven bottom) bitwise the oracle's."""
+    if case == "amr32":
+        h = small_amr(alpha=0.3, beta=0.9)
+        rhs = H.bump_rhs(h, 1031)
+    else:
+        h = small_single(**SINGLE_CASES[case])
+        rhs = H.sinusoid_rhs(h, 1031)
+    g, o = _pair(gpu_ctx, h)
+    g.set_relax_mode(4)
+    o.set_relax_mode(4)
+    rng = np.random.default_rng(15)
+    for d in range(o.mg_depths):
+        n = o.mg_cells(d)
+        phi, r = rng.standard_normal(n), rng.standard_normal(n)
+        np.testing.assert_array_equal(g.level_relax(d, phi, r, 2), o.level_relax(d, phi, r, 2))
+    sp, rp = H.solver_params(pre=4, post=4)
+    g.setSolverParameters(sp, rp)
+    g.set_field(g.RHS, rhs)
+    hist_g, info_g = g.solve()
+    phi_o, hist_o, info_o = o.solve(rhs, sp, rp)
+    assert info_g["exit_status"] & 1
+    assert info_g["iterations"] == info_o["iterations"]
+    np.testing.assert_array_equal(hist_g, hist_o)
+    np.testing.assert_array_equal(g.get_field(g.PHI), phi_o)
+    g.set_relax_mode(1)
+    g.close()
+
+
 def test_solve_linearity_and_determinism_full_c1(gpu_ctx):
     """Size-independent properties at the 64^3 / 32^3-box workload: solving with
     2*rhs gives exactly 2*phi (scaling by 2 is exact), and repeated solves are

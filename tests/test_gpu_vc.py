@@ -360,6 +360,39 @@ def test_vc_amr_mixed_precision_solve(gpu_ctx):
     g.close()
 
 
+@pytest.mark.parametrize("hier", ["single", "amr32"])
+def test_vc_jacobi_relax_mode_bitwise(gpu_ctx, hier):
+    print(f"vc jacobi {hier}", flush=True)
+    if hier == "single":
+        h = small_single(**CASES["vc32"])
+        rhs = H.sinusoid_rhs(h, 1032)
+    else:
+        h = _amr("amr32")
+        rhs = H.bump_rhs(h, 1032)
+    a, b = H.vc_coefficients(h, seed=51)
+    g = amrmg.AMRMultiGrid(gpu_ctx, h)
+    g.set_vc_coefs(a, b)
+    g.set_relax_mode(4)
+    o = Oracle(h)
+    o.vc_set(a, b)
+    o.set_relax_mode(4)
+    rng = np.random.default_rng(16)
+    for d in range(o.mg_depths):
+        n = o.mg_cells(d)
+        phi, r = rng.standard_normal(n), rng.standard_normal(n)
+        np.testing.assert_array_equal(g.level_relax(d, phi, r, 2), o.level_relax(d, phi, r, 2))
+    sp, rp = H.solver_params(pre=4, post=4)
+    g.setSolverParameters(sp, rp)
+    g.set_field(g.RHS, rhs)
+    hist_g, info_g = g.solve()
+    phi_o, hist_o, info_o = o.solve(rhs, sp, rp)
+    assert info_g["exit_status"] & 1
+    assert info_g["iterations"] == info_o["iterations"]
+    np.testing.assert_array_equal(hist_g, hist_o)
+    np.testing.assert_array_equal(g.get_field(g.PHI), phi_o)
+    g.close()
+
+
 def test_vc_rejects_bad_input(gpu_ctx):
     print("vc errors", flush=True)
     h = small_single(n=32, box=16)

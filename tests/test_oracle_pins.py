@@ -392,3 +392,39 @@ def test_fcycle_solve_matches_sparse_direct_solve(numMG):
     assert np.abs(h.to_global(phi, 0) - x).max() / np.abs(x).max() < 1e-10
     _, hist_v, info_v = o.solve(rhs, *H.solver_params(eps=1e-12))
     assert info["iterations"] <= info_v["iterations"]
+
+
+@pytest.mark.parametrize("alpha,beta", [(0.0, 1.0), (0.7, 1.3)])
+def test_jacobi_sweep_on_sine_mode(alpha, beta):
+    """levelJacobi (s_relaxMode 4): one sweep on a Dirichlet sine mode e with rhs 0
+    gives e (1 - 0.666 (alpha + beta lam) / (alpha - 6 beta / dx^2)), lam the
+    discrete eigenvalue of the Laplacian."""
+    n, k = 16, (1, 2, 3)
+    h = small_single(n=n, box=8, alpha=alpha, beta=beta)
+    o = Oracle(h)
+    o.set_relax_mode(4)
+    dx = 1.0 / n
+    e = h.pack(lambda x, y, z, l: np.sin(np.pi * k[0] * x) * np.sin(np.pi * k[1] * y) * np.sin(np.pi * k[2] * z))
+    lam = sum((2 / dx ** 2) * (np.cos(np.pi * kk * dx) - 1) for kk in k)
+    out = o.level_relax(0, e, np.zeros_like(e), 1)
+    factor = 1 - 0.666 * (alpha + beta * lam) / (alpha - 6 * beta / dx ** 2)
+    assert np.abs(out - factor * e).max() < 1e-12 * np.abs(e).max()
+    # fixed point: rhs = L(phi) leaves phi unchanged
+    rng = np.random.default_rng(14)
+    phi = rng.standard_normal(h.cells(0))
+    f = o.level_apply(0, phi, homog=True)
+    assert np.abs(o.level_relax(0, phi, f, 3) - phi).max() < 1e-11 * np.abs(phi).max()
+
+
+def test_jacobi_multigrid_solve_matches_sparse_direct_solve():
+    n = 16
+    h = small_single(n=n, box=8)
+    o = Oracle(h)
+    o.set_relax_mode(4)
+    rhs = H.sinusoid_rhs(h, 1001)
+    sp_, rp = H.solver_params(eps=1e-12, pre=4, post=4)
+    phi, hist, info = o.solve(rhs, sp_, rp)
+    assert info["exit_status"] & 1
+    x = spla.spsolve(_assemble_dirichlet(n).tocsc(), h.to_global(rhs, 0).ravel(order="F")).reshape((n, n, n),
+                                                                                                  order="F")
+    assert np.abs(h.to_global(phi, 0) - x).max() / np.abs(x).max() < 1e-10

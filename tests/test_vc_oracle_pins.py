@@ -343,3 +343,20 @@ def test_amr_solve_matches_dense_solve_of_composite_system():
     assert info["exit_status"] & 1
     x = np.linalg.solve(A[np.ix_(keep, keep)], rhs_k[keep])
     assert np.abs(phi[keep] - x).max() / np.abs(x).max() < 1e-9
+
+
+def test_vc_jacobi_sweep_unit_coefficients():
+    """VCAMRPoissonOp2::levelJacobi with a = b = 1, beta -> -beta: one sweep on a
+    Dirichlet sine mode with rhs 0 multiplies it by 1 - 0.5 (alpha + beta lam) /
+    (alpha - 6 beta / dx^2) (lambda = 1 / (alpha - 6 beta / dx^2))."""
+    n, k, alpha, beta = 16, (2, 1, 1), 0.5, 1.1
+    h = small_single(n=n, box=8, alpha=alpha, beta=-beta)
+    o = Oracle(h)
+    _unit(h, o)
+    o.set_relax_mode(4)
+    dx = 1.0 / n
+    e = h.pack(lambda x, y, z, l: np.sin(np.pi * k[0] * x) * np.sin(np.pi * k[1] * y) * np.sin(np.pi * k[2] * z))
+    lam = sum((2 / dx ** 2) * (np.cos(np.pi * kk * dx) - 1) for kk in k)
+    out = o.level_relax(0, e, np.zeros_like(e), 1)
+    factor = 1 - 0.5 * (alpha + beta * lam) / (alpha - 6 * beta / dx ** 2)
+    assert np.abs(out - factor * e).max() < 1e-12 * np.abs(e).max()
