@@ -274,7 +274,36 @@ template <class Real>
 void AMRPoissonOp<Real>::relax(T& e, const T& r, int n) {
   for (int i = 0; i < n; i++) {
     if (m_relaxMode == 4) levelJacobi(e, r);
+    else if (m_relaxMode == 3) levelGSRBLazy(e, r);
+    else if (m_relaxMode == 5) levelMultiColor(e, r);
     else levelGSRB(e, r);
+  }
+}
+// levelGSRBLazy (:1524-1578): the eight parity colours in nextColorLoc's order
+// (x fastest); for each, the homogeneous operator on the whole level, then
+// GSRBLAZY on the cells of that colour
+template <class Real>
+void AMRPoissonOp<Real>::levelGSRBLazy(T& phi, const T& rhs) {
+  if (m_vc) throw std::runtime_error("levelGSRBLazy is not implemented for variable coefficients");
+  need(phi.lay, m_lay, "levelGSRBLazy"); need(rhs.lay, m_lay, "levelGSRBLazy");
+  if (!m_jacobiTmp.isDefined()) m_jacobiTmp.define(m_lay);
+  for (int c = 0; c < 8; c++) {
+    const int color[3] = {c & 1, (c >> 1) & 1, (c >> 2) & 1};
+    applyOp(m_jacobiTmp, phi, true);
+    ops::gsrb_lazy<Real>(*m_lay, phi.d, m_jacobiTmp.d, rhs.d, color, m_dx, m_alpha, m_beta);
+  }
+}
+// levelMultiColor (:1284-1331) over amrpgetMultiColors' 3D order (:36-56): for each
+// colour a homogeneous ghost refresh, then GSMCAMRPOP on the cells of that colour
+template <class Real>
+void AMRPoissonOp<Real>::levelMultiColor(T& phi, const T& rhs) {
+  if (m_vc) throw std::runtime_error("levelMultiColor is not implemented for variable coefficients");
+  need(phi.lay, m_lay, "levelMultiColor"); need(rhs.lay, m_lay, "levelMultiColor");
+  static const int colors[8][3] = {{0, 0, 0}, {1, 1, 0}, {0, 1, 1}, {1, 0, 1},
+                                   {0, 1, 0}, {1, 0, 0}, {0, 0, 1}, {1, 1, 1}};
+  for (int c = 0; c < 8; c++) {
+    fillGhosts(phi, true, true);
+    ops::gsmc<Real>(*m_lay, phi.d, rhs.d, colors[c], m_dx, m_alpha, m_beta);
   }
 }
 // levelJacobi (AMRPoissonOp.cpp:1579-1596; VCAMRPoissonOp2.cpp:762-786): the homogeneous

@@ -54,10 +54,13 @@ class AMRPoissonOp : public AMRLevelOp<LevelData<Real>> {
   // GSRBHELMHOLTZVC; the one-launch constant-coefficient relaxation is not used.
   std::shared_ptr<VCCoefs<Real>> m_vc;
   int m_vcShape = -1;                              // tile shape of the one-launch VC relaxation (-1 automatic)
-  // AMRPoissonOp::s_relaxMode (AMRPoissonOp.cpp:31, 600-632): 1 levelGSRB, 4 levelJacobi
+  // AMRPoissonOp::s_relaxMode (AMRPoissonOp.cpp:31, 600-632): 1 levelGSRB, 3 levelGSRBLazy,
+  // 4 levelJacobi, 5 levelMultiColor
   int m_relaxMode = 1;
   LevelData<Real> m_jacobiTmp;
   void levelJacobi(T& phi, const T& rhs);
+  void levelGSRBLazy(T& phi, const T& rhs);
+  void levelMultiColor(T& phi, const T& rhs);
   // reflux with variable coefficients: each contribution's face in the face arrays
   std::vector<RefluxFace> m_rcFace, m_rfFace;
   VCFaceIdx* d_vcRcFace = nullptr;

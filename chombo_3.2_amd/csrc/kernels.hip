@@ -452,6 +452,61 @@ void gsrb(const DeviceLayout& L, T* phi, const T* rhs, int color, double dx, dou
     LAUNCH_PU(KC_GSRB, 12.0 * sizeof(T) / 8.0 * L.valid_cells, 0.5 * L.valid_cells, (k_gsrb<T, false>), L.ntiles, L.d_boxes, L.d_tiles, geom(L), phi, rhs, color, T(dxinv), T(lambda), T(alpha), T(beta));
 }
 
+// Eight-colour point updates: colour (cx, cy, cz) is the set of cells whose
+// index parities are (i & 1, j & 1, k & 1) -- in two's complement i & 1 is the
+// parity of |i|, the test the reference's coloured boxes use.
+//
+// GSMCAMRPOP (AMRPoissonOpF.ChF:10-51): phi += (rhs - lap phi) / (alpha - beta sum_b),
+// the Laplacian summed direction by direction as (phi+ + phi-) - 2 phi.
+template <class T>
+__global__ void __launch_bounds__(256) k_gsmc(const DBox* __restrict__ boxes, const Tile* __restrict__ tiles,
+                                              TileGeom g, T* __restrict__ phi, const T* __restrict__ rhs,
+                                              int cx, int cy, int cz, T dxinv, T lambda) {
+  DBox b;
+  int i, j, k;
+  if (!tile_cell(boxes, tiles, g, b, i, j, k)) return;
+  if ((i & 1) != cx || (j & 1) != cy || (k & 1) != cz) return;
+  const long long c = cidx(b, i, j, k);
+  const long long sy = b.sy, sz = b.sz;
+  const T p = phi[c];
+  T lphi = ((phi[c + 1] + phi[c - 1] - T(2.0) * p) + (phi[c + sy] + phi[c - sy] - T(2.0) * p)) +
+           (phi[c + sz] + phi[c - sz] - T(2.0) * p);
+  lphi = lphi * dxinv;
+  phi[c] = p + lambda * (rhs[c] - lphi);
+}
+template <class T>
+void gsmc(const DeviceLayout& L, T* phi, const T* rhs, const int color[3], double dx, double alpha, double beta) {
+  double dxinv = 1.0 / (dx * dx);
+  double sum_b = 0.0;
+  for (int d = 0; d < 3; d++) sum_b = sum_b + 2.0 * dxinv;
+  double lambda = 1.0 / (alpha - beta * sum_b);
+  LAUNCH_PU(KC_GSRB, 3.0 * sizeof(T) / 8.0 * L.valid_cells, 0.125 * L.valid_cells, (k_gsmc<T>), L.ntiles,
+            L.d_boxes, L.d_tiles, geom(L), phi, rhs, color[0], color[1], color[2], T(dxinv), T(lambda));
+}
+// GSRBLAZY (AMRPoissonOpF.ChF:200-232): phi -= lambda (rhs - lphi), lambda = -1 / (alpha - beta sum_b),
+// with lphi the operator applied to the whole level beforehand
+template <class T>
+__global__ void __launch_bounds__(256) k_gsrb_lazy(const DBox* __restrict__ boxes, const Tile* __restrict__ tiles,
+                                                   TileGeom g, T* __restrict__ phi, const T* __restrict__ lphi,
+                                                   const T* __restrict__ rhs, int cx, int cy, int cz, T lambda) {
+  DBox b;
+  int i, j, k;
+  if (!tile_cell(boxes, tiles, g, b, i, j, k)) return;
+  if ((i & 1) != cx || (j & 1) != cy || (k & 1) != cz) return;
+  const long long c = cidx(b, i, j, k);
+  phi[c] = phi[c] - lambda * (rhs[c] - lphi[c]);
+}
+template <class T>
+void gsrb_lazy(const DeviceLayout& L, T* phi, const T* lphi, const T* rhs, const int color[3], double dx,
+               double alpha, double beta) {
+  double dxinv = 1.0 / (dx * dx);
+  double sum_b = 0.0;
+  for (int d = 0; d < 3; d++) sum_b = sum_b + 2.0 * dxinv;
+  double lambda = -1.0 / (alpha - beta * sum_b);
+  LAUNCH_PU(KC_GSRB, 4.0 * sizeof(T) / 8.0 * L.valid_cells, 0.125 * L.valid_cells, (k_gsrb_lazy<T>), L.ntiles,
+            L.d_boxes, L.d_tiles, geom(L), phi, lphi, rhs, color[0], color[1], color[2], T(lambda));
+}
+
 // ---------------------------------------------------------------------------
 // One whole GSRB iteration in one pass over HBM, on a merged level.
 //
@@ -1442,6 +1497,8 @@ void reflux(Context* ctx, const RefluxCell* cells, int ncells, const RefluxCoars
 #define INST(T)                                                                                                   \
   template void fill_ghosts<T>(const DeviceLayout&, const GhostPlan&, T*, int, int, double, double);           \
   template void gsrb<T>(const DeviceLayout&, T*, const T*, int, double, double, double);                       \
+  template void gsmc<T>(const DeviceLayout&, T*, const T*, const int*, double, double, double);                \
+  template void gsrb_lazy<T>(const DeviceLayout&, T*, const T*, const T*, const int*, double, double, double); \
   template void gsrb_fused<T>(const DeviceLayout&, const FaceRules&, T*, const T*, const T*, double, double,      \
                               double, int, int, bool);                                                        \
   template std::pair<int, int> tune_gsrb_fused<T>(const DeviceLayout&, const FaceRules&, T*, const T*, const T*,  \

@@ -46,6 +46,13 @@ void fill_ghosts(const DeviceLayout& L, const GhostPlan& P, T* phi, int doCF, in
                  double dxCrse);
 template <class T>
 void gsrb(const DeviceLayout& L, T* phi, const T* rhs, int color, double dx, double alpha, double beta);
+// Eight-colour point updates on the cells with index parities color[0..2]:
+// GSMCAMRPOP (levelMultiColor) and GSRBLAZY (levelGSRBLazy, lphi = L phi)
+template <class T>
+void gsmc(const DeviceLayout& L, T* phi, const T* rhs, const int color[3], double dx, double alpha, double beta);
+template <class T>
+void gsrb_lazy(const DeviceLayout& L, T* phi, const T* lphi, const T* rhs, const int color[3], double dx,
+               double alpha, double beta);
 
 // Ghost rule of each face of a merged level's rectangle during relaxation
 // (homogeneous): 0 Neumann, 1 Dirichlet, 2 quadratic coarse-fine with c1, c2.

@@ -330,6 +330,46 @@ static void k_gsrb_helm(Fab* phi, const Fab* rhs, const Box* r, double dx,
       }
     }
 }
+/* The first cell of a coloured box: lo raised by one in each direction whose
+   parity differs from the colour (AMRPoissonOp.cpp:1306-1313, 1556-1563). */
+static int colored_box(const Box* v, const int color[3], Box* out) {
+  *out = *v;
+  for (int d = 0; d < 3; d++)
+    if (abs(out->lo[d]) % 2 != color[d]) out->lo[d]++;
+  return out->lo[0] <= out->hi[0] && out->lo[1] <= out->hi[1] && out->lo[2] <= out->hi[2];
+}
+/* GSMCAMRPOP (AMRPoissonOpF.ChF:10-51): a point update with the Laplacian only,
+   phi += (rhs - lap phi) / (alpha - beta sum_b), over every second cell of the
+   coloured box in each direction */
+static void k_gsmc(Fab* phi, const Fab* rhs, const Box* cb, double dx, double alpha, double beta) {
+  double dxinv = 1.0/(dx*dx);
+  double sum_b = 0.0;
+  for (int d = 0; d < 3; d++) sum_b = sum_b + 2.0*dxinv;
+  double lambda = 1.0/(alpha - beta*sum_b);
+  for (int k = cb->lo[2]; k <= cb->hi[2]; k += 2)
+    for (int j = cb->lo[1]; j <= cb->hi[1]; j += 2)
+      for (int i = cb->lo[0]; i <= cb->hi[0]; i += 2) {
+        double p = *P(phi,i,j,k);
+        double lphi = ((*P(phi,i+1,j,k) + *P(phi,i-1,j,k) - 2.0*p)
+                     + (*P(phi,i,j+1,k) + *P(phi,i,j-1,k) - 2.0*p))
+                     + (*P(phi,i,j,k+1) + *P(phi,i,j,k-1) - 2.0*p);
+        lphi = lphi*dxinv;
+        *P(phi,i,j,k) = p + lambda*(*P(rhs,i,j,k) - lphi);
+      }
+}
+/* GSRBLAZY (AMRPoissonOpF.ChF:200-232): phi -= lambda (rhs - L phi) with
+   lambda = -1 / (alpha - beta sum_b), over every second cell of the coloured box */
+static void k_gsrb_lazy(Fab* phi, const Fab* lphi, const Fab* rhs, const Box* cb, double alpha, double beta,
+                        double dx) {
+  double dxinv = 1.0/(dx*dx);
+  double sum_b = 0.0;
+  for (int d = 0; d < 3; d++) sum_b = sum_b + 2.0*dxinv;
+  double lambda = -1.0/(alpha - beta*sum_b);
+  for (int k = cb->lo[2]; k <= cb->hi[2]; k += 2)
+    for (int j = cb->lo[1]; j <= cb->hi[1]; j += 2)
+      for (int i = cb->lo[0]; i <= cb->hi[0]; i += 2)
+        *P(phi,i,j,k) = *P(phi,i,j,k) - lambda*(*P(rhs,i,j,k) - *P(lphi,i,j,k));
+}
 /* OPERATORLAP (AMRPoissonOpF.ChF:287-324) */
 static void k_oplap(Fab* lofphi, const Fab* phi, const Box* r, double dx, double alpha, double beta) {
   double dxinv = 1.0/(dx*dx);
@@ -1347,10 +1387,48 @@ static void op_levelJacobi(const Op* op, LD* phi, const LD* rhs) {
   }
   ld_free(&resid);
 }
-/* relax (AMRPoissonOp.cpp:600-632): s_relaxMode 1 (levelGSRB) or 4 (levelJacobi) */
+/* levelGSRBLazy (AMRPoissonOp.cpp:1524-1578): eight colours, the parities of
+   (i, j, k) in the order nextColorLoc steps them (x fastest); for each, the whole
+   homogeneous operator (applyOp, with its CF interpolation, exchange and BC) and
+   then GSRBLAZY on the cells of that colour */
+static void op_levelGSRBLazy(const Op* op, LD* phi, const LD* rhs) {
+  g_cell_relax += lay_cells(&op->L);
+  LD lphi = ld_make(&op->L, 0);
+  for (int c = 0; c < 8; c++) {
+    const int color[3] = {c & 1, (c >> 1) & 1, (c >> 2) & 1};
+    op_apply(op, &lphi, phi, 1);
+    for (int b = 0; b < op->L.n; b++) {
+      Box cb;
+      if (colored_box(&op->L.b[b], color, &cb))
+        k_gsrb_lazy(&phi->f[b], &lphi.f[b], &rhs->f[b], &cb, op->alpha, op->beta, op->dx);
+    }
+  }
+  ld_free(&lphi);
+}
+/* amrpgetMultiColors (AMRPoissonOp.cpp:36-56), 3D order */
+static const int k_multi_colors[8][3] = {{0,0,0},{1,1,0},{0,1,1},{1,0,1},{0,1,0},{1,0,0},{0,0,1},{1,1,1}};
+/* levelMultiColor (AMRPoissonOp.cpp:1284-1331): for each of the eight colours,
+   homogeneous CF interpolation, exchange, homogeneous BC, then GSMCAMRPOP */
+static void op_levelMultiColor(const Op* op, LD* phi, const LD* rhs) {
+  g_cell_relax += lay_cells(&op->L);
+  for (int c = 0; c < 8; c++) {
+    op_homog_cf(op, phi);
+    op_exchange(op, phi);
+    for (int b = 0; b < op->L.n; b++) {
+      Box cb;
+      apply_bc(&phi->f[b], &op->L.b[b], &op->domain, op->dx, 1, &op->bc);
+      if (colored_box(&op->L.b[b], k_multi_colors[c], &cb))
+        k_gsmc(&phi->f[b], &rhs->f[b], &cb, op->dx, op->alpha, op->beta);
+    }
+  }
+}
+/* relax (AMRPoissonOp.cpp:600-632): s_relaxMode 1 (levelGSRB), 3 (levelGSRBLazy),
+   4 (levelJacobi) or 5 (levelMultiColor) */
 static void op_relax(const Op* op, LD* e, const LD* r, int n) {
   for (int i = 0; i < n; i++) {
     if (op->relaxMode == 4) op_levelJacobi(op, e, r);
+    else if (op->relaxMode == 3) op_levelGSRBLazy(op, e, r);
+    else if (op->relaxMode == 5) op_levelMultiColor(op, e, r);
     else op_levelGSRB(op, e, r);
   }
 }
@@ -1711,6 +1789,10 @@ static const Op* mg_op(const OHier* h, int d);
 int oracle_vc_set(OHier* h, const double* a, const double* b0, const double* b1, const double* b2) {
   const double* bs[3] = {b0, b1, b2};
   long ao = 0, bo[3] = {0, 0, 0};
+  if (h->op[0].relaxMode == 3 || h->op[0].relaxMode == 5) {
+    set_err("relax modes 3 and 5 are not implemented for variable coefficients");
+    return -1;
+  }
   /* every AMR level: its own coefficients (VCAMRPoissonOp2Factory::AMRnewOp, :1074-1171) */
   for (int l = 0; l < h->nlev; l++) {
     Op* op = &h->op[l];
@@ -1743,7 +1825,15 @@ int oracle_vc_set(OHier* h, const double* a, const double* b0, const double* b1,
 }
 /* AMRPoissonOp::s_relaxMode (AMRPoissonOp.cpp:31, 600-632) for every operator: 1 or 4 */
 int oracle_set_relax_mode(OHier* h, int mode) {
-  if (mode != 1 && mode != 4) { set_err("relax mode must be 1 (levelGSRB) or 4 (levelJacobi)"); return -1; }
+  if (mode != 1 && mode != 3 && mode != 4 && mode != 5) {
+    set_err("relax mode must be 1 (levelGSRB), 3 (levelGSRBLazy), 4 (levelJacobi) or 5 (levelMultiColor)");
+    return -1;
+  }
+  /* VCAMRPoissonOp2 aborts in levelGSRBLazy and levelMultiColor (VCAMRPoissonOp2.cpp:628-633, 755-760) */
+  if ((mode == 3 || mode == 5) && h->op[0].vc) {
+    set_err("relax modes 3 and 5 are not implemented for variable coefficients");
+    return -1;
+  }
   for (int l = 0; l < h->nlev; l++) h->op[l].relaxMode = mode;
   for (int d = 1; d < h->mgD; d++) h->mgop[d].relaxMode = mode;
   return 0;

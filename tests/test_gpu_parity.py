@@ -208,19 +208,23 @@ def test_fcycle_solve_bitwise(gpu_ctx, numMG, case):
     np.testing.assert_array_equal(phi_g, phi_o)
 
 
-@pytest.mark.parametrize("case", ["helm24", "amr32"])
-def test_jacobi_relax_mode_bitwise(gpu_ctx, case):
-    """s_relaxMode 4 (levelJacobi) everywhere: level relaxation at every depth and
-    whole solves (host-driven bottom) bitwise the oracle's."""
+@pytest.mark.parametrize("mode,case", [(4, "helm24"), (4, "amr32"), (3, "helm24"), (3, "lap32_negidx"), (3, "amr32"),
+                                       (5, "helm24"), (5, "ragged40"), (5, "amr32")])
+def test_relax_modes_bitwise(gpu_ctx, mode, case):
+    """s_relaxMode 3 (levelGSRBLazy), 4 (levelJacobi) and 5 (levelMultiColor)
+    everywhere: level relaxation at every depth and whole solves (host-driven
+    bottom) bitwise the oracle's (GSMCAMRPOP relaxes with the Laplacian alone; the
+    Helmholtz solve still converges through the coarse corrections)."""
+    print(f"relax mode {mode} {case}", flush=True)
     if case == "amr32":
-        h = small_amr(alpha=0.3, beta=0.9)
+        h = small_amr(alpha=0.0 if mode == 5 else 0.3, beta=1.0 if mode == 5 else 0.9)
         rhs = H.bump_rhs(h, 1031)
     else:
         h = small_single(**SINGLE_CASES[case])
         rhs = H.sinusoid_rhs(h, 1031)
     g, o = _pair(gpu_ctx, h)
-    g.set_relax_mode(4)
-    o.set_relax_mode(4)
+    g.set_relax_mode(mode)
+    o.set_relax_mode(mode)
     rng = np.random.default_rng(15)
     for d in range(o.mg_depths):
         n = o.mg_cells(d)
@@ -236,6 +240,27 @@ def test_jacobi_relax_mode_bitwise(gpu_ctx, case):
     np.testing.assert_array_equal(hist_g, hist_o)
     np.testing.assert_array_equal(g.get_field(g.PHI), phi_o)
     g.set_relax_mode(1)
+    g.close()
+
+
+def test_relax_mode_errors(gpu_ctx):
+    """Modes 0 and 2 are refused; 3 and 5 are refused with variable coefficients,
+    and variable coefficients are refused while 3 or 5 is set."""
+    h = small_single(n=16, box=8)
+    g = amrmg.AMRMultiGrid(gpu_ctx, h)
+    for mode in (0, 2, 6):
+        with pytest.raises(RuntimeError):
+            g.set_relax_mode(mode)
+    a, b = H.vc_coefficients(h, seed=3)
+    g.set_relax_mode(5)
+    with pytest.raises(RuntimeError):
+        g.set_vc_coefs(a, b)
+    g.set_relax_mode(1)
+    g.set_vc_coefs(a, b)
+    for mode in (3, 5):
+        with pytest.raises(RuntimeError):
+            g.set_relax_mode(mode)
+    g.set_relax_mode(4)
     g.close()
 
 

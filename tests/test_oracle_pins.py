@@ -428,3 +428,102 @@ def test_jacobi_multigrid_solve_matches_sparse_direct_solve():
     x = spla.spsolve(_assemble_dirichlet(n).tocsc(), h.to_global(rhs, 0).ravel(order="F")).reshape((n, n, n),
                                                                                                   order="F")
     assert np.abs(h.to_global(phi, 0) - x).max() / np.abs(x).max() < 1e-10
+
+
+def _colour_sweep(A, L, x, b, colours, lo, n, d):
+    """A brute-force eight-colour sweep on assembled matrices: for each colour in
+    order, x += (b - L x) / d on the cells whose absolute index parities are that
+    colour (L is the full operator for levelGSRBLazy, the Laplacian alone for
+    levelMultiColor's GSMCAMRPOP)."""
+    i = np.arange(n) + lo
+    I, J, K = np.meshgrid(i, i, i, indexing="ij")
+    par = [(I & 1).ravel(order="F"), (J & 1).ravel(order="F"), (K & 1).ravel(order="F")]
+    x = x.copy()
+    for c in colours:
+        m = (par[0] == c[0]) & (par[1] == c[1]) & (par[2] == c[2])
+        r = b - L @ x
+        x[m] += r[m] / d
+    return x
+
+
+LAZY_COLOURS = [(c & 1, (c >> 1) & 1, (c >> 2) & 1) for c in range(8)]
+MULTI_COLOURS = [(0, 0, 0), (1, 1, 0), (0, 1, 1), (1, 0, 1), (0, 1, 0), (1, 0, 0), (0, 0, 1), (1, 1, 1)]
+
+
+@pytest.mark.parametrize("mode", [3, 5])
+@pytest.mark.parametrize("alpha,beta,lo", [(0.0, 1.0, 0), (0.7, 1.3, -8), (0.0, 1.0, -5)])
+def test_colour_relax_matches_assembled_sweep(mode, alpha, beta, lo):
+    """levelGSRBLazy (mode 3) and levelMultiColor (mode 5): two sweeps equal two
+    brute-force sweeps over an independently assembled Dirichlet operator with the
+    colours in the reference's orders (nextColorLoc, x fastest; amrpgetMultiColors);
+    the relaxation weight is 1 / (alpha - 6 beta / dx^2) on every cell, boundary
+    cells included, and GSMCAMRPOP uses the Laplacian alone."""
+    n = 16
+    h = small_single(n=n, box=8, block=1 if lo % 8 else 8, lo=lo, alpha=alpha, beta=beta)
+    o = Oracle(h)
+    o.set_relax_mode(mode)
+    rng = np.random.default_rng(21 + mode)
+    phi = rng.standard_normal(h.cells(0))
+    rhs = rng.standard_normal(h.cells(0))
+    out = h.to_global(o.level_relax(0, phi, rhs, 2), 0).ravel(order="F")
+    dx = 1.0 / n
+    A = _assemble_dirichlet(n, alpha, beta)
+    L = A if mode == 3 else _assemble_dirichlet(n, 0.0, 1.0)
+    x = h.to_global(phi, 0).ravel(order="F")
+    b = h.to_global(rhs, 0).ravel(order="F")
+    for _ in range(2):
+        x = _colour_sweep(A, L, x, b, LAZY_COLOURS if mode == 3 else MULTI_COLOURS, lo, n, alpha - 6 * beta / dx ** 2)
+    assert np.abs(out - x).max() < 1e-12 * np.abs(x).max()
+    # the other colour order (or the other mode's update) is a different sweep
+    y = h.to_global(phi, 0).ravel(order="F")
+    for _ in range(2):
+        y = _colour_sweep(A, L, y, b, MULTI_COLOURS if mode == 3 else LAZY_COLOURS, lo, n,
+                          alpha - 6 * beta / dx ** 2)
+    assert np.abs(out - y).max() > 1e-6 * np.abs(y).max()
+
+
+def test_colour_relax_interior_cells_are_gauss_seidel():
+    """Away from the boundary one colour step of levelGSRBLazy zeroes the residual on
+    that colour's cells (exact point Gauss-Seidel): after a whole sweep the last
+    colour's interior residual is zero."""
+    n = 16
+    h = small_single(n=n, box=8, alpha=0.4, beta=1.0)
+    o = Oracle(h)
+    o.set_relax_mode(3)
+    rng = np.random.default_rng(31)
+    phi = rng.standard_normal(h.cells(0))
+    rhs = rng.standard_normal(h.cells(0))
+    out = o.level_relax(0, phi, rhs, 1)
+    r = h.to_global(rhs - o.level_apply(0, out, homog=True), 0)
+    inner = r[1:-1, 1:-1, 1:-1]
+    last = inner[0::2, 0::2, 0::2]     # absolute parity (1, 1, 1) starts at index 1
+    assert np.abs(last).max() < 1e-10 * np.abs(r).max()
+    assert np.abs(inner[1::2, 1::2, 1::2]).max() > 1e-3 * np.abs(r).max()
+
+
+@pytest.mark.parametrize("mode", [3, 5])
+def test_colour_relax_multigrid_solve_matches_sparse_direct_solve(mode):
+    n = 16
+    h = small_single(n=n, box=8)
+    o = Oracle(h)
+    o.set_relax_mode(mode)
+    rhs = H.sinusoid_rhs(h, 1003)
+    sp_, rp = H.solver_params(eps=1e-12)
+    phi, hist, info = o.solve(rhs, sp_, rp)
+    assert info["exit_status"] & 1
+    x = spla.spsolve(_assemble_dirichlet(n).tocsc(), h.to_global(rhs, 0).ravel(order="F")).reshape((n, n, n),
+                                                                                                  order="F")
+    assert np.abs(h.to_global(phi, 0) - x).max() / np.abs(x).max() < 1e-10
+
+
+def test_colour_relax_modes_refused_with_variable_coefficients():
+    h = small_single(n=16, box=8)
+    o = Oracle(h)
+    a, b = H.vc_coefficients(h, seed=3)
+    o.vc_set(a, b)
+    for mode in (3, 5):
+        with pytest.raises(RuntimeError):
+            o.set_relax_mode(mode)
+    for mode in (0, 2, 6):
+        with pytest.raises(RuntimeError):
+            o.set_relax_mode(mode)
