@@ -368,3 +368,24 @@ def test_set_alpha_beta_equals_fresh_solver(gpu_ctx):
     np.testing.assert_array_equal(hist_a, hist_b)
     np.testing.assert_array_equal(phi_a, f.get_field(f.PHI))
     f.close()
+
+
+@pytest.mark.parametrize("kw", [dict(iterMax=3), dict(hang=0.95, imin=2), dict(normThresh=1e3),
+                                dict(eps=0.5, iterMax=1)])
+def test_exit_status_cases_match_oracle(gpu_ctx, kw):
+    """Each stopping test of solveNoInitResid forced on its own: the GPU solve stops
+    at the same iteration with the same status, history and (bitwise) solution."""
+    print(f"exit status {kw}", flush=True)
+    h = small_single(n=16, box=8)
+    rhs = H.sinusoid_rhs(h, 1403)
+    g, o = _pair(gpu_ctx, h)
+    sp, rp = H.solver_params(**kw)
+    g.setSolverParameters(sp, rp)
+    g.set_field(g.RHS, rhs)
+    hist_g, info_g = g.solve()
+    phi_o, hist_o, info_o = o.solve(rhs, sp, rp)
+    assert info_g["exit_status"] == info_o["exit_status"]
+    assert info_g["iterations"] == info_o["iterations"]
+    np.testing.assert_array_equal(hist_g, hist_o)
+    np.testing.assert_array_equal(g.get_field(g.PHI), phi_o)
+    g.close()
