@@ -23,7 +23,8 @@ import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
 import hierarchy as H
-from helpers import ghost_dir, ghost_masks, ghosted_from_fn, lshape_qcfi, small_amr, small_single, uncovered_mask
+from helpers import (ghost_dir, ghost_masks, ghosted_from_fn, lshape_amr, lshape_qcfi, small_amr, small_single,
+                     uncovered_mask)
 from oracle import Oracle
 
 
@@ -527,3 +528,30 @@ def test_colour_relax_modes_refused_with_variable_coefficients():
     for mode in (0, 2, 6):
         with pytest.raises(RuntimeError):
             o.set_relax_mode(mode)
+
+
+@pytest.mark.parametrize("case,lev", [("qcfi", 1), ("lshape", 1), ("lshape", 2), ("amr", 1), ("amr", 2)])
+def test_quadcfinterp_never_reads_covered_coarse_cells(case, lev):
+    """Every QuadCFStencil branch (centred, shifted, dropped order, mixed) may only
+    use coarse cells that no finer box covers (ivsAllGood, QuadCFStencil.cpp:578-654):
+    coarse values under the fine level are never read, so poisoning them with 1e300
+    leaves every CF ghost bitwise unchanged.  A centred stencil taken where a
+    neighbour is covered reads a poisoned cell."""
+    h = {"qcfi": lshape_qcfi, "lshape": lshape_amr, "amr": small_amr}[case]()
+    o = Oracle(h)
+    rng = np.random.default_rng(40 + lev)
+    offs = h.level_offsets()
+    coarse = rng.standard_normal(h.cells(lev - 1))
+    covered = ~uncovered_mask(h, lev - 1)
+    assert covered.any()
+    poisoned = np.where(covered, 1e300, coarse)
+    from helpers import ghosted_random
+    pg = ghosted_random(h, lev, rng)
+    _, _, cf = ghost_masks(h, lev)
+    a = o.quad_cf(lev, pg, coarse)
+    b = o.quad_cf(lev, pg, poisoned)
+    assert np.isfinite(b[cf]).all() and np.abs(b[cf]).max() < 1e3
+    np.testing.assert_array_equal(a[cf], b[cf])
+    # the poison is not inert: the same coarse cells are read when uncovered
+    c = o.quad_cf(lev, pg, np.where(~covered, 1e300, coarse))
+    assert np.abs(c[cf]).max() > 1e200
