@@ -1095,6 +1095,7 @@ void BatchSolver::setRelaxShape(int d, int shape, int kc) {
     for (int dir = 0; dir < 3; dir++)
       for (int side = 0; side < 2; side++) a.kind[dir][side] = L.h_coef[p].kind[dir][side];
     a.zsrc = 0;
+    ops::check_pair_rows(a);
     L.h_items.push_back(a);
     L.wStart.push_back(L.wStart.back() + a.nwork);
   }
@@ -1222,6 +1223,8 @@ void BatchSolver::relaxDepth(int d, int np, int iters, bool fromZero) {
     const int zsrc = (fromZero && it == 0) ? 1 : 0;
     T* src = (T*)((it & 1) ? L.tmp : L.corr);
     T* dst = (T*)((it & 1) ? L.corr : L.tmp);
+    ops::check_pair_pointer(src, "batch relaxation");
+    ops::check_pair_pointer(dst, "batch relaxation");
     hipEvent_t ev = profile_begin(m_ctx, KC_GSRB);
     if (anyLap)
       launch_batch_fused<T, true>(m_ctx, L.shape, grid, L.d_items, L.d_wStart, np, L.d_coef, d_active, zsrc, dst, src,

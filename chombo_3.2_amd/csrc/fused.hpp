@@ -5,11 +5,20 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
 namespace amrmg {
 namespace ops {
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 constexpr int NSLOT = 5;  // LDS plane buffers: planes k-1 .. k+2 in use plus the one being filled
+// Two x-adjacent values moved as one 8- or 16-byte access (loads, LDS, stores).
+template <class T>
+struct alignas(2 * sizeof(T)) Pair {
+  T x, y;
+};
 struct FusedArgs {
   int lo[3], hi[3];   // the level's rectangle
   long long off;      // slab index of (lo-1, lo-1, lo-1)
@@ -52,6 +61,16 @@ __device__ __forceinline__ T fused_new(const FusedArgs& a, const T* Sm, const T*
     return own + lambda * (helmop - r);
   }
 }
+// Host-side checks of what the pair accesses of fused_column assume.
+inline void check_pair_rows(const FusedArgs& a) {
+  if ((a.off + 1) % 2 != 0 || a.sy % 2 != 0 || a.sz % 2 != 0)
+    throw std::runtime_error("one-launch relaxation: rows must start on an even slab index");
+}
+template <class T>
+inline void check_pair_pointer(const T* p, const char* what) {
+  if (reinterpret_cast<uintptr_t>(p) % (2 * sizeof(T)) != 0)
+    throw std::runtime_error(std::string(what) + ": field not aligned to a pair of values");
+}
 // Workgroup barrier for LDS traffic only: waits for this wave's LDS operations,
 // not for its global loads, so the next planes' loads stay in flight across it.
 // Global memory needs no ordering here (src and rhs are read-only, each dst cell
@@ -60,19 +79,30 @@ __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(
 
 // One workgroup's share of a one-launch relaxation: work item w (a column of
 // FX x FY cells, FX even, over one z-chunk) of the rectangle described by a.
+//
+// Old values move in pairs of x-adjacent cells (i, i+1) with i - lo[0] even: one
+// 8- or 16-byte load, LDS write, LDS read and store per pair.  The launchers check
+// that this is aligned: the slab index of (lo[0], j, k) is even for every row
+// (off odd, sy and sz even) and the field pointers are aligned to a pair.  A pair
+// load clamps its first cell into [lo[0], last pair start], so it stays inside the
+// row's storage (the pair that holds hi[0] may end on the ghost cell hi[0]+1).
 template <class T, bool LAP, int FX, int FY, int NT>
 __device__ __forceinline__ void fused_column(const FusedArgs& a, int w, T* __restrict__ dst,
                                              const T* __restrict__ src, const T* __restrict__ rhs, T dxinv,
                                              T lambda, T alpha, T beta, T c1, T c2) {
   constexpr int LW = FX + 4, LH = FY + 4, LPLANE = LW * LH;  // LDS plane: the column grown by 2
-  constexpr int NLD = (LPLANE + NT - 1) / NT;                   // plane values per thread
+  constexpr int LW2 = LW / 2, NPAIR = LW2 * LH;                // pairs per LDS row, per plane
+  constexpr int NLD = (NPAIR + NT - 1) / NT;                    // plane pairs per thread
+  constexpr int SW2 = FX / 2, NSP = SW2 * FY;                   // pairs per stored row, per plane
   constexpr int NRC = (FY + 2) * (FX / 2 + 1);                // red cells of the column grown by 1
   constexpr int NRED = (NRC + NT - 1) / NT;
   constexpr int NBC = FX * FY / 2;                            // black cells of the column
   constexpr int NBLK = (NBC + NT - 1) / NT;
-  constexpr int NST = (FX * FY + NT - 1) / NT;
+  constexpr int NST = (NSP + NT - 1) / NT;
   static_assert(FX % 2 == 0, "column width must be even");
-  __shared__ T P[NSLOT][LPLANE];
+  __shared__ __align__(16) T P[NSLOT][LPLANE];
+  using P2 = Pair<T>;
+  const int pmax = a.lo[0] + 2 * ((a.hi[0] - a.lo[0]) / 2);  // start of the pair that holds hi[0]
   const int tx = w % a.ntx, ty = (w / a.ntx) % a.nty, ch = w / (a.ntx * a.nty);
   const int x0 = a.lo[0] + tx * FX, y0 = a.lo[1] + ty * FY;
   const int k0 = a.lo[2] + ch * a.kc;
@@ -102,20 +132,21 @@ __device__ __forceinline__ void fused_column(const FusedArgs& a, int w, T* __res
   // of a cell that is not updated) is never selected into a result.  A select at
   // load time would make the wave wait for the load there, ending the pipeline.
   struct Pre {
-    T phi[NLD], rr[NRED], rb[NBLK];
+    P2 phi[NLD];
+    T rr[NRED], rb[NBLK];
   };
   Pre A, B;
   auto fetch = [&](Pre& f, int p, int r, int k) {  // plane p; red cells of plane r; black cells of plane k
     const int pc = clampi(p, a.lo[2], a.hi[2]);
     if (a.zsrc) {  // uniform branch: relaxation of a field known to be zero
 #pragma unroll
-      for (int u = 0; u < NLD; u++) f.phi[u] = T(0.0);
+      for (int u = 0; u < NLD; u++) f.phi[u] = P2{T(0.0), T(0.0)};
     } else {
 #pragma unroll
       for (int u = 0; u < NLD; u++) {
         const int q = tid + NT * u;
-        const int i = x0 - 2 + q % LW, j = y0 - 2 + q / LW;
-        f.phi[u] = src[gidx(clampi(i, a.lo[0], a.hi[0]), clampi(j, a.lo[1], a.hi[1]), pc)];
+        const int i = x0 - 2 + 2 * (q % LW2), j = y0 - 2 + q / LW2;
+        f.phi[u] = *reinterpret_cast<const P2*>(src + gidx(clampi(i, a.lo[0], pmax), clampi(j, a.lo[1], a.hi[1]), pc));
       }
     }
     const int rc = clampi(r, a.lo[2], a.hi[2]);
@@ -134,26 +165,30 @@ __device__ __forceinline__ void fused_column(const FusedArgs& a, int w, T* __res
     }
   };
   auto put_phi = [&](const Pre& f, int p) {
-    T* S = P[slot(p)];
+    P2* S = reinterpret_cast<P2*>(P[slot(p)]);  // pair q of the plane: LDS values 2q, 2q+1
 #pragma unroll
     for (int u = 0; u < NLD; u++) {
       const int q = tid + NT * u;
-      if (q < LPLANE) S[q] = f.phi[u];
+      if (q < NPAIR) S[q] = f.phi[u];
     }
   };
   auto store = [&](int p) {
-    const T* S = P[slot(p)];
-    T v[NST];
+    const P2* S = reinterpret_cast<const P2*>(P[slot(p)]);
+    P2 v[NST];
 #pragma unroll
     for (int u = 0; u < NST; u++) {
-      const int q = min(tid + NT * u, FX * FY - 1);
-      v[u] = S[(q / FX + 2) * LW + (q % FX + 2)];
+      const int q = min(tid + NT * u, NSP - 1);
+      v[u] = S[(q / SW2 + 2) * LW2 + (q % SW2 + 1)];
     }
 #pragma unroll
     for (int u = 0; u < NST; u++) {
       const int q = tid + NT * u;
-      const int i = x0 + q % FX, j = y0 + q / FX;
-      if (q < FX * FY && i <= a.hi[0] && j <= a.hi[1]) dst[gidx(i, j, p)] = v[u];
+      const int i = x0 + 2 * (q % SW2), j = y0 + q / SW2;
+      if (q < NSP && i <= a.hi[0] && j <= a.hi[1]) {
+        T* d = dst + gidx(i, j, p);
+        if (i < a.hi[0]) *reinterpret_cast<P2*>(d) = v[u];
+        else *d = v[u].x;  // the pair's second cell is the ghost cell hi[0]+1
+      }
     }
   };
   // step k: plane k+2 into LDS, red on plane k+1, black on plane k, store plane k-1

@@ -585,6 +585,7 @@ static FusedArgs fused_args(const DeviceLayout& L, const FaceRules& fr, int shap
   a.off = R.off;
   a.sy = R.sy;
   a.sz = R.sz;
+  check_pair_rows(a);
   a.zsrc = zero_src ? 1 : 0;
   a.ntx = (L.rect.len(0) + FX - 1) / FX;
   a.nty = (L.rect.len(1) + FY - 1) / FY;
@@ -605,6 +606,8 @@ template <class T>
 void gsrb_fused(const DeviceLayout& L, const FaceRules& fr, T* dst, const T* src, const T* rhs, double dx,
                 double alpha, double beta, int shape, int kc, bool zero_src) {
   if (dst == src || dst == rhs) throw std::runtime_error("gsrb_fused: dst must be a separate field");
+  check_pair_pointer(dst, "gsrb_fused");
+  check_pair_pointer(src, "gsrb_fused");
   const FusedArgs a = fused_args(L, fr, shape, kc, zero_src);
   const int grid = (a.nwork + 7) / 8 * 8;
   double dxinv = 1.0 / (dx * dx);
