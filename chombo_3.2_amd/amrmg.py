@@ -214,7 +214,8 @@ class AMRMultiGrid:
         return lib().amrmg_solver_device_bytes(self.p)
 
     RELAX_SHAPES = [(64, 8, 256), (32, 8, 256), (64, 4, 256), (32, 16, 256), (128, 4, 256), (64, 16, 256),
-                    (32, 16, 512), (64, 16, 512), (64, 32, 1024), (128, 8, 1024)]
+                    (32, 16, 512), (64, 16, 512), (64, 32, 1024), (128, 8, 1024), (64, 13, 512), (32, 13, 256),
+                    (64, 29, 1024)]
     RELAX_CHUNKS = [0, 8, 16, 32]
 
     def tune_relax(self):

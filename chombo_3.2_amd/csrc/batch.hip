@@ -1187,6 +1187,9 @@ static void launch_batch_fused(Context* ctx, int shape, int grid, const ops::Fus
     BF_CASE(7, 64, 16, 512)
     BF_CASE(8, 64, 32, 1024)
     BF_CASE(9, 128, 8, 1024)
+    BF_CASE(10, 64, 13, 512)
+    BF_CASE(11, 32, 13, 256)
+    BF_CASE(12, 64, 29, 1024)
     default: throw std::runtime_error("batch: bad tile shape");
   }
 #undef BF_CASE

@@ -246,9 +246,11 @@ __device__ __forceinline__ void fused_column(const FusedArgs& a, int w, T* __res
 }
 // Tile shapes of the one-launch relaxation: column FX x FY, NT threads.  All give
 // bitwise the same result; the fastest depends on the level's size.
-constexpr int kFusedShapeTable[][3] = {{64, 8, 256},  {32, 8, 256},  {64, 4, 256},   {32, 16, 256},
-                                       {128, 4, 256}, {64, 16, 256}, {32, 16, 512},  {64, 16, 512},
-                                       {64, 32, 1024}, {128, 8, 1024}};
+// The last three are sized so that the red cells of the column grown by one,
+// (FY + 2)(FX / 2 + 1), just fit in one pass of the NT threads.
+constexpr int kFusedShapeTable[][3] = {{64, 8, 256},  {32, 8, 256},   {64, 4, 256},   {32, 16, 256}, {128, 4, 256},
+                                       {64, 16, 256}, {32, 16, 512},  {64, 16, 512},  {64, 32, 1024},
+                                       {128, 8, 1024}, {64, 13, 512}, {32, 13, 256}, {64, 29, 1024}};
 
 // consecutive work items on one XCD (workgroups go round-robin over the 8 XCDs,
 // grid a multiple of 8), so neighbouring columns share that XCD's L2 for their

@@ -563,6 +563,9 @@ static void launch_fused(Context* ctx, int shape, int grid, const FusedArgs& a, 
     FUSED_CASE(7, 64, 16, 512)
     FUSED_CASE(8, 64, 32, 1024)
     FUSED_CASE(9, 128, 8, 1024)
+    FUSED_CASE(10, 64, 13, 512)
+    FUSED_CASE(11, 32, 13, 256)
+    FUSED_CASE(12, 64, 29, 1024)
     default: throw std::runtime_error("gsrb_fused: bad tile shape");
   }
 #undef FUSED_CASE

@@ -139,7 +139,7 @@ int amrmg_bottom_solve(amrmg_solver* s, double* corr, const double* res, int nbo
    z-chunk, its ms per iteration, then ms of each shape x chunk}; *nrec = number
    of tuned levels. */
 int amrmg_solver_tune_relax(amrmg_solver* s, double* table, int cap, int* nrec);
-/* Forces tile shape (0..9) and z-chunk (0 = automatic) of the one-launch
+/* Forces tile shape (0..12) and z-chunk (0 = automatic) of the one-launch
    relaxation on every level. */
 int amrmg_solver_set_relax_shape(amrmg_solver* s, int shape, int kc);
 /* AMRPoissonOp::s_relaxMode (AMRPoissonOp.cpp:31, 600-632) for every operator of the
