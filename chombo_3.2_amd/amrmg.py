@@ -243,8 +243,8 @@ class AMRMultiGrid:
         _chk(lib().amrmg_solver_set_alpha_beta(self.p, float(alpha), float(beta)))
 
     def set_relax_mode(self, mode):
-        """AMRPoissonOp::s_relaxMode for every operator: 1 levelGSRB, 3 levelGSRBLazy,
-        4 levelJacobi, 5 levelMultiColor."""
+        """AMRPoissonOp::s_relaxMode for every operator: 0 looseGSRB, 1 levelGSRB,
+        2 overlapGSRB, 3 levelGSRBLazy, 4 levelJacobi, 5 levelMultiColor."""
         _chk(lib().amrmg_solver_set_relax_mode(self.p, int(mode)))
 
     def set_vc_relax_shape(self, shape):

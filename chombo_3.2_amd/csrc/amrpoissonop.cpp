@@ -32,6 +32,7 @@ AMRPoissonOp<Real>::~AMRPoissonOp() {
     dev_free(m_ctx, d_rf);
     dev_free(m_ctx, d_vcRcFace);
     dev_free(m_ctx, d_vcRfFace);
+    if (m_lay && d_boxTiles != m_lay->d_tiles) dev_free(m_ctx, d_boxTiles);
   }
 }
 
@@ -274,10 +275,74 @@ template <class Real>
 void AMRPoissonOp<Real>::relax(T& e, const T& r, int n) {
   for (int i = 0; i < n; i++) {
     if (m_relaxMode == 4) levelJacobi(e, r);
+    else if (m_relaxMode == 0) looseGSRB(e, r);
+    else if (m_relaxMode == 2) overlapGSRB(e, r);
     else if (m_relaxMode == 3) levelGSRBLazy(e, r);
     else if (m_relaxMode == 5) levelMultiColor(e, r);
     else levelGSRB(e, r);
   }
+}
+template <class Real>
+const Tile* AMRPoissonOp<Real>::boxTiles() {
+  if (!m_lay->merged) return m_lay->d_tiles;
+  if (!d_boxTiles) {
+    std::vector<Tile> tiles;
+    for (int b = 0; b < m_lay->nbox; b++) {
+      const DBox& x = m_lay->h_boxes[b];
+      for (int k = x.lo[2]; k <= x.hi[2]; k += m_lay->TZ)
+        for (int j = x.lo[1]; j <= x.hi[1]; j += m_lay->TY)
+          for (int i = x.lo[0]; i <= x.hi[0]; i += m_lay->TX) tiles.push_back(Tile{b, i, j, k});
+    }
+    if (tiles.size() > (size_t)INT32_MAX) throw std::runtime_error("too many box tiles");
+    d_boxTiles = upload_vec(m_ctx, tiles);
+    n_boxTiles = (int)tiles.size();
+  }
+  return d_boxTiles;
+}
+// The neighbour values a box saw at the last ghost refresh: on per-box storage
+// its own ghost cells (phi itself), on merged storage a copy of the level.
+template <class Real>
+const Real* AMRPoissonOp<Real>::relaxSnapshot(T& phi) {
+  if (!m_lay->merged) return phi.d;
+  if (!m_relaxTmp.isDefined()) m_relaxTmp.define(m_lay);
+  ops::copy_all<Real>(*m_lay, m_relaxTmp.d, phi.d);
+  return m_relaxTmp.d;
+}
+// looseGSRB (:1333-1413): one ghost refresh, then both colours box by box with
+// the ghost values of that refresh
+template <class Real>
+void AMRPoissonOp<Real>::looseGSRB(T& phi, const T& rhs) {
+  if (m_vc) throw std::runtime_error("looseGSRB is not implemented for variable coefficients");
+  need(phi.lay, m_lay, "looseGSRB"); need(rhs.lay, m_lay, "looseGSRB");
+  fillGhosts(phi, true, true);
+  const Tile* tiles = boxTiles();
+  const int nt = m_lay->merged ? n_boxTiles : m_lay->ntiles;
+  const Real* snap = relaxSnapshot(phi);
+  for (int pass = 0; pass <= 1; pass++)
+    ops::gsrb_region<Real>(*m_lay, tiles, nt, phi.d, snap, rhs.d, pass, 0, 0, 0, false, m_dx, m_alpha, m_beta);
+}
+// overlapGSRB (:1416-1521) without MPI (exchangeBegin copies every ghost at once,
+// LevelDataI.H:533-538): looseGSRB unless alpha == 1, beta == 0 and several boxes;
+// a ghost refresh; GSRBLAPLACIAN on every box's interior, then on its six face
+// layers in the order x lo, x hi, y lo, ..., both colours each.  The boxes are
+// independent (each reads the others only through the refresh's copies), so
+// one launch covers a layer of every box.
+template <class Real>
+void AMRPoissonOp<Real>::overlapGSRB(T& phi, const T& rhs) {
+  if (m_vc) throw std::runtime_error("overlapGSRB is not implemented for variable coefficients");
+  need(phi.lay, m_lay, "overlapGSRB"); need(rhs.lay, m_lay, "overlapGSRB");
+  if (m_lay->nbox == 1 || m_alpha != 1 || m_beta != 0) looseGSRB(phi, rhs);
+  fillGhosts(phi, true, true);
+  const Tile* tiles = boxTiles();
+  const int nt = m_lay->merged ? n_boxTiles : m_lay->ntiles;
+  const Real* snap = relaxSnapshot(phi);
+  for (int pass = 0; pass <= 1; pass++)
+    ops::gsrb_region<Real>(*m_lay, tiles, nt, phi.d, snap, rhs.d, pass, 1, 0, 0, true, m_dx, m_alpha, m_beta);
+  for (int dir = 0; dir < 3; dir++)
+    for (int side = 0; side < 2; side++)
+      for (int pass = 0; pass <= 1; pass++)
+        ops::gsrb_region<Real>(*m_lay, tiles, nt, phi.d, snap, rhs.d, pass, 2, dir, side, true, m_dx, m_alpha,
+                               m_beta);
 }
 // levelGSRBLazy (:1524-1578): the eight parity colours in nextColorLoc's order
 // (x fastest); for each, the homogeneous operator on the whole level, then

@@ -54,13 +54,20 @@ class AMRPoissonOp : public AMRLevelOp<LevelData<Real>> {
   // GSRBHELMHOLTZVC; the one-launch constant-coefficient relaxation is not used.
   std::shared_ptr<VCCoefs<Real>> m_vc;
   int m_vcShape = -1;                              // tile shape of the one-launch VC relaxation (-1 automatic)
-  // AMRPoissonOp::s_relaxMode (AMRPoissonOp.cpp:31, 600-632): 1 levelGSRB, 3 levelGSRBLazy,
-  // 4 levelJacobi, 5 levelMultiColor
+  // AMRPoissonOp::s_relaxMode (AMRPoissonOp.cpp:31, 600-632): 0 looseGSRB, 1 levelGSRB,
+  // 2 overlapGSRB, 3 levelGSRBLazy, 4 levelJacobi, 5 levelMultiColor
   int m_relaxMode = 1;
   LevelData<Real> m_jacobiTmp;
   void levelJacobi(T& phi, const T& rhs);
   void levelGSRBLazy(T& phi, const T& rhs);
   void levelMultiColor(T& phi, const T& rhs);
+  void looseGSRB(T& phi, const T& rhs);
+  void overlapGSRB(T& phi, const T& rhs);
+  // tiles over each box (merged levels tile their rectangle otherwise), for looseGSRB / overlapGSRB
+  Tile* d_boxTiles = nullptr;
+  int n_boxTiles = 0;
+  const Tile* boxTiles();
+  const Real* relaxSnapshot(T& phi);
   // reflux with variable coefficients: each contribution's face in the face arrays
   std::vector<RefluxFace> m_rcFace, m_rfFace;
   VCFaceIdx* d_vcRcFace = nullptr;

@@ -300,8 +300,8 @@ int amrmg_solver_set_vc_coefs(amrmg_solver* s, const double* a, const double* b0
     lays.push_back(s->mglay(d));
   }
   Context* c = s->ctx->c.get();
-  if (a && (ops[0]->m_relaxMode == 3 || ops[0]->m_relaxMode == 5))
-    throw std::runtime_error("relax modes 3 and 5 are not implemented for variable coefficients");
+  if (a && ops[0]->m_relaxMode != 1 && ops[0]->m_relaxMode != 4)
+    throw std::runtime_error("only relax modes 1 and 4 are implemented for variable coefficients");
   if (!a) {  // back to constant coefficients
     for (auto* op : ops) op->setVC(nullptr);
     for (auto* op : ops32) op->setVC(nullptr);
@@ -604,12 +604,14 @@ int amrmg_solver_set_relax_shape(amrmg_solver* s, int shape, int kc) {
 }
 int amrmg_solver_set_relax_mode(amrmg_solver* s, int mode) {
   API_BEGIN
-  if (mode != 1 && mode != 3 && mode != 4 && mode != 5)
+  if (mode < 0 || mode > 5)
     throw std::runtime_error(
-        "relax mode must be 1 (levelGSRB), 3 (levelGSRBLazy), 4 (levelJacobi) or 5 (levelMultiColor)");
-  // VCAMRPoissonOp2 aborts in levelGSRBLazy and levelMultiColor (VCAMRPoissonOp2.cpp:628-633, 755-760)
-  if ((mode == 3 || mode == 5) && !s->vc.empty())
-    throw std::runtime_error("relax modes 3 and 5 are not implemented for variable coefficients");
+        "relax mode must be 0 (looseGSRB), 1 (levelGSRB), 2 (overlapGSRB), 3 (levelGSRBLazy), 4 (levelJacobi) or "
+        "5 (levelMultiColor)");
+  // VCAMRPoissonOp2 aborts in looseGSRB, overlapGSRB, levelGSRBLazy and levelMultiColor
+  // (VCAMRPoissonOp2.cpp:628-640, 748-760)
+  if (mode != 1 && mode != 4 && !s->vc.empty())
+    throw std::runtime_error("only relax modes 1 and 4 are implemented for variable coefficients");
   for (auto& op : s->amg.m_op) op->m_relaxMode = mode;
   for (int d = 1; d < s->H.mgDepth(); d++) s->mgop(d)->m_relaxMode = mode;
   if (s->mixed) {

@@ -452,6 +452,67 @@ void gsrb(const DeviceLayout& L, T* phi, const T* rhs, int color, double dx, dou
     LAUNCH_PU(KC_GSRB, 12.0 * sizeof(T) / 8.0 * L.valid_cells, 0.5 * L.valid_cells, (k_gsrb<T, false>), L.ntiles, L.d_boxes, L.d_tiles, geom(L), phi, rhs, color, T(dxinv), T(lambda), T(alpha), T(beta));
 }
 
+// One GSRB colour over a region of every box, each box reading its neighbours'
+// values from snap (per-box FABs hold copies made at the last ghost refresh, so
+// on merged storage the cells outside the box come from a snapshot; on per-box
+// storage snap is phi itself, whose ghost cells a pass never writes).  region 0:
+// the box (looseGSRB, AMRPoissonOp.cpp:1333-1413); 1: the box grown by -1; 2: the
+// one-cell layer of the box at side `side` of direction `dir`, shrunk by one at
+// both ends in the directions below dir (overlapGSRB, :1416-1521).  Tiles cover
+// each box.
+template <class T, bool LAP>
+__global__ void __launch_bounds__(256) k_gsrb_region(const DBox* __restrict__ boxes, const Tile* __restrict__ tiles,
+                                                     TileGeom g, T* phi, const T* snap, const T* __restrict__ rhs,
+                                                     int color, int region, int dir, int side, T dxinv, T lambda,
+                                                     T alpha, T beta) {
+  DBox b;
+  int i, j, k;
+  if (!tile_cell(boxes, tiles, g, b, i, j, k)) return;
+  if ((i + j + k + color) & 1) return;
+  const int x[3] = {i, j, k};
+  if (region == 1) {
+    for (int d = 0; d < 3; d++)
+      if (x[d] <= b.lo[d] || x[d] >= b.hi[d]) return;
+  } else if (region == 2) {
+    if (x[dir] != (side ? b.hi[dir] : b.lo[dir])) return;
+    for (int e = 0; e < dir; e++)
+      if (x[e] <= b.lo[e] || x[e] >= b.hi[e]) return;
+  }
+  const long long c = cidx(b, i, j, k);
+  const long long sy = b.sy, sz = b.sz;
+  const T xp = i < b.hi[0] ? phi[c + 1] : snap[c + 1];
+  const T xm = i > b.lo[0] ? phi[c - 1] : snap[c - 1];
+  const T yp = j < b.hi[1] ? phi[c + sy] : snap[c + sy];
+  const T ym = j > b.lo[1] ? phi[c - sy] : snap[c - sy];
+  const T zp = k < b.hi[2] ? phi[c + sz] : snap[c + sz];
+  const T zm = k > b.lo[2] ? phi[c - sz] : snap[c - sz];
+  const T s6 = xp + xm + yp + ym + zp + zm;
+  if (LAP) {
+    const T lphi = s6 * dxinv;
+    phi[c] = lambda * (rhs[c] - lphi);
+  } else {
+    const T p = phi[c];
+    const T lphi = (s6 - T(6.0) * p) * dxinv;
+    const T helmop = alpha * p + beta * lphi;
+    phi[c] = p + lambda * (helmop - rhs[c]);
+  }
+}
+template <class T>
+void gsrb_region(const DeviceLayout& L, const Tile* tiles, int ntiles, T* phi, const T* snap, const T* rhs, int color,
+                 int region, int dir, int side, bool lapOnly, double dx, double alpha, double beta) {
+  double dxinv = 1.0 / (dx * dx);
+  double sum_b = 0.0;
+  for (int d = 0; d < 3; d++) sum_b = sum_b + 2.0 * dxinv;
+  const bool lap = lapOnly || (alpha == 0.0 && beta == 1.0);
+  double lambda = lap ? -1.0 / sum_b : -1.0 / (alpha - beta * sum_b);
+  if (lap)
+    LAUNCH_PU(KC_GSRB, 0.0, 0.0, (k_gsrb_region<T, true>), ntiles, L.d_boxes, tiles, geom(L), phi, snap, rhs, color,
+              region, dir, side, T(dxinv), T(lambda), T(alpha), T(beta));
+  else
+    LAUNCH_PU(KC_GSRB, 0.0, 0.0, (k_gsrb_region<T, false>), ntiles, L.d_boxes, tiles, geom(L), phi, snap, rhs, color,
+              region, dir, side, T(dxinv), T(lambda), T(alpha), T(beta));
+}
+
 // Eight-colour point updates: colour (cx, cy, cz) is the set of cells whose
 // index parities are (i & 1, j & 1, k & 1) -- in two's complement i & 1 is the
 // parity of |i|, the test the reference's coloured boxes use.
@@ -1504,6 +1565,8 @@ void reflux(Context* ctx, const RefluxCell* cells, int ncells, const RefluxCoars
   template void fill_ghosts<T>(const DeviceLayout&, const GhostPlan&, T*, int, int, double, double);           \
   template void gsrb<T>(const DeviceLayout&, T*, const T*, int, double, double, double);                       \
   template void gsmc<T>(const DeviceLayout&, T*, const T*, const int*, double, double, double);                \
+  template void gsrb_region<T>(const DeviceLayout&, const Tile*, int, T*, const T*, const T*, int, int, int, int, bool, \
+                               double, double, double);                                                      \
   template void gsrb_lazy<T>(const DeviceLayout&, T*, const T*, const T*, const int*, double, double, double); \
   template void gsrb_fused<T>(const DeviceLayout&, const FaceRules&, T*, const T*, const T*, double, double,      \
                               double, int, int, bool);                                                        \

@@ -46,6 +46,13 @@ void fill_ghosts(const DeviceLayout& L, const GhostPlan& P, T* phi, int doCF, in
                  double dxCrse);
 template <class T>
 void gsrb(const DeviceLayout& L, T* phi, const T* rhs, int color, double dx, double alpha, double beta);
+// One GSRB colour over a region of every box (tiles over each box), neighbours
+// outside the box read from snap: region 0 the box, 1 the box grown by -1, 2 the
+// face layer (dir, side) shrunk in the directions below dir.  lapOnly forces
+// GSRBLAPLACIAN; otherwise the levelGSRB selection by alpha and beta.
+template <class T>
+void gsrb_region(const DeviceLayout& L, const Tile* tiles, int ntiles, T* phi, const T* snap, const T* rhs, int color,
+                 int region, int dir, int side, bool lapOnly, double dx, double alpha, double beta);
 // Eight-colour point updates on the cells with index parities color[0..2]:
 // GSMCAMRPOP (levelMultiColor) and GSRBLAZY (levelGSRBLazy, lphi = L phi)
 template <class T>

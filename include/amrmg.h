@@ -143,12 +143,12 @@ int amrmg_solver_tune_relax(amrmg_solver* s, double* table, int cap, int* nrec);
    relaxation on every level. */
 int amrmg_solver_set_relax_shape(amrmg_solver* s, int shape, int kc);
 /* AMRPoissonOp::s_relaxMode (AMRPoissonOp.cpp:31, 600-632) for every operator of the
-   solver: 1 levelGSRB (default), 3 levelGSRBLazy (AMRPoissonOp.cpp:1524-1578),
-   4 levelJacobi (AMRPoissonOp.cpp:1579-1596, VCAMRPoissonOp2.cpp:762-786), 5
-   levelMultiColor (AMRPoissonOp.cpp:1284-1331).  Modes 3 and 5 are refused with
-   variable coefficients, as VCAMRPoissonOp2 aborts in them; modes 0 (looseGSRB) and
-   2 (overlapGSRB) are refused.  With any mode but 1 the bottom solve is the
-   host-driven BiCGStab. */
+   solver: 0 looseGSRB (AMRPoissonOp.cpp:1333-1413), 1 levelGSRB (default), 2
+   overlapGSRB (:1416-1521, as run without MPI), 3 levelGSRBLazy (:1524-1578), 4
+   levelJacobi (:1579-1596, VCAMRPoissonOp2.cpp:762-786), 5 levelMultiColor
+   (:1284-1331).  With variable coefficients only 1 and 4 are accepted, as
+   VCAMRPoissonOp2 aborts in the others.  With any mode but 1 the bottom solve is
+   the host-driven BiCGStab. */
 int amrmg_solver_set_relax_mode(amrmg_solver* s, int mode);
 /* Tile shape of the one-launch variable-coefficient relaxation on every multigrid
    depth: -1 automatic, 0 64x8, 1 32x8, 2 64x4 (256 threads), 3 64x16 (512

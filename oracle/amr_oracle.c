@@ -1387,6 +1387,58 @@ static void op_levelJacobi(const Op* op, LD* phi, const LD* rhs) {
   }
   ld_free(&resid);
 }
+/* one GSRB pass of box b over region r: GSRBLAPLACIAN when lap, else the
+   alpha/beta selection of levelGSRB (AMRPoissonOp.cpp:1260) */
+static void box_gsrb(const Op* op, LD* phi, const LD* rhs, int b, const Box* r, int lap, int pass) {
+  if (r->lo[0] > r->hi[0] || r->lo[1] > r->hi[1] || r->lo[2] > r->hi[2]) return;
+  if (lap || (op->alpha == 0.0 && op->beta == 1.0))
+    k_gsrb_lap(&phi->f[b], &rhs->f[b], r, op->dx, pass);
+  else
+    k_gsrb_helm(&phi->f[b], &rhs->f[b], r, op->dx, op->alpha, op->beta, pass);
+}
+/* looseGSRB (AMRPoissonOp.cpp:1333-1413): one ghost refresh (homogeneous CF
+   interpolation, exchange, homogeneous BC), then both passes box by box with
+   those ghost values (no refresh between the colours) */
+static void op_looseGSRB(const Op* op, LD* phi, const LD* rhs) {
+  g_cell_relax += lay_cells(&op->L);
+  op_homog_cf(op, phi);
+  op_exchange(op, phi);
+  for (int b = 0; b < op->L.n; b++) {
+    apply_bc(&phi->f[b], &op->L.b[b], &op->domain, op->dx, 1, &op->bc);
+    box_gsrb(op, phi, rhs, b, &op->L.b[b], 0, 0);
+    box_gsrb(op, phi, rhs, b, &op->L.b[b], 0, 1);
+  }
+}
+/* overlapGSRB (AMRPoissonOp.cpp:1416-1521) without MPI, where exchangeBegin has
+   already copied every ghost (LevelDataI.H:533-538): looseGSRB unless the level
+   is one box or (the reference's test) alpha != 1 or beta != 0, then exchange,
+   homogeneous CF interpolation, per box the homogeneous BC and GSRBLAPLACIAN
+   (both passes) on the box grown by -1, then per box the six one-cell face
+   layers (x lo, x hi, y lo, ..., each shrunk by one in the lower directions),
+   both passes each, with GSRBLAPLACIAN whatever alpha and beta */
+static void op_overlapGSRB(const Op* op, LD* phi, const LD* rhs) {
+  if (op->L.n == 1 || op->alpha != 1 || op->beta != 0) op_looseGSRB(op, phi, rhs);
+  g_cell_relax += lay_cells(&op->L);
+  op_exchange(op, phi);
+  op_homog_cf(op, phi);
+  for (int b = 0; b < op->L.n; b++) {
+    apply_bc(&phi->f[b], &op->L.b[b], &op->domain, op->dx, 1, &op->bc);
+    Box r = op->L.b[b];
+    for (int d = 0; d < 3; d++) { r.lo[d]++; r.hi[d]--; }
+    box_gsrb(op, phi, rhs, b, &r, 1, 0);
+    box_gsrb(op, phi, rhs, b, &r, 1, 1);
+  }
+  for (int b = 0; b < op->L.n; b++)
+    for (int dir = 0; dir < 3; dir++)
+      for (int side = 0; side < 2; side++) {
+        Box r = op->L.b[b];
+        if (side == 0) r.hi[dir] = r.lo[dir];
+        else r.lo[dir] = r.hi[dir];
+        for (int e = 0; e < dir; e++) { r.lo[e]++; r.hi[e]--; }
+        box_gsrb(op, phi, rhs, b, &r, 1, 0);
+        box_gsrb(op, phi, rhs, b, &r, 1, 1);
+      }
+}
 /* levelGSRBLazy (AMRPoissonOp.cpp:1524-1578): eight colours, the parities of
    (i, j, k) in the order nextColorLoc steps them (x fastest); for each, the whole
    homogeneous operator (applyOp, with its CF interpolation, exchange and BC) and
@@ -1422,11 +1474,13 @@ static void op_levelMultiColor(const Op* op, LD* phi, const LD* rhs) {
     }
   }
 }
-/* relax (AMRPoissonOp.cpp:600-632): s_relaxMode 1 (levelGSRB), 3 (levelGSRBLazy),
-   4 (levelJacobi) or 5 (levelMultiColor) */
+/* relax (AMRPoissonOp.cpp:600-632): s_relaxMode 0 (looseGSRB), 1 (levelGSRB),
+   2 (overlapGSRB), 3 (levelGSRBLazy), 4 (levelJacobi) or 5 (levelMultiColor) */
 static void op_relax(const Op* op, LD* e, const LD* r, int n) {
   for (int i = 0; i < n; i++) {
-    if (op->relaxMode == 4) op_levelJacobi(op, e, r);
+    if (op->relaxMode == 0) op_looseGSRB(op, e, r);
+    else if (op->relaxMode == 2) op_overlapGSRB(op, e, r);
+    else if (op->relaxMode == 4) op_levelJacobi(op, e, r);
     else if (op->relaxMode == 3) op_levelGSRBLazy(op, e, r);
     else if (op->relaxMode == 5) op_levelMultiColor(op, e, r);
     else op_levelGSRB(op, e, r);
@@ -1789,8 +1843,8 @@ static const Op* mg_op(const OHier* h, int d);
 int oracle_vc_set(OHier* h, const double* a, const double* b0, const double* b1, const double* b2) {
   const double* bs[3] = {b0, b1, b2};
   long ao = 0, bo[3] = {0, 0, 0};
-  if (h->op[0].relaxMode == 3 || h->op[0].relaxMode == 5) {
-    set_err("relax modes 3 and 5 are not implemented for variable coefficients");
+  if (h->op[0].relaxMode != 1 && h->op[0].relaxMode != 4) {
+    set_err("only relax modes 1 and 4 are implemented for variable coefficients");
     return -1;
   }
   /* every AMR level: its own coefficients (VCAMRPoissonOp2Factory::AMRnewOp, :1074-1171) */
@@ -1825,13 +1879,15 @@ int oracle_vc_set(OHier* h, const double* a, const double* b0, const double* b1,
 }
 /* AMRPoissonOp::s_relaxMode (AMRPoissonOp.cpp:31, 600-632) for every operator: 1 or 4 */
 int oracle_set_relax_mode(OHier* h, int mode) {
-  if (mode != 1 && mode != 3 && mode != 4 && mode != 5) {
-    set_err("relax mode must be 1 (levelGSRB), 3 (levelGSRBLazy), 4 (levelJacobi) or 5 (levelMultiColor)");
+  if (mode < 0 || mode > 5) {
+    set_err("relax mode must be 0 (looseGSRB), 1 (levelGSRB), 2 (overlapGSRB), 3 (levelGSRBLazy), "
+            "4 (levelJacobi) or 5 (levelMultiColor)");
     return -1;
   }
-  /* VCAMRPoissonOp2 aborts in levelGSRBLazy and levelMultiColor (VCAMRPoissonOp2.cpp:628-633, 755-760) */
-  if ((mode == 3 || mode == 5) && h->op[0].vc) {
-    set_err("relax modes 3 and 5 are not implemented for variable coefficients");
+  /* VCAMRPoissonOp2 aborts in looseGSRB, overlapGSRB, levelGSRBLazy and levelMultiColor
+     (VCAMRPoissonOp2.cpp:628-640, 748-760) */
+  if (mode != 1 && mode != 4 && h->op[0].vc) {
+    set_err("only relax modes 1 and 4 are implemented for variable coefficients");
     return -1;
   }
   for (int l = 0; l < h->nlev; l++) h->op[l].relaxMode = mode;

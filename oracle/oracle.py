@@ -213,8 +213,8 @@ class Oracle:
         return float(out[0])
 
     def set_relax_mode(self, mode):
-        """AMRPoissonOp::s_relaxMode: 1 levelGSRB, 3 levelGSRBLazy, 4 levelJacobi,
-        5 levelMultiColor."""
+        """AMRPoissonOp::s_relaxMode: 0 looseGSRB, 1 levelGSRB, 2 overlapGSRB,
+        3 levelGSRBLazy, 4 levelJacobi, 5 levelMultiColor."""
         self._chk(lib().oracle_set_relax_mode(self.h, int(mode)))
 
     def set_alpha_beta(self, alpha, beta):

@@ -209,15 +209,22 @@ def test_fcycle_solve_bitwise(gpu_ctx, numMG, case):
 
 
 @pytest.mark.parametrize("mode,case", [(4, "helm24"), (4, "amr32"), (3, "helm24"), (3, "lap32_negidx"), (3, "amr32"),
-                                       (5, "helm24"), (5, "ragged40"), (5, "amr32")])
+                                       (5, "helm24"), (5, "ragged40"), (5, "amr32"), (0, "helm24"),
+                                       (0, "lap32_negidx"), (0, "amr32"), (0, "lshape"), (2, "lap32"), (2, "ragged40"),
+                                       (2, "amr32"), (2, "lshape")])
 def test_relax_modes_bitwise(gpu_ctx, mode, case):
-    """s_relaxMode 3 (levelGSRBLazy), 4 (levelJacobi) and 5 (levelMultiColor)
-    everywhere: level relaxation at every depth and whole solves (host-driven
-    bottom) bitwise the oracle's (GSMCAMRPOP relaxes with the Laplacian alone; the
-    Helmholtz solve still converges through the coarse corrections)."""
+    """s_relaxMode 0 (looseGSRB), 2 (overlapGSRB), 3 (levelGSRBLazy), 4
+    (levelJacobi) and 5 (levelMultiColor) everywhere: level relaxation at every
+    depth and whole solves (host-driven bottom) bitwise the oracle's (GSMCAMRPOP and
+    overlapGSRB's face layers relax with the Laplacian alone; the Helmholtz solves
+    still converge through the coarse corrections).  lshape has a level stored box
+    by box; the others are merged."""
     print(f"relax mode {mode} {case}", flush=True)
     if case == "amr32":
-        h = small_amr(alpha=0.0 if mode == 5 else 0.3, beta=1.0 if mode == 5 else 0.9)
+        h = small_amr(alpha=0.0 if mode in (2, 5) else 0.3, beta=1.0 if mode in (2, 5) else 0.9)
+        rhs = H.bump_rhs(h, 1031)
+    elif case == "lshape":
+        h = lshape_amr()
         rhs = H.bump_rhs(h, 1031)
     else:
         h = small_single(**SINGLE_CASES[case])
@@ -244,20 +251,21 @@ def test_relax_modes_bitwise(gpu_ctx, mode, case):
 
 
 def test_relax_mode_errors(gpu_ctx):
-    """Modes 0 and 2 are refused; 3 and 5 are refused with variable coefficients,
-    and variable coefficients are refused while 3 or 5 is set."""
+    """Modes outside 0..5 are refused; with variable coefficients only 1 and 4 are
+    accepted, and variable coefficients are refused while another mode is set."""
     h = small_single(n=16, box=8)
     g = amrmg.AMRMultiGrid(gpu_ctx, h)
-    for mode in (0, 2, 6):
+    for mode in (-1, 6):
         with pytest.raises(RuntimeError):
             g.set_relax_mode(mode)
     a, b = H.vc_coefficients(h, seed=3)
-    g.set_relax_mode(5)
-    with pytest.raises(RuntimeError):
-        g.set_vc_coefs(a, b)
+    for mode in (0, 2, 3, 5):
+        g.set_relax_mode(mode)
+        with pytest.raises(RuntimeError):
+            g.set_vc_coefs(a, b)
     g.set_relax_mode(1)
     g.set_vc_coefs(a, b)
-    for mode in (3, 5):
+    for mode in (0, 2, 3, 5):
         with pytest.raises(RuntimeError):
             g.set_relax_mode(mode)
     g.set_relax_mode(4)

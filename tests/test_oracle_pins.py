@@ -555,3 +555,113 @@ def test_quadcfinterp_never_reads_covered_coarse_cells(case, lev):
     # the poison is not inert: the same coarse cells are read when uncovered
     c = o.quad_cf(lev, pg, np.where(~covered, 1e300, coarse))
     assert np.abs(c[cf]).max() > 1e200
+
+
+def _box_sweeps(h, n, x0, b, alpha, beta, steps):
+    """Brute-force GSRB sweeps with per-box ghost copies on one Dirichlet level: a
+    step (region mask, colour, lap) updates the cells of that colour in the
+    region; a neighbour inside the cell's own box is read live, one in another
+    box from the snapshot taken at the last refresh, and a domain ghost is the
+    homogeneous Dirichlet -phi(cell) of the snapshot.  steps holds the strings
+    "refresh" and (mask, colour, lap) triples."""
+    dx = 1.0 / n
+    idx = np.arange(n)
+    I, J, K = np.meshgrid(idx, idx, idx, indexing="ij")
+    owner = np.zeros((n, n, n), dtype=int)
+    for q, (lo, hi) in enumerate(h.levels[0]):
+        owner[lo[0]:hi[0] + 1, lo[1]:hi[1] + 1, lo[2]:hi[2] + 1] = q
+    x = x0.copy()
+    snap = x.copy()
+    for st in steps:
+        if isinstance(st, str):
+            snap = x.copy()
+            continue
+        mask, colour, lap = st
+        s6 = np.zeros_like(x)
+        for d in range(3):
+            for sgn in (1, -1):
+                sh = np.roll(x, -sgn, axis=d)
+                ssh = np.roll(snap, -sgn, axis=d)
+                osh = np.roll(owner, -sgn, axis=d)
+                coord = (I, J, K)[d] + sgn
+                inside = (coord >= 0) & (coord < n)
+                s6 += np.where(inside, np.where(osh == owner, sh, ssh), -snap)
+        sel = mask & (((I + J + K + colour) & 1) == 0)
+        if lap or (alpha == 0.0 and beta == 1.0):
+            new = -(dx * dx / 6.0) * (b - s6 / dx ** 2)
+        else:
+            lam = -1.0 / (alpha - 6.0 * beta / dx ** 2)
+            new = x + lam * (alpha * x + beta * (s6 - 6.0 * x) / dx ** 2 - b)
+        x = np.where(sel, new, x)
+    return x
+
+
+def _loose_steps(full):
+    return ["refresh", (full, 0, False), (full, 1, False)]
+
+
+def _overlap_steps(h, n, alpha, beta):
+    full = np.ones((n, n, n), dtype=bool)
+    steps = _loose_steps(full) if (len(h.levels[0]) == 1 or alpha != 1 or beta != 0) else []
+    steps.append("refresh")
+    inner = np.zeros((n, n, n), dtype=bool)
+    for (lo, hi) in h.levels[0]:
+        inner[lo[0] + 1:hi[0], lo[1] + 1:hi[1], lo[2] + 1:hi[2]] = True
+    steps += [(inner, 0, True), (inner, 1, True)]
+    for dir in range(3):
+        for side in range(2):
+            m = np.zeros((n, n, n), dtype=bool)
+            for (lo, hi) in h.levels[0]:
+                sl = [slice(lo[e], hi[e] + 1) for e in range(3)]
+                for e in range(dir):
+                    sl[e] = slice(lo[e] + 1, hi[e])
+                sl[dir] = slice(lo[dir], lo[dir] + 1) if side == 0 else slice(hi[dir], hi[dir] + 1)
+                m[tuple(sl)] = True
+            # every box's face layer at once: boxes read each other only from the snapshot
+            steps += [(m, 0, True), (m, 1, True)]
+    return steps
+
+
+@pytest.mark.parametrize("mode,alpha,beta", [(0, 0.0, 1.0), (0, 0.7, 1.3), (2, 0.0, 1.0), (2, 0.7, 1.3)])
+def test_loose_and_overlap_relax_match_box_sweeps(mode, alpha, beta):
+    """looseGSRB (mode 0) and overlapGSRB (mode 2): one relaxation equals a
+    brute-force sweep in which each box sees its neighbours and its Dirichlet
+    ghosts as they were at the last ghost refresh (per-box FAB copies), with
+    overlapGSRB's interior / face-layer order and its Laplacian-only passes."""
+    n = 16
+    h = small_single(n=n, box=8, alpha=alpha, beta=beta)
+    o = Oracle(h)
+    o.set_relax_mode(mode)
+    rng = np.random.default_rng(50 + mode)
+    phi = rng.standard_normal(h.cells(0))
+    rhs = rng.standard_normal(h.cells(0))
+    out = h.to_global(o.level_relax(0, phi, rhs, 1), 0)
+    x0, b = h.to_global(phi, 0), h.to_global(rhs, 0)
+    full = np.ones((n, n, n), dtype=bool)
+    steps = _loose_steps(full) if mode == 0 else _overlap_steps(h, n, alpha, beta)
+    want = _box_sweeps(h, n, x0, b, alpha, beta, steps)
+    assert np.abs(out - want).max() < 1e-12 * np.abs(want).max()
+    # a refresh between the colours (levelGSRB) is a different sweep
+    ref1 = _box_sweeps(h, n, x0, b, alpha, beta, ["refresh", (full, 0, False), "refresh", (full, 1, False)])
+    assert np.abs(out - ref1).max() > 1e-6 * np.abs(ref1).max()
+
+
+@pytest.mark.parametrize("mode", [0, 2])
+def test_loose_and_overlap_relax_fixed_point_and_solve(mode):
+    """A manufactured discrete solution is a fixed point, and the multigrid solve
+    equals a sparse direct solve."""
+    n = 16
+    h = small_single(n=n, box=8)
+    o = Oracle(h)
+    o.set_relax_mode(mode)
+    rng = np.random.default_rng(60 + mode)
+    phi = rng.standard_normal(h.cells(0))
+    f = o.level_apply(0, phi, homog=True)
+    assert np.abs(o.level_relax(0, phi, f, 2) - phi).max() < 1e-11 * np.abs(phi).max()
+    rhs = H.sinusoid_rhs(h, 1005)
+    sp_, rp = H.solver_params(eps=1e-12)
+    sol, hist, info = o.solve(rhs, sp_, rp)
+    assert info["exit_status"] & 1
+    x = spla.spsolve(_assemble_dirichlet(n).tocsc(), h.to_global(rhs, 0).ravel(order="F")).reshape((n, n, n),
+                                                                                                  order="F")
+    assert np.abs(h.to_global(sol, 0) - x).max() / np.abs(x).max() < 1e-10
