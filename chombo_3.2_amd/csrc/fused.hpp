@@ -124,9 +124,12 @@ __device__ __forceinline__ void fused_column(const FusedArgs& a, int w, T* __res
     i = x0 + ((x0 + j + k + 1) & 1) + 2 * (q % (FX / 2));
   };
   // software pipeline, two steps deep: the old values of plane k+4 and the rhs of
-  // the cells of step k+2 are loaded into registers during step k, into one of
-  // two register sets used alternately (no register moves between the sets, so
-  // each step waits only for the loads issued two steps before).
+  // the red and the black cells of plane k+3 are loaded into registers during
+  // step k, into one of two register sets used alternately (no register moves
+  // between the sets, so each step waits only for the loads issued two steps
+  // before).  A plane's red and black rhs values share cache lines and are loaded
+  // together; the black ones are used a step after the red ones, so step k keeps
+  // them in `carry` for step k+1.
   // Every load is unconditional from an address clamped into the rectangle, and
   // its value is used as loaded: a value at a position outside the rectangle (or
   // of a cell that is not updated) is never selected into a result.  A select at
@@ -191,16 +194,21 @@ __device__ __forceinline__ void fused_column(const FusedArgs& a, int w, T* __res
       }
     }
   };
+  // black rhs values of the plane after the current one (see the pipeline above)
+  T carry[NBLK];
   // step k: plane k+2 into LDS, red on plane k+1, black on plane k, store plane k-1
   auto step = [&](Pre& f, int k) {
     put_phi(f, k + 2);
-    T rr[NRED], rb[NBLK];
+    T rr[NRED], rb[NBLK], rbn[NBLK];
 #pragma unroll
     for (int u = 0; u < NRED; u++) rr[u] = f.rr[u];
 #pragma unroll
-    for (int u = 0; u < NBLK; u++) rb[u] = f.rb[u];
+    for (int u = 0; u < NBLK; u++) {
+      rb[u] = carry[u];
+      rbn[u] = f.rb[u];
+    }
     lds_barrier();
-    fetch(f, k + 4, k + 3, k + 2);
+    fetch(f, k + 4, k + 3, k + 3);
     if (k - 1 >= k0) store(k - 1);
     const int r = k + 1;
     if (r >= a.lo[2] && r <= a.hi[2]) {
@@ -230,13 +238,17 @@ __device__ __forceinline__ void fused_column(const FusedArgs& a, int w, T* __res
         }
       }
     }
+#pragma unroll
+    for (int u = 0; u < NBLK; u++) carry[u] = rbn[u];
   };
-  fetch(A, k0 - 2, k0 - 3, k0 - 4);
+  fetch(A, k0 - 2, k0 - 3, k0 - 3);
   put_phi(A, k0 - 2);
-  fetch(A, k0 - 1, k0 - 3, k0 - 4);
+  fetch(A, k0 - 1, k0 - 3, k0 - 3);
   put_phi(A, k0 - 1);
-  fetch(A, k0, k0 - 1, k0 - 2);
-  fetch(B, k0 + 1, k0, k0 - 1);
+  fetch(A, k0, k0 - 1, k0 - 1);
+  fetch(B, k0 + 1, k0, k0);
+#pragma unroll
+  for (int u = 0; u < NBLK; u++) carry[u] = T(0.0);  // black rhs of plane k0-2: never used
   for (int k = k0 - 2; k < k1; k += 2) {
     step(A, k);
     if (k + 1 < k1) step(B, k + 1);
