@@ -490,9 +490,12 @@ __global__ void __launch_bounds__(NT) k_vc_gsrb_fused(ops::FusedArgs a, VCFused<
       if (q < FX * FY && i <= a.hi[0] && j <= a.hi[1]) dst[gidx(i, j, p)] = val[u];
     }
   };
-  // step k consumes the set f (plane k+2, red cells of plane k+1, black cells of
-  // plane k) and then refills it for step k+2, so the loads have step k+1 to land
-  // and the operands are held in one place (no copies between sets)
+  // step k consumes the set f (plane k+2, red and black cells of plane k+1) and
+  // then refills it for step k+2, so the loads have step k+1 to land.  A plane's
+  // red and black operands share cache lines and are loaded together; the black
+  // ones are used a step after the red ones, so step k keeps them in `carry` for
+  // step k+1 and relaxes its own black cells (plane k) from the previous carry.
+  VCCell<T> carry[NBLK];
   auto step = [&](Pre& f, int k) {
     put_phi(f, k + 2);
     ops::lds_barrier();
@@ -520,19 +523,23 @@ __global__ void __launch_bounds__(NT) k_vc_gsrb_fused(ops::FusedArgs a, VCFused<
         black_ij(q, k, i, j);
         if (q < NBC && i <= a.hi[0] && j <= a.hi[1]) {
           const int c = lidx(i, j);
-          P[slot(k)][c] = vc_fused_new<T, LW>(a, P[slot(k - 1)], P[slot(k)], P[slot(k + 1)], c, i, j, k, f.rb[u],
+          P[slot(k)][c] = vc_fused_new<T, LW>(a, P[slot(k - 1)], P[slot(k)], P[slot(k + 1)], c, i, j, k, carry[u],
                                               dxinv, scale, alpha, beta, c1, c2);
         }
       }
     }
-    fetch(f, k + 4, k + 3, k + 2);
+#pragma unroll
+    for (int u = 0; u < NBLK; u++) carry[u] = f.rb[u];
+    fetch(f, k + 4, k + 3, k + 3);
   };
-  fetch(A, k0 - 2, k0 - 3, k0 - 4);
+  fetch(A, k0 - 2, k0 - 3, k0 - 3);
   put_phi(A, k0 - 2);
-  fetch(A, k0 - 1, k0 - 3, k0 - 4);
+  fetch(A, k0 - 1, k0 - 3, k0 - 3);
   put_phi(A, k0 - 1);
-  fetch(A, k0, k0 - 1, k0 - 2);
-  fetch(B, k0 + 1, k0, k0 - 1);
+  fetch(A, k0, k0 - 1, k0 - 1);
+  fetch(B, k0 + 1, k0, k0);
+#pragma unroll
+  for (int u = 0; u < NBLK; u++) carry[u] = VCCell<T>{};  // black operands of plane k0-2: never used
   for (int k = k0 - 2; k < k1; k += 2) {
     step(A, k);
     if (k + 1 < k1) step(B, k + 1);
