@@ -45,6 +45,7 @@ def lib():
         L.amrmg_solver_set_vc_coefs.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
         L.amrmg_level_norm.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _dp]
         L.amrmg_solver_set_relax_mode.argtypes = [C.c_void_p, C.c_int]
+        L.amrmg_solver_set_vc_average.argtypes = [C.c_void_p, C.c_int]
         L.amrmg_heat_create.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]
         L.amrmg_heat_set_source.argtypes = [C.c_void_p, _dp]
         L.amrmg_heat_step.argtypes = [C.c_void_p, C.c_double, C.c_int, _ip]
@@ -241,6 +242,11 @@ class AMRMultiGrid:
     def set_alpha_beta(self, alpha, beta):
         """AMRPoissonOp::setAlphaAndBeta on every operator of the solver."""
         _chk(lib().amrmg_solver_set_alpha_beta(self.p, float(alpha), float(beta)))
+
+    def set_vc_average(self, harmonic):
+        """Coefficient coarsening of the next set_vc_coefs: arithmetic (False, the
+        default) or harmonic (VCAMRPoissonOp2Factory's m_coefficient_average_type)."""
+        _chk(lib().amrmg_solver_set_vc_average(self.p, 1 if harmonic else 0))
 
     def set_relax_mode(self, mode):
         """AMRPoissonOp::s_relaxMode for every operator: 0 looseGSRB, 1 levelGSRB,

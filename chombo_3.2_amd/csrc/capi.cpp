@@ -77,6 +77,7 @@ struct amrmg_solver {
   // variable coefficients per multigrid depth (empty: constant coefficients)
   std::vector<std::shared_ptr<VCCoefs<double>>> vc;
   std::vector<std::shared_ptr<VCCoefs<float>>> vc32;
+  int vcHarmonic = 0;  // VCAMRPoissonOp2Factory's coefficient average type: 0 arithmetic, 1 harmonic
   // Per-box storage for the FAB-level entry points (amrmg_kernel_gsrb,
   // amrmg_homog_cf, amrmg_quad_cf), whose ghost cells are caller inputs like the
   // ghost cells of the reference's FArrayBoxes; built on first use.
@@ -323,7 +324,7 @@ int amrmg_solver_set_vc_coefs(amrmg_solver* s, const double* a, const double* b0
       }
     }
     // multigrid depths: level 0's coefficients averaged with ratio 2^d (MGnewOp)
-    for (int d = 1; d < nd; d++) vc::average_coefs<double>(*v[0], *v[nl + d - 1], 1 << d);
+    for (int d = 1; d < nd; d++) vc::average_coefs<double>(*v[0], *v[nl + d - 1], 1 << d, s->vcHarmonic);
     for (size_t q = 0; q < ops.size(); q++) ops[q]->setVC(v[q]);
     s->vc = v;
     s->vc32.clear();
@@ -620,6 +621,12 @@ int amrmg_solver_set_relax_mode(amrmg_solver* s, int mode) {
   }
   s->amg.m_mg0->setBottomSolver(s->bottom.get());
   if (s->mixed) s->mixed->rebindBottom();
+  API_END
+}
+int amrmg_solver_set_vc_average(amrmg_solver* s, int type) {
+  API_BEGIN
+  if (type != 0 && type != 1) throw std::runtime_error("coefficient average type must be 0 (arithmetic) or 1 (harmonic)");
+  s->vcHarmonic = type;
   API_END
 }
 int amrmg_solver_set_vc_relax_shape(amrmg_solver* s, int shape) {

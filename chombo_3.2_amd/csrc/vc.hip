@@ -279,7 +279,7 @@ __global__ void __launch_bounds__(256) k_vc_precond(const DBox* __restrict__ box
 template <class T>
 __global__ void __launch_bounds__(256) k_vc_avg_cells(const DBox* __restrict__ cboxes, const Tile* __restrict__ ctiles,
                                                       Geom g, const DBox* __restrict__ fboxes, T* __restrict__ coarse,
-                                                      const T* __restrict__ fine, int r, T refScale) {
+                                                      const T* __restrict__ fine, int r, T refScale, int harmonic) {
   DBox cb;
   int box, I, J, K;
   if (!tile_cell(cboxes, ctiles, g, box, cb, I, J, K)) return;
@@ -289,7 +289,8 @@ __global__ void __launch_bounds__(256) k_vc_avg_cells(const DBox* __restrict__ c
   for (int kk = 0; kk < r; kk++)
     for (int jj = 0; jj < r; jj++)
       for (int ii = 0; ii < r; ii++) {
-        const T f = fine[cidx(fb, I * r + ii, J * r + jj, K * r + kk)];
+        const T v = fine[cidx(fb, I * r + ii, J * r + jj, K * r + kk)];
+        const T f = harmonic ? T(1.0) / v : v;
         if (first) {
           sum = f;
           first = false;
@@ -297,35 +298,41 @@ __global__ void __launch_bounds__(256) k_vc_avg_cells(const DBox* __restrict__ c
           sum = sum + f;
         }
       }
-  coarse[cidx(cb, I, J, K)] = (r <= 4) ? refScale * sum : sum * refScale;
+  // AVERAGEHARMONIC (AverageF.ChF:165-314): the reciprocal of refScale times the sum of 1/fine
+  if (harmonic) coarse[cidx(cb, I, J, K)] = T(1.0) / ((r <= 4) ? refScale * sum : sum * refScale);
+  else coarse[cidx(cb, I, J, K)] = (r <= 4) ? refScale * sum : sum * refScale;
 }
 // CoarseAverageFace (AverageFaceF.ChF AVERAGEFACE): each coarse face sums its r^2 fine
 // faces (i fastest) from zero, times (1/r)^2.  A thread owns the low face of its
 // cell in direction d, and the high face too when its cell is the last of the box.
+// harmonic: AVERAGEFACEHARMONIC (AverageFaceF.ChF:84-128), 1 / (ref_scale * sum of 1/fine).
 template <class T>
 __device__ __forceinline__ T face_avg(const T* __restrict__ fine, const DBox& ff, int d, int I, int J, int K, int r,
-                                      T ref_scale) {
+                                      T ref_scale, int harmonic) {
   const int n0 = d == 0 ? 1 : r, n1 = d == 1 ? 1 : r, n2 = d == 2 ? 1 : r;
   T crseSum = T(0.0);
   for (int kk = 0; kk < n2; kk++)
     for (int jj = 0; jj < n1; jj++)
-      for (int ii = 0; ii < n0; ii++) crseSum = crseSum + fine[fidx(ff, I * r + ii, J * r + jj, K * r + kk)];
-  return ref_scale * crseSum;
+      for (int ii = 0; ii < n0; ii++) {
+        const T v = fine[fidx(ff, I * r + ii, J * r + jj, K * r + kk)];
+        crseSum = crseSum + (harmonic ? T(1.0) / v : v);
+      }
+  return harmonic ? T(1.0) / (ref_scale * crseSum) : ref_scale * crseSum;
 }
 template <class T>
 __global__ void __launch_bounds__(256) k_vc_avg_faces(const DBox* __restrict__ cboxes, const Tile* __restrict__ ctiles,
                                                       Geom g, const DBox* __restrict__ cf, const DBox* __restrict__ ff,
                                                       T* __restrict__ coarse, const T* __restrict__ fine, int d, int r,
-                                                      T ref_scale) {
+                                                      T ref_scale, int harmonic) {
   DBox cb;
   int box, I, J, K;
   if (!tile_cell(cboxes, ctiles, g, box, cb, I, J, K)) return;
   const DBox c = cf[box], f = ff[box];
-  coarse[fidx(c, I, J, K)] = face_avg(fine, f, d, I, J, K, r, ref_scale);
+  coarse[fidx(c, I, J, K)] = face_avg(fine, f, d, I, J, K, r, ref_scale, harmonic);
   const int idx[3] = {I, J, K};
   if (idx[d] == cb.hi[d]) {
     const int I1 = I + (d == 0), J1 = J + (d == 1), K1 = K + (d == 2);
-    coarse[fidx(c, I1, J1, K1)] = face_avg(fine, f, d, I1, J1, K1, r, ref_scale);
+    coarse[fidx(c, I1, J1, K1)] = face_avg(fine, f, d, I1, J1, K1, r, ref_scale, harmonic);
   }
 }
 
@@ -702,18 +709,18 @@ void precond_init(const DeviceLayout& L, const VCCoefs<T>& C, T* phi, const T* r
   VC_LAUNCH(L, KC_BLAS, 0.0, 0.0, k_vc_precond<T>, L.ntiles, L.d_boxes, L.d_tiles, geom(L), C.lambda.d, phi, rhs);
 }
 template <class T>
-void average_coefs(const VCCoefs<T>& F, VCCoefs<T>& C, int r) {
+void average_coefs(const VCCoefs<T>& F, VCCoefs<T>& C, int r, int harmonic) {
   const DeviceLayout& Lf = *F.lay;
   const DeviceLayout& Lc = *C.lay;
   if (r < 2 || (r & (r - 1))) throw std::runtime_error("vc::average_coefs: ratio must be a power of two");
   check_ratio(Lf, Lc, r, "vc::average_coefs");
   const double refScale = 1.0 / (double)(r * r * r);
   VC_LAUNCH(Lc, KC_AVERAGE, 0.0, 0.0, k_vc_avg_cells<T>, Lc.ntiles, Lc.d_boxes, Lc.d_tiles, geom(Lc), Lf.d_boxes,
-            C.a.d, F.a.d, r, T(refScale));
+            C.a.d, F.a.d, r, T(refScale), harmonic);
   const double ref_scale = (1.0 / r) * (1.0 / r);
   for (int d = 0; d < 3; d++)
     VC_LAUNCH(Lc, KC_AVERAGE, 0.0, 0.0, k_vc_avg_faces<T>, Lc.ntiles, Lc.d_boxes, Lc.d_tiles, geom(Lc), C.d_fbox[d],
-              F.d_fbox[d], C.b[d], F.b[d], d, r, T(ref_scale));
+              F.d_fbox[d], C.b[d], F.b[d], d, r, T(ref_scale), harmonic);
 }
 
 template <class T>
@@ -809,7 +816,7 @@ void narrow(const VCCoefs<double>& src, VCCoefs<float>& dst) {
                                 const T*, double, double, double);                                                  \
   template void lambda<T>(VCCoefs<T>&, double, double, double);                                                     \
   template void precond_init<T>(const DeviceLayout&, const VCCoefs<T>&, T*, const T*);                              \
-  template void average_coefs<T>(const VCCoefs<T>&, VCCoefs<T>&, int);                                             \
+  template void average_coefs<T>(const VCCoefs<T>&, VCCoefs<T>&, int, int);                                             \
   template void scale_by_a<T>(const DeviceLayout&, const VCCoefs<T>&, T*, bool);                                    \
   template void scale_by_lambda<T>(const DeviceLayout&, const VCCoefs<T>&, T*);                                     \
   template void reflux<T>(Context*, const RefluxCell*, int, const RefluxCoarse*, const RefluxFine*, const VCFaceIdx*, \

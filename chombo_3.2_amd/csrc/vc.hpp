@@ -79,7 +79,7 @@ void precond_init(const DeviceLayout& L, const VCCoefs<T>& C, T* phi, const T* r
 // MGnewOp's coefficient coarsening (VCAMRPoissonOp2.cpp:1024-1060): C = the level-0
 // coefficients F averaged with ratio r (CoarseAverage / CoarseAverageFace, arithmetic)
 template <class T>
-void average_coefs(const VCCoefs<T>& F, VCCoefs<T>& C, int r);
+void average_coefs(const VCCoefs<T>& F, VCCoefs<T>& C, int r, int harmonic = 0);
 // VCAMRPoissonOp2::reflux (VCAMRPoissonOp2.cpp:339-430): the refluxed residual with
 // fluxes -b * gradient (getFlux, :787-837), from the tables of ops::reflux and the
 // faces of their records

@@ -155,6 +155,11 @@ int amrmg_solver_set_relax_mode(amrmg_solver* s, int mode);
    threads); results are bitwise the same for every choice.  No counterpart in the
    reference (its GSRBHELMHOLTZVC runs per box and colour). */
 int amrmg_solver_set_vc_relax_shape(amrmg_solver* s, int shape);
+/* m_coefficient_average_type of VCAMRPoissonOp2Factory (VCAMRPoissonOp2.cpp:1048-1063)
+   for the multigrid coarsening of the next amrmg_solver_set_vc_coefs: 0
+   CoarseAverage::arithmetic (default), 1 harmonic (averageToCoarseHarmonic,
+   AverageF.ChF AVERAGEHARMONIC and AverageFaceF.ChF AVERAGEFACEHARMONIC). */
+int amrmg_solver_set_vc_average(amrmg_solver* s, int type);
 
 /* ---- batched single-level solves: many independent problems, each V-cycle stage
    of all of them in one launch (relaxation, restriction, prolongation, residual

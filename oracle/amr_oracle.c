@@ -1259,8 +1259,10 @@ static void k_vclambda(Fab* lam, const Fab* a, Fab* const* b, const Box* r, doub
 }
 /* CoarseAverage::averageToCoarse -> average (AverageF.ChF:9-162): the r^3 children
    summed with i fastest; refRatio 2 and 4 are unrolled sums multiplied by refScale,
-   other ratios a loop from zero times refScale */
-static void k_vcavg_cells(Fab* coarse, const Fab* fine, const Box* cb, int r) {
+   other ratios a loop from zero times refScale.  harmonic: averageToCoarseHarmonic ->
+   AVERAGEHARMONIC (AverageF.ChF:165-314), the same sums of 1/fine and the reciprocal
+   of refScale times the sum (0 + x == x, so both branches give 1/(refScale*sum)). */
+static void k_vcavg_cells(Fab* coarse, const Fab* fine, const Box* cb, int r, int harmonic) {
   const double refScale = 1.0/(double)(r*r*r);
   for (int K = cb->lo[2]; K <= cb->hi[2]; K++)
     for (int J = cb->lo[1]; J <= cb->hi[1]; J++)
@@ -1270,16 +1272,20 @@ static void k_vcavg_cells(Fab* coarse, const Fab* fine, const Box* cb, int r) {
         for (int kk = 0; kk < r; kk++)
           for (int jj = 0; jj < r; jj++)
             for (int ii = 0; ii < r; ii++) {
-              const double f = *P(fine, I*r+ii, J*r+jj, K*r+kk);
+              const double v = *P(fine, I*r+ii, J*r+jj, K*r+kk);
+              const double f = harmonic ? 1.0/v : v;
               if (first && r <= 4) { sum = f; first = 0; }
               else sum = sum + f;
             }
-        *P(coarse,I,J,K) = (r <= 4) ? refScale * sum : sum * refScale;
+        if (harmonic) *P(coarse,I,J,K) = (r <= 4) ? 1.0/(refScale * sum) : 1.0/(sum * refScale);
+        else *P(coarse,I,J,K) = (r <= 4) ? refScale * sum : sum * refScale;
       }
 }
 /* CoarseAverageFace::averageGridData -> AVERAGEFACE (AverageFaceF.ChF): each coarse
-   face sums the r^2 fine faces on it (i fastest) from zero, times (1/r)^2 */
-static void k_vcavg_faces(Fab* coarse, const Fab* fine, const Box* cfb, int dir, int r) {
+   face sums the r^2 fine faces on it (i fastest) from zero, times (1/r)^2.
+   harmonic: AVERAGEFACEHARMONIC (AverageFaceF.ChF:84-128), the sum of 1/fine and
+   1/(ref_scale*sum). */
+static void k_vcavg_faces(Fab* coarse, const Fab* fine, const Box* cfb, int dir, int r, int harmonic) {
   const double ref_scale = (1.0/r)*(1.0/r);
   const int n0 = dir == 0 ? 1 : r, n1 = dir == 1 ? 1 : r, n2 = dir == 2 ? 1 : r;
   for (int K = cfb->lo[2]; K <= cfb->hi[2]; K++)
@@ -1288,8 +1294,11 @@ static void k_vcavg_faces(Fab* coarse, const Fab* fine, const Box* cfb, int dir,
         double crseSum = 0;
         for (int kk = 0; kk < n2; kk++)
           for (int jj = 0; jj < n1; jj++)
-            for (int ii = 0; ii < n0; ii++) crseSum = crseSum + *P(fine, I*r+ii, J*r+jj, K*r+kk);
-        *P(coarse,I,J,K) = ref_scale*crseSum;
+            for (int ii = 0; ii < n0; ii++) {
+              const double v = *P(fine, I*r+ii, J*r+jj, K*r+kk);
+              crseSum = crseSum + (harmonic ? 1.0/v : v);
+            }
+        *P(coarse,I,J,K) = harmonic ? 1.0/(ref_scale*crseSum) : ref_scale*crseSum;
       }
 }
 static Box vc_facebox(Box b, int d) { b.hi[d] += 1; return b; }
@@ -1735,6 +1744,7 @@ static void op_reflux(Op* cop, Op* fop, LD* phiFine, LD* phi, LD* res) {
 /* Hierarchy: AMRPoissonOpFactory + AMRMultiGrid + MultiGrid + BiCGStab    */
 /* ======================================================================= */
 struct OHier {
+  int vcHarmonic;     /* VCAMRPoissonOp2Factory's coefficient average type: 0 arithmetic, 1 harmonic */
   int nlev;
   Layout* grids;      /* per AMR level */
   Box* domains;
@@ -1867,17 +1877,24 @@ int oracle_vc_set(OHier* h, const double* a, const double* b0, const double* b1,
     if (!op->vc) op_vc_alloc(op);
     const int r = 1 << dd;
     for (int q = 0; q < op->L.n; q++) {
-      k_vcavg_cells(&op->va.f[q], &op0->va.f[q], &op->L.b[q], r);
+      k_vcavg_cells(&op->va.f[q], &op0->va.f[q], &op->L.b[q], r, h->vcHarmonic);
       for (int d = 0; d < 3; d++) {
         Box cfb = vc_facebox(op->L.b[q], d);
-        k_vcavg_faces(&op->vb[d][q], &op0->vb[d][q], &cfb, d, r);
+        k_vcavg_faces(&op->vb[d][q], &op0->vb[d][q], &cfb, d, r, h->vcHarmonic);
       }
     }
     op_vc_lambda(op);
   }
   return 0;
 }
-/* AMRPoissonOp::s_relaxMode (AMRPoissonOp.cpp:31, 600-632) for every operator: 1 or 4 */
+/* m_coefficient_average_type of VCAMRPoissonOp2Factory (VCAMRPoissonOp2.cpp:1048-1063):
+   0 CoarseAverage::arithmetic, 1 harmonic; used by the next oracle_vc_set */
+int oracle_set_vc_average(OHier* h, int type) {
+  if (type != 0 && type != 1) { set_err("coefficient average type must be 0 (arithmetic) or 1 (harmonic)"); return -1; }
+  h->vcHarmonic = type;
+  return 0;
+}
+/* AMRPoissonOp::s_relaxMode (AMRPoissonOp.cpp:31, 600-632) for every operator */
 int oracle_set_relax_mode(OHier* h, int mode) {
   if (mode < 0 || mode > 5) {
     set_err("relax mode must be 0 (looseGSRB), 1 (levelGSRB), 2 (overlapGSRB), 3 (levelGSRBLazy), "

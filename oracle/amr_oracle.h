@@ -96,6 +96,7 @@ long oracle_face_cells(const OHier* h, int lev, int dir);
 int  oracle_level_norm(OHier* h, int d, const double* x, int p, double* out);
 /* AMRPoissonOp::s_relaxMode for every operator: 1 levelGSRB (default), 4 levelJacobi */
 int  oracle_set_relax_mode(OHier* h, int mode);
+int  oracle_set_vc_average(OHier* h, int type);
 /* setAlphaAndBeta on every operator (AMR levels and MG depths), lambda recomputed */
 int  oracle_set_alpha_beta(OHier* h, double alpha, double beta);
 int  oracle_vc_lambda(OHier* h, int depth, double* lambda_out);

@@ -62,6 +62,7 @@ def lib():
         L.oracle_set_alpha_beta.argtypes = [C.c_void_p, C.c_double, C.c_double]
         L.oracle_level_norm.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _dp]
         L.oracle_set_relax_mode.argtypes = [C.c_void_p, C.c_int]
+        L.oracle_set_vc_average.argtypes = [C.c_void_p, C.c_int]
         _lib = L
     return _lib
 
@@ -211,6 +212,10 @@ class Oracle:
         out = np.zeros(1)
         self._chk(lib().oracle_level_norm(self.h, depth, px, int(p), out.ctypes.data_as(_dp)))
         return float(out[0])
+
+    def set_vc_average(self, harmonic):
+        """Coefficient coarsening of the next vc_set: arithmetic (False) or harmonic."""
+        self._chk(lib().oracle_set_vc_average(self.h, 1 if harmonic else 0))
 
     def set_relax_mode(self, mode):
         """AMRPoissonOp::s_relaxMode: 0 looseGSRB, 1 levelGSRB, 2 overlapGSRB,

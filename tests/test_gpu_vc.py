@@ -393,6 +393,58 @@ def test_vc_jacobi_relax_mode_bitwise(gpu_ctx, hier):
     g.close()
 
 
+@pytest.mark.parametrize("hier", ["single", "amr32"])
+def test_vc_harmonic_coarsening_bitwise(gpu_ctx, hier):
+    """Harmonic coefficient coarsening on the multigrid depths: every depth's lambda,
+    level relaxation, and whole solves (FP64 bitwise; mixed precision within 1e-9)
+    equal the oracle's, and differ from the arithmetic coarsening."""
+    print(f"vc harmonic {hier}", flush=True)
+    if hier == "single":
+        h = small_single(**CASES["vc32"])
+        rhs = H.sinusoid_rhs(h, 1033)
+    else:
+        h = _amr("amr32")
+        rhs = H.bump_rhs(h, 1033)
+    a, b = H.vc_coefficients(h, seed=52, contrast=8.0)
+    g = amrmg.AMRMultiGrid(gpu_ctx, h)
+    g.set_vc_average(True)
+    g.set_vc_coefs(a, b)
+    o = Oracle(h)
+    o.set_vc_average(True)
+    o.vc_set(a, b)
+    oa = Oracle(h)
+    oa.vc_set(a, b)
+    rng = np.random.default_rng(17)
+    for d in range(o.mg_depths):
+        n = o.mg_cells(d)
+        phi, r = rng.standard_normal(n), rng.standard_normal(n)
+        np.testing.assert_array_equal(g.level_relax(d, phi, r, 2), o.level_relax(d, phi, r, 2))
+        if d > 0:
+            assert np.abs(o.vc_lambda(d) - oa.vc_lambda(d)).max() > 0
+    sp, rp = H.solver_params()
+    g.setSolverParameters(sp, rp)
+    g.set_field(g.RHS, rhs)
+    hist_g, info_g = g.solve()
+    phi_o, hist_o, info_o = o.solve(rhs, sp, rp)
+    assert info_g["exit_status"] & 1
+    assert info_g["iterations"] == info_o["iterations"]
+    np.testing.assert_array_equal(hist_g, hist_o)
+    np.testing.assert_array_equal(g.get_field(g.PHI), phi_o)
+    g.close()
+    gm = amrmg.AMRMultiGrid(gpu_ctx, h, precision=1)
+    gm.set_vc_average(True)
+    gm.set_vc_coefs(a, b)
+    gm.setSolverParameters(sp, rp)
+    gm.set_field(gm.RHS, rhs)
+    _, info_m = gm.solve()
+    assert info_m["exit_status"] & 1
+    pm = gm.get_field(gm.PHI)
+    # covered cells hold no part of the composite solution
+    m = np.concatenate([uncovered_mask(h, lev) for lev in range(h.nlev)])
+    assert np.abs(pm - phi_o)[m].max() / np.abs(phi_o)[m].max() < 1e-9
+    gm.close()
+
+
 def test_vc_rejects_bad_input(gpu_ctx):
     print("vc errors", flush=True)
     h = small_single(n=32, box=16)

@@ -360,3 +360,40 @@ def test_vc_jacobi_sweep_unit_coefficients():
     out = o.level_relax(0, e, np.zeros_like(e), 1)
     factor = 1 - 0.5 * (alpha + beta * lam) / (alpha - 6 * beta / dx ** 2)
     assert np.abs(out - factor * e).max() < 1e-12 * np.abs(e).max()
+
+
+def test_harmonic_coarsening_of_reciprocal_linear_coefficients():
+    """Harmonic averaging (VCAMRPoissonOp2Factory with CoarseAverage::harmonic): with a
+    and b the reciprocals of linear functions, the mean of 1/a over the children (of
+    1/b over the fine faces) is the centre value of the linear function, so every
+    depth's coefficients and lambda have a closed form; the arithmetic average of the
+    same data does not."""
+    alpha, beta = 0.9, 1.1
+    n = 32
+    h = small_single(n=n, box=16, alpha=alpha, beta=beta)
+    la = lambda x, y, z: 1.0 + 0.5 * x - 0.25 * y + 0.125 * z
+    lb = [lambda x, y, z: 2.0 + x + 0.5 * y - 0.5 * z, lambda x, y, z: 1.5 - 0.5 * x + y + 0.25 * z,
+          lambda x, y, z: 1.0 + 0.25 * x + 0.25 * y + z]
+    a = h.pack(lambda x, y, z, l: 1.0 / la(x, y, z))
+    b = [h.pack_faces(lambda x, y, z, f=lb[d]: 1.0 / f(x, y, z), d) for d in range(3)]
+    errs = {}
+    for harmonic in (True, False):
+        o = Oracle(h)
+        o.set_vc_average(harmonic)
+        o.vc_set(a, b)
+        assert o.mg_depths >= 3
+        worst = 0.0
+        for dep in range(1, o.mg_depths):
+            r = 2 ** dep
+            hc = H.Hierarchy(H.box_coarsen(h.domain, r), [[H.box_coarsen(bx, r) for bx in h.levels[0]]], h.dx(0) * r)
+            dx = h.dx(0) * r
+            X, Y, Z = (hc.pack(lambda x, y, z, l, q=q: (x, y, z)[q]) for q in range(3))
+            s = beta / dx ** 2
+            den = alpha / la(X, Y, Z)
+            for d, e in enumerate([(1, 0, 0), (0, 1, 0), (0, 0, 1)]):
+                den = den + s * (1.0 / lb[d](X + 0.5 * dx * e[0], Y + 0.5 * dx * e[1], Z + 0.5 * dx * e[2])
+                                 + 1.0 / lb[d](X - 0.5 * dx * e[0], Y - 0.5 * dx * e[1], Z - 0.5 * dx * e[2]))
+            worst = max(worst, np.abs(o.vc_lambda(dep) * den - 1).max())
+        errs[harmonic] = worst
+    assert errs[True] < 1e-13
+    assert errs[False] > 1e-6
