@@ -642,6 +642,7 @@ typedef struct {
   IVS standard, quadd;
   Box osbox;
   unsigned char* dropOrd;      /* over osbox */
+  unsigned char* nmix;         /* over osbox: corner stencils averaged into the mixed derivative */
   DSten* mixed;                /* over osbox */
   DSten* first[3];             /* per derivative direction, over osbox */
   DSten* second[3];
@@ -660,6 +661,7 @@ static void qcfs_build_stencils(QCFS* s, const IVS* allGood) {
   s->osbox = ivs_minbox(&s->quadd);
   long no = b_npts(&s->osbox);
   s->dropOrd = calloc(no, 1);
+  s->nmix = calloc(no, 1);
   s->mixed = calloc(no, sizeof(DSten));
   for (int d = 0; d < 3; d++) { s->first[d] = calloc(no, sizeof(DSten)); s->second[d] = calloc(no, sizeof(DSten)); }
   int t1 = vt[0], t2 = vt[1];
@@ -690,6 +692,7 @@ static void qcfs_build_stencils(QCFS* s, const IVS* allGood) {
         }
         if (imixed > 0) { double rn = imixed; for (int q = 0; q < ds->n; q++) ds->w[q] /= rn; }
         s->dropOrd[os_off(s,i,j,k)] = !hasamixed;
+        s->nmix[os_off(s,i,j,k)] = (unsigned char)imixed;
       }
   for (int ivec = 0; ivec < 2; ivec++) {
     int idir = vt[ivec];
@@ -796,7 +799,7 @@ static void qcfs_define(QCFS* s, const Box* fineDomain, const Box* grid, const L
 }
 static void qcfs_free(QCFS* s) {
   ivs_free(&s->fine); ivs_free(&s->coar); ivs_free(&s->standard); ivs_free(&s->quadd);
-  free(s->dropOrd); free(s->mixed);
+  free(s->dropOrd); free(s->nmix); free(s->mixed);
   for (int d = 0; d < 3; d++) { free(s->first[d]); free(s->second[d]); }
 }
 
@@ -1761,6 +1764,9 @@ struct OHier {
   Layout* cfL;        /* coarsened fine layouts (resC) per level >= 1 */
   Plan* resCopier;    /* resC[l] -> grids[l-1] */
   Plan* revCopier;    /* grids[l-1] -> resC[l] */
+  /* BiCGStabSolver's public parameters (BiCGStabSolver.H:105-153) for every bottom solve */
+  int bic_imax, bic_numRestarts;
+  double bic_reps, bic_hang, bic_small;
   /* statistics */
   int bottom_calls;
 };
@@ -1776,6 +1782,8 @@ OHier* oracle_create(const int* is, const double* rs) {
   h->domains[0] = mkbox(is[1], is[2], is[3], is[4], is[5], is[6]);
   for (int d = 0; d < 3; d++) { h->bc.type[0][d] = is[7+d]; h->bc.type[1][d] = is[10+d]; }
   h->dx[0] = rs[0]; h->alpha = rs[1]; h->beta = rs[2];
+  /* BiCGStabSolver constructor defaults (BiCGStabSolver.H:166-183) */
+  h->bic_imax = 80; h->bic_numRestarts = 5; h->bic_reps = 1e-12; h->bic_hang = 1e-8; h->bic_small = 1e-30;
   for (int d = 0; d < 3; d++) { h->bc.val[0][d] = rs[3+d]; h->bc.val[1][d] = rs[6+d]; }
   const int* bp = is + 13 + nlev;
   for (int l = 0; l < nlev; l++) {
@@ -1960,7 +1968,7 @@ static const Op* mg_op(const OHier* h, int d) { return d == 0 ? &h->op[0] : &h->
 /* ---- BiCGStabSolver<T>::solve (BiCGStabSolver.H:199-519) ---- */
 typedef struct { double eps, reps, hang, small, metric; int imax, numRestarts, normType; } BiCG;
 
-static int bicgstab_solve(const Op* op, LD* phi, const LD* rhs, const BiCG* p) {
+static int bicgstab_solve(const Op* op, LD* phi, const LD* rhs, const BiCG* p, int* iters_out) {
   LD r = ld_make(rhs->L, rhs->g), r_tilde = ld_make(rhs->L, rhs->g), e = ld_make(phi->L, phi->g),
      pp = ld_make(rhs->L, rhs->g), p_tilde = ld_make(phi->L, phi->g), s_tilde = ld_make(phi->L, phi->g),
      t = ld_make(rhs->L, rhs->g), v = ld_make(rhs->L, rhs->g);
@@ -2031,6 +2039,7 @@ static int bicgstab_solve(const Op* op, LD* phi, const LD* rhs, const BiCG* p) {
 done:
   ld_free(&r); ld_free(&r_tilde); ld_free(&e); ld_free(&pp); ld_free(&p_tilde);
   ld_free(&s_tilde); ld_free(&t); ld_free(&v);
+  if (iters_out) *iters_out = iter;
   return status;
 }
 
@@ -2048,7 +2057,7 @@ static void mg_cycle(MGState* s, int depth, LD* corr, const LD* res) {
     if (h->bottomCells == 1) op_relax(op, corr, res, 1);
     else {
       op_relax(op, corr, res, s->bottom);
-      bicgstab_solve(op, corr, res, &s->bic);
+      bicgstab_solve(op, corr, res, &s->bic, NULL);
       h->bottom_calls++;
     }
     return;
@@ -2182,8 +2191,8 @@ static void amr_state_make(AMRState* a, OHier* h, int pre, int post, int bottom,
   m->h = h; m->pre = pre; m->post = post; m->bottom = bottom; m->cycles = numMG;
   m->mgres = calloc(h->mgD, sizeof(LD)); m->mgcor = calloc(h->mgD, sizeof(LD));
   for (int d = 1; d < h->mgD; d++) { m->mgres[d] = ld_make(&h->mgL[d], 0); m->mgcor[d] = ld_make(&h->mgL[d], 1); }
-  m->bic.eps = 1e-6; m->bic.reps = 1e-12; m->bic.hang = 1e-8; m->bic.small = 1e-30;
-  m->bic.metric = -1.0; m->bic.imax = 80; m->bic.numRestarts = 5; m->bic.normType = 2;
+  m->bic.eps = 1e-6; m->bic.reps = h->bic_reps; m->bic.hang = h->bic_hang; m->bic.small = h->bic_small;
+  m->bic.metric = -1.0; m->bic.imax = h->bic_imax; m->bic.numRestarts = h->bic_numRestarts; m->bic.normType = 2;
 }
 static void amr_state_free(AMRState* a) {
   OHier* h = a->h;
@@ -2373,11 +2382,116 @@ int oracle_bottom_solve(OHier* h, double* corr_io, const double* res_in, int nbo
   const Op* op = mg_op(h, d);
   LD c = ld_make(&h->mgL[d], 1), r = ld_make(&h->mgL[d], 0);
   ld_from_packed(&c, corr_io); ld_from_packed(&r, res_in);
-  BiCG b = {eps, 1e-12, 1e-8, 1e-30, metric, 80, 5, 2};
+  BiCG b = {eps, h->bic_reps, h->bic_hang, h->bic_small, metric, h->bic_imax, h->bic_numRestarts, 2};
   op_relax(op, &c, &r, nbottom);
-  int st = bicgstab_solve(op, &c, &r, &b);
+  int it = 0;
+  int st = bicgstab_solve(op, &c, &r, &b, &it);
   ld_to_packed(&c, corr_io);
-  if (info) info[0] = st;
+  if (info) { info[0] = st; info[1] = it; }
   ld_free(&c); ld_free(&r);
+  return 0;
+}
+
+/* BiCGStabSolver's public parameters (BiCGStabSolver.H:105-153) for every later
+   bottom solve of h: oracle_bottom_solve and the multigrid bottom of oracle_solve. */
+int oracle_set_bottom_params(OHier* h, int imax, int numRestarts, double reps, double hang, double small) {
+  if (imax < 0 || numRestarts < 0) { set_err("bad BiCGStab parameters"); return -1; }
+  h->bic_imax = imax; h->bic_numRestarts = numRestarts;
+  h->bic_reps = reps; h->bic_hang = hang; h->bic_small = small;
+  return 0;
+}
+
+/* ---- operator applications on AMR level lev with its own CFRegion: the level's
+   AMRPoissonOp (homogeneousCFInterp with dxCrse = dx of level lev-1, exchange, BC) ---- */
+static int chk_lev(const OHier* h, int lev) {
+  if (lev < 0 || lev >= h->nlev) { set_err("bad level"); return -1; }
+  return 0;
+}
+/* AMRPoissonOp::applyOp (AMRPoissonOp.cpp:332-340) */
+int oracle_amrlevel_apply(OHier* h, int lev, const double* phi_in, double* out, int homog) {
+  if (chk_lev(h, lev)) return -1;
+  LD phi = ld_make(&h->grids[lev], 1), l = ld_make(&h->grids[lev], 0);
+  ld_from_packed(&phi, phi_in);
+  op_apply(&h->op[lev], &l, &phi, homog);
+  ld_to_packed(&l, out);
+  ld_free(&phi); ld_free(&l);
+  return 0;
+}
+/* AMRPoissonOp::residual (AMRPoissonOp.cpp:224-236) */
+int oracle_amrlevel_residual(OHier* h, int lev, const double* phi_in, const double* rhs_in, double* out, int homog) {
+  if (chk_lev(h, lev)) return -1;
+  LD phi = ld_make(&h->grids[lev], 1), rhs = ld_make(&h->grids[lev], 0), r = ld_make(&h->grids[lev], 0);
+  ld_from_packed(&phi, phi_in); ld_from_packed(&rhs, rhs_in);
+  op_residual(&h->op[lev], &r, &phi, &rhs, homog);
+  ld_to_packed(&r, out);
+  ld_free(&phi); ld_free(&rhs); ld_free(&r);
+  return 0;
+}
+/* AMRPoissonOp::relax (AMRPoissonOp.cpp:600-632) */
+int oracle_amrlevel_relax(OHier* h, int lev, double* phi_io, const double* rhs_in, int iters) {
+  if (chk_lev(h, lev)) return -1;
+  LD phi = ld_make(&h->grids[lev], 1), rhs = ld_make(&h->grids[lev], 0);
+  ld_from_packed(&phi, phi_io); ld_from_packed(&rhs, rhs_in);
+  op_relax(&h->op[lev], &phi, &rhs, iters);
+  ld_to_packed(&phi, phi_io);
+  ld_free(&phi); ld_free(&rhs);
+  return 0;
+}
+
+/* Which QuadCFStencil branch (QuadCFStencil.cpp:662-918) each tangential derivative
+   of level lev's coarse-fine stencils took, summed over every fine box and face:
+   counts[0] standard cells (centred formulas throughout), [1] cells with special
+   stencils (m_ivsQuadd), [2] no admissible mixed stencil (order dropped),
+   [3] centred, [4] shifted toward +, [5] shifted toward -, [6] dropped order with a
+   one-sided first derivative toward +, [7] toward -, [8] no admissible first
+   derivative (zero), [9] mixed stencils averaged over 1, [10] 2, [11] 3, [12] 4
+   corners.  Branches [3..8] count (cell, tangential direction) pairs. */
+int oracle_cf_branch_counts(OHier* h, int lev, long* counts) {
+  if (chk_lev(h, lev) || lev == 0) { set_err("bad level"); return -1; }
+  for (int q = 0; q < 13; q++) counts[q] = 0;
+  const QCFI* qi = &h->op[lev].qcfi;
+  if (!qi->level) return 0;
+  for (int b = 0; b < h->grids[lev].n; b++)
+    for (int f = 0; f < 6; f++) {
+      const QCFS* s = &qi->st[b*6+f];
+      if (!s->defined_nonempty) continue;
+      counts[0] += ivs_numpts(&s->standard);
+      if (ivs_numpts(&s->quadd) == 0) continue;
+      int vt[2], n = 0;
+      for (int d = 0; d < 3; d++) if (d != s->dir) vt[n++] = d;
+      for (int k = s->osbox.lo[2]; k <= s->osbox.hi[2]; k++)
+        for (int j = s->osbox.lo[1]; j <= s->osbox.hi[1]; j++)
+          for (int i = s->osbox.lo[0]; i <= s->osbox.hi[0]; i++) {
+            if (!ivs_has(&s->quadd, i, j, k)) continue;
+            counts[1]++;
+            long o = os_off(s, i, j, k);
+            const int nm = s->nmix[o];
+            if (nm == 0) { counts[2]++; continue; }
+            counts[8+nm]++;
+            int iv[3] = {i, j, k};
+            for (int t = 0; t < 2; t++) {
+              int dd = vt[t];
+              const DSten* sec = &s->second[dd][o];
+              const DSten* fir = &s->first[dd][o];
+              if (sec->n == 3) {
+                int lo = 1 << 30, hi = -(1 << 30);
+                for (int q = 0; q < 3; q++) { int off = sec->iv[q][dd] - iv[dd]; if (off < lo) lo = off; if (off > hi) hi = off; }
+                if (lo == -1 && hi == 1) counts[3]++;
+                else if (lo == 0 && hi == 2) counts[4]++;
+                else if (lo == -2 && hi == 0) counts[5]++;
+              } else {
+                int plus = 0, minus = 0;
+                for (int q = 0; q < fir->n; q++) {
+                  int off = fir->iv[q][dd] - iv[dd];
+                  if (off == 1 && fir->w[q] != 0.0) plus = 1;
+                  if (off == -1 && fir->w[q] != 0.0) minus = 1;
+                }
+                if (plus) counts[6]++;
+                else if (minus) counts[7]++;
+                else counts[8]++;
+              }
+            }
+          }
+    }
   return 0;
 }

@@ -101,6 +101,20 @@ int  oracle_set_vc_average(OHier* h, int type);
 int  oracle_set_alpha_beta(OHier* h, double alpha, double beta);
 int  oracle_vc_lambda(OHier* h, int depth, double* lambda_out);
 
+/* BiCGStabSolver's public parameters (BiCGStabSolver.H:105-153) for every later bottom
+   solve (oracle_bottom_solve and the multigrid bottom of oracle_solve); defaults
+   imax 80, numRestarts 5, reps 1e-12, hang 1e-8, small 1e-30.  oracle_bottom_solve's
+   info = {exit status, iterations}. */
+int  oracle_set_bottom_params(OHier* h, int imax, int numRestarts, double reps, double hang, double small);
+/* applyOp / residual / relax of AMR level lev's operator (its own CFRegion,
+   homogeneousCFInterp with dxCrse = dx of level lev-1); valid-packed level data */
+int  oracle_amrlevel_apply(OHier* h, int lev, const double* phi, double* out, int homog);
+int  oracle_amrlevel_residual(OHier* h, int lev, const double* phi, const double* rhs, double* out, int homog);
+int  oracle_amrlevel_relax(OHier* h, int lev, double* phi, const double* rhs, int iters);
+/* QuadCFStencil branch statistics of level lev's coarse-fine stencils (13 counts,
+   documented at the definition) */
+int  oracle_cf_branch_counts(OHier* h, int lev, long* counts);
+
 /* number of cell-relaxations (one red+black GSRB iteration of one cell) since reset */
 long long oracle_cell_relax_count(void);
 void oracle_reset_counters(void);

@@ -63,6 +63,11 @@ def lib():
         L.oracle_level_norm.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _dp]
         L.oracle_set_relax_mode.argtypes = [C.c_void_p, C.c_int]
         L.oracle_set_vc_average.argtypes = [C.c_void_p, C.c_int]
+        L.oracle_set_bottom_params.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double]
+        L.oracle_amrlevel_apply.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.c_int]
+        L.oracle_amrlevel_residual.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, C.c_int]
+        L.oracle_amrlevel_relax.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.c_int]
+        L.oracle_cf_branch_counts.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_long)]
         _lib = L
     return _lib
 
@@ -187,13 +192,50 @@ class Oracle:
         self._chk(lib().oracle_average(self.h, lev, pf, out.ctypes.data_as(_dp)))
         return out
 
-    def bottom_solve(self, corr, res, nbottom, metric, eps):
+    def bottom_solve(self, corr, res, nbottom, metric, eps, iterations=False):
+        """relax(nbottom) + BiCGStabSolver::solve on the deepest multigrid level; returns
+        (correction, exit status) or, with iterations=True, (correction, status, iterations)."""
         corr = np.array(corr, dtype=np.float64)
         res, pr = _d(res)
-        info = np.zeros(1, dtype=np.int32)
+        info = np.zeros(2, dtype=np.int32)
         self._chk(lib().oracle_bottom_solve(self.h, corr.ctypes.data_as(_dp), pr, nbottom, metric, eps,
                                             info.ctypes.data_as(_ip)))
+        if iterations:
+            return corr, int(info[0]), int(info[1])
         return corr, int(info[0])
+
+    def set_bottom_params(self, imax=80, numRestarts=5, reps=1e-12, hang=1e-8, small=1e-30):
+        """BiCGStabSolver's public parameters for every later bottom solve."""
+        self._chk(lib().oracle_set_bottom_params(self.h, int(imax), int(numRestarts), float(reps), float(hang),
+                                                 float(small)))
+
+    def amrlevel_apply(self, lev, phi, homog=False):
+        phi, pp = _d(phi)
+        out = np.zeros_like(phi)
+        self._chk(lib().oracle_amrlevel_apply(self.h, lev, pp, out.ctypes.data_as(_dp), int(homog)))
+        return out
+
+    def amrlevel_residual(self, lev, phi, rhs, homog=False):
+        phi, pp = _d(phi)
+        rhs, pr = _d(rhs)
+        out = np.zeros_like(phi)
+        self._chk(lib().oracle_amrlevel_residual(self.h, lev, pp, pr, out.ctypes.data_as(_dp), int(homog)))
+        return out
+
+    def amrlevel_relax(self, lev, phi, rhs, iters):
+        phi = np.array(phi, dtype=np.float64)
+        rhs, pr = _d(rhs)
+        self._chk(lib().oracle_amrlevel_relax(self.h, lev, phi.ctypes.data_as(_dp), pr, iters))
+        return phi
+
+    CF_BRANCHES = ("standard", "special", "mixed_dropped", "centred", "shift_plus", "shift_minus", "drop_plus",
+                   "drop_minus", "drop_zero", "mixed1", "mixed2", "mixed3", "mixed4")
+
+    def cf_branch_counts(self, lev):
+        """QuadCFStencil branch statistics of level lev (dict keyed by CF_BRANCHES)."""
+        out = (C.c_long * 13)()
+        self._chk(lib().oracle_cf_branch_counts(self.h, lev, out))
+        return dict(zip(self.CF_BRANCHES, [int(v) for v in out]))
 
     def vc_set(self, a, b):
         """Variable coefficients (VCAMRPoissonOp2) on every level: a valid-packed over all
