@@ -96,6 +96,8 @@ def lib():
         L.amrmg_quad_cf.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
         L.amrmg_average.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
         L.amrmg_bottom_solve.argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.c_double, C.c_double, _ip]
+        L.amrmg_bottom_solve_device.argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.c_double, C.c_double, _ip]
+        L.amrmg_solver_set_bottom_params.argtypes = [C.c_void_p, _ip, _dp]
         L.amrmg_plan_check.argtypes = [_ip, _dp, C.POINTER(C.c_longlong), C.c_int]
         L.amrmg_copy_probe.argtypes = [C.c_void_p, C.c_longlong, C.c_int, _dp]
         L.amrmg_profile_enable.argtypes = [C.c_void_p, C.c_int]
@@ -369,13 +371,24 @@ class AMRMultiGrid:
         _chk(lib().amrmg_average(self.p, lev, pf, out.ctypes.data_as(_dp)))
         return out
 
-    def bottom_solve(self, corr, res, nbottom, metric, eps):
+    def bottom_solve(self, corr, res, nbottom, metric, eps, device=False):
+        """relax(nbottom) + BiCGStabSolver::solve on the deepest multigrid level: the
+        host-driven BiCGStab (returns (correction, exit status, iterations)) or the
+        device bottom solver (device=True: (correction, None, iterations))."""
         corr = np.array(corr, dtype=np.float64)
         res, pr = _d(res)
-        info = np.zeros(1, dtype=np.int32)
-        _chk(lib().amrmg_bottom_solve(self.p, corr.ctypes.data_as(_dp), pr, nbottom, metric, eps,
-                                       info.ctypes.data_as(_ip)))
-        return corr, int(info[0])
+        info = np.zeros(2, dtype=np.int32)
+        fn = lib().amrmg_bottom_solve_device if device else lib().amrmg_bottom_solve
+        _chk(fn(self.p, corr.ctypes.data_as(_dp), pr, nbottom, metric, eps, info.ctypes.data_as(_ip)))
+        if device:
+            return corr, None, int(info[0])
+        return corr, int(info[0]), int(info[1])
+
+    def set_bottom_params(self, imax=80, numRestarts=5, reps=1e-12, hang=1e-8, small=1e-30):
+        """BiCGStabSolver's public parameters for every bottom solve of this solver."""
+        ip = np.asarray([imax, numRestarts], dtype=np.int32)
+        rp = np.asarray([reps, hang, small], dtype=np.float64)
+        _chk(lib().amrmg_solver_set_bottom_params(self.p, ip.ctypes.data_as(_ip), rp.ctypes.data_as(_dp)))
 
 
 class HeatSolver:

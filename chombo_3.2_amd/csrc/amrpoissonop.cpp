@@ -642,84 +642,108 @@ static void setup_fused(AMRPoissonOp<Real>* op) {
   op->m_fused = true;
 }
 
+// AMRPoissonOp::define (AMRPoissonOp.cpp:60-221) on one level: exchange, BC and (when
+// the level has a coarser one) homogeneous coarse-fine ghost rules from the level's
+// own CFRegion, and the one-launch relaxation where it applies.
 template <class Real>
-std::unique_ptr<AMRPoissonOp<Real>> AMRPoissonOpFactory<Real>::AMRnewOp(int l, AMRPoissonOp<Real>* coarserOp) {
-  std::unique_ptr<AMRPoissonOp<Real>> op(new AMRPoissonOp<Real>());
-  op->m_ctx = H->ctx;
-  op->m_lay = H->amrLay[l].get();
-  op->m_domain = H->domains[l];
-  op->m_dx = H->dxs[l];
-  op->m_dxCrse = (l > 0) ? H->dxs[l - 1] : -1.0;
+void defineLevelOp(AMRPoissonOp<Real>* op, Context* ctx, const DeviceLayout* lay, const Box& domain, double dx,
+                   double dxCrse, double alpha, double beta, const BCSpec& bc, const DeviceLayout* mgCoarse) {
+  op->m_ctx = ctx;
+  op->m_lay = lay;
+  op->m_domain = domain;
+  op->m_dx = dx;
+  op->m_dxCrse = dxCrse;
   op->m_alpha = alpha;
   op->m_beta = beta;
   op->m_bc = bc;
-  op->m_mgCoarse = (l == 0 && !H->mgLay.empty()) ? H->mgLay[0].get() : (l > 0 ? H->cfLay[l].get() : nullptr);
-  op->m_ghost = build_ghost_plan(H->grids[l], H->domains[l], bc, l > 0, H->amrLay[l]->merged);
-  validate_ghost_plan(op->m_ghost, H->grids[l]);
-  op->m_ghost.upload(H->ctx);
-  setup_fused(op.get());
+  op->m_mgCoarse = mgCoarse;
+  op->m_ghost = build_ghost_plan(lay->dbl, domain, bc, dxCrse > 0, lay->merged);
+  validate_ghost_plan(op->m_ghost, lay->dbl);
+  op->m_ghost.upload(ctx);
+  setup_fused(op);
   if (op->m_mgCoarse) validate_pair(*op->m_lay, *op->m_mgCoarse);
-  if (l > 0) {
-    op->m_coarserLay = H->amrLay[l - 1].get();
-    op->m_cfLay = H->cfLay[l].get();
-    CFPlanHost cf = build_cf_plan(H->grids[l], *H->amrLay[l], H->grids[l - 1], *H->amrLay[l - 1], H->domains[l],
-                                  H->dxs[l], 2);
-    {
-      const long long fs = H->amrLay[l]->slab, cs = H->amrLay[l - 1]->slab;
-      for (const CFCoarse& c : cf.coarse) {
-        require(c.pc >= 0 && c.pc < cs, "CF coarse cell");
-        if (c.std)
-          for (int q = 0; q < 8; q++) require(c.nbr[q] >= 0 && c.nbr[q] < cs, "CF coarse neighbour");
-        else require(c.gen >= 0 && c.gen < (int)cf.gens.size(), "CF stencil index");
-      }
-      for (const CFGen& g : cf.gens)
-        for (int q = 0; q < 5; q++) {
-          require(g.n[q] >= 0 && g.n[q] <= 9, "CF stencil length");
-          for (int t = 0; t < g.n[q]; t++) require(g.off[q][t] >= 0 && g.off[q][t] < cs, "CF stencil cell");
-        }
-      for (const CFFine& f : cf.fine) {
-        require(f.g >= 0 && f.g < fs && f.a1 >= 0 && f.a1 < fs && f.a2 >= 0 && f.a2 < fs, "CF fine cell");
-        require(f.coarse >= 0 && f.coarse < (int)cf.coarse.size(), "CF fine record");
-      }
+}
+
+// The coarse-fine part of AMRnewOp for a level with a coarser AMR level (refinement
+// ratio 2): QuadCFInterp tables, the copiers between coarsen(fine, 2) (cfLay) and the
+// coarser level, and the coarser operator's reflux tables and zeroCovered plan.
+template <class Real>
+void linkCoarser(AMRPoissonOp<Real>* op, AMRPoissonOp<Real>* coarserOp, const DeviceLayout* cfLay) {
+  Context* ctx = op->m_ctx;
+  const DeviceLayout& F = *op->m_lay;
+  const DeviceLayout& Cl = *coarserOp->m_lay;
+  if (!(refine(coarserOp->m_domain, 2) == op->m_domain)) throw std::runtime_error("coarser level: domain must be coarsen(domain, 2)");
+  op->m_refToCoarser = 2;
+  op->m_coarserLay = &Cl;
+  op->m_cfLay = cfLay;
+  CFPlanHost cf = build_cf_plan(F.dbl, F, Cl.dbl, Cl, op->m_domain, op->m_dx, 2);
+  {
+    const long long fs = F.slab, cs = Cl.slab;
+    for (const CFCoarse& c : cf.coarse) {
+      require(c.pc >= 0 && c.pc < cs, "CF coarse cell");
+      if (c.std)
+        for (int q = 0; q < 8; q++) require(c.nbr[q] >= 0 && c.nbr[q] < cs, "CF coarse neighbour");
+      else require(c.gen >= 0 && c.gen < (int)cf.gens.size(), "CF stencil index");
     }
-    op->n_cfc = (int)cf.coarse.size();
-    op->n_cfg = (int)cf.gens.size();
-    op->n_cff = (int)cf.fine.size();
-    op->d_cfc = upload_vec(H->ctx, cf.coarse);
-    op->d_cfg = upload_vec(H->ctx, cf.gens);
-    op->d_cff = upload_vec(H->ctx, cf.fine);
-    dev_alloc(H->ctx, (void**)&op->d_deriv, sizeof(Real) * 6 * (size_t)op->n_cfc, "hipMalloc deriv");
-    op->m_resCopier = build_copier(H->grids[l].coarsened(2), H->grids[l - 1]);
-    validate_motion(op->m_resCopier, H->grids[l].coarsened(2), H->grids[l - 1]);
-    op->m_resCopier.upload(H->ctx);
-    op->m_revCopier = build_copier(H->grids[l - 1], H->grids[l].coarsened(2));
-    validate_motion(op->m_revCopier, H->grids[l - 1], H->grids[l].coarsened(2));
-    op->m_revCopier.upload(H->ctx);
-    validate_pair(*op->m_lay, *op->m_cfLay);
-    // the coarser operator refluxes against this level and zeroes what it covers
-    coarserOp->m_finerLay = op->m_lay;
-    coarserOp->m_zeroPlan = &op->m_resCopier;
-    coarserOp->m_finerCfLay = op->m_cfLay;
-    RefluxPlanHost rp = build_reflux_plan(H->grids[l - 1], *H->amrLay[l - 1], H->grids[l], *H->amrLay[l],
-                                          H->domains[l - 1], 2);
-    {
-      const long long fs = H->amrLay[l]->slab, cs = H->amrLay[l - 1]->slab;
-      for (const RefluxCell& c : rp.cells) {
-        require(c.u >= 0 && c.u < cs, "reflux cell");
-        require(c.c0 >= 0 && c.nc >= 0 && c.c0 + c.nc <= (int)rp.coarse.size(), "reflux coarse range");
-        require(c.f0 >= 0 && c.nf >= 0 && c.f0 + c.nf <= (int)rp.fine.size(), "reflux fine range");
+    for (const CFGen& g : cf.gens)
+      for (int q = 0; q < 5; q++) {
+        require(g.n[q] >= 0 && g.n[q] <= 9, "CF stencil length");
+        for (int t = 0; t < g.n[q]; t++) require(g.off[q][t] >= 0 && g.off[q][t] < cs, "CF stencil cell");
       }
-      for (const RefluxCoarse& x : rp.coarse) require(x.a >= 0 && x.a < cs && x.b >= 0 && x.b < cs, "reflux coarse face");
-      for (const RefluxFine& x : rp.fine)
-        for (int t = 0; t < 4; t++) require(x.a[t] >= 0 && x.a[t] < fs && x.b[t] >= 0 && x.b[t] < fs, "reflux fine face");
+    for (const CFFine& f : cf.fine) {
+      require(f.g >= 0 && f.g < fs && f.a1 >= 0 && f.a1 < fs && f.a2 >= 0 && f.a2 < fs, "CF fine cell");
+      require(f.coarse >= 0 && f.coarse < (int)cf.coarse.size(), "CF fine record");
     }
-    coarserOp->n_rcell = (int)rp.cells.size();
-    coarserOp->m_rcFace = rp.coarseFace;
-    coarserOp->m_rfFace = rp.fineFace;
-    coarserOp->d_rcell = upload_vec(H->ctx, rp.cells);
-    coarserOp->d_rc = upload_vec(H->ctx, rp.coarse);
-    coarserOp->d_rf = upload_vec(H->ctx, rp.fine);
   }
+  op->n_cfc = (int)cf.coarse.size();
+  op->n_cfg = (int)cf.gens.size();
+  op->n_cff = (int)cf.fine.size();
+  op->d_cfc = upload_vec(ctx, cf.coarse);
+  op->d_cfg = upload_vec(ctx, cf.gens);
+  op->d_cff = upload_vec(ctx, cf.fine);
+  dev_alloc(ctx, (void**)&op->d_deriv, sizeof(Real) * 6 * (size_t)op->n_cfc, "hipMalloc deriv");
+  const DBL fc = F.dbl.coarsened(2);
+  op->m_resCopier = build_copier(fc, Cl.dbl);
+  validate_motion(op->m_resCopier, fc, Cl.dbl);
+  op->m_resCopier.upload(ctx);
+  op->m_revCopier = build_copier(Cl.dbl, fc);
+  validate_motion(op->m_revCopier, Cl.dbl, fc);
+  op->m_revCopier.upload(ctx);
+  validate_pair(*op->m_lay, *op->m_cfLay);
+  // the coarser operator refluxes against this level and zeroes what it covers
+  coarserOp->m_finerLay = op->m_lay;
+  coarserOp->m_zeroPlan = &op->m_resCopier;
+  coarserOp->m_finerCfLay = op->m_cfLay;
+  RefluxPlanHost rp = build_reflux_plan(Cl.dbl, Cl, F.dbl, F, coarserOp->m_domain, 2);
+  {
+    const long long fs = F.slab, cs = Cl.slab;
+    for (const RefluxCell& c : rp.cells) {
+      require(c.u >= 0 && c.u < cs, "reflux cell");
+      require(c.c0 >= 0 && c.nc >= 0 && c.c0 + c.nc <= (int)rp.coarse.size(), "reflux coarse range");
+      require(c.f0 >= 0 && c.nf >= 0 && c.f0 + c.nf <= (int)rp.fine.size(), "reflux fine range");
+    }
+    for (const RefluxCoarse& x : rp.coarse) require(x.a >= 0 && x.a < cs && x.b >= 0 && x.b < cs, "reflux coarse face");
+    for (const RefluxFine& x : rp.fine)
+      for (int t = 0; t < 4; t++) require(x.a[t] >= 0 && x.a[t] < fs && x.b[t] >= 0 && x.b[t] < fs, "reflux fine face");
+  }
+  dev_free(ctx, coarserOp->d_rcell);
+  dev_free(ctx, coarserOp->d_rc);
+  dev_free(ctx, coarserOp->d_rf);
+  coarserOp->n_rcell = (int)rp.cells.size();
+  coarserOp->m_rcFace = rp.coarseFace;
+  coarserOp->m_rfFace = rp.fineFace;
+  coarserOp->d_rcell = upload_vec(ctx, rp.cells);
+  coarserOp->d_rc = upload_vec(ctx, rp.coarse);
+  coarserOp->d_rf = upload_vec(ctx, rp.fine);
+}
+
+template <class Real>
+std::unique_ptr<AMRPoissonOp<Real>> AMRPoissonOpFactory<Real>::AMRnewOp(int l, AMRPoissonOp<Real>* coarserOp) {
+  std::unique_ptr<AMRPoissonOp<Real>> op(new AMRPoissonOp<Real>());
+  const DeviceLayout* mgc = (l == 0 && !H->mgLay.empty()) ? H->mgLay[0].get() : (l > 0 ? H->cfLay[l].get() : nullptr);
+  defineLevelOp(op.get(), H->ctx, H->amrLay[l].get(), H->domains[l], H->dxs[l], (l > 0) ? H->dxs[l - 1] : -1.0,
+                alpha, beta, bc, mgc);
+  if (l > 0) linkCoarser(op.get(), coarserOp, H->cfLay[l].get());
   return op;
 }
 
@@ -729,20 +753,8 @@ std::vector<std::unique_ptr<MGLevelOp<LevelData<Real>>>> AMRPoissonOpFactory<Rea
   for (size_t d = 0; d < H->mgLay.size(); d++) {
     int coarsening = 1 << (d + 1);
     AMRPoissonOp<Real>* op = new AMRPoissonOp<Real>();
-    op->m_ctx = H->ctx;
-    op->m_lay = H->mgLay[d].get();
-    op->m_domain = coarsen(H->domains[0], coarsening);
-    op->m_dx = H->dxs[0] * coarsening;
-    op->m_dxCrse = -1.0;
-    op->m_alpha = alpha;
-    op->m_beta = beta;
-    op->m_bc = bc;
-    op->m_mgCoarse = (d + 1 < H->mgLay.size()) ? H->mgLay[d + 1].get() : nullptr;
-    op->m_ghost = build_ghost_plan(H->mgLay[d]->dbl, op->m_domain, bc, false, H->mgLay[d]->merged);
-    validate_ghost_plan(op->m_ghost, H->mgLay[d]->dbl);
-    op->m_ghost.upload(H->ctx);
-    setup_fused(op);
-    if (op->m_mgCoarse) validate_pair(*op->m_lay, *op->m_mgCoarse);
+    defineLevelOp(op, H->ctx, H->mgLay[d].get(), coarsen(H->domains[0], coarsening), H->dxs[0] * coarsening, -1.0,
+                  alpha, beta, bc, (d + 1 < H->mgLay.size()) ? H->mgLay[d + 1].get() : nullptr);
     out.emplace_back(op);
   }
   return out;
@@ -951,6 +963,12 @@ void AMRMultiGrid<Real>::solveNoInitResid(std::vector<T*>& phi, std::vector<T*>&
   m_exitStatus = int(!goRedu) + int(!goIter) * 2 + int(!goHang) * 4 + int(!goNorm) * 8;
 }
 
+template void defineLevelOp<double>(AMRPoissonOp<double>*, Context*, const DeviceLayout*, const Box&, double, double,
+                                    double, double, const BCSpec&, const DeviceLayout*);
+template void defineLevelOp<float>(AMRPoissonOp<float>*, Context*, const DeviceLayout*, const Box&, double, double,
+                                   double, double, const BCSpec&, const DeviceLayout*);
+template void linkCoarser<double>(AMRPoissonOp<double>*, AMRPoissonOp<double>*, const DeviceLayout*);
+template void linkCoarser<float>(AMRPoissonOp<float>*, AMRPoissonOp<float>*, const DeviceLayout*);
 template class AMRPoissonOp<double>;
 template class AMRPoissonOp<float>;
 template class AMRPoissonOpFactory<double>;

@@ -133,6 +133,18 @@ class AMRPoissonOp : public AMRLevelOp<LevelData<Real>> {
   bool tuneRelax(const T& phi, const T& rhs, double* times);
 };
 
+// AMRPoissonOp::define on one level (ghost rules from the level's own CFRegion when
+// dxCrse > 0, the one-launch relaxation where it applies); mgCoarse: the multigrid
+// child layout coarsen(level, 2) or null.
+template <class Real>
+void defineLevelOp(AMRPoissonOp<Real>* op, Context* ctx, const DeviceLayout* lay, const Box& domain, double dx,
+                   double dxCrse, double alpha, double beta, const BCSpec& bc, const DeviceLayout* mgCoarse);
+// The coarse-fine tables between op's level and the coarser AMR level of coarserOp
+// (ratio 2): QuadCFInterp, the restriction / prolongation copiers through cfLay =
+// coarsen(op's level, 2), and coarserOp's reflux tables and zeroCovered plan.
+template <class Real>
+void linkCoarser(AMRPoissonOp<Real>* op, AMRPoissonOp<Real>* coarserOp, const DeviceLayout* cfLay);
+
 // Device layouts of one AMR hierarchy: AMR levels, the multigrid levels of level
 // 0, and the coarsened-fine layouts used for restriction/prolongation between
 // AMR levels.  Shared by the operator factories of every precision.

@@ -15,31 +15,16 @@
 #include <vector>
 
 #include "amrpoissonop.hpp"
-#include "mixed.hpp"
 #include "batch.hpp"
+#include "handles.hpp"
+#include "mixed.hpp"
 
 using namespace amrmg;
 
-static thread_local std::string g_err;
+thread_local std::string amrmg_g_err;
 
-extern "C" const char* amrmg_last_error(void) { return g_err.c_str(); }
+extern "C" const char* amrmg_last_error(void) { return amrmg_g_err.c_str(); }
 
-#define API_BEGIN try {
-#define API_END                    \
-  }                                \
-  catch (const std::exception& e) { \
-    g_err = e.what();              \
-    return -1;                     \
-  }                                \
-  catch (...) {                    \
-    g_err = "unknown error";       \
-    return -1;                     \
-  }                                \
-  return 0;
-
-struct amrmg_ctx {
-  std::unique_ptr<Context> c;
-};
 // A batch runs in one or more lanes: each lane is a BatchSolver over a subset of the
 // problems, and lanes after the first run on streams of their own, so their lockstep
 // iterations overlap on the device (problem results do not depend on the lanes).
@@ -533,6 +518,33 @@ int amrmg_average(amrmg_solver* s, int lev, const double* fine, double* coarse) 
   download_valid(*Lc, c.d, coarse);
   API_END
 }
+// BiCGStabSolver's public parameters of every bottom solve of the solver
+static void copy_bottom_params(BiCGStabSolver<LevelData<double>>& to, const BiCGStabSolver<LevelData<double>>& from) {
+  to.m_imax = from.m_imax;
+  to.m_numRestarts = from.m_numRestarts;
+  to.m_reps = from.m_reps;
+  to.m_hang = from.m_hang;
+  to.m_small = from.m_small;
+}
+int amrmg_solver_set_bottom_params(amrmg_solver* s, const int* ip, const double* rp) {
+  API_BEGIN
+  if (ip[0] < 0 || ip[1] < 0) throw std::runtime_error("BiCGStab: imax and numRestarts must be >= 0");
+  BiCGStabSolver<LevelData<double>>& f = s->bottom->m_fallback;
+  f.m_imax = ip[0];
+  f.m_numRestarts = ip[1];
+  f.m_reps = rp[0];
+  f.m_hang = rp[1];
+  f.m_small = rp[2];
+  if (s->mixed) {
+    copy_bottom_params(s->mixed->bottom.m_bicg, f);
+    if (s->mixed->devBottom) {
+      auto& g = s->mixed->devBottom->m_fallback;
+      g.m_imax = f.m_imax; g.m_numRestarts = f.m_numRestarts; g.m_reps = f.m_reps; g.m_hang = f.m_hang;
+      g.m_small = f.m_small;
+    }
+  }
+  API_END
+}
 int amrmg_bottom_solve(amrmg_solver* s, double* corr, const double* res, int nbottom, double metric, double eps,
                        int* info) {
   API_BEGIN
@@ -543,12 +555,37 @@ int amrmg_bottom_solve(amrmg_solver* s, double* corr, const double* res, int nbo
   upload_valid(*L, r.d, res);
   AMRPoissonOp<double>* op = s->mgop(d);
   BiCGStabSolver<LevelData<double>> b;
+  copy_bottom_params(b, s->bottom->m_fallback);
   b.define(op, true);
   b.setConvergenceMetrics(metric, eps);
   op->relax(c, r, nbottom);
   b.solve(c, r);
   download_valid(*L, c.d, corr);
-  if (info) info[0] = b.m_exitStatus;
+  if (info) {
+    info[0] = b.m_exitStatus;
+    info[1] = b.m_iter;
+  }
+  API_END
+}
+int amrmg_bottom_solve_device(amrmg_solver* s, double* corr, const double* res, int nbottom, double metric,
+                              double eps, int* info) {
+  API_BEGIN
+  int d = s->H.mgDepth() - 1;
+  const DeviceLayout* L = s->mglay(d);
+  AMRPoissonOp<double>* op = s->mgop(d);
+  DeviceBottomSolver<double> b(s->ctx->c.get());
+  copy_bottom_params(b.m_fallback, s->bottom->m_fallback);
+  b.define(op, true);
+  if (!b.onDevice()) throw std::runtime_error("the deepest multigrid level does not allow the device bottom solver");
+  LevelData<double> c(L), r(L);
+  upload_valid(*L, c.d, corr);
+  upload_valid(*L, r.d, res);
+  b.setConvergenceMetrics(metric, eps);
+  op->relax(c, r, nbottom);
+  b.solve(c, r);
+  const long long it = b.totalIterations();
+  download_valid(*L, c.d, corr);
+  if (info) info[0] = (int)it;
   API_END
 }
 
@@ -651,7 +688,7 @@ struct amrmg_heat {
     if (op(l)->m_vc) vc::scale_by_a<double>(*x.lay, *op(l)->m_vc, x.d, divide);
   }
   void setAlphaBeta(double alpha, double beta) {
-    if (amrmg_solver_set_alpha_beta(s, alpha, beta) != 0) throw std::runtime_error(g_err);
+    if (amrmg_solver_set_alpha_beta(s, alpha, beta) != 0) throw std::runtime_error(amrmg_g_err);
   }
   // AMRMultiGrid::solveNoInit (zeroPhi as given) on the solver's phi with rhs rhst
   void solve(bool zeroPhi, int* info) {

@@ -364,9 +364,14 @@ __global__ void __launch_bounds__(256) k_fill_ghosts(const DBox* __restrict__ bo
                                                      int doCF, int homog, double dx, double dxCrse) {
   const PatchWork w = work[blockIdx.x];
   const Patch p = patches[w.patch];
+  // doCF: 0 exchange and BC, 1 exchange, BC and coarse-fine, 2 coarse-fine only,
+  // 3 exchange only, 4 BC only
   const bool isCF = (p.kind == PK_CFHOMO || p.kind == PK_CFHOMOLIN);
-  if (isCF && !doCF) return;
-  if (!isCF && doCF == 2) return;  // coarse-fine rules only
+  const bool isEx = p.kind == PK_EXCHANGE;
+  if (isCF && doCF != 1 && doCF != 2) return;
+  if (!isCF && doCF == 2) return;
+  if (!isEx && doCF == 3) return;
+  if ((isEx || isCF) && doCF == 4) return;
   const int nx = p.hi[0] - p.lo[0] + 1, ny = p.hi[1] - p.lo[1] + 1, nz = p.hi[2] - p.lo[2] + 1;
   const int q = w.start + threadIdx.x;
   if (q >= nx * ny * nz) return;

@@ -41,6 +41,8 @@ struct MotionPlan {  // Copier between two layouts
   void release();
 };
 
+// doCF: 0 exchange and BC rules, 1 exchange, BC and homogeneous coarse-fine, 2
+// coarse-fine only, 3 exchange only, 4 BC only
 template <class T>
 void fill_ghosts(const DeviceLayout& L, const GhostPlan& P, T* phi, int doCF, int homog, double dx,
                  double dxCrse);

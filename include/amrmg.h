@@ -126,9 +126,20 @@ int amrmg_homog_cf(amrmg_solver* s, int lev, double* phi_ghosted);
 int amrmg_quad_cf(amrmg_solver* s, int lev, double* phi_ghosted, const double* coarse_valid);
 /* FORT_AVERAGE of level lev onto coarsen(grids[lev], 2) */
 int amrmg_average(amrmg_solver* s, int lev, const double* fine, double* coarse);
-/* MultiGrid bottom: relax(nbottom) + BiCGStabSolver::solve on the deepest MG level */
+/* MultiGrid bottom: relax(nbottom) + BiCGStabSolver::solve on the deepest MG level
+   (host-driven BiCGStab); info = {exit status, iterations} */
 int amrmg_bottom_solve(amrmg_solver* s, double* corr, const double* res, int nbottom, double metric, double eps,
                        int* info);
+/* The same on the device bottom solver the solves use (one workgroup, no host round
+   trip per iteration; the deepest multigrid level must be stored as one array over
+   its whole domain, with constant coefficients); info = {iterations} */
+int amrmg_bottom_solve_device(amrmg_solver* s, double* corr, const double* res, int nbottom, double metric,
+                              double eps, int* info);
+/* BiCGStabSolver's public parameters (BiCGStabSolver.H:105-153) for every bottom solve of
+   the solver (device and host-driven, FP64 and mixed, and amrmg_bottom_solve*):
+   ip = {m_imax, m_numRestarts}, rp = {m_reps, m_hang, m_small}; the defaults are the
+   reference constructor's (BiCGStabSolver.H:166-183): 80, 5, 1e-12, 1e-8, 1e-30. */
+int amrmg_solver_set_bottom_params(amrmg_solver* s, const int* ip, const double* rp);
 
 /* Autotuner of the one-launch relaxation: on every level that uses it, times its
    10 tile shapes (columns 64x8, 32x8, 64x4, 32x16, 128x4, 64x16 of 256 threads;
@@ -235,6 +246,138 @@ int amrmg_serve_wait(amrmg_serve* e, long long id, double* phi, double* times, i
 /* {max HBM bytes held, hierarchies created, reused, evicted, max requests running at once} */
 int amrmg_serve_stats(amrmg_serve* e, long long* stats, int n);
 int amrmg_serve_destroy(amrmg_serve* e);
+
+/* ---- operator level: device-resident levels, fields, operators and copiers ----
+   The reference's operator plug-in API is AMRPoissonOp, a LinearOp / MGLevelOp /
+   AMRLevelOp over LevelData<FArrayBox> (LinearSolver.H:27-149, MultiGrid.H:88-287,
+   AMRMultiGrid.H:38-226) that Chombo's AMRMultiGrid, MultiGrid and BiCGStabSolver
+   drive.  These entry points give a host operator class the same operations on
+   device-resident data, one call per virtual:
+     amrmg_level  a DisjointBoxLayout with its ProblemDomain, resident in HBM;
+     amrmg_field  a LevelData<FArrayBox> on a level (one component, one ghost cell);
+     amrmg_op     one AMRPoissonOp (alpha I + beta Laplacian) on a level;
+     amrmg_plan   a Copier between two levels (copyTo, copyToZero) or an exchange.
+   Every operation is queued on the context's stream in call order; those returning
+   a host scalar (dot, norm) wait for it.  Operands are checked against the level
+   each operation assumes (a field on another level is an error). */
+typedef struct amrmg_level amrmg_level;
+typedef struct amrmg_field amrmg_field;
+typedef struct amrmg_op amrmg_op;
+typedef struct amrmg_plan amrmg_plan;
+
+/* DisjointBoxLayout::define + ProblemDomain (DisjointBoxLayout.H:30, ProblemDomain.H:141):
+   boxes 6 ints each (lo0 lo1 lo2 hi0 hi1 hi2), sorted by Box::operator<, disjoint;
+   domain 6 ints.  merge = 1 stores a level whose boxes tile a rectangle as one array
+   (then a box's ghost cell inside another box is that box's valid cell, as after an
+   exchange); 0 gives every box its own FAB. */
+int amrmg_level_register(amrmg_ctx* ctx, const int* boxes, int nbox, const int* domain, int merge,
+                         amrmg_level** out);
+int amrmg_level_destroy(amrmg_level* L);
+/* coarsen(DisjointBoxLayout, 2) (DisjointBoxLayout.H:354) of a level whose boxes are
+   coarsenable by 2, with the coarsened domain */
+int amrmg_level_coarsen(amrmg_level* L, amrmg_level** out);
+/* valid cells (ghosted = 0) or cells of the ghosted FABs (1), summed over boxes */
+long long amrmg_level_cells(amrmg_level* L, int ghosted);
+
+/* LevelData<FArrayBox>(grids, 1, IntVect::Unit), every cell zero */
+int amrmg_field_create(amrmg_level* L, amrmg_field** out);
+int amrmg_field_destroy(amrmg_field* f);
+/* FArrayBox-order transfers, box by box in layout order: ghosted = 0 moves each
+   box's valid cells (Fortran order), 1 each box's FAB (the box grown by one cell;
+   on a merged level upload skips ghost cells that are another box's valid cells) */
+int amrmg_field_upload(amrmg_field* f, const double* host, int ghosted);
+int amrmg_field_download(amrmg_field* f, double* host, int ghosted);
+
+/* LevelDataOps<FArrayBox> (LevelDataOps.H:93-294) on valid cells: assignLocal (every
+   cell of the FABs), incr (lhs += x*scale), axby (lhs = x*a + y*b), scale, setVal
+   (setToZero is setVal 0); dotProduct (per box in Fortran order, boxes in layout
+   order) and norm(BoxLayoutData, p) (BoxLayoutData.cpp:23-92; p = 0 max, 1, 2) */
+int amrmg_field_assign(amrmg_field* lhs, const amrmg_field* rhs);
+int amrmg_field_incr(amrmg_field* lhs, const amrmg_field* x, double scale);
+int amrmg_field_axby(amrmg_field* lhs, const amrmg_field* x, const amrmg_field* y, double a, double b);
+int amrmg_field_scale(amrmg_field* x, double s);
+int amrmg_field_setval(amrmg_field* x, double v);
+int amrmg_field_dot(const amrmg_field* a, const amrmg_field* b, double* out);
+int amrmg_field_norm(const amrmg_field* x, int p, double* out);
+
+/* AMRPoissonOp::define (AMRPoissonOp.cpp:60-221) on level L: dx, dxCrse (> 0: the
+   level has a coarser level, and homogeneousCFInterp fills its coarse-fine ghosts
+   from its own CFRegion; <= 0: no coarse-fine faces), alpha, beta, and
+   ConstDiriNeumBC-style faces bc_type[6] / bc_val[6] (lo x, y, z, then hi; 0 Neumann,
+   1 Dirichlet linear, 2 Dirichlet quadratic).  coarser: the operator of the next
+   coarser AMR level (ratio 2) or NULL; with it the operator gets QuadCFInterp,
+   AMRRestrictS / AMRProlongS through coarsen(L, 2), and the coarser operator gets
+   this level's reflux registers and zeroCovered (as AMRnewOp wires them).  An
+   operator must be destroyed before its coarser one. */
+int amrmg_op_create(amrmg_level* L, amrmg_op* coarser, double dx, double dxCrse, double alpha, double beta,
+                    const int* bc_type, const double* bc_val, amrmg_op** out);
+/* MGnewOp at the next depth (AMRPoissonOp.cpp:1923-1989): an operator on coarsen(L, 2)
+   with 2 dx, the same dxCrse and coefficients, whose level is *level_out (owned by
+   the new operator); the parent's restrictResidual / prolongIncrement target it */
+int amrmg_op_create_coarser(amrmg_op* op, amrmg_op** out, amrmg_level** level_out);
+/* the level of coarsen(L, 2) of an operator with a coarser AMR level
+   (createCoarsened: resC, AMRRestrictS / AMRProlongS temporaries), owned by op */
+int amrmg_op_coarsened_level(amrmg_op* op, amrmg_level** out);
+int amrmg_op_destroy(amrmg_op* op);
+/* AMRPoissonOp::setAlphaAndBeta (AMRPoissonOp.cpp:1103-1108) */
+int amrmg_op_set_alpha_beta(amrmg_op* op, double alpha, double beta);
+/* s_relaxMode of this operator (0..5, as amrmg_solver_set_relax_mode) */
+int amrmg_op_set_relax_mode(amrmg_op* op, int mode);
+
+/* LinearOp / MGLevelOp virtuals of AMRPoissonOp: residual (:224-236), applyOp
+   (:332-340), preCond (:304-329), relax (:600-632), restrictResidual (:651-703),
+   prolongIncrement (:706-733) */
+int amrmg_op_residual(amrmg_op* op, amrmg_field* lhs, amrmg_field* phi, const amrmg_field* rhs, int homogeneous);
+int amrmg_op_apply(amrmg_op* op, amrmg_field* lhs, amrmg_field* phi, int homogeneous);
+int amrmg_op_precond(amrmg_op* op, amrmg_field* cor, const amrmg_field* residual);
+int amrmg_op_relax(amrmg_op* op, amrmg_field* correction, const amrmg_field* residual, int iterations);
+int amrmg_op_restrict_residual(amrmg_op* op, amrmg_field* res_coarse, amrmg_field* phi, const amrmg_field* rhs);
+int amrmg_op_prolong_increment(amrmg_op* op, amrmg_field* phi, const amrmg_field* correct_coarse);
+/* AMRLevelOp virtuals: AMRResidual / AMROperator with their NC (phi_coarse NULL) and
+   NF (phi_fine NULL) forms (:736-858); AMRRestrictS (:877-919); AMRProlongS
+   (:959-990); AMRUpdateResidual (:1045-1054, coarse_correction NULL on the base
+   level); assignCopier (:486-492: lhs on this level <- rhs on the finer operator's
+   coarsened level); zeroCovered (:495-502); localMaxNorm (:565-576) */
+int amrmg_op_amr_residual(amrmg_op* op, amrmg_field* res, amrmg_field* phi_fine, amrmg_field* phi,
+                          amrmg_field* phi_coarse, const amrmg_field* rhs, int homogeneous, amrmg_op* finer);
+int amrmg_op_amr_operator(amrmg_op* op, amrmg_field* lphi, amrmg_field* phi_fine, amrmg_field* phi,
+                          amrmg_field* phi_coarse, int homogeneous, amrmg_op* finer);
+int amrmg_op_amr_restrict(amrmg_op* op, amrmg_field* res_coarse, const amrmg_field* residual,
+                          amrmg_field* correction, amrmg_field* coarse_correction, amrmg_field* scratch);
+int amrmg_op_amr_prolong(amrmg_op* op, amrmg_field* correction, const amrmg_field* coarse_correction,
+                         amrmg_field* temp);
+int amrmg_op_amr_update_residual(amrmg_op* op, amrmg_field* residual, amrmg_field* correction,
+                                 amrmg_field* coarse_correction);
+int amrmg_op_assign_copier(amrmg_op* op, amrmg_field* lhs, const amrmg_field* rhs);
+int amrmg_op_zero_covered(amrmg_op* op, amrmg_field* lhs);
+int amrmg_op_max_norm(amrmg_op* op, const amrmg_field* x, double* out);
+/* the ghost-cell and box kernels the virtuals are made of: exchange (exchangeNoOverlap
+   with the trimmed Copier), the BC functor on every box (homogeneous or with the face
+   values), homogeneousCFInterp (INTERPHOMO / INTERPHOMOLINEAR), QuadCFInterp::
+   coarseFineInterp from the coarser operator's level, one FORT_GSRBLAPLACIAN /
+   FORT_GSRBHELMHOLTZ colour pass on every box with the ghosts as they are, and
+   reflux (:1117-1201) of lphi with the finer level's phi */
+int amrmg_op_exchange(amrmg_op* op, amrmg_field* phi);
+int amrmg_op_bc(amrmg_op* op, amrmg_field* phi, int homogeneous);
+int amrmg_op_cf_homog(amrmg_op* op, amrmg_field* phi);
+int amrmg_op_cf_inhomog(amrmg_op* op, amrmg_field* phi, const amrmg_field* phi_coarse);
+int amrmg_op_gsrb(amrmg_op* op, amrmg_field* phi, const amrmg_field* rhs, int color);
+int amrmg_op_reflux(amrmg_op* op, amrmg_field* phi_fine, amrmg_field* phi, amrmg_field* lphi, amrmg_op* finer);
+
+/* Copier (Copier.H:36-57, Copier.cpp:286-330) from level src to level dst: items are
+   MotionItems of 8 ints (from box, to box, region lo0..2, hi0..2) in the order the
+   copies run, or NULL to build Copier::define(src, dst) here (every intersection of a
+   src box with a dst box, dst boxes in order).  amrmg_plan_exchange is the exchange
+   Copier of one level (exchangeDefine + trimEdges, Copier.cpp:1464-1541) from items or
+   NULL.  Regions must lie in the from box's valid cells and the to box's FAB. */
+int amrmg_plan_copy(amrmg_level* src, amrmg_level* dst, const int* items, int nitems, amrmg_plan** out);
+int amrmg_plan_exchange(amrmg_level* L, const int* items, int nitems, amrmg_plan** out);
+/* LevelData::copyTo (op 0, dst <- src over each item's region; LevelDataI.H:221-253)
+   or LevelDataOps::copyToZero (op 1, dst = 0 there, src unused; LevelDataOps.H:231-252) */
+int amrmg_plan_exec(amrmg_plan* p, amrmg_field* dst, const amrmg_field* src, int op);
+int amrmg_plan_destroy(amrmg_plan* p);
+/* wait for every queued operation of the context */
+int amrmg_ctx_sync(amrmg_ctx* ctx);
 
 /* Plain device copy of nbytes (reads nbytes, writes nbytes), iters launches;
    average ms per launch.  Calibrates HBM byte counters and the copy bandwidth. */

@@ -61,7 +61,7 @@ def test_vc_bottom_solve_bitwise(gpu_ctx):
     rng = np.random.default_rng(3)
     res = rng.standard_normal(o.mg_cells(d))
     c0 = np.zeros_like(res)
-    cg, sg = g.bottom_solve(c0, res, 2, 1.0, 1e-10)
+    cg, sg, _ = g.bottom_solve(c0, res, 2, 1.0, 1e-10)
     co, so = o.bottom_solve(c0, res, 2, 1.0, 1e-10)
     assert sg == so
     np.testing.assert_array_equal(cg, co)
