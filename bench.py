@@ -11,9 +11,8 @@ Prints one JSON line.  value = Mcell-relax/s (cells x red-black GSRB iterations
 over every level relaxed, per second of solve wall time).  roofline = the GSRB
 kernel's algorithmic bytes (24 B per cell-relax in FP64) over its HIP-event time
 inside the timed region, against 8 TB/s HBM.  cpu_baseline = the oracle (a C
-restatement of the reference's FP64 path, OpenMP over boxes) timed on a bounded
-sample of the same workload: setup excluded, initial residual + four AMR V-cycles
-with their residuals.
+restatement of the reference's FP64 path, OpenMP over boxes) built for the host CPU
+and timed on one full solve of the same workload (setup excluded).
 """
 import argparse
 import json
@@ -26,7 +25,6 @@ sys.path.insert(0, os.path.join(ROOT, "chombo_3.2_amd"))
 
 METRIC = "Mcell-relax/s & %HBM roofline; solves/s + p99 latency at 1e-10, MP vs FP64"
 PEAK_HBM_GBS = 8000.0
-CPU_VCYCLES = 4
 
 
 def log(msg):
@@ -90,27 +88,119 @@ def summarize_pmc(fetch_csv, write_csv, fetch_json, copy_bytes, out_path):
     print(json.dumps(d))
 
 
-def cpu_baseline(hier, rhs, sp, rp):
+def _oracle_native():
+    """The oracle built for this host's CPU (gcc -O3 -march=native -ffp-contract=off,
+    the parity build's arithmetic); None if the compiler is unavailable (then the
+    parity build, -O2, is timed)."""
+    import tempfile
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    from oracle import Oracle, build as build_oracle, _LIB_PATH
-    if not os.path.exists(_LIB_PATH):
-        build_oracle()
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    t0 = time.time()
-    o = Oracle(hier)
-    t_setup = time.time() - t0
-    sp1 = sp.copy()
-    sp1[4] = CPU_VCYCLES  # initial residual, CPU_VCYCLES AMR V-cycles, residuals
-    Oracle.reset_counters()
-    t0 = time.time()
-    o.solve(rhs, sp1, rp)
-    dt = time.time() - t0
-    cr = Oracle.cell_relax_count()
+    import oracle as ORC
+    try:
+        path = ORC.build_native(os.path.join(tempfile.mkdtemp(prefix="oracle_native_"), "liboracle_native.so"))
+        ORC.use_library(path)
+        return "gcc -O3 -march=native -ffp-contract=off -fopenmp, built on this host"
+    except Exception as e:  # noqa: BLE001
+        log(f"native oracle build failed ({e}); timing the parity build")
+        if not os.path.exists(ORC._LIB_PATH):
+            ORC.build()
+        return "gcc -O2 -ffp-contract=off -fopenmp (parity build)"
+
+
+def _cpu_solve(hier, rhs, sp, rp, threads):
+    """One full solve on the CPU oracle: (seconds, cell-relaxations, info); setup excluded."""
+    import oracle as ORC
+    ORC.set_threads(threads)
+    o = ORC.Oracle(hier)
+    ORC.Oracle.reset_counters()
+    t0 = time.perf_counter()
+    _, hist, info = o.solve(rhs, sp, rp)
+    dt = time.perf_counter() - t0
+    cr = ORC.Oracle.cell_relax_count()
     o.close()
-    return dict(value=cr / dt / 1e6, unit="Mcell-relax/s", cores=threads, kind="port",
-                sample=f"same 3-level 256^3 hierarchy and rhs; initial residual + {CPU_VCYCLES} AMR V-cycles with residuals "
-                       f"({cr / 1e6:.1f} M cell-relaxations in {dt:.2f} s; oracle setup {t_setup:.1f} s "
-                       f"excluded); OMP threads={threads}")
+    return dt, cr, info
+
+
+def cpu_baseline(args, hier, rhs, sp, rp, c1):
+    """The oracle (a C restatement of the reference's FP64 path, OpenMP over boxes like
+    the reference) on the host cores: one full solve of the benchmark's workload to
+    1e-10 from zero, setup excluded.  With --cpu-full, BASELINE.md's protocol: one
+    warm-up, then the median of 5 solves on all cores, one solve on 1 core, and the
+    single-level 64^3 workload the same way."""
+    build = _oracle_native()
+    import oracle as ORC
+    threads = int(os.environ.get("OMP_NUM_THREADS", 0)) or ORC.get_threads()
+    dt, cr, info = _cpu_solve(hier, rhs, sp, rp, threads)
+    out = dict(value=cr / dt / 1e6, unit="Mcell-relax/s", cores=threads, kind="port",
+               sample=f"one full solve of the same {hier.nlev}-level {hier.domain[1][0] + 1}^3 workload to 1e-10 "
+                      f"({info['iterations']} V-cycles, "
+                      f"{cr / 1e6:.1f} M cell-relaxations in {dt:.2f} s; oracle setup excluded); {build}",
+               solves_per_s=round(1.0 / dt, 5), build=build)
+    if args.cpu_full:
+        import numpy as np
+        times = []
+        _cpu_solve(hier, rhs, sp, rp, threads)  # warm-up
+        for _ in range(5):
+            times.append(_cpu_solve(hier, rhs, sp, rp, threads)[0])
+        t1, cr1, _ = _cpu_solve(hier, rhs, sp, rp, 1)
+        ch, crhs, csp, crp = c1
+        c1_all = [_cpu_solve(ch, crhs, csp, crp, threads)[0] for _ in range(6)][1:]
+        c1_one = _cpu_solve(ch, crhs, csp, crp, 1)[0]
+        out["protocol"] = {
+            "c2_solves_per_s_all_cores_median5": round(1.0 / float(np.median(times)), 5),
+            "c2_mcell_relax_per_s_all_cores": round(cr / float(np.median(times)) / 1e6, 2),
+            "c2_solves_per_s_1core": round(1.0 / t1, 6),
+            "c2_mcell_relax_per_s_1core": round(cr1 / t1 / 1e6, 2),
+            "c1_solves_per_s_all_cores_median5": round(1.0 / float(np.median(c1_all)), 3),
+            "c1_solves_per_s_1core": round(1.0 / c1_one, 3),
+            "threads": threads,
+            "cpu": _cpu_model(),
+        }
+    ORC.set_threads(threads)
+    return out
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def c1_leg(args, ctx):
+    """The reference's CPU-runnable case on the GPU: single-level 64^3 Poisson,
+    8 boxes of 32^3, Dirichlet 0, rhs = sin 2pi x sin 2pi y sin 2pi z + N(0, 1e-3)
+    (seed 1001), V(2,2) cycles with a BiCGStab bottom on 4^3, tol 1e-10.  At 0.26 M
+    cells the solve is launch- and latency-bound on the GPU."""
+    import amrmg
+    import hierarchy as H
+    h = H.single_level(64, 32, 8)
+    rhs = H.sinusoid_rhs(h, 1001)
+    sp, rp = H.solver_params(eps=1e-10, hang=1e-15, iterMax=100)
+    s = amrmg.AMRMultiGrid(ctx, h)
+    s.setSolverParameters(sp, rp)
+    s.set_field(s.RHS, rhs)
+    for _ in range(3):
+        s.solve()
+    n = max(20, args.steps)
+    ctx.profile(["gsrb"])
+    ctx.profile_reset()
+    t0 = time.perf_counter()
+    for _ in range(n):
+        hist, info = s.solve()
+    wall = time.perf_counter() - t0
+    prof = ctx.profile_read("gsrb")
+    ctx.profile([])
+    s.close()
+    out = {"workload": "single-level 64^3 Poisson, 8 boxes of 32^3, Dirichlet 0, sin*sin*sin + N(0,1e-3) rhs "
+                       "seed 1001, V(2,2) + BiCGStab bottom, tol 1e-10, FP64",
+           "ms_per_solve": round(wall / n * 1e3, 3), "solves_per_s": round(n / wall, 2),
+           "iterations": info["iterations"], "final_rel_residual": float(hist[-1] / hist[0]),
+           "mcell_relax_per_s": round(prof["units"] / wall / 1e6, 2)}
+    log(f"c1: {out}")
+    return out, (h, rhs, sp, rp)
 
 
 def mixed_leg(args, ctx, hier, rhs, sp, rp, solver64, fp64_s_per_step):
@@ -437,6 +527,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--base", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="CPU baseline protocol: median of 5 full solves on all cores, one on 1 core, and the 64^3 case")
+    ap.add_argument("--no-c1", action="store_true", help="skip the single-level 64^3 leg")
     ap.add_argument("--iter-max", type=int, default=100,
                     help="cap on V-cycles per solve (a short cap is for profiling runs only)")
     ap.add_argument("--no-tune", action="store_true", help="skip the relaxation autotuner (default shapes)")
@@ -534,6 +627,9 @@ def main():
     if args.serve > 0:
         solver.close()
         serve = serve_leg(args)
+    c1, c1case = None, None
+    if not args.no_c1:
+        c1, c1case = c1_leg(args, ctx)
     copy_gbs = None
     if args.copy_mb > 0:
         nbytes = args.copy_mb << 20
@@ -543,9 +639,13 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        log("timing the CPU oracle on a bounded sample ...")
-        cpu = cpu_baseline(hier, rhs, sp, rp)
-        log(f"cpu baseline {cpu['value']:.1f} Mcell-relax/s")
+        log("timing the CPU oracle (one full solve) ...")
+        if c1case is None:
+            import hierarchy as H
+            ch = H.single_level(64, 32, 8)
+            c1case = (ch, H.sinusoid_rhs(ch, 1001), *H.solver_params(eps=1e-10, hang=1e-15, iterMax=100))
+        cpu = cpu_baseline(args, hier, rhs, sp, rp, c1case)
+        log(f"cpu baseline {cpu['value']:.1f} Mcell-relax/s, {cpu['solves_per_s']:.4f} solves/s")
 
     out = {
         "metric": METRIC,
@@ -592,6 +692,7 @@ def main():
         "heat": heat,
         "batched": batch,
         "serving": serve,
+        "single_level_64": c1,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -2495,3 +2495,19 @@ int oracle_cf_branch_counts(OHier* h, int lev, long* counts) {
     }
   return 0;
 }
+
+/* OpenMP threads of every later oracle call (the CPU baseline times all cores and one) */
+void oracle_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+int oracle_get_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}

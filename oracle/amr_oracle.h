@@ -115,6 +115,10 @@ int  oracle_amrlevel_relax(OHier* h, int lev, double* phi, const double* rhs, in
    documented at the definition) */
 int  oracle_cf_branch_counts(OHier* h, int lev, long* counts);
 
+/* OpenMP threads of every later call (n > 0), and the current count */
+void oracle_set_threads(int n);
+int  oracle_get_threads(void);
+
 /* number of cell-relaxations (one red+black GSRB iteration of one cell) since reset */
 long long oracle_cell_relax_count(void);
 void oracle_reset_counters(void);

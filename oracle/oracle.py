@@ -23,6 +23,29 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
+def build_native(out_path):
+    """Builds the CPU-baseline timing variant (-O3 -march=native, no contraction, no
+    fast-math: the parity build's arithmetic) for the CPU this runs on."""
+    subprocess.run(["make", "-s", "-B", "-C", _HERE, "native", f"NATIVE_OUT={out_path}"], check=True)
+    return out_path
+
+
+def use_library(path):
+    """Load the oracle from another build of the same source (before the first call)."""
+    global _LIB_PATH, _lib
+    if _lib is not None and path != _LIB_PATH:
+        raise RuntimeError("the oracle library is already loaded")
+    _LIB_PATH = path
+
+
+def set_threads(n):
+    lib().oracle_set_threads(int(n))
+
+
+def get_threads():
+    return int(lib().oracle_get_threads())
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -68,6 +91,7 @@ def lib():
         L.oracle_amrlevel_residual.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, C.c_int]
         L.oracle_amrlevel_relax.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.c_int]
         L.oracle_cf_branch_counts.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_long)]
+        L.oracle_set_threads.argtypes = [C.c_int]
         _lib = L
     return _lib
 

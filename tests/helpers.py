@@ -71,20 +71,12 @@ def uncovered_mask(h, lev):
     """Valid-packed mask of level-lev cells not covered by level lev+1."""
     if lev + 1 >= h.nlev:
         return np.ones(h.cells(lev), dtype=bool)
-    cov = [H.box_coarsen(b, 2) for b in h.levels[lev + 1]]
-    parts = []
-    for b in h.levels[lev]:
-        (l, hh) = b
-        idx = [np.arange(l[d], hh[d] + 1) for d in range(3)]
-        I = np.meshgrid(*idx, indexing="ij")
-        c = np.zeros(I[0].shape, dtype=bool)
-        for cb in cov:
-            m = np.ones_like(c)
-            for d in range(3):
-                m &= (I[d] >= cb[0][d]) & (I[d] <= cb[1][d])
-            c |= m
-        parts.append((~c).ravel(order="F"))
-    return np.concatenate(parts)
+    dom = h.level_domain(lev)
+    cov = np.zeros(H.box_shape(dom), dtype=bool)
+    sl = lambda b: tuple(slice(b[0][d] - dom[0][d], b[1][d] - dom[0][d] + 1) for d in range(3))
+    for b in h.levels[lev + 1]:
+        cov[sl(H.box_coarsen(b, 2))] = True
+    return np.concatenate([(~cov[sl(b)]).ravel(order="F") for b in h.levels[lev]])
 
 
 def small_single(n=32, box=16, block=8, lo=0, alpha=0.0, beta=1.0, bc=(1, 1, 1)):
