@@ -549,12 +549,15 @@ class ServeEngine:
         return i.value
 
     def wait(self, rid, want_phi=True):
-        n = self._size.pop(rid)
+        n = self._size.get(rid)
+        if n is None:
+            raise AMRMGError("serve: unknown request id")
         phi = np.zeros(n) if want_phi else None
         t = np.zeros(3)
         info = np.zeros(2, np.int32)
         _chk(lib().amrmg_serve_wait(self.p, int(rid), phi.ctypes.data_as(_dp) if want_phi else None,
                                      t.ctypes.data_as(_dp), info.ctypes.data_as(_ip)))
+        self._size.pop(rid, None)
         return dict(phi=phi, t_arrival=t[0], t_admit=t[1], t_done=t[2], latency=t[2] - t[0],
                     iterations=int(info[0]), exit_status=int(info[1]))
 

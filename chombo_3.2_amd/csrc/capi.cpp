@@ -171,6 +171,19 @@ static std::vector<DBL> parse_spec(const int* is, const double* rs, Box& dom, BC
   return g;
 }
 
+// Bytes of the fields a solver holds on a hierarchy.  FP64 per AMR level: phi, rhs,
+// res, uber residual / correction, V-cycle correction, residual, resC and the
+// relaxation buffer; per multigrid level: correction, residual, relaxation buffer.
+// Mixed precision adds the FP32 copies of the V-cycle fields.
+static long long hierarchy_field_bytes(const DeviceHierarchy& H, int precision) {
+  long long n = 0, mg = 0;
+  for (auto& l : H.amrLay) n += l->slab;
+  for (auto& m : H.mgLay) mg += m->slab;
+  long long total = n * 9 * 8 + mg * 3 * 8;
+  if (precision == 1) total += n * 6 * 4 + mg * 3 * 4;
+  return total;
+}
+
 extern "C" {
 
 int amrmg_ctx_create(int device, amrmg_ctx** out) {
@@ -337,17 +350,7 @@ long long amrmg_solver_mg_cells(amrmg_solver* s, int depth) {
   return s->mglay(depth)->valid_cells;
 }
 long long amrmg_solver_device_bytes(amrmg_solver* s) {
-  long long n = 0;
-  for (int l = 0; l < s->nlev(); l++) n += s->H.amrLay[l]->slab;
-  // FP64 per AMR level: phi, rhs, res, uber residual/correction, V-cycle correction,
-  // residual, resC and the relaxation buffer; per multigrid level: correction,
-  // residual, relaxation buffer.  Mixed precision adds the FP32 copies of the
-  // V-cycle fields.
-  long long total = n * 9 * 8;
-  long long mg = 0;
-  for (auto& m : s->H.mgLay) mg += m->slab;
-  total += mg * 3 * 8;
-  if (s->mixed) total += n * 6 * 4 + mg * 3 * 4;
+  long long total = hierarchy_field_bytes(s->H, s->mixed ? 1 : 0);
   for (auto& v : s->vc) total += v->bytes();
   for (auto& v : s->vc32) total += v->bytes();
   return total;
@@ -664,6 +667,22 @@ int amrmg_solver_set_vc_average(amrmg_solver* s, int type) {
   API_BEGIN
   if (type != 0 && type != 1) throw std::runtime_error("coefficient average type must be 0 (arithmetic) or 1 (harmonic)");
   s->vcHarmonic = type;
+  if (!s->vc.empty()) {
+    // coefficients are already set: the multigrid depths take level 0's coefficients
+    // averaged the new way now (MGnewOp), with lambda and the FP32 copies redone
+    const int nd = s->H.mgDepth(), nl = s->nlev();
+    for (int d = 1; d < nd; d++) {
+      auto& v = s->vc[nl + d - 1];
+      vc::average_coefs<double>(*s->vc[0], *v, 1 << d, s->vcHarmonic);
+      s->mgop(d)->setVC(v);
+      if (s->mixed) {
+        auto& f = s->vc32[nl + d - 1];
+        vc::narrow(*v, *f);
+        s->mixedop(d)->setVC(f);
+      }
+    }
+    s->ctx->c->sync();
+  }
   API_END
 }
 int amrmg_solver_set_vc_relax_shape(amrmg_solver* s, int shape) {
@@ -992,15 +1011,13 @@ namespace {
 double serve_now(const std::chrono::steady_clock::time_point& t0) {
   return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
-// HBM footprint estimate of a solver for a spec (fields as amrmg_solver_device_bytes
-// counts them, a 1.1 ghost factor, multigrid levels as N0/7)
-long long serve_footprint(const std::vector<DBL>& g, int precision) {
-  double n = 0;
-  for (auto& d : g) n += (double)d.numCells();
-  const double n0 = (double)g[0].numCells();
-  double b = 1.1 * 8.0 * (9.0 * n + 3.0 * n0 / 7.0);
-  if (precision == 1) b += 1.1 * 4.0 * (6.0 * n + 3.0 * n0 / 7.0);
-  return (long long)b + (1LL << 20);
+// HBM footprint of a solver for a spec: its layouts built on the host only (a dry
+// context), counted as amrmg_solver_device_bytes counts a created solver's fields
+long long serve_footprint(const std::vector<DBL>& g, const Box& dom, double dx0, int precision) {
+  Context dry(0, true);
+  DeviceHierarchy H;
+  H.define(&dry, g, dom, dx0);
+  return hierarchy_field_bytes(H, precision);
 }
 struct ServeInstance {
   amrmg_ctx* ctx = nullptr;  // its own stream and reduction scratch
@@ -1022,8 +1039,14 @@ struct ServeReq {
   double tArrival = 0, tAdmit = 0, tDone = 0;
   int iters = 0, exitStatus = -1;
   bool done = false, taken = false;
+  int bypassed = 0;                          // times a later request was admitted before it
+  int sp[6] = {0};                           // solver parameters, copied at admission
+  double rp[3] = {0};
   std::string error;
 };
+// a request at the head of the queue that does not fit yet lets at most this many later
+// (fitting) requests be admitted before it; then admission waits for it
+constexpr int kServeMaxBypass = 8;
 }  // namespace
 
 struct amrmg_serve {
@@ -1033,7 +1056,8 @@ struct amrmg_serve {
   double rp[3] = {1e-10, 1e-15, 1e-30};
   std::mutex mu;
   std::condition_variable cv;
-  std::deque<ServeReq*> queue;                              // waiting for admission, FIFO
+  std::deque<ServeReq*> queue;                              // waiting for admission, in arrival order
+  std::unordered_map<std::string, long long> measured;     // device bytes of a created hierarchy, by spec
   std::unordered_map<long long, std::unique_ptr<ServeReq>> reqs;
   std::vector<std::unique_ptr<ServeInstance>> idle;        // device-resident, not in use
   long long bytesInUse = 0, bytesIdle = 0, maxBytes = 0;
@@ -1051,11 +1075,26 @@ struct amrmg_serve {
       std::vector<std::unique_ptr<ServeInstance>> evict;
       {
         std::unique_lock<std::mutex> lk(mu);
-        // admission: the head request, once its footprint fits beside the running ones
-        cv.wait(lk, [&] { return stop || (!queue.empty() && bytesInUse + queue.front()->bytes <= budget); });
+        // admission: the first request in arrival order whose footprint fits beside the
+        // running ones; a head that does not fit is passed at most kServeMaxBypass times
+        size_t pick = 0;
+        auto admissible = [&] {
+          if (queue.empty()) return false;
+          const size_t n = queue.front()->bypassed >= kServeMaxBypass ? 1 : queue.size();
+          for (size_t q = 0; q < n; q++)
+            if (bytesInUse + queue[q]->bytes <= budget) {
+              pick = q;
+              return true;
+            }
+          return false;
+        };
+        cv.wait(lk, [&] { return stop || admissible(); });
         if (stop) return;
-        r = queue.front();
-        queue.pop_front();
+        r = queue[pick];
+        if (pick > 0) queue.front()->bypassed++;
+        queue.erase(queue.begin() + pick);
+        std::memcpy(r->sp, sp, sizeof sp);
+        std::memcpy(r->rp, rp, sizeof rp);
         for (size_t q = 0; q < idle.size(); q++)
           if (idle[q]->key == r->key) {
             inst = std::move(idle[q]);
@@ -1088,12 +1127,20 @@ struct amrmg_serve {
           if (amrmg_ctx_create(device, &inst->ctx) || amrmg_solver_create(inst->ctx, r->ispec.data(), r->rspec.data(),
                                                                           precision, &inst->s))
             throw std::runtime_error(amrmg_last_error());
+          // account the hierarchy by what it holds from now on (and admit later
+          // requests of this spec by it)
+          const long long mb = amrmg_solver_device_bytes(inst->s);
           std::lock_guard<std::mutex> lk(mu);
           created++;
+          bytesInUse += mb - r->bytes;
+          r->bytes = mb;
+          inst->bytes = mb;
+          measured[r->key] = mb;
+          maxBytes = std::max(maxBytes, bytesInUse + bytesIdle);
         }
         double hist[256];
         int info[3];
-        if (amrmg_solver_set_params(inst->s, sp, rp) || amrmg_solver_set_field(inst->s, 1, r->rhs.data()) ||
+        if (amrmg_solver_set_params(inst->s, r->sp, r->rp) || amrmg_solver_set_field(inst->s, 1, r->rhs.data()) ||
             amrmg_solver_solve(inst->s, 1, hist, 256, info))
           throw std::runtime_error(amrmg_last_error());
         r->phi.resize(r->rhs.size());
@@ -1160,9 +1207,11 @@ int amrmg_serve_submit(amrmg_serve* e, const int* ispec, const double* rspec, co
   r->rhs.assign(rhs, rhs + ncell);
   r->key.assign((const char*)ispec, sizeof(int) * nint);
   r->key.append((const char*)rspec, sizeof(double) * 9);
-  r->bytes = serve_footprint(g, e->precision);
-  if (r->bytes > e->budget) throw std::runtime_error("serve: request larger than the HBM budget");
+  r->bytes = serve_footprint(g, dom, dx0, e->precision);
   std::lock_guard<std::mutex> lk(e->mu);
+  auto m = e->measured.find(r->key);
+  if (m != e->measured.end()) r->bytes = m->second;
+  if (r->bytes > e->budget) throw std::runtime_error("serve: request larger than the HBM budget");
   r->id = e->nextId++;
   r->tArrival = serve_now(e->t0);
   *id = r->id;
@@ -1180,6 +1229,8 @@ int amrmg_serve_wait(amrmg_serve* e, long long id, double* phi, double* times, i
   auto it = e->reqs.find(id);
   if (it == e->reqs.end()) throw std::runtime_error("serve: unknown request id");
   ServeReq* r = it->second.get();
+  if (r->taken) throw std::runtime_error("serve: another thread is already waiting for this request");
+  r->taken = true;
   e->cv.wait(lk, [&] { return r->done; });
   if (!r->error.empty()) {
     std::string m = r->error;

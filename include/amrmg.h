@@ -167,7 +167,8 @@ int amrmg_solver_set_relax_mode(amrmg_solver* s, int mode);
    reference (its GSRBHELMHOLTZVC runs per box and colour). */
 int amrmg_solver_set_vc_relax_shape(amrmg_solver* s, int shape);
 /* m_coefficient_average_type of VCAMRPoissonOp2Factory (VCAMRPoissonOp2.cpp:1048-1063)
-   for the multigrid coarsening of the next amrmg_solver_set_vc_coefs: 0
+   for the multigrid coarsening (re-averaged at once when coefficients are set, else
+   applied by the next amrmg_solver_set_vc_coefs): 0
    CoarseAverage::arithmetic (default), 1 harmonic (averageToCoarseHarmonic,
    AverageF.ChF AVERAGEHARMONIC and AverageFaceF.ChF AVERAGEFACEHARMONIC). */
 int amrmg_solver_set_vc_average(amrmg_solver* s, int type);
@@ -229,11 +230,15 @@ int amrmg_batch_set_graph(amrmg_batch* b, int on, long long* launches);
 
 /* ---- serving: a request loop with continuous admission.  nslots worker threads
    each run one request at a time on its own HIP stream, so the solves of
-   different requests overlap on the device.  A request is admitted (FIFO) as soon
-   as a slot is free and its HBM footprint estimate fits in the budget beside the
-   running ones; hierarchies stay device-resident after a solve and are reused by
-   later requests with the same spec (least recently used ones are evicted to make
-   room).  No counterpart in the reference. ---- */
+   different requests overlap on the device.  When a slot frees, the first request
+   in arrival order whose HBM footprint fits in the budget beside the running ones is
+   admitted; a head request that does not fit is passed by at most 8 later ones, then
+   admission waits for it.  The footprint is the bytes the request's solver holds
+   (amrmg_solver_device_bytes), computed from its layouts before it is created;
+   hierarchies stay device-resident after a solve and are reused by later requests
+   with the same spec (least recently used ones are evicted to make room).  Solver
+   parameters are those set when a request is admitted.  No counterpart in the
+   reference. ---- */
 typedef struct amrmg_serve amrmg_serve;
 int amrmg_serve_create(int device, int nslots, long long hbm_budget_bytes, int precision, amrmg_serve** out);
 int amrmg_serve_set_params(amrmg_serve* e, const int* sparams, const double* rparams);
@@ -241,7 +246,8 @@ int amrmg_serve_set_params(amrmg_serve* e, const int* sparams, const double* rpa
 int amrmg_serve_submit(amrmg_serve* e, const int* ispec, const double* rspec, const double* rhs, long long* id);
 /* blocks until request id is solved; phi valid-packed (may be null); times =
    {arrival, admission, completion} in s since the engine started; info =
-   {iterations, exit status} */
+   {iterations, exit status}.  One wait per request: a second waiter on the same id
+   gets an error, and the id is forgotten once the wait returns. */
 int amrmg_serve_wait(amrmg_serve* e, long long id, double* phi, double* times, int* info);
 /* {max HBM bytes held, hierarchies created, reused, evicted, max requests running at once} */
 int amrmg_serve_stats(amrmg_serve* e, long long* stats, int n);

@@ -78,3 +78,28 @@ def test_mixed_precision_tuner_and_determinism(gpu_ctx):
     np.testing.assert_array_equal(h1, h2)
     np.testing.assert_array_equal(g.get_field(g.PHI), p1)
     g.close()
+
+
+@pytest.mark.parametrize("mode,case", [(0, "amr32"), (2, "lshape"), (3, "amr32"), (5, "helm_single24"),
+                                       (4, "amr32")])
+def test_mixed_relax_modes_within_tolerance(gpu_ctx, mode, case):
+    """s_relaxMode 0 (looseGSRB), 2 (overlapGSRB), 3 (levelGSRBLazy), 4 (levelJacobi)
+    and 5 (levelMultiColor) on the FP32 operators of a mixed solve: converged, and the
+    composite solution within 1e-9 of the FP64 oracle solving with the same mode."""
+    print(f"mixed relax mode {mode} {case}", flush=True)
+    h, rhs = _case(case)
+    sp, rp = H.solver_params(pre=4, post=4)
+    g = amrmg.AMRMultiGrid(gpu_ctx, h, precision=1)
+    g.set_relax_mode(mode)
+    g.setSolverParameters(sp, rp)
+    g.set_field(g.RHS, rhs)
+    hist, info = g.solve()
+    phi = g.get_field(g.PHI)
+    g.close()
+    o = Oracle(h)
+    o.set_relax_mode(mode)
+    phi_o, hist_o, info_o = o.solve(rhs, sp, rp)
+    assert info["exit_status"] & 1 and hist[-1] <= 1e-10 * hist[0]
+    m = np.concatenate([uncovered_mask(h, lev) for lev in range(h.nlev)])
+    err = np.abs(phi - phi_o)[m].max() / np.abs(phi_o)[m].max()
+    assert err < MIXED_TOL, err

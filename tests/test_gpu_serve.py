@@ -71,3 +71,45 @@ def test_serve_reuses_resident_hierarchies(gpu_ctx):
     e.close()
     assert st["created"] == 1 and st["reused"] == 1
     np.testing.assert_array_equal(r2["phi"], 2 * r1["phi"])
+
+
+def test_serve_every_result_matches_oracle(gpu_ctx):
+    """Each request's result, taken in a mix of overlapping solves, equals the oracle's
+    solve of that request bit for bit."""
+    from oracle import Oracle
+    reqs = _requests(8, seed=80)
+    sp, rp = H.solver_params()
+    e = amrmg.ServeEngine(0, slots=4, budget_bytes=32 << 30, sparams=sp, rparams=rp)
+    ids = [e.submit(h, rhs) for h, rhs in reqs]
+    res = [e.wait(i) for i in ids]
+    e.close()
+    for q, ((h, rhs), r) in enumerate(zip(reqs, res)):
+        o = Oracle(h)
+        phi_o, hist_o, info_o = o.solve(rhs, sp, rp)
+        o.close()
+        assert r["iterations"] == info_o["iterations"] and r["exit_status"] == info_o["exit_status"], q
+        np.testing.assert_array_equal(r["phi"], phi_o)
+
+
+def test_serve_one_waiter_per_request(gpu_ctx):
+    """A request is waited on once: of two threads waiting for the same id, one gets
+    the result and the other an error."""
+    import threading
+    h, rhs = _requests(1, seed=81)[0]
+    sp, rp = H.solver_params()
+    e = amrmg.ServeEngine(0, slots=1, budget_bytes=16 << 30, sparams=sp, rparams=rp)
+    rid = e.submit(h, rhs)
+    out = []
+
+    def waiter():
+        try:
+            out.append(("ok", e.wait(rid, want_phi=False)["exit_status"]))
+        except amrmg.AMRMGError as x:
+            out.append(("err", str(x)))
+    ts = [threading.Thread(target=waiter) for _ in range(2)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    e.close()
+    assert sorted(k for k, _ in out) == ["err", "ok"], out

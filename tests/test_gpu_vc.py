@@ -458,3 +458,30 @@ def test_vc_rejects_bad_input(gpu_ctx):
     with pytest.raises(amrmg.AMRMGError, match="shared by two boxes"):
         g.set_vc_coefs(a, [b[0], b[1], bad])
     g.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_vc_average_after_coefs_takes_effect(gpu_ctx, precision):
+    """Choosing harmonic coarsening after the coefficients are set re-averages the
+    multigrid depths at once: the solve equals the one of a solver that chose it
+    first (FP64 and mixed precision), and differs from the arithmetic one."""
+    h = small_single(**CASES["vc32"])
+    rhs = H.sinusoid_rhs(h, 1034)
+    a, b = H.vc_coefficients(h, seed=53, contrast=8.0)
+    sp, rp = H.solver_params()
+    out = []
+    for order in ("before", "after", "never"):
+        g = amrmg.AMRMultiGrid(gpu_ctx, h, precision=precision)
+        if order == "before":
+            g.set_vc_average(True)
+        g.set_vc_coefs(a, b)
+        if order == "after":
+            g.set_vc_average(True)
+        g.setSolverParameters(sp, rp)
+        g.set_field(g.RHS, rhs)
+        hist, info = g.solve()
+        out.append((hist, g.get_field(g.PHI)))
+        g.close()
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+    assert not np.array_equal(out[0][0], out[2][0])
