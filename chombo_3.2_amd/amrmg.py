@@ -216,9 +216,12 @@ class AMRMultiGrid:
     def device_bytes(self):
         return lib().amrmg_solver_device_bytes(self.p)
 
-    RELAX_SHAPES = [(64, 8, 256), (32, 8, 256), (64, 4, 256), (32, 16, 256), (128, 4, 256), (64, 16, 256),
-                    (32, 16, 512), (64, 16, 512), (64, 32, 1024), (128, 8, 1024), (64, 13, 512), (32, 13, 256),
-                    (64, 29, 1024)]
+    # (column x, column y, threads, scheme): scheme 0 keeps five LDS planes, scheme 1
+    # keeps the z neighbours in registers (one x pair of cells per thread)
+    RELAX_SHAPES = [(64, 8, 256, 0), (32, 8, 256, 0), (64, 4, 256, 0), (32, 16, 256, 0), (128, 4, 256, 0),
+                    (64, 16, 256, 0), (32, 16, 512, 0), (64, 16, 512, 0), (64, 32, 1024, 0), (128, 8, 1024, 0),
+                    (64, 13, 512, 0), (32, 13, 256, 0), (64, 29, 1024, 0), (64, 26, 1024, 1), (32, 24, 512, 1),
+                    (64, 11, 512, 1), (32, 10, 256, 1), (128, 11, 1024, 1), (16, 21, 256, 1)]
     RELAX_CHUNKS = [0, 8, 16, 32]
 
     def tune_relax(self):

@@ -11,6 +11,7 @@
 #include <type_traits>
 
 #include "batch.hpp"
+#include "zmarch.hpp"
 
 #include <cstring>
 
@@ -70,7 +71,8 @@ __device__ __forceinline__ bool btile_cell(const Tile& t, const DBox& R, int lx,
 // ---------------------------------------------------------------------------
 // one-launch relaxation of every problem present at a depth
 // ---------------------------------------------------------------------------
-template <class T, bool LAP, int FX, int FY, int NT>
+template <class T, bool LAP, int SHAPE, int FX = ops::kFusedShapeTable[SHAPE][0],
+          int FY = ops::kFusedShapeTable[SHAPE][1], int NT = ops::kFusedShapeTable[SHAPE][2]>
 __global__ void __launch_bounds__(NT)
     k_batch_fused(const ops::FusedArgs* __restrict__ items, const int* __restrict__ wstart, int nitems,
                   const BatchCoef* __restrict__ coef, const int* __restrict__ active, int zsrc, T* __restrict__ dst,
@@ -88,8 +90,12 @@ __global__ void __launch_bounds__(NT)
   const BatchCoef cf = coef[p];
   ops::FusedArgs a = items[p];
   a.zsrc = zsrc;
-  ops::fused_column<T, LAP, FX, FY, NT>(a, wg - wstart[p], dst, src, rhs, T(cf.dxinv), T(cf.lambda), T(cf.alpha),
-                                        T(cf.beta), T(0.0), T(0.0));
+  if constexpr (ops::kFusedShapeTable[SHAPE][3] == 1)
+    ops::zmarch_column<T, LAP, FX, FY, NT>(a, wg - wstart[p], dst, src, rhs, T(cf.dxinv), T(cf.lambda),
+                                           T(cf.alpha), T(cf.beta), T(0.0), T(0.0));
+  else
+    ops::fused_column<T, LAP, FX, FY, NT>(a, wg - wstart[p], dst, src, rhs, T(cf.dxinv), T(cf.lambda),
+                                          T(cf.alpha), T(cf.beta), T(0.0), T(0.0));
 }
 
 // RESTRICTRES (AMRPoissonOpF.ChF:394-438) after the homogeneous BC refresh: one
@@ -1171,25 +1177,15 @@ template <class T, bool LAP>
 static void launch_batch_fused(Context* ctx, int shape, int grid, const ops::FusedArgs* items, const int* wstart,
                                int n, const BatchCoef* coef, const int* active, int zsrc, T* dst, const T* src,
                                const T* rhs) {
-#define BF_CASE(q, X, Y, NT)                                                                                   \
-  case q:                                                                                                      \
-    hipLaunchKernelGGL((k_batch_fused<T, LAP, X, Y, NT>), dim3(grid), dim3(NT), 0, ctx->stream, items, wstart, n, \
-                       coef, active, zsrc, dst, src, rhs);                                                     \
+#define BF_CASE(q)                                                                                             \
+  case q:                                                                                                        \
+    hipLaunchKernelGGL((k_batch_fused<T, LAP, q>), dim3(grid), dim3(ops::kFusedShapeTable[q][2]), 0, ctx->stream, \
+                       items, wstart, n, coef, active, zsrc, dst, src, rhs);                                     \
     break;
+  static_assert(ops::kFusedShapeCount == 19, "batch: one case per tile shape");
   switch (shape) {
-    BF_CASE(0, 64, 8, 256)
-    BF_CASE(1, 32, 8, 256)
-    BF_CASE(2, 64, 4, 256)
-    BF_CASE(3, 32, 16, 256)
-    BF_CASE(4, 128, 4, 256)
-    BF_CASE(5, 64, 16, 256)
-    BF_CASE(6, 32, 16, 512)
-    BF_CASE(7, 64, 16, 512)
-    BF_CASE(8, 64, 32, 1024)
-    BF_CASE(9, 128, 8, 1024)
-    BF_CASE(10, 64, 13, 512)
-    BF_CASE(11, 32, 13, 256)
-    BF_CASE(12, 64, 29, 1024)
+    BF_CASE(0) BF_CASE(1) BF_CASE(2) BF_CASE(3) BF_CASE(4) BF_CASE(5) BF_CASE(6) BF_CASE(7) BF_CASE(8) BF_CASE(9)
+    BF_CASE(10) BF_CASE(11) BF_CASE(12) BF_CASE(13) BF_CASE(14) BF_CASE(15) BF_CASE(16) BF_CASE(17) BF_CASE(18)
     default: throw std::runtime_error("batch: bad tile shape");
   }
 #undef BF_CASE

@@ -256,13 +256,18 @@ __device__ __forceinline__ void fused_column(const FusedArgs& a, int w, T* __res
   lds_barrier();
   store(k1 - 1);
 }
-// Tile shapes of the one-launch relaxation: column FX x FY, NT threads.  All give
-// bitwise the same result; the fastest depends on the level's size.
-// The last three are sized so that the red cells of the column grown by one,
+// Tile shapes of the one-launch relaxation: {FX, FY, NT, scheme}, column FX x FY,
+// NT threads; scheme 0 is fused_column (five LDS planes), scheme 1 is
+// zmarch_column (zmarch.hpp: z neighbours in registers, (FX/2 + 2) x (FY + 4)
+// threads own the column grown by 2).  All give bitwise the same result; the
+// fastest depends on the level's size.
+// Shapes 10-12 are sized so that the red cells of the column grown by one,
 // (FY + 2)(FX / 2 + 1), just fit in one pass of the NT threads.
-constexpr int kFusedShapeTable[][3] = {{64, 8, 256},  {32, 8, 256},   {64, 4, 256},   {32, 16, 256}, {128, 4, 256},
-                                       {64, 16, 256}, {32, 16, 512},  {64, 16, 512},  {64, 32, 1024},
-                                       {128, 8, 1024}, {64, 13, 512}, {32, 13, 256}, {64, 29, 1024}};
+constexpr int kFusedShapeTable[][4] = {
+    {64, 8, 256, 0},   {32, 8, 256, 0},    {64, 4, 256, 0},    {32, 16, 256, 0},  {128, 4, 256, 0},
+    {64, 16, 256, 0},  {32, 16, 512, 0},   {64, 16, 512, 0},   {64, 32, 1024, 0}, {128, 8, 1024, 0},
+    {64, 13, 512, 0},  {32, 13, 256, 0},   {64, 29, 1024, 0},  {64, 26, 1024, 1}, {32, 24, 512, 1},
+    {64, 11, 512, 1},  {32, 10, 256, 1},   {128, 11, 1024, 1}, {16, 21, 256, 1}};
 
 // consecutive work items on one XCD (workgroups go round-robin over the 8 XCDs,
 // grid a multiple of 8), so neighbouring columns share that XCD's L2 for their

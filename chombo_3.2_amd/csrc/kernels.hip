@@ -16,6 +16,7 @@
 
 #include "fused.hpp"
 #include "ops.hpp"
+#include "zmarch.hpp"
 
 namespace amrmg {
 
@@ -605,6 +606,15 @@ __global__ void __launch_bounds__(NT) k_gsrb_fused(FusedArgs a, T* __restrict__ 
   if (w >= a.nwork) return;
   fused_column<T, LAP, FX, FY, NT>(a, w, dst, src, rhs, dxinv, lambda, alpha, beta, c1, c2);
 }
+// the same relaxation with the z neighbours in registers (zmarch.hpp)
+template <class T, bool LAP, int FX, int FY, int NT>
+__global__ void __launch_bounds__(NT) k_gsrb_zmarch(FusedArgs a, T* __restrict__ dst, const T* __restrict__ src,
+                                                     const T* __restrict__ rhs, T dxinv, T lambda, T alpha, T beta,
+                                                     T c1, T c2) {
+  const int w = xcd_contiguous_block();
+  if (w >= a.nwork) return;
+  zmarch_column<T, LAP, FX, FY, NT>(a, w, dst, src, rhs, dxinv, lambda, alpha, beta, c1, c2);
+}
 
 constexpr int kNumFusedShapes = kFusedShapeCount;
 static_assert(sizeof(kFusedShapeTable) / sizeof(kFusedShapeTable[0]) == kNumFusedShapes, "shape table");
@@ -617,6 +627,11 @@ static void launch_fused(Context* ctx, int shape, int grid, const FusedArgs& a, 
   case n:                                                                                                     \
     hipLaunchKernelGGL((k_gsrb_fused<T, LAP, X, Y, NT>), dim3(grid), dim3(NT), 0, ctx->stream, a, dst, src, rhs, \
                        dxinv, lambda, alpha, beta, c1, c2);                                                   \
+    break;
+#define ZMARCH_CASE(n, X, Y, NT)                                                                                \
+  case n:                                                                                                       \
+    hipLaunchKernelGGL((k_gsrb_zmarch<T, LAP, X, Y, NT>), dim3(grid), dim3(NT), 0, ctx->stream, a, dst, src, rhs, \
+                       dxinv, lambda, alpha, beta, c1, c2);                                                     \
     break;
   switch (shape) {
     FUSED_CASE(0, 64, 8, 256)
@@ -632,9 +647,16 @@ static void launch_fused(Context* ctx, int shape, int grid, const FusedArgs& a, 
     FUSED_CASE(10, 64, 13, 512)
     FUSED_CASE(11, 32, 13, 256)
     FUSED_CASE(12, 64, 29, 1024)
+    ZMARCH_CASE(13, 64, 26, 1024)
+    ZMARCH_CASE(14, 32, 24, 512)
+    ZMARCH_CASE(15, 64, 11, 512)
+    ZMARCH_CASE(16, 32, 10, 256)
+    ZMARCH_CASE(17, 128, 11, 1024)
+    ZMARCH_CASE(18, 16, 21, 256)
     default: throw std::runtime_error("gsrb_fused: bad tile shape");
   }
 #undef FUSED_CASE
+#undef ZMARCH_CASE
   hip_check(hipGetLastError(), "k_gsrb_fused");
   prof_end(ctx, ev, KC_GSRB, bytes, units);
 }

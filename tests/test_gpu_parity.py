@@ -352,6 +352,32 @@ def test_relax_tile_shapes_and_tuner_bitwise(gpu_ctx):
     g.close()
 
 
+@pytest.mark.parametrize("case", ["helm_mixed_bc", "lap36"])
+def test_relax_tile_shapes_single_level_vs_oracle(gpu_ctx, case):
+    """Every tile shape of the one-launch relaxation, on a Helmholtz level whose
+    faces mix Neumann and Dirichlet rules and on a 36^3 Laplacian level (tiles cut
+    the rectangle raggedly), equals the oracle's solve bit for bit."""
+    if case == "helm_mixed_bc":
+        h = small_single(n=40, box=16, alpha=0.7, beta=1.3, bc=(0, 1, 0))
+    else:
+        h = small_single(n=36, box=12, block=4)
+    rhs = H.bump_rhs(h, 1016)
+    sp, rp = H.solver_params()
+    o = Oracle(h)
+    phi_o, hist_o, info_o = o.solve(rhs, sp, rp)
+    o.close()
+    g = amrmg.AMRMultiGrid(gpu_ctx, h)
+    g.setSolverParameters(sp, rp)
+    g.set_field(g.RHS, rhs)
+    for shape in range(len(g.RELAX_SHAPES)):
+        g.set_relax_shape(shape, 0 if shape % 2 else 8)
+        hist, info = g.solve()
+        assert info["iterations"] == info_o["iterations"], shape
+        np.testing.assert_array_equal(hist, hist_o)
+        np.testing.assert_array_equal(g.get_field(g.PHI), phi_o)
+    g.close()
+
+
 def test_set_alpha_beta_equals_fresh_solver(gpu_ctx):
     """AMRPoissonOp::setAlphaAndBeta on a solver that has already solved gives the
     same solve, bit for bit, as a solver defined with the new coefficients
