@@ -12,7 +12,10 @@
 #include <climits>
 #include <cmath>
 #include <cstdio>
+#include <mutex>
 #include <stdexcept>
+#include <string>
+#include <vector>
 
 #include "fused.hpp"
 #include "ops.hpp"
@@ -43,9 +46,39 @@ void dev_h2d(Context* c, void* dst, const void* src, size_t bytes, const char* w
   hip_check(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), what);
 }
 
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free entry points (amrmg_field_dot_noalloc): the
+// per-box partial sums of the ordered dot product live in `scratch`, a
+// file-scope device array of kNoAllocBoxes doubles (64 KiB; each GPU holds its
+// own copy until the process ends).  Its device address is looked up once per
+// device, when the first context on that device is created, and only that
+// address is ever handed to kernels or copies.  Every call on one GPU shares
+// the array: calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[ops::kNoAllocBoxes];
+static std::mutex g_scratch_mu;
+static std::vector<double*> g_scratch_ptr;  // per device: the address of `scratch`
+
+static void resolve_scratch(int dev) {  // the device must be current
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+  if (g_scratch_ptr[dev]) return;
+  double* p = nullptr;
+  hip_check(hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+  if (!p) throw std::runtime_error("hipGetSymbolAddress(scratch) returned no address");
+  g_scratch_ptr[dev] = p;
+}
+static double* device_scratch(int dev) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (dev < 0 || dev >= (int)g_scratch_ptr.size() || !g_scratch_ptr[dev])
+    throw std::runtime_error("no-allocation scratch not set up on this device");
+  return g_scratch_ptr[dev];
+}
+
 Context::Context(int dev, bool dry_run) : device(dev), dry(dry_run) {
   if (dry) return;
   hip_check(hipSetDevice(dev), "hipSetDevice");
+  resolve_scratch(dev);
   hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
   hip_check(hipMalloc(&d_scalars, 64 * sizeof(double)), "hipMalloc scalars");
   hip_check(hipHostMalloc(&h_scalars, 64 * sizeof(double)), "hipHostMalloc scalars");
@@ -1364,6 +1397,25 @@ double dot_ordered(const DeviceLayout& L, const T* a, const T* b, int slot) {
   L.ctx->sync();
   return L.ctx->h_scalars[slot];
 }
+// dot_ordered with its partial sums in the device's `scratch` array: no device
+// memory is allocated or freed (the result goes through the context's scalar
+// slot and its pinned host mirror, both made when the context was created)
+template <class T>
+double dot_ordered_noalloc(const DeviceLayout& L, const T* a, const T* b, int slot) {
+  if (L.nbox > kNoAllocBoxes)
+    throw std::runtime_error("dot without allocation: the level has " + std::to_string(L.nbox) +
+                             " boxes, at most " + std::to_string(kNoAllocBoxes) + " are supported");
+  double* part = device_scratch(L.ctx->device);
+  hipLaunchKernelGGL(k_dot_ordered<T>, dim3(1), dim3(256), 0, L.ctx->stream, L.d_boxes, L.nbox, a, b, part,
+                     L.ctx->d_scalars + slot);
+  hip_check(hipGetLastError(), "k_dot_ordered");
+  hip_check(hipMemcpyAsync(L.ctx->h_scalars + slot, L.ctx->d_scalars + slot, sizeof(double), hipMemcpyDeviceToHost,
+                           L.ctx->stream),
+            "copy scalar");
+  L.ctx->sync();
+  return L.ctx->h_scalars[slot];
+}
+template double dot_ordered_noalloc<double>(const DeviceLayout&, const double*, const double*, int);
 template <class T, bool ABS>
 static double sum_ordered(const DeviceLayout& L, const T* x, int slot) {
   int W = 0;

@@ -230,6 +230,12 @@ int amrmg_field_dot(const amrmg_field* a, const amrmg_field* b, double* out) {
   *out = ops::dot_ordered<double>(lay(a), a->ld.d, b->ld.d, 1);
   API_END
 }
+int amrmg_field_dot_noalloc(const amrmg_field* a, const amrmg_field* b, double* out) {
+  API_BEGIN
+  same_level(a, b, "dot");
+  *out = ops::dot_ordered_noalloc<double>(lay(a), a->ld.d, b->ld.d, 1);
+  API_END
+}
 int amrmg_field_norm(const amrmg_field* x, int p, double* out) {
   API_BEGIN
   const DeviceLayout& L = lay(x);

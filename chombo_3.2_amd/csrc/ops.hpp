@@ -134,6 +134,11 @@ template <class T>
 double maxnorm(const DeviceLayout& L, const T* x, int slot);
 template <class T>
 double dot_ordered(const DeviceLayout& L, const T* a, const T* b, int slot);
+// the same with the per-box partial sums in a file-scope device array (no device
+// allocation); levels of at most kNoAllocBoxes boxes
+constexpr int kNoAllocBoxes = 8192;
+template <class T>
+double dot_ordered_noalloc(const DeviceLayout& L, const T* a, const T* b, int slot);
 template <class T>
 double norm2_ordered(const DeviceLayout& L, const T* x, int slot);
 // norm(.., 1): the same per-box column sums of |x|, no root

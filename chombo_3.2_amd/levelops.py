@@ -43,6 +43,7 @@ def _lib():
             "amrmg_field_scale": [_vp, C.c_double],
             "amrmg_field_setval": [_vp, C.c_double],
             "amrmg_field_dot": [_vp, _vp, _dp],
+            "amrmg_field_dot_noalloc": [_vp, _vp, _dp],
             "amrmg_field_norm": [_vp, C.c_int, _dp],
             "amrmg_op_create": [_vp, _vp, C.c_double, C.c_double, C.c_double, C.c_double, _ip, _dp, _pp],
             "amrmg_op_create_coarser": [_vp, _pp, _pp],

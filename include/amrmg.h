@@ -305,6 +305,14 @@ int amrmg_field_scale(amrmg_field* x, double s);
 int amrmg_field_setval(amrmg_field* x, double v);
 int amrmg_field_dot(const amrmg_field* a, const amrmg_field* b, double* out);
 int amrmg_field_norm(const amrmg_field* x, int p, double* out);
+/* amrmg_field_dot without any device allocation.  No counterpart in the reference:
+   it is for callers that cannot allocate device memory (code under HIP graph
+   capture, tight serving loops, hosts that forbid allocation).  The per-box partial
+   sums go to a fixed device array of 8192 doubles per GPU (held until the process
+   ends), so levels of more than 8192 boxes are rejected with an error; the result
+   is bitwise amrmg_field_dot's.  All calls on one GPU share that array: calls that
+   overlap in time on one GPU race. */
+int amrmg_field_dot_noalloc(const amrmg_field* a, const amrmg_field* b, double* out);
 
 /* AMRPoissonOp::define (AMRPoissonOp.cpp:60-221) on level L: dx, dxCrse (> 0: the
    level has a coarser level, and homogeneousCFInterp fills its coarse-fine ghosts
