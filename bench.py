@@ -463,7 +463,8 @@ def serve_leg(args):
     budget = int(args.serve_budget_gb * (1 << 30))
 
     def run(arrivals):
-        eng = amrmg.ServeEngine(0, slots=args.serve_slots, budget_bytes=budget, sparams=sp, rparams=rp)
+        eng = amrmg.ServeEngine(0, slots=args.serve_slots, budget_bytes=budget, sparams=sp, rparams=rp,
+                                max_large=args.serve_max_large, large_cells=args.serve_large_cells)
         # warm the resident hierarchies (one request per distinct hierarchy)
         seen = {}
         for h, rhs in reqs:
@@ -499,11 +500,20 @@ def serve_leg(args):
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(reqs)))
     res2, wall2, st2 = run(arrivals)
     lat = np.array([r["latency"] for r in res2]) * 1e3
+    base = np.array([h.domain[1][0] + 1 for h, _ in reqs])
+    by_base = {}
+    for b in np.unique(base):
+        m = base == b
+        by_base[str(int(b))] = {"requests": int(m.sum()),
+                                "latency_ms_p50": round(float(np.percentile(lat[m], 50)), 2),
+                                "latency_ms_p99": round(float(np.percentile(lat[m], 99)), 2)}
     out = {
         "workload": f"{len(reqs)} requests: 1-2-level AMR Poisson, base U{{32,64,128,256}}^3, refined fraction "
                     "U(0.05,0.25), Gaussian-bump rhs, tol 1e-10, Poisson arrivals at 0.8x closed-loop capacity "
                     "(seed 1005)",
         "slots": args.serve_slots,
+        "max_large_running": args.serve_max_large,
+        "large_cells": args.serve_large_cells,
         "hbm_budget_gb": args.serve_budget_gb,
         "closed_loop_solves_per_s": round(cap, 3),
         "arrival_rate_per_s": round(rate, 3),
@@ -511,6 +521,7 @@ def serve_leg(args):
         "latency_ms_p50": round(float(np.percentile(lat, 50)), 2),
         "latency_ms_p99": round(float(np.percentile(lat, 99)), 2),
         "latency_ms_max": round(float(lat.max()), 2),
+        "latency_by_base": by_base,
         "converged_all": all(r["exit_status"] & 1 for r in res2),
         "max_running": st2["max_running"],
         "hierarchies_created": st2["created"],
@@ -539,8 +550,11 @@ def main():
     ap.add_argument("--batch", type=int, default=512, help="problems in the batched leg (0 = skip)")
     ap.add_argument("--batch-lanes", type=int, default=3, choices=(1, 2, 3),
                     help="2 or 3: the problems with large multigrid bottoms lockstep on further streams")
-    ap.add_argument("--serve", type=int, default=600, help="requests in the serving leg (0 = skip)")
-    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve", type=int, default=2000, help="requests in the serving leg (0 = skip)")
+    ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--serve-max-large", type=int, default=4,
+                    help="at most this many requests of at least --serve-large-cells cells run at once")
+    ap.add_argument("--serve-large-cells", type=int, default=1 << 22)
     ap.add_argument("--serve-budget-gb", type=float, default=96.0)
     ap.add_argument("--copy-mb", type=int, default=2048,
                     help="size of the plain copy kernel run after the timed region (0 = skip)")

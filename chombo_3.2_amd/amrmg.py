@@ -53,6 +53,7 @@ def lib():
         L.amrmg_serve_create.argtypes = [C.c_int, C.c_int, C.c_longlong, C.c_int, C.POINTER(C.c_void_p)]
         L.amrmg_serve_set_params.argtypes = [C.c_void_p, _ip, _dp]
         L.amrmg_serve_submit.argtypes = [C.c_void_p, _ip, _dp, _dp, C.POINTER(C.c_longlong)]
+        L.amrmg_serve_set_lanes.argtypes = [C.c_void_p, C.c_int, C.c_longlong]
         L.amrmg_serve_wait.argtypes = [C.c_void_p, C.c_longlong, _dp, _dp, _ip]
         L.amrmg_serve_stats.argtypes = [C.c_void_p, C.POINTER(C.c_longlong), C.c_int]
         L.amrmg_serve_destroy.argtypes = [C.c_void_p]
@@ -529,15 +530,21 @@ class ServeEngine:
     allow; hierarchies stay device-resident for later requests with the same spec."""
 
     def __init__(self, device: int = 0, slots: int = 4, budget_bytes: int = 64 << 30, precision: int = 0,
-                 sparams=None, rparams=None):
+                 sparams=None, rparams=None, max_large=None, large_cells=1 << 22):
         p = C.c_void_p()
         _chk(lib().amrmg_serve_create(int(device), int(slots), int(budget_bytes), int(precision), C.byref(p)))
         self.p = p
         self._size = {}
+        if max_large is not None:
+            self.set_lanes(max_large, large_cells)
         if sparams is not None:
             sp = np.ascontiguousarray(sparams, dtype=np.int32)
             rp = np.ascontiguousarray(rparams, dtype=np.float64)
             _chk(lib().amrmg_serve_set_params(self.p, sp.ctypes.data_as(_ip), rp.ctypes.data_as(_dp)))
+
+    def set_lanes(self, max_large, large_cells=1 << 22):
+        """At most max_large requests of at least large_cells cells run at once."""
+        _chk(lib().amrmg_serve_set_lanes(self.p, int(max_large), C.c_longlong(int(large_cells))))
 
     def submit(self, hier, rhs):
         isp, rsp = hier.spec()

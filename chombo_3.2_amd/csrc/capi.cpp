@@ -1035,7 +1035,8 @@ struct ServeReq {
   std::vector<int> ispec;
   std::vector<double> rspec, rhs, phi, hist;
   std::string key;
-  long long bytes = 0;
+  long long bytes = 0, cells = 0;
+  bool large = false;                        // runs in the lane of large requests
   double tArrival = 0, tAdmit = 0, tDone = 0;
   int iters = 0, exitStatus = -1;
   bool done = false, taken = false;
@@ -1063,6 +1064,11 @@ struct amrmg_serve {
   long long bytesInUse = 0, bytesIdle = 0, maxBytes = 0;
   long long created = 0, reused = 0, evicted = 0;
   int running = 0, maxRunning = 0;
+  // lanes: requests of at least largeCells cells are large, and at most maxLarge of
+  // them run at once, so the other slots stay free for small requests (which then
+  // never wait behind a large one for a slot)
+  long long largeCells = 0;
+  int maxLarge = 0, runningLarge = 0;
   bool stop = false;
   long long nextId = 1;
   std::vector<std::thread> workers;
@@ -1076,13 +1082,17 @@ struct amrmg_serve {
       {
         std::unique_lock<std::mutex> lk(mu);
         // admission: the first request in arrival order whose footprint fits beside the
-        // running ones; a head that does not fit is passed at most kServeMaxBypass times
+        // running ones and whose lane has room; a head that does not fit the budget is
+        // passed at most kServeMaxBypass times (a head waiting for its lane only waits)
         size_t pick = 0;
+        auto fits = [&](const ServeReq* q) { return bytesInUse + q->bytes <= budget; };
+        auto lane_free = [&](const ServeReq* q) { return !q->large || runningLarge < maxLarge; };
         auto admissible = [&] {
           if (queue.empty()) return false;
-          const size_t n = queue.front()->bypassed >= kServeMaxBypass ? 1 : queue.size();
+          const ServeReq* h = queue.front();
+          const size_t n = (!fits(h) && h->bypassed >= kServeMaxBypass) ? 1 : queue.size();
           for (size_t q = 0; q < n; q++)
-            if (bytesInUse + queue[q]->bytes <= budget) {
+            if (fits(queue[q]) && lane_free(queue[q])) {
               pick = q;
               return true;
             }
@@ -1091,8 +1101,9 @@ struct amrmg_serve {
         cv.wait(lk, [&] { return stop || admissible(); });
         if (stop) return;
         r = queue[pick];
-        if (pick > 0) queue.front()->bypassed++;
+        if (pick > 0 && !fits(queue.front())) queue.front()->bypassed++;
         queue.erase(queue.begin() + pick);
+        if (r->large) runningLarge++;
         std::memcpy(r->sp, sp, sizeof sp);
         std::memcpy(r->rp, rp, sizeof rp);
         for (size_t q = 0; q < idle.size(); q++)
@@ -1159,6 +1170,7 @@ struct amrmg_serve {
       r->rhs.shrink_to_fit();
       bytesInUse -= r->bytes;
       running--;
+      if (r->large) runningLarge--;
       if (inst) {
         inst->lastUse = r->tDone;
         bytesIdle += inst->bytes;
@@ -1178,6 +1190,7 @@ int amrmg_serve_create(int device, int nslots, long long hbm_budget, int precisi
   e->device = device;
   e->precision = precision;
   e->budget = hbm_budget;
+  e->maxLarge = nslots;  // one lane until amrmg_serve_set_lanes
   for (int q = 0; q < nslots; q++) e->workers.emplace_back([p = e.get()] { p->worker(); });
   *out = e.release();
   API_END
@@ -1187,6 +1200,17 @@ int amrmg_serve_set_params(amrmg_serve* e, const int* sp, const double* rp) {
   std::lock_guard<std::mutex> lk(e->mu);
   std::memcpy(e->sp, sp, sizeof e->sp);
   std::memcpy(e->rp, rp, sizeof e->rp);
+  API_END
+}
+int amrmg_serve_set_lanes(amrmg_serve* e, int max_large, long long large_cells) {
+  API_BEGIN
+  if (max_large < 1) throw std::runtime_error("serve: at least one large request must be able to run");
+  if (large_cells < 0) throw std::runtime_error("serve: the large-request threshold must not be negative");
+  std::lock_guard<std::mutex> lk(e->mu);
+  e->maxLarge = max_large;
+  e->largeCells = large_cells;
+  for (ServeReq* q : e->queue) q->large = q->cells >= large_cells;
+  e->cv.notify_all();
   API_END
 }
 int amrmg_serve_submit(amrmg_serve* e, const int* ispec, const double* rspec, const double* rhs, long long* id) {
@@ -1208,7 +1232,9 @@ int amrmg_serve_submit(amrmg_serve* e, const int* ispec, const double* rspec, co
   r->key.assign((const char*)ispec, sizeof(int) * nint);
   r->key.append((const char*)rspec, sizeof(double) * 9);
   r->bytes = serve_footprint(g, dom, dx0, e->precision);
+  r->cells = ncell;
   std::lock_guard<std::mutex> lk(e->mu);
+  r->large = ncell >= e->largeCells;
   auto m = e->measured.find(r->key);
   if (m != e->measured.end()) r->bytes = m->second;
   if (r->bytes > e->budget) throw std::runtime_error("serve: request larger than the HBM budget");

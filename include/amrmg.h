@@ -242,6 +242,11 @@ int amrmg_batch_set_graph(amrmg_batch* b, int on, long long* launches);
 typedef struct amrmg_serve amrmg_serve;
 int amrmg_serve_create(int device, int nslots, long long hbm_budget_bytes, int precision, amrmg_serve** out);
 int amrmg_serve_set_params(amrmg_serve* e, const int* sparams, const double* rparams);
+/* Lanes of the request loop (no counterpart in the reference): requests of at least
+   large_cells AMR cells are "large", and at most max_large of them run at once, so
+   the remaining slots stay free for smaller requests (default: max_large = slots,
+   one lane).  Waiting requests are reclassified. */
+int amrmg_serve_set_lanes(amrmg_serve* e, int max_large, long long large_cells);
 /* copies the spec and the valid-packed rhs; the arrival time is the call's time */
 int amrmg_serve_submit(amrmg_serve* e, const int* ispec, const double* rspec, const double* rhs, long long* id);
 /* blocks until request id is solved; phi valid-packed (may be null); times =

@@ -113,3 +113,42 @@ def test_serve_one_waiter_per_request(gpu_ctx):
         t.join()
     e.close()
     assert sorted(k for k, _ in out) == ["err", "ok"], out
+
+
+def test_serve_lanes_keep_small_requests_moving(gpu_ctx):
+    """With one lane for large requests (at least large_cells cells), a second large
+    request is admitted only after the first one is done, while small requests that
+    arrived after it are admitted meanwhile; every result equals the request solved
+    alone, bit for bit."""
+    # 256^3 large requests run for tens of ms, long after the small ones were submitted
+    big = [r for r in H.serve_requests(40, seed=82, bases=(256,)) if r[0].nlev == 1][:2]
+    small = [r for r in H.serve_requests(40, seed=83, bases=(32,))][:4]
+    assert len(big) == 2 and len(small) == 4
+    thresh = big[0][0].total_cells()  # 256^3: large; 32^3 requests stay small
+    assert all(h.total_cells() < thresh for h, _ in small)
+    sp, rp = H.solver_params()
+    alone = []
+    e1 = amrmg.ServeEngine(0, slots=1, budget_bytes=16 << 30, sparams=sp, rparams=rp)
+    for h, rhs in big + small:
+        alone.append(e1.wait(e1.submit(h, rhs)))
+    e1.close()
+    e = amrmg.ServeEngine(0, slots=3, budget_bytes=16 << 30, sparams=sp, rparams=rp, max_large=1,
+                          large_cells=thresh)
+    ids = [e.submit(h, rhs) for h, rhs in big + small]
+    res = [e.wait(i) for i in ids]
+    e.close()
+    b0, b1 = res[0], res[1]
+    assert b1["t_admit"] >= b0["t_done"]  # one large request at a time
+    assert any(r["t_admit"] < b1["t_admit"] for r in res[2:])  # small ones went ahead of it
+    for q, (a, m) in enumerate(zip(alone, res)):
+        assert a["iterations"] == m["iterations"] and a["exit_status"] == m["exit_status"], q
+        np.testing.assert_array_equal(a["phi"], m["phi"])
+
+
+def test_serve_lane_arguments_checked(gpu_ctx):
+    e = amrmg.ServeEngine(0, slots=2, budget_bytes=1 << 30)
+    with pytest.raises(amrmg.AMRMGError):
+        e.set_lanes(0)
+    with pytest.raises(amrmg.AMRMGError):
+        e.set_lanes(1, -5)
+    e.close()
