@@ -383,9 +383,11 @@ def batch_leg(args, ctx):
         s.close()
     log(f"plain FP64 one-by-one: {t_plain:.3f}s for {len(probs)} solves")
 
-    def batched(prec):
+    def batched(prec, order=0):
         b = amrmg.BatchSolver(ctx, [h for h, _ in probs], prec, lanes=args.batch_lanes)
         b.setSolverParameters(sp, rp)
+        if order:
+            b.set_reduction_order(order)
         for q, (_, rhs) in enumerate(probs):
             b.set_rhs(q, rhs)
         if not args.no_tune:
@@ -417,6 +419,23 @@ def batch_leg(args, ctx):
     wmp, itmp, exmp, relmp, resmp, phimp, histmp, stmp, repmp = batched(1)
     err = max(float(np.abs(phimp[q] - phi64[q]).max() / np.abs(phi64[q]).max()) for q in phi64)
     conv = all(h[-1] <= 1e-10 * h[0] for h in histmp)
+    # the same two batches with the bottom solves' reductions in the tree order
+    wt64, _, _, _, _, phit64, histt64, _, _ = batched(0, order=1)
+    wtmp, _, _, _, _, phitmp, histtmp, _, _ = batched(1, order=1)
+
+    def max_err(phis):
+        return max(float(np.abs(phis[q] - phi_plain[q]).max() / np.abs(phi_plain[q]).max()) for q in phi_plain)
+    tree = {
+        "batched_fp64_solves_per_s": round(len(probs) / wt64, 2),
+        "batched_mixed_solves_per_s": round(len(probs) / wtmp, 2),
+        "mixed_speedup_vs_fp64_reference_order": round(w64 / wtmp, 3),
+        "ms_per_batch_fp64": round(wt64 * 1e3, 3),
+        "ms_per_batch_mixed": round(wtmp * 1e3, 3),
+        "max_rel_err_fp64_vs_plain_sampled": max_err(phit64),
+        "max_rel_err_mixed_vs_plain_sampled": max_err(phitmp),
+        "converged_all": all(h[-1] <= 1e-10 * h[0] for h in histt64 + histtmp),
+    }
+    log(f"batched, tree reduction order in the bottoms: {tree}")
     out = {
         "workload": f"{len(probs)} single-level Helmholtz problems, edge 16..128 log-uniform (seed 1004), "
                     "64^3 max boxes, alpha~U(0,1), beta~U(0.5,2), Dirichlet 0, sinusoid+N(0,1e-3) rhs, tol 1e-10",
@@ -440,6 +459,7 @@ def batch_leg(args, ctx):
         "residual_mixed_achieved_GBs": round(resmp["bytes"] / (resmp["ms"] * 1e-3) / 1e9, 1) if resmp["ms"] else None,
         "mcell_relax_per_s_mixed": round(stmp["cell_relax"] / wmp / 1e6, 1),
         "vcycles_from_hip_graph": {"fp64": rep64, "mixed": repmp},
+        "tree_order_bottoms": tree,
         "lanes": args.batch_lanes,
     }
     log(f"batched: {out}")

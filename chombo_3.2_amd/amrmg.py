@@ -70,6 +70,7 @@ def lib():
         L.amrmg_batch_tune_relax.argtypes = [C.c_void_p, _dp, C.c_int, _ip]
         L.amrmg_batch_stats.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
         L.amrmg_batch_set_graph.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_longlong)]
+        L.amrmg_batch_set_reduction_order.argtypes = [C.c_void_p, C.c_int]
         L.amrmg_solver_create.argtypes = [C.c_void_p, _ip, _dp, C.c_int, C.POINTER(C.c_void_p)]
         L.amrmg_solver_destroy.argtypes = [C.c_void_p]
         L.amrmg_solver_set_params.argtypes = [C.c_void_p, _ip, _dp]
@@ -514,6 +515,11 @@ class BatchSolver:
         bi, db = C.c_longlong(), C.c_longlong()
         _chk(lib().amrmg_batch_stats(self.p, C.byref(cr), C.byref(bi), C.byref(db)))
         return dict(cell_relax=cr.value, bottom_iters=bi.value, device_bytes=db.value)
+
+    def set_reduction_order(self, order):
+        """Bottom-solve dot products and norms: 0 the reference's order (default), 1 a
+        fixed tree per problem (faster; same result alone and in any batch)."""
+        _chk(lib().amrmg_batch_set_reduction_order(self.p, int(order)))
 
     def set_graph(self, on=None):
         """V-cycles replayed from one captured HIP graph (True, the default) or

@@ -261,6 +261,8 @@ struct BottomArgs {
   int nbottom;                 // relax iterations before the Krylov solve
   int relaxFirst;              // 1: zero the correction and relax nbottom times first (the
                                // whole bottom of MultiGrid::cycle); 0: start from the correction
+  int tree;                    // reduction order of dot products and norms: 0 the reference's
+                               // (serial per box, boxes in layout order), 1 a fixed tree
 };
 
 // A walk over the cells of a box of extents (ea, eb, *), a fastest, in steps of
@@ -746,6 +748,49 @@ __global__ void __launch_bounds__(kBottomThreads)
     return rr[0];
   };
   auto norm = [&](const double* a) { return block_norm2_boxes(stage, sh, B, a); };
+  // Tree order (ba.tree): each thread adds the terms of its cells in its walk order,
+  // then the kBottomThreads partial sums are added by a fixed pairwise tree.  The
+  // result depends only on the problem's own bottom level (the workgroup size is
+  // fixed), so a problem solves the same alone and in any batch; it differs from the
+  // reference order by rounding only.
+  auto tree_sum = [&](double acc) {
+    sh[tid] = acc;
+    __syncthreads();
+    for (int st = kBottomThreads / 2; st > 0; st >>= 1) {
+      if (tid < st) sh[tid] = sh[tid] + sh[tid + st];
+      __syncthreads();
+    }
+    const double total = sh[0];
+    __syncthreads();
+    return total;
+  };
+  auto tree_dot2 = [&](const double* a1, const double* b1, const double* a2, const double* b2, double& d1,
+                       double& d2) {
+    double acc1 = 0.0, acc2 = 0.0;
+    struct D4 {
+      double a, b, c, d;
+    };
+    L.sweep<kUS>(
+        [&](int c) { return D4{a1[c], b1[c], a2[c], b2[c]}; },
+        [&](int, int, int, int, D4 w) {
+          acc1 += w.a * w.b;
+          acc2 += w.c * w.d;
+        });
+    d1 = tree_sum(acc1);
+    d2 = tree_sum(acc2);
+  };
+  auto tree_dot = [&](const double* a1, const double* b1) {
+    double acc = 0.0;
+    L.sweep<kUE>([&](int c) { return D2{a1[c], b1[c]}; }, [&](int, int, int, int, D2 w) { acc += w.a * w.b; });
+    return tree_sum(acc);
+  };
+  auto tree_norm = [&](const double* a) {
+    double acc = 0.0;
+    L.sweep<kUE>([&](int c) { return a[c]; }, [&](int, int, int, int, double w) { acc += w * w; });
+    return std::sqrt(tree_sum(acc));
+  };
+  auto rdot = [&](const double* a1, const double* b1) { return ba.tree ? tree_dot(a1, b1) : dot(a1, b1); };
+  auto rnorm = [&](const double* a) { return ba.tree ? tree_norm(a) : norm(a); };
 
   const double eps = metric[2 * p + 1];
   const double convergenceMetric = metric[2 * p];
@@ -755,7 +800,7 @@ __global__ void __launch_bounds__(kBottomThreads)
   zero3(e, p_tilde, s_tilde);
   int iter = 0;
   double rho[4] = {0, 0, 0, 0}, nrm[2];
-  nrm[0] = norm(r);
+  nrm[0] = rnorm(r);
   double initial_norm = nrm[0];
   const double initial_residual = nrm[0];
   nrm[1] = nrm[0];
@@ -768,7 +813,7 @@ __global__ void __launch_bounds__(kBottomThreads)
     iter++;
     nrm[1] = nrm[0]; alpha[1] = alpha[0]; beta[1] = beta[0]; omega[1] = omega[0];
     rho[3] = rho[2]; rho[2] = rho[1];
-    rho[1] = dot(r_tilde, r);
+    rho[1] = rdot(r_tilde, r);
     if (rho[1] == 0.0) {  // exit status 2
       incr(x, e, 1.0);
       done = true;
@@ -794,11 +839,11 @@ __global__ void __launch_bounds__(kBottomThreads)
     }
     preCond(p_tilde, pp);
     applyOp(v, p_tilde);
-    const double m = dot(r_tilde, v);
+    const double m = rdot(r_tilde, v);
     alpha[0] = rho[1] / m;
     if (fabs(m) > ba.small * fabs(rho[1])) {
       incr2(r, v, -alpha[0], e, p_tilde, alpha[0]);
-      nrm[0] = norm(r);
+      nrm[0] = rnorm(r);
     } else {
       L.sweep<kUE>([&](int) { return 0; }, [&](int c, int, int, int, int) { r[c] = 0.0; });
       nrm[0] = 0.0;
@@ -806,13 +851,17 @@ __global__ void __launch_bounds__(kBottomThreads)
     if (nrm[0] > eps * initial_norm && nrm[0] > ba.reps * initial_residual) {
       preCond(s_tilde, r);
       applyOp(t, s_tilde);
-      const double* const aa[2] = {t, t};
-      const double* const bb[2] = {r, t};
       double tt[2];
-      block_dots<2>(stage, sh, B, aa, bb, tt);
+      if (ba.tree) {
+        tree_dot2(t, r, t, t, tt[0], tt[1]);
+      } else {
+        const double* const aa[2] = {t, t};
+        const double* const bb[2] = {r, t};
+        block_dots<2>(stage, sh, B, aa, bb, tt);
+      }
       omega[0] = tt[0] / tt[1];
       incr2(e, s_tilde, omega[0], r, t, -omega[0]);
-      nrm[0] = norm(r);
+      nrm[0] = rnorm(r);
     }
     if (nrm[0] <= eps * initial_norm || nrm[0] <= ba.reps * initial_residual) break;  // exit status 1
     if (omega[0] == 0.0 || nrm[0] > (1 - ba.hang) * nrm[1]) {
@@ -826,7 +875,7 @@ __global__ void __launch_bounds__(kBottomThreads)
           break;
         }
         residual(r, x, b);
-        nrm[0] = norm(r);
+        nrm[0] = rnorm(r);
         rho[1] = 0.0; rho[2] = 0.0; rho[3] = 0.0;
         alpha[0] = 0; beta[0] = 0; omega[0] = 0;
         assign(r_tilde, r);
@@ -1253,7 +1302,7 @@ void BatchSolver::vcycle() {
     BATCH_LAUNCH(k_batch_restrict<T>, C.tileStart[np], 256, C.d_tiles, 6, 2, C.d_rect, F.d_rect, F.d_coef, d_active,
                  (T*)C.res, (const T*)F.corr, (const T*)F.res);
   }
-  BottomArgs ba{80, 1e-12, 1e-8, 1e-30, 5, m_nbottom, 1};
+  BottomArgs ba{80, 1e-12, 1e-8, 1e-30, 5, m_nbottom, 1, m_treeOrder ? 1 : 0};
   BATCH_LAUNCH(k_batch_bottom<T>, m_np, kBottomThreads, d_bottom, m_bcoef, d_bbox, (T* const*)d_corrPtr,
                (const T* const*)d_resPtr, d_bscratch, d_bscratchOff, d_metric, d_active, d_bstats, ba);
   for (int d = D - 2; d >= 0; d--) {
@@ -1321,6 +1370,11 @@ void BatchSolver::dropGraph() {
 void BatchSolver::setGraph(bool on) {
   if (!on) dropGraph();
   m_useGraph = on;
+}
+
+void BatchSolver::setTreeOrder(bool on) {
+  if (on != m_treeOrder) dropGraph();  // the captured cycle holds the bottom's arguments
+  m_treeOrder = on;
 }
 
 template <class T>

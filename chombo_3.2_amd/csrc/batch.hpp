@@ -120,6 +120,11 @@ class BatchSolver {
   // V-cycles replayed from one captured HIP graph (default on; the graph is not
   // used while the context's profiler is on, so per-kernel timing stays exact)
   void setGraph(bool on);
+  // reduction order of the bottom solves' dot products and norms: the reference's
+  // (default) or a fixed tree per problem (same result alone and in any batch,
+  // different from the reference order by rounding)
+  void setTreeOrder(bool on);
+  bool treeOrder() const { return m_treeOrder; }
   // run later launches on another context's stream (same device)
   void setContext(Context* c);
   Context* context() const { return m_ctx; }
@@ -195,6 +200,7 @@ class BatchSolver {
   double cycleUnits() const;                // cell-relaxations of one V-cycle of the active problems
   void dropGraph();
   bool m_useGraph = true;
+  bool m_treeOrder = false;
   hipGraphExec_t m_graph = nullptr;
   long long m_graphLaunches = 0;
   template <class T> void relaxDepth(int d, int np, int iters, bool fromZero);

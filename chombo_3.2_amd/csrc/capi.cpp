@@ -998,6 +998,12 @@ int amrmg_batch_set_graph(amrmg_batch* b, int on, long long* launches) {
   if (launches) *launches = n;
   API_END
 }
+int amrmg_batch_set_reduction_order(amrmg_batch* b, int order) {
+  API_BEGIN
+  if (order != 0 && order != 1) throw std::runtime_error("batch: reduction order must be 0 or 1");
+  for (auto& l : b->lanes) l.solver->setTreeOrder(order == 1);
+  API_END
+}
 int amrmg_batch_lanes(amrmg_batch* b, int* lanes, int* lane_of_problem) {
   API_BEGIN
   if (lanes) *lanes = (int)b->lanes.size();

@@ -227,6 +227,12 @@ int amrmg_batch_lanes(amrmg_batch* b, int* lanes, int* lane_of_problem);
    (the default); 0: direct launches; -1: leave as is.  The graph is not used while
    the context's profiler is on.  launches: graph replays so far. */
 int amrmg_batch_set_graph(amrmg_batch* b, int on, long long* launches);
+/* Reduction order of the bottom solves' dot products and norms (no counterpart in
+   the reference): 0 (default) the reference's LevelDataOps / FArrayBox::sumPow order
+   (results bitwise those of solving each problem alone on the reference's path), 1 a
+   fixed pairwise tree per problem (faster bottom solves; each result is still the same
+   alone and in any batch, and differs from order 0 by rounding only). */
+int amrmg_batch_set_reduction_order(amrmg_batch* b, int order);
 
 /* ---- serving: a request loop with continuous admission.  nslots worker threads
    each run one request at a time on its own HIP stream, so the solves of

@@ -44,9 +44,11 @@ def _single(gpu_ctx, h, rhs, sp, rp):
     return hist, info, phi
 
 
-def _batch(gpu_ctx, probs, sp, rp, precision=0, tune=False):
+def _batch(gpu_ctx, probs, sp, rp, precision=0, tune=False, order=0):
     b = amrmg.BatchSolver(gpu_ctx, [h for h, _ in probs], precision)
     b.setSolverParameters(sp, rp)
+    if order:
+        b.set_reduction_order(order)
     for q, (_, rhs) in enumerate(probs):
         b.set_rhs(q, rhs)
     if tune:
@@ -183,4 +185,35 @@ def test_batch_three_lanes_bitwise_equal_one_lane(gpu_ctx):
         assert it[q] == ref[q][1] and ex[q] == ref[q][2], q
         np.testing.assert_array_equal(b.history(q), ref[q][0])
         np.testing.assert_array_equal(b.get_phi(q), ref[q][3])
+    b.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_tree_order_same_alone_and_in_batch_and_near_oracle(gpu_ctx, precision):
+    """With the tree reduction order in the bottom solves, a problem's result does not
+    depend on what else is in the batch (bitwise equal solved in a batch of one), and
+    every converged solution is within 1e-10 of the oracle's FP64 solve (relative,
+    max norm), with the same iteration count or one more or fewer."""
+    probs = _edge_set()
+    sp, rp = H.solver_params()
+    mix = _batch(gpu_ctx, probs, sp, rp, precision=precision, order=1)
+    for q, (h, rhs) in enumerate(probs):
+        hist, it, ex, phi = mix[q]
+        assert ex & 1, q
+        hist1, it1, ex1, phi1 = _batch(gpu_ctx, [(h, rhs)], sp, rp, precision=precision, order=1)[0]
+        assert it1 == it and ex1 == ex, q
+        np.testing.assert_array_equal(hist1, hist)
+        np.testing.assert_array_equal(phi1, phi)
+        o = Oracle(h)
+        phi_o, hist_o, info_o = o.solve(rhs, sp, rp)
+        o.close()
+        err = float(np.abs(phi - phi_o).max() / np.abs(phi_o).max())
+        assert err <= 1e-10, (q, err)
+        assert abs(it - info_o["iterations"]) <= 1, q
+
+
+def test_batch_reduction_order_argument_checked(gpu_ctx):
+    b = amrmg.BatchSolver(gpu_ctx, [_problem(16, 0.0, 1.0)[0]], 0)
+    with pytest.raises(amrmg.AMRMGError):
+        b.set_reduction_order(2)
     b.close()
