@@ -261,8 +261,6 @@ struct BottomArgs {
   int nbottom;                 // relax iterations before the Krylov solve
   int relaxFirst;              // 1: zero the correction and relax nbottom times first (the
                                // whole bottom of MultiGrid::cycle); 0: start from the correction
-  int tree;                    // reduction order of dot products and norms: 0 the reference's
-                               // (serial per box, boxes in layout order), 1 a fixed tree
 };
 
 // A walk over the cells of a box of extents (ea, eb, *), a fastest, in steps of
@@ -606,12 +604,16 @@ constexpr int kColumnPlanes = 8;     // planes of a column tile
 constexpr int kBottomThreads = 512;  // one workgroup per bottom solve
 constexpr int kStage = kFillSteps * kBottomThreads + kBottomThreads;
 constexpr int kUS = 4, kUE = 8;      // cells per thread and pass: stencil / elementwise sweeps
-template <class T>
+template <class T, bool TREE>
 __global__ void __launch_bounds__(kBottomThreads)
     k_batch_bottom(const BatchBottom* __restrict__ bots, const BatchCoef* __restrict__ bcoef,
                    const DBox* __restrict__ bboxes, T* const* __restrict__ corr, const T* const* __restrict__ res,
                    double* __restrict__ scratch, const long long* __restrict__ soff, const double* __restrict__ metric,
                    const int* __restrict__ active, int* __restrict__ stats, BottomArgs ba) {
+  // cells per thread and pass of the stencil / elementwise sweeps (fewer cells per
+  // pass, for more problems per CU, measured slower: the sweeps are bound by the
+  // loads each thread keeps in flight)
+  constexpr int US = kUS, UE = kUE;
   const int p = blockIdx.x;
   if (!active[p]) return;
   __shared__ double sh[kBottomThreads];
@@ -660,8 +662,8 @@ __global__ void __launch_bounds__(kBottomThreads)
   const T* r32 = res[p] + bt.rect.off;
   if (ba.relaxFirst) {
     // MultiGrid::cycle at the bottom: the correction is zero on entry
-    L.sweep<kUE>([&](int) { return 0; }, [&](int c, int, int, int, int) { e32[c] = T(0.0); });
-    L.relax<kUS>(e32, r32, ba.nbottom);
+    L.sweep<UE>([&](int) { return 0; }, [&](int c, int, int, int, int) { e32[c] = T(0.0); });
+    L.relax<US>(e32, r32, ba.nbottom);
     if (L.ncell == 1) return;  // m_bottomCells == 1: relax only
   }
   const long long vs = 15 + (long long)L.R.sz * (L.nz + 2);  // elements per scratch vector
@@ -673,7 +675,7 @@ __global__ void __launch_bounds__(kBottomThreads)
     double a, b;
   };
   // PromotedBottomSolver: widen the correction and the residual
-  L.sweep<kUE>(
+  L.sweep<UE>(
       [&](int c) { return D2{(double)e32[c], (double)r32[c]}; },
       [&](int c, int, int, int, D2 w) {
         x[c] = w.a;
@@ -682,7 +684,7 @@ __global__ void __launch_bounds__(kBottomThreads)
   const BatchCoef cf = L.cf;
   // residual / applyOp of AMRPoissonOp on this level, homogeneous BC
   auto residual = [&](double* out, const double* phi, const double* rhs) {
-    L.sweep<kUS>(
+    L.sweep<US>(
         [&](int c) { return L.load_nb<double>(phi, rhs, c); },
         [&](int c, int i, int j, int k, BottomLevel::Nb<double> s) {
           L.bc(s, i, j, k);
@@ -692,7 +694,7 @@ __global__ void __launch_bounds__(kBottomThreads)
   };
   // zero(out) followed by applyOp(out, phi): applyOp writes every valid cell
   auto applyOp = [&](double* out, const double* phi) {
-    L.sweep<kUS>(
+    L.sweep<US>(
         [&](int c) { return L.load_nb<double>(phi, nullptr, c); },
         [&](int c, int i, int j, int k, BottomLevel::Nb<double> s) {
           L.bc(s, i, j, k);
@@ -704,13 +706,13 @@ __global__ void __launch_bounds__(kBottomThreads)
   // the scaling is folded into the first colour
   const double mult = 1.0 / (cf.alpha - 2.0 * 3 * cf.beta / (cf.dx * cf.dx));
   auto preCond = [&](double* cor, const double* rv) {
-    L.gsrb<kUS>(cor, rv, 0, rv, mult);
-    L.gsrb<kUS>(cor, rv, 1);
-    L.relax<kUS>(cor, rv, 1);
+    L.gsrb<US>(cor, rv, 0, rv, mult);
+    L.gsrb<US>(cor, rv, 1);
+    L.relax<US>(cor, rv, 1);
   };
   // lhs += xv * s
   auto incr = [&](double* lhs, const double* xv, double s) {
-    L.sweep<kUE>(
+    L.sweep<UE>(
         [&](int c) { return D2{lhs[c], xv[c]}; },
         [&](int c, int, int, int, D2 w) { lhs[c] = w.a + w.b * s; });
   };
@@ -719,7 +721,7 @@ __global__ void __launch_bounds__(kBottomThreads)
     struct D4 {
       double a, b, c, d;
     };
-    L.sweep<kUS>(
+    L.sweep<US>(
         [&](int c) { return D4{l1[c], x1[c], l2[c], x2[c]}; },
         [&](int c, int, int, int, D4 w) {
           l1[c] = w.a + w.b * s1;
@@ -727,12 +729,12 @@ __global__ void __launch_bounds__(kBottomThreads)
         });
   };
   auto assign = [&](double* lhs, const double* rv) {
-    L.sweep<kUE>(
+    L.sweep<UE>(
         [&](int c) { return rv[c]; }, [&](int c, int, int, int, double w) { lhs[c] = w; });
   };
   // setToZero of several vectors in one sweep
   auto zero3 = [&](double* z1, double* z2, double* z3) {
-    L.sweep<kUE>(
+    L.sweep<UE>(
         [&](int) { return 0; },
         [&](int c, int, int, int, int) {
           z1[c] = 0.0;
@@ -748,7 +750,7 @@ __global__ void __launch_bounds__(kBottomThreads)
     return rr[0];
   };
   auto norm = [&](const double* a) { return block_norm2_boxes(stage, sh, B, a); };
-  // Tree order (ba.tree): each thread adds the terms of its cells in its walk order,
+  // Tree order (TREE): each thread adds the terms of its cells in its walk order,
   // then the kBottomThreads partial sums are added by a fixed pairwise tree.  The
   // result depends only on the problem's own bottom level (the workgroup size is
   // fixed), so a problem solves the same alone and in any batch; it differs from the
@@ -770,7 +772,7 @@ __global__ void __launch_bounds__(kBottomThreads)
     struct D4 {
       double a, b, c, d;
     };
-    L.sweep<kUS>(
+    L.sweep<US>(
         [&](int c) { return D4{a1[c], b1[c], a2[c], b2[c]}; },
         [&](int, int, int, int, D4 w) {
           acc1 += w.a * w.b;
@@ -781,16 +783,22 @@ __global__ void __launch_bounds__(kBottomThreads)
   };
   auto tree_dot = [&](const double* a1, const double* b1) {
     double acc = 0.0;
-    L.sweep<kUE>([&](int c) { return D2{a1[c], b1[c]}; }, [&](int, int, int, int, D2 w) { acc += w.a * w.b; });
+    L.sweep<UE>([&](int c) { return D2{a1[c], b1[c]}; }, [&](int, int, int, int, D2 w) { acc += w.a * w.b; });
     return tree_sum(acc);
   };
   auto tree_norm = [&](const double* a) {
     double acc = 0.0;
-    L.sweep<kUE>([&](int c) { return a[c]; }, [&](int, int, int, int, double w) { acc += w * w; });
+    L.sweep<UE>([&](int c) { return a[c]; }, [&](int, int, int, int, double w) { acc += w * w; });
     return std::sqrt(tree_sum(acc));
   };
-  auto rdot = [&](const double* a1, const double* b1) { return ba.tree ? tree_dot(a1, b1) : dot(a1, b1); };
-  auto rnorm = [&](const double* a) { return ba.tree ? tree_norm(a) : norm(a); };
+  auto rdot = [&](const double* a1, const double* b1) {
+    if constexpr (TREE) return tree_dot(a1, b1);
+    else return dot(a1, b1);
+  };
+  auto rnorm = [&](const double* a) {
+    if constexpr (TREE) return tree_norm(a);
+    else return norm(a);
+  };
 
   const double eps = metric[2 * p + 1];
   const double convergenceMetric = metric[2 * p];
@@ -829,7 +837,7 @@ __global__ void __launch_bounds__(kBottomThreads)
       struct D3 {
         double p, v, r;
       };
-      L.sweep<kUS>(
+      L.sweep<US>(
           [&](int c) { return D3{pp[c], v[c], r[c]}; },
           [&](int c, int, int, int, D3 w) {
             const double s1 = w.p * bt1;
@@ -845,14 +853,14 @@ __global__ void __launch_bounds__(kBottomThreads)
       incr2(r, v, -alpha[0], e, p_tilde, alpha[0]);
       nrm[0] = rnorm(r);
     } else {
-      L.sweep<kUE>([&](int) { return 0; }, [&](int c, int, int, int, int) { r[c] = 0.0; });
+      L.sweep<UE>([&](int) { return 0; }, [&](int c, int, int, int, int) { r[c] = 0.0; });
       nrm[0] = 0.0;
     }
     if (nrm[0] > eps * initial_norm && nrm[0] > ba.reps * initial_residual) {
       preCond(s_tilde, r);
       applyOp(t, s_tilde);
       double tt[2];
-      if (ba.tree) {
+      if constexpr (TREE) {
         tree_dot2(t, r, t, t, tt[0], tt[1]);
       } else {
         const double* const aa[2] = {t, t};
@@ -879,7 +887,7 @@ __global__ void __launch_bounds__(kBottomThreads)
         rho[1] = 0.0; rho[2] = 0.0; rho[3] = 0.0;
         alpha[0] = 0; beta[0] = 0; omega[0] = 0;
         assign(r_tilde, r);
-        L.sweep<kUE>([&](int) { return 0; }, [&](int c, int, int, int, int) { e[c] = 0.0; });
+        L.sweep<UE>([&](int) { return 0; }, [&](int c, int, int, int, int) { e[c] = 0.0; });
         restarts++;
         init = true;
       }
@@ -888,7 +896,7 @@ __global__ void __launch_bounds__(kBottomThreads)
   if (!done) incr(x, e, 1.0);
   if (tid == 0) stats[p] += iter;
   // round the correction back into the V-cycle's precision
-  L.sweep<kUE>([&](int c) { return x[c]; }, [&](int c, int, int, int, double w) { e32[c] = (T)w; });
+  L.sweep<UE>([&](int c) { return x[c]; }, [&](int c, int, int, int, double w) { e32[c] = (T)w; });
 }
 
 }  // namespace
@@ -1302,9 +1310,13 @@ void BatchSolver::vcycle() {
     BATCH_LAUNCH(k_batch_restrict<T>, C.tileStart[np], 256, C.d_tiles, 6, 2, C.d_rect, F.d_rect, F.d_coef, d_active,
                  (T*)C.res, (const T*)F.corr, (const T*)F.res);
   }
-  BottomArgs ba{80, 1e-12, 1e-8, 1e-30, 5, m_nbottom, 1, m_treeOrder ? 1 : 0};
-  BATCH_LAUNCH(k_batch_bottom<T>, m_np, kBottomThreads, d_bottom, m_bcoef, d_bbox, (T* const*)d_corrPtr,
-               (const T* const*)d_resPtr, d_bscratch, d_bscratchOff, d_metric, d_active, d_bstats, ba);
+  BottomArgs ba{80, 1e-12, 1e-8, 1e-30, 5, m_nbottom, 1};
+  if (m_treeOrder)
+    BATCH_LAUNCH((k_batch_bottom<T, true>), m_np, kBottomThreads, d_bottom, m_bcoef, d_bbox, (T* const*)d_corrPtr,
+                 (const T* const*)d_resPtr, d_bscratch, d_bscratchOff, d_metric, d_active, d_bstats, ba);
+  else
+    BATCH_LAUNCH((k_batch_bottom<T, false>), m_np, kBottomThreads, d_bottom, m_bcoef, d_bbox, (T* const*)d_corrPtr,
+                 (const T* const*)d_resPtr, d_bscratch, d_bscratchOff, d_metric, d_active, d_bstats, ba);
   for (int d = D - 2; d >= 0; d--) {
     const int np = m_depth[d + 1].np;
     Depth& F = m_depth[d];
@@ -1650,7 +1662,7 @@ void DeviceBottomSolver<T>::solve(LevelData<T>& phi, const LevelData<T>& rhs) {
   }
   BottomArgs ba{m_fallback.m_imax, m_fallback.m_reps, m_fallback.m_hang, m_fallback.m_small,
                 m_fallback.m_numRestarts, 0, 0};
-  hipLaunchKernelGGL(k_batch_bottom<T>, dim3(1), dim3(kBottomThreads), 0, m_ctx->stream, (const BatchBottom*)d_bottom,
+  hipLaunchKernelGGL((k_batch_bottom<T, false>), dim3(1), dim3(kBottomThreads), 0, m_ctx->stream, (const BatchBottom*)d_bottom,
                      (const BatchCoef*)d_bcoef, (const DBox*)d_bbox, (T* const*)d_ptrs,
                      (const T* const*)((void**)d_ptrs + 1), d_scratch, (const long long*)d_soff,
                      (const double*)d_metric, (const int*)d_active, (int*)d_stats, ba);
