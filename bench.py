@@ -43,6 +43,68 @@ def read_traffic():
     return None
 
 
+def summarize_trace(trace_csv, out_path, names, command):
+    """Markdown summary of a rocprofv3 kernel trace of a run made with --trace-marks:
+    the dispatches between each pair of marker copies (k_copy_probe) are one timed
+    region; per region, kernel time by kernel (templates merged) and by instantiation."""
+    import csv
+    import re
+    rows = []
+    with open(trace_csv) as f:
+        for r in csv.DictReader(f):
+            rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"]))
+    rows.sort()
+    # segments between runs of marker dispatches; the timed regions are every second one
+    segs, cur, in_mark = [], [], False
+    for r in rows:
+        mark = "k_copy_probe" in r[2]
+        if mark and not in_mark:
+            segs.append(cur)
+            cur = []
+        if not mark:
+            cur.append(r)
+        in_mark = mark
+    segs.append(cur)
+    regions = segs[1::2]
+
+    def base(name):
+        n = name.replace("void ", "").replace("(anonymous namespace)::", "")
+        n = n.split("(")[0]
+        n = re.sub(r"^amrmg::(\(anonymous namespace\)::)?(ops::|vc::)?", "", n)
+        return n
+
+    out = [f"# {names[0] if names else 'Kernel time'}", "", "Command (repository root on the GPU host; the output "
+           "directory is scratch), then this summary:", "", "```", command,
+           f"python3 bench.py --summarize-trace {trace_csv} {out_path}", "```", ""]
+    for q, reg in enumerate(regions):
+        if not reg:
+            continue
+        title = names[q + 1] if q + 1 < len(names) else f"timed region {q + 1}"
+        busy = sum(e - s for s, e, _ in reg) * 1e-6
+        span = (reg[-1][1] - reg[0][0]) * 1e-6
+        cls, inst = {}, {}
+        for s_, e_, n in reg:
+            k = re.sub(r"<.*", "", base(n))
+            cls.setdefault(k, [0, 0.0])
+            cls[k][0] += 1
+            cls[k][1] += (e_ - s_) * 1e-6
+            i = base(n)
+            inst.setdefault(i, [0, 0.0])
+            inst[i][0] += 1
+            inst[i][1] += (e_ - s_) * 1e-6
+        out += [f"## {title}", "", f"{len(reg)} dispatches, span {span:.2f} ms under the profiler, kernels busy "
+                f"{busy:.2f} ms.", "", "| kernel | calls | ms | % of busy |", "|---|---:|---:|---:|"]
+        for k, (c, ms) in sorted(cls.items(), key=lambda kv: -kv[1][1])[:16]:
+            out.append(f"| `{k}` | {c} | {ms:.2f} | {100 * ms / busy:.1f} |")
+        out += ["", "| instantiation | calls | ms | avg us |", "|---|---:|---:|---:|"]
+        for k, (c, ms) in sorted(inst.items(), key=lambda kv: -kv[1][1])[:16]:
+            out.append(f"| `{k}` | {c} | {ms:.2f} | {1000 * ms / c:.1f} |")
+        out.append("")
+    with open(out_path, "w") as f:
+        f.write("\n".join(out))
+    log(f"wrote {out_path} ({len(regions)} regions)")
+
+
 def summarize_pmc(fetch_csv, write_csv, fetch_json, copy_bytes, out_path):
     """Writes the GSRB traffic summary from two rocprofv3 --pmc runs of this script
     (FETCH_SIZE, then WRITE_SIZE; both with --steps 1 --warmup 0 --no-cpu).
@@ -62,7 +124,8 @@ def summarize_pmc(fetch_csv, write_csv, fetch_json, copy_bytes, out_path):
         # relaxations before the copy calibration (the autotuner's come earlier)
         rows = sorted(csv.DictReader(open(path)), key=lambda r: int(r["Dispatch_Id"]))
         first_cp = min(q for q, r in enumerate(rows) if "k_copy_probe" in r["Kernel_Name"])
-        gs = [float(r["Counter_Value"]) * 1024.0 for r in rows[:first_cp] if "k_gsrb_fused" in r["Kernel_Name"]]
+        gs = [float(r["Counter_Value"]) * 1024.0 for r in rows[:first_cp]
+              if "k_gsrb_fused" in r["Kernel_Name"] or "k_gsrb_zmarch" in r["Kernel_Name"]]
         cp = [float(r["Counter_Value"]) * 1024.0 for r in rows if "k_copy_probe" in r["Kernel_Name"]]
         if len(gs) < n:
             raise SystemExit(f"dispatch count mismatch: {len(gs)} relaxations before the calibration, {n} timed")
@@ -75,7 +138,7 @@ def summarize_pmc(fetch_csv, write_csv, fetch_json, copy_bytes, out_path):
     meas_total = sum(gf) * fscale + sum(gw) * wscale
     ratio = meas_total / alg_total
     d = {
-        "kernel": "GSRB relaxation class (k_gsrb_fused)",
+        "kernel": "GSRB relaxation class (k_gsrb_fused, k_gsrb_zmarch)",
         "fetch_scale": round(fscale, 4), "write_scale": round(wscale, 4),
         "copy_bytes": copy_bytes,
         "gsrb_dispatches": len(gf),
@@ -221,10 +284,14 @@ def mixed_leg(args, ctx, hier, rhs, sp, rp, solver64, fp64_s_per_step):
         ms.solve()
     ctx.profile(["gsrb"])
     ctx.profile_reset()
+    if args.trace_marks:
+        ctx.copy_probe(4096, 1)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         hist, info = ms.solve()
     wall = time.perf_counter() - t0
+    if args.trace_marks:
+        ctx.copy_probe(4096, 1)
     prof = ctx.profile_read("gsrb")
     ctx.profile([])
     phi = ms.get_field(ms.PHI)
@@ -578,9 +645,19 @@ def main():
     ap.add_argument("--serve-budget-gb", type=float, default=96.0)
     ap.add_argument("--copy-mb", type=int, default=2048,
                     help="size of the plain copy kernel run after the timed region (0 = skip)")
+    ap.add_argument("--trace-marks", action="store_true",
+                    help="launch a 4 KiB copy kernel before and after each timed region of the FP64 and mixed "
+                         "legs (region boundaries for --summarize-trace)")
+    ap.add_argument("--summarize-trace", nargs=2, metavar=("KERNEL_TRACE_CSV", "OUT_MD"))
+    ap.add_argument("--trace-names", default="Kernel time, 3-level AMR Poisson (base 256^3, 113.5 M cells);"
+                    "FP64 solve (timed);mixed-precision solve (timed)")
+    ap.add_argument("--trace-command", default="")
     ap.add_argument("--summarize-pmc", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "FETCH_RUN_JSON"),
                     help="write the GSRB traffic summary from two rocprofv3 --pmc runs and exit")
     args = ap.parse_args()
+    if args.summarize_trace:
+        summarize_trace(*args.summarize_trace, names=args.trace_names.split(";"), command=args.trace_command)
+        return
     if args.summarize_pmc:
         summarize_pmc(*args.summarize_pmc, copy_bytes=args.copy_mb << 20, out_path=TRAFFIC_JSON)
         return
@@ -621,6 +698,8 @@ def main():
 
     ctx.profile(["gsrb"])
     ctx.profile_reset()
+    if args.trace_marks:
+        ctx.copy_probe(4096, 1)
     t0 = time.perf_counter()
     infos = []
     for s in range(args.steps):
@@ -628,6 +707,8 @@ def main():
         infos.append((hist, info))
         log(f"step {s}: {info['iterations']} iterations, rel residual {hist[-1] / hist[0]:.3e}")
     t1 = time.perf_counter()
+    if args.trace_marks:
+        ctx.copy_probe(4096, 1)
     prof = ctx.profile_read("gsrb")
     ctx.profile([])
 
@@ -713,7 +794,7 @@ def main():
             "frac": round(achieved / PEAK_HBM_GBS, 4),
             "traffic": traffic,
             "traffic_over_algorithmic": ratio,
-            "kernel": "GSRB relaxation class: k_gsrb_fused (one red+black iteration of a whole level per launch)",
+            "kernel": "GSRB relaxation class: k_gsrb_fused / k_gsrb_zmarch (one red+black iteration of a whole level per launch)",
             "avg_launch_ms": round(avg_launch_ms, 5),
             "algorithmic_bytes_per_launch": round(bytes_per_launch, 1),
             "launches": prof["launches"],

@@ -121,6 +121,22 @@ void AMRPoissonOp<Real>::incrAssign(T& lhs, T& x) {
   ops::incr_assign<Real>(*lhs.lay, lhs.d, x.d);
 }
 template <class Real>
+void AMRPoissonOp<Real>::incrFromZero(T& lhs, const T& x) {
+  need(x.lay, lhs.lay, "incrFromZero");
+  ops::incr_from_zero<Real>(*lhs.lay, lhs.d, x.d);
+}
+template <class Real>
+void AMRPoissonOp<Real>::incrAssignUpdate(T& m, T& d, T& phi) {
+  need(d.lay, m.lay, "incrAssignUpdate");
+  need(phi.lay, m.lay, "incrAssignUpdate");
+  ops::incr_assign_update<Real>(*m.lay, m.d, d.d, phi.d);
+}
+template <class Real>
+void AMRPoissonOp<Real>::incrPhiFromZero(T& phi, const T& x) {
+  need(x.lay, phi.lay, "incrPhiFromZero");
+  ops::incr_phi_from_zero<Real>(*phi.lay, phi.d, x.d);
+}
+template <class Real>
 void AMRPoissonOp<Real>::axby(T& lhs, const T& x, const T& y, double a, double b) {
   need(x.lay, lhs.lay, "axby"); need(y.lay, lhs.lay, "axby");
   ops::axby<Real>(*lhs.lay, lhs.d, x.d, y.d, a, b);
@@ -897,10 +913,15 @@ void AMRMultiGrid<Real>::AMRVCycle(std::vector<T*>& uberCorrection, std::vector<
   } else if (ilev == l_base) {
     if (fromZero) m_mg0->oneCycleFromZero(*m_correction[ilev], *m_residual[ilev]);
     else m_mg0->oneCycle(*m_correction[ilev], *m_residual[ilev]);
-    m_op[ilev]->incr(*uberCorrection[ilev], *m_correction[ilev], 1.0);
+    // uberCorrection[l_base] is zero here and read next only by the update of phi
+    if (!m_foldPhi.empty()) m_op[ilev]->incrPhiFromZero(*m_foldPhi[ilev], *m_correction[ilev]);
+    else m_op[ilev]->incr(*uberCorrection[ilev], *m_correction[ilev], 1.0);
   } else {
     relax(*m_correction[ilev], *m_residual[ilev], ilev, m_pre, fromZero);
-    m_op[ilev]->incr(*uberCorrection[ilev], *m_correction[ilev], 1.0);
+    // on the first visit of a cycle uberCorrection[ilev] is zero (zeroed before the
+    // cycle and not written since), so the incr need not read it
+    if (fromZero) m_op[ilev]->incrFromZero(*uberCorrection[ilev], *m_correction[ilev]);
+    else m_op[ilev]->incr(*uberCorrection[ilev], *m_correction[ilev], 1.0);
     m_op[ilev - 1]->setToZero(*m_correction[ilev - 1]);
     std::vector<T*> res(m_residual.size());
     for (size_t q = 0; q < res.size(); q++) res[q] = m_residual[q].get();
@@ -915,8 +936,11 @@ void AMRMultiGrid<Real>::AMRVCycle(std::vector<T*>& uberCorrection, std::vector<
     T& dCorr = *uberCorrection[ilev];
     relax(dCorr, *m_residual[ilev], ilev, m_post, true);  // setToZero(dCorr) + relax
     // incr(m_correction, dCorr, 1.0) then assignLocal(uberCorrection, m_correction),
-    // in one pass over the valid cells (every later read of a ghost refills it)
-    m_op[ilev]->incrAssign(*m_correction[ilev], dCorr);
+    // in one pass over the valid cells (every later read of a ghost refills it);
+    // with the phi update folded in, uberCorrection[ilev] is read next only by
+    // solveNoInitResid's incr(phi) and setToZero, which this pass performs too
+    if (!m_foldPhi.empty()) m_op[ilev]->incrAssignUpdate(*m_correction[ilev], dCorr, *m_foldPhi[ilev]);
+    else m_op[ilev]->incrAssign(*m_correction[ilev], dCorr);
   }
 }
 // solveNoInitResid (AMRMultiGrid.H:935-1124)
@@ -943,6 +967,17 @@ void AMRMultiGrid<Real>::solveNoInitResid(std::vector<T*>& phi, std::vector<T*>&
     norm_last = rnorm;
     if (m_innerCycleUpdate) {
       m_innerCycleUpdate(phi, uberResidual, l_max, l_base);  // uberCorrection stays zero
+    } else if (m_numMG == 1 && l_max > l_base) {
+      // every level is visited once per cycle: its correction goes into phi at the
+      // end of its upsweep, with the arithmetic of the update below
+      m_foldPhi.assign(phi.begin(), phi.end());
+      try {
+        AMRVCycle(uberCorrection, uberResidual, l_max, l_max, l_base);
+      } catch (...) {
+        m_foldPhi.clear();
+        throw;
+      }
+      m_foldPhi.clear();
     } else {
       AMRVCycle(uberCorrection, uberResidual, l_max, l_max, l_base);
       for (int ilev = l_base; ilev <= l_max; ilev++) {

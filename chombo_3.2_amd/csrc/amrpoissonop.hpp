@@ -90,6 +90,9 @@ class AMRPoissonOp : public AMRLevelOp<LevelData<Real>> {
   void incr(T& lhs, const T& x, double scale) override;
   // lhs = lhs + x * 1.0 then x = lhs on the valid cells (incr + assignLocal fused)
   void incrAssign(T& lhs, T& x);
+  void incrFromZero(T& lhs, const T& x);
+  void incrAssignUpdate(T& m, T& d, T& phi);
+  void incrPhiFromZero(T& phi, const T& x);
   void axby(T& lhs, const T& x, const T& y, double a, double b) override;
   void scale(T& lhs, double s) override;
   double norm(const T& x, int ord) override;
@@ -204,6 +207,10 @@ class AMRMultiGrid {
   // m_innerBottomMetrics receives the bottom solver's convergence metrics.
   std::function<void(std::vector<T*>&, std::vector<T*>&, int, int)> m_innerCycleUpdate;
   std::function<void(double, double)> m_innerBottomMetrics;
+  // While solveNoInitResid runs a V-cycle with the phi update folded in (one
+  // coarser cycle per level, FP64 update): the phi of every level.  The cycle then
+  // adds each level's correction to phi itself and leaves uberCorrection zero.
+  std::vector<T*> m_foldPhi;
 
   void define(AMRPoissonOpFactory<Real>& factory, LinearSolver<T>* bottom, int numLevels);
   void setSolverParameters(int pre, int post, int bottom, int numMG, int iterMax, double eps, double hang,

@@ -1093,6 +1093,59 @@ template <class T>
 void incr_assign(const DeviceLayout& L, T* lhs, T* x) {
   LAUNCH(k_incr_assign<T>, L.ntiles, L.d_boxes, L.d_tiles, geom(L), lhs, x);
 }
+// The V-cycle's correction updates, each the arithmetic of a sequence of the
+// reference's LevelDataOps calls in one pass over the valid cells (ghost values are
+// refilled before any later read):
+// setToZero(lhs) then incr(lhs, x, 1):  lhs = 0 + x * 1 (lhs not read)
+template <class T>
+__global__ void __launch_bounds__(256) k_incr_from_zero(const DBox* __restrict__ boxes,
+                                                        const Tile* __restrict__ tiles, TileGeom g, T* lhs,
+                                                        const T* x) {
+  DBox b;
+  int i, j, k;
+  if (!tile_cell(boxes, tiles, g, b, i, j, k)) return;
+  const long long c = cidx(b, i, j, k);
+  lhs[c] = T(0.0) + x[c] * T(1.0);
+}
+template <class T>
+void incr_from_zero(const DeviceLayout& L, T* lhs, const T* x) {
+  LAUNCH(k_incr_from_zero<T>, L.ntiles, L.d_boxes, L.d_tiles, geom(L), lhs, x);
+}
+// incr(m, d, 1), assignLocal(d, m), incr(phi, d, 1), setToZero(d)
+template <class T>
+__global__ void __launch_bounds__(256) k_incr_assign_update(const DBox* __restrict__ boxes,
+                                                            const Tile* __restrict__ tiles, TileGeom g, T* m, T* d,
+                                                            T* phi) {
+  DBox b;
+  int i, j, k;
+  if (!tile_cell(boxes, tiles, g, b, i, j, k)) return;
+  const long long c = cidx(b, i, j, k);
+  const T v = m[c] + d[c] * T(1.0);
+  m[c] = v;
+  phi[c] = phi[c] + v * T(1.0);
+  d[c] = T(0.0);
+}
+template <class T>
+void incr_assign_update(const DeviceLayout& L, T* m, T* d, T* phi) {
+  LAUNCH(k_incr_assign_update<T>, L.ntiles, L.d_boxes, L.d_tiles, geom(L), m, d, phi);
+}
+// setToZero(u), incr(u, x, 1), incr(phi, u, 1), setToZero(u): phi += (0 + x * 1) * 1
+// (u is zero before and after and is not touched)
+template <class T>
+__global__ void __launch_bounds__(256) k_incr_phi_from_zero(const DBox* __restrict__ boxes,
+                                                            const Tile* __restrict__ tiles, TileGeom g, T* phi,
+                                                            const T* x) {
+  DBox b;
+  int i, j, k;
+  if (!tile_cell(boxes, tiles, g, b, i, j, k)) return;
+  const long long c = cidx(b, i, j, k);
+  const T u = T(0.0) + x[c] * T(1.0);
+  phi[c] = phi[c] + u * T(1.0);
+}
+template <class T>
+void incr_phi_from_zero(const DeviceLayout& L, T* phi, const T* x) {
+  LAUNCH(k_incr_phi_from_zero<T>, L.ntiles, L.d_boxes, L.d_tiles, geom(L), phi, x);
+}
 template <class T>
 __global__ void __launch_bounds__(256) k_copy_valid(const DBox* __restrict__ boxes, const Tile* __restrict__ tiles,
                                                     TileGeom g, T* dst, const T* src) {
@@ -1664,6 +1717,9 @@ void reflux(Context* ctx, const RefluxCell* cells, int ncells, const RefluxCoars
                                    const T*, const T*, const T*, double, double, double, double, double, double); \
   template void incr<T>(const DeviceLayout&, T*, const T*, double);                                            \
   template void incr_assign<T>(const DeviceLayout&, T*, T*);                                                   \
+  template void incr_from_zero<T>(const DeviceLayout&, T*, const T*);                                          \
+  template void incr_assign_update<T>(const DeviceLayout&, T*, T*, T*);                                        \
+  template void incr_phi_from_zero<T>(const DeviceLayout&, T*, const T*);                                                   \
   template void axby<T>(const DeviceLayout&, T*, const T*, const T*, double, double);                          \
   template void scale<T>(const DeviceLayout&, T*, double);                                                     \
   template void setval<T>(const DeviceLayout&, T*, double);                                                    \

@@ -106,6 +106,15 @@ void incr(const DeviceLayout& L, T* lhs, const T* x, double scale);
 // lhs = lhs + x * 1.0; x = lhs (valid cells)
 template <class T>
 void incr_assign(const DeviceLayout& L, T* lhs, T* x);
+// lhs = 0 + x * 1 (setToZero then incr, lhs not read)
+template <class T>
+void incr_from_zero(const DeviceLayout& L, T* lhs, const T* x);
+// m += d; d = m; phi += d; d = 0 (incr, assignLocal, incr, setToZero)
+template <class T>
+void incr_assign_update(const DeviceLayout& L, T* m, T* d, T* phi);
+// phi += (0 + x * 1) * 1 (an incr into a zero field, then into phi)
+template <class T>
+void incr_phi_from_zero(const DeviceLayout& L, T* phi, const T* x);
 template <class T>
 void axby(const DeviceLayout& L, T* lhs, const T* x, const T* y, double a, double b);
 template <class T>
