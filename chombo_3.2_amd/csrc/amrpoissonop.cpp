@@ -554,7 +554,8 @@ void AMRPoissonOp<Real>::reflux(T& phiFine, T& phi, T& res, AMRPoissonOp* finerO
 // ---------------------------------------------------------------------------
 // DeviceHierarchy / factory
 // ---------------------------------------------------------------------------
-void DeviceHierarchy::define(Context* c, const std::vector<DBL>& g, const Box& domain0, double dx0, bool merge) {
+void DeviceHierarchy::define(Context* c, const std::vector<DBL>& g, const Box& domain0, double dx0, bool merge,
+                             const BCSpec* bc) {
   ctx = c;
   grids = g;
   domains.assign(g.size(), domain0);
@@ -563,12 +564,35 @@ void DeviceHierarchy::define(Context* c, const std::vector<DBL>& g, const Box& d
     domains[l] = refine(domains[l - 1], 2);
     dxs[l] = dxs[l - 1] / 2;
   }
+  // periodic directions: a layout with a box on a periodic face is stored per box
+  // (its ghosts there come from periodic images); the levels above the base stay 2
+  // coarse cells (4 fine cells) away from periodic faces, so coarse-fine stencils
+  // and flux registers never reach across them
+  bool per[3] = {false, false, false};
+  if (bc)
+    for (int d = 0; d < 3; d++) {
+      if ((bc->type[0][d] == 3) != (bc->type[1][d] == 3))
+        throw std::runtime_error("a periodic direction needs BC type 3 on both faces");
+      per[d] = bc->type[0][d] == 3;
+    }
+  for (size_t l = 1; l < g.size(); l++)
+    for (int d = 0; d < 3; d++)
+      if (per[d])
+        for (const Box& b : g[l].boxes)
+          if (b.lo[d] < domains[l].lo[d] + 4 || b.hi[d] > domains[l].hi[d] - 4)
+            throw std::runtime_error("a refined level comes within 2 coarse cells of a periodic face");
+  auto touches = [&](const DBL& d, const Box& dom) {
+    for (const Box& b : d.boxes)
+      for (int q = 0; q < 3; q++)
+        if (per[q] && (b.lo[q] == dom.lo[q] || b.hi[q] == dom.hi[q])) return true;
+    return false;
+  };
   amrLay.clear();
   cfLay.clear();
   mgLay.clear();
   for (size_t l = 0; l < g.size(); l++) {
     if (!g[l].sorted()) throw std::runtime_error("boxes of a level must be sorted (Box::operator<)");
-    amrLay.emplace_back(new DeviceLayout(c, g[l], domains[l], merge));
+    amrLay.emplace_back(new DeviceLayout(c, g[l], domains[l], merge && !touches(g[l], domains[l])));
     if (l == 0) cfLay.emplace_back(nullptr);
     else {
       if (!g[l].coarsenable(2)) throw std::runtime_error("fine level boxes must be coarsenable by 2");
@@ -580,7 +604,9 @@ void DeviceHierarchy::define(Context* c, const std::vector<DBL>& g, const Box& d
   for (int depth = 1;; depth++) {
     int coarsening = 1 << depth;
     if (!g[0].coarsenable(coarsening * 2)) break;
-    mgLay.emplace_back(new DeviceLayout(c, g[0].coarsened(coarsening), coarsen(domain0, coarsening), merge));
+    const DBL gc = g[0].coarsened(coarsening);
+    const Box dc = coarsen(domain0, coarsening);
+    mgLay.emplace_back(new DeviceLayout(c, gc, dc, merge && !touches(gc, dc)));
   }
 }
 
@@ -600,7 +626,11 @@ static bool in_grown(const DBL& d, int b, const int lo[3], const int hi[3], int 
 static void validate_ghost_plan(const ops::GhostPlan& P, const DBL& d) {
   for (const Patch& p : P.h_patches) {
     require(in_grown(d, p.dst, p.lo, p.hi, 1), "ghost patch outside its box");
-    if (p.kind == PK_EXCHANGE) require(in_grown(d, p.src, p.lo, p.hi, 0), "exchange source outside its box");
+    if (p.kind == PK_EXCHANGE) {
+      const int slo[3] = {p.lo[0] + p.sh[0], p.lo[1] + p.sh[1], p.lo[2] + p.sh[2]};
+      const int shi[3] = {p.hi[0] + p.sh[0], p.hi[1] + p.sh[1], p.hi[2] + p.sh[2]};
+      require(in_grown(d, p.src, slo, shi, 0), "exchange source outside its box");
+    }
     else {
       // the cells read (one and two inward) must stay inside the grown box
       int lo[3] = {p.lo[0], p.lo[1], p.lo[2]}, hi[3] = {p.hi[0], p.hi[1], p.hi[2]};

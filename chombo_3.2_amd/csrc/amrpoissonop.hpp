@@ -160,7 +160,8 @@ struct DeviceHierarchy {
   std::vector<std::unique_ptr<DeviceLayout>> cfLay;   // coarsen(grids[l], 2); index l >= 1
   std::vector<std::unique_ptr<DeviceLayout>> mgLay;   // MG levels of AMR level 0, depth >= 1
   // merge: store each level whose boxes tile a rectangle as one array
-  void define(Context* c, const std::vector<DBL>& g, const Box& domain0, double dx0, bool merge = true);
+  void define(Context* c, const std::vector<DBL>& g, const Box& domain0, double dx0, bool merge = true,
+              const BCSpec* bc = nullptr);
   int nlev() const { return (int)grids.size(); }
   int mgDepth() const { return 1 + (int)mgLay.size(); }
 };

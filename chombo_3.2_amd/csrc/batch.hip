@@ -1000,7 +1000,8 @@ BatchSolver::BatchSolver(Context* ctx, std::vector<BatchProblemSpec> problems, i
       for (int dir = 0; dir < 3; dir++)
         for (int side = 0; side < 2; side++) {
           if (P.bc.type[side][dir] > 1)
-            throw std::runtime_error("batch: quadratic Dirichlet faces are not supported (use the single solver)");
+            throw std::runtime_error("batch: quadratic Dirichlet and periodic faces are not supported "
+                                     "(use the single solver)");
           c.kind[dir][side] = P.bc.type[side][dir];
           c.val[dir][side] = P.bc.val[side][dir];
         }

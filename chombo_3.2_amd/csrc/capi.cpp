@@ -152,7 +152,7 @@ static std::vector<DBL> parse_spec(const int* is, const double* rs, Box& dom, BC
     bc.val[0][d] = rs[3 + d];
     bc.val[1][d] = rs[6 + d];
     for (int s = 0; s < 2; s++)
-      if (bc.type[s][d] < 0 || bc.type[s][d] > 2) throw std::runtime_error("unknown BC type");
+      if (bc.type[s][d] < 0 || bc.type[s][d] > 3) throw std::runtime_error("unknown BC type");
   }
   dx0 = rs[0];
   a = rs[1];
@@ -213,7 +213,7 @@ int amrmg_solver_create(amrmg_ctx* ctx, const int* ispec, const double* rspec, i
   Box dom;
   double dx0;
   std::vector<DBL> g = parse_spec(ispec, rspec, dom, s->bc, dx0, s->alpha, s->beta);
-  s->H.define(ctx->c.get(), g, dom, dx0);
+  s->H.define(ctx->c.get(), g, dom, dx0, true, &s->bc);
   s->fac.define(&s->H, s->alpha, s->beta, s->bc);
   s->bottom.reset(new DeviceBottomSolver<double>(ctx->c.get()));
   s->amg.define(s->fac, s->bottom.get(), s->nlev());
@@ -1019,10 +1019,10 @@ double serve_now(const std::chrono::steady_clock::time_point& t0) {
 }
 // HBM footprint of a solver for a spec: its layouts built on the host only (a dry
 // context), counted as amrmg_solver_device_bytes counts a created solver's fields
-long long serve_footprint(const std::vector<DBL>& g, const Box& dom, double dx0, int precision) {
+long long serve_footprint(const std::vector<DBL>& g, const Box& dom, double dx0, int precision, const BCSpec& bc) {
   Context dry(0, true);
   DeviceHierarchy H;
-  H.define(&dry, g, dom, dx0);
+  H.define(&dry, g, dom, dx0, true, &bc);
   return hierarchy_field_bytes(H, precision);
 }
 struct ServeInstance {
@@ -1237,7 +1237,7 @@ int amrmg_serve_submit(amrmg_serve* e, const int* ispec, const double* rspec, co
   r->rhs.assign(rhs, rhs + ncell);
   r->key.assign((const char*)ispec, sizeof(int) * nint);
   r->key.append((const char*)rspec, sizeof(double) * 9);
-  r->bytes = serve_footprint(g, dom, dx0, e->precision);
+  r->bytes = serve_footprint(g, dom, dx0, e->precision, bc);
   r->cells = ncell;
   std::lock_guard<std::mutex> lk(e->mu);
   r->large = ncell >= e->largeCells;
@@ -1311,7 +1311,7 @@ int amrmg_plan_check(const int* ispec, const double* rspec, long long* stats, in
   BCSpec bc{};
   std::vector<DBL> g = parse_spec(ispec, rspec, dom, bc, dx0, a, b);
   DeviceHierarchy H;
-  H.define(&dry, g, dom, dx0);
+  H.define(&dry, g, dom, dx0, true, &bc);
   AMRPoissonOpFactory<double> fac;
   fac.define(&H, a, b, bc);
   BiCGStabSolver<LevelData<double>> bottom;

@@ -43,6 +43,7 @@ enum PatchKind : int {
 struct Patch {
   int kind, dst, src, dir, side;
   int lo[3], hi[3];
+  int sh[3];  // PK_EXCHANGE: ghost cell c takes box src's cell c + sh (a periodic image; else 0)
   double val;
 };
 struct PatchWork {  // a patch split into chunks of at most 256 cells

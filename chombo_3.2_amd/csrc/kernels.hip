@@ -414,7 +414,7 @@ __global__ void __launch_bounds__(256) k_fill_ghosts(const DBox* __restrict__ bo
   const long long g = cidx(bd, i, j, k);
   if (p.kind == PK_EXCHANGE) {
     const DBox bs = boxes[p.src];
-    phi[g] = phi[cidx(bs, i, j, k)];
+    phi[g] = phi[cidx(bs, i + p.sh[0], j + p.sh[1], k + p.sh[2])];
     return;
   }
   const int isign = p.side ? 1 : -1;

@@ -79,7 +79,7 @@ std::vector<int> BoxFinder::overlapping(const Box& r0) const {
 static Patch make_patch(int kind, int dst, int src, int dir, int side, const Box& r, double val) {
   Patch p;
   p.kind = kind; p.dst = dst; p.src = src; p.dir = dir; p.side = side; p.val = val;
-  for (int q = 0; q < 3; q++) { p.lo[q] = r.lo[q]; p.hi[q] = r.hi[q]; }
+  for (int q = 0; q < 3; q++) { p.lo[q] = r.lo[q]; p.hi[q] = r.hi[q]; p.sh[q] = 0; }
   return p;
 }
 
@@ -91,7 +91,10 @@ static Patch make_patch(int kind, int dst, int src, int dir, int side, const Box
 // 7-point stencil and are not filled.
 // ---------------------------------------------------------------------------
 // With merged storage (one array per level) an exchange copies every cell onto
-// itself, so no exchange patch is emitted.
+// itself, so no exchange patch is emitted.  In a periodic direction a face ghost
+// outside the domain is filled from the periodic image, the boxes shifted by the
+// domain length (the NeighborIterator images of Copier::exchangeDefine); such a
+// level is stored per box.
 ops::GhostPlan build_ghost_plan(const DBL& dbl, const Box& domain, const BCSpec& bc, bool withCF, bool merged) {
   ops::GhostPlan P;
   BoxFinder bf(dbl);
@@ -102,6 +105,22 @@ ops::GhostPlan build_ghost_plan(const DBL& dbl, const Box& domain, const BCSpec&
         Box R = adjCell(bx, dir, side, 1);
         int vend = side ? bx.hi[dir] : bx.lo[dir];
         int dend = side ? domain.hi[dir] : domain.lo[dir];
+        if (vend == dend && bc.type[side][dir] == 3) {
+          if (merged) throw std::runtime_error("ghost plan: a periodic face on a merged level");
+          const int len = domain.len(dir), shift = side ? -len : len;
+          Box img = R;
+          img.lo[dir] += shift;
+          img.hi[dir] += shift;
+          for (int n : bf.overlapping(img)) {
+            Box r = img & dbl[n];
+            r.lo[dir] -= shift;
+            r.hi[dir] -= shift;
+            Patch pt = make_patch(PK_EXCHANGE, b, n, dir, side, r, 0.0);
+            pt.sh[dir] = shift;
+            P.h_patches.push_back(pt);
+          }
+          continue;
+        }
         if (vend == dend) {
           int t = bc.type[side][dir];
           const int kind = t == 1 ? PK_DIRICHLET : (t == 2 ? PK_DIRICHLET2 : PK_NEUMANN);

@@ -11,7 +11,8 @@
 namespace amrmg {
 
 struct BCSpec {
-  int type[2][3];     // [side][dir]: 0 Neumann, 1 Dirichlet (linear), 2 Dirichlet (quadratic, DiriBC order 2)
+  int type[2][3];     // [side][dir]: 0 Neumann, 1 Dirichlet (linear), 2 Dirichlet (quadratic, DiriBC order 2),
+                      // 3 periodic (both sides of the direction)
   double val[2][3];
 };
 

@@ -98,7 +98,7 @@ class Hierarchy:
     dx0: float
     alpha: float = 0.0
     beta: float = 1.0
-    bc_type_lo: Tuple[int, int, int] = (1, 1, 1)  # 0 Neumann, 1 Dirichlet, 2 Dirichlet (quadratic rule)
+    bc_type_lo: Tuple[int, int, int] = (1, 1, 1)  # 0 Neumann, 1 Dirichlet, 2 Dirichlet (quadratic), 3 periodic
     bc_type_hi: Tuple[int, int, int] = (1, 1, 1)
     bc_val_lo: Tuple[float, float, float] = (0.0, 0.0, 0.0)
     bc_val_hi: Tuple[float, float, float] = (0.0, 0.0, 0.0)

@@ -17,7 +17,8 @@
  *  - A hierarchy is described by (ispec, rspec):
  *      ispec[0]            nlev (AMR levels, refinement ratio 2)
  *      ispec[1..6]         level-0 domain box lo0 lo1 lo2 hi0 hi1 hi2
- *      ispec[7..9]         BC type on the lo faces (0 Neumann, 1 Dirichlet)
+ *      ispec[7..9]         BC type on the lo faces (0 Neumann, 1 Dirichlet, 2 Dirichlet
+ *                          quadratic, 3 periodic)
  *      ispec[10..12]       BC type on the hi faces
  *      ispec[13..13+nlev)  number of boxes per level
  *      then 6 ints per box (lo0 lo1 lo2 hi0 hi1 hi2), level by level, each level
@@ -42,7 +43,10 @@ extern "C" {
 
 /* BC type codes in ispec: 0 Neumann (ConstBCFunction type 0), 1 Dirichlet with the
    linear rule 2v - near (ConstBCFunction type 1), 2 Dirichlet with DiriBC's order-2
-   rule (8/3)v + (1/3)far - 2near (BCFunc.cpp:34-40, 291-354); values are constant per face. */
+   rule (8/3)v + (1/3)far - 2near (BCFunc.cpp:34-40, 291-354); values are constant per face.
+   3 periodic, on both faces of a direction (ProblemDomain's isPeriodic; the ghosts come
+   from the boxes' periodic images as Copier::exchangeDefine takes them, Copier.cpp:1464-1541);
+   every level above the base must stay at least 2 coarse cells away from periodic faces. */
 typedef struct amrmg_ctx amrmg_ctx;
 typedef struct amrmg_solver amrmg_solver;
 

@@ -224,6 +224,12 @@ static void fab_copy_region(Fab* dst, const Fab* src, Box r) {
     for (int j = r.lo[1]; j <= r.hi[1]; j++)
       for (int i = r.lo[0]; i <= r.hi[0]; i++) *P(dst,i,j,k) = *P(src,i,j,k);
 }
+/* the same from a periodic image: cell c of dst takes src's cell c - sh */
+static void fab_copy_region_sh(Fab* dst, const Fab* src, Box r, const int sh[3]) {
+  for (int k = r.lo[2]; k <= r.hi[2]; k++)
+    for (int j = r.lo[1]; j <= r.hi[1]; j++)
+      for (int i = r.lo[0]; i <= r.hi[0]; i++) *P(dst,i,j,k) = *P(src,i-sh[0],j-sh[1],k-sh[2]);
+}
 
 /* DisjointBoxLayout: boxes in Box::operator< order (BoxLayout::close sorts) */
 typedef struct { int n; Box* b; } Layout;
@@ -483,6 +489,7 @@ static void apply_bc(Fab* st, const Box* valid, const Box* domain, double dx, in
       int dend = side ? domain->hi[idir] : domain->lo[idir];
       if (vend != dend) continue;
       int type = bc->type[side][idir];
+      if (type == 3) continue;   /* periodic direction: no BC (ghosts come from the exchange) */
       double value = bc->val[side][idir];
       int isign = side ? 1 : -1;
       Box to = b_isect(b_adj(*valid, idir, side, 1), &st->b);
@@ -528,12 +535,20 @@ static void apply_bc(Fab* st, const Box* valid, const Box* domain, double dx, in
 /* ======================================================================= */
 /* Copier motion plans (lib/src/BoxTools/Copier.cpp)                       */
 /* ======================================================================= */
-typedef struct { int from, to; Box region; } Motion;
+/* region: the cells written (toRegion); sh: the periodic shift of the source box,
+   whose cells are region - sh (fromRegion = unshift(toRegion)); zero otherwise */
+typedef struct { int from, to; Box region; int sh[3]; } Motion;
 typedef struct { int n, cap; Motion* m; } Plan;
 
-static void plan_push(Plan* p, int from, int to, Box r) {
+static void plan_push_sh(Plan* p, int from, int to, Box r, const int sh[3]) {
   if (p->n == p->cap) { p->cap = p->cap ? 2*p->cap : 16; p->m = realloc(p->m, sizeof(Motion)*p->cap); }
-  p->m[p->n].from = from; p->m[p->n].to = to; p->m[p->n].region = r; p->n++;
+  p->m[p->n].from = from; p->m[p->n].to = to; p->m[p->n].region = r;
+  for (int d = 0; d < 3; d++) p->m[p->n].sh[d] = sh[d];
+  p->n++;
+}
+static void plan_push(Plan* p, int from, int to, Box r) {
+  const int zero[3] = {0, 0, 0};
+  plan_push_sh(p, from, to, r, zero);
 }
 static void plan_free(Plan* p) { free(p->m); p->m = NULL; p->n = p->cap = 0; }
 /* MotionItemSorter (Copier.cpp:1547-1563): by toRegion, smallEnd then bigEnd */
@@ -543,26 +558,43 @@ static int motion_cmp(const void* a, const void* b) {
   if (b_lt(&y->region, &x->region)) return 1;
   return 0;
 }
-/* Copier::exchangeDefine + trimEdges (Copier.cpp:1464-1541, 142-177) */
-static Plan plan_exchange(const Layout* L) {
+/* Copier::exchangeDefine + trimEdges (Copier.cpp:1464-1541, 142-177).  With a
+   periodic domain the NeighborIterator also yields every box's periodic images
+   (the box shifted by -L, 0 or +L in each periodic direction of domain length L,
+   a box's own unshifted self excluded); an image's cells are copied from the box
+   itself (nit.unshift).  periodic[d]: direction d is periodic. */
+static Plan plan_exchange_p(const Layout* L, const Box* domain, const int periodic[3]) {
   Plan p = {0,0,NULL};
+  int len[3], ns[3];
+  for (int d = 0; d < 3; d++) {
+    len[d] = domain->hi[d] - domain->lo[d] + 1;
+    ns[d] = periodic[d] ? 3 : 1;
+  }
   for (int t = 0; t < L->n; t++) {
     Box bg = b_grow(L->b[t], 1);
-    for (int f = 0; f < L->n; f++) {
-      if (f == t) continue;
-      Box r = b_isect(L->b[f], &bg);
-      if (b_empty(&r)) continue;
-      int keep = 0;
-      for (int d = 0; d < 3 && !keep; d++) {
-        Box g = b_growd(L->b[t], d, 1);
-        if (b_intersects(&g, &r)) keep = 1;
-      }
-      if (keep) plan_push(&p, f, t, r);
-    }
+    for (int f = 0; f < L->n; f++)
+      for (int a = 0; a < ns[0]; a++)
+        for (int b = 0; b < ns[1]; b++)
+          for (int c = 0; c < ns[2]; c++) {
+            const int sh[3] = {periodic[0] ? (a - 1) * len[0] : 0, periodic[1] ? (b - 1) * len[1] : 0,
+                               periodic[2] ? (c - 1) * len[2] : 0};
+            if (f == t && sh[0] == 0 && sh[1] == 0 && sh[2] == 0) continue;
+            Box img = L->b[f];
+            for (int d = 0; d < 3; d++) { img.lo[d] += sh[d]; img.hi[d] += sh[d]; }
+            Box r = b_isect(img, &bg);
+            if (b_empty(&r)) continue;
+            int keep = 0;
+            for (int d = 0; d < 3 && !keep; d++) {
+              Box g = b_growd(L->b[t], d, 1);
+              if (b_intersects(&g, &r)) keep = 1;
+            }
+            if (keep) plan_push_sh(&p, f, t, r, sh);
+          }
   }
   qsort(p.m, p.n, sizeof(Motion), motion_cmp);
   return p;
 }
+
 /* Copier::define(src layout, dest layout, ghost) (Copier.cpp:903-1140): for each
    dest box grown by ghost, every intersecting src box */
 static Plan plan_copy(const Layout* src, const Layout* dst, int dst_ghost) {
@@ -583,7 +615,7 @@ static void plan_reverse(Plan* p) {
 }
 static void exec_exchange(LD* x, const Plan* p) {
   for (int q = 0; q < p->n; q++)
-    fab_copy_region(&x->f[p->m[q].to], &x->f[p->m[q].from], p->m[q].region);
+    fab_copy_region_sh(&x->f[p->m[q].to], &x->f[p->m[q].from], p->m[q].region, p->m[q].sh);
 }
 /* LevelData::copyTo(..., copier) with plain copy */
 static void exec_copy(LD* dst, const LD* src, const Plan* p) {
@@ -1113,7 +1145,11 @@ static void op_define(Op* op, const Layout* L, const Box* domain, double dx, dou
   memcpy(op->L.b, L->b, sizeof(Box)*L->n);
   op->domain = *domain; op->dx = dx; op->dxCrse = 2*dx; op->alpha = alpha; op->beta = beta;
   op->bc = *bc; op->refToCoarser = 2; op->refToFiner = 2; op->relaxMode = 1;
-  op->ex = plan_exchange(&op->L);
+  {
+    int per[3];
+    for (int d = 0; d < 3; d++) per[d] = bc->type[0][d] == 3;
+    op->ex = plan_exchange_p(&op->L, domain, per);
+  }
   op->cf = calloc((size_t)L->n*6, sizeof(CFIVS));
   for (int b = 0; b < L->n; b++)
     for (int d = 0; d < 3; d++)
@@ -1793,6 +1829,21 @@ OHier* oracle_create(const int* is, const double* rs) {
     for (int q = 1; q < n; q++)
       if (!b_lt(&h->grids[l].b[q-1], &h->grids[l].b[q])) { set_err("boxes not sorted"); return NULL; }
     if (l > 0) { h->domains[l] = b_refine(h->domains[l-1], 2); h->dx[l] = h->dx[l-1]/2; }
+  }
+  /* periodic directions (BC type 3 on both faces): the levels above the base stay at
+     least 2 coarse cells (4 fine cells) away from the periodic faces, so their
+     coarse-fine stencils and flux registers never reach across them */
+  for (int d = 0; d < 3; d++) {
+    if ((h->bc.type[0][d] == 3) != (h->bc.type[1][d] == 3)) { set_err("periodic on one face only"); return NULL; }
+    if (h->bc.type[0][d] != 3) continue;
+    for (int l = 1; l < nlev; l++)
+      for (int q = 0; q < h->grids[l].n; q++) {
+        const Box* b = &h->grids[l].b[q];
+        if (b->lo[d] < h->domains[l].lo[d] + 4 || b->hi[d] > h->domains[l].hi[d] - 4) {
+          set_err("a refined level comes within 2 coarse cells of a periodic face");
+          return NULL;
+        }
+      }
   }
   /* AMRPoissonOpFactory::AMRnewOp */
   h->op = calloc(nlev, sizeof(Op));
