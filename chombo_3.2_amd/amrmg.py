@@ -20,6 +20,22 @@ HEADER = os.path.join(os.path.dirname(_HERE), "include", "amrmg.h")
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
+# amrmg_bc_value_fn: BCValueFunc (BCFunc.H:205-208) with a caller pointer
+BC_VALUE_FN = C.CFUNCTYPE(None, _dp, _ip, _ip, _dp, C.c_void_p)
+
+
+def bc_value_callback(fn):
+    """An amrmg_bc_value_fn from fn(pos, dir, side) -> value (pos a 3-tuple of face
+    coordinates, side 0 low / 1 high); None gives a NULL pointer.  Keep the result alive while
+    the library may call it."""
+    if fn is None:
+        return C.cast(None, BC_VALUE_FN)
+    if isinstance(fn, BC_VALUE_FN):
+        return fn
+
+    def cb(pos, d, s, v, _user):
+        v[0] = float(fn((pos[0], pos[1], pos[2]), d[0], s[0]))
+    return BC_VALUE_FN(cb)
 _lib = None
 
 KERNEL_CLASSES = ["gsrb", "residual", "apply", "restrict", "prolong", "average", "ghost", "blas",
@@ -74,6 +90,7 @@ def lib():
         L.amrmg_solver_create.argtypes = [C.c_void_p, _ip, _dp, C.c_int, C.POINTER(C.c_void_p)]
         L.amrmg_solver_destroy.argtypes = [C.c_void_p]
         L.amrmg_solver_set_params.argtypes = [C.c_void_p, _ip, _dp]
+        L.amrmg_solver_set_bc_values.argtypes = [C.c_void_p, BC_VALUE_FN, C.c_void_p]
         L.amrmg_solver_level_cells.restype = C.c_longlong
         L.amrmg_solver_level_cells.argtypes = [C.c_void_p, C.c_int]
         L.amrmg_solver_num_mg_depths.argtypes = [C.c_void_p]
@@ -249,6 +266,14 @@ class AMRMultiGrid:
     def set_alpha_beta(self, alpha, beta):
         """AMRPoissonOp::setAlphaAndBeta on every operator of the solver."""
         _chk(lib().amrmg_solver_set_alpha_beta(self.p, float(alpha), float(beta)))
+
+    def set_bc_values(self, fn):
+        """DiriBC / NeumBC with a BCValueHolder on every operator: the inhomogeneous
+        value of each Dirichlet or Neumann face cell is fn(pos, dir, side) at the face
+        centre (BCFunc.cpp:187-202); None restores the constant face values."""
+        cb = bc_value_callback(fn)
+        _chk(lib().amrmg_solver_set_bc_values(self.p, cb, None))
+        self._bc_fn = cb
 
     def set_vc_average(self, harmonic):
         """Coefficient coarsening of the next set_vc_coefs: arithmetic (False, the

@@ -705,9 +705,20 @@ void defineLevelOp(AMRPoissonOp<Real>* op, Context* ctx, const DeviceLayout* lay
   op->m_mgCoarse = mgCoarse;
   op->m_ghost = build_ghost_plan(lay->dbl, domain, bc, dxCrse > 0, lay->merged);
   validate_ghost_plan(op->m_ghost, lay->dbl);
+  if (bc.fn) attach_bc_values(op->m_ghost, dx, bc);
   op->m_ghost.upload(ctx);
   setup_fused(op);
   if (op->m_mgCoarse) validate_pair(*op->m_lay, *op->m_mgCoarse);
+}
+
+// A BCValueHolder for the operator's domain faces (fn null: the face constants again);
+// every later inhomogeneous ghost refresh uses the values fn gives here.
+template <class Real>
+void setLevelOpBCValues(AMRPoissonOp<Real>* op, BCValueFn fn, void* user) {
+  op->m_bc.fn = fn;
+  op->m_bc.user = user;
+  attach_bc_values(op->m_ghost, op->m_dx, op->m_bc);
+  op->m_ghost.upload(op->m_ctx);
 }
 
 // The coarse-fine part of AMRnewOp for a level with a coarser AMR level (refinement
@@ -1028,6 +1039,8 @@ void AMRMultiGrid<Real>::solveNoInitResid(std::vector<T*>& phi, std::vector<T*>&
   m_exitStatus = int(!goRedu) + int(!goIter) * 2 + int(!goHang) * 4 + int(!goNorm) * 8;
 }
 
+template void setLevelOpBCValues<double>(AMRPoissonOp<double>*, BCValueFn, void*);
+template void setLevelOpBCValues<float>(AMRPoissonOp<float>*, BCValueFn, void*);
 template void defineLevelOp<double>(AMRPoissonOp<double>*, Context*, const DeviceLayout*, const Box&, double, double,
                                     double, double, const BCSpec&, const DeviceLayout*);
 template void defineLevelOp<float>(AMRPoissonOp<float>*, Context*, const DeviceLayout*, const Box&, double, double,

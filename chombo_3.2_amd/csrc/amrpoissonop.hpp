@@ -142,6 +142,8 @@ class AMRPoissonOp : public AMRLevelOp<LevelData<Real>> {
 template <class Real>
 void defineLevelOp(AMRPoissonOp<Real>* op, Context* ctx, const DeviceLayout* lay, const Box& domain, double dx,
                    double dxCrse, double alpha, double beta, const BCSpec& bc, const DeviceLayout* mgCoarse);
+template <class Real>
+void setLevelOpBCValues(AMRPoissonOp<Real>* op, BCValueFn fn, void* user);
 // The coarse-fine tables between op's level and the coarser AMR level of coarserOp
 // (ratio 2): QuadCFInterp, the restriction / prolongation copiers through cfLay =
 // coarsen(op's level, 2), and coarserOp's reflux tables and zeroCovered plan.

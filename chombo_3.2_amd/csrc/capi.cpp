@@ -279,6 +279,28 @@ int amrmg_solver_set_alpha_beta(amrmg_solver* s, double alpha, double beta) {
   }
   API_END
 }
+// DiriBC / NeumBC with a BCValueHolder (BCFunc.cpp:217-258, 291-354) on every
+// operator of the solver: the AMR levels and multigrid depths (FP64 and the FP32
+// inner ones); the values are taken once, here, at every domain face cell.
+int amrmg_solver_set_bc_values(amrmg_solver* s, amrmg_bc_value_fn fn, void* user) {
+  API_BEGIN
+  s->bc.fn = fn;
+  s->bc.user = user;
+  s->fac.bc.fn = fn;
+  s->fac.bc.user = user;
+  for (auto& op : s->amg.m_op) setLevelOpBCValues(op.get(), fn, user);
+  for (int d = 1; d < s->H.mgDepth(); d++) setLevelOpBCValues(s->mgop(d), fn, user);
+  if (s->mixed) {
+    s->mixed->fac.bc.fn = fn;
+    s->mixed->fac.bc.user = user;
+    for (auto& op : s->mixed->amg.m_op) setLevelOpBCValues(op.get(), fn, user);
+    for (int d = 1; d < s->H.mgDepth(); d++) setLevelOpBCValues(s->mixedop(d), fn, user);
+  }
+  s->fabOps.clear();
+  s->fabFac.reset();
+  s->fabH.reset();
+  API_END
+}
 int amrmg_solver_set_vc_coefs(amrmg_solver* s, const double* a, const double* b0, const double* b1,
                               const double* b2) {
   API_BEGIN

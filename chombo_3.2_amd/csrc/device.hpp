@@ -44,6 +44,8 @@ struct Patch {
   int kind, dst, src, dir, side;
   int lo[3], hi[3];
   int sh[3];  // PK_EXCHANGE: ghost cell c takes box src's cell c + sh (a periodic image; else 0)
+  int voff;   // BC rules: -1 for the face constant val, else the offset of this patch's face
+              // values (one per ghost cell, x fastest) in the level's value array
   double val;
 };
 struct PatchWork {  // a patch split into chunks of at most 256 cells

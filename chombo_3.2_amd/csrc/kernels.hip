@@ -310,14 +310,20 @@ void GhostPlan::upload(Context* ctx) {
   dev_h2d(ctx, d_patches, h_patches.data(), sizeof(Patch) * npatch, "copy patches");
   dev_alloc(ctx, (void**)&d_work, sizeof(PatchWork) * nwork, "hipMalloc patch work");
   dev_h2d(ctx, d_work, work.data(), sizeof(PatchWork) * nwork, "copy patch work");
+  if (!h_bcv.empty()) {
+    dev_alloc(ctx, (void**)&d_bcv, sizeof(double) * h_bcv.size(), "hipMalloc BC values");
+    dev_h2d(ctx, d_bcv, h_bcv.data(), sizeof(double) * h_bcv.size(), "copy BC values");
+  }
 }
 void GhostPlan::release() {
   if (owner) {
     dev_free(owner, d_patches);
     dev_free(owner, d_work);
+    dev_free(owner, d_bcv);
   }
   d_patches = nullptr;
   d_work = nullptr;
+  d_bcv = nullptr;
 }
 void MotionPlan::upload(Context* ctx) {
   release();
@@ -394,8 +400,8 @@ __device__ __forceinline__ bool tile_cell(const DBox* __restrict__ boxes, const 
 // ---------------------------------------------------------------------------
 template <class T>
 __global__ void __launch_bounds__(256) k_fill_ghosts(const DBox* __restrict__ boxes, const Patch* __restrict__ patches,
-                                                     const PatchWork* __restrict__ work, T* __restrict__ phi,
-                                                     int doCF, int homog, double dx, double dxCrse) {
+                                                     const PatchWork* __restrict__ work, const double* __restrict__ bcv,
+                                                     T* __restrict__ phi, int doCF, int homog, double dx, double dxCrse) {
   const PatchWork w = work[blockIdx.x];
   const Patch p = patches[w.patch];
   // doCF: 0 exchange and BC, 1 exchange, BC and coarse-fine, 2 coarse-fine only,
@@ -421,20 +427,23 @@ __global__ void __launch_bounds__(256) k_fill_ghosts(const DBox* __restrict__ bo
   int a[3] = {i, j, k};
   a[p.dir] -= isign;  // one cell inward
   const T nearv = phi[cidx(bd, a[0], a[1], a[2])];
+  // the face value: the constant of the face, or this ghost cell's value from the
+  // BC value function
+  const double val = (homog || p.voff < 0) ? p.val : bcv[p.voff + q];
   if (p.kind == PK_DIRICHLET) {
     T inhomogVal = T(0.0);
-    if (!homog) inhomogVal = T(p.val);
+    if (!homog) inhomogVal = T(val);
     phi[g] = T(2) * inhomogVal - nearv;
   } else if (p.kind == PK_DIRICHLET2) {  // DiriBC order 2 (BCFunc.cpp:34-40, 291-354)
     int b2[3] = {a[0], a[1], a[2]};
     b2[p.dir] -= isign;  // two cells inward
     T inhomogVal = T(0.0);
-    if (!homog) inhomogVal = T(p.val);
+    if (!homog) inhomogVal = T(val);
     const T farv = phi[cidx(bd, b2[0], b2[1], b2[2])];
     phi[g] = T(8.0 / 3.0) * inhomogVal + T(1.0 / 3.0) * farv - T(2) * nearv;
   } else if (p.kind == PK_NEUMANN) {
     T v = nearv;
-    if (!homog) v += T(double(isign) * dx) * T(p.val);
+    if (!homog) v += T(double(isign) * dx) * T(val);
     phi[g] = v;
   } else if (p.kind == PK_CFHOMO) {
     int b2[3] = {a[0], a[1], a[2]};
@@ -450,7 +459,8 @@ __global__ void __launch_bounds__(256) k_fill_ghosts(const DBox* __restrict__ bo
 }
 template <class T>
 void fill_ghosts(const DeviceLayout& L, const GhostPlan& P, T* phi, int doCF, int homog, double dx, double dxCrse) {
-  LAUNCH_P(KC_GHOST, 0.0, k_fill_ghosts<T>, P.nwork, L.d_boxes, P.d_patches, P.d_work, phi, doCF, homog, dx, dxCrse);
+  LAUNCH_P(KC_GHOST, 0.0, k_fill_ghosts<T>, P.nwork, L.d_boxes, P.d_patches, P.d_work, P.d_bcv, phi, doCF, homog, dx,
+           dxCrse);
 }
 
 // ---------------------------------------------------------------------------

@@ -491,6 +491,11 @@ int amrmg_op_bc(amrmg_op* op, amrmg_field* phi, int homogeneous) {
   ops::fill_ghosts<double>(*p.m_lay, p.m_ghost, phi->ld.d, 4, homogeneous != 0, p.m_dx, p.m_dxCrse);
   API_END
 }
+int amrmg_op_set_bc_values(amrmg_op* op, amrmg_bc_value_fn fn, void* user) {
+  API_BEGIN
+  setLevelOpBCValues<double>(op->op.get(), fn, user);
+  API_END
+}
 int amrmg_op_cf_homog(amrmg_op* op, amrmg_field* phi) {
   API_BEGIN
   AMRPoissonOp<double>& p = aop(op);

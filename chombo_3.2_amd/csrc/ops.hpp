@@ -24,9 +24,11 @@ namespace ops {
 struct GhostPlan {  // face-ghost fill rules of one level (exchange + BC + homogeneous CF)
   Patch* d_patches = nullptr;
   PatchWork* d_work = nullptr;
+  double* d_bcv = nullptr;  // per-face-cell BC values (null: every rule uses its face constant)
   int npatch = 0, nwork = 0;
   Context* owner = nullptr;
   std::vector<Patch> h_patches;
+  std::vector<double> h_bcv;
   void upload(Context* ctx);
   void release();
 };

@@ -3,6 +3,7 @@
 #include "plans.hpp"
 
 #include <cmath>
+#include <cstdint>
 #include <map>
 #include <stdexcept>
 
@@ -78,7 +79,7 @@ std::vector<int> BoxFinder::overlapping(const Box& r0) const {
 
 static Patch make_patch(int kind, int dst, int src, int dir, int side, const Box& r, double val) {
   Patch p;
-  p.kind = kind; p.dst = dst; p.src = src; p.dir = dir; p.side = side; p.val = val;
+  p.kind = kind; p.dst = dst; p.src = src; p.dir = dir; p.side = side; p.val = val; p.voff = -1;
   for (int q = 0; q < 3; q++) { p.lo[q] = r.lo[q]; p.hi[q] = r.hi[q]; p.sh[q] = 0; }
   return p;
 }
@@ -145,6 +146,35 @@ ops::GhostPlan build_ghost_plan(const DBL& dbl, const Box& domain, const BCSpec&
       }
   }
   return P;
+}
+
+// DiriBC / NeumBC with a BCValueHolder (BCFunc.cpp:217-258, 291-354): the value of
+// ghost cell ivTo is the holder's at the centre of the domain face of the valid cell
+// next to it, getDomainFacePosition (BCFunc.cpp:187-202): dx*(iv+0.5) in every
+// direction, moved half a cell outward in dir.
+void attach_bc_values(ops::GhostPlan& P, double dx, const BCSpec& bc) {
+  P.h_bcv.clear();
+  for (Patch& p : P.h_patches) {
+    p.voff = -1;
+    if (!bc.fn || (p.kind != PK_DIRICHLET && p.kind != PK_DIRICHLET2 && p.kind != PK_NEUMANN)) continue;
+    const long long n = (long long)(p.hi[0] - p.lo[0] + 1) * (p.hi[1] - p.lo[1] + 1) * (p.hi[2] - p.lo[2] + 1);
+    if ((long long)P.h_bcv.size() + n > (long long)INT32_MAX) throw std::runtime_error("BC values: too many face cells");
+    p.voff = (int)P.h_bcv.size();
+    const int isign = p.side ? 1 : -1;
+    for (int k = p.lo[2]; k <= p.hi[2]; k++)
+      for (int j = p.lo[1]; j <= p.hi[1]; j++)
+        for (int i = p.lo[0]; i <= p.hi[0]; i++) {
+          int iv[3] = {i, j, k};
+          iv[p.dir] -= isign;  // ivClose
+          double pos[3];
+          for (int d = 0; d < 3; d++) pos[d] = dx * (iv[d] + 0.5);
+          pos[p.dir] += 0.5 * double(isign) * dx;
+          int dir = p.dir, side = p.side;
+          double v = 0.0;
+          bc.fn(pos, &dir, &side, &v, bc.user);
+          P.h_bcv.push_back(v);
+        }
+  }
 }
 
 ops::MotionPlan build_copier(const DBL& src, const DBL& dst) {

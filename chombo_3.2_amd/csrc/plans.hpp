@@ -10,10 +10,18 @@
 
 namespace amrmg {
 
+// BCValueFunc (BCFunc.H:205-208) with a caller pointer: the value at face position
+// pos of the face (dir, side), side 0 low and 1 high
+typedef void (*BCValueFn)(double* pos, int* dir, int* side, double* value, void* user);
+
 struct BCSpec {
   int type[2][3];     // [side][dir]: 0 Neumann, 1 Dirichlet (linear), 2 Dirichlet (quadratic, DiriBC order 2),
                       // 3 periodic (both sides of the direction)
   double val[2][3];
+  // When set, the inhomogeneous rules take their value per face cell from fn (at the
+  // face centre, as DiriBC / NeumBC call their BCValueHolder) instead of val.
+  BCValueFn fn = nullptr;
+  void* user = nullptr;
 };
 
 // point -> box lookup over a layout (bucketed)
@@ -36,6 +44,10 @@ class BoxFinder {
 // Face-ghost rules of one level: exchange from neighbours, BCs on the domain
 // boundary, homogeneous CF interpolation on coarse-fine faces (CFRegion/CFIVS).
 ops::GhostPlan build_ghost_plan(const DBL& dbl, const Box& domain, const BCSpec& bc, bool withCF, bool merged);
+
+// The face values of the plan's BC rules from bc.fn at level spacing dx (voff and
+// h_bcv; bc.fn null restores the face constants).  The plan must be uploaded again.
+void attach_bc_values(ops::GhostPlan& P, double dx, const BCSpec& bc);
 
 // Copier from src layout to dst layout (Copier::define with zero ghost).
 ops::MotionPlan build_copier(const DBL& src, const DBL& dst);

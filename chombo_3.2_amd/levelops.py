@@ -67,6 +67,7 @@ def _lib():
             "amrmg_op_max_norm": [_vp, _vp, _dp],
             "amrmg_op_exchange": [_vp, _vp],
             "amrmg_op_bc": [_vp, _vp, C.c_int],
+            "amrmg_op_set_bc_values": [_vp, amrmg.BC_VALUE_FN, _vp],
             "amrmg_op_cf_homog": [_vp, _vp],
             "amrmg_op_cf_inhomog": [_vp, _vp, _vp],
             "amrmg_op_gsrb": [_vp, _vp, _vp, C.c_int],
@@ -347,6 +348,13 @@ class AMRPoissonOp:
 
     def applyBC(self, phi, homogeneous):
         _chk(_lib().amrmg_op_bc(self.p, phi.p, int(homogeneous)))
+
+    def setBCValues(self, fn):
+        """This operator's inhomogeneous BC values from fn(pos, dir, side) at the domain
+        face centres (a BCValueHolder in DiriBC / NeumBC); None: the face constants."""
+        cb = amrmg.bc_value_callback(fn)
+        _chk(_lib().amrmg_op_set_bc_values(self.p, cb, None))
+        self._bc_fn = cb
 
     def homogeneousCFInterp(self, phi):
         _chk(_lib().amrmg_op_cf_homog(self.p, phi.p))

@@ -72,6 +72,18 @@ int amrmg_solver_set_params(amrmg_solver* s, const int* sparams, const double* r
    every operator of the solver (AMR levels and multigrid levels): the same as defining the solver
    with these coefficients. */
 int amrmg_solver_set_alpha_beta(amrmg_solver* s, double alpha, double beta);
+/* BCValueFunc (BCFunc.H:205-208) with a caller pointer: writes value[0], the boundary
+   value at face position pos[3] of the domain face (dir 0..2, side 0 low / 1 high). */
+typedef void (*amrmg_bc_value_fn)(double* pos, int* dir, int* side, double* value, void* user);
+/* DiriBC / NeumBC with a BCValueHolder (BCFunc.cpp:217-258, 291-354) in place of the
+   constant face values of ispec/rspec, on every operator of the solver: each
+   inhomogeneous ghost on a Dirichlet or Neumann face takes fn's value at the centre of
+   its domain face, pos = dx*(iv+0.5) of the valid cell next to it moved half a cell
+   outward (getDomainFacePosition, BCFunc.cpp:187-202, no origin offset).  fn is called
+   here, once per face cell of every level and multigrid depth, and must be a pure
+   function of its arguments; fn = NULL restores the constant values.  The BC types
+   stay those of ispec. */
+int amrmg_solver_set_bc_values(amrmg_solver* s, amrmg_bc_value_fn fn, void* user);
 /* Variable coefficients: the solver's operator becomes VCAMRPoissonOp2's
    alpha a(x) phi - beta div(b(x) grad phi) (VCAMRPoissonOp2Factory::define,
    VCAMRPoissonOp2.cpp:868-940, with MGnewOp's arithmetic coefficient averaging,
@@ -388,6 +400,9 @@ int amrmg_op_max_norm(amrmg_op* op, const amrmg_field* x, double* out);
    reflux (:1117-1201) of lphi with the finer level's phi */
 int amrmg_op_exchange(amrmg_op* op, amrmg_field* phi);
 int amrmg_op_bc(amrmg_op* op, amrmg_field* phi, int homogeneous);
+/* this operator's inhomogeneous BC values from fn (as amrmg_solver_set_bc_values; a
+   multigrid child made afterwards by amrmg_op_create_coarser inherits fn) */
+int amrmg_op_set_bc_values(amrmg_op* op, amrmg_bc_value_fn fn, void* user);
 int amrmg_op_cf_homog(amrmg_op* op, amrmg_field* phi);
 int amrmg_op_cf_inhomog(amrmg_op* op, amrmg_field* phi, const amrmg_field* phi_coarse);
 int amrmg_op_gsrb(amrmg_op* op, amrmg_field* phi, const amrmg_field* rhs, int color);

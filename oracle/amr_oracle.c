@@ -479,7 +479,26 @@ static void k_interphomolinear(Fab* phi, const Box* r, double x1, double dxCrse,
 /* ======================================================================= */
 /* ConstBCFunction (lib/src/AMRElliptic/BCFunc.cpp:58-158)                 */
 /* ======================================================================= */
-typedef struct { int type[2][3]; double val[2][3]; } BCSpec;
+/* BCValueFunc (BCFunc.H:205-208) with a caller pointer */
+typedef void (*BCValueFn)(double* pos, int* dir, int* side, double* value, void* user);
+/* fn: when set, the inhomogeneous value of each ghost cell comes from fn in place of val */
+typedef struct { int type[2][3]; double val[2][3]; BCValueFn fn; void* user; } BCSpec;
+
+/* The BCValueHolder call of DiriBC / NeumBC (BCFunc.cpp:242-250, 309-313) for ghost
+   cell (i,j,k) on face (idir, side): at getDomainFacePosition (BCFunc.cpp:187-202) of
+   the valid cell next to it, dx*(iv+0.5) moved half a cell outward in idir. */
+static double face_value(const BCSpec* bc, int i, int j, int k, int idir, int side, double dx) {
+  int isign = side ? 1 : -1;
+  int iv[3] = {i, j, k};
+  iv[idir] -= isign;
+  double pos[3];
+  for (int d = 0; d < 3; d++) pos[d] = dx*(iv[d] + 0.5);
+  pos[idir] += 0.5*(double)isign*dx;
+  int dir = idir, sd = side;
+  double v = 0.0;
+  bc->fn(pos, &dir, &sd, &v, bc->user);
+  return v;
+}
 
 static void apply_bc(Fab* st, const Box* valid, const Box* domain, double dx, int homog,
                      const BCSpec* bc) {
@@ -504,8 +523,10 @@ static void apply_bc(Fab* st, const Box* valid, const Box* domain, double dx, in
         if (!homog)
           for (int k = to.lo[2]; k <= to.hi[2]; k++)
             for (int j = to.lo[1]; j <= to.hi[1]; j++)
-              for (int i = to.lo[0]; i <= to.hi[0]; i++)
-                *P(st,i,j,k) += (double)isign*dx*value;
+              for (int i = to.lo[0]; i <= to.hi[0]; i++) {
+                double v = bc->fn ? face_value(bc, i, j, k, idir, side, dx) : value;
+                *P(st,i,j,k) += (double)isign*dx*v;
+              }
       } else if (type == 2) {    /* DiriBC order 2, quadraticInterp (BCFunc.cpp:34-40, 291-354) */
         for (int k = to.lo[2]; k <= to.hi[2]; k++)
           for (int j = to.lo[1]; j <= to.hi[1]; j++)
@@ -513,7 +534,7 @@ static void apply_bc(Fab* st, const Box* valid, const Box* domain, double dx, in
               int c[3] = {i,j,k}; c[idir] -= isign;
               int f[3] = {c[0],c[1],c[2]}; f[idir] -= isign;
               double inhomogVal = 0.0;
-              if (!homog) inhomogVal = value;
+              if (!homog) inhomogVal = bc->fn ? face_value(bc, i, j, k, idir, side, dx) : value;
               double nearVal = *P(st,c[0],c[1],c[2]);
               double farVal = *P(st,f[0],f[1],f[2]);
               *P(st,i,j,k) = (8.0/3.0)*inhomogVal + (1.0/3.0)*(farVal) - 2*(nearVal);
@@ -524,7 +545,7 @@ static void apply_bc(Fab* st, const Box* valid, const Box* domain, double dx, in
             for (int i = to.lo[0]; i <= to.hi[0]; i++) {
               int c[3] = {i,j,k}; c[idir] -= isign;
               double inhomogVal = 0.0;
-              if (!homog) inhomogVal = value;
+              if (!homog) inhomogVal = bc->fn ? face_value(bc, i, j, k, idir, side, dx) : value;
               double nearVal = *P(st,c[0],c[1],c[2]);
               *P(st,i,j,k) = 2*inhomogVal - nearVal;
             }
@@ -1948,6 +1969,13 @@ int oracle_vc_set(OHier* h, const double* a, const double* b0, const double* b1,
 }
 /* m_coefficient_average_type of VCAMRPoissonOp2Factory (VCAMRPoissonOp2.cpp:1048-1063):
    0 CoarseAverage::arithmetic, 1 harmonic; used by the next oracle_vc_set */
+/* DiriBC / NeumBC with a BCValueHolder on every operator (fn NULL: the face constants) */
+int oracle_set_bc_values(OHier* h, BCValueFn fn, void* user) {
+  h->bc.fn = fn; h->bc.user = user;
+  for (int l = 0; l < h->nlev; l++) { h->op[l].bc.fn = fn; h->op[l].bc.user = user; }
+  for (int d = 1; d < h->mgD; d++) { h->mgop[d].bc.fn = fn; h->mgop[d].bc.user = user; }
+  return 0;
+}
 int oracle_set_vc_average(OHier* h, int type) {
   if (type != 0 && type != 1) { set_err("coefficient average type must be 0 (arithmetic) or 1 (harmonic)"); return -1; }
   h->vcHarmonic = type;

@@ -17,6 +17,8 @@ _lib = None
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
+# BCValueFunc (BCFunc.H:205-208) with a caller pointer
+BC_VALUE_FN = C.CFUNCTYPE(None, _dp, _ip, _ip, _dp, C.c_void_p)
 
 
 def build():
@@ -83,6 +85,7 @@ def lib():
         L.oracle_face_cells.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.oracle_vc_lambda.argtypes = [C.c_void_p, C.c_int, _dp]
         L.oracle_set_alpha_beta.argtypes = [C.c_void_p, C.c_double, C.c_double]
+        L.oracle_set_bc_values.argtypes = [C.c_void_p, BC_VALUE_FN, C.c_void_p]
         L.oracle_level_norm.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _dp]
         L.oracle_set_relax_mode.argtypes = [C.c_void_p, C.c_int]
         L.oracle_set_vc_average.argtypes = [C.c_void_p, C.c_int]
@@ -291,6 +294,19 @@ class Oracle:
     def set_alpha_beta(self, alpha, beta):
         """setAlphaAndBeta on every operator (AMRMultiGrid::getAllOperators)."""
         self._chk(lib().oracle_set_alpha_beta(self.h, float(alpha), float(beta)))
+
+    def set_bc_values(self, fn):
+        """DiriBC / NeumBC with a BCValueHolder: fn(pos, dir, side) -> value at each
+        domain face centre, called at every inhomogeneous ghost fill; None restores the
+        constant face values."""
+        if fn is None:
+            cb = C.cast(None, BC_VALUE_FN)
+        else:
+            def _cb(pos, d, s, v, _user):
+                v[0] = float(fn((pos[0], pos[1], pos[2]), d[0], s[0]))
+            cb = BC_VALUE_FN(_cb)
+        self._chk(lib().oracle_set_bc_values(self.h, cb, None))
+        self._bc_fn = cb
 
     # ---- heat solvers over the hierarchy's AMRMultiGrid (valid-packed fields) ----
     def _acoef(self):
