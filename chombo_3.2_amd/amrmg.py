@@ -397,7 +397,7 @@ class AMRMultiGrid:
 
     def average(self, lev, fine):
         fine, pf = _d(fine)
-        out = np.zeros(self.hier.cells(lev) // 8)
+        out = np.zeros(self.hier.cells(lev) // self.hier.ratio(lev) ** 3)
         _chk(lib().amrmg_average(self.p, lev, pf, out.ctypes.data_as(_dp)))
         return out
 

@@ -19,7 +19,7 @@ from __future__ import annotations
 
 import math
 from dataclasses import dataclass, field
-from typing import List, Sequence, Tuple
+from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -91,7 +91,8 @@ def domain_split(domain: Box, max_box: int, block_factor: int) -> List[Box]:
 
 @dataclass
 class Hierarchy:
-    """An AMR hierarchy: level-0 domain, sorted boxes per level (ratio 2)."""
+    """An AMR hierarchy: level-0 domain, sorted boxes per level, refinement ratio
+    ratios[l-1] (2 or 4) between levels l-1 and l (None: 2 everywhere)."""
 
     domain: Box
     levels: List[List[Box]]
@@ -102,6 +103,7 @@ class Hierarchy:
     bc_type_hi: Tuple[int, int, int] = (1, 1, 1)
     bc_val_lo: Tuple[float, float, float] = (0.0, 0.0, 0.0)
     bc_val_hi: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    ratios: Optional[Tuple[int, ...]] = None
 
     def __post_init__(self):
         self.levels = [sorted(lv, key=sort_key) for lv in self.levels]
@@ -110,16 +112,20 @@ class Hierarchy:
     def nlev(self) -> int:
         return len(self.levels)
 
+    def ratio(self, lev: int) -> int:
+        """Refinement ratio between levels lev-1 and lev (lev >= 1)."""
+        return 2 if self.ratios is None else int(self.ratios[lev - 1])
+
     def dx(self, lev: int) -> float:
         d = self.dx0
-        for _ in range(lev):
-            d = d / 2
+        for l in range(1, lev + 1):
+            d = d / self.ratio(l)
         return d
 
     def level_domain(self, lev: int) -> Box:
         b = self.domain
-        for _ in range(lev):
-            b = box_refine(b, 2)
+        for l in range(1, lev + 1):
+            b = box_refine(b, self.ratio(l))
         return b
 
     def cells(self, lev: int) -> int:

@@ -437,6 +437,25 @@ static void k_prolong(Fab* phi, const Fab* coarse, const Box* r, int m) {
         *P(phi,i,j,k) = *P(phi,i,j,k) + *P(coarse,ii,jj,kk);
       }
 }
+/* average, refRatio 4 branch (lib/src/AMRTools/AverageF.ChF:60-134): the 64 children
+   summed left to right, x fastest, then times refScale */
+static void k_average4(Fab* coarse, const Fab* fine, const Box* b) {
+  double refScale = 1.0/64.0;
+  for (int k = b->lo[2]; k <= b->hi[2]; k++)
+    for (int j = b->lo[1]; j <= b->hi[1]; j++)
+      for (int i = b->lo[0]; i <= b->hi[0]; i++) {
+        double sum = 0.0;
+        int first = 1;
+        for (int kk = 0; kk < 4; kk++)
+          for (int jj = 0; jj < 4; jj++)
+            for (int ii = 0; ii < 4; ii++) {
+              double v = *P(fine, 4*i+ii, 4*j+jj, 4*k+kk);
+              sum = first ? v : sum + v;
+              first = 0;
+            }
+        *P(coarse,i,j,k) = refScale * sum;
+      }
+}
 /* average, refRatio 2 branch (lib/src/AMRTools/AverageF.ChF:9-60) */
 static void k_average2(Fab* coarse, const Fab* fine, const Box* b) {
   double refScale = 1.0/8.0;
@@ -448,6 +467,10 @@ static void k_average2(Fab* coarse, const Fab* fine, const Box* b) {
           (*P(fine,a,c,e) + *P(fine,a+1,c,e) + *P(fine,a,c+1,e) + *P(fine,a+1,c+1,e)
          + *P(fine,a,c,e+1) + *P(fine,a+1,c,e+1) + *P(fine,a,c+1,e+1) + *P(fine,a+1,c+1,e+1));
       }
+}
+static void k_average(Fab* coarse, const Fab* fine, const Box* b, int ratio) {
+  if (ratio == 4) k_average4(coarse, fine, b);
+  else k_average2(coarse, fine, b);
 }
 /* INTERPHOMO (lib/src/AMRTools/InterpF.ChF:435-464) */
 static void k_interphomo(Fab* phi, const Box* r, double x1, double dxCrse, int idir, int ihilo) {
@@ -1758,11 +1781,15 @@ static void op_reflux(Op* cop, Op* fop, LD* phiFine, LD* phi, LD* res) {
     for (int idir = 0; idir < 3; idir++) {
       double scale_flux = cop->beta*cop->refToFiner/cop->dx;
       for (int side = 0; side < 2; side++) {
-        double scale = (side ? 1.0 : -1.0)*1.0/4.0;
-        Box clip = b_refine(r->cfine.b[f], r->nRefine);
+        /* sign(side)*a_scale/denom, denom = nRef^(D-1) (LevelFluxRegister.cpp:468-481) */
+        const int nr = r->nRefine;
+        double denom = 1.0;
+        for (int d = 0; d < 3; d++) if (d != idir) denom *= nr;
+        double scale = (side ? 1.0 : -1.0)*1.0/denom;
+        Box clip = b_refine(r->cfine.b[f], nr);
         Box fineBox = b_adj(clip, idir, side, 1);  /* cells outside the box on this side */
         int iv[3] = {fineBox.lo[0], fineBox.lo[1], fineBox.lo[2]};
-        int civ[3] = {fdiv(iv[0],2), fdiv(iv[1],2), fdiv(iv[2],2)};
+        int civ[3] = {fdiv(iv[0],nr), fdiv(iv[1],nr), fdiv(iv[2],nr)};
         for (int k = fineBox.lo[2]; k <= fineBox.hi[2]; k++)
           for (int j = fineBox.lo[1]; j <= fineBox.hi[1]; j++)
             for (int i = fineBox.lo[0]; i <= fineBox.hi[0]; i++) {
@@ -1771,7 +1798,7 @@ static void op_reflux(Op* cop, Op* fop, LD* phiFine, LD* phi, LD* res) {
               int m1[3] = {n0[0],n0[1],n0[2]}; m1[idir] -= 1;
               double F = (*P(pf,n0[0],n0[1],n0[2]) - *P(pf,m1[0],m1[1],m1[2]))*scale_flux;
               if (fop->vc) F = -*P(&fop->vb[idir][f],n0[0],n0[1],n0[2]) * F;
-              int ii = civ[0] + (i-iv[0])/2, jj = civ[1] + (j-iv[1])/2, kk = civ[2] + (k-iv[2])/2;
+              int ii = civ[0] + (i-iv[0])/nr, jj = civ[1] + (j-iv[1])/nr, kk = civ[2] + (k-iv[2])/nr;
               double* cfin = P(&r->fineFlux.f[f], ii, jj, kk);
               *cfin = *cfin + scale*F;
             }
@@ -1806,6 +1833,8 @@ static void op_reflux(Op* cop, Op* fop, LD* phiFine, LD* phi, LD* res) {
 struct OHier {
   int vcHarmonic;     /* VCAMRPoissonOp2Factory's coefficient average type: 0 arithmetic, 1 harmonic */
   int nlev;
+  int* ratio;         /* ratio[l]: refinement ratio between level l-1 and l (2 or 4), l >= 1 */
+  long* mgBottomCells; /* m_mg[l]'s m_bottomCells (MultiGrid.H:477-508), for AMRMultiGrid::relax */
   Layout* grids;      /* per AMR level */
   Box* domains;
   double* dx;
@@ -1828,11 +1857,19 @@ struct OHier {
   int bottom_calls;
 };
 
-OHier* oracle_create(const int* is, const double* rs) {
+/* ratios: nlev-1 refinement ratios (2 or 4) between consecutive levels (NULL: all 2),
+   AMRPoissonOpFactory::define's refRatios (AMRPoissonOp.cpp:1857-1903) */
+OHier* oracle_create_r(const int* is, const double* rs, const int* ratios) {
   OHier* h = calloc(1, sizeof(OHier));
   int nlev = is[0];
   if (nlev < 1 || nlev > 8) { set_err("bad nlev"); free(h); return NULL; }
   h->nlev = nlev;
+  h->ratio = calloc(nlev, sizeof(int));
+  h->mgBottomCells = calloc(nlev, sizeof(long));
+  for (int l = 1; l < nlev; l++) {
+    h->ratio[l] = ratios ? ratios[l-1] : 2;
+    if (h->ratio[l] != 2 && h->ratio[l] != 4) { set_err("refinement ratio must be 2 or 4"); return NULL; }
+  }
   h->grids = calloc(nlev, sizeof(Layout));
   h->domains = calloc(nlev, sizeof(Box));
   h->dx = calloc(nlev, sizeof(double));
@@ -1849,18 +1886,19 @@ OHier* oracle_create(const int* is, const double* rs) {
     for (int q = 0; q < n; q++) { h->grids[l].b[q] = mkbox(bp[0],bp[1],bp[2],bp[3],bp[4],bp[5]); bp += 6; }
     for (int q = 1; q < n; q++)
       if (!b_lt(&h->grids[l].b[q-1], &h->grids[l].b[q])) { set_err("boxes not sorted"); return NULL; }
-    if (l > 0) { h->domains[l] = b_refine(h->domains[l-1], 2); h->dx[l] = h->dx[l-1]/2; }
+    if (l > 0) { h->domains[l] = b_refine(h->domains[l-1], h->ratio[l]); h->dx[l] = h->dx[l-1]/h->ratio[l]; }
   }
   /* periodic directions (BC type 3 on both faces): the levels above the base stay at
-     least 2 coarse cells (4 fine cells) away from the periodic faces, so their
-     coarse-fine stencils and flux registers never reach across them */
+     least 2 coarse cells away from the periodic faces, so their coarse-fine stencils
+     and flux registers never reach across them */
   for (int d = 0; d < 3; d++) {
     if ((h->bc.type[0][d] == 3) != (h->bc.type[1][d] == 3)) { set_err("periodic on one face only"); return NULL; }
     if (h->bc.type[0][d] != 3) continue;
     for (int l = 1; l < nlev; l++)
       for (int q = 0; q < h->grids[l].n; q++) {
         const Box* b = &h->grids[l].b[q];
-        if (b->lo[d] < h->domains[l].lo[d] + 4 || b->hi[d] > h->domains[l].hi[d] - 4) {
+        const int m = 2*h->ratio[l];
+        if (b->lo[d] < h->domains[l].lo[d] + m || b->hi[d] > h->domains[l].hi[d] - m) {
           set_err("a refined level comes within 2 coarse cells of a periodic face");
           return NULL;
         }
@@ -1873,12 +1911,19 @@ OHier* oracle_create(const int* is, const double* rs) {
     op_define(op, &h->grids[l], &h->domains[l], h->dx[l], h->alpha, h->beta, &h->bc);
     op->dxCrse = (l > 0) ? h->dx[l-1] : -1.0;
     if (l > 0) {
-      op->hasCoarser = 1; op->refToCoarser = 2;
-      qcfi_define(&op->qcfi, &h->grids[l], &h->grids[l-1], h->dx[l], 2, &h->domains[l]);
+      op->hasCoarser = 1; op->refToCoarser = h->ratio[l];
+      qcfi_define(&op->qcfi, &h->grids[l], &h->grids[l-1], h->dx[l], h->ratio[l], &h->domains[l]);
     }
     if (l < nlev-1) {
-      op->hasFiner = 1; op->refToFiner = 2;
-      fr_define(&op->fr, &h->grids[l+1], &h->grids[l], 2);
+      op->hasFiner = 1; op->refToFiner = h->ratio[l+1];
+      fr_define(&op->fr, &h->grids[l+1], &h->grids[l], h->ratio[l+1]);
+    }
+    /* m_mg[l]'s bottom cell count: the level's domain over the multigrid depths its
+       layout allows (MGnewOp: coarsenable by coarsening*2) */
+    {
+      long cells = b_npts(&h->domains[l]);
+      for (int c = 2; lay_coarsenable(&h->grids[l], c*2); c *= 2) cells /= 8;
+      h->mgBottomCells[l] = cells;
     }
   }
   /* MultiGrid::define on level 0 via MGnewOp (AMRPoissonOp.cpp:1923-1989) */
@@ -1905,7 +1950,7 @@ OHier* oracle_create(const int* is, const double* rs) {
   h->resCopier = calloc(nlev, sizeof(Plan));
   h->revCopier = calloc(nlev, sizeof(Plan));
   for (int l = 1; l < nlev; l++) {
-    h->cfL[l] = lay_coarsen(&h->grids[l], 2);
+    h->cfL[l] = lay_coarsen(&h->grids[l], h->ratio[l]);
     h->resCopier[l] = plan_copy(&h->cfL[l], &h->grids[l-1], 0);
     h->revCopier[l] = h->resCopier[l];
     h->revCopier[l].m = malloc(sizeof(Motion)*(h->resCopier[l].n ? h->resCopier[l].n : 1));
@@ -1923,8 +1968,9 @@ void oracle_destroy(OHier* h) {
   for (int l = 1; l < h->nlev; l++) { plan_free(&h->resCopier[l]); plan_free(&h->revCopier[l]); free(h->cfL[l].b); }
   for (int l = 0; l < h->nlev; l++) free(h->grids[l].b);
   free(h->grids); free(h->domains); free(h->dx); free(h->op); free(h->mgL); free(h->mgop);
-  free(h->cfL); free(h->resCopier); free(h->revCopier); free(h);
+  free(h->cfL); free(h->resCopier); free(h->revCopier); free(h->ratio); free(h->mgBottomCells); free(h);
 }
+OHier* oracle_create(const int* is, const double* rs) { return oracle_create_r(is, rs, NULL); }
 /* VCAMRPoissonOp2Factory::define + MGnewOp (VCAMRPoissonOp2.cpp:868-1070) on a
    one-level hierarchy: the given coefficients on level 0, and on MG depth d the
    coefficients averaged from level 0 with ratio 2^d (CoarseAverage, CoarseAverageFace,
@@ -2216,13 +2262,22 @@ static void amr_restrictS(AMRState* a, int l, LD* resC, const LD* residual, LD* 
   OHier* h = a->h; Op* op = &h->op[l];
   qcfi_interp(&op->qcfi, correction, coarseCorrection);      /* AMRResidualNF */
   op_residualI(op, scratch, correction, residual, 1);
-  for (int b = 0; b < op->L.n; b++) k_average2(&resC->f[b], &scratch->f[b], &h->cfL[l].b[b]);
+  for (int b = 0; b < op->L.n; b++) k_average(&resC->f[b], &scratch->f[b], &h->cfL[l].b[b], h->ratio[l]);
 }
 /* AMRProlongS (AMRPoissonOp.cpp:959-990) */
 static void amr_prolongS(AMRState* a, int l, LD* correction, const LD* coarseCorrection, LD* temp) {
   OHier* h = a->h; Op* op = &h->op[l];
   exec_copy(temp, coarseCorrection, &h->revCopier[l]);
-  for (int b = 0; b < op->L.n; b++) k_prolong(&correction->f[b], &temp->f[b], &op->L.b[b], 2);
+  for (int b = 0; b < op->L.n; b++) k_prolong(&correction->f[b], &temp->f[b], &op->L.b[b], h->ratio[l]);
+}
+/* AMRMultiGrid::relax (AMRMultiGrid.H:633-666): with ratio 4 to the coarser level,
+   m_mg[l] runs MultiGrid::cycle with m_depth 1 (set by init, :1224-1236), which is
+   its bottom branch (MultiGrid.H:634-646): one relaxation when m_bottomCells is 1,
+   else m_bottom relaxations (the bottom solver is the no-op one) */
+static void amr_relax(AMRState* a, int l, LD* corr, const LD* res, int n) {
+  OHier* h = a->h;
+  if (h->ratio[l] > 2) n = (h->mgBottomCells[l] == 1) ? 1 : a->mg.bottom;
+  op_relax(&h->op[l], corr, res, n);
 }
 /* AMRVCycle (AMRMultiGrid.H:1342-1419) */
 static void amr_vcycle(AMRState* a, LD* uberCorr, LD* uberRes, int ilev, int lmax, int lbase) {
@@ -2239,7 +2294,7 @@ static void amr_vcycle(AMRState* a, LD* uberCorr, LD* uberRes, int ilev, int lma
     return;
   }
   Op* op = &h->op[ilev];
-  op_relax(op, &a->corr[ilev], &a->resid[ilev], a->mg.pre);
+  amr_relax(a, ilev, &a->corr[ilev], &a->resid[ilev], a->mg.pre);
   ldo_incr(&uberCorr[ilev], &a->corr[ilev], 1.0);
   ldo_setzero(&a->corr[ilev-1]);
   amr_residual_level(a, a->resid, uberCorr, uberRes, lmax, lbase, ilev-1, 1);
@@ -2252,7 +2307,7 @@ static void amr_vcycle(AMRState* a, LD* uberCorr, LD* uberRes, int ilev, int lma
   op_residualI(op, &a->resid[ilev], &a->corr[ilev], &a->resid[ilev], 1);
   LD* dCorr = &uberCorr[ilev];
   ldo_setzero(dCorr);
-  op_relax(op, dCorr, &a->resid[ilev], a->mg.post);
+  amr_relax(a, ilev, dCorr, &a->resid[ilev], a->mg.post);
   ldo_incr(&a->corr[ilev], dCorr, 1.0);
   ldo_assign_local(&uberCorr[ilev], &a->corr[ilev]);
 }
@@ -2450,8 +2505,20 @@ int oracle_average(OHier* h, int lev, const double* fine, double* out) {
   if (lev < 1 || lev >= h->nlev) { set_err("bad level"); return -1; }
   LD f = ld_make(&h->grids[lev], 0), c = ld_make(&h->cfL[lev], 0);
   ld_from_packed(&f, fine);
-  for (int b = 0; b < h->grids[lev].n; b++) k_average2(&c.f[b], &f.f[b], &h->cfL[lev].b[b]);
+  for (int b = 0; b < h->grids[lev].n; b++) k_average(&c.f[b], &f.f[b], &h->cfL[lev].b[b], h->ratio[lev]);
   ld_to_packed(&c, out);
+  ld_free(&f); ld_free(&c);
+  return 0;
+}
+/* AMRProlongS's PROLONG (AMRPoissonOp.cpp:959-990) on AMR level lev: fine (valid-packed,
+   in/out) += the coarsened-fine-layout field coarse (valid-packed over coarsen(grids, ratio)) */
+int oracle_amr_prolong(OHier* h, int lev, double* fine_io, const double* coarse) {
+  if (lev < 1 || lev >= h->nlev) { set_err("bad level"); return -1; }
+  LD f = ld_make(&h->grids[lev], 0), c = ld_make(&h->cfL[lev], 0);
+  ld_from_packed(&f, fine_io);
+  ld_from_packed(&c, coarse);
+  for (int b = 0; b < h->grids[lev].n; b++) k_prolong(&f.f[b], &c.f[b], &h->grids[lev].b[b], h->ratio[lev]);
+  ld_to_packed(&f, fine_io);
   ld_free(&f); ld_free(&c);
   return 0;
 }

@@ -31,6 +31,11 @@ typedef struct OHier OHier;
 /* solver parameters: sparams = {pre, post, bottom, numMG, iterMax, imin},
    rparams = {eps, hang, normThresh} */
 OHier* oracle_create(const int* ispec, const double* rspec);
+/* the same with refinement ratios (2 or 4) between consecutive levels; NULL: all 2 */
+OHier* oracle_create_r(const int* ispec, const double* rspec, const int* ratios);
+/* BCValueFunc with a caller pointer; fn NULL restores the constant face values */
+typedef void (*oracle_bc_value_fn)(double* pos, int* dir, int* side, double* value, void* user);
+int    oracle_set_bc_values(OHier* h, oracle_bc_value_fn fn, void* user);
 void   oracle_destroy(OHier* h);
 long   oracle_level_cells(const OHier* h, int lev);
 int    oracle_num_mg_depths(const OHier* h);

@@ -56,6 +56,9 @@ def lib():
         L = C.CDLL(_LIB_PATH)
         L.oracle_create.restype = C.c_void_p
         L.oracle_create.argtypes = [_ip, _dp]
+        L.oracle_create_r.restype = C.c_void_p
+        L.oracle_create_r.argtypes = [_ip, _dp, _ip]
+        L.oracle_amr_prolong.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
         L.oracle_destroy.argtypes = [C.c_void_p]
         L.oracle_last_error.restype = C.c_char_p
         L.oracle_level_cells.restype = C.c_long
@@ -112,7 +115,13 @@ class Oracle:
         isp, rsp = hier.spec()
         self._isp, self._rsp = isp, rsp
         L = lib()
-        self.h = L.oracle_create(isp.ctypes.data_as(_ip), rsp.ctypes.data_as(_dp))
+        ratios = getattr(hier, "ratios", None)
+        if ratios is None:
+            self.h = L.oracle_create(isp.ctypes.data_as(_ip), rsp.ctypes.data_as(_dp))
+        else:
+            self._ratios = np.ascontiguousarray(ratios, dtype=np.int32)
+            self.h = L.oracle_create_r(isp.ctypes.data_as(_ip), rsp.ctypes.data_as(_dp),
+                                       self._ratios.ctypes.data_as(_ip))
         if not self.h:
             raise RuntimeError(L.oracle_last_error().decode())
 
@@ -215,8 +224,16 @@ class Oracle:
 
     def average(self, lev, fine):
         fine, pf = _d(fine)
-        out = np.zeros(self.hier.cells(lev) // 8)
+        out = np.zeros(self.hier.cells(lev) // self.hier.ratio(lev) ** 3)
         self._chk(lib().oracle_average(self.h, lev, pf, out.ctypes.data_as(_dp)))
+        return out
+
+    def amr_prolong(self, lev, fine, coarse):
+        """AMRProlongS's PROLONG on AMR level lev: fine + coarse (coarse valid-packed
+        over the level's boxes coarsened by its ratio)."""
+        out = np.array(fine, dtype=np.float64)
+        coarse, pc = _d(coarse)
+        self._chk(lib().oracle_amr_prolong(self.h, lev, out.ctypes.data_as(_dp), pc))
         return out
 
     def bottom_solve(self, corr, res, nbottom, metric, eps, iterations=False):

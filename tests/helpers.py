@@ -75,7 +75,7 @@ def uncovered_mask(h, lev):
     cov = np.zeros(H.box_shape(dom), dtype=bool)
     sl = lambda b: tuple(slice(b[0][d] - dom[0][d], b[1][d] - dom[0][d] + 1) for d in range(3))
     for b in h.levels[lev + 1]:
-        cov[sl(H.box_coarsen(b, 2))] = True
+        cov[sl(H.box_coarsen(b, h.ratio(lev + 1)))] = True
     return np.concatenate([(~cov[sl(b)]).ravel(order="F") for b in h.levels[lev]])
 
 
