@@ -88,6 +88,7 @@ def lib():
         L.amrmg_batch_set_graph.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_longlong)]
         L.amrmg_batch_set_reduction_order.argtypes = [C.c_void_p, C.c_int]
         L.amrmg_solver_create.argtypes = [C.c_void_p, _ip, _dp, C.c_int, C.POINTER(C.c_void_p)]
+        L.amrmg_solver_create_ratios.argtypes = [C.c_void_p, _ip, _dp, _ip, C.c_int, C.POINTER(C.c_void_p)]
         L.amrmg_solver_destroy.argtypes = [C.c_void_p]
         L.amrmg_solver_set_params.argtypes = [C.c_void_p, _ip, _dp]
         L.amrmg_solver_set_bc_values.argtypes = [C.c_void_p, BC_VALUE_FN, C.c_void_p]
@@ -118,6 +119,7 @@ def lib():
         L.amrmg_bottom_solve_device.argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.c_double, C.c_double, _ip]
         L.amrmg_solver_set_bottom_params.argtypes = [C.c_void_p, _ip, _dp]
         L.amrmg_plan_check.argtypes = [_ip, _dp, C.POINTER(C.c_longlong), C.c_int]
+        L.amrmg_plan_check_ratios.argtypes = [_ip, _dp, _ip, C.POINTER(C.c_longlong), C.c_int]
         L.amrmg_copy_probe.argtypes = [C.c_void_p, C.c_longlong, C.c_int, _dp]
         L.amrmg_profile_enable.argtypes = [C.c_void_p, C.c_int]
         L.amrmg_profile_reset.argtypes = [C.c_void_p]
@@ -147,7 +149,13 @@ def plan_check(hier):
     """Build and bounds-check every device table of `hier` on the host (no device)."""
     isp, rsp = hier.spec()
     st = (C.c_longlong * 8)()
-    _chk(lib().amrmg_plan_check(isp.ctypes.data_as(_ip), rsp.ctypes.data_as(_dp), st, 8))
+    ratios = getattr(hier, "ratios", None)
+    if ratios is None:
+        _chk(lib().amrmg_plan_check(isp.ctypes.data_as(_ip), rsp.ctypes.data_as(_dp), st, 8))
+    else:
+        r = np.ascontiguousarray(ratios, dtype=np.int32)
+        _chk(lib().amrmg_plan_check_ratios(isp.ctypes.data_as(_ip), rsp.ctypes.data_as(_dp),
+                                           r.ctypes.data_as(_ip), st, 8))
     keys = ["levels", "mg_depths", "ghost_patches", "cf_coarse", "cf_fine", "reflux_cells", "tiles", "cells"]
     return dict(zip(keys, list(st)))
 
@@ -209,8 +217,14 @@ class AMRMultiGrid:
         self.hier = hier
         isp, rsp = hier.spec()
         p = C.c_void_p()
-        _chk(lib().amrmg_solver_create(ctx.p, isp.ctypes.data_as(_ip), rsp.ctypes.data_as(_dp), precision,
-                                        C.byref(p)))
+        ratios = getattr(hier, "ratios", None)
+        if ratios is None:
+            _chk(lib().amrmg_solver_create(ctx.p, isp.ctypes.data_as(_ip), rsp.ctypes.data_as(_dp), precision,
+                                            C.byref(p)))
+        else:
+            r = np.ascontiguousarray(ratios, dtype=np.int32)
+            _chk(lib().amrmg_solver_create_ratios(ctx.p, isp.ctypes.data_as(_ip), rsp.ctypes.data_as(_dp),
+                                                   r.ctypes.data_as(_ip), precision, C.byref(p)))
         self.p = p
         self.ncells = sum(lib().amrmg_solver_level_cells(p, l) for l in range(hier.nlev))
 

@@ -425,7 +425,7 @@ void AMRPoissonOp<Real>::coarseFineInterp(T& phif, const T& phic) {
   if (!m_coarserLay) return;
   need(phif.lay, m_lay, "coarseFineInterp"); need(phic.lay, m_coarserLay, "coarseFineInterp");
   ops::cf_inhomog<Real>(m_ctx, d_cfc, n_cfc, d_cfg, d_cff, n_cff, d_deriv, phif.d, phic.d, m_dx,
-                        m_refToCoarser * m_dx);
+                        m_refToCoarser * m_dx, m_refToCoarser);
 }
 // AMRResidual (:736-750)
 template <class Real>
@@ -498,11 +498,12 @@ void AMRPoissonOp<Real>::AMRRestrictS(T& resCoarse, const T& residual, T& correc
     need(correction.lay, m_lay, "AMRRestrictS"); need(residual.lay, m_lay, "AMRRestrictS");
     if (coarseCorrection.isDefined()) coarseFineInterp(correction, coarseCorrection);
     fillGhosts(correction, false, true);
-    ops::residual_average<Real>(*m_lay, *m_cfLay, resCoarse.d, correction.d, residual.d, m_dx, m_alpha, m_beta);
+    ops::residual_average<Real>(*m_lay, *m_cfLay, resCoarse.d, correction.d, residual.d, m_dx, m_alpha, m_beta,
+                                m_refToCoarser);
     return;
   }
   AMRResidualNF(scratch, correction, coarseCorrection, residual, true);
-  ops::average<Real>(*m_lay, *m_cfLay, resCoarse.d, scratch.d);
+  ops::average<Real>(*m_lay, *m_cfLay, resCoarse.d, scratch.d, m_refToCoarser);
 }
 // AMRProlongS (:959-990)
 template <class Real>
@@ -510,7 +511,7 @@ void AMRPoissonOp<Real>::AMRProlongS(T& correction, const T& coarseCorrection, T
   need(temp.lay, m_cfLay, "AMRProlongS"); need(coarseCorrection.lay, m_coarserLay, "AMRProlongS");
   need(correction.lay, m_lay, "AMRProlongS");
   ops::motion<Real>(m_revCopier, *m_coarserLay, *m_cfLay, temp.d, coarseCorrection.d, 0);
-  ops::prolong<Real>(*m_lay, *m_cfLay, correction.d, temp.d);
+  ops::prolong<Real>(*m_lay, *m_cfLay, correction.d, temp.d, m_refToCoarser);
 }
 // AMRUpdateResidual (:1045-1054)
 template <class Real>
@@ -555,19 +556,23 @@ void AMRPoissonOp<Real>::reflux(T& phiFine, T& phi, T& res, AMRPoissonOp* finerO
 // DeviceHierarchy / factory
 // ---------------------------------------------------------------------------
 void DeviceHierarchy::define(Context* c, const std::vector<DBL>& g, const Box& domain0, double dx0, bool merge,
-                             const BCSpec* bc) {
+                             const BCSpec* bc, const int* ratio) {
   ctx = c;
   grids = g;
   domains.assign(g.size(), domain0);
   dxs.assign(g.size(), dx0);
+  ratios.assign(g.size(), 1);
   for (size_t l = 1; l < g.size(); l++) {
-    domains[l] = refine(domains[l - 1], 2);
-    dxs[l] = dxs[l - 1] / 2;
+    const int r = ratio ? ratio[l - 1] : 2;
+    if (r != 2 && r != 4) throw std::runtime_error("refinement ratio must be 2 or 4");
+    ratios[l] = r;
+    domains[l] = refine(domains[l - 1], r);
+    dxs[l] = dxs[l - 1] / r;
   }
   // periodic directions: a layout with a box on a periodic face is stored per box
   // (its ghosts there come from periodic images); the levels above the base stay 2
-  // coarse cells (4 fine cells) away from periodic faces, so coarse-fine stencils
-  // and flux registers never reach across them
+  // coarse cells away from periodic faces, so coarse-fine stencils and flux
+  // registers never reach across them
   bool per[3] = {false, false, false};
   if (bc)
     for (int d = 0; d < 3; d++) {
@@ -579,7 +584,7 @@ void DeviceHierarchy::define(Context* c, const std::vector<DBL>& g, const Box& d
     for (int d = 0; d < 3; d++)
       if (per[d])
         for (const Box& b : g[l].boxes)
-          if (b.lo[d] < domains[l].lo[d] + 4 || b.hi[d] > domains[l].hi[d] - 4)
+          if (b.lo[d] < domains[l].lo[d] + 2 * ratios[l] || b.hi[d] > domains[l].hi[d] - 2 * ratios[l])
             throw std::runtime_error("a refined level comes within 2 coarse cells of a periodic face");
   auto touches = [&](const DBL& d, const Box& dom) {
     for (const Box& b : d.boxes)
@@ -595,9 +600,18 @@ void DeviceHierarchy::define(Context* c, const std::vector<DBL>& g, const Box& d
     amrLay.emplace_back(new DeviceLayout(c, g[l], domains[l], merge && !touches(g[l], domains[l])));
     if (l == 0) cfLay.emplace_back(nullptr);
     else {
-      if (!g[l].coarsenable(2)) throw std::runtime_error("fine level boxes must be coarsenable by 2");
-      cfLay.emplace_back(new DeviceLayout(c, g[l].coarsened(2), domains[l - 1], merge));
+      if (!g[l].coarsenable(ratios[l]))
+        throw std::runtime_error("fine level boxes must be coarsenable by the refinement ratio");
+      cfLay.emplace_back(new DeviceLayout(c, g[l].coarsened(ratios[l]), domains[l - 1], merge));
     }
+  }
+  // MultiGrid::define's m_bottomCells for each level's m_mg (MultiGrid.H:477-508): the
+  // level's domain divided by 8 for every multigrid level MGnewOp creates
+  mgBottomCells.assign(g.size(), 0);
+  for (size_t l = 0; l < g.size(); l++) {
+    long long cells = domains[l].npts();
+    for (int cz = 2; cz < (1 << 24) && g[l].coarsenable(cz * 2); cz *= 2) cells /= 8;
+    mgBottomCells[l] = cells;
   }
   // MultiGrid::define via MGnewOp: stop when not coarsenable by coarsening*2
   // (AMRPoissonOp.cpp:1955-1959, s_maxCoarse = 2)
@@ -649,9 +663,9 @@ static void validate_motion(const ops::MotionPlan& P, const DBL& src, const DBL&
     require(in_grown(dst, m.to, m.lo, m.hi, 1), "copier destination region outside its box");
   }
 }
-static void validate_pair(const DeviceLayout& f, const DeviceLayout& c) {
+static void validate_pair(const DeviceLayout& f, const DeviceLayout& c, int ratio = 2) {
   require(f.dbl.size() == c.dbl.size(), "coarsened layout must match the fine layout box by box");
-  for (int b = 0; b < f.dbl.size(); b++) require(coarsen(f.dbl[b], 2) == c.dbl[b], "coarsened box mismatch");
+  for (int b = 0; b < f.dbl.size(); b++) require(coarsen(f.dbl[b], ratio) == c.dbl[b], "coarsened box mismatch");
 }
 
 // The one-launch relaxation applies when the level is stored merged and every
@@ -722,18 +736,20 @@ void setLevelOpBCValues(AMRPoissonOp<Real>* op, BCValueFn fn, void* user) {
 }
 
 // The coarse-fine part of AMRnewOp for a level with a coarser AMR level (refinement
-// ratio 2): QuadCFInterp tables, the copiers between coarsen(fine, 2) (cfLay) and the
-// coarser level, and the coarser operator's reflux tables and zeroCovered plan.
+// ratio 2 or 4): QuadCFInterp tables, the copiers between coarsen(fine, ratio) (cfLay)
+// and the coarser level, and the coarser operator's reflux tables and zeroCovered plan.
 template <class Real>
-void linkCoarser(AMRPoissonOp<Real>* op, AMRPoissonOp<Real>* coarserOp, const DeviceLayout* cfLay) {
+void linkCoarser(AMRPoissonOp<Real>* op, AMRPoissonOp<Real>* coarserOp, const DeviceLayout* cfLay, int ratio) {
   Context* ctx = op->m_ctx;
   const DeviceLayout& F = *op->m_lay;
   const DeviceLayout& Cl = *coarserOp->m_lay;
-  if (!(refine(coarserOp->m_domain, 2) == op->m_domain)) throw std::runtime_error("coarser level: domain must be coarsen(domain, 2)");
-  op->m_refToCoarser = 2;
+  if (ratio != 2 && ratio != 4) throw std::runtime_error("refinement ratio must be 2 or 4");
+  if (!(refine(coarserOp->m_domain, ratio) == op->m_domain))
+    throw std::runtime_error("coarser level: domain must be coarsen(domain, ratio)");
+  op->m_refToCoarser = ratio;
   op->m_coarserLay = &Cl;
   op->m_cfLay = cfLay;
-  CFPlanHost cf = build_cf_plan(F.dbl, F, Cl.dbl, Cl, op->m_domain, op->m_dx, 2);
+  CFPlanHost cf = build_cf_plan(F.dbl, F, Cl.dbl, Cl, op->m_domain, op->m_dx, ratio);
   {
     const long long fs = F.slab, cs = Cl.slab;
     for (const CFCoarse& c : cf.coarse) {
@@ -759,19 +775,20 @@ void linkCoarser(AMRPoissonOp<Real>* op, AMRPoissonOp<Real>* coarserOp, const De
   op->d_cfg = upload_vec(ctx, cf.gens);
   op->d_cff = upload_vec(ctx, cf.fine);
   dev_alloc(ctx, (void**)&op->d_deriv, sizeof(Real) * 6 * (size_t)op->n_cfc, "hipMalloc deriv");
-  const DBL fc = F.dbl.coarsened(2);
+  const DBL fc = F.dbl.coarsened(ratio);
   op->m_resCopier = build_copier(fc, Cl.dbl);
   validate_motion(op->m_resCopier, fc, Cl.dbl);
   op->m_resCopier.upload(ctx);
   op->m_revCopier = build_copier(Cl.dbl, fc);
   validate_motion(op->m_revCopier, Cl.dbl, fc);
   op->m_revCopier.upload(ctx);
-  validate_pair(*op->m_lay, *op->m_cfLay);
+  validate_pair(*op->m_lay, *op->m_cfLay, ratio);
   // the coarser operator refluxes against this level and zeroes what it covers
+  coarserOp->m_refToFiner = ratio;
   coarserOp->m_finerLay = op->m_lay;
   coarserOp->m_zeroPlan = &op->m_resCopier;
   coarserOp->m_finerCfLay = op->m_cfLay;
-  RefluxPlanHost rp = build_reflux_plan(Cl.dbl, Cl, F.dbl, F, coarserOp->m_domain, 2);
+  RefluxPlanHost rp = build_reflux_plan(Cl.dbl, Cl, F.dbl, F, coarserOp->m_domain, ratio);
   {
     const long long fs = F.slab, cs = Cl.slab;
     for (const RefluxCell& c : rp.cells) {
@@ -797,10 +814,13 @@ void linkCoarser(AMRPoissonOp<Real>* op, AMRPoissonOp<Real>* coarserOp, const De
 template <class Real>
 std::unique_ptr<AMRPoissonOp<Real>> AMRPoissonOpFactory<Real>::AMRnewOp(int l, AMRPoissonOp<Real>* coarserOp) {
   std::unique_ptr<AMRPoissonOp<Real>> op(new AMRPoissonOp<Real>());
-  const DeviceLayout* mgc = (l == 0 && !H->mgLay.empty()) ? H->mgLay[0].get() : (l > 0 ? H->cfLay[l].get() : nullptr);
+  // the multigrid child: level 0's first multigrid depth; for a level above the base
+  // with ratio 2, coarsen(level, 2) is its coarsened-fine layout (ratio 4 has none)
+  const DeviceLayout* mgc = (l == 0 && !H->mgLay.empty()) ? H->mgLay[0].get()
+                            : (l > 0 && H->ratios[l] == 2 ? H->cfLay[l].get() : nullptr);
   defineLevelOp(op.get(), H->ctx, H->amrLay[l].get(), H->domains[l], H->dxs[l], (l > 0) ? H->dxs[l - 1] : -1.0,
                 alpha, beta, bc, mgc);
-  if (l > 0) linkCoarser(op.get(), coarserOp, H->cfLay[l].get());
+  if (l > 0) linkCoarser(op.get(), coarserOp, H->cfLay[l].get(), H->ratios[l]);
   return op;
 }
 
@@ -825,6 +845,7 @@ void AMRMultiGrid<Real>::define(AMRPoissonOpFactory<Real>& factory, LinearSolver
   m_op.clear();
   for (int l = 0; l < numLevels; l++) m_op.push_back(factory.AMRnewOp(l, l > 0 ? m_op[l - 1].get() : nullptr));
   m_bottomSolver = bottom;
+  m_mgBottomCells = factory.H->mgBottomCells;
   m_mg0.reset(new MultiGrid<T>());
   m_mg0->define(m_op[0].get(), factory.MGnewOps(), factory.H->domains[0].npts(), bottom);
   m_mg0->m_pre = m_pre;
@@ -862,6 +883,11 @@ void AMRMultiGrid<Real>::init(std::vector<T*>& phi, std::vector<T*>& rhs, int l_
 }
 template <class Real>
 void AMRMultiGrid<Real>::relax(T& phi, T& R, int depth, int nRelax, bool fromZero) {
+  // ratio 4 to the coarser level: the level's MultiGrid cycles with m_depth 1, its
+  // bottom branch (AMRMultiGrid.H:638-657 with init's intermediateDepth,
+  // MultiGrid.H:634-646): one relaxation when its m_bottomCells is 1, else m_bottom
+  // (the bottom solver there is the no-op one)
+  if (m_op[depth]->m_refToCoarser > 2) nRelax = (m_mgBottomCells[depth] == 1) ? 1 : m_bottom;
   if (fromZero) m_op[depth]->relaxFromZero(phi, R, nRelax);
   else m_op[depth]->relax(phi, R, nRelax);
 }
@@ -1045,8 +1071,8 @@ template void defineLevelOp<double>(AMRPoissonOp<double>*, Context*, const Devic
                                     double, double, const BCSpec&, const DeviceLayout*);
 template void defineLevelOp<float>(AMRPoissonOp<float>*, Context*, const DeviceLayout*, const Box&, double, double,
                                    double, double, const BCSpec&, const DeviceLayout*);
-template void linkCoarser<double>(AMRPoissonOp<double>*, AMRPoissonOp<double>*, const DeviceLayout*);
-template void linkCoarser<float>(AMRPoissonOp<float>*, AMRPoissonOp<float>*, const DeviceLayout*);
+template void linkCoarser<double>(AMRPoissonOp<double>*, AMRPoissonOp<double>*, const DeviceLayout*, int);
+template void linkCoarser<float>(AMRPoissonOp<float>*, AMRPoissonOp<float>*, const DeviceLayout*, int);
 template class AMRPoissonOp<double>;
 template class AMRPoissonOp<float>;
 template class AMRPoissonOpFactory<double>;

@@ -148,7 +148,7 @@ void setLevelOpBCValues(AMRPoissonOp<Real>* op, BCValueFn fn, void* user);
 // (ratio 2): QuadCFInterp, the restriction / prolongation copiers through cfLay =
 // coarsen(op's level, 2), and coarserOp's reflux tables and zeroCovered plan.
 template <class Real>
-void linkCoarser(AMRPoissonOp<Real>* op, AMRPoissonOp<Real>* coarserOp, const DeviceLayout* cfLay);
+void linkCoarser(AMRPoissonOp<Real>* op, AMRPoissonOp<Real>* coarserOp, const DeviceLayout* cfLay, int ratio = 2);
 
 // Device layouts of one AMR hierarchy: AMR levels, the multigrid levels of level
 // 0, and the coarsened-fine layouts used for restriction/prolongation between
@@ -158,12 +158,15 @@ struct DeviceHierarchy {
   std::vector<DBL> grids;
   std::vector<Box> domains;
   std::vector<double> dxs;
+  std::vector<int> ratios;                            // ratios[l]: level l-1 to l (2 or 4); ratios[0] = 1
+  std::vector<long long> mgBottomCells;               // per level: MultiGrid::m_bottomCells of its m_mg
   std::vector<std::unique_ptr<DeviceLayout>> amrLay;  // per AMR level
-  std::vector<std::unique_ptr<DeviceLayout>> cfLay;   // coarsen(grids[l], 2); index l >= 1
+  std::vector<std::unique_ptr<DeviceLayout>> cfLay;   // coarsen(grids[l], ratios[l]); index l >= 1
   std::vector<std::unique_ptr<DeviceLayout>> mgLay;   // MG levels of AMR level 0, depth >= 1
-  // merge: store each level whose boxes tile a rectangle as one array
+  // merge: store each level whose boxes tile a rectangle as one array; ratio: the
+  // nlev-1 refinement ratios between consecutive levels (null: 2 everywhere)
   void define(Context* c, const std::vector<DBL>& g, const Box& domain0, double dx0, bool merge = true,
-              const BCSpec* bc = nullptr);
+              const BCSpec* bc = nullptr, const int* ratio = nullptr);
   int nlev() const { return (int)grids.size(); }
   int mgDepth() const { return 1 + (int)mgLay.size(); }
 };
@@ -199,6 +202,7 @@ class AMRMultiGrid {
   std::vector<std::unique_ptr<AMRPoissonOp<Real>>> m_op;
   std::unique_ptr<MultiGrid<T>> m_mg0;                   // multigrid of level 0
   std::vector<std::unique_ptr<T>> m_correction, m_residual, m_resC;
+  std::vector<long long> m_mgBottomCells;  // per level: its MultiGrid's m_bottomCells (ratio-4 relax)
   LinearSolver<T>* m_bottomSolver = nullptr;
   std::vector<double> m_history;
   int m_iterations = 0;

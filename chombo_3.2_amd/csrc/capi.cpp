@@ -74,7 +74,7 @@ struct amrmg_solver {
   AMRPoissonOp<double>* fabOp(int lev) {
     if (!fabH) {
       fabH.reset(new DeviceHierarchy());
-      fabH->define(ctx->c.get(), H.grids, H.domains[0], H.dxs[0], false);
+      fabH->define(ctx->c.get(), H.grids, H.domains[0], H.dxs[0], false, nullptr, H.ratios.data() + 1);
       fabFac.reset(new AMRPoissonOpFactory<double>());
       fabFac->define(fabH.get(), alpha, beta, bc);
     }
@@ -205,6 +205,10 @@ int amrmg_ctx_destroy(amrmg_ctx* ctx) {
 }
 
 int amrmg_solver_create(amrmg_ctx* ctx, const int* ispec, const double* rspec, int precision, amrmg_solver** out) {
+  return amrmg_solver_create_ratios(ctx, ispec, rspec, nullptr, precision, out);
+}
+int amrmg_solver_create_ratios(amrmg_ctx* ctx, const int* ispec, const double* rspec, const int* ref_ratios,
+                               int precision, amrmg_solver** out) {
   API_BEGIN
   if (precision != 0 && precision != 1) throw std::runtime_error("precision must be 0 (FP64) or 1 (mixed)");
   std::unique_ptr<amrmg_solver> s(new amrmg_solver());
@@ -213,7 +217,7 @@ int amrmg_solver_create(amrmg_ctx* ctx, const int* ispec, const double* rspec, i
   Box dom;
   double dx0;
   std::vector<DBL> g = parse_spec(ispec, rspec, dom, s->bc, dx0, s->alpha, s->beta);
-  s->H.define(ctx->c.get(), g, dom, dx0, true, &s->bc);
+  s->H.define(ctx->c.get(), g, dom, dx0, true, &s->bc, ref_ratios);
   s->fac.define(&s->H, s->alpha, s->beta, s->bc);
   s->bottom.reset(new DeviceBottomSolver<double>(ctx->c.get()));
   s->amg.define(s->fac, s->bottom.get(), s->nlev());
@@ -539,7 +543,7 @@ int amrmg_average(amrmg_solver* s, int lev, const double* fine, double* coarse) 
   const DeviceLayout* Lc = s->H.cfLay[lev].get();
   LevelData<double> f(L), c(Lc);
   upload_valid(*L, f.d, fine);
-  ops::average<double>(*L, *Lc, c.d, f.d);
+  ops::average<double>(*L, *Lc, c.d, f.d, s->H.ratios[lev]);
   download_valid(*Lc, c.d, coarse);
   API_END
 }
@@ -1326,6 +1330,10 @@ int amrmg_copy_probe(amrmg_ctx* ctx, long long nbytes, int iters, double* ms_out
 }
 
 int amrmg_plan_check(const int* ispec, const double* rspec, long long* stats, int nstats) {
+  return amrmg_plan_check_ratios(ispec, rspec, nullptr, stats, nstats);
+}
+int amrmg_plan_check_ratios(const int* ispec, const double* rspec, const int* ref_ratios, long long* stats,
+                            int nstats) {
   API_BEGIN
   Context dry(0, true);
   Box dom;
@@ -1333,7 +1341,7 @@ int amrmg_plan_check(const int* ispec, const double* rspec, long long* stats, in
   BCSpec bc{};
   std::vector<DBL> g = parse_spec(ispec, rspec, dom, bc, dx0, a, b);
   DeviceHierarchy H;
-  H.define(&dry, g, dom, dx0, true, &bc);
+  H.define(&dry, g, dom, dx0, true, &bc, ref_ratios);
   AMRPoissonOpFactory<double> fac;
   fac.define(&H, a, b, bc);
   BiCGStabSolver<LevelData<double>> bottom;

@@ -98,9 +98,10 @@ struct RefluxCoarse {  // reg += (phi[a]-phi[b])*(beta/dx) * sign
   long long a, b;
   double sign;
 };
-struct RefluxFine {    // reg_f = sum_q scale*(phi_f[a_q]-phi_f[b_q])*(beta*2/dx)
+struct RefluxFine {    // reg_f = sum_q scale*(phi_f[a_q]-phi_f[b_q])*(beta*ref/dx)
   long long a[4], b[4];
   double scale;
+  int first, last;     // a register of ratio^2 fluxes spans ratio^2/4 records, first..last
 };
 
 // ---------------- host-side runtime ----------------

@@ -976,12 +976,14 @@ __global__ void __launch_bounds__(256) k_restrict(const DBox* __restrict__ cboxe
 }
 // Kernels over two layouts index the second layout's box table with the tile's box
 // index: the layouts must pair box for box (one is the coarsening of the other).
-static void check_paired(const DeviceLayout& Lf, const DeviceLayout& Lc, const char* what) {
+static void check_paired(const DeviceLayout& Lf, const DeviceLayout& Lc, const char* what, int ratio = 2) {
+  if (ratio != 2 && ratio != 4) throw std::runtime_error(std::string(what) + ": ratio must be 2 or 4");
+  const int sh = ratio == 4 ? 2 : 1;
   if (Lf.nbox != Lc.nbox || Lf.merged != Lc.merged || Lf.h_boxes.size() != Lc.h_boxes.size())
     throw std::runtime_error(std::string(what) + ": fine and coarse layouts do not pair box for box");
   for (size_t b = 0; b < Lf.h_boxes.size(); b++)
     for (int q = 0; q < 3; q++)
-      if ((Lf.h_boxes[b].lo[q] >> 1) != Lc.h_boxes[b].lo[q] || (Lf.h_boxes[b].hi[q] >> 1) != Lc.h_boxes[b].hi[q])
+      if ((Lf.h_boxes[b].lo[q] >> sh) != Lc.h_boxes[b].lo[q] || (Lf.h_boxes[b].hi[q] >> sh) != Lc.h_boxes[b].hi[q])
         throw std::runtime_error(std::string(what) + ": coarse box is not the coarsened fine box");
 }
 template <class T>
@@ -994,8 +996,9 @@ void restrict_res(const DeviceLayout& Lf, const DeviceLayout& Lc, T* resc, const
          T(beta));
 }
 
-// PROLONG (AMRPoissonOpF.ChF:453-478), refinement ratio 2
-template <class T>
+// PROLONG (AMRPoissonOpF.ChF:453-478), refinement ratio 1 << SH (floor division of
+// the fine index)
+template <class T, int SH>
 __global__ void __launch_bounds__(256) k_prolong(const DBox* __restrict__ fboxes, const Tile* __restrict__ ftiles,
                                                  TileGeom g, const DBox* __restrict__ cboxes, T* __restrict__ phi,
                                                  const T* __restrict__ coarse) {
@@ -1004,13 +1007,17 @@ __global__ void __launch_bounds__(256) k_prolong(const DBox* __restrict__ fboxes
   if (!tile_cell(fboxes, ftiles, g, fb, i, j, k)) return;
   const DBox cb = cboxes[ftiles[blockIdx.x].box];
   const long long c = cidx(fb, i, j, k);
-  phi[c] = phi[c] + coarse[cidx(cb, i >> 1, j >> 1, k >> 1)];
+  phi[c] = phi[c] + coarse[cidx(cb, i >> SH, j >> SH, k >> SH)];
 }
 template <class T>
-void prolong(const DeviceLayout& Lf, const DeviceLayout& Lc, T* phi, const T* coarse) {
+void prolong(const DeviceLayout& Lf, const DeviceLayout& Lc, T* phi, const T* coarse, int ratio) {
   const DeviceLayout& L = Lf;
-  check_paired(Lf, Lc, "prolong");
-  LAUNCH_P(KC_PROLONG, (2.0 + 1.0 / 8.0) * sizeof(T) * Lf.valid_cells, k_prolong<T>, Lf.ntiles, Lf.d_boxes, Lf.d_tiles, geom(Lf), Lc.d_boxes, phi, coarse);
+  check_paired(Lf, Lc, "prolong", ratio);
+  const double bytes = (2.0 + 1.0 / (ratio * ratio * ratio)) * sizeof(T) * Lf.valid_cells;
+  if (ratio == 4)
+    LAUNCH_P(KC_PROLONG, bytes, (k_prolong<T, 2>), Lf.ntiles, Lf.d_boxes, Lf.d_tiles, geom(Lf), Lc.d_boxes, phi, coarse);
+  else
+    LAUNCH_P(KC_PROLONG, bytes, (k_prolong<T, 1>), Lf.ntiles, Lf.d_boxes, Lf.d_tiles, geom(Lf), Lc.d_boxes, phi, coarse);
 }
 
 // average, refRatio 2 branch (lib/src/AMRTools/AverageF.ChF:40-57)
@@ -1028,11 +1035,33 @@ __global__ void __launch_bounds__(256) k_average(const DBox* __restrict__ cboxes
   coarse[cidx(cb, I, J, K)] = refScale * (fine[c] + fine[c + 1] + fine[c + sy] + fine[c + 1 + sy] + fine[c + sz] +
                                           fine[c + 1 + sz] + fine[c + sy + sz] + fine[c + 1 + sy + sz]);
 }
+// average, refRatio 4 branch (AverageF.ChF:60-134): the 64 children summed left to
+// right, x fastest, then times refScale
 template <class T>
-void average(const DeviceLayout& Lf, const DeviceLayout& Lc, T* coarse, const T* fine) {
+__global__ void __launch_bounds__(256) k_average4(const DBox* __restrict__ cboxes, const Tile* __restrict__ ctiles,
+                                                  TileGeom g, const DBox* __restrict__ fboxes, T* __restrict__ coarse,
+                                                  const T* __restrict__ fine) {
+  DBox cb;
+  int I, J, K;
+  if (!tile_cell(cboxes, ctiles, g, cb, I, J, K)) return;
+  const DBox fb = fboxes[ctiles[blockIdx.x].box];
+  const long long c = cidx(fb, 4 * I, 4 * J, 4 * K);
+  const long long sy = fb.sy, sz = fb.sz;
+  T sum = fine[c];
+#pragma unroll
+  for (int t = 1; t < 64; t++) sum = sum + fine[c + (t & 3) + ((t >> 2) & 3) * sy + (t >> 4) * sz];
+  const T refScale = T(1.0 / 64.0);
+  coarse[cidx(cb, I, J, K)] = refScale * sum;
+}
+template <class T>
+void average(const DeviceLayout& Lf, const DeviceLayout& Lc, T* coarse, const T* fine, int ratio) {
   const DeviceLayout& L = Lc;
-  check_paired(Lf, Lc, "average");
-  LAUNCH_P(KC_AVERAGE, (1.0 + 1.0 / 8.0) * sizeof(T) * Lf.valid_cells, k_average<T>, Lc.ntiles, Lc.d_boxes, Lc.d_tiles, geom(Lc), Lf.d_boxes, coarse, fine);
+  check_paired(Lf, Lc, "average", ratio);
+  const double bytes = (1.0 + 1.0 / (ratio * ratio * ratio)) * sizeof(T) * Lf.valid_cells;
+  if (ratio == 4)
+    LAUNCH_P(KC_AVERAGE, bytes, k_average4<T>, Lc.ntiles, Lc.d_boxes, Lc.d_tiles, geom(Lc), Lf.d_boxes, coarse, fine);
+  else
+    LAUNCH_P(KC_AVERAGE, bytes, k_average<T>, Lc.ntiles, Lc.d_boxes, Lc.d_tiles, geom(Lc), Lf.d_boxes, coarse, fine);
 }
 
 // AMRRestrictS's residual and average in one pass: OPERATORLAPRES on the eight fine
@@ -1062,14 +1091,43 @@ __global__ void __launch_bounds__(256) k_residual_average(const DBox* __restrict
   const T refScale = T(1.0 / 8.0);
   coarse[cidx(cb, I, J, K)] = refScale * (r0 + r1 + r2 + r3 + r4 + r5 + r6 + r7);
 }
+// the same for ratio 4: the 64 fine residuals of each coarse cell summed left to
+// right in AVERAGE's order (x fastest), then times refScale
+template <class T>
+__global__ void __launch_bounds__(256) k_residual_average4(const DBox* __restrict__ cboxes,
+                                                           const Tile* __restrict__ ctiles, TileGeom g,
+                                                           const DBox* __restrict__ fboxes, T* __restrict__ coarse,
+                                                           const T* __restrict__ phi, const T* __restrict__ rhs,
+                                                           T dxinv, T alpha, T beta) {
+  DBox cb;
+  int I, J, K;
+  if (!tile_cell(cboxes, ctiles, g, cb, I, J, K)) return;
+  const DBox fb = fboxes[ctiles[blockIdx.x].box];
+  const long long c = cidx(fb, 4 * I, 4 * J, 4 * K);
+  const long long sy = fb.sy, sz = fb.sz;
+  auto res = [&](long long q) {
+    const T p = phi[q];
+    const T lap = (phi[q + 1] + phi[q - 1] + phi[q + sy] + phi[q - sy] + phi[q + sz] + phi[q - sz] - T(6.0) * p) * dxinv;
+    return -(alpha * p) - beta * lap + rhs[q];
+  };
+  T sum = res(c);
+  for (int t = 1; t < 64; t++) sum = sum + res(c + (t & 3) + ((t >> 2) & 3) * sy + (t >> 4) * sz);
+  const T refScale = T(1.0 / 64.0);
+  coarse[cidx(cb, I, J, K)] = refScale * sum;
+}
 template <class T>
 void residual_average(const DeviceLayout& Lf, const DeviceLayout& Lc, T* coarse, const T* phi, const T* rhs,
-                      double dx, double alpha, double beta) {
+                      double dx, double alpha, double beta, int ratio) {
   const DeviceLayout& L = Lc;
-  check_paired(Lf, Lc, "residual_average");
+  check_paired(Lf, Lc, "residual_average", ratio);
   double dxinv = 1.0 / (dx * dx);
-  LAUNCH_P(KC_RESTRICT, (2.0 + 1.0 / 8.0) * sizeof(T) * Lf.valid_cells, k_residual_average<T>, Lc.ntiles, Lc.d_boxes,
-           Lc.d_tiles, geom(Lc), Lf.d_boxes, coarse, phi, rhs, T(dxinv), T(alpha), T(beta));
+  const double bytes = (2.0 + 1.0 / (ratio * ratio * ratio)) * sizeof(T) * Lf.valid_cells;
+  if (ratio == 4)
+    LAUNCH_P(KC_RESTRICT, bytes, k_residual_average4<T>, Lc.ntiles, Lc.d_boxes, Lc.d_tiles, geom(Lc), Lf.d_boxes,
+             coarse, phi, rhs, T(dxinv), T(alpha), T(beta));
+  else
+    LAUNCH_P(KC_RESTRICT, bytes, k_residual_average<T>, Lc.ntiles, Lc.d_boxes, Lc.d_tiles, geom(Lc), Lf.d_boxes,
+             coarse, phi, rhs, T(dxinv), T(alpha), T(beta));
 }
 
 // ---------------------------------------------------------------------------
@@ -1556,9 +1614,11 @@ __global__ void __launch_bounds__(256) k_cf_coarse(const CFCoarse* __restrict__ 
     o[1 + q] = v;
   }
 }
+// np1 = nRef + 1, np3 = nRef + 3, den = nRef^2 + 4 nRef + 3 of FORT_QUADINTERP
+// (QuadCFInterpF.ChF:9-47) and interpOnIVS (QuadCFInterp.cpp:775-839)
 template <class T>
 __global__ void __launch_bounds__(256) k_cf_fine(const CFFine* __restrict__ recs, int n, const T* __restrict__ deriv,
-                                                 T* __restrict__ phif, T h) {
+                                                 T* __restrict__ phif, T h, T np1, T np3, T den) {
   const int r = blockIdx.x * 256 + threadIdx.x;
   if (r >= n) return;
   const CFFine f = recs[r];
@@ -1573,16 +1633,16 @@ __global__ void __launch_bounds__(256) k_cf_fine(const CFFine* __restrict__ recs
   T g;
   if (f.packed) {
     const T frac = T(2.0) / (h * h);
-    const T denom = T(15.0);
+    const T denom = den;
     const T mult = frac / denom;
     const T invh = T(1.0) / h;
     const T x = T(2.0) * h;
     const T xsquared = T(4.0) * h * h;
-    const T a = mult * (T(2.0) * ps + T(3.0) * pa - T(5.0) * pb);
+    const T a = mult * (T(2.0) * ps + np1 * pa - np3 * pb);
     const T b = (pb - pa) * invh - a * h;
     g = xsquared * a + b * x + pa;
   } else {
-    const T a = (T(2.) / h / h) * (T(2.) * ps + pa * T(3.0) - pb * T(5.0)) / T(15.0);
+    const T a = (T(2.) / h / h) * (T(2.) * ps + pa * np1 - pb * np3) / den;
     const T b = (pb - pa) / h - a * h;
     const T x = T(2.) * h;
     g = a * x * x + b * x + pa;
@@ -1591,7 +1651,7 @@ __global__ void __launch_bounds__(256) k_cf_fine(const CFFine* __restrict__ recs
 }
 template <class T>
 void cf_inhomog(Context* ctx, const CFCoarse* dc, int ncoarse, const CFGen* dg, const CFFine* df, int nfine,
-                T* deriv, T* phif, const T* phic, double dxf, double dxc) {
+                T* deriv, T* phif, const T* phic, double dxf, double dxc, int nref) {
   hipEvent_t ev = prof_begin(ctx, KC_CF);
   if (ncoarse > 0) {
     hipLaunchKernelGGL(k_cf_coarse<T>, dim3((ncoarse + 255) / 256), dim3(256), 0, ctx->stream, dc, ncoarse, dg, phic,
@@ -1600,7 +1660,7 @@ void cf_inhomog(Context* ctx, const CFCoarse* dc, int ncoarse, const CFGen* dg, 
   }
   if (nfine > 0) {
     hipLaunchKernelGGL(k_cf_fine<T>, dim3((nfine + 255) / 256), dim3(256), 0, ctx->stream, df, nfine, deriv, phif,
-                       T(dxf));
+                       T(dxf), T(nref + 1.0), T(nref + 3.0), T(nref * nref + 4 * nref + 3.0));
     hip_check(hipGetLastError(), "k_cf_fine");
   }
   prof_end(ctx, ev, KC_CF, 0.0);
@@ -1628,14 +1688,17 @@ __global__ void __launch_bounds__(256) k_reflux(const RefluxCell* __restrict__ c
     }
     val = val + reg * minus_s;
   }
+  // a fine register is one record (ratio 2) or a chain of first..last records of
+  // four fluxes each (ratio 4), accumulated in INCREMENTFINE's order
+  T reg = T(0.0);
   for (int q = c.f0; q < c.f0 + c.nf; q++) {
     const RefluxFine x = rf[q];
-    T reg = T(0.0);
+    if (x.first) reg = T(0.0);
     for (int t = 0; t < 4; t++) {
       const T F = (phif[x.a[t]] - phif[x.b[t]]) * beta_ref_dx_c;
       reg = reg + T(x.scale) * F;
     }
-    val = val + reg * minus_s;
+    if (x.last) val = val + reg * minus_s;
   }
   u[c.u] = val;
 }
@@ -1665,14 +1728,15 @@ __global__ void __launch_bounds__(256) k_reflux_residual(const RefluxCell* __res
     }
     val = val + reg * minus_s;
   }
+  T freg = T(0.0);
   for (int t = c.f0; t < c.f0 + c.nf; t++) {
     const RefluxFine x = rf[t];
-    T reg = T(0.0);
+    if (x.first) freg = T(0.0);
     for (int m = 0; m < 4; m++) {
       const T F = (phif[x.a[m]] - phif[x.b[m]]) * beta_ref_dx_c;
-      reg = reg + T(x.scale) * F;
+      freg = freg + T(x.scale) * F;
     }
-    val = val + reg * minus_s;
+    if (x.last) val = val + freg * minus_s;
   }
   res[q] = apply_axby(val, rhs[q]);
 }
@@ -1718,10 +1782,10 @@ void reflux(Context* ctx, const RefluxCell* cells, int ncells, const RefluxCoars
   template void residual<T>(const DeviceLayout&, T*, const T*, const T*, double, double, double);              \
   template void restrict_res<T>(const DeviceLayout&, const DeviceLayout&, T*, const T*, const T*, double, double, \
                                 double);                                                                      \
-  template void prolong<T>(const DeviceLayout&, const DeviceLayout&, T*, const T*);                            \
-  template void average<T>(const DeviceLayout&, const DeviceLayout&, T*, const T*);                            \
+  template void prolong<T>(const DeviceLayout&, const DeviceLayout&, T*, const T*, int);                       \
+  template void average<T>(const DeviceLayout&, const DeviceLayout&, T*, const T*, int);                       \
   template void residual_average<T>(const DeviceLayout&, const DeviceLayout&, T*, const T*, const T*, double,     \
-                                    double, double);                                                          \
+                                    double, double, int);                                                     \
   template void apply_residual<T>(const DeviceLayout&, T*, const T*, const T*, double, double, double);        \
   template void reflux_residual<T>(Context*, const RefluxCell*, int, const RefluxCoarse*, const RefluxFine*, T*, \
                                    const T*, const T*, const T*, double, double, double, double, double, double); \
@@ -1743,7 +1807,7 @@ void reflux(Context* ctx, const RefluxCell* cells, int ncells, const RefluxCoars
   template double norm2_ordered<T>(const DeviceLayout&, const T*, int);                                        \
   template double norm1_ordered<T>(const DeviceLayout&, const T*, int);                                        \
   template void cf_inhomog<T>(Context*, const CFCoarse*, int, const CFGen*, const CFFine*, int, T*, T*, const T*, \
-                              double, double);                                                                \
+                              double, double, int);                                                           \
   template void reflux<T>(Context*, const RefluxCell*, int, const RefluxCoarse*, const RefluxFine*, T*, const T*, \
                           const T*, double, double, double);
 INST(double)

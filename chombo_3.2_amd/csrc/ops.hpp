@@ -93,15 +93,16 @@ void residual(const DeviceLayout& L, T* res, const T* phi, const T* rhs, double 
 template <class T>
 void restrict_res(const DeviceLayout& Lf, const DeviceLayout& Lc, T* resc, const T* phi, const T* rhs,
                   double dx, double alpha, double beta);
+// PROLONG and AVERAGE between a layout and its coarsening by ratio (2 or 4)
 template <class T>
-void prolong(const DeviceLayout& Lf, const DeviceLayout& Lc, T* phi, const T* coarse);
+void prolong(const DeviceLayout& Lf, const DeviceLayout& Lc, T* phi, const T* coarse, int ratio = 2);
 template <class T>
-void average(const DeviceLayout& Lf, const DeviceLayout& Lc, T* coarse, const T* fine);
-// coarse = AVERAGE(rhs - L(phi)) over coarsen(fine, 2) without storing the fine
+void average(const DeviceLayout& Lf, const DeviceLayout& Lc, T* coarse, const T* fine, int ratio = 2);
+// coarse = AVERAGE(rhs - L(phi)) over coarsen(fine, ratio) without storing the fine
 // residual (bitwise residual followed by average); phi's ghosts must be filled
 template <class T>
 void residual_average(const DeviceLayout& Lf, const DeviceLayout& Lc, T* coarse, const T* phi, const T* rhs,
-                      double dx, double alpha, double beta);
+                      double dx, double alpha, double beta, int ratio = 2);
 
 template <class T>
 void incr(const DeviceLayout& L, T* lhs, const T* x, double scale);
@@ -162,7 +163,7 @@ void maxnorm_async(const DeviceLayout& L, const T* x, int slot, int accumulate);
 // inhomogeneous coarse-fine interpolation (QuadCFInterp::coarseFineInterp)
 template <class T>
 void cf_inhomog(Context* ctx, const CFCoarse* dc, int ncoarse, const CFGen* dg, const CFFine* df, int nfine,
-                T* deriv_scratch, T* phif, const T* phic, double dxf, double dxc);
+                T* deriv_scratch, T* phif, const T* phic, double dxf, double dxc, int nref = 2);
 
 // AMRResidual's AMROperator + axby(res, res, rhs, -1, 1) in two launches: every valid
 // cell from its stencil, then the refluxed cells again with their corrections;

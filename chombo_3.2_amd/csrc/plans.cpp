@@ -224,7 +224,7 @@ struct QStencils {
 CFPlanHost build_cf_plan(const DBL& fine, const DeviceLayout& fineLay, const DBL& coarse,
                          const DeviceLayout& coarseLay, const Box& fineDomain, double dxFine, int ref) {
   CFPlanHost out;
-  if (ref != 2) throw std::runtime_error("build_cf_plan: refinement ratio 2 only");
+  if (ref != 2 && ref != 4) throw std::runtime_error("build_cf_plan: refinement ratio 2 or 4 only");
   // QuadCFInterp::define: no interpolation when the fine level covers the coarse one
   if (fine.numCells() / (ref * ref * ref) - coarse.numCells() == 0) return out;
   BoxFinder ff(fine), cfind(coarse);
@@ -451,7 +451,7 @@ CFPlanHost build_cf_plan(const DBL& fine, const DeviceLayout& fineLay, const DBL
 RefluxPlanHost build_reflux_plan(const DBL& coarse, const DeviceLayout& coarseLay, const DBL& fine,
                                  const DeviceLayout& fineLay, const Box& coarseDomain, int ref) {
   RefluxPlanHost out;
-  if (ref != 2) throw std::runtime_error("reflux: refinement ratio 2 only");
+  if (ref != 2 && ref != 4) throw std::runtime_error("reflux: refinement ratio 2 or 4 only");
   DBL cfine = fine.coarsened(ref);
   BoxFinder cff(cfine), cfind(coarse);
   (void)coarseDomain;
@@ -518,12 +518,14 @@ RefluxPlanHost build_reflux_plan(const DBL& coarse, const DeviceLayout& coarseLa
             else if (c[d] > fbx.hi[d]) { nout++; dir = d; side = 1; }
           }
           if (nout != 1) continue;  // interior or edge/corner ghost: register holds zero
-          RefluxFine x;
-          RefluxFace xf{it.F, dir, {}};
-          x.scale = (side ? 1.0 : -1.0) * 1.0 / 4.0;
+          // sign(side)*a_scale/denom, denom = nRef^(D-1) (LevelFluxRegister.cpp:468-481)
+          double denom = 1.0;
+          for (int d = 0; d < 3; d++)
+            if (d != dir) denom *= ref;
+          const double scale = (side ? 1.0 : -1.0) * 1.0 / denom;
           // fine cells just outside fgrid under c, INCREMENTFINE loop order
           Box fcell = refine(pointBox(c), ref);
-          int t = 0;
+          std::vector<IV> nodes;
           for (int kk = fcell.lo[2]; kk <= fcell.hi[2]; kk++)
             for (int jj = fcell.lo[1]; jj <= fcell.hi[1]; jj++)
               for (int ii = fcell.lo[0]; ii <= fcell.hi[0]; ii++) {
@@ -532,17 +534,28 @@ RefluxPlanHost build_reflux_plan(const DBL& coarse, const DeviceLayout& coarseLa
                 if (side == 1 && q[dir] != fgrid.hi[dir] + 1) continue;
                 IV n = q;
                 if (side == 0) n[dir] += 1;
-                IV m1 = n;
-                m1[dir] -= 1;
-                x.a[t] = fineLay.index(it.F, n[0], n[1], n[2]);
-                x.b[t] = fineLay.index(it.F, m1[0], m1[1], m1[2]);
-                for (int q3 = 0; q3 < 3; q3++) xf.iv[t][q3] = n[q3];
-                t++;
+                nodes.push_back(n);
               }
-          if (t != 4) throw std::runtime_error("reflux: fine face must have 4 cells");
+          if ((int)nodes.size() != ref * ref) throw std::runtime_error("reflux: fine face must have ratio^2 cells");
+          // the register as records of four fluxes, first..last
           CellLists& cl = cells[{it.C, coarseLay.index(it.C, i, j, k)}];
-          cl.f.push_back(x);
-          cl.fface.push_back(xf);
+          for (size_t c0 = 0; c0 < nodes.size(); c0 += 4) {
+            RefluxFine x;
+            RefluxFace xf{it.F, dir, {}};
+            x.scale = scale;
+            x.first = c0 == 0;
+            x.last = c0 + 4 == nodes.size();
+            for (int t = 0; t < 4; t++) {
+              const IV& n = nodes[c0 + t];
+              IV m1 = n;
+              m1[dir] -= 1;
+              x.a[t] = fineLay.index(it.F, n[0], n[1], n[2]);
+              x.b[t] = fineLay.index(it.F, m1[0], m1[1], m1[2]);
+              for (int q3 = 0; q3 < 3; q3++) xf.iv[t][q3] = n[q3];
+            }
+            cl.f.push_back(x);
+            cl.fface.push_back(xf);
+          }
         }
   }
   for (auto& kv : cells) {

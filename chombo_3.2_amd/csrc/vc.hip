@@ -768,16 +768,17 @@ __global__ void __launch_bounds__(256) k_vc_reflux(const RefluxCell* __restrict_
     }
     val = val + reg * minus_s;
   }
+  T freg = T(0.0);
   for (int q = c.f0; q < c.f0 + c.nf; q++) {
     const RefluxFine x = rf[q];
     const VCFaceIdx f = ff[q];
-    T reg = T(0.0);
+    if (x.first) freg = T(0.0);  // a chain of records per register with ratio 4
     for (int t = 0; t < 4; t++) {
       T F = (phif[x.a[t]] - phif[x.b[t]]) * beta_ref_dx_c;
       F = -bf.b[f.dir][f.x[t]] * F;
-      reg = reg + T(x.scale) * F;
+      freg = freg + T(x.scale) * F;
     }
-    val = val + reg * minus_s;
+    if (x.last) val = val + freg * minus_s;
   }
   u[c.u] = val;
 }

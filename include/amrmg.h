@@ -66,6 +66,13 @@ int amrmg_ctx_set_fused_relax(amrmg_ctx* ctx, int on);
    bottom solver (replaces AMRPoissonOp.cpp:1857-2070, AMRMultiGrid.H:1212-1328) */
 int amrmg_solver_create(amrmg_ctx* ctx, const int* ispec, const double* rspec, int precision,
                         amrmg_solver** out);
+/* the same with the refinement ratio between each pair of consecutive levels:
+   ref_ratios[l-1] (2 or 4) between levels l-1 and l, nlev-1 entries (NULL: 2 everywhere),
+   as AMRPoissonOpFactory::define's refRatios (AMRPoissonOp.cpp:1857-1903).  A level of
+   ratio 4 smooths as AMRMultiGrid::relax does there (AMRMultiGrid.H:633-666): m_bottom
+   relaxations per visit (one when its multigrid bottom has one cell), not m_pre / m_post. */
+int amrmg_solver_create_ratios(amrmg_ctx* ctx, const int* ispec, const double* rspec, const int* ref_ratios,
+                               int precision, amrmg_solver** out);
 int amrmg_solver_destroy(amrmg_solver* s);
 int amrmg_solver_set_params(amrmg_solver* s, const int* sparams, const double* rparams);
 /* AMRPoissonOp::setAlphaAndBeta (AMRPoissonOp.cpp:1103-1108, unit aCoef and bCoef) on
@@ -433,6 +440,9 @@ int amrmg_copy_probe(amrmg_ctx* ctx, long long nbytes, int iters, double* ms_out
    {levels, multigrid depths, ghost patches, CF coarse records, CF fine records,
    reflux cells, tiles, valid cells}. */
 int amrmg_plan_check(const int* ispec, const double* rspec, long long* stats, int nstats);
+/* the same with refinement ratios as for amrmg_solver_create_ratios */
+int amrmg_plan_check_ratios(const int* ispec, const double* rspec, const int* ref_ratios, long long* stats,
+                            int nstats);
 
 /* ---- event profiler over kernel classes (0 GSRB, 1 residual, 2 applyOp,
    3 restrict, 4 prolong, 5 average, 6 ghost fill, 7 BLAS-1, 8 reductions,
