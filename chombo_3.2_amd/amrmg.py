@@ -84,6 +84,7 @@ def lib():
         L.amrmg_batch_solve.argtypes = [C.c_void_p, _ip, _ip]
         L.amrmg_batch_history.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _ip]
         L.amrmg_batch_tune_relax.argtypes = [C.c_void_p, _dp, C.c_int, _ip]
+        L.amrmg_batch_set_relax_shape.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
         L.amrmg_batch_stats.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
         L.amrmg_batch_set_graph.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_longlong)]
         L.amrmg_batch_set_reduction_order.argtypes = [C.c_void_p, C.c_int]
@@ -254,7 +255,9 @@ class AMRMultiGrid:
     RELAX_SHAPES = [(64, 8, 256, 0), (32, 8, 256, 0), (64, 4, 256, 0), (32, 16, 256, 0), (128, 4, 256, 0),
                     (64, 16, 256, 0), (32, 16, 512, 0), (64, 16, 512, 0), (64, 32, 1024, 0), (128, 8, 1024, 0),
                     (64, 13, 512, 0), (32, 13, 256, 0), (64, 29, 1024, 0), (64, 26, 1024, 1), (32, 24, 512, 1),
-                    (64, 11, 512, 1), (32, 10, 256, 1), (128, 11, 1024, 1), (16, 21, 256, 1)]
+                    (64, 11, 512, 1), (32, 10, 256, 1), (128, 11, 1024, 1), (16, 21, 256, 1),
+                    (64, 11, 256, 2), (128, 11, 512, 2), (64, 26, 512, 2), (32, 24, 256, 2),
+                    (128, 26, 1024, 2), (64, 56, 1024, 2), (16, 47, 256, 2)]
     RELAX_CHUNKS = [0, 8, 16, 32]
 
     def tune_relax(self):
@@ -548,6 +551,11 @@ class BatchSolver:
         n = C.c_int()
         _chk(lib().amrmg_batch_tune_relax(self.p, ms.ctypes.data_as(_dp), 64, C.byref(n)))
         return ms[:n.value].copy()
+
+    def set_relax_shape(self, depth, shape, kc=0):
+        """Forces the relaxation tile shape (index into AMRMultiGrid.RELAX_SHAPES) and
+        z-chunk of one multigrid depth."""
+        _chk(lib().amrmg_batch_set_relax_shape(self.p, int(depth), int(shape), int(kc)))
 
     def stats(self):
         cr = C.c_double()

@@ -90,7 +90,11 @@ __global__ void __launch_bounds__(NT)
   const BatchCoef cf = coef[p];
   ops::FusedArgs a = items[p];
   a.zsrc = zsrc;
-  if constexpr (ops::kFusedShapeTable[SHAPE][3] == 1)
+  if constexpr (ops::kFusedShapeTable[SHAPE][3] == 2) {
+    static_assert(sizeof(T) == 4, "x-quad tiles are FP32 only");
+    ops::zmarch4_column<T, LAP, FX, FY, NT>(a, wg - wstart[p], dst, src, rhs, T(cf.dxinv), T(cf.lambda),
+                                            T(cf.alpha), T(cf.beta), T(0.0), T(0.0));
+  } else if constexpr (ops::kFusedShapeTable[SHAPE][3] == 1)
     ops::zmarch_column<T, LAP, FX, FY, NT>(a, wg - wstart[p], dst, src, rhs, T(cf.dxinv), T(cf.lambda),
                                            T(cf.alpha), T(cf.beta), T(0.0), T(0.0));
   else
@@ -1113,22 +1117,24 @@ BatchSolver::~BatchSolver() {
 
 long long BatchSolver::cells(int q) const { return m_prob[m_slot[q]].domain.npts(); }
 
+// whether tile shape `shape` applies to every problem of depth d (the relaxation
+// runs in FP32 for a mixed batch)
+bool BatchSolver::shapeApplies(int d, int shape) const {
+  const Depth& L = m_depth[d];
+  for (int p = 0; p < L.np; p++)
+    if (!ops::fused_shape_ok(shape, m_prec ? 4 : 8, L.h_rect[p].hi[0] - L.h_rect[p].lo[0] + 1)) return false;
+  return true;
+}
+
 void BatchSolver::setRelaxShape(int d, int shape, int kc) {
-  dropGraph();  // the captured cycle holds the old work tables and shape
-  Depth& L = m_depth[d];
-  if (L.d_items) { dev_free(m_ctx, L.d_items); L.d_items = nullptr; }
-  if (L.d_wStart) { dev_free(m_ctx, L.d_wStart); L.d_wStart = nullptr; }
+  // the work tables are built and checked before anything is released, so a
+  // rejected shape leaves the depth as it was
+  if (d < 0 || d >= (int)m_depth.size()) throw std::runtime_error("batch: bad depth");
   if (shape < 0 || shape >= ops::kFusedShapeCount) throw std::runtime_error("batch: bad tile shape");
+  if (!shapeApplies(d, shape))
+    throw std::runtime_error("batch: x-quad tile shapes need mixed precision and x extents that are multiples of 4");
+  Depth& L = m_depth[d];
   const int FX = ops::kFusedShapeTable[shape][0], FY = ops::kFusedShapeTable[shape][1];
-  L.shape = shape;
-  L.kc = kc;
-  L.h_items.clear();
-  L.wStart.assign(1, 0);
-  long long cols = 0;
-  for (int p = 0; p < L.np; p++) {
-    const DBox& R = L.h_rect[p];
-    cols += (long long)((R.hi[0] - R.lo[0] + FX) / FX) * ((R.hi[1] - R.lo[1] + FY) / FY);
-  }
   int k = kc;
   if (k <= 0) {  // long enough chunks to amortise the prologue, enough work items to fill the chip
     k = 32;
@@ -1143,6 +1149,8 @@ void BatchSolver::setRelaxShape(int d, int shape, int kc) {
       k >>= 1;
     }
   }
+  std::vector<ops::FusedArgs> items;
+  std::vector<int> wStart(1, 0);
   for (int p = 0; p < L.np; p++) {
     const DBox& R = L.h_rect[p];
     ops::FusedArgs a{};
@@ -1154,16 +1162,23 @@ void BatchSolver::setRelaxShape(int d, int shape, int kc) {
     a.nty = (R.hi[1] - R.lo[1] + FY) / FY;
     a.kc = k;
     const long long nw = (long long)a.ntx * a.nty * ((R.hi[2] - R.lo[2] + k) / k);
-    if (nw + L.wStart.back() > INT32_MAX - 8) throw std::runtime_error("batch: too many work items");
+    if (nw + wStart.back() > INT32_MAX - 8) throw std::runtime_error("batch: too many work items");
     a.nwork = (int)nw;
     for (int dir = 0; dir < 3; dir++)
       for (int side = 0; side < 2; side++) a.kind[dir][side] = L.h_coef[p].kind[dir][side];
     a.zsrc = 0;
     ops::check_pair_rows(a);
-    L.h_items.push_back(a);
-    L.wStart.push_back(L.wStart.back() + a.nwork);
+    if (ops::kFusedShapeTable[shape][3] == 2) ops::check_quad_rows(a);
+    items.push_back(a);
+    wStart.push_back(wStart.back() + a.nwork);
   }
-  (void)cols;
+  dropGraph();  // the captured cycle holds the old work tables and shape
+  if (L.d_items) { dev_free(m_ctx, L.d_items); L.d_items = nullptr; }
+  if (L.d_wStart) { dev_free(m_ctx, L.d_wStart); L.d_wStart = nullptr; }
+  L.shape = shape;
+  L.kc = kc;
+  L.h_items = std::move(items);
+  L.wStart = std::move(wStart);
   L.d_items = upload(m_ctx, L.h_items, &m_bytes);
   L.d_wStart = upload(m_ctx, L.wStart, &m_bytes);
 }
@@ -1240,11 +1255,19 @@ static void launch_batch_fused(Context* ctx, int shape, int grid, const ops::Fus
     hipLaunchKernelGGL((k_batch_fused<T, LAP, q>), dim3(grid), dim3(ops::kFusedShapeTable[q][2]), 0, ctx->stream, \
                        items, wstart, n, coef, active, zsrc, dst, src, rhs);                                     \
     break;
-  static_assert(ops::kFusedShapeCount == 19, "batch: one case per tile shape");
+  static_assert(ops::kFusedShapeCount == 26, "batch: one case per tile shape");
   switch (shape) {
     BF_CASE(0) BF_CASE(1) BF_CASE(2) BF_CASE(3) BF_CASE(4) BF_CASE(5) BF_CASE(6) BF_CASE(7) BF_CASE(8) BF_CASE(9)
     BF_CASE(10) BF_CASE(11) BF_CASE(12) BF_CASE(13) BF_CASE(14) BF_CASE(15) BF_CASE(16) BF_CASE(17) BF_CASE(18)
-    default: throw std::runtime_error("batch: bad tile shape");
+    default:
+      if constexpr (sizeof(T) == 4) {  // x-quad shapes (FP32)
+        switch (shape) {
+          BF_CASE(19) BF_CASE(20) BF_CASE(21) BF_CASE(22) BF_CASE(23) BF_CASE(24) BF_CASE(25)
+          default: throw std::runtime_error("batch: bad tile shape");
+        }
+      } else {
+        throw std::runtime_error("batch: bad tile shape for FP64");
+      }
   }
 #undef BF_CASE
   hip_check(hipGetLastError(), "k_batch_fused");
@@ -1502,6 +1525,7 @@ std::vector<double> BatchSolver::tuneRelax() {
     int bs = 0, bk = 0;
     for (int sh = 0; sh < ops::kFusedShapeCount; sh++)
       for (int q = 0; q < 4; q++) {
+        if (!shapeApplies((int)d, sh)) continue;
         setRelaxShape((int)d, sh, kcs[q]);
         auto run = [&]() {
           if (m_prec) relaxDepth<float>((int)d, np, 1, false);

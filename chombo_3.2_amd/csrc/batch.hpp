@@ -124,6 +124,7 @@ class BatchSolver {
   // (default) or a fixed tree per problem (same result alone and in any batch,
   // different from the reference order by rounding)
   void setTreeOrder(bool on);
+  bool shapeApplies(int d, int shape) const;
   bool treeOrder() const { return m_treeOrder; }
   // run later launches on another context's stream (same device)
   void setContext(Context* c);

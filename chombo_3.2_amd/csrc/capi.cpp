@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "amrpoissonop.hpp"
+#include "fused.hpp"
 #include "batch.hpp"
 #include "handles.hpp"
 #include "mixed.hpp"
@@ -656,16 +657,19 @@ int amrmg_solver_set_relax_shape(amrmg_solver* s, int shape, int kc) {
   API_BEGIN
   if (shape < 0 || shape >= ops::kFusedShapeCount || kc < 0 || kc > 1024)
     throw std::runtime_error("relax shape out of range");
-  for (int l = 0; l < s->nlev(); l++) { s->amg.m_op[l]->m_relaxShape = shape; s->amg.m_op[l]->m_relaxKc = kc; }
-  for (int d = 1; d < s->H.mgDepth(); d++) { s->mgop(d)->m_relaxShape = shape; s->mgop(d)->m_relaxKc = kc; }
+  // a shape that does not apply to a level (x quads on FP64 or on an x extent that
+  // is not a multiple of 4) leaves that level's shape as it is
+  auto set = [&](auto* op, int bytes) {
+    if (!ops::fused_shape_ok(shape, bytes, op->m_lay->rect.len(0))) return;
+    op->m_relaxShape = shape;
+    op->m_relaxKc = kc;
+  };
+  for (int l = 0; l < s->nlev(); l++) set(s->amg.m_op[l].get(), 8);
+  for (int d = 1; d < s->H.mgDepth(); d++) set(s->mgop(d), 8);
   if (s->mixed) {
     auto& m = s->mixed->amg;
-    for (int l = 0; l < s->nlev(); l++) { m.m_op[l]->m_relaxShape = shape; m.m_op[l]->m_relaxKc = kc; }
-    for (int d = 1; d < s->H.mgDepth(); d++) {
-      auto* op = static_cast<AMRPoissonOp<float>*>(m.m_mg0->m_op[d]);
-      op->m_relaxShape = shape;
-      op->m_relaxKc = kc;
-    }
+    for (int l = 0; l < s->nlev(); l++) set(m.m_op[l].get(), 4);
+    for (int d = 1; d < s->H.mgDepth(); d++) set(static_cast<AMRPoissonOp<float>*>(m.m_mg0->m_op[d]), 4);
   }
   API_END
 }
@@ -985,6 +989,15 @@ int amrmg_batch_history(amrmg_batch* b, int q, double* hist, int cap, int* n) {
   const std::vector<double>& h = s.history(k);
   for (int i = 0; i < cap && i < (int)h.size(); i++) hist[i] = h[i];
   if (n) *n = (int)h.size();
+  API_END
+}
+int amrmg_batch_set_relax_shape(amrmg_batch* b, int depth, int shape, int kc) {
+  API_BEGIN
+  if (kc < 0 || kc > 1024) throw std::runtime_error("batch: z-chunk out of range");
+  for (auto& l : b->lanes) {
+    if (depth < 0 || depth >= l.solver->numDepths()) throw std::runtime_error("batch: bad depth");
+    l.solver->setRelaxShape(depth, shape, kc);
+  }
   API_END
 }
 int amrmg_batch_tune_relax(amrmg_batch* b, double* ms, int cap, int* ndepth) {

@@ -259,15 +259,31 @@ __device__ __forceinline__ void fused_column(const FusedArgs& a, int w, T* __res
 // Tile shapes of the one-launch relaxation: {FX, FY, NT, scheme}, column FX x FY,
 // NT threads; scheme 0 is fused_column (five LDS planes), scheme 1 is
 // zmarch_column (zmarch.hpp: z neighbours in registers, (FX/2 + 2) x (FY + 4)
-// threads own the column grown by 2).  All give bitwise the same result; the
-// fastest depends on the level's size.
+// threads own the column grown by 2 in x pairs), scheme 2 is zmarch4_column (the
+// same with x quads, (FX/4 + 1) x (FY + 4) threads; FP32 and x extents that are a
+// multiple of 4 only).  All give bitwise the same result; the fastest depends on
+// the level's size.
 // Shapes 10-12 are sized so that the red cells of the column grown by one,
 // (FY + 2)(FX / 2 + 1), just fit in one pass of the NT threads.
 constexpr int kFusedShapeTable[][4] = {
     {64, 8, 256, 0},   {32, 8, 256, 0},    {64, 4, 256, 0},    {32, 16, 256, 0},  {128, 4, 256, 0},
     {64, 16, 256, 0},  {32, 16, 512, 0},   {64, 16, 512, 0},   {64, 32, 1024, 0}, {128, 8, 1024, 0},
     {64, 13, 512, 0},  {32, 13, 256, 0},   {64, 29, 1024, 0},  {64, 26, 1024, 1}, {32, 24, 512, 1},
-    {64, 11, 512, 1},  {32, 10, 256, 1},   {128, 11, 1024, 1}, {16, 21, 256, 1}};
+    {64, 11, 512, 1},  {32, 10, 256, 1},   {128, 11, 1024, 1}, {16, 21, 256, 1},  {64, 11, 256, 2},
+    {128, 11, 512, 2}, {64, 26, 512, 2},   {32, 24, 256, 2},   {128, 26, 1024, 2}, {64, 56, 1024, 2},
+    {16, 47, 256, 2}};
+// Whether a shape applies to a rectangle of x extent nx in elem_bytes-wide values
+// (scheme 2: FP32, nx a multiple of 4).
+inline bool fused_shape_ok(int shape, int elem_bytes, int nx) {
+  return kFusedShapeTable[shape][3] != 2 || (elem_bytes == 4 && nx % 4 == 0);
+}
+// x-quad rows: quads start at lo[0] - 2 (one value precedes the first row) and the
+// one holding hi[0] ends at hi[0] + 2 (rows padded by at least 4 values)
+inline void check_quad_rows(const FusedArgs& a) {
+  const int nx = a.hi[0] - a.lo[0] + 1;
+  if (nx % 4 != 0 || a.off < 1 || a.sy < nx + 4)
+    throw std::runtime_error("one-launch relaxation: x-quad tiles need an x extent that is a multiple of 4 and padded rows");
+}
 
 // consecutive work items on one XCD (workgroups go round-robin over the 8 XCDs,
 // grid a multiple of 8), so neighbouring columns share that XCD's L2 for their

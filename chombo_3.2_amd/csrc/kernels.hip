@@ -658,6 +658,16 @@ __global__ void __launch_bounds__(NT) k_gsrb_zmarch(FusedArgs a, T* __restrict__
   if (w >= a.nwork) return;
   zmarch_column<T, LAP, FX, FY, NT>(a, w, dst, src, rhs, dxinv, lambda, alpha, beta, c1, c2);
 }
+// the register march with x quads (FP32)
+template <class T, bool LAP, int FX, int FY, int NT>
+__global__ void __launch_bounds__(NT) k_gsrb_zmarch4(FusedArgs a, T* __restrict__ dst, const T* __restrict__ src,
+                                                      const T* __restrict__ rhs, T dxinv, T lambda, T alpha, T beta,
+                                                      T c1, T c2) {
+  static_assert(sizeof(T) == 4, "x-quad tiles are FP32 only");
+  const int w = xcd_contiguous_block();
+  if (w >= a.nwork) return;
+  zmarch4_column<T, LAP, FX, FY, NT>(a, w, dst, src, rhs, dxinv, lambda, alpha, beta, c1, c2);
+}
 
 constexpr int kNumFusedShapes = kFusedShapeCount;
 static_assert(sizeof(kFusedShapeTable) / sizeof(kFusedShapeTable[0]) == kNumFusedShapes, "shape table");
@@ -696,7 +706,27 @@ static void launch_fused(Context* ctx, int shape, int grid, const FusedArgs& a, 
     ZMARCH_CASE(16, 32, 10, 256)
     ZMARCH_CASE(17, 128, 11, 1024)
     ZMARCH_CASE(18, 16, 21, 256)
-    default: throw std::runtime_error("gsrb_fused: bad tile shape");
+    default:
+      if constexpr (sizeof(T) == 4) {
+#define ZMARCH4_CASE(n, X, Y, NT)                                                                                 \
+  case n:                                                                                                         \
+    hipLaunchKernelGGL((k_gsrb_zmarch4<T, LAP, X, Y, NT>), dim3(grid), dim3(NT), 0, ctx->stream, a, dst, src, rhs, \
+                       dxinv, lambda, alpha, beta, c1, c2);                                                       \
+    break;
+        switch (shape) {
+          ZMARCH4_CASE(19, 64, 11, 256)
+          ZMARCH4_CASE(20, 128, 11, 512)
+          ZMARCH4_CASE(21, 64, 26, 512)
+          ZMARCH4_CASE(22, 32, 24, 256)
+          ZMARCH4_CASE(23, 128, 26, 1024)
+          ZMARCH4_CASE(24, 64, 56, 1024)
+          ZMARCH4_CASE(25, 16, 47, 256)
+          default: throw std::runtime_error("gsrb_fused: bad tile shape");
+        }
+#undef ZMARCH4_CASE
+      } else {
+        throw std::runtime_error("gsrb_fused: bad tile shape for FP64");
+      }
   }
 #undef FUSED_CASE
 #undef ZMARCH_CASE
@@ -704,9 +734,12 @@ static void launch_fused(Context* ctx, int shape, int grid, const FusedArgs& a, 
   prof_end(ctx, ev, KC_GSRB, bytes, units);
 }
 
-static FusedArgs fused_args(const DeviceLayout& L, const FaceRules& fr, int shape, int kc, bool zero_src = false) {
+static FusedArgs fused_args(const DeviceLayout& L, const FaceRules& fr, int shape, int kc, bool zero_src,
+                            int elem_bytes) {
   if (!L.merged) throw std::runtime_error("gsrb_fused: level is not stored merged");
   if (shape < 0 || shape >= kNumFusedShapes) throw std::runtime_error("gsrb_fused: bad tile shape");
+  if (!fused_shape_ok(shape, elem_bytes, L.rect.len(0)))
+    throw std::runtime_error("gsrb_fused: x-quad tile shapes need FP32 and an x extent that is a multiple of 4");
   const int FX = kFusedShapeTable[shape][0], FY = kFusedShapeTable[shape][1];
   const DBox& R = L.h_boxes[L.nbox];
   FusedArgs a;
@@ -720,6 +753,7 @@ static FusedArgs fused_args(const DeviceLayout& L, const FaceRules& fr, int shap
   a.sy = R.sy;
   a.sz = R.sz;
   check_pair_rows(a);
+  if (kFusedShapeTable[shape][3] == 2) check_quad_rows(a);
   a.zsrc = zero_src ? 1 : 0;
   a.ntx = (L.rect.len(0) + FX - 1) / FX;
   a.nty = (L.rect.len(1) + FY - 1) / FY;
@@ -742,7 +776,7 @@ void gsrb_fused(const DeviceLayout& L, const FaceRules& fr, T* dst, const T* src
   if (dst == src || dst == rhs) throw std::runtime_error("gsrb_fused: dst must be a separate field");
   check_pair_pointer(dst, "gsrb_fused");
   check_pair_pointer(src, "gsrb_fused");
-  const FusedArgs a = fused_args(L, fr, shape, kc, zero_src);
+  const FusedArgs a = fused_args(L, fr, shape, kc, zero_src, (int)sizeof(T));
   const int grid = (a.nwork + 7) / 8 * 8;
   double dxinv = 1.0 / (dx * dx);
   double sum_b = 0.0;
@@ -775,6 +809,10 @@ std::pair<int, int> tune_gsrb_fused(const DeviceLayout& L, const FaceRules& fr, 
   std::pair<int, int> pick{0, 0};
   for (int sh = 0; sh < kNumFusedShapes; sh++)
     for (int q = 0; q < 4; q++) {
+      if (!fused_shape_ok(sh, (int)sizeof(T), L.rect.len(0))) {
+        if (times) times[sh * 4 + q] = -1.0;  // does not apply to this level
+        continue;
+      }
       gsrb_fused<T>(L, fr, dst, src, rhs, dx, alpha, beta, sh, kcs[q]);  // warm
       hip_check(hipEventRecord(e0, L.ctx->stream), "record");
       const int reps = 3;

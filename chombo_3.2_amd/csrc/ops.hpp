@@ -81,7 +81,7 @@ void gsrb_fused(const DeviceLayout& L, const FaceRules& fr, T* dst, const T* src
 // Autotuner of the one-launch relaxation: times every tile shape with z-chunk
 // lengths {auto, 8, 16, 32} on the given fields, returns the fastest {shape, kc};
 // times (kFusedShapeCount x 4 entries, ms per launch) receives all timings when not null.
-constexpr int kFusedShapeCount = 19;
+constexpr int kFusedShapeCount = 26;
 template <class T>
 std::pair<int, int> tune_gsrb_fused(const DeviceLayout& L, const FaceRules& fr, T* dst, const T* src, const T* rhs,
                                     double dx, double alpha, double beta, double* times);

@@ -234,6 +234,11 @@ int amrmg_batch_solve(amrmg_batch* b, int* iterations, int* exit_status);
 int amrmg_batch_history(amrmg_batch* b, int problem, double* hist, int cap, int* n);
 /* autotunes the relaxation tile shape per multigrid depth; ms per depth */
 int amrmg_batch_tune_relax(amrmg_batch* b, double* ms, int cap, int* ndepth);
+/* No counterpart in the reference: forces the relaxation tile shape (index into the
+   solver's shape table, as amrmg_solver_set_relax_shape) and z-chunk of one multigrid
+   depth in every lane; results are bitwise the same for every shape that applies, and
+   a shape that does not apply to every problem of the depth is rejected. */
+int amrmg_batch_set_relax_shape(amrmg_batch* b, int depth, int shape, int kc);
 /* cell-relaxations and bottom BiCGStab iterations of the last solve; HBM bytes held */
 int amrmg_batch_stats(amrmg_batch* b, double* cell_relax, long long* bottom_iters, long long* device_bytes);
 /* No counterpart in the reference.  As amrmg_batch_create, with lanes = 1, 2 or 3.

@@ -217,3 +217,44 @@ def test_batch_reduction_order_argument_checked(gpu_ctx):
     with pytest.raises(amrmg.AMRMGError):
         b.set_reduction_order(2)
     b.close()
+
+
+def test_batch_quad_tiles_bitwise_equal_pair_tiles(gpu_ctx):
+    """Mixed-precision batches: every x-quad tile shape on every depth where it
+    applies gives the x-pair march's results bit for bit; FP64 batches and depths
+    with an x extent that is not a multiple of 4 reject them."""
+    probs = [p for p in _edge_set() if p[0].domain[1][0] - p[0].domain[0][0] + 1 in (16, 32, 48, 64)]
+    sp, rp = H.solver_params()
+    quad = [s for s, t in enumerate(amrmg.AMRMultiGrid.RELAX_SHAPES) if t[3] == 2]
+
+    def run(shape, kc):
+        b = amrmg.BatchSolver(gpu_ctx, [h for h, _ in probs], 1)
+        b.setSolverParameters(sp, rp)
+        for q, (_, rhs) in enumerate(probs):
+            b.set_rhs(q, rhs)
+        nd = len(b.tune_relax())
+        for d in range(nd):
+            b.set_relax_shape(d, 15, 0)
+            if shape is not None:
+                try:
+                    b.set_relax_shape(d, shape, kc)
+                except amrmg.AMRMGError:
+                    pass   # a depth with an x extent that is not a multiple of 4
+        it, ex = b.solve()
+        out = [(b.history(q), int(it[q]), int(ex[q]), b.get_phi(q)) for q in range(len(probs))]
+        b.close()
+        return out
+
+    ref = run(None, 0)
+    for shape in quad:
+        for kc in (0, 16):
+            print(f"batch quad shape {shape} kc {kc}", flush=True)
+            got = run(shape, kc)
+            for (h1, i1, e1, p1), (h2, i2, e2, p2) in zip(ref, got):
+                assert (i1, e1) == (i2, e2), shape
+                np.testing.assert_array_equal(h1, h2)
+                np.testing.assert_array_equal(p1, p2)
+    b = amrmg.BatchSolver(gpu_ctx, [h for h, _ in probs], 0)
+    with pytest.raises(amrmg.AMRMGError, match="x-quad"):
+        b.set_relax_shape(0, quad[0], 0)
+    b.close()
