@@ -249,7 +249,8 @@ class AMRMultiGrid:
     def define(self, ops, domainCells, bottomSolver):
         """ops: the AMR level operators; domainCells: numPts of each level's domain.
         Every level gets a MultiGrid; the one at l_base keeps its full depth, the
-        others a single level (refinement ratio 2: no intermediate depths)."""
+        others a single level (no intermediate depths: ratio 4 makes that level's
+        relaxation the MultiGrid bottom branch, see relax)."""
         self.m_op = list(ops)
         self.m_bottomSolver = bottomSolver
         self.m_domainCells = list(domainCells)
@@ -285,7 +286,7 @@ class AMRMultiGrid:
                 mg.init(phi[i], rhs[i])
                 self.m_mg[i] = mg
             else:
-                self.m_resC[i] = op.createCoarsened(rhs[i], 2)
+                self.m_resC[i] = op.createCoarsened(rhs[i], op.refToCoarser())
 
     def revert(self, l_max, l_base):
         for i in range(l_base, l_max + 1):
@@ -392,8 +393,24 @@ class AMRMultiGrid:
         return rnorm
 
     def relax(self, correction, residual, depth, nRelax):
-        """AMRMultiGrid::relax (AMRMultiGrid.H:634-666), refinement ratio 2."""
-        self.m_op[depth].relax(correction, residual, nRelax)
+        """AMRMultiGrid::relax (AMRMultiGrid.H:634-666).  With ratio 4 to the coarser
+        level, the level's MultiGrid cycles with m_depth 1 (init's intermediateDepth),
+        which is its bottom branch (MultiGrid.H:634-646): one relaxation when its
+        m_bottomCells is 1, else m_bottom, and the no-op bottom solver."""
+        op = self.m_op[depth]
+        if op.refToCoarser() > 2:
+            nRelax = 1 if self._bottom_cells(depth) == 1 else self.m_bottom
+        op.relax(correction, residual, nRelax)
+
+    def _bottom_cells(self, depth):
+        """MultiGrid::define's m_bottomCells for the level's MultiGrid: its domain over
+        the multigrid levels MGnewOp would create (coarsenable by coarsening*2)."""
+        cells, c = self.m_domainCells[depth], 2
+        boxes = self.m_op[depth].level.boxes
+        while c < (1 << 24) and is_coarsenable(boxes, c * 2):
+            cells //= 8
+            c *= 2
+        return cells
 
     def AMRVCycle(self, uberCorrection, uberResidual, ilev, l_max, l_base):
         """AMRMultiGrid::AMRVCycle (AMRMultiGrid.H:1342-1419)."""

@@ -5,6 +5,7 @@
 // whole-solve entry points run, so a host operator class built on these calls gets
 // bitwise the same arithmetic.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <stdexcept>
 #include <vector>
@@ -264,20 +265,27 @@ int amrmg_op_create(amrmg_level* L, amrmg_op* coarser, double dx, double dxCrse,
   h->op.reset(new AMRPoissonOp<double>());
   Context* c = L->ctx->c.get();
   const DeviceLayout* mgc = nullptr;
+  int ratio = 2;
   if (coarser) {
     if (coarser->finer) throw std::runtime_error("the coarser operator already has a finer level");
     if (coarser->ctx != L->ctx) throw std::runtime_error("operators of different contexts");
-    if (!L->lay->dbl.coarsenable(2)) throw std::runtime_error("a level with a coarser AMR level must be coarsenable by 2");
     if (!(dxCrse > 0)) throw std::runtime_error("a level with a coarser AMR level needs dxCrse > 0");
+    // the refinement ratio to the coarser level, dxCrse / dx: 2 or 4
+    ratio = (int)std::lround(dxCrse / dx);
+    if ((ratio != 2 && ratio != 4) || std::fabs(dxCrse - ratio * dx) > 1e-12 * dxCrse)
+      throw std::runtime_error("dxCrse / dx must be the refinement ratio, 2 or 4");
+    if (!L->lay->dbl.coarsenable(ratio))
+      throw std::runtime_error("a level with a coarser AMR level must be coarsenable by the refinement ratio");
     h->cfLevel.reset(new amrmg_level());
     h->cfLevel->ctx = L->ctx;
     h->cfLevel->ownedByOp = true;
-    h->cfLevel->lay.reset(new DeviceLayout(c, L->lay->dbl.coarsened(2), coarsen(L->lay->domain, 2), L->lay->merged));
-    mgc = h->cfLevel->lay.get();
+    h->cfLevel->lay.reset(
+        new DeviceLayout(c, L->lay->dbl.coarsened(ratio), coarsen(L->lay->domain, ratio), L->lay->merged));
+    if (ratio == 2) mgc = h->cfLevel->lay.get();  // coarsen(level, 2) is also its multigrid child
   }
   defineLevelOp<double>(h->op.get(), c, L->lay.get(), L->lay->domain, dx, dxCrse, alpha, beta, bc, mgc);
   if (coarser) {
-    linkCoarser<double>(h->op.get(), coarser->op.get(), h->cfLevel->lay.get());
+    linkCoarser<double>(h->op.get(), coarser->op.get(), h->cfLevel->lay.get(), ratio);
     coarser->finer = h.get();
     h->coarser = coarser;
   }
@@ -293,9 +301,9 @@ int amrmg_op_create_coarser(amrmg_op* op, amrmg_op** out, amrmg_level** level_ou
   if (!L.dbl.coarsenable(2)) throw std::runtime_error("level not coarsenable by 2");
   std::unique_ptr<amrmg_op> h(new amrmg_op());
   h->ctx = op->ctx;
-  if (op->cfLevel) {
-    // an AMR level with a coarser one already holds coarsen(level, 2), which its
-    // restrictResidual targets
+  if (op->cfLevel && p.m_refToCoarser == 2) {
+    // an AMR level with a coarser one at ratio 2 already holds coarsen(level, 2),
+    // which its restrictResidual targets
     h->lev = op->cfLevel.get();
     h->lev->refs++;
   } else {

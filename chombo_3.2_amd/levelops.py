@@ -188,8 +188,9 @@ class AMRPoissonOp:
 
     bc: (types, values), 6 each: lo x, y, z then hi x, y, z; type 0 Neumann,
     1 Dirichlet (linear rule), 2 Dirichlet (quadratic rule).  coarser: the operator of
-    the next coarser AMR level (refinement ratio 2) or None; dxCrse > 0 turns on the
-    homogeneous coarse-fine ghosts of the level's own CFRegion."""
+    the next coarser AMR level or None; dxCrse > 0 turns on the homogeneous
+    coarse-fine ghosts of the level's own CFRegion, and with a coarser operator
+    dxCrse / dx is the refinement ratio (2 or 4)."""
 
     def __init__(self, level: Level, dx, dxCrse=-1.0, alpha=0.0, beta=1.0, bc=((1,) * 6, (0.0,) * 6),
                  coarser: "AMRPoissonOp" = None, _handle=None):
@@ -226,11 +227,13 @@ class AMRPoissonOp:
         return op
 
     def coarsenedLevel(self):
-        """coarsen(level, 2) of an operator with a coarser AMR level (createCoarsened)."""
+        """coarsen(level, refToCoarser) of an operator with a coarser AMR level
+        (createCoarsened)."""
         lp = C.c_void_p()
         _chk(_lib().amrmg_op_coarsened_level(self.p, C.byref(lp)))
-        cb = [(tuple(v // 2 for v in lo), tuple(v // 2 for v in hi)) for (lo, hi) in self.level.boxes]
-        dom = (tuple(v // 2 for v in self.level.domain[0]), tuple(v // 2 for v in self.level.domain[1]))
+        r = self.refToCoarser()
+        cb = [(tuple(v // r for v in lo), tuple(v // r for v in hi)) for (lo, hi) in self.level.boxes]
+        dom = (tuple(v // r for v in self.level.domain[0]), tuple(v // r for v in self.level.domain[1]))
         return Level(self.level.ctx, cb, dom, _handle=lp, _owned=True)
 
     def setAlphaAndBeta(self, alpha, beta):
@@ -296,7 +299,10 @@ class AMRPoissonOp:
 
     # ---- AMRLevelOp ----
     def refToCoarser(self):
-        return 2
+        """dxCrse / dx for an AMR level with a coarser one (2 or 4), else 2."""
+        if self.coarser is None or self.m_dxCrse <= 0:
+            return 2
+        return int(round(self.m_dxCrse / self.m_dx))
 
     def AMRResidual(self, residual, phiFine, phi, phiCoarse, rhs, homogeneousBC, finerOp):
         _chk(_lib().amrmg_op_amr_residual(self.p, residual.p, _h(phiFine), phi.p, _h(phiCoarse), rhs.p,

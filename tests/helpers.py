@@ -106,3 +106,16 @@ def lshape_amr(alpha=0.0, beta=1.0):
     lev1 = [((16, 16, 16), (31, 31, 47)), ((16, 32, 16), (31, 47, 47)), ((32, 16, 16), (47, 31, 47))]
     lev2 = [((40, 40, 40), (55, 55, 63)), ((40, 40, 64), (55, 55, 87))]
     return H.Hierarchy(dom, [H.domain_split(dom, 16, 8), lev1, lev2], 1.0 / 32, alpha, beta)
+
+
+def mixed_ratio_amr(ratios, bc=(1, 1, 1), alpha=0.0, beta=1.0):
+    """Three levels over [-8, 7]^3 (dx 1/16) with refinement ratios (r1, r2): level 1
+    over coarse [-4, 7] x [-4, 3]^2 and level 2 over the last 4 coarse cells of level 1
+    in x, both reaching the high x face."""
+    r1, r2 = ratios
+    dom = H.box((-8,) * 3, (7,) * 3)
+    lev1 = H.box_refine(H.box((-4, -4, -4), (7, 3, 3)), r1)
+    d1 = H.box_refine(dom, r1)
+    lev2 = H.box_refine(H.box((d1[1][0] - 4 * r1 + 1, -2 * r1, -2 * r1), (d1[1][0], 2 * r1 - 1, 2 * r1 - 1)), r2)
+    return H.Hierarchy(dom, [H.domain_split(dom, 8, 4), H.domain_split(lev1, 16, 4), H.domain_split(lev2, 16, 4)],
+                       1.0 / 16, alpha, beta, bc_type_lo=bc, bc_type_hi=bc, ratios=tuple(ratios))

@@ -13,7 +13,7 @@ import amrmg
 import amrmultigrid as AMG
 import hierarchy as H
 import levelops as LO
-from helpers import ghosted_random, lshape_amr, small_amr, small_single
+from helpers import ghosted_random, lshape_amr, mixed_ratio_amr, small_amr, small_single
 from oracle import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -75,7 +75,7 @@ def handle_solve(ctx, h, rhs, sp, rp, merge=True, l_base=0, phi0=None, bottom=No
     return res
 
 
-@pytest.mark.parametrize("case", ["single32", "helm24", "amr32", "lshape", "amr32_w"])
+@pytest.mark.parametrize("case", ["single32", "helm24", "amr32", "lshape", "amr32_w", "ratio42", "ratio24_helm"])
 def test_handle_driven_solve_equals_library_solve(gpu_ctx, case):
     """The reference's AMRMultiGrid loop over the device operators equals the library's
     fused solve bit for bit (the fusions, buffer swaps and the device bottom solver
@@ -91,6 +91,10 @@ def test_handle_driven_solve_equals_library_solve(gpu_ctx, case):
     elif case == "lshape":
         h = lshape_amr()
         rhs = H.bump_rhs(h, 1002)
+    elif case.startswith("ratio"):   # refinement ratios 4 then 2, or 2 then 4
+        h = mixed_ratio_amr((4, 2)) if case == "ratio42" else mixed_ratio_amr((2, 4), (0, 1, 2), 0.5, 1.2)
+        rhs = H.bump_rhs(h, 1061)
+        kw = dict(iterMax=60)
     else:
         h = small_amr()
         rhs = H.bump_rhs(h, 1002)
