@@ -1846,6 +1846,11 @@ struct OHier {
   Layout* mgL;        /* mgL[0] = grids[0] */
   Op* mgop;           /* mgop[0] aliases op[0] */
   long bottomCells;
+  /* MultiGrid of AMR level l > 0 (m_mg[l] at its full depth, for solves with
+     l_base = l), made on first use: depth count, layouts, ops (index 0 unused) */
+  int* lmgD;
+  Layout** lmgL;
+  Op** lmgop;
   /* AMRMultiGrid copiers */
   Layout* cfL;        /* coarsened fine layouts (resC) per level >= 1 */
   Plan* resCopier;    /* resC[l] -> grids[l-1] */
@@ -1871,6 +1876,7 @@ OHier* oracle_create_r(const int* is, const double* rs, const int* ratios) {
     if (h->ratio[l] != 2 && h->ratio[l] != 4) { set_err("refinement ratio must be 2 or 4"); return NULL; }
   }
   h->grids = calloc(nlev, sizeof(Layout));
+  h->lmgD = calloc(nlev, sizeof(int)); h->lmgL = calloc(nlev, sizeof(Layout*)); h->lmgop = calloc(nlev, sizeof(Op*));
   h->domains = calloc(nlev, sizeof(Box));
   h->dx = calloc(nlev, sizeof(double));
   h->domains[0] = mkbox(is[1], is[2], is[3], is[4], is[5], is[6]);
@@ -1961,8 +1967,20 @@ OHier* oracle_create_r(const int* is, const double* rs, const int* ratios) {
   return h;
 }
 
+/* drops the MultiGrids of the levels above the base (remade on next use with the
+   operators' current coefficients, BC and relaxation mode) */
+static void mg_levels_free(OHier* h) {
+  for (int l = 1; l < h->nlev; l++) {
+    if (!h->lmgD[l]) continue;
+    for (int d = 1; d < h->lmgD[l]; d++) { op_free(&h->lmgop[l][d]); free(h->lmgL[l][d].b); }
+    free(h->lmgL[l]); free(h->lmgop[l]);
+    h->lmgD[l] = 0; h->lmgL[l] = NULL; h->lmgop[l] = NULL;
+  }
+}
 void oracle_destroy(OHier* h) {
   if (!h) return;
+  mg_levels_free(h);
+  free(h->lmgD); free(h->lmgL); free(h->lmgop);
   for (int l = 0; l < h->nlev; l++) op_free(&h->op[l]);
   for (int d = 1; d < h->mgD; d++) { op_free(&h->mgop[d]); free(h->mgL[d].b); }
   for (int l = 1; l < h->nlev; l++) { plan_free(&h->resCopier[l]); plan_free(&h->revCopier[l]); free(h->cfL[l].b); }
@@ -1977,6 +1995,7 @@ OHier* oracle_create(const int* is, const double* rs) { return oracle_create_r(i
    arithmetic), each op with its lambda. */
 static const Op* mg_op(const OHier* h, int d);
 int oracle_vc_set(OHier* h, const double* a, const double* b0, const double* b1, const double* b2) {
+  mg_levels_free(h);
   const double* bs[3] = {b0, b1, b2};
   long ao = 0, bo[3] = {0, 0, 0};
   if (h->op[0].relaxMode != 1 && h->op[0].relaxMode != 4) {
@@ -2017,6 +2036,7 @@ int oracle_vc_set(OHier* h, const double* a, const double* b0, const double* b1,
    0 CoarseAverage::arithmetic, 1 harmonic; used by the next oracle_vc_set */
 /* DiriBC / NeumBC with a BCValueHolder on every operator (fn NULL: the face constants) */
 int oracle_set_bc_values(OHier* h, BCValueFn fn, void* user) {
+  mg_levels_free(h);
   h->bc.fn = fn; h->bc.user = user;
   for (int l = 0; l < h->nlev; l++) { h->op[l].bc.fn = fn; h->op[l].bc.user = user; }
   for (int d = 1; d < h->mgD; d++) { h->mgop[d].bc.fn = fn; h->mgop[d].bc.user = user; }
@@ -2029,6 +2049,7 @@ int oracle_set_vc_average(OHier* h, int type) {
 }
 /* AMRPoissonOp::s_relaxMode (AMRPoissonOp.cpp:31, 600-632) for every operator */
 int oracle_set_relax_mode(OHier* h, int mode) {
+  mg_levels_free(h);
   if (mode < 0 || mode > 5) {
     set_err("relax mode must be 0 (looseGSRB), 1 (levelGSRB), 2 (overlapGSRB), 3 (levelGSRBLazy), "
             "4 (levelJacobi) or 5 (levelMultiColor)");
@@ -2047,6 +2068,7 @@ int oracle_set_relax_mode(OHier* h, int mode) {
 /* setAlphaAndBeta (AMRPoissonOp.cpp:1103-1108, VCAMRPoissonOp2.cpp:258-266) on every
    operator of the hierarchy (AMRMultiGrid::getAllOperators), lambda recomputed */
 int oracle_set_alpha_beta(OHier* h, double alpha, double beta) {
+  mg_levels_free(h);
   h->alpha = alpha; h->beta = beta;
   for (int l = 0; l < h->nlev; l++) {
     h->op[l].alpha = alpha; h->op[l].beta = beta;
@@ -2171,15 +2193,22 @@ done:
 /* ---- MultiGrid<T>::cycle (MultiGrid.H:630-696), V/W cycle ---- */
 typedef struct {
   OHier* h; int pre, post, bottom, cycles;
+  int lev, D;                     /* the AMR level whose MultiGrid this is, its depth count */
+  const Layout* L;                /* layouts per depth (L[0] = the level's grids) */
+  Op* ops;                        /* ops per depth >= 1 (depth 0 is the AMR level's op) */
+  long bottomCells;
   LD* mgres; LD* mgcor;           /* per depth >= 1 */
   BiCG bic;
 } MGState;
 
+static const Op* mgs_op(const MGState* s, int depth) {
+  return depth == 0 ? &s->h->op[s->lev] : &s->ops[depth];
+}
 static void mg_cycle(MGState* s, int depth, LD* corr, const LD* res) {
   OHier* h = s->h;
-  const Op* op = mg_op(h, depth);
-  if (depth == h->mgD-1) {
-    if (h->bottomCells == 1) op_relax(op, corr, res, 1);
+  const Op* op = mgs_op(s, depth);
+  if (depth == s->D-1) {
+    if (s->bottomCells == 1) op_relax(op, corr, res, 1);
     else {
       op_relax(op, corr, res, s->bottom);
       bicgstab_solve(op, corr, res, &s->bic, NULL);
@@ -2235,8 +2264,12 @@ static void amr_residual_level(AMRState* a, LD* res, LD* phi, const LD* rhs, int
       op_reflux(op, &h->op[ilev+1], &phi[ilev+1], &phi[ilev], &res[ilev]);
       ldo_axby(&res[ilev], &res[ilev], &rhs[ilev], -1.0, 1.0);
     }
-  } else {
+  } else if (lbase == 0) {
     op_residual(&h->op[0], &res[0], &phi[0], &rhs[0], homog);
+  } else {                                  /* one level above the base: AMRResidualNF */
+    Op* op = &h->op[lmax];
+    qcfi_interp(&op->qcfi, &phi[lmax], &phi[lmax-1]);
+    op_residualI(op, &res[lmax], &phi[lmax], &rhs[lmax], homog);
   }
 }
 /* zeroCovered -> LevelDataOps::copyToZero over the copier's toRegions */
@@ -2312,8 +2345,27 @@ static void amr_vcycle(AMRState* a, LD* uberCorr, LD* uberRes, int ilev, int lma
   ldo_assign_local(&uberCorr[ilev], &a->corr[ilev]);
 }
 
-static void amr_state_make(AMRState* a, OHier* h, int pre, int post, int bottom, int numMG) {
-  memset(a, 0, sizeof *a);
+/* MultiGrid::define on AMR level l > 0 via MGnewOp (AMRPoissonOp.cpp:1923-1989):
+   coarsenings of the level's grids while coarsenable by coarsening*2, each op with
+   the level's coarser dx as dxCrse (its coarse-fine faces keep homogeneous
+   interpolation), and m_bottomCells from the level's domain */
+static void mg_level_define(OHier* h, int l) {
+  if (h->lmgD[l]) return;
+  int D = 1;
+  while (lay_coarsenable(&h->grids[l], (1 << D)*2)) D++;
+  h->lmgD[l] = D;
+  h->lmgL[l] = calloc(D, sizeof(Layout));
+  h->lmgop[l] = calloc(D, sizeof(Op));
+  h->lmgL[l][0] = h->grids[l];
+  for (int d = 1; d < D; d++) {
+    h->lmgL[l][d] = lay_coarsen(&h->grids[l], 1 << d);
+    Box dom = b_coarsen(h->domains[l], 1 << d);
+    op_define(&h->lmgop[l][d], &h->lmgL[l][d], &dom, h->dx[l]*(1 << d), h->alpha, h->beta, &h->bc);
+    h->lmgop[l][d].dxCrse = h->dx[l-1];
+    h->lmgop[l][d].relaxMode = h->op[l].relaxMode;
+  }
+}
+static void amr_state_make(AMRState* a, OHier* h, int pre, int post, int bottom, int numMG, int lbase) {
   a->h = h;
   a->corr = calloc(h->nlev, sizeof(LD)); a->resid = calloc(h->nlev, sizeof(LD)); a->resC = calloc(h->nlev, sizeof(LD));
   for (int l = 0; l < h->nlev; l++) {
@@ -2323,15 +2375,23 @@ static void amr_state_make(AMRState* a, OHier* h, int pre, int post, int bottom,
   }
   MGState* m = &a->mg;
   m->h = h; m->pre = pre; m->post = post; m->bottom = bottom; m->cycles = numMG;
-  m->mgres = calloc(h->mgD, sizeof(LD)); m->mgcor = calloc(h->mgD, sizeof(LD));
-  for (int d = 1; d < h->mgD; d++) { m->mgres[d] = ld_make(&h->mgL[d], 0); m->mgcor[d] = ld_make(&h->mgL[d], 1); }
+  m->lev = lbase;
+  if (lbase == 0) {
+    m->D = h->mgD; m->L = h->mgL; m->ops = h->mgop; m->bottomCells = h->bottomCells;
+  } else {
+    mg_level_define(h, lbase);
+    m->D = h->lmgD[lbase]; m->L = h->lmgL[lbase]; m->ops = h->lmgop[lbase];
+    m->bottomCells = h->mgBottomCells[lbase];
+  }
+  m->mgres = calloc(m->D, sizeof(LD)); m->mgcor = calloc(m->D, sizeof(LD));
+  for (int d = 1; d < m->D; d++) { m->mgres[d] = ld_make(&m->L[d], 0); m->mgcor[d] = ld_make(&m->L[d], 1); }
   m->bic.eps = 1e-6; m->bic.reps = h->bic_reps; m->bic.hang = h->bic_hang; m->bic.small = h->bic_small;
   m->bic.metric = -1.0; m->bic.imax = h->bic_imax; m->bic.numRestarts = h->bic_numRestarts; m->bic.normType = 2;
 }
 static void amr_state_free(AMRState* a) {
   OHier* h = a->h;
   for (int l = 0; l < h->nlev; l++) { ld_free(&a->corr[l]); ld_free(&a->resid[l]); if (l > 0) ld_free(&a->resC[l]); }
-  for (int d = 1; d < h->mgD; d++) { ld_free(&a->mg.mgres[d]); ld_free(&a->mg.mgcor[d]); }
+  for (int d = 1; d < a->mg.D; d++) { ld_free(&a->mg.mgres[d]); ld_free(&a->mg.mgcor[d]); }
   free(a->corr); free(a->resid); free(a->resC); free(a->mg.mgres); free(a->mg.mgcor);
 }
 
@@ -2348,20 +2408,24 @@ static void levels_to_packed(OHier* h, const LD* x, double* v) {
   long o = 0; for (int l = 0; l < h->nlev; l++) { ld_to_packed(&x[l], v + o); o += lay_cells(&h->grids[l]); }
 }
 
-/* AMRMultiGrid::solve -> solveNoInit -> solveNoInitResid (AMRMultiGrid.H:895-1124) */
-int oracle_solve(OHier* h, const int* sp, const double* rp, const double* rhs_in, double* phi_io,
-                 int zero_phi, double* hist, int hist_cap, int* info) {
+/* AMRMultiGrid::solve(phi, rhs, l_max, l_base) -> solveNoInit -> solveNoInitResid
+   (AMRMultiGrid.H:895-1124).  phi and rhs are valid-packed over all levels; only
+   levels l_base..l_max are solved for, phi of level l_base-1 gives the coarse-fine
+   boundary values of level l_base, and the other levels are returned unchanged. */
+int oracle_solve_levels(OHier* h, const int* sp, const double* rp, const double* rhs_in, double* phi_io,
+                        int zero_phi, int lmax, int lbase, double* hist, int hist_cap, int* info) {
   int pre = sp[0], post = sp[1], bottom = sp[2], numMG = sp[3], iterMax = sp[4], imin = sp[5];
   double eps = rp[0], hang = rp[1], normThresh = rp[2];
-  int lmax = h->nlev-1, lbase = 0;
-  AMRState a; amr_state_make(&a, h, pre, post, bottom, numMG);
+  if (lbase < 0 || lbase > lmax || lmax >= h->nlev) { set_err("bad l_base / l_max"); return -1; }
+  if (lbase > 0 && h->op[lbase].vc) { set_err("l_base > 0 with variable coefficients"); return -1; }
+  AMRState a; amr_state_make(&a, h, pre, post, bottom, numMG, lbase);
   LD* phi = levels_make(h, 1);
   LD* rhs = levels_make(h, 0);
   LD* uberRes = levels_make(h, 0);
   LD* uberCorr = levels_make(h, 1);
   levels_from_packed(h, rhs, rhs_in);
   levels_from_packed(h, phi, phi_io);
-  if (zero_phi) for (int l = 0; l <= lmax; l++) ldo_setzero(&phi[l]);
+  if (zero_phi) for (int l = lbase; l <= lmax; l++) ldo_setzero(&phi[l]);
   h->bottom_calls = 0;
   double initial = amr_residual(&a, uberRes, phi, rhs, lmax, lbase, 0);
   double rnorm = initial, norm_last = 2*initial;
@@ -2387,9 +2451,13 @@ int oracle_solve(OHier* h, const int* sp, const double* rp, const double* rhs_in
   amr_state_free(&a);
   return 0;
 }
+int oracle_solve(OHier* h, const int* sp, const double* rp, const double* rhs_in, double* phi_io,
+                 int zero_phi, double* hist, int hist_cap, int* info) {
+  return oracle_solve_levels(h, sp, rp, rhs_in, phi_io, zero_phi, h->nlev-1, 0, hist, hist_cap, info);
+}
 
 double oracle_amr_residual(OHier* h, const double* phi_in, const double* rhs_in, double* res_out, int homog) {
-  AMRState a; amr_state_make(&a, h, 2, 2, 2, 1);
+  AMRState a; amr_state_make(&a, h, 2, 2, 2, 1, 0);
   LD* phi = levels_make(h, 1); LD* rhs = levels_make(h, 0); LD* res = levels_make(h, 0);
   levels_from_packed(h, phi, phi_in); levels_from_packed(h, rhs, rhs_in);
   double n = amr_residual(&a, res, phi, rhs, h->nlev-1, 0, homog);

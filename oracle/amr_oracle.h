@@ -31,6 +31,10 @@ typedef struct OHier OHier;
 /* solver parameters: sparams = {pre, post, bottom, numMG, iterMax, imin},
    rparams = {eps, hang, normThresh} */
 OHier* oracle_create(const int* ispec, const double* rspec);
+/* AMRMultiGrid::solve(phi, rhs, l_max, l_base): levels l_base..l_max only, with
+   phi of level l_base-1 as coarse-fine boundary data (phi/rhs packed over all levels) */
+int oracle_solve_levels(OHier* h, const int* sparams, const double* rparams, const double* rhs,
+                        double* phi, int zero_phi, int l_max, int l_base, double* hist, int hist_cap, int* info);
 /* the same with refinement ratios (2 or 4) between consecutive levels; NULL: all 2 */
 OHier* oracle_create_r(const int* ispec, const double* rspec, const int* ratios);
 /* BCValueFunc with a caller pointer; fn NULL restores the constant face values */

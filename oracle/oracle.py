@@ -69,6 +69,8 @@ def lib():
         L.oracle_ghosted_cells.argtypes = [C.c_void_p, C.c_int]
         L.oracle_num_mg_depths.argtypes = [C.c_void_p]
         L.oracle_solve.argtypes = [C.c_void_p, _ip, _dp, _dp, _dp, C.c_int, _dp, C.c_int, _ip]
+        L.oracle_solve_levels.argtypes = [C.c_void_p, _ip, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int, _dp, C.c_int,
+                                          _ip]
         L.oracle_amr_residual.restype = C.c_double
         L.oracle_amr_residual.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_int]
         L.oracle_amr_operator.argtypes = [C.c_void_p, _dp, _dp, C.c_int]
@@ -143,7 +145,10 @@ class Oracle:
     def mg_cells(self, d: int) -> int:
         return lib().oracle_mg_cells(self.h, d)
 
-    def solve(self, rhs, sparams, rparams, phi0=None, hist_cap=128):
+    def solve(self, rhs, sparams, rparams, phi0=None, hist_cap=128, l_max=None, l_base=0, zero_phi=None):
+        """AMRMultiGrid::solve(phi, rhs, l_max, l_base, zeroPhi) on valid-packed fields of
+        all levels; levels outside l_base..l_max keep phi0 (level l_base-1 gives the
+        coarse-fine boundary values).  zero_phi defaults to phi0 is None."""
         rhs, prhs = _d(rhs)
         phi = np.zeros_like(rhs) if phi0 is None else np.array(phi0, dtype=np.float64)
         phi, pphi = _d(phi)
@@ -151,9 +156,11 @@ class Oracle:
         info = np.zeros(3, dtype=np.int32)
         sp = np.ascontiguousarray(sparams, dtype=np.int32)
         rp = np.ascontiguousarray(rparams, dtype=np.float64)
-        rc = lib().oracle_solve(self.h, sp.ctypes.data_as(_ip), rp.ctypes.data_as(_dp), prhs, pphi,
-                                1 if phi0 is None else 0, hist.ctypes.data_as(_dp), hist_cap,
-                                info.ctypes.data_as(_ip))
+        lmax = self.hier.nlev - 1 if l_max is None else int(l_max)
+        zp = (phi0 is None) if zero_phi is None else bool(zero_phi)
+        rc = lib().oracle_solve_levels(self.h, sp.ctypes.data_as(_ip), rp.ctypes.data_as(_dp), prhs, pphi,
+                                       1 if zp else 0, lmax, int(l_base), hist.ctypes.data_as(_dp), hist_cap,
+                                       info.ctypes.data_as(_ip))
         if rc != 0:
             raise RuntimeError(lib().oracle_last_error().decode())
         it = int(info[0])
