@@ -105,6 +105,7 @@ def lib():
         L.amrmg_solver_set_field.argtypes = [C.c_void_p, C.c_int, _dp]
         L.amrmg_solver_get_field.argtypes = [C.c_void_p, C.c_int, _dp]
         L.amrmg_solver_solve.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _ip]
+        L.amrmg_solver_solve_levels.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _dp, C.c_int, _ip]
         L.amrmg_solver_residual.argtypes = [C.c_void_p, C.c_int, _dp]
         L.amrmg_solver_operator.argtypes = [C.c_void_p, C.c_int]
         L.amrmg_level_apply.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.c_int]
@@ -338,11 +339,14 @@ class AMRMultiGrid:
         _chk(lib().amrmg_solver_get_field(self.p, field, out.ctypes.data_as(_dp)))
         return out
 
-    def solve(self, zero_phi=True, hist_cap=128):
+    def solve(self, zero_phi=True, hist_cap=128, l_max=None, l_base=0):
+        """AMRMultiGrid::solve on levels l_base..l_max (default all); with l_base > 0 the
+        phi of level l_base-1 as set gives the coarse-fine boundary values."""
         hist = np.zeros(hist_cap)
         info = np.zeros(3, dtype=np.int32)
-        _chk(lib().amrmg_solver_solve(self.p, int(zero_phi), hist.ctypes.data_as(_dp), hist_cap,
-                                       info.ctypes.data_as(_ip)))
+        lmax = self.hier.nlev - 1 if l_max is None else int(l_max)
+        _chk(lib().amrmg_solver_solve_levels(self.p, lmax, int(l_base), int(zero_phi), hist.ctypes.data_as(_dp),
+                                              hist_cap, info.ctypes.data_as(_ip)))
         it = int(info[0])
         return hist[:it + 1].copy(), dict(iterations=it, exit_status=int(info[1]), bottom_iters=int(info[2]))
 

@@ -555,10 +555,32 @@ void AMRPoissonOp<Real>::reflux(T& phiFine, T& phi, T& res, AMRPoissonOp* finerO
 // ---------------------------------------------------------------------------
 // DeviceHierarchy / factory
 // ---------------------------------------------------------------------------
+// MultiGrid::define on AMR level l via MGnewOp (AMRPoissonOp.cpp:1955-1980): the level's
+// boxes coarsened by 2, 4, ... while coarsenable by coarsening * s_maxCoarse (2)
+void DeviceHierarchy::defineLevelMG(int l) {
+  if (l < 1 || l >= nlev()) throw std::runtime_error("defineLevelMG: level out of range");
+  if (!lmgLay[l].empty() || !grids[l].coarsenable(4)) return;
+  for (int depth = 1;; depth++) {
+    const int coarsening = 1 << depth;
+    if (!grids[l].coarsenable(coarsening * 2)) break;
+    if (depth == 1 && ratios[l] == 2) {
+      lmgLay[l].push_back(cfLay[l].get());  // coarsen(level, 2) over coarsen(domain, 2): the same layout
+      continue;
+    }
+    lmgOwned[l].emplace_back(
+        new DeviceLayout(ctx, grids[l].coarsened(coarsening), coarsen(domains[l], coarsening), merged));
+    lmgLay[l].push_back(lmgOwned[l].back().get());
+  }
+}
+
 void DeviceHierarchy::define(Context* c, const std::vector<DBL>& g, const Box& domain0, double dx0, bool merge,
                              const BCSpec* bc, const int* ratio) {
   ctx = c;
   grids = g;
+  merged = merge;
+  lmgLay.assign(g.size(), {});
+  lmgOwned.clear();
+  lmgOwned.resize(g.size());
   domains.assign(g.size(), domain0);
   dxs.assign(g.size(), dx0);
   ratios.assign(g.size(), 1);
@@ -825,6 +847,22 @@ std::unique_ptr<AMRPoissonOp<Real>> AMRPoissonOpFactory<Real>::AMRnewOp(int l, A
 }
 
 template <class Real>
+std::vector<std::unique_ptr<MGLevelOp<LevelData<Real>>>> AMRPoissonOpFactory<Real>::MGnewOpsLevel(int l) {
+  std::vector<std::unique_ptr<MGLevelOp<LevelData<Real>>>> out;
+  H->defineLevelMG(l);
+  const auto& lay = H->lmgLay[l];
+  for (size_t d = 0; d < lay.size(); d++) {
+    const int coarsening = 1 << (d + 1);
+    AMRPoissonOp<Real>* op = new AMRPoissonOp<Real>();
+    // MGnewOp keeps the AMR level's dxCrse: the coarse-fine faces of every depth get
+    // the homogeneous interpolation with the coarser AMR level's spacing
+    defineLevelOp(op, H->ctx, lay[d], coarsen(H->domains[l], coarsening), H->dxs[l] * coarsening, H->dxs[l - 1],
+                  alpha, beta, bc, (d + 1 < lay.size()) ? lay[d + 1] : nullptr);
+    out.emplace_back(op);
+  }
+  return out;
+}
+template <class Real>
 std::vector<std::unique_ptr<MGLevelOp<LevelData<Real>>>> AMRPoissonOpFactory<Real>::MGnewOps() {
   std::vector<std::unique_ptr<MGLevelOp<LevelData<Real>>>> out;
   for (size_t d = 0; d < H->mgLay.size(); d++) {
@@ -845,6 +883,9 @@ void AMRMultiGrid<Real>::define(AMRPoissonOpFactory<Real>& factory, LinearSolver
   m_op.clear();
   for (int l = 0; l < numLevels; l++) m_op.push_back(factory.AMRnewOp(l, l > 0 ? m_op[l - 1].get() : nullptr));
   m_bottomSolver = bottom;
+  m_factory = &factory;
+  m_mgL.clear();
+  m_mgL.resize(numLevels);
   m_mgBottomCells = factory.H->mgBottomCells;
   m_mg0.reset(new MultiGrid<T>());
   m_mg0->define(m_op[0].get(), factory.MGnewOps(), factory.H->domains[0].npts(), bottom);
@@ -863,6 +904,11 @@ void AMRMultiGrid<Real>::setSolverParameters(int pre, int post, int bottom, int 
   m_iterMax = iterMax; m_numMG = numMG;
   m_mg0->m_pre = pre; m_mg0->m_post = post; m_mg0->m_bottom = bottom;
   m_mg0->m_numMG = numMG; m_mg0->m_cycle = numMG;
+  for (auto& m : m_mgL)
+    if (m) {
+      m->m_pre = pre; m->m_post = post; m->m_bottom = bottom;
+      m->m_numMG = numMG; m->m_cycle = numMG;
+    }
   m_bottomSolverEpsCushion = 1.0;
 }
 // init (AMRMultiGrid.H:1212-1249)
@@ -880,6 +926,27 @@ void AMRMultiGrid<Real>::init(std::vector<T*>& phi, std::vector<T*>& rhs, int l_
       op.createCoarsened(*m_resC[i], *rhs[i], 2);
     }
   }
+}
+template <class Real>
+MultiGrid<LevelData<Real>>* AMRMultiGrid<Real>::mg(int l) {
+  if (l == 0) return m_mg0.get();
+  if (l < 0 || l >= (int)m_op.size()) throw std::runtime_error("l_base out of range");
+  if (!m_mgL[l]) {
+    if (m_op[l]->m_vc) throw std::runtime_error("l_base > 0 is not supported with variable coefficients");
+    if (!m_correction[l] || !m_residual[l]) throw std::runtime_error("AMRMultiGrid: init first");
+    AMRPoissonOp<Real>* top = m_op[l].get();
+    auto ops = m_factory->MGnewOpsLevel(l);
+    // the AMR level's multigrid child: coarsen(level, 2) (its coarsened-fine layout
+    // serves at ratio 2; at ratio 4 the level has none of its own)
+    if (!ops.empty() && !top->m_mgCoarse) top->m_mgCoarse = static_cast<AMRPoissonOp<Real>*>(ops[0].get())->m_lay;
+    std::unique_ptr<MultiGrid<T>> m(new MultiGrid<T>());
+    m->define(top, std::move(ops), m_factory->H->domains[l].npts(), nullptr);
+    m->m_pre = m_pre; m->m_post = m_post; m->m_bottom = m_bottom;
+    m->m_numMG = m_numMG; m->m_cycle = m_numMG;
+    m->init(*m_correction[l], *m_residual[l]);
+    m_mgL[l] = std::move(m);
+  }
+  return m_mgL[l].get();
 }
 template <class Real>
 void AMRMultiGrid<Real>::relax(T& phi, T& R, int depth, int nRelax, bool fromZero) {
@@ -975,11 +1042,11 @@ void AMRMultiGrid<Real>::AMRVCycle(std::vector<T*>& uberCorrection, std::vector<
   // of numMG recursive calls sees it zero)
   if (l_max == l_base) {
     // uberCorrection is zero on entry: zeroed before the first iteration and after each update
-    if (fromZero) m_mg0->oneCycleFromZero(*uberCorrection[ilev], *uberResidual[ilev]);
-    else m_mg0->oneCycle(*uberCorrection[ilev], *uberResidual[ilev]);
+    if (fromZero) mg(l_base)->oneCycleFromZero(*uberCorrection[ilev], *uberResidual[ilev]);
+    else mg(l_base)->oneCycle(*uberCorrection[ilev], *uberResidual[ilev]);
   } else if (ilev == l_base) {
-    if (fromZero) m_mg0->oneCycleFromZero(*m_correction[ilev], *m_residual[ilev]);
-    else m_mg0->oneCycle(*m_correction[ilev], *m_residual[ilev]);
+    if (fromZero) mg(l_base)->oneCycleFromZero(*m_correction[ilev], *m_residual[ilev]);
+    else mg(l_base)->oneCycle(*m_correction[ilev], *m_residual[ilev]);
     // uberCorrection[l_base] is zero here and read next only by the update of phi
     if (!m_foldPhi.empty()) m_op[ilev]->incrPhiFromZero(*m_foldPhi[ilev], *m_correction[ilev]);
     else m_op[ilev]->incr(*uberCorrection[ilev], *m_correction[ilev], 1.0);
@@ -1015,7 +1082,11 @@ template <class Real>
 void AMRMultiGrid<Real>::solveNoInitResid(std::vector<T*>& phi, std::vector<T*>& uberResidual,
                                           std::vector<T*>& uberCorrection, std::vector<T*>& rhs, int l_max,
                                           int l_base, bool zeroPhi) {
-  m_mg0->setBottomSolver(m_bottomSolver);
+  if (l_base < 0 || l_base > l_max || l_max >= (int)m_op.size()) throw std::runtime_error("bad l_base / l_max");
+  mg(l_base)->setBottomSolver(m_bottomSolver);
+  // with l_base > 0 the residual of level l_base takes coarse-fine data from the
+  // uberCorrection one level lower, which stays zero (AMRMultiGrid.H:952-967)
+  if (l_base > 0) m_op[l_base - 1]->setToZero(*uberCorrection[l_base - 1]);
   for (int ilev = l_base; ilev <= l_max; ilev++) {
     m_op[ilev]->setToZero(*uberResidual[ilev]);
     m_op[ilev]->setToZero(*uberCorrection[ilev]);

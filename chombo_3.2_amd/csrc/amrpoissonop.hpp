@@ -163,6 +163,12 @@ struct DeviceHierarchy {
   std::vector<std::unique_ptr<DeviceLayout>> amrLay;  // per AMR level
   std::vector<std::unique_ptr<DeviceLayout>> cfLay;   // coarsen(grids[l], ratios[l]); index l >= 1
   std::vector<std::unique_ptr<DeviceLayout>> mgLay;   // MG levels of AMR level 0, depth >= 1
+  // MG levels of AMR level l >= 1 (depth >= 1), made by defineLevelMG for solves with
+  // l_base = l: depth 1 is cfLay[l] at ratio 2, the others are owned here
+  std::vector<std::vector<const DeviceLayout*>> lmgLay;
+  std::vector<std::vector<std::unique_ptr<DeviceLayout>>> lmgOwned;
+  bool merged = true;
+  void defineLevelMG(int l);
   // merge: store each level whose boxes tile a rectangle as one array; ratio: the
   // nlev-1 refinement ratios between consecutive levels (null: 2 everywhere)
   void define(Context* c, const std::vector<DBL>& g, const Box& domain0, double dx0, bool merge = true,
@@ -185,6 +191,8 @@ class AMRPoissonOpFactory {
   // operator for AMR level l; the operator of level l-1 must be passed for l > 0
   std::unique_ptr<AMRPoissonOp<Real>> AMRnewOp(int l, AMRPoissonOp<Real>* coarserOp);
   std::vector<std::unique_ptr<MGLevelOp<LevelData<Real>>>> MGnewOps();
+  // MGnewOp at every depth >= 1 of AMR level l >= 1 (coarser dx of the AMR level as dxCrse)
+  std::vector<std::unique_ptr<MGLevelOp<LevelData<Real>>>> MGnewOpsLevel(int l);
 };
 
 // ---------------------------------------------------------------------------
@@ -201,6 +209,10 @@ class AMRMultiGrid {
   double m_bottomSolverEpsCushion = 1.0;
   std::vector<std::unique_ptr<AMRPoissonOp<Real>>> m_op;
   std::unique_ptr<MultiGrid<T>> m_mg0;                   // multigrid of level 0
+  std::vector<std::unique_ptr<MultiGrid<T>>> m_mgL;      // multigrid of level l >= 1, made on first use
+  AMRPoissonOpFactory<Real>* m_factory = nullptr;
+  // the MultiGrid of AMR level l (m_mg[l] at its full depth), bound to the bottom solver
+  MultiGrid<T>* mg(int l);
   std::vector<std::unique_ptr<T>> m_correction, m_residual, m_resC;
   std::vector<long long> m_mgBottomCells;  // per level: its MultiGrid's m_bottomCells (ratio-4 relax)
   LinearSolver<T>* m_bottomSolver = nullptr;

@@ -411,10 +411,17 @@ int amrmg_solver_get_field(amrmg_solver* s, int field, double* h) {
 }
 
 int amrmg_solver_solve(amrmg_solver* s, int zero_phi, double* hist, int hist_cap, int* info) {
+  return amrmg_solver_solve_levels(s, s->nlev() - 1, 0, zero_phi, hist, hist_cap, info);
+}
+int amrmg_solver_solve_levels(amrmg_solver* s, int l_max, int l_base, int zero_phi, double* hist, int hist_cap,
+                              int* info) {
   API_BEGIN
+  if (l_base < 0 || l_base > l_max || l_max >= s->nlev()) throw std::runtime_error("need 0 <= l_base <= l_max < nlev");
+  if (l_base > 0 && s->mixed) throw std::runtime_error("l_base > 0 is not supported in mixed precision");
+  if (l_base > 0 && !s->vc.empty()) throw std::runtime_error("l_base > 0 is not supported with variable coefficients");
   auto ph = s->ptrs(s->phi), rh = s->ptrs(s->rhs), ur = s->ptrs(s->uberRes), uc = s->ptrs(s->uberCorr);
   long long b0 = s->mixed ? s->mixed->bottomIterations() : s->bottom->totalIterations();
-  s->amg.solveNoInitResid(ph, ur, uc, rh, s->nlev() - 1, 0, zero_phi != 0);
+  s->amg.solveNoInitResid(ph, ur, uc, rh, l_max, l_base, zero_phi != 0);
   s->ctx->c->sync();
   if (hist)
     for (int q = 0; q < hist_cap && q < (int)s->amg.m_history.size(); q++) hist[q] = s->amg.m_history[q];
@@ -1370,6 +1377,11 @@ int amrmg_plan_check_ratios(const int* ispec, const double* rspec, const int* re
   }
   for (auto& l : H.mgLay) st[6] += l->ntiles;
   for (int l = 0; l < H.nlev(); l++) st[7] += H.amrLay[l]->valid_cells;
+  // the multigrid operators of the levels above the base (solves with l_base > 0)
+  for (int l = 1; l < H.nlev(); l++) {
+    auto ops = fac.MGnewOpsLevel(l);
+    for (auto& o : ops) st[2] += (long long)static_cast<AMRPoissonOp<double>*>(o.get())->m_ghost.h_patches.size();
+  }
   for (int q = 0; q < nstats && q < 8; q++) stats[q] = st[q];
   API_END
 }

@@ -121,6 +121,14 @@ int amrmg_solver_get_field(amrmg_solver* s, int field, double* valid_packed);
    fields.  hist[0..] = max-norm residual history (up to hist_cap entries),
    info = {iterations, exitStatus, bottom BiCGStab iterations}. */
 int amrmg_solver_solve(amrmg_solver* s, int zero_phi, double* hist, int hist_cap, int* info);
+/* AMRMultiGrid::solve(phi, rhs, l_max, l_base, zeroPhi) (AMRMultiGrid.H:895-1124): only
+   levels l_base..l_max are solved for; the phi of level l_base-1 (as set) gives the
+   coarse-fine boundary values of level l_base, and level l_base cycles with its own
+   MultiGrid (MGnewOp on that AMR level, made on first use) down to the BiCGStab
+   bottom.  Fields of other levels are left as they are.  FP64 with constant
+   coefficients; amrmg_solver_solve is (nlev-1, 0). */
+int amrmg_solver_solve_levels(amrmg_solver* s, int l_max, int l_base, int zero_phi, double* hist, int hist_cap,
+                              int* info);
 
 /* AMRMultiGrid::computeAMRResidual(resid, phi, rhs) -> field 2; norm out */
 int amrmg_solver_residual(amrmg_solver* s, int homogeneous, double* norm_out);
