@@ -248,35 +248,49 @@ static int lay_coarsenable(const Layout* L, int r) {
 }
 static long lay_cells(const Layout* L) { long n = 0; for (int q = 0; q < L->n; q++) n += b_npts(&L->b[q]); return n; }
 
-/* LevelData<FArrayBox> with ghost width g */
-typedef struct { const Layout* L; int g; Fab* f; } LD;
-static LD ld_make(const Layout* L, int g) {
-  LD x; x.L = L; x.g = g; x.f = malloc(sizeof(Fab)*(L->n ? L->n : 1));
-  for (int q = 0; q < L->n; q++) fab_alloc(&x.f[q], b_grow(L->b[q], g));
+/* LevelData<FArrayBox> with ghost width g and nc components.  The FAB of box q,
+   component c is f[c*L->n + q], so component c alone is the one-component LevelData
+   COMP(x, c) over the same storage: every ChF kernel of the path loops
+   "do n = 0, ncomp-1" around cell updates that never mix components, so running
+   the one-component routine per component gives the same values. */
+typedef struct { const Layout* L; int g; Fab* f; int nc, comp; } LD;
+static LD ld_make_nc(const Layout* L, int g, int nc) {
+  LD x; x.L = L; x.g = g; x.nc = nc; x.comp = 0;
+  x.f = malloc(sizeof(Fab)*((long)L->n*nc > 0 ? (size_t)L->n*nc : 1));
+  for (int c = 0; c < nc; c++)
+    for (int q = 0; q < L->n; q++) fab_alloc(&x.f[(size_t)c*L->n + q], b_grow(L->b[q], g));
   return x;
 }
+static LD ld_make(const Layout* L, int g) { return ld_make_nc(L, g, 1); }
+#define COMP(x, c) (&(LD){ (x)->L, (x)->g, (x)->f + (size_t)(c)*(x)->L->n, 1, (c) })
 static void ld_free(LD* x) {
   if (!x->f) return;
-  for (int q = 0; q < x->L->n; q++) fab_free(&x->f[q]);
+  for (long q = 0; q < (long)x->L->n*x->nc; q++) fab_free(&x->f[q]);
   free(x->f); x->f = NULL;
 }
+/* valid-packed host order: box by box, in each box component by component (the
+   order of an FArrayBox's data), cells x fastest */
 static void ld_from_packed(LD* x, const double* v) {
   long o = 0;
-  for (int q = 0; q < x->L->n; q++) {
-    const Box* b = &x->L->b[q];
-    for (int k = b->lo[2]; k <= b->hi[2]; k++)
-      for (int j = b->lo[1]; j <= b->hi[1]; j++)
-        for (int i = b->lo[0]; i <= b->hi[0]; i++) *P(&x->f[q],i,j,k) = v[o++];
-  }
+  for (int q = 0; q < x->L->n; q++)
+    for (int c = 0; c < x->nc; c++) {
+      const Box* b = &x->L->b[q];
+      Fab* f = &x->f[(size_t)c*x->L->n + q];
+      for (int k = b->lo[2]; k <= b->hi[2]; k++)
+        for (int j = b->lo[1]; j <= b->hi[1]; j++)
+          for (int i = b->lo[0]; i <= b->hi[0]; i++) *P(f,i,j,k) = v[o++];
+    }
 }
 static void ld_to_packed(const LD* x, double* v) {
   long o = 0;
-  for (int q = 0; q < x->L->n; q++) {
-    const Box* b = &x->L->b[q];
-    for (int k = b->lo[2]; k <= b->hi[2]; k++)
-      for (int j = b->lo[1]; j <= b->hi[1]; j++)
-        for (int i = b->lo[0]; i <= b->hi[0]; i++) v[o++] = *P(&x->f[q],i,j,k);
-  }
+  for (int q = 0; q < x->L->n; q++)
+    for (int c = 0; c < x->nc; c++) {
+      const Box* b = &x->L->b[q];
+      const Fab* f = &x->f[(size_t)c*x->L->n + q];
+      for (int k = b->lo[2]; k <= b->hi[2]; k++)
+        for (int j = b->lo[1]; j <= b->hi[1]; j++)
+          for (int i = b->lo[0]; i <= b->hi[0]; i++) v[o++] = *P(f,i,j,k);
+    }
 }
 static void ld_from_ghosted(LD* x, const double* v) {
   long o = 0;
@@ -509,8 +523,10 @@ typedef struct { int type[2][3]; double val[2][3]; BCValueFn fn; void* user; } B
 
 /* The BCValueHolder call of DiriBC / NeumBC (BCFunc.cpp:242-250, 309-313) for ghost
    cell (i,j,k) on face (idir, side): at getDomainFacePosition (BCFunc.cpp:187-202) of
-   the valid cell next to it, dx*(iv+0.5) moved half a cell outward in idir. */
-static double face_value(const BCSpec* bc, int i, int j, int k, int idir, int side, double dx) {
+   the valid cell next to it, dx*(iv+0.5) moved half a cell outward in idir.  The
+   function fills one value per component (a_values[icomp]); comp picks one. */
+#define ORACLE_MAX_NCOMP 16
+static double face_value(const BCSpec* bc, int i, int j, int k, int idir, int side, double dx, int comp) {
   int isign = side ? 1 : -1;
   int iv[3] = {i, j, k};
   iv[idir] -= isign;
@@ -518,13 +534,13 @@ static double face_value(const BCSpec* bc, int i, int j, int k, int idir, int si
   for (int d = 0; d < 3; d++) pos[d] = dx*(iv[d] + 0.5);
   pos[idir] += 0.5*(double)isign*dx;
   int dir = idir, sd = side;
-  double v = 0.0;
-  bc->fn(pos, &dir, &sd, &v, bc->user);
-  return v;
+  double v[ORACLE_MAX_NCOMP] = {0.0};
+  bc->fn(pos, &dir, &sd, v, bc->user);
+  return v[comp];
 }
 
 static void apply_bc(Fab* st, const Box* valid, const Box* domain, double dx, int homog,
-                     const BCSpec* bc) {
+                     const BCSpec* bc, int comp) {
   for (int idir = 0; idir < 3; idir++)
     for (int side = 0; side < 2; side++) {
       int vend = side ? valid->hi[idir] : valid->lo[idir];
@@ -547,7 +563,7 @@ static void apply_bc(Fab* st, const Box* valid, const Box* domain, double dx, in
           for (int k = to.lo[2]; k <= to.hi[2]; k++)
             for (int j = to.lo[1]; j <= to.hi[1]; j++)
               for (int i = to.lo[0]; i <= to.hi[0]; i++) {
-                double v = bc->fn ? face_value(bc, i, j, k, idir, side, dx) : value;
+                double v = bc->fn ? face_value(bc, i, j, k, idir, side, dx, comp) : value;
                 *P(st,i,j,k) += (double)isign*dx*v;
               }
       } else if (type == 2) {    /* DiriBC order 2, quadraticInterp (BCFunc.cpp:34-40, 291-354) */
@@ -557,7 +573,7 @@ static void apply_bc(Fab* st, const Box* valid, const Box* domain, double dx, in
               int c[3] = {i,j,k}; c[idir] -= isign;
               int f[3] = {c[0],c[1],c[2]}; f[idir] -= isign;
               double inhomogVal = 0.0;
-              if (!homog) inhomogVal = bc->fn ? face_value(bc, i, j, k, idir, side, dx) : value;
+              if (!homog) inhomogVal = bc->fn ? face_value(bc, i, j, k, idir, side, dx, comp) : value;
               double nearVal = *P(st,c[0],c[1],c[2]);
               double farVal = *P(st,f[0],f[1],f[2]);
               *P(st,i,j,k) = (8.0/3.0)*inhomogVal + (1.0/3.0)*(farVal) - 2*(nearVal);
@@ -568,7 +584,7 @@ static void apply_bc(Fab* st, const Box* valid, const Box* domain, double dx, in
             for (int i = to.lo[0]; i <= to.hi[0]; i++) {
               int c[3] = {i,j,k}; c[idir] -= isign;
               double inhomogVal = 0.0;
-              if (!homog) inhomogVal = bc->fn ? face_value(bc, i, j, k, idir, side, dx) : value;
+              if (!homog) inhomogVal = bc->fn ? face_value(bc, i, j, k, idir, side, dx, comp) : value;
               double nearVal = *P(st,c[0],c[1],c[2]);
               *P(st,i,j,k) = 2*inhomogVal - nearVal;
             }
@@ -658,11 +674,13 @@ static void plan_reverse(Plan* p) {
   qsort(p->m, p->n, sizeof(Motion), motion_cmp);
 }
 static void exec_exchange(LD* x, const Plan* p) {
+  if (x->nc > 1) { for (int c = 0; c < x->nc; c++) exec_exchange(COMP(x, c), p); return; }
   for (int q = 0; q < p->n; q++)
     fab_copy_region_sh(&x->f[p->m[q].to], &x->f[p->m[q].from], p->m[q].region, p->m[q].sh);
 }
 /* LevelData::copyTo(..., copier) with plain copy */
 static void exec_copy(LD* dst, const LD* src, const Plan* p) {
+  if (dst->nc > 1) { for (int c = 0; c < dst->nc; c++) exec_copy(COMP(dst, c), COMP(src, c), p); return; }
   for (int q = 0; q < p->n; q++)
     fab_copy_region(&dst->f[p->m[q].to], &src->f[p->m[q].from], p->m[q].region);
 }
@@ -1109,6 +1127,7 @@ static void qcfi_face(const QCFI* q, Fab* phif, const Fab* phic, const QCFS* s, 
 
 /* QuadCFInterp::coarseFineInterp(LevelData&, const LevelData&) (QuadCFInterp.cpp:846-889) */
 static void qcfi_interp(QCFI* q, LD* phif, const LD* phic) {
+  if (phif->nc > 1) { for (int c = 0; c < phif->nc; c++) qcfi_interp(q, COMP(phif, c), COMP(phic, c)); return; }
   if (!q->level) return;
   exec_copy(&q->buf, phic, &q->copier);
   for (int idir = 0; idir < 3; idir++)
@@ -1213,10 +1232,12 @@ static void op_free(Op* op) {
 
 static void op_exchange(const Op* op, LD* phi) { exec_exchange(phi, &op->ex); }
 static void op_bc(const Op* op, LD* phi, int homog) {
-  for (int b = 0; b < op->L.n; b++) apply_bc(&phi->f[b], &op->L.b[b], &op->domain, op->dx, homog, &op->bc);
+  if (phi->nc > 1) { for (int c = 0; c < phi->nc; c++) op_bc(op, COMP(phi, c), homog); return; }
+  for (int b = 0; b < op->L.n; b++) apply_bc(&phi->f[b], &op->L.b[b], &op->domain, op->dx, homog, &op->bc, phi->comp);
 }
 /* homogeneousCFInterp (AMRPoissonOp.cpp:1598-1794) */
 static void op_homog_cf(const Op* op, LD* phi) {
+  if (phi->nc > 1) { for (int c = 0; c < phi->nc; c++) op_homog_cf(op, COMP(phi, c)); return; }
   for (int b = 0; b < op->L.n; b++)
     for (int idir = 0; idir < 3; idir++)
       for (int side = 0; side < 2; side++) {
@@ -1402,6 +1423,7 @@ static void op_vc_lambda(Op* op) {
 
 /* residualI (AMRPoissonOp.cpp:260-297; VCAMRPoissonOp2.cpp:30-87) */
 static void op_residualI(const Op* op, LD* lhs, LD* phi, const LD* rhs, int homog) {
+  if (phi->nc > 1) { for (int c = 0; c < phi->nc; c++) op_residualI(op, COMP(lhs, c), COMP(phi, c), COMP(rhs, c), homog); return; }
   op_exchange(op, phi);
   op_bc(op, phi, homog);
   #pragma omp parallel for schedule(static)
@@ -1414,11 +1436,13 @@ static void op_residualI(const Op* op, LD* lhs, LD* phi, const LD* rhs, int homo
   }
 }
 static void op_residual(const Op* op, LD* lhs, LD* phi, const LD* rhs, int homog) {
+  if (phi->nc > 1) { for (int c = 0; c < phi->nc; c++) op_residual(op, COMP(lhs, c), COMP(phi, c), COMP(rhs, c), homog); return; }
   if (homog) op_homog_cf(op, phi);
   op_residualI(op, lhs, phi, rhs, homog);
 }
 /* applyOpI (AMRPoissonOp.cpp:343-385) */
 static void op_applyI(const Op* op, LD* lhs, LD* phi, int homog) {
+  if (phi->nc > 1) { for (int c = 0; c < phi->nc; c++) op_applyI(op, COMP(lhs, c), COMP(phi, c), homog); return; }
   op_exchange(op, phi);
   op_bc(op, phi, homog);
   #pragma omp parallel for schedule(static)
@@ -1430,6 +1454,7 @@ static void op_applyI(const Op* op, LD* lhs, LD* phi, int homog) {
   }
 }
 static void op_apply(const Op* op, LD* lhs, LD* phi, int homog) {
+  if (phi->nc > 1) { for (int c = 0; c < phi->nc; c++) op_apply(op, COMP(lhs, c), COMP(phi, c), homog); return; }
   op_homog_cf(op, phi);
   op_applyI(op, lhs, phi, homog);
 }
@@ -1446,7 +1471,7 @@ static void op_levelGSRB(const Op* op, LD* phi, const LD* rhs) {
     op_exchange(op, phi);
     #pragma omp parallel for schedule(static)
     for (int b = 0; b < op->L.n; b++) {
-      apply_bc(&phi->f[b], &op->L.b[b], &op->domain, op->dx, 1, &op->bc);
+      apply_bc(&phi->f[b], &op->L.b[b], &op->domain, op->dx, 1, &op->bc, phi->comp);
       if (op->vc)  /* VCAMRPoissonOp2::levelGSRB (VCAMRPoissonOp2.cpp:547-626) */
         k_vcgsrb(&phi->f[b], &rhs->f[b], &op->va.f[b], VCB(op, b), &op->vlam.f[b], &op->L.b[b], op->dx,
                  op->alpha, op->beta, pass);
@@ -1496,7 +1521,7 @@ static void op_looseGSRB(const Op* op, LD* phi, const LD* rhs) {
   op_homog_cf(op, phi);
   op_exchange(op, phi);
   for (int b = 0; b < op->L.n; b++) {
-    apply_bc(&phi->f[b], &op->L.b[b], &op->domain, op->dx, 1, &op->bc);
+    apply_bc(&phi->f[b], &op->L.b[b], &op->domain, op->dx, 1, &op->bc, phi->comp);
     box_gsrb(op, phi, rhs, b, &op->L.b[b], 0, 0);
     box_gsrb(op, phi, rhs, b, &op->L.b[b], 0, 1);
   }
@@ -1514,7 +1539,7 @@ static void op_overlapGSRB(const Op* op, LD* phi, const LD* rhs) {
   op_exchange(op, phi);
   op_homog_cf(op, phi);
   for (int b = 0; b < op->L.n; b++) {
-    apply_bc(&phi->f[b], &op->L.b[b], &op->domain, op->dx, 1, &op->bc);
+    apply_bc(&phi->f[b], &op->L.b[b], &op->domain, op->dx, 1, &op->bc, phi->comp);
     Box r = op->L.b[b];
     for (int d = 0; d < 3; d++) { r.lo[d]++; r.hi[d]--; }
     box_gsrb(op, phi, rhs, b, &r, 1, 0);
@@ -1560,7 +1585,7 @@ static void op_levelMultiColor(const Op* op, LD* phi, const LD* rhs) {
     op_exchange(op, phi);
     for (int b = 0; b < op->L.n; b++) {
       Box cb;
-      apply_bc(&phi->f[b], &op->L.b[b], &op->domain, op->dx, 1, &op->bc);
+      apply_bc(&phi->f[b], &op->L.b[b], &op->domain, op->dx, 1, &op->bc, phi->comp);
       if (colored_box(&op->L.b[b], k_multi_colors[c], &cb))
         k_gsmc(&phi->f[b], &rhs->f[b], &cb, op->dx, op->alpha, op->beta);
     }
@@ -1569,6 +1594,7 @@ static void op_levelMultiColor(const Op* op, LD* phi, const LD* rhs) {
 /* relax (AMRPoissonOp.cpp:600-632): s_relaxMode 0 (looseGSRB), 1 (levelGSRB),
    2 (overlapGSRB), 3 (levelGSRBLazy), 4 (levelJacobi) or 5 (levelMultiColor) */
 static void op_relax(const Op* op, LD* e, const LD* r, int n) {
+  if (e->nc > 1) { for (int c = 0; c < e->nc; c++) op_relax(op, COMP(e, c), COMP(r, c), n); return; }
   for (int i = 0; i < n; i++) {
     if (op->relaxMode == 0) op_looseGSRB(op, e, r);
     else if (op->relaxMode == 2) op_overlapGSRB(op, e, r);
@@ -1580,6 +1606,7 @@ static void op_relax(const Op* op, LD* e, const LD* r, int n) {
 }
 /* restrictResidual (AMRPoissonOp.cpp:651-703) */
 static void op_restrict(const Op* op, LD* resC, LD* phi, const LD* rhs) {
+  if (phi->nc > 1) { for (int c = 0; c < phi->nc; c++) op_restrict(op, COMP(resC, c), COMP(phi, c), COMP(rhs, c)); return; }
   op_homog_cf(op, phi);
   op_exchange(op, phi);
   op_bc(op, phi, 1);
@@ -1595,17 +1622,20 @@ static void op_restrict(const Op* op, LD* resC, LD* phi, const LD* rhs) {
 }
 /* prolongIncrement (AMRPoissonOp.cpp:706-733) */
 static void op_prolong(const Op* op, LD* phi, const LD* coarse) {
+  if (phi->nc > 1) { for (int c = 0; c < phi->nc; c++) op_prolong(op, COMP(phi, c), COMP(coarse, c)); return; }
   #pragma omp parallel for schedule(static)
   for (int b = 0; b < op->L.n; b++) k_prolong(&phi->f[b], &coarse->f[b], &op->L.b[b], 2);
 }
 
 /* ---- LevelDataOps<FArrayBox> (LevelDataOps.H:93-294) ---- */
 static void ldo_setzero(LD* x) {
+  if (x->nc > 1) { for (int c = 0; c < x->nc; c++) ldo_setzero(COMP(x, c)); return; }
   #pragma omp parallel for schedule(static)
   for (int b = 0; b < x->L->n; b++) memset(x->f[b].d, 0, sizeof(double)*b_npts(&x->f[b].b));
 }
 /* incr: lhs += scale*x on valid boxes (BaseFab::plus) */
 static void ldo_incr(LD* lhs, const LD* x, double scale) {
+  if (lhs->nc > 1) { for (int c = 0; c < lhs->nc; c++) ldo_incr(COMP(lhs, c), COMP(x, c), scale); return; }
   #pragma omp parallel for schedule(static)
   for (int b = 0; b < lhs->L->n; b++) {
     const Box* r = &lhs->L->b[b];
@@ -1616,6 +1646,7 @@ static void ldo_incr(LD* lhs, const LD* x, double scale) {
 }
 /* axby on whole FABs: copy x, mult a, plus(y, b) */
 static void ldo_axby(LD* lhs, const LD* x, const LD* y, double a, double bb) {
+  if (lhs->nc > 1) { for (int c = 0; c < lhs->nc; c++) ldo_axby(COMP(lhs, c), COMP(x, c), COMP(y, c), a, bb); return; }
   #pragma omp parallel for schedule(static)
   for (int b = 0; b < lhs->L->n; b++) {
     Fab* f = &lhs->f[b];
@@ -1630,10 +1661,12 @@ static void ldo_axby(LD* lhs, const LD* x, const LD* y, double a, double bb) {
   }
 }
 static void ldo_scale(LD* x, double s) {
+  if (x->nc > 1) { for (int c = 0; c < x->nc; c++) ldo_scale(COMP(x, c), s); return; }
   for (int b = 0; b < x->L->n; b++) { long n = b_npts(&x->f[b].b); for (long q = 0; q < n; q++) x->f[b].d[q] *= s; }
 }
 /* assignLocal: FArrayBox::copy over the intersection of the two FAB boxes */
 static void ldo_assign_local(LD* lhs, const LD* rhs) {
+  if (lhs->nc > 1) { for (int c = 0; c < lhs->nc; c++) ldo_assign_local(COMP(lhs, c), COMP(rhs, c)); return; }
   #pragma omp parallel for schedule(static)
   for (int b = 0; b < lhs->L->n; b++) {
     Box r = b_isect(lhs->f[b].b, &rhs->f[b].b);
@@ -1644,12 +1677,17 @@ static void ldo_assign_local(LD* lhs, const LD* rhs) {
    LevelDataOps::dotProduct sums boxes in layout order */
 static double ldo_dot(const LD* a, const LD* c) {
   double val = 0.0;
-  for (int b = 0; b < a->L->n; b++) {
+  const int n = a->L->n;
+  for (int b = 0; b < n; b++) {
     const Box* r = &a->L->b[b];
     double dot = 0.0;
-    for (int k = r->lo[2]; k <= r->hi[2]; k++)
-      for (int j = r->lo[1]; j <= r->hi[1]; j++)
-        for (int i = r->lo[0]; i <= r->hi[0]; i++) dot += *P(&a->f[b],i,j,k) * *P(&c->f[b],i,j,k);
+    for (int m = 0; m < a->nc; m++) {      /* components outermost, one running sum */
+      const Fab* fa = &a->f[(size_t)m*n + b];
+      const Fab* fc = &c->f[(size_t)m*n + b];
+      for (int k = r->lo[2]; k <= r->hi[2]; k++)
+        for (int j = r->lo[1]; j <= r->hi[1]; j++)
+          for (int i = r->lo[0]; i <= r->hi[0]; i++) dot += *P(fa,i,j,k) * *P(fc,i,j,k);
+    }
     val += dot;
   }
   return val;
@@ -1662,63 +1700,65 @@ static double fab_maxnorm(const Fab* f, const Box* r) {
       for (int i = r->lo[0]; i <= r->hi[0]; i++) { double a = fabs(*P(f,i,j,k)); if (a > m) m = a; }
   return m;
 }
-/* FArrayBox::sumPow(box, 2): per-column accumulation over rows, then over columns
-   (FArrayBox.cpp:165-230) */
-static double fab_sumpow2(const Fab* f, const Box* r) {
+/* FArrayBox::sumPow(box, 2) (FArrayBox.cpp:165-230; ABS: FArrayBox::norm(box, 1),
+   :110-141) of box b over every component: the pencils of ForAllThisCPencil, component
+   outermost, then z, then y, accumulate into one per-column array (the first pencil
+   sets it), then the columns are summed left to right */
+static double ld_box_sumpow(const LD* x, int b, int absval) {
+  const Box* r = &x->L->b[b];
   int nx = b_len(r, 0);
   double* tmp = malloc(sizeof(double)*nx);
   int first = 1;
-  for (int k = r->lo[2]; k <= r->hi[2]; k++)
-    for (int j = r->lo[1]; j <= r->hi[1]; j++) {
-      const double* row = P(f, r->lo[0], j, k);
-      if (first) { for (int i = 0; i < nx; i++) tmp[i] = row[i]*row[i]; first = 0; }
-      else for (int i = 0; i < nx; i++) tmp[i] += row[i]*row[i];
-    }
+  for (int m = 0; m < x->nc; m++) {
+    const Fab* f = &x->f[(size_t)m*x->L->n + b];
+    for (int k = r->lo[2]; k <= r->hi[2]; k++)
+      for (int j = r->lo[1]; j <= r->hi[1]; j++) {
+        const double* row = P(f, r->lo[0], j, k);
+        if (first) {
+          for (int i = 0; i < nx; i++) tmp[i] = absval ? fabs(row[i]) : row[i]*row[i];
+          first = 0;
+        } else {
+          for (int i = 0; i < nx; i++) tmp[i] += absval ? fabs(row[i]) : row[i]*row[i];
+        }
+      }
+  }
   double sum = tmp[0];
   for (int i = 1; i < nx; i++) sum += tmp[i];
   free(tmp);
   return sum;
 }
-/* FArrayBox::norm(box, 1) (FArrayBox.cpp:110-141): per-column sums of |x| over rows,
-   then over columns */
-static double fab_sumabs(const Fab* f, const Box* r) {
-  int nx = b_len(r, 0);
-  double* tmp = malloc(sizeof(double)*nx);
-  int first = 1;
-  for (int k = r->lo[2]; k <= r->hi[2]; k++)
-    for (int j = r->lo[1]; j <= r->hi[1]; j++) {
-      const double* row = P(f, r->lo[0], j, k);
-      if (first) { for (int i = 0; i < nx; i++) tmp[i] = fabs(row[i]); first = 0; }
-      else for (int i = 0; i < nx; i++) tmp[i] += fabs(row[i]);
-    }
-  double sum = tmp[0];
-  for (int i = 1; i < nx; i++) sum += tmp[i];
-  free(tmp);
-  return sum;
+static double ld_box_maxnorm(const LD* x, int b) {
+  double m = 0.0;
+  for (int c = 0; c < x->nc; c++) {
+    double v = fab_maxnorm(&x->f[(size_t)c*x->L->n + b], &x->L->b[b]);
+    if (v > m) m = v;
+  }
+  return m;
 }
 /* norm(BoxLayoutData, interval, p) (BoxLayoutData.cpp:23-92), p = 0, 1 or 2 */
 static double ldo_norm(const LD* x, int p) {
   double norm = 0;
   if (p == 0) {
-    for (int b = 0; b < x->L->n; b++) { double m = fab_maxnorm(&x->f[b], &x->L->b[b]); norm = norm > m ? norm : m; }
+    for (int b = 0; b < x->L->n; b++) { double m = ld_box_maxnorm(x, b); norm = norm > m ? norm : m; }
     return norm;
   }
   if (p == 1) {
-    for (int b = 0; b < x->L->n; b++) norm += fab_sumabs(&x->f[b], &x->L->b[b]);
+    for (int b = 0; b < x->L->n; b++) norm += ld_box_sumpow(x, b, 1);
     return norm;
   }
-  for (int b = 0; b < x->L->n; b++) norm += fab_sumpow2(&x->f[b], &x->L->b[b]);
+  for (int b = 0; b < x->L->n; b++) norm += ld_box_sumpow(x, b, 0);
   return sqrt(norm);
 }
 /* AMRPoissonOp::localMaxNorm */
 static double op_localMaxNorm(const LD* x) {
   double m = 0;
   #pragma omp parallel for reduction(max:m) schedule(static)
-  for (int b = 0; b < x->L->n; b++) { double v = fab_maxnorm(&x->f[b], &x->L->b[b]); if (v > m) m = v; }
+  for (int b = 0; b < x->L->n; b++) { double v = ld_box_maxnorm(x, b); if (v > m) m = v; }
   return m;
 }
 /* preCond (AMRPoissonOp.cpp:304-329) */
 static void op_precond(const Op* op, LD* phi, const LD* rhs) {
+  if (phi->nc > 1) { for (int c = 0; c < phi->nc; c++) op_precond(op, COMP(phi, c), COMP(rhs, c)); return; }
   if (op->vc) {  /* VCAMRPoissonOp2::preCond (VCAMRPoissonOp2.cpp:93-127): phi = rhs * lambda */
     for (int b = 0; b < op->L.n; b++) {
       Box r = b_isect(phi->f[b].b, &rhs->f[b].b);
@@ -1743,6 +1783,7 @@ static void op_precond(const Op* op, LD* phi, const LD* rhs) {
 
 /* reflux (AMRPoissonOp.cpp:1117-1201) + NEWGETFLUX + LevelFluxRegister */
 static void op_reflux(Op* cop, Op* fop, LD* phiFine, LD* phi, LD* res) {
+  if (phi->nc > 1) { for (int c = 0; c < phi->nc; c++) op_reflux(cop, fop, COMP(phiFine, c), COMP(phi, c), COMP(res, c)); return; }
   FluxReg* r = &cop->fr;
   ldo_setzero(&r->coarFlux);
   ldo_setzero(&r->fineFlux);
@@ -1860,6 +1901,7 @@ struct OHier {
   double bic_reps, bic_hang, bic_small;
   /* statistics */
   int bottom_calls;
+  int nc;             /* components of the solution, rhs and every solver field */
 };
 
 /* ratios: nlev-1 refinement ratios (2 or 4) between consecutive levels (NULL: all 2),
@@ -1869,6 +1911,7 @@ OHier* oracle_create_r(const int* is, const double* rs, const int* ratios) {
   int nlev = is[0];
   if (nlev < 1 || nlev > 8) { set_err("bad nlev"); free(h); return NULL; }
   h->nlev = nlev;
+  h->nc = 1;
   h->ratio = calloc(nlev, sizeof(int));
   h->mgBottomCells = calloc(nlev, sizeof(long));
   for (int l = 1; l < nlev; l++) {
@@ -1995,6 +2038,7 @@ OHier* oracle_create(const int* is, const double* rs) { return oracle_create_r(i
    arithmetic), each op with its lambda. */
 static const Op* mg_op(const OHier* h, int d);
 int oracle_vc_set(OHier* h, const double* a, const double* b0, const double* b1, const double* b2) {
+  if (h->nc > 1) { set_err("variable coefficients with ncomp > 1 are not supported"); return -1; }
   mg_levels_free(h);
   const double* bs[3] = {b0, b1, b2};
   long ao = 0, bo[3] = {0, 0, 0};
@@ -2093,15 +2137,35 @@ int oracle_vc_lambda(OHier* h, int depth, double* out) {
   return 0;
 }
 long oracle_level_cells(const OHier* h, int lev) { return lay_cells(&h->grids[lev]); }
-/* norm(BoxLayoutData, interval, p) on MG depth d of level 0 (p = 0, 1, 2) */
+/* norm(BoxLayoutData, interval, p) on MG depth d of level 0 (p = 0, 1, 2), over the
+   solver's components (x valid-packed with them) */
 int oracle_level_norm(OHier* h, int d, const double* x_in, int p, double* out) {
   if (d < 0 || d >= h->mgD) { set_err("bad depth"); return -1; }
   if (p < 0 || p > 2) { set_err("norm order must be 0, 1 or 2"); return -1; }
   const Op* op = mg_op(h, d);
-  LD x = ld_make(&op->L, 0);
+  LD x = ld_make_nc(&op->L, 0, h->nc);
   ld_from_packed(&x, x_in);
   *out = ldo_norm(&x, p);
   ld_free(&x);
+  return 0;
+}
+/* LevelDataOps::dotProduct on MG depth d of level 0, over the solver's components */
+int oracle_level_dot(OHier* h, int d, const double* a_in, const double* b_in, double* out) {
+  if (d < 0 || d >= h->mgD) { set_err("bad depth"); return -1; }
+  const Op* op = mg_op(h, d);
+  LD a = ld_make_nc(&op->L, 0, h->nc), b = ld_make_nc(&op->L, 0, h->nc);
+  ld_from_packed(&a, a_in);
+  ld_from_packed(&b, b_in);
+  *out = ldo_dot(&a, &b);
+  ld_free(&a); ld_free(&b);
+  return 0;
+}
+/* number of components of the solver's fields (AMRPoissonOp acts on LevelData with any
+   nComp; every component shares the operator and its boundary conditions) */
+int oracle_set_ncomp(OHier* h, int nc) {
+  if (nc < 1 || nc > ORACLE_MAX_NCOMP) { set_err("ncomp must be 1..16"); return -1; }
+  if (nc > 1 && h->op[0].vc) { set_err("ncomp > 1 with variable coefficients is not supported"); return -1; }
+  h->nc = nc;
   return 0;
 }
 int oracle_num_mg_depths(const OHier* h) { return h->mgD; }
@@ -2116,9 +2180,10 @@ static const Op* mg_op(const OHier* h, int d) { return d == 0 ? &h->op[0] : &h->
 typedef struct { double eps, reps, hang, small, metric; int imax, numRestarts, normType; } BiCG;
 
 static int bicgstab_solve(const Op* op, LD* phi, const LD* rhs, const BiCG* p, int* iters_out) {
-  LD r = ld_make(rhs->L, rhs->g), r_tilde = ld_make(rhs->L, rhs->g), e = ld_make(phi->L, phi->g),
-     pp = ld_make(rhs->L, rhs->g), p_tilde = ld_make(phi->L, phi->g), s_tilde = ld_make(phi->L, phi->g),
-     t = ld_make(rhs->L, rhs->g), v = ld_make(rhs->L, rhs->g);
+  const int nc = rhs->nc;
+  LD r = ld_make_nc(rhs->L, rhs->g, nc), r_tilde = ld_make_nc(rhs->L, rhs->g, nc), e = ld_make_nc(phi->L, phi->g, nc),
+     pp = ld_make_nc(rhs->L, rhs->g, nc), p_tilde = ld_make_nc(phi->L, phi->g, nc),
+     s_tilde = ld_make_nc(phi->L, phi->g, nc), t = ld_make_nc(rhs->L, rhs->g, nc), v = ld_make_nc(rhs->L, rhs->g, nc);
   int recount = 0, status = -1;
   ldo_setzero(&r);
   op_residual(op, &r, phi, rhs, 1);
@@ -2274,6 +2339,7 @@ static void amr_residual_level(AMRState* a, LD* res, LD* phi, const LD* rhs, int
 }
 /* zeroCovered -> LevelDataOps::copyToZero over the copier's toRegions */
 static void zero_covered(OHier* h, LD* res, int lev) {
+  if (res->nc > 1) { for (int c = 0; c < res->nc; c++) zero_covered(h, COMP(res, c), lev); return; }
   const Plan* p = &h->resCopier[lev+1];
   for (int q = 0; q < p->n; q++) fab_setval(&res->f[p->m[q].to], p->m[q].region, 0.0);
 }
@@ -2292,6 +2358,12 @@ static double amr_residual(AMRState* a, LD* res, LD* phi, const LD* rhs, int lma
 /* AMRRestrictS (AMRPoissonOp.cpp:877-919) */
 static void amr_restrictS(AMRState* a, int l, LD* resC, const LD* residual, LD* correction,
                           const LD* coarseCorrection, LD* scratch) {
+  if (resC->nc > 1) {
+    for (int c = 0; c < resC->nc; c++)
+      amr_restrictS(a, l, COMP(resC, c), COMP(residual, c), COMP(correction, c), COMP(coarseCorrection, c),
+                    COMP(scratch, c));
+    return;
+  }
   OHier* h = a->h; Op* op = &h->op[l];
   qcfi_interp(&op->qcfi, correction, coarseCorrection);      /* AMRResidualNF */
   op_residualI(op, scratch, correction, residual, 1);
@@ -2299,6 +2371,11 @@ static void amr_restrictS(AMRState* a, int l, LD* resC, const LD* residual, LD* 
 }
 /* AMRProlongS (AMRPoissonOp.cpp:959-990) */
 static void amr_prolongS(AMRState* a, int l, LD* correction, const LD* coarseCorrection, LD* temp) {
+  if (correction->nc > 1) {
+    for (int c = 0; c < correction->nc; c++)
+      amr_prolongS(a, l, COMP(correction, c), COMP(coarseCorrection, c), COMP(temp, c));
+    return;
+  }
   OHier* h = a->h; Op* op = &h->op[l];
   exec_copy(temp, coarseCorrection, &h->revCopier[l]);
   for (int b = 0; b < op->L.n; b++) k_prolong(&correction->f[b], &temp->f[b], &op->L.b[b], h->ratio[l]);
@@ -2369,9 +2446,9 @@ static void amr_state_make(AMRState* a, OHier* h, int pre, int post, int bottom,
   a->h = h;
   a->corr = calloc(h->nlev, sizeof(LD)); a->resid = calloc(h->nlev, sizeof(LD)); a->resC = calloc(h->nlev, sizeof(LD));
   for (int l = 0; l < h->nlev; l++) {
-    a->corr[l] = ld_make(&h->grids[l], 1);
-    a->resid[l] = ld_make(&h->grids[l], 0);
-    if (l > 0) a->resC[l] = ld_make(&h->cfL[l], 0);
+    a->corr[l] = ld_make_nc(&h->grids[l], 1, h->nc);
+    a->resid[l] = ld_make_nc(&h->grids[l], 0, h->nc);
+    if (l > 0) a->resC[l] = ld_make_nc(&h->cfL[l], 0, h->nc);
   }
   MGState* m = &a->mg;
   m->h = h; m->pre = pre; m->post = post; m->bottom = bottom; m->cycles = numMG;
@@ -2384,7 +2461,10 @@ static void amr_state_make(AMRState* a, OHier* h, int pre, int post, int bottom,
     m->bottomCells = h->mgBottomCells[lbase];
   }
   m->mgres = calloc(m->D, sizeof(LD)); m->mgcor = calloc(m->D, sizeof(LD));
-  for (int d = 1; d < m->D; d++) { m->mgres[d] = ld_make(&m->L[d], 0); m->mgcor[d] = ld_make(&m->L[d], 1); }
+  for (int d = 1; d < m->D; d++) {
+    m->mgres[d] = ld_make_nc(&m->L[d], 0, h->nc);
+    m->mgcor[d] = ld_make_nc(&m->L[d], 1, h->nc);
+  }
   m->bic.eps = 1e-6; m->bic.reps = h->bic_reps; m->bic.hang = h->bic_hang; m->bic.small = h->bic_small;
   m->bic.metric = -1.0; m->bic.imax = h->bic_imax; m->bic.numRestarts = h->bic_numRestarts; m->bic.normType = 2;
 }
@@ -2397,15 +2477,15 @@ static void amr_state_free(AMRState* a) {
 
 static LD* levels_make(OHier* h, int g) {
   LD* x = calloc(h->nlev, sizeof(LD));
-  for (int l = 0; l < h->nlev; l++) x[l] = ld_make(&h->grids[l], g);
+  for (int l = 0; l < h->nlev; l++) x[l] = ld_make_nc(&h->grids[l], g, h->nc);
   return x;
 }
 static void levels_free(OHier* h, LD* x) { for (int l = 0; l < h->nlev; l++) ld_free(&x[l]); free(x); }
 static void levels_from_packed(OHier* h, LD* x, const double* v) {
-  long o = 0; for (int l = 0; l < h->nlev; l++) { ld_from_packed(&x[l], v + o); o += lay_cells(&h->grids[l]); }
+  long o = 0; for (int l = 0; l < h->nlev; l++) { ld_from_packed(&x[l], v + o); o += lay_cells(&h->grids[l])*h->nc; }
 }
 static void levels_to_packed(OHier* h, const LD* x, double* v) {
-  long o = 0; for (int l = 0; l < h->nlev; l++) { ld_to_packed(&x[l], v + o); o += lay_cells(&h->grids[l]); }
+  long o = 0; for (int l = 0; l < h->nlev; l++) { ld_to_packed(&x[l], v + o); o += lay_cells(&h->grids[l])*h->nc; }
 }
 
 /* AMRMultiGrid::solve(phi, rhs, l_max, l_base) -> solveNoInit -> solveNoInitResid

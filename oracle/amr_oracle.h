@@ -37,9 +37,17 @@ int oracle_solve_levels(OHier* h, const int* sparams, const double* rparams, con
                         double* phi, int zero_phi, int l_max, int l_base, double* hist, int hist_cap, int* info);
 /* the same with refinement ratios (2 or 4) between consecutive levels; NULL: all 2 */
 OHier* oracle_create_r(const int* ispec, const double* rspec, const int* ratios);
-/* BCValueFunc with a caller pointer; fn NULL restores the constant face values */
+/* BCValueFunc with a caller pointer; fn NULL restores the constant face values.
+   value holds one entry per component. */
 typedef void (*oracle_bc_value_fn)(double* pos, int* dir, int* side, double* value, void* user);
 int    oracle_set_bc_values(OHier* h, oracle_bc_value_fn fn, void* user);
+/* components (1..16) of the fields of oracle_solve / _solve_levels / _amr_residual /
+   _amr_operator / _level_norm / _level_dot: valid-packed box by box, in each box
+   component by component.  Every component shares the operator and its boundary
+   conditions (ConstDiriNeumBC gives each the same face value); the dot products and
+   norms of the solver run over all components in FArrayBox order.  The other level
+   functions take one component.  Not with variable coefficients. */
+int    oracle_set_ncomp(OHier* h, int ncomp);
 void   oracle_destroy(OHier* h);
 long   oracle_level_cells(const OHier* h, int lev);
 int    oracle_num_mg_depths(const OHier* h);
@@ -103,6 +111,8 @@ int  oracle_vc_set(OHier* h, const double* a, const double* b0, const double* b1
 long oracle_face_cells(const OHier* h, int lev, int dir);
 /* norm(BoxLayoutData, interval, p) (BoxLayoutData.cpp:23-92) on MG depth d, p = 0, 1, 2 */
 int  oracle_level_norm(OHier* h, int d, const double* x, int p, double* out);
+/* LevelDataOps::dotProduct (per box FArrayBox::dotProduct, LevelDataOps.H:109-136) on MG depth d */
+int  oracle_level_dot(OHier* h, int d, const double* a, const double* b, double* out);
 /* AMRPoissonOp::s_relaxMode for every operator: 1 levelGSRB (default), 4 levelJacobi */
 int  oracle_set_relax_mode(OHier* h, int mode);
 int  oracle_set_vc_average(OHier* h, int type);

@@ -92,6 +92,8 @@ def lib():
         L.oracle_set_alpha_beta.argtypes = [C.c_void_p, C.c_double, C.c_double]
         L.oracle_set_bc_values.argtypes = [C.c_void_p, BC_VALUE_FN, C.c_void_p]
         L.oracle_level_norm.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _dp]
+        L.oracle_level_dot.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp]
+        L.oracle_set_ncomp.argtypes = [C.c_void_p, C.c_int]
         L.oracle_set_relax_mode.argtypes = [C.c_void_p, C.c_int]
         L.oracle_set_vc_average.argtypes = [C.c_void_p, C.c_int]
         L.oracle_set_bottom_params.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double]
@@ -306,6 +308,19 @@ class Oracle:
         self._chk(lib().oracle_level_norm(self.h, depth, px, int(p), out.ctypes.data_as(_dp)))
         return float(out[0])
 
+    def level_dot(self, depth, a, b):
+        a, pa = _d(a)
+        b, pb = _d(b)
+        out = np.zeros(1)
+        self._chk(lib().oracle_level_dot(self.h, depth, pa, pb, out.ctypes.data_as(_dp)))
+        return float(out[0])
+
+    def set_ncomp(self, ncomp):
+        """Components of the solve / AMR residual / operator / level norm and dot fields
+        (valid-packed box by box, each box component by component)."""
+        self._chk(lib().oracle_set_ncomp(self.h, int(ncomp)))
+        self.ncomp = int(ncomp)
+
     def set_vc_average(self, harmonic):
         """Coefficient coarsening of the next vc_set: arithmetic (False) or harmonic."""
         self._chk(lib().oracle_set_vc_average(self.h, 1 if harmonic else 0))
@@ -320,14 +335,16 @@ class Oracle:
         self._chk(lib().oracle_set_alpha_beta(self.h, float(alpha), float(beta)))
 
     def set_bc_values(self, fn):
-        """DiriBC / NeumBC with a BCValueHolder: fn(pos, dir, side) -> value at each
-        domain face centre, called at every inhomogeneous ghost fill; None restores the
-        constant face values."""
+        """DiriBC / NeumBC with a BCValueHolder: fn(pos, dir, side) -> value (or one
+        value per component) at each domain face centre, called at every inhomogeneous
+        ghost fill; None restores the constant face values."""
         if fn is None:
             cb = C.cast(None, BC_VALUE_FN)
         else:
             def _cb(pos, d, s, v, _user):
-                v[0] = float(fn((pos[0], pos[1], pos[2]), d[0], s[0]))
+                val = fn((pos[0], pos[1], pos[2]), d[0], s[0])
+                for c, x in enumerate(np.atleast_1d(val)):
+                    v[c] = float(x)
             cb = BC_VALUE_FN(_cb)
         self._chk(lib().oracle_set_bc_values(self.h, cb, None))
         self._bc_fn = cb

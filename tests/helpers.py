@@ -119,3 +119,29 @@ def mixed_ratio_amr(ratios, bc=(1, 1, 1), alpha=0.0, beta=1.0):
     lev2 = H.box_refine(H.box((d1[1][0] - 4 * r1 + 1, -2 * r1, -2 * r1), (d1[1][0], 2 * r1 - 1, 2 * r1 - 1)), r2)
     return H.Hierarchy(dom, [H.domain_split(dom, 8, 4), H.domain_split(lev1, 16, 4), H.domain_split(lev2, 16, 4)],
                        1.0 / 16, alpha, beta, bc_type_lo=bc, bc_type_hi=bc, ratios=tuple(ratios))
+
+
+def comps_pack(h, parts):
+    """nc valid-packed one-component arrays (all levels) -> one array packed box by
+    box, each box component by component (the order of an FArrayBox's data)."""
+    out = []
+    o = 0
+    for l in range(h.nlev):
+        for b in h.levels[l]:
+            n = H.box_npts(b)
+            out.extend(p[o:o + n] for p in parts)
+            o += n
+    return np.concatenate(out)
+
+
+def comps_unpack(h, v, nc):
+    """Inverse of comps_pack: the nc one-component arrays."""
+    parts = [[] for _ in range(nc)]
+    o = 0
+    for l in range(h.nlev):
+        for b in h.levels[l]:
+            n = H.box_npts(b)
+            for c in range(nc):
+                parts[c].append(v[o:o + n])
+                o += n
+    return [np.concatenate(p) for p in parts]
