@@ -25,16 +25,18 @@ BC_VALUE_FN = C.CFUNCTYPE(None, _dp, _ip, _ip, _dp, C.c_void_p)
 
 
 def bc_value_callback(fn):
-    """An amrmg_bc_value_fn from fn(pos, dir, side) -> value (pos a 3-tuple of face
-    coordinates, side 0 low / 1 high); None gives a NULL pointer.  Keep the result alive while
-    the library may call it."""
+    """An amrmg_bc_value_fn from fn(pos, dir, side) -> value, or a sequence of one
+    value per component (pos a 3-tuple of face coordinates, side 0 low / 1 high); None
+    gives a NULL pointer.  Keep the result alive while the library may call it."""
     if fn is None:
         return C.cast(None, BC_VALUE_FN)
     if isinstance(fn, BC_VALUE_FN):
         return fn
 
     def cb(pos, d, s, v, _user):
-        v[0] = float(fn((pos[0], pos[1], pos[2]), d[0], s[0]))
+        val = fn((pos[0], pos[1], pos[2]), d[0], s[0])
+        for c, x in enumerate(np.atleast_1d(val)):
+            v[c] = float(x)
     return BC_VALUE_FN(cb)
 _lib = None
 
@@ -90,6 +92,8 @@ def lib():
         L.amrmg_batch_set_reduction_order.argtypes = [C.c_void_p, C.c_int]
         L.amrmg_solver_create.argtypes = [C.c_void_p, _ip, _dp, C.c_int, C.POINTER(C.c_void_p)]
         L.amrmg_solver_create_ratios.argtypes = [C.c_void_p, _ip, _dp, _ip, C.c_int, C.POINTER(C.c_void_p)]
+        L.amrmg_solver_create_ncomp.argtypes = [C.c_void_p, _ip, _dp, _ip, C.c_int, C.c_int,
+                                                C.POINTER(C.c_void_p)]
         L.amrmg_solver_destroy.argtypes = [C.c_void_p]
         L.amrmg_solver_set_params.argtypes = [C.c_void_p, _ip, _dp]
         L.amrmg_solver_set_bc_values.argtypes = [C.c_void_p, BC_VALUE_FN, C.c_void_p]
@@ -210,17 +214,24 @@ class Context:
 
 class AMRMultiGrid:
     """AMRPoissonOpFactory::define + AMRMultiGrid::define/init over a Hierarchy,
-    with a BiCGStab bottom solver; fields phi/rhs live in HBM."""
+    with a BiCGStab bottom solver; fields phi/rhs live in HBM.  ncomp > 1: fields of
+    that many components, valid-packed box by box, each box component by component."""
 
     PHI, RHS, RES = 0, 1, 2
 
-    def __init__(self, ctx: Context, hier, precision: int = 0):
+    def __init__(self, ctx: Context, hier, precision: int = 0, ncomp: int = 1):
         self.ctx = ctx
         self.hier = hier
+        self.ncomp = int(ncomp)
         isp, rsp = hier.spec()
         p = C.c_void_p()
         ratios = getattr(hier, "ratios", None)
-        if ratios is None:
+        if self.ncomp != 1:
+            r = None if ratios is None else np.ascontiguousarray(ratios, dtype=np.int32)
+            _chk(lib().amrmg_solver_create_ncomp(ctx.p, isp.ctypes.data_as(_ip), rsp.ctypes.data_as(_dp),
+                                                  None if r is None else r.ctypes.data_as(_ip), self.ncomp,
+                                                  precision, C.byref(p)))
+        elif ratios is None:
             _chk(lib().amrmg_solver_create(ctx.p, isp.ctypes.data_as(_ip), rsp.ctypes.data_as(_dp), precision,
                                             C.byref(p)))
         else:
@@ -228,7 +239,7 @@ class AMRMultiGrid:
             _chk(lib().amrmg_solver_create_ratios(ctx.p, isp.ctypes.data_as(_ip), rsp.ctypes.data_as(_dp),
                                                    r.ctypes.data_as(_ip), precision, C.byref(p)))
         self.p = p
-        self.ncells = sum(lib().amrmg_solver_level_cells(p, l) for l in range(hier.nlev))
+        self.ncells = sum(lib().amrmg_solver_level_cells(p, l) for l in range(hier.nlev)) * self.ncomp
 
     def close(self):
         if self.p:
@@ -288,7 +299,8 @@ class AMRMultiGrid:
     def set_bc_values(self, fn):
         """DiriBC / NeumBC with a BCValueHolder on every operator: the inhomogeneous
         value of each Dirichlet or Neumann face cell is fn(pos, dir, side) at the face
-        centre (BCFunc.cpp:187-202); None restores the constant face values."""
+        centre (BCFunc.cpp:187-202), one value per component; None restores the
+        constant face values."""
         cb = bc_value_callback(fn)
         _chk(lib().amrmg_solver_set_bc_values(self.p, cb, None))
         self._bc_fn = cb
@@ -316,7 +328,7 @@ class AMRMultiGrid:
             _chk(lib().amrmg_solver_set_vc_coefs(self.p, None, None, None, None))
             return
         a, pa = _d(a)
-        if a.size != self.ncells:
+        if a.size != self.ncells // self.ncomp:
             raise ValueError("aCoef size does not match the hierarchy")
         bs = [_d(x) for x in b]
         if len(bs) != 3:

@@ -41,13 +41,31 @@ AMRPoissonOp<Real>::~AMRPoissonOp() {
 static void need(const void* have, const void* want, const char* what) {
   if (have != want || have == nullptr) throw std::runtime_error(std::string("level data layout mismatch in ") + what);
 }
+// Fields of several components: every kernel of the operator updates each component
+// from that component alone (the reference's Fortran loops "do n = 0, ncomp-1" around
+// per-cell updates, e.g. AMRPoissonOpF.ChF:158), so an operation on nc components is
+// the operation on each component's one-component view in turn.  The reductions
+// (dotProduct, norm, localMaxNorm) and the one-launch relaxation handle all
+// components themselves.
+template <class Real>
+struct CompView {
+  LevelData<Real> v;
+  CompView(const LevelData<Real>& x, int c) { v.alias(x, c); }
+};
 
 // ---- ghost refresh: homogeneousCFInterp (AMRPoissonOp.cpp:1598), exchange
 // (exchangeNoOverlap with the trimmed Copier), BC functor (BCFunc.cpp:58) ----
 template <class Real>
 void AMRPoissonOp<Real>::fillGhosts(T& phi, bool doCF, bool homogBC) {
+  if (phi.nc > 1) {
+    for (int c = 0; c < phi.nc; c++) {
+      CompView<Real> phi_(phi, c);
+      fillGhosts(phi_.v, doCF, homogBC);
+    }
+    return;
+  }
   need(phi.lay, m_lay, "fillGhosts");
-  ops::fill_ghosts<Real>(*m_lay, m_ghost, phi.d, doCF ? 1 : 0, homogBC ? 1 : 0, m_dx, m_dxCrse);
+  ops::fill_ghosts<Real>(*m_lay, m_ghost, phi.d, doCF ? 1 : 0, homogBC ? 1 : 0, m_dx, m_dxCrse, phi.comp);
 }
 template <class Real>
 void AMRPoissonOp<Real>::homogeneousCFInterp(T& phi) {
@@ -59,6 +77,13 @@ void AMRPoissonOp<Real>::homogeneousCFInterp(T& phi) {
 // residual (AMRPoissonOp.cpp:224-236) / residualI (:260-297)
 template <class Real>
 void AMRPoissonOp<Real>::residual(T& lhs, T& phi, const T& rhs, bool homog) {
+  if (phi.nc > 1) {
+    for (int c = 0; c < phi.nc; c++) {
+      CompView<Real> lhs_(lhs, c), phi_(phi, c), rhs_(rhs, c);
+      residual(lhs_.v, phi_.v, rhs_.v, homog);
+    }
+    return;
+  }
   need(lhs.lay, m_lay, "residual"); need(rhs.lay, m_lay, "residual");
   fillGhosts(phi, homog, homog);
   if (m_vc) vc::residual<Real>(*m_lay, *m_vc, lhs.d, phi.d, rhs.d, m_dx, m_alpha, m_beta);
@@ -66,6 +91,13 @@ void AMRPoissonOp<Real>::residual(T& lhs, T& phi, const T& rhs, bool homog) {
 }
 template <class Real>
 void AMRPoissonOp<Real>::residualI(T& lhs, T& phi, const T& rhs, bool homog) {
+  if (phi.nc > 1) {
+    for (int c = 0; c < phi.nc; c++) {
+      CompView<Real> lhs_(lhs, c), phi_(phi, c), rhs_(rhs, c);
+      residualI(lhs_.v, phi_.v, rhs_.v, homog);
+    }
+    return;
+  }
   need(lhs.lay, m_lay, "residualI"); need(rhs.lay, m_lay, "residualI");
   fillGhosts(phi, false, homog);
   if (m_vc) vc::residual<Real>(*m_lay, *m_vc, lhs.d, phi.d, rhs.d, m_dx, m_alpha, m_beta);
@@ -74,6 +106,13 @@ void AMRPoissonOp<Real>::residualI(T& lhs, T& phi, const T& rhs, bool homog) {
 // applyOp (:332-340) / applyOpI (:343-385)
 template <class Real>
 void AMRPoissonOp<Real>::applyOp(T& lhs, T& phi, bool homog) {
+  if (phi.nc > 1) {
+    for (int c = 0; c < phi.nc; c++) {
+      CompView<Real> lhs_(lhs, c), phi_(phi, c);
+      applyOp(lhs_.v, phi_.v, homog);
+    }
+    return;
+  }
   need(lhs.lay, m_lay, "applyOp");
   fillGhosts(phi, true, homog);
   if (m_vc) vc::apply<Real>(*m_lay, *m_vc, lhs.d, phi.d, m_dx, m_alpha, m_beta);
@@ -81,6 +120,13 @@ void AMRPoissonOp<Real>::applyOp(T& lhs, T& phi, bool homog) {
 }
 template <class Real>
 void AMRPoissonOp<Real>::applyOpI(T& lhs, T& phi, bool homog) {
+  if (phi.nc > 1) {
+    for (int c = 0; c < phi.nc; c++) {
+      CompView<Real> lhs_(lhs, c), phi_(phi, c);
+      applyOpI(lhs_.v, phi_.v, homog);
+    }
+    return;
+  }
   need(lhs.lay, m_lay, "applyOpI");
   fillGhosts(phi, false, homog);
   if (m_vc) vc::apply<Real>(*m_lay, *m_vc, lhs.d, phi.d, m_dx, m_alpha, m_beta);
@@ -90,6 +136,16 @@ void AMRPoissonOp<Real>::applyOpI(T& lhs, T& phi, bool homog) {
 template <class Real>
 void AMRPoissonOp<Real>::preCond(T& phi, const T& rhs) {
   need(phi.lay, m_lay, "preCond"); need(rhs.lay, m_lay, "preCond");
+  if (phi.nc > 1) {  // the scaling per component, the relaxation on the whole field
+    if (m_vc) throw std::runtime_error("preCond: variable coefficients take one component");
+    const double mult = 1.0 / (m_alpha - 2.0 * 3 * m_beta / (m_dx * m_dx));
+    for (int c = 0; c < phi.nc; c++) {
+      CompView<Real> phi_(phi, c), rhs_(rhs, c);
+      ops::precond_init<Real>(*m_lay, phi_.v.d, rhs_.v.d, mult);
+    }
+    relax(phi, rhs, 2);
+    return;
+  }
   if (m_vc) {  // VCAMRPoissonOp2::preCond (VCAMRPoissonOp2.cpp:93-127): phi = rhs * lambda, then relax
     vc::precond_init<Real>(*m_lay, *m_vc, phi.d, rhs.d);
     relax(phi, rhs, 2);
@@ -102,47 +158,104 @@ void AMRPoissonOp<Real>::preCond(T& phi, const T& rhs) {
 // LevelDataOps on the operands' own layout (LevelDataOps.H:93-294)
 template <class Real>
 void AMRPoissonOp<Real>::assignLocal(T& lhs, const T& rhs) {
+  if (lhs.nc > 1) {
+    for (int c = 0; c < lhs.nc; c++) {
+      CompView<Real> lhs_(lhs, c), rhs_(rhs, c);
+      assignLocal(lhs_.v, rhs_.v);
+    }
+    return;
+  }
   need(rhs.lay, lhs.lay, "assignLocal");
   ops::copy_all<Real>(*lhs.lay, lhs.d, rhs.d);
 }
 template <class Real>
 double AMRPoissonOp<Real>::dotProduct(const T& a, const T& b) {
   need(b.lay, a.lay, "dotProduct");
-  return ops::dot_ordered<Real>(*a.lay, a.d, b.d, 1);
+  if (b.nc != a.nc) throw std::runtime_error("dotProduct: component counts differ");
+  return ops::dot_ordered<Real>(*a.lay, a.d, b.d, 1, a.nc, a.cs);
 }
 template <class Real>
 void AMRPoissonOp<Real>::incr(T& lhs, const T& x, double s) {
+  if (lhs.nc > 1) {
+    for (int c = 0; c < lhs.nc; c++) {
+      CompView<Real> lhs_(lhs, c), x_(x, c);
+      incr(lhs_.v, x_.v, s);
+    }
+    return;
+  }
   need(x.lay, lhs.lay, "incr");
   ops::incr<Real>(*lhs.lay, lhs.d, x.d, s);
 }
 template <class Real>
 void AMRPoissonOp<Real>::incrAssign(T& lhs, T& x) {
+  if (lhs.nc > 1) {
+    for (int c = 0; c < lhs.nc; c++) {
+      CompView<Real> lhs_(lhs, c), x_(x, c);
+      incrAssign(lhs_.v, x_.v);
+    }
+    return;
+  }
   need(x.lay, lhs.lay, "incrAssign");
   ops::incr_assign<Real>(*lhs.lay, lhs.d, x.d);
 }
 template <class Real>
 void AMRPoissonOp<Real>::incrFromZero(T& lhs, const T& x) {
+  if (lhs.nc > 1) {
+    for (int c = 0; c < lhs.nc; c++) {
+      CompView<Real> lhs_(lhs, c), x_(x, c);
+      incrFromZero(lhs_.v, x_.v);
+    }
+    return;
+  }
   need(x.lay, lhs.lay, "incrFromZero");
   ops::incr_from_zero<Real>(*lhs.lay, lhs.d, x.d);
 }
 template <class Real>
 void AMRPoissonOp<Real>::incrAssignUpdate(T& m, T& d, T& phi) {
+  if (m.nc > 1) {
+    for (int c = 0; c < m.nc; c++) {
+      CompView<Real> m_(m, c), d_(d, c), phi_(phi, c);
+      incrAssignUpdate(m_.v, d_.v, phi_.v);
+    }
+    return;
+  }
   need(d.lay, m.lay, "incrAssignUpdate");
   need(phi.lay, m.lay, "incrAssignUpdate");
   ops::incr_assign_update<Real>(*m.lay, m.d, d.d, phi.d);
 }
 template <class Real>
 void AMRPoissonOp<Real>::incrPhiFromZero(T& phi, const T& x) {
+  if (phi.nc > 1) {
+    for (int c = 0; c < phi.nc; c++) {
+      CompView<Real> phi_(phi, c), x_(x, c);
+      incrPhiFromZero(phi_.v, x_.v);
+    }
+    return;
+  }
   need(x.lay, phi.lay, "incrPhiFromZero");
   ops::incr_phi_from_zero<Real>(*phi.lay, phi.d, x.d);
 }
 template <class Real>
 void AMRPoissonOp<Real>::axby(T& lhs, const T& x, const T& y, double a, double b) {
+  if (lhs.nc > 1) {
+    for (int c = 0; c < lhs.nc; c++) {
+      CompView<Real> lhs_(lhs, c), x_(x, c), y_(y, c);
+      axby(lhs_.v, x_.v, y_.v, a, b);
+    }
+    return;
+  }
   need(x.lay, lhs.lay, "axby"); need(y.lay, lhs.lay, "axby");
   ops::axby<Real>(*lhs.lay, lhs.d, x.d, y.d, a, b);
 }
 template <class Real>
 void AMRPoissonOp<Real>::scale(T& lhs, double s) {
+  if (lhs.nc > 1) {
+    for (int c = 0; c < lhs.nc; c++) {
+      CompView<Real> lhs_(lhs, c);
+      scale(lhs_.v, s);
+    }
+    return;
+  }
   need(lhs.lay, lhs.lay, "scale");
   ops::scale<Real>(*lhs.lay, lhs.d, s);
 }
@@ -150,25 +263,32 @@ void AMRPoissonOp<Real>::scale(T& lhs, double s) {
 template <class Real>
 double AMRPoissonOp<Real>::norm(const T& x, int ord) {
   need(x.lay, x.lay, "norm");
-  if (ord == 0) return ops::maxnorm<Real>(*x.lay, x.d, 2);
-  if (ord == 1) return ops::norm1_ordered<Real>(*x.lay, x.d, 3);
-  if (ord == 2) return ops::norm2_ordered<Real>(*x.lay, x.d, 3);
+  if (ord == 0) return ops::maxnorm<Real>(*x.lay, x.d, 2, x.nc, x.cs);
+  if (ord == 1) return ops::norm1_ordered<Real>(*x.lay, x.d, 3, x.nc, x.cs);
+  if (ord == 2) return ops::norm2_ordered<Real>(*x.lay, x.d, 3, x.nc, x.cs);
   throw std::runtime_error("AMRPoissonOp::norm: order must be 0, 1 or 2");
 }
 template <class Real>
 void AMRPoissonOp<Real>::setToZero(T& lhs) {
+  if (lhs.nc > 1) {
+    for (int c = 0; c < lhs.nc; c++) {
+      CompView<Real> lhs_(lhs, c);
+      setToZero(lhs_.v);
+    }
+    return;
+  }
   need(lhs.lay, lhs.lay, "setToZero");
   ops::setval<Real>(*lhs.lay, lhs.d, 0.0);
 }
 template <class Real>
 double AMRPoissonOp<Real>::localMaxNorm(const T& x) {
   need(x.lay, m_lay, "localMaxNorm");
-  return ops::maxnorm<Real>(*m_lay, x.d, 4);
+  return ops::maxnorm<Real>(*m_lay, x.d, 4, x.nc, x.cs);
 }
 template <class Real>
 void AMRPoissonOp<Real>::localMaxNormAsync(const T& x, int slot, bool accumulate) {
   need(x.lay, m_lay, "localMaxNorm");
-  ops::maxnorm_async<Real>(*m_lay, x.d, slot, accumulate ? 1 : 0);
+  for (int c = 0; c < x.nc; c++) ops::maxnorm_async<Real>(*m_lay, x.d + c * x.cs, slot, (accumulate || c > 0) ? 1 : 0);
 }
 
 // ---- variable coefficients ----
@@ -226,12 +346,33 @@ template <class Real>
 void AMRPoissonOp<Real>::createCoarser(T& coarse, const T& fine, bool) {
   if (!m_mgCoarse) throw std::runtime_error("createCoarser: level not coarsenable");
   need(fine.lay, m_lay, "createCoarser");
-  coarse.define(m_mgCoarse);
+  coarse.define(m_mgCoarse, fine.nc);
 }
 // relax (:600-632) with levelGSRB (:1216-1281)
 template <class Real>
 void AMRPoissonOp<Real>::levelGSRB(T& phi, const T& rhs) {
   need(rhs.lay, m_lay, "levelGSRB");
+  if (phi.nc > 1) {
+    if (rhs.nc != phi.nc) throw std::runtime_error("levelGSRB: component counts differ");
+    if (m_fused && m_ctx->fused_relax && !m_vc) {
+      // every component relaxed into the second buffer, then the buffers swapped
+      need(phi.lay, m_lay, "levelGSRB");
+      if (!m_relaxTmpN.isDefined() || m_relaxTmpN.nc != phi.nc) m_relaxTmpN.define(m_lay, phi.nc);
+      if (m_relaxTmpN.cs != phi.cs || rhs.cs != phi.cs) throw std::runtime_error("levelGSRB: component strides differ");
+      for (int c = 0; c < phi.nc; c++)
+        ops::gsrb_fused<Real>(*m_lay, m_faces, m_relaxTmpN.d + c * phi.cs, phi.d + c * phi.cs, rhs.d + c * phi.cs, m_dx,
+                              m_alpha, m_beta, m_relaxShape, m_relaxKc);
+      std::swap(phi.d, m_relaxTmpN.d);
+      return;
+    }
+    for (int c = 0; c < phi.nc; c++) {
+      CompView<Real> phi_(phi, c), rhs_(rhs, c);
+      levelGSRB(phi_.v, rhs_.v);
+    }
+    return;
+  }
+  if (!phi.own && m_fused && m_ctx->fused_relax)
+    throw std::runtime_error("levelGSRB: the one-launch relaxation swaps buffers and takes whole fields");
   if (m_vc) {  // VCAMRPoissonOp2::levelGSRB (VCAMRPoissonOp2.cpp:547-626)
     need(phi.lay, m_lay, "levelGSRB");
     if (m_fused && m_ctx->fused_relax) {
@@ -262,6 +403,10 @@ void AMRPoissonOp<Real>::levelGSRB(T& phi, const T& rhs) {
 template <class Real>
 bool AMRPoissonOp<Real>::tuneRelax(const T& phi, const T& rhs, double* times) {
   if (!m_fused || m_vc) return false;
+  if (phi.nc > 1) {  // timed on the first component
+    CompView<Real> phi_(phi, 0), rhs_(rhs, 0);
+    return tuneRelax(phi_.v, rhs_.v, times);
+  }
   need(phi.lay, m_lay, "tuneRelax"); need(rhs.lay, m_lay, "tuneRelax");
   if (!m_relaxTmp.isDefined()) m_relaxTmp.define(m_lay);
   const std::pair<int, int> pick =
@@ -275,7 +420,7 @@ bool AMRPoissonOp<Real>::tuneRelax(const T& phi, const T& rhs, double* times) {
 // dropped; ghost cells are refilled before any later read, as after any relax.
 template <class Real>
 void AMRPoissonOp<Real>::relaxFromZero(T& e, const T& r, int n) {
-  if (!(m_fused && m_ctx->fused_relax) || m_vc || m_relaxMode != 1 || n <= 0) {
+  if (!(m_fused && m_ctx->fused_relax) || m_vc || m_relaxMode != 1 || n <= 0 || e.nc > 1) {
     setToZero(e);
     relax(e, r, n);
     return;
@@ -289,6 +434,13 @@ void AMRPoissonOp<Real>::relaxFromZero(T& e, const T& r, int n) {
 }
 template <class Real>
 void AMRPoissonOp<Real>::relax(T& e, const T& r, int n) {
+  if (e.nc > 1 && m_relaxMode != 1) {  // levelGSRB takes the components itself
+    for (int c = 0; c < e.nc; c++) {
+      CompView<Real> e_(e, c), r_(r, c);
+      relax(e_.v, r_.v, n);
+    }
+    return;
+  }
   for (int i = 0; i < n; i++) {
     if (m_relaxMode == 4) levelJacobi(e, r);
     else if (m_relaxMode == 0) looseGSRB(e, r);
@@ -405,6 +557,13 @@ void AMRPoissonOp<Real>::levelJacobi(T& phi, const T& rhs) {
 // restrictResidual (:651-703)
 template <class Real>
 void AMRPoissonOp<Real>::restrictResidual(T& resC, T& phi, const T& rhs) {
+  if (phi.nc > 1) {
+    for (int c = 0; c < phi.nc; c++) {
+      CompView<Real> resC_(resC, c), phi_(phi, c), rhs_(rhs, c);
+      restrictResidual(resC_.v, phi_.v, rhs_.v);
+    }
+    return;
+  }
   need(rhs.lay, m_lay, "restrictResidual"); need(resC.lay, m_mgCoarse, "restrictResidual");
   fillGhosts(phi, true, true);
   if (m_vc)  // VCAMRPoissonOp2::restrictResidual (VCAMRPoissonOp2.cpp:195-256)
@@ -415,6 +574,13 @@ void AMRPoissonOp<Real>::restrictResidual(T& resC, T& phi, const T& rhs) {
 // prolongIncrement (:706-733)
 template <class Real>
 void AMRPoissonOp<Real>::prolongIncrement(T& phi, const T& c) {
+  if (phi.nc > 1) {
+    for (int q = 0; q < phi.nc; q++) {
+      CompView<Real> phi_(phi, q), c_(c, q);
+      prolongIncrement(phi_.v, c_.v);
+    }
+    return;
+  }
   need(phi.lay, m_lay, "prolongIncrement"); need(c.lay, m_mgCoarse, "prolongIncrement");
   ops::prolong<Real>(*m_lay, *m_mgCoarse, phi.d, c.d);
 }
@@ -422,6 +588,13 @@ void AMRPoissonOp<Real>::prolongIncrement(T& phi, const T& c) {
 // ---- AMR ----
 template <class Real>
 void AMRPoissonOp<Real>::coarseFineInterp(T& phif, const T& phic) {
+  if (phif.nc > 1) {
+    for (int c = 0; c < phif.nc; c++) {
+      CompView<Real> phif_(phif, c), phic_(phic, c);
+      coarseFineInterp(phif_.v, phic_.v);
+    }
+    return;
+  }
   if (!m_coarserLay) return;
   need(phif.lay, m_lay, "coarseFineInterp"); need(phic.lay, m_coarserLay, "coarseFineInterp");
   ops::cf_inhomog<Real>(m_ctx, d_cfc, n_cfc, d_cfg, d_cff, n_cff, d_deriv, phif.d, phic.d, m_dx,
@@ -431,6 +604,13 @@ void AMRPoissonOp<Real>::coarseFineInterp(T& phif, const T& phic) {
 template <class Real>
 void AMRPoissonOp<Real>::AMRResidual(T& res, T& phiFine, T& phi, T& phiCoarse, const T& rhs, bool homog,
                                      AMRLevelOp<T>* finerOp) {
+  if (phi.nc > 1) {
+    for (int c = 0; c < phi.nc; c++) {
+      CompView<Real> res_(res, c), phiFine_(phiFine, c), phi_(phi, c), phiCoarse_(phiCoarse, c), rhs_(rhs, c);
+      AMRResidual(res_.v, phiFine_.v, phi_.v, phiCoarse_.v, rhs_.v, homog, finerOp);
+    }
+    return;
+  }
   if (!m_vc && phiFine.isDefined()) {
     if (phiCoarse.isDefined()) coarseFineInterp(phi, phiCoarse);
     residualFused(res, phiFine, phi, rhs, homog, static_cast<AMRPoissonOp*>(finerOp));
@@ -442,6 +622,13 @@ void AMRPoissonOp<Real>::AMRResidual(T& res, T& phiFine, T& phi, T& phiCoarse, c
 template <class Real>
 void AMRPoissonOp<Real>::AMRResidualNC(T& res, T& phiFine, T& phi, const T& rhs, bool homog,
                                        AMRLevelOp<T>* finerOp) {
+  if (phi.nc > 1) {
+    for (int c = 0; c < phi.nc; c++) {
+      CompView<Real> res_(res, c), phiFine_(phiFine, c), phi_(phi, c), rhs_(rhs, c);
+      AMRResidualNC(res_.v, phiFine_.v, phi_.v, rhs_.v, homog, finerOp);
+    }
+    return;
+  }
   if (!m_vc && phiFine.isDefined()) {
     residualFused(res, phiFine, phi, rhs, homog, static_cast<AMRPoissonOp*>(finerOp));
     return;
@@ -453,6 +640,13 @@ void AMRPoissonOp<Real>::AMRResidualNC(T& res, T& phiFine, T& phi, const T& rhs,
 // launches (valid cells, then the refluxed cells), bitwise the same values
 template <class Real>
 void AMRPoissonOp<Real>::residualFused(T& res, T& phiFine, T& phi, const T& rhs, bool homog, AMRPoissonOp* finerOp) {
+  if (phi.nc > 1) {
+    for (int c = 0; c < phi.nc; c++) {
+      CompView<Real> res_(res, c), phiFine_(phiFine, c), phi_(phi, c), rhs_(rhs, c);
+      residualFused(res_.v, phiFine_.v, phi_.v, rhs_.v, homog, finerOp);
+    }
+    return;
+  }
   need(res.lay, m_lay, "AMRResidual"); need(rhs.lay, m_lay, "AMRResidual");
   need(phiFine.lay, m_finerLay, "AMRResidual"); need(phi.lay, m_lay, "AMRResidual");
   fillGhosts(phi, false, homog);
@@ -465,6 +659,13 @@ void AMRPoissonOp<Real>::residualFused(T& res, T& phiFine, T& phi, const T& rhs,
 // AMRResidualNF (:770-787)
 template <class Real>
 void AMRPoissonOp<Real>::AMRResidualNF(T& res, T& phi, T& phiCoarse, const T& rhs, bool homog) {
+  if (phi.nc > 1) {
+    for (int c = 0; c < phi.nc; c++) {
+      CompView<Real> res_(res, c), phi_(phi, c), phiCoarse_(phiCoarse, c), rhs_(rhs, c);
+      AMRResidualNF(res_.v, phi_.v, phiCoarse_.v, rhs_.v, homog);
+    }
+    return;
+  }
   if (phiCoarse.isDefined()) coarseFineInterp(phi, phiCoarse);
   residualI(res, phi, rhs, homog);
 }
@@ -472,17 +673,38 @@ void AMRPoissonOp<Real>::AMRResidualNF(T& res, T& phi, T& phiCoarse, const T& rh
 template <class Real>
 void AMRPoissonOp<Real>::AMROperator(T& LofPhi, T& phiFine, T& phi, T& phiCoarse, bool homog,
                                      AMRLevelOp<T>* finerOp) {
+  if (phi.nc > 1) {
+    for (int c = 0; c < phi.nc; c++) {
+      CompView<Real> LofPhi_(LofPhi, c), phiFine_(phiFine, c), phi_(phi, c), phiCoarse_(phiCoarse, c);
+      AMROperator(LofPhi_.v, phiFine_.v, phi_.v, phiCoarse_.v, homog, finerOp);
+    }
+    return;
+  }
   if (phiCoarse.isDefined()) coarseFineInterp(phi, phiCoarse);
   applyOpI(LofPhi, phi, homog);
   if (phiFine.isDefined()) reflux(phiFine, phi, LofPhi, static_cast<AMRPoissonOp*>(finerOp));
 }
 template <class Real>
 void AMRPoissonOp<Real>::AMROperatorNC(T& LofPhi, T& phiFine, T& phi, bool homog, AMRLevelOp<T>* finerOp) {
+  if (phi.nc > 1) {
+    for (int c = 0; c < phi.nc; c++) {
+      CompView<Real> LofPhi_(LofPhi, c), phiFine_(phiFine, c), phi_(phi, c);
+      AMROperatorNC(LofPhi_.v, phiFine_.v, phi_.v, homog, finerOp);
+    }
+    return;
+  }
   applyOpI(LofPhi, phi, homog);
   if (phiFine.isDefined()) reflux(phiFine, phi, LofPhi, static_cast<AMRPoissonOp*>(finerOp));
 }
 template <class Real>
 void AMRPoissonOp<Real>::AMROperatorNF(T& LofPhi, T& phi, T& phiCoarse, bool homog) {
+  if (phi.nc > 1) {
+    for (int c = 0; c < phi.nc; c++) {
+      CompView<Real> LofPhi_(LofPhi, c), phi_(phi, c), phiCoarse_(phiCoarse, c);
+      AMROperatorNF(LofPhi_.v, phi_.v, phiCoarse_.v, homog);
+    }
+    return;
+  }
   if (phiCoarse.isDefined()) coarseFineInterp(phi, phiCoarse);
   applyOpI(LofPhi, phi, homog);
 }
@@ -490,6 +712,13 @@ void AMRPoissonOp<Real>::AMROperatorNF(T& LofPhi, T& phi, T& phiCoarse, bool hom
 template <class Real>
 void AMRPoissonOp<Real>::AMRRestrictS(T& resCoarse, const T& residual, T& correction, T& coarseCorrection,
                                       T& scratch) {
+  if (correction.nc > 1) {
+    for (int c = 0; c < correction.nc; c++) {
+      CompView<Real> resCoarse_(resCoarse, c), residual_(residual, c), correction_(correction, c), coarseCorrection_(coarseCorrection, c), scratch_(scratch, c);
+      AMRRestrictS(resCoarse_.v, residual_.v, correction_.v, coarseCorrection_.v, scratch_.v);
+    }
+    return;
+  }
   need(resCoarse.lay, m_cfLay, "AMRRestrictS");
   if (!m_vc) {
     // AMRResidualNF (coarse-fine interpolation, homogeneous ghost refresh,
@@ -508,6 +737,13 @@ void AMRPoissonOp<Real>::AMRRestrictS(T& resCoarse, const T& residual, T& correc
 // AMRProlongS (:959-990)
 template <class Real>
 void AMRPoissonOp<Real>::AMRProlongS(T& correction, const T& coarseCorrection, T& temp) {
+  if (correction.nc > 1) {
+    for (int c = 0; c < correction.nc; c++) {
+      CompView<Real> correction_(correction, c), coarseCorrection_(coarseCorrection, c), temp_(temp, c);
+      AMRProlongS(correction_.v, coarseCorrection_.v, temp_.v);
+    }
+    return;
+  }
   need(temp.lay, m_cfLay, "AMRProlongS"); need(coarseCorrection.lay, m_coarserLay, "AMRProlongS");
   need(correction.lay, m_lay, "AMRProlongS");
   ops::motion<Real>(m_revCopier, *m_coarserLay, *m_cfLay, temp.d, coarseCorrection.d, 0);
@@ -519,18 +755,32 @@ void AMRPoissonOp<Real>::AMRUpdateResidual(T& residual, T& correction, T& coarse
   AMRResidualNF(residual, correction, coarseCorrection, residual, true);
 }
 template <class Real>
-void AMRPoissonOp<Real>::createCoarsened(T& lhs, const T&, int) {
-  lhs.define(m_cfLay);
+void AMRPoissonOp<Real>::createCoarsened(T& lhs, const T& rhs, int) {
+  lhs.define(m_cfLay, rhs.nc);
 }
 // assignCopier (:486-492): lhs (this level) <- rhs (coarsened finer level)
 template <class Real>
 void AMRPoissonOp<Real>::assignCopier(T& lhs, const T& rhs) {
+  if (lhs.nc > 1) {
+    for (int c = 0; c < lhs.nc; c++) {
+      CompView<Real> lhs_(lhs, c), rhs_(rhs, c);
+      assignCopier(lhs_.v, rhs_.v);
+    }
+    return;
+  }
   need(lhs.lay, m_lay, "assignCopier"); need(rhs.lay, m_finerCfLay, "assignCopier");
   ops::motion<Real>(*m_zeroPlan, *m_finerCfLay, *m_lay, lhs.d, rhs.d, 0);
 }
 // zeroCovered (:495-502) -> LevelDataOps::copyToZero
 template <class Real>
 void AMRPoissonOp<Real>::zeroCovered(T& lhs) {
+  if (lhs.nc > 1) {
+    for (int c = 0; c < lhs.nc; c++) {
+      CompView<Real> lhs_(lhs, c);
+      zeroCovered(lhs_.v);
+    }
+    return;
+  }
   need(lhs.lay, m_lay, "zeroCovered");
   if (!m_zeroPlan) throw std::runtime_error("zeroCovered: no finer level");
   ops::motion<Real>(*m_zeroPlan, *m_finerCfLay, *m_lay, lhs.d, (const Real*)nullptr, 1);
@@ -538,6 +788,13 @@ void AMRPoissonOp<Real>::zeroCovered(T& lhs) {
 // reflux (:1117-1201)
 template <class Real>
 void AMRPoissonOp<Real>::reflux(T& phiFine, T& phi, T& res, AMRPoissonOp* finerOp) {
+  if (phi.nc > 1) {
+    for (int c = 0; c < phi.nc; c++) {
+      CompView<Real> phiFine_(phiFine, c), phi_(phi, c), res_(res, c);
+      reflux(phiFine_.v, phi_.v, res_.v, finerOp);
+    }
+    return;
+  }
   need(phiFine.lay, m_finerLay, "reflux"); need(phi.lay, m_lay, "reflux"); need(res.lay, m_lay, "reflux");
   if (!m_vc != !finerOp->m_vc) throw std::runtime_error("reflux: variable coefficients on one level of two");
   finerOp->coarseFineInterp(phiFine, phi);
@@ -741,7 +998,7 @@ void defineLevelOp(AMRPoissonOp<Real>* op, Context* ctx, const DeviceLayout* lay
   op->m_mgCoarse = mgCoarse;
   op->m_ghost = build_ghost_plan(lay->dbl, domain, bc, dxCrse > 0, lay->merged);
   validate_ghost_plan(op->m_ghost, lay->dbl);
-  if (bc.fn) attach_bc_values(op->m_ghost, dx, bc);
+  if (bc.fn) attach_bc_values(op->m_ghost, dx, bc, op->m_ncomp);
   op->m_ghost.upload(ctx);
   setup_fused(op);
   if (op->m_mgCoarse) validate_pair(*op->m_lay, *op->m_mgCoarse);
@@ -753,7 +1010,7 @@ template <class Real>
 void setLevelOpBCValues(AMRPoissonOp<Real>* op, BCValueFn fn, void* user) {
   op->m_bc.fn = fn;
   op->m_bc.user = user;
-  attach_bc_values(op->m_ghost, op->m_dx, op->m_bc);
+  attach_bc_values(op->m_ghost, op->m_dx, op->m_bc, op->m_ncomp);
   op->m_ghost.upload(op->m_ctx);
 }
 
@@ -836,6 +1093,7 @@ void linkCoarser(AMRPoissonOp<Real>* op, AMRPoissonOp<Real>* coarserOp, const De
 template <class Real>
 std::unique_ptr<AMRPoissonOp<Real>> AMRPoissonOpFactory<Real>::AMRnewOp(int l, AMRPoissonOp<Real>* coarserOp) {
   std::unique_ptr<AMRPoissonOp<Real>> op(new AMRPoissonOp<Real>());
+  op->m_ncomp = ncomp;
   // the multigrid child: level 0's first multigrid depth; for a level above the base
   // with ratio 2, coarsen(level, 2) is its coarsened-fine layout (ratio 4 has none)
   const DeviceLayout* mgc = (l == 0 && !H->mgLay.empty()) ? H->mgLay[0].get()
@@ -854,6 +1112,7 @@ std::vector<std::unique_ptr<MGLevelOp<LevelData<Real>>>> AMRPoissonOpFactory<Rea
   for (size_t d = 0; d < lay.size(); d++) {
     const int coarsening = 1 << (d + 1);
     AMRPoissonOp<Real>* op = new AMRPoissonOp<Real>();
+    op->m_ncomp = ncomp;
     // MGnewOp keeps the AMR level's dxCrse: the coarse-fine faces of every depth get
     // the homogeneous interpolation with the coarser AMR level's spacing
     defineLevelOp(op, H->ctx, lay[d], coarsen(H->domains[l], coarsening), H->dxs[l] * coarsening, H->dxs[l - 1],
@@ -868,6 +1127,7 @@ std::vector<std::unique_ptr<MGLevelOp<LevelData<Real>>>> AMRPoissonOpFactory<Rea
   for (size_t d = 0; d < H->mgLay.size(); d++) {
     int coarsening = 1 << (d + 1);
     AMRPoissonOp<Real>* op = new AMRPoissonOp<Real>();
+    op->m_ncomp = ncomp;
     defineLevelOp(op, H->ctx, H->mgLay[d].get(), coarsen(H->domains[0], coarsening), H->dxs[0] * coarsening, -1.0,
                   alpha, beta, bc, (d + 1 < H->mgLay.size()) ? H->mgLay[d + 1].get() : nullptr);
     out.emplace_back(op);

@@ -26,6 +26,7 @@ class AMRPoissonOp : public AMRLevelOp<LevelData<Real>> {
   Box m_domain;
   double m_dx = 1, m_dxCrse = -1, m_alpha = 0, m_beta = 1;
   BCSpec m_bc{};
+  int m_ncomp = 1;                            // components of the fields (one BC value each)
   ops::GhostPlan m_ghost;
   // one-launch relaxation on merged levels (ops::gsrb_fused); m_relaxTmp is the
   // second buffer it relaxes into, swapped with the relaxed field afterwards
@@ -33,6 +34,7 @@ class AMRPoissonOp : public AMRLevelOp<LevelData<Real>> {
   ops::FaceRules m_faces{};
   int m_relaxShape = 0, m_relaxKc = 0;        // tile shape and z-chunk (0: automatic), see tuneRelax
   LevelData<Real> m_relaxTmp;
+  LevelData<Real> m_relaxTmpN;                // the same for fields of several components
   // coarser AMR level (QuadCFInterp + restriction/prolongation copiers)
   const DeviceLayout* m_coarserLay = nullptr;
   const DeviceLayout* m_cfLay = nullptr;      // coarsen(this level, refToCoarser): resC layout
@@ -84,7 +86,7 @@ class AMRPoissonOp : public AMRLevelOp<LevelData<Real>> {
   void residual(T& lhs, T& phi, const T& rhs, bool homogeneous = false) override;
   void preCond(T& cor, const T& residual) override;
   void applyOp(T& lhs, T& phi, bool homogeneous = false) override;
-  void create(T& lhs, const T& rhs) override { lhs.define(rhs.lay); }
+  void create(T& lhs, const T& rhs) override { lhs.define(rhs.lay, rhs.nc); }
   void assignLocal(T& lhs, const T& rhs) override;
   double dotProduct(const T& a, const T& b) override;
   void incr(T& lhs, const T& x, double scale) override;
@@ -185,8 +187,9 @@ class AMRPoissonOpFactory {
   DeviceHierarchy* H = nullptr;
   double alpha = 0, beta = 1;
   BCSpec bc{};
-  void define(DeviceHierarchy* h, double a, double b, const BCSpec& bcs) {
-    H = h; alpha = a; beta = b; bc = bcs;
+  int ncomp = 1;  // components of the fields the operators act on
+  void define(DeviceHierarchy* h, double a, double b, const BCSpec& bcs, int nc = 1) {
+    H = h; alpha = a; beta = b; bc = bcs; ncomp = nc;
   }
   // operator for AMR level l; the operator of level l-1 must be passed for l > 0
   std::unique_ptr<AMRPoissonOp<Real>> AMRnewOp(int l, AMRPoissonOp<Real>* coarserOp);

@@ -1660,7 +1660,7 @@ void DeviceBottomSolver<T>::solve(LevelData<T>& phi, const LevelData<T>& rhs) {
   if (m_device && (static_cast<const AMRPoissonOp<T>*>(m_ap)->m_vc ||
                    static_cast<const AMRPoissonOp<T>*>(m_ap)->m_relaxMode != 1))
     throw std::runtime_error("DeviceBottomSolver: operator changed after define");
-  if (!m_device) {
+  if (!m_device || phi.nc > 1) {  // fields of several components: the host-driven solve
     m_fallback.solve(phi, rhs);
     return;
   }

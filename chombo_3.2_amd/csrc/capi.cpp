@@ -22,6 +22,9 @@
 
 using namespace amrmg;
 
+// components a solver's fields may have (amrmg_solver_create_ncomp)
+constexpr int kMaxComps = 16;
+
 thread_local std::string amrmg_g_err;
 
 extern "C" const char* amrmg_last_error(void) { return amrmg_g_err.c_str(); }
@@ -49,6 +52,7 @@ struct amrmg_batch {
 struct amrmg_solver {
   amrmg_ctx* ctx = nullptr;
   int precision = 0;
+  int ncomp = 1;  // components of phi, rhs and the residual
   DeviceHierarchy H;
   BCSpec bc{};
   double alpha = 0, beta = 1;
@@ -103,34 +107,43 @@ struct amrmg_solver {
 };
 
 // ---- host <-> device transfers of valid-packed / ghosted-packed data ----
-static void upload_valid(const DeviceLayout& L, double* d, const double* h) {
-  std::vector<double> slab((size_t)L.slab, 0.0);
+// Valid-packed host order: box by box, in each box component by component (an
+// FArrayBox's data order), cells x fastest.  On the device component c is the field
+// at c * cs.
+static void upload_valid(const DeviceLayout& L, double* d, const double* h, int nc = 1, long long cs = 0) {
+  if (nc == 1) cs = L.slab;
+  const size_t n = (size_t)(cs * (nc - 1) + L.slab);
+  std::vector<double> slab(n, 0.0);
   long long o = 0;
   for (int b = 0; b < L.dbl.size(); b++) {
     const Box& bx = L.dbl[b];
-    for (int k = bx.lo[2]; k <= bx.hi[2]; k++)
-      for (int j = bx.lo[1]; j <= bx.hi[1]; j++) {
-        long long c = L.index(b, bx.lo[0], j, k);
-        std::memcpy(&slab[(size_t)c], h + o, sizeof(double) * bx.len(0));
-        o += bx.len(0);
-      }
+    for (int m = 0; m < nc; m++)
+      for (int k = bx.lo[2]; k <= bx.hi[2]; k++)
+        for (int j = bx.lo[1]; j <= bx.hi[1]; j++) {
+          long long c = L.index(b, bx.lo[0], j, k) + m * cs;
+          std::memcpy(&slab[(size_t)c], h + o, sizeof(double) * bx.len(0));
+          o += bx.len(0);
+        }
   }
-  hip_check(hipMemcpyAsync(d, slab.data(), sizeof(double) * L.slab, hipMemcpyHostToDevice, L.ctx->stream), "upload");
+  hip_check(hipMemcpyAsync(d, slab.data(), sizeof(double) * n, hipMemcpyHostToDevice, L.ctx->stream), "upload");
   L.ctx->sync();
 }
-static void download_valid(const DeviceLayout& L, const double* d, double* h) {
-  std::vector<double> slab((size_t)L.slab);
+static void download_valid(const DeviceLayout& L, const double* d, double* h, int nc = 1, long long cs = 0) {
+  if (nc == 1) cs = L.slab;
+  const size_t n = (size_t)(cs * (nc - 1) + L.slab);
+  std::vector<double> slab(n);
   L.ctx->sync();
-  hip_check(hipMemcpy(slab.data(), d, sizeof(double) * L.slab, hipMemcpyDeviceToHost), "download");
+  hip_check(hipMemcpy(slab.data(), d, sizeof(double) * n, hipMemcpyDeviceToHost), "download");
   long long o = 0;
   for (int b = 0; b < L.dbl.size(); b++) {
     const Box& bx = L.dbl[b];
-    for (int k = bx.lo[2]; k <= bx.hi[2]; k++)
-      for (int j = bx.lo[1]; j <= bx.hi[1]; j++) {
-        long long c = L.index(b, bx.lo[0], j, k);
-        std::memcpy(h + o, &slab[(size_t)c], sizeof(double) * bx.len(0));
-        o += bx.len(0);
-      }
+    for (int m = 0; m < nc; m++)
+      for (int k = bx.lo[2]; k <= bx.hi[2]; k++)
+        for (int j = bx.lo[1]; j <= bx.hi[1]; j++) {
+          long long c = L.index(b, bx.lo[0], j, k) + m * cs;
+          std::memcpy(h + o, &slab[(size_t)c], sizeof(double) * bx.len(0));
+          o += bx.len(0);
+        }
   }
 }
 static void upload_ghosted(const DeviceLayout& L, double* d, const double* h) {
@@ -210,27 +223,34 @@ int amrmg_solver_create(amrmg_ctx* ctx, const int* ispec, const double* rspec, i
 }
 int amrmg_solver_create_ratios(amrmg_ctx* ctx, const int* ispec, const double* rspec, const int* ref_ratios,
                                int precision, amrmg_solver** out) {
+  return amrmg_solver_create_ncomp(ctx, ispec, rspec, ref_ratios, 1, precision, out);
+}
+int amrmg_solver_create_ncomp(amrmg_ctx* ctx, const int* ispec, const double* rspec, const int* ref_ratios,
+                              int ncomp, int precision, amrmg_solver** out) {
   API_BEGIN
   if (precision != 0 && precision != 1) throw std::runtime_error("precision must be 0 (FP64) or 1 (mixed)");
+  if (ncomp < 1 || ncomp > kMaxComps) throw std::runtime_error("ncomp must be 1.." + std::to_string(kMaxComps));
+  if (ncomp > 1 && precision != 0) throw std::runtime_error("ncomp > 1 is not supported in mixed precision");
   std::unique_ptr<amrmg_solver> s(new amrmg_solver());
   s->ctx = ctx;
   s->precision = precision;
+  s->ncomp = ncomp;
   Box dom;
   double dx0;
   std::vector<DBL> g = parse_spec(ispec, rspec, dom, s->bc, dx0, s->alpha, s->beta);
   s->H.define(ctx->c.get(), g, dom, dx0, true, &s->bc, ref_ratios);
-  s->fac.define(&s->H, s->alpha, s->beta, s->bc);
+  s->fac.define(&s->H, s->alpha, s->beta, s->bc, ncomp);
   s->bottom.reset(new DeviceBottomSolver<double>(ctx->c.get()));
   s->amg.define(s->fac, s->bottom.get(), s->nlev());
   s->amg.setSolverParameters(s->sp[0], s->sp[1], s->sp[2], s->sp[3], s->sp[4], s->rp[0], s->rp[1], s->rp[2]);
   s->amg.m_imin = s->sp[5];
   for (int l = 0; l < s->nlev(); l++) {
     const DeviceLayout* L = s->H.amrLay[l].get();
-    s->phi.emplace_back(new LevelData<double>(L));
-    s->rhs.emplace_back(new LevelData<double>(L));
-    s->res.emplace_back(new LevelData<double>(L));
-    s->uberRes.emplace_back(new LevelData<double>(L));
-    s->uberCorr.emplace_back(new LevelData<double>(L));
+    s->phi.emplace_back(new LevelData<double>(L, ncomp));
+    s->rhs.emplace_back(new LevelData<double>(L, ncomp));
+    s->res.emplace_back(new LevelData<double>(L, ncomp));
+    s->uberRes.emplace_back(new LevelData<double>(L, ncomp));
+    s->uberCorr.emplace_back(new LevelData<double>(L, ncomp));
   }
   auto ph = s->ptrs(s->phi), rh = s->ptrs(s->rhs);
   s->amg.init(ph, rh, s->nlev() - 1, 0);
@@ -309,6 +329,7 @@ int amrmg_solver_set_bc_values(amrmg_solver* s, amrmg_bc_value_fn fn, void* user
 int amrmg_solver_set_vc_coefs(amrmg_solver* s, const double* a, const double* b0, const double* b1,
                               const double* b2) {
   API_BEGIN
+  if (s->ncomp > 1) throw std::runtime_error("variable coefficients take one component (ncomp > 1 solver)");
   // the operators that carry coefficients: AMR levels 0..nlev-1, then the multigrid
   // depths 1.. of level 0 (depth 0 is AMR level 0)
   const int nd = s->H.mgDepth(), nl = s->nlev();
@@ -377,7 +398,7 @@ long long amrmg_solver_mg_cells(amrmg_solver* s, int depth) {
   return s->mglay(depth)->valid_cells;
 }
 long long amrmg_solver_device_bytes(amrmg_solver* s) {
-  long long total = hierarchy_field_bytes(s->H, s->mixed ? 1 : 0);
+  long long total = hierarchy_field_bytes(s->H, s->mixed ? 1 : 0) * s->ncomp;
   for (auto& v : s->vc) total += v->bytes();
   for (auto& v : s->vc32) total += v->bytes();
   return total;
@@ -394,8 +415,8 @@ int amrmg_solver_set_field(amrmg_solver* s, int field, const double* h) {
   auto& f = field_of(s, field);
   long long o = 0;
   for (int l = 0; l < s->nlev(); l++) {
-    upload_valid(*s->H.amrLay[l], f[l]->d, h + o);
-    o += s->H.amrLay[l]->valid_cells;
+    upload_valid(*s->H.amrLay[l], f[l]->d, h + o, f[l]->nc, f[l]->cs);
+    o += s->H.amrLay[l]->valid_cells * f[l]->nc;
   }
   API_END
 }
@@ -404,8 +425,8 @@ int amrmg_solver_get_field(amrmg_solver* s, int field, double* h) {
   auto& f = field_of(s, field);
   long long o = 0;
   for (int l = 0; l < s->nlev(); l++) {
-    download_valid(*s->H.amrLay[l], f[l]->d, h + o);
-    o += s->H.amrLay[l]->valid_cells;
+    download_valid(*s->H.amrLay[l], f[l]->d, h + o, f[l]->nc, f[l]->cs);
+    o += s->H.amrLay[l]->valid_cells * f[l]->nc;
   }
   API_END
 }
@@ -758,6 +779,7 @@ struct amrmg_heat {
 int amrmg_heat_create(amrmg_solver* s, int scheme, amrmg_heat** out) {
   API_BEGIN
   if (scheme != 0 && scheme != 1) throw std::runtime_error("heat scheme must be 0 (backward Euler) or 1 (TGA)");
+  if (s->ncomp > 1) throw std::runtime_error("the heat steppers take one component (ncomp > 1 solver)");
   std::unique_ptr<amrmg_heat> h(new amrmg_heat());
   h->s = s;
   h->scheme = scheme;

@@ -13,6 +13,7 @@
 
 #include <cstdint>
 #include <memory>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -173,19 +174,38 @@ struct DeviceLayout {
 
 enum Prec : int { PREC_F64 = 0, PREC_F32 = 1 };
 
-// A LevelData<FArrayBox> resident in HBM (ncomp = 1, ghost width 1).
+// A LevelData<FArrayBox> resident in HBM (ghost width 1) with nc components: each
+// component is one field of the layout's slab, the components cs elements apart (cs
+// is the slab rounded up to 64 elements, so every component keeps the slab's
+// alignment).  A component view (alias) is a one-component LevelData over one
+// component's storage that does not own it; comp says which component it is.
 template <class T>
 struct LevelData {
   const DeviceLayout* lay = nullptr;
   T* d = nullptr;
+  int nc = 1;
+  int comp = 0;
+  long long cs = 0;
+  bool own = true;
   LevelData() = default;
-  explicit LevelData(const DeviceLayout* l) { define(l); }
+  explicit LevelData(const DeviceLayout* l, int ncomp = 1) { define(l, ncomp); }
   ~LevelData() { release(); }
   LevelData(const LevelData&) = delete;
   LevelData& operator=(const LevelData&) = delete;
-  void define(const DeviceLayout* l);
+  void define(const DeviceLayout* l, int ncomp = 1);
+  void alias(const LevelData& x, int c) {
+    release();
+    if (x.d && (c < 0 || c >= x.nc)) throw std::runtime_error("LevelData: component out of range");
+    lay = x.lay;
+    d = x.d ? x.d + (long long)c * x.cs : nullptr;
+    nc = 1;
+    comp = c;
+    cs = x.cs;
+    own = false;
+  }
   void release();
   bool isDefined() const { return d != nullptr; }
+  static long long compStride(const DeviceLayout* l) { return (l->slab + 63) / 64 * 64; }
 };
 
 void hip_check(hipError_t e, const char* what);

@@ -274,21 +274,28 @@ DeviceLayout::~DeviceLayout() {
 }
 
 template <class T>
-void LevelData<T>::define(const DeviceLayout* l) {
+void LevelData<T>::define(const DeviceLayout* l, int ncomp) {
   release();
+  if (ncomp < 1) throw std::runtime_error("LevelData: ncomp must be at least 1");
   lay = l;
-  if (l->slab > 0 && !l->ctx->dry) {
-    hip_check(hipMalloc(&d, sizeof(T) * l->slab), "hipMalloc field");
-    hip_check(hipMemsetAsync(d, 0, sizeof(T) * l->slab, l->ctx->stream), "hipMemset field");
+  nc = ncomp;
+  comp = 0;
+  own = true;
+  cs = ncomp > 1 ? compStride(l) : l->slab;
+  const long long n = cs * ncomp;
+  if (n > 0 && !l->ctx->dry) {
+    hip_check(hipMalloc(&d, sizeof(T) * n), "hipMalloc field");
+    hip_check(hipMemsetAsync(d, 0, sizeof(T) * n, l->ctx->stream), "hipMemset field");
   }
 }
 template <class T>
 void LevelData<T>::release() {
-  if (d) {
+  if (d && own) {
     (void)hipStreamSynchronize(lay->ctx->stream);
     (void)hipFree(d);
   }
   d = nullptr;
+  own = true;
 }
 template struct LevelData<double>;
 template struct LevelData<float>;
@@ -458,8 +465,10 @@ __global__ void __launch_bounds__(256) k_fill_ghosts(const DBox* __restrict__ bo
   }
 }
 template <class T>
-void fill_ghosts(const DeviceLayout& L, const GhostPlan& P, T* phi, int doCF, int homog, double dx, double dxCrse) {
-  LAUNCH_P(KC_GHOST, 0.0, k_fill_ghosts<T>, P.nwork, L.d_boxes, P.d_patches, P.d_work, P.d_bcv, phi, doCF, homog, dx,
+void fill_ghosts(const DeviceLayout& L, const GhostPlan& P, T* phi, int doCF, int homog, double dx, double dxCrse,
+                 int comp) {
+  const double* bcv = P.d_bcv ? P.d_bcv + (long long)comp * P.nbcv : nullptr;
+  LAUNCH_P(KC_GHOST, 0.0, k_fill_ghosts<T>, P.nwork, L.d_boxes, P.d_patches, P.d_work, bcv, phi, doCF, homog, dx,
            dxCrse);
 }
 
@@ -1429,28 +1438,29 @@ void maxnorm_async(const DeviceLayout& L, const T* x, int slot, int accumulate) 
            L.ctx->d_scalars + slot);
 }
 template <class T>
-double maxnorm(const DeviceLayout& L, const T* x, int slot) {
-  maxnorm_async(L, x, slot, 0);
+double maxnorm(const DeviceLayout& L, const T* x, int slot, int nc, long long cs) {
+  for (int c = 0; c < nc; c++) maxnorm_async(L, x + c * cs, slot, c > 0 ? 1 : 0);
   hip_check(hipMemcpyAsync(L.ctx->h_scalars + slot, L.ctx->d_scalars + slot, sizeof(double), hipMemcpyDeviceToHost,
                            L.ctx->stream),
             "copy scalar");
   L.ctx->sync();
   return L.ctx->h_scalars[slot];
 }
-// LevelDataOps::dotProduct: per box a serial sum in Fortran order
-// (FArrayBox::dotProduct, ForAllThisCBNNXC), boxes summed in layout order.
+// LevelDataOps::dotProduct: per box a serial sum in Fortran order, components
+// outermost (FArrayBox::dotProduct, ForAllThisCBNNXC), boxes summed in layout order.
 template <class T>
 __global__ void __launch_bounds__(256) k_dot_ordered(const DBox* __restrict__ boxes, int nbox, const T* a, const T* b,
-                                                     double* partial, double* out) {
+                                                     double* partial, double* out, int nc, long long cs) {
   for (int q = threadIdx.x; q < nbox; q += blockDim.x) {
     const DBox bx = boxes[q];
     double dot = 0.0;
-    for (int k = bx.lo[2]; k <= bx.hi[2]; k++)
-      for (int j = bx.lo[1]; j <= bx.hi[1]; j++)
-        for (int i = bx.lo[0]; i <= bx.hi[0]; i++) {
-          const long long c = cidx(bx, i, j, k);
-          dot += (double)a[c] * (double)b[c];
-        }
+    for (int n = 0; n < nc; n++)
+      for (int k = bx.lo[2]; k <= bx.hi[2]; k++)
+        for (int j = bx.lo[1]; j <= bx.hi[1]; j++)
+          for (int i = bx.lo[0]; i <= bx.hi[0]; i++) {
+            const long long c = cidx(bx, i, j, k) + n * cs;
+            dot += (double)a[c] * (double)b[c];
+          }
     partial[q] = dot;
   }
   __syncthreads();
@@ -1471,7 +1481,7 @@ __device__ __forceinline__ double sumpow_term(T v) {
 }
 template <class T, bool ABS = false>
 __global__ void __launch_bounds__(256) k_sumpow_ordered(const DBox* __restrict__ boxes, int nbox, const T* x,
-                                                        double* out) {
+                                                        double* out, int nc, long long cs) {
   __shared__ double tmp[1024];
   double norm = 0.0;
   for (int q = 0; q < nbox; q++) {
@@ -1480,12 +1490,13 @@ __global__ void __launch_bounds__(256) k_sumpow_ordered(const DBox* __restrict__
     for (int i = threadIdx.x; i < nx; i += blockDim.x) {
       double t = 0.0;
       bool first = true;
-      for (int k = bx.lo[2]; k <= bx.hi[2]; k++)
-        for (int j = bx.lo[1]; j <= bx.hi[1]; j++) {
-          const double r = sumpow_term<ABS>(x[cidx(bx, bx.lo[0] + i, j, k)]);
-          if (first) { t = r; first = false; }
-          else t += r;
-        }
+      for (int n = 0; n < nc; n++)  // pencils: component, then z, then y (ForAllThisCPencil)
+        for (int k = bx.lo[2]; k <= bx.hi[2]; k++)
+          for (int j = bx.lo[1]; j <= bx.hi[1]; j++) {
+            const double r = sumpow_term<ABS>(x[cidx(bx, bx.lo[0] + i, j, k) + n * cs]);
+            if (first) { t = r; first = false; }
+            else t += r;
+          }
       tmp[i] = t;
     }
     __syncthreads();
@@ -1504,7 +1515,7 @@ __global__ void __launch_bounds__(256) k_sumpow_ordered(const DBox* __restrict__
 constexpr int kSumpowChains = 4096;
 template <class T, bool ABS = false>
 __global__ void __launch_bounds__(256) k_sumpow_chains(const DBox* __restrict__ boxes, int nbox, int W, const T* x,
-                                                       double* out) {
+                                                       double* out, int nc, long long cs) {
   __shared__ double tmp[kSumpowChains];
   for (int p = threadIdx.x; p < nbox * W; p += blockDim.x) {
     const int q = p / W, i = p % W;
@@ -1512,12 +1523,13 @@ __global__ void __launch_bounds__(256) k_sumpow_chains(const DBox* __restrict__ 
     if (i >= bx.hi[0] - bx.lo[0] + 1) continue;
     double t = 0.0;
     bool first = true;
-    for (int k = bx.lo[2]; k <= bx.hi[2]; k++)
-      for (int j = bx.lo[1]; j <= bx.hi[1]; j++) {
-        const double r = sumpow_term<ABS>(x[cidx(bx, bx.lo[0] + i, j, k)]);
-        if (first) { t = r; first = false; }
-        else t += r;
-      }
+    for (int n = 0; n < nc; n++)
+      for (int k = bx.lo[2]; k <= bx.hi[2]; k++)
+        for (int j = bx.lo[1]; j <= bx.hi[1]; j++) {
+          const double r = sumpow_term<ABS>(x[cidx(bx, bx.lo[0] + i, j, k) + n * cs]);
+          if (first) { t = r; first = false; }
+          else t += r;
+        }
     tmp[p] = t;
   }
   __syncthreads();
@@ -1545,10 +1557,10 @@ static double* scratch_partials(Context* ctx, int n) {
   return buf;
 }
 template <class T>
-double dot_ordered(const DeviceLayout& L, const T* a, const T* b, int slot) {
+double dot_ordered(const DeviceLayout& L, const T* a, const T* b, int slot, int nc, long long cs) {
   double* part = scratch_partials(L.ctx, L.nbox);
   hipLaunchKernelGGL(k_dot_ordered<T>, dim3(1), dim3(256), 0, L.ctx->stream, L.d_boxes, L.nbox, a, b,
-                     part, L.ctx->d_scalars + slot);
+                     part, L.ctx->d_scalars + slot, nc, cs);
   hip_check(hipGetLastError(), "k_dot_ordered");
   hip_check(hipMemcpyAsync(L.ctx->h_scalars + slot, L.ctx->d_scalars + slot, sizeof(double), hipMemcpyDeviceToHost,
                            L.ctx->stream),
@@ -1566,7 +1578,7 @@ double dot_ordered_noalloc(const DeviceLayout& L, const T* a, const T* b, int sl
                              " boxes, at most " + std::to_string(kNoAllocBoxes) + " are supported");
   double* part = device_scratch(L.ctx->device);
   hipLaunchKernelGGL(k_dot_ordered<T>, dim3(1), dim3(256), 0, L.ctx->stream, L.d_boxes, L.nbox, a, b, part,
-                     L.ctx->d_scalars + slot);
+                     L.ctx->d_scalars + slot, 1, 0LL);
   hip_check(hipGetLastError(), "k_dot_ordered");
   hip_check(hipMemcpyAsync(L.ctx->h_scalars + slot, L.ctx->d_scalars + slot, sizeof(double), hipMemcpyDeviceToHost,
                            L.ctx->stream),
@@ -1576,7 +1588,7 @@ double dot_ordered_noalloc(const DeviceLayout& L, const T* a, const T* b, int sl
 }
 template double dot_ordered_noalloc<double>(const DeviceLayout&, const double*, const double*, int);
 template <class T, bool ABS>
-static double sum_ordered(const DeviceLayout& L, const T* x, int slot) {
+static double sum_ordered(const DeviceLayout& L, const T* x, int slot, int nc, long long cs) {
   int W = 0;
   for (int q = 0; q < L.nbox; q++) {
     W = std::max(W, L.h_boxes[q].hi[0] - L.h_boxes[q].lo[0] + 1);
@@ -1584,11 +1596,11 @@ static double sum_ordered(const DeviceLayout& L, const T* x, int slot) {
   }
   if ((long long)L.nbox * W <= kSumpowChains) {
     hipLaunchKernelGGL((k_sumpow_chains<T, ABS>), dim3(1), dim3(256), 0, L.ctx->stream, L.d_boxes, L.nbox, W, x,
-                       L.ctx->d_scalars + slot);
+                       L.ctx->d_scalars + slot, nc, cs);
     hip_check(hipGetLastError(), "k_sumpow_chains");
   } else {
     hipLaunchKernelGGL((k_sumpow_ordered<T, ABS>), dim3(1), dim3(256), 0, L.ctx->stream, L.d_boxes, L.nbox, x,
-                       L.ctx->d_scalars + slot);
+                       L.ctx->d_scalars + slot, nc, cs);
     hip_check(hipGetLastError(), "k_sumpow_ordered");
   }
   hip_check(hipMemcpyAsync(L.ctx->h_scalars + slot, L.ctx->d_scalars + slot, sizeof(double), hipMemcpyDeviceToHost,
@@ -1598,12 +1610,12 @@ static double sum_ordered(const DeviceLayout& L, const T* x, int slot) {
   return L.ctx->h_scalars[slot];
 }
 template <class T>
-double norm2_ordered(const DeviceLayout& L, const T* x, int slot) {
-  return std::sqrt(sum_ordered<T, false>(L, x, slot));
+double norm2_ordered(const DeviceLayout& L, const T* x, int slot, int nc, long long cs) {
+  return std::sqrt(sum_ordered<T, false>(L, x, slot, nc, cs));
 }
 template <class T>
-double norm1_ordered(const DeviceLayout& L, const T* x, int slot) {
-  return sum_ordered<T, true>(L, x, slot);
+double norm1_ordered(const DeviceLayout& L, const T* x, int slot, int nc, long long cs) {
+  return sum_ordered<T, true>(L, x, slot, nc, cs);
 }
 
 // ---------------------------------------------------------------------------
@@ -1806,7 +1818,7 @@ void reflux(Context* ctx, const RefluxCell* cells, int ncells, const RefluxCoars
 // explicit instantiations
 // ---------------------------------------------------------------------------
 #define INST(T)                                                                                                   \
-  template void fill_ghosts<T>(const DeviceLayout&, const GhostPlan&, T*, int, int, double, double);           \
+  template void fill_ghosts<T>(const DeviceLayout&, const GhostPlan&, T*, int, int, double, double, int);      \
   template void gsrb<T>(const DeviceLayout&, T*, const T*, int, double, double, double);                       \
   template void gsmc<T>(const DeviceLayout&, T*, const T*, const int*, double, double, double);                \
   template void gsrb_region<T>(const DeviceLayout&, const Tile*, int, T*, const T*, const T*, int, int, int, int, bool, \
@@ -1839,11 +1851,11 @@ void reflux(Context* ctx, const RefluxCell* cells, int ncells, const RefluxCoars
   template void copy_all<T>(const DeviceLayout&, T*, const T*);                                                \
   template void precond_init<T>(const DeviceLayout&, T*, const T*, double);                                    \
   template void motion<T>(const MotionPlan&, const DeviceLayout&, const DeviceLayout&, T*, const T*, int);     \
-  template double maxnorm<T>(const DeviceLayout&, const T*, int);                                              \
+  template double maxnorm<T>(const DeviceLayout&, const T*, int, int, long long);                              \
   template void maxnorm_async<T>(const DeviceLayout&, const T*, int, int);                                      \
-  template double dot_ordered<T>(const DeviceLayout&, const T*, const T*, int);                                \
-  template double norm2_ordered<T>(const DeviceLayout&, const T*, int);                                        \
-  template double norm1_ordered<T>(const DeviceLayout&, const T*, int);                                        \
+  template double dot_ordered<T>(const DeviceLayout&, const T*, const T*, int, int, long long);                \
+  template double norm2_ordered<T>(const DeviceLayout&, const T*, int, int, long long);                        \
+  template double norm1_ordered<T>(const DeviceLayout&, const T*, int, int, long long);                        \
   template void cf_inhomog<T>(Context*, const CFCoarse*, int, const CFGen*, const CFFine*, int, T*, T*, const T*, \
                               double, double, int);                                                           \
   template void reflux<T>(Context*, const RefluxCell*, int, const RefluxCoarse*, const RefluxFine*, T*, const T*, \

@@ -25,6 +25,7 @@ struct GhostPlan {  // face-ghost fill rules of one level (exchange + BC + homog
   Patch* d_patches = nullptr;
   PatchWork* d_work = nullptr;
   double* d_bcv = nullptr;  // per-face-cell BC values (null: every rule uses its face constant)
+  long long nbcv = 0;       // values per component (component c's block starts at c * nbcv)
   int npatch = 0, nwork = 0;
   Context* owner = nullptr;
   std::vector<Patch> h_patches;
@@ -45,9 +46,10 @@ struct MotionPlan {  // Copier between two layouts
 
 // doCF: 0 exchange and BC rules, 1 exchange, BC and homogeneous coarse-fine, 2
 // coarse-fine only, 3 exchange only, 4 BC only
+// comp: the component phi is (selects its block of per-face-cell BC values)
 template <class T>
 void fill_ghosts(const DeviceLayout& L, const GhostPlan& P, T* phi, int doCF, int homog, double dx,
-                 double dxCrse);
+                 double dxCrse, int comp = 0);
 template <class T>
 void gsrb(const DeviceLayout& L, T* phi, const T* rhs, int color, double dx, double alpha, double beta);
 // One GSRB colour over a region of every box (tiles over each box), neighbours
@@ -141,21 +143,22 @@ template <class T>
 void motion(const MotionPlan& P, const DeviceLayout& Lsrc, const DeviceLayout& Ldst, T* dst, const T* src,
             int zero);
 
-// reductions; result left in ctx->d_scalars[slot], returned after a stream sync
+// reductions; result left in ctx->d_scalars[slot], returned after a stream sync.
+// nc components cs elements apart, taken in FArrayBox order (component outermost)
 template <class T>
-double maxnorm(const DeviceLayout& L, const T* x, int slot);
+double maxnorm(const DeviceLayout& L, const T* x, int slot, int nc = 1, long long cs = 0);
 template <class T>
-double dot_ordered(const DeviceLayout& L, const T* a, const T* b, int slot);
+double dot_ordered(const DeviceLayout& L, const T* a, const T* b, int slot, int nc = 1, long long cs = 0);
 // the same with the per-box partial sums in a file-scope device array (no device
 // allocation); levels of at most kNoAllocBoxes boxes
 constexpr int kNoAllocBoxes = 8192;
 template <class T>
 double dot_ordered_noalloc(const DeviceLayout& L, const T* a, const T* b, int slot);
 template <class T>
-double norm2_ordered(const DeviceLayout& L, const T* x, int slot);
+double norm2_ordered(const DeviceLayout& L, const T* x, int slot, int nc = 1, long long cs = 0);
 // norm(.., 1): the same per-box column sums of |x|, no root
 template <class T>
-double norm1_ordered(const DeviceLayout& L, const T* x, int slot);
+double norm1_ordered(const DeviceLayout& L, const T* x, int slot, int nc = 1, long long cs = 0);
 // asynchronous variant: max-norm into d_scalars[slot] without a sync (value read later)
 template <class T>
 void maxnorm_async(const DeviceLayout& L, const T* x, int slot, int accumulate);

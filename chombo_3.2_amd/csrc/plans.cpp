@@ -2,6 +2,8 @@
 // routine whose bookkeeping it reproduces.
 #include "plans.hpp"
 
+#include <algorithm>
+
 #include <cmath>
 #include <cstdint>
 #include <map>
@@ -152,27 +154,38 @@ ops::GhostPlan build_ghost_plan(const DBL& dbl, const Box& domain, const BCSpec&
 // ghost cell ivTo is the holder's at the centre of the domain face of the valid cell
 // next to it, getDomainFacePosition (BCFunc.cpp:187-202): dx*(iv+0.5) in every
 // direction, moved half a cell outward in dir.
-void attach_bc_values(ops::GhostPlan& P, double dx, const BCSpec& bc) {
+void attach_bc_values(ops::GhostPlan& P, double dx, const BCSpec& bc, int ncomp) {
   P.h_bcv.clear();
+  P.nbcv = 0;
+  if (ncomp < 1) throw std::runtime_error("BC values: ncomp must be at least 1");
+  long long n = 0;  // face cells with a value (per component)
   for (Patch& p : P.h_patches) {
     p.voff = -1;
     if (!bc.fn || (p.kind != PK_DIRICHLET && p.kind != PK_DIRICHLET2 && p.kind != PK_NEUMANN)) continue;
-    const long long n = (long long)(p.hi[0] - p.lo[0] + 1) * (p.hi[1] - p.lo[1] + 1) * (p.hi[2] - p.lo[2] + 1);
-    if ((long long)P.h_bcv.size() + n > (long long)INT32_MAX) throw std::runtime_error("BC values: too many face cells");
-    p.voff = (int)P.h_bcv.size();
+    const long long np = (long long)(p.hi[0] - p.lo[0] + 1) * (p.hi[1] - p.lo[1] + 1) * (p.hi[2] - p.lo[2] + 1);
+    if (n + np > (long long)INT32_MAX) throw std::runtime_error("BC values: too many face cells");
+    p.voff = (int)n;
+    n += np;
+  }
+  P.nbcv = n;
+  P.h_bcv.assign((size_t)(n * ncomp), 0.0);
+  std::vector<double> v((size_t)ncomp);
+  for (const Patch& p : P.h_patches) {
+    if (p.voff < 0) continue;
+    long long q = p.voff;
     const int isign = p.side ? 1 : -1;
     for (int k = p.lo[2]; k <= p.hi[2]; k++)
       for (int j = p.lo[1]; j <= p.hi[1]; j++)
-        for (int i = p.lo[0]; i <= p.hi[0]; i++) {
+        for (int i = p.lo[0]; i <= p.hi[0]; i++, q++) {
           int iv[3] = {i, j, k};
           iv[p.dir] -= isign;  // ivClose
           double pos[3];
           for (int d = 0; d < 3; d++) pos[d] = dx * (iv[d] + 0.5);
           pos[p.dir] += 0.5 * double(isign) * dx;
           int dir = p.dir, side = p.side;
-          double v = 0.0;
-          bc.fn(pos, &dir, &side, &v, bc.user);
-          P.h_bcv.push_back(v);
+          std::fill(v.begin(), v.end(), 0.0);
+          bc.fn(pos, &dir, &side, v.data(), bc.user);  // one value per component
+          for (int c = 0; c < ncomp; c++) P.h_bcv[(size_t)(c * n + q)] = v[(size_t)c];
         }
   }
 }

@@ -46,8 +46,10 @@ class BoxFinder {
 ops::GhostPlan build_ghost_plan(const DBL& dbl, const Box& domain, const BCSpec& bc, bool withCF, bool merged);
 
 // The face values of the plan's BC rules from bc.fn at level spacing dx (voff and
-// h_bcv; bc.fn null restores the face constants).  The plan must be uploaded again.
-void attach_bc_values(ops::GhostPlan& P, double dx, const BCSpec& bc);
+// h_bcv; bc.fn null restores the face constants).  bc.fn fills one value per
+// component; component c's values are the block at c * nbcv.  The plan must be
+// uploaded again.
+void attach_bc_values(ops::GhostPlan& P, double dx, const BCSpec& bc, int ncomp = 1);
 
 // Copier from src layout to dst layout (Copier::define with zero ghost).
 ops::MotionPlan build_copier(const DBL& src, const DBL& dst);

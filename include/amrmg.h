@@ -73,6 +73,19 @@ int amrmg_solver_create(amrmg_ctx* ctx, const int* ispec, const double* rspec, i
    relaxations per visit (one when its multigrid bottom has one cell), not m_pre / m_post. */
 int amrmg_solver_create_ratios(amrmg_ctx* ctx, const int* ispec, const double* rspec, const int* ref_ratios,
                                int precision, amrmg_solver** out);
+/* The same for fields of ncomp components (1..16): AMRPoissonOp on LevelData<FArrayBox>
+   with nComp = ncomp (AMRPoissonOp.cpp, every ChF kernel looping over components,
+   e.g. AMRPoissonOpF.ChF:158).  Every component shares the operator and the BC types;
+   ConstDiriNeumBC gives each component the same face value, a value function fills
+   one value per component (amrmg_solver_set_bc_values).  The solver's dot products and
+   norms run over all components in FArrayBox order (LevelDataOps::dotProduct,
+   norm(BoxLayoutData, p)), so the bottom BiCGStab couples the components as the
+   reference's does.  Fields are valid-packed box by box, each box component by
+   component.  FP64 (precision 0) and constant coefficients only; the level, FAB and
+   bottom entry points below take one component; the heat steppers, the batched and
+   serving paths take single-component problems. */
+int amrmg_solver_create_ncomp(amrmg_ctx* ctx, const int* ispec, const double* rspec, const int* ref_ratios,
+                              int ncomp, int precision, amrmg_solver** out);
 int amrmg_solver_destroy(amrmg_solver* s);
 int amrmg_solver_set_params(amrmg_solver* s, const int* sparams, const double* rparams);
 /* AMRPoissonOp::setAlphaAndBeta (AMRPoissonOp.cpp:1103-1108, unit aCoef and bCoef) on
@@ -89,7 +102,8 @@ typedef void (*amrmg_bc_value_fn)(double* pos, int* dir, int* side, double* valu
    outward (getDomainFacePosition, BCFunc.cpp:187-202, no origin offset).  fn is called
    here, once per face cell of every level and multigrid depth, and must be a pure
    function of its arguments; fn = NULL restores the constant values.  The BC types
-   stay those of ispec. */
+   stay those of ispec.  value holds one entry per component of the solver's fields
+   (BCValueFunc's a_values[icomp]). */
 int amrmg_solver_set_bc_values(amrmg_solver* s, amrmg_bc_value_fn fn, void* user);
 /* Variable coefficients: the solver's operator becomes VCAMRPoissonOp2's
    alpha a(x) phi - beta div(b(x) grad phi) (VCAMRPoissonOp2Factory::define,
@@ -113,7 +127,8 @@ long long amrmg_solver_mg_cells(amrmg_solver* s, int depth);
 /* bytes of HBM the solver holds (fields + tables) */
 long long amrmg_solver_device_bytes(amrmg_solver* s);
 
-/* field 0 = phi, 1 = rhs, 2 = residual (last computeAMRResidual) */
+/* field 0 = phi, 1 = rhs, 2 = residual (last computeAMRResidual); valid-packed level
+   after level, box after box, each box component by component */
 int amrmg_solver_set_field(amrmg_solver* s, int field, const double* valid_packed);
 int amrmg_solver_get_field(amrmg_solver* s, int field, double* valid_packed);
 
