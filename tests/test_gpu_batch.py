@@ -68,10 +68,11 @@ def test_batch_fp64_bitwise_equals_single_solves(gpu_ctx):
         assert it == info["iterations"] and ex == info["exit_status"], q
         np.testing.assert_array_equal(hist, hs)
         np.testing.assert_array_equal(phi, phis)
-    # and against the oracle directly on two of them
-    for q in (1, 4):
-        h, rhs = probs[q]
+    # and against the oracle directly on every one of them
+    for q, (h, rhs) in enumerate(probs):
         phi_o, hist_o, info_o = Oracle(h).solve(rhs, sp, rp)
+        assert res[q][1] == info_o["iterations"] and res[q][2] == info_o["exit_status"], q
+        np.testing.assert_array_equal(res[q][0], hist_o)
         np.testing.assert_array_equal(res[q][3], phi_o)
 
 
@@ -108,6 +109,10 @@ def test_batch_c4_sample_bitwise(gpu_ctx):
         hs, info, phis = _single(gpu_ctx, h, rhs, sp, rp)
         assert res[q][1] == info["iterations"], q
         np.testing.assert_array_equal(res[q][3], phis)
+        phi_o, hist_o, info_o = Oracle(h).solve(rhs, sp, rp)
+        assert res[q][1] == info_o["iterations"] and res[q][2] == info_o["exit_status"], q
+        np.testing.assert_array_equal(res[q][0], hist_o)
+        np.testing.assert_array_equal(res[q][3], phi_o)
 
 
 def test_batch_graph_replays_bitwise_equal_direct_launches(gpu_ctx):
