@@ -265,7 +265,12 @@ struct BottomArgs {
   int nbottom;                 // relax iterations before the Krylov solve
   int relaxFirst;              // 1: zero the correction and relax nbottom times first (the
                                // whole bottom of MultiGrid::cycle); 0: start from the correction
+  int ldsDoubles;              // dynamic LDS of the launch, in doubles: a problem whose ten
+                               // Krylov vectors fit keeps them there instead of in scratch
 };
+// dynamic LDS offered to a single solver's bottom Krylov vectors (the static LDS of
+// the kernel is at most 70 KB, the workgroup may hold 160 KB)
+constexpr int kBottomLdsDoubles = 8192;
 
 // A walk over the cells of a box of extents (ea, eb, *), a fastest, in steps of
 // `step` cells: the position (a, b, c) and its element offset are advanced with
@@ -671,7 +676,11 @@ __global__ void __launch_bounds__(kBottomThreads)
     if (L.ncell == 1) return;  // m_bottomCells == 1: relax only
   }
   const long long vs = 15 + (long long)L.R.sz * (L.nz + 2);  // elements per scratch vector
-  double* base = scratch + soff[p] + 15;                      // rectangle base of the first
+  // the ten vectors live in LDS when they fit there (every cell a vector is read at
+  // is written first in the solve, and its ghost cells are never used: the face
+  // rules replace them), else in the problem's global scratch
+  extern __shared__ double ldsVec[];
+  double* base = (10 * vs <= ba.ldsDoubles) ? ldsVec + 15 : scratch + soff[p] + 15;  // rectangle base of the first
   double *x = base, *b = base + vs, *r = base + 2 * vs, *r_tilde = base + 3 * vs, *e = base + 4 * vs,
          *pp = base + 5 * vs, *p_tilde = base + 6 * vs, *s_tilde = base + 7 * vs, *t = base + 8 * vs,
          *v = base + 9 * vs;
@@ -1334,7 +1343,9 @@ void BatchSolver::vcycle() {
     BATCH_LAUNCH(k_batch_restrict<T>, C.tileStart[np], 256, C.d_tiles, 6, 2, C.d_rect, F.d_rect, F.d_coef, d_active,
                  (T*)C.res, (const T*)F.corr, (const T*)F.res);
   }
-  BottomArgs ba{80, 1e-12, 1e-8, 1e-30, 5, m_nbottom, 1};
+  // Krylov vectors in global scratch: a batch's bottom time is set by its largest
+  // bottoms, which do not fit in LDS (measured no faster with the small ones there)
+  BottomArgs ba{80, 1e-12, 1e-8, 1e-30, 5, m_nbottom, 1, 0};
   if (m_treeOrder)
     BATCH_LAUNCH((k_batch_bottom<T, true>), m_np, kBottomThreads, d_bottom, m_bcoef, d_bbox, (T* const*)d_corrPtr,
                  (const T* const*)d_resPtr, d_bscratch, d_bscratchOff, d_metric, d_active, d_bstats, ba);
@@ -1627,6 +1638,7 @@ void DeviceBottomSolver<T>::define(LinearOp<LevelData<T>>* op, bool homogeneous)
       c.val[dir][side] = ap->m_bc.val[side][dir];
     }
   const long long vs = 15 + (long long)b.rect.sz * (b.rect.hi[2] - b.rect.lo[2] + 3);
+  m_vs = vs;
   m_ap = ap;
   m_coef = c;
   long long bytes = 0;
@@ -1685,9 +1697,12 @@ void DeviceBottomSolver<T>::solve(LevelData<T>& phi, const LevelData<T>& rhs) {
     hip_check(hipMemcpyAsync(d_bcoef, (char*)h_stage + 64, sizeof(BatchCoef), hipMemcpyHostToDevice, m_ctx->stream),
               "bottom coefficients");
   }
+  // the Krylov vectors in LDS when the level is small enough
+  const int lds = (10 * m_vs <= kBottomLdsDoubles) ? (int)(10 * m_vs) : 0;
   BottomArgs ba{m_fallback.m_imax, m_fallback.m_reps, m_fallback.m_hang, m_fallback.m_small,
-                m_fallback.m_numRestarts, 0, 0};
-  hipLaunchKernelGGL((k_batch_bottom<T, false>), dim3(1), dim3(kBottomThreads), 0, m_ctx->stream, (const BatchBottom*)d_bottom,
+                m_fallback.m_numRestarts, 0, 0, lds};
+  hipLaunchKernelGGL((k_batch_bottom<T, false>), dim3(1), dim3(kBottomThreads), sizeof(double) * lds, m_ctx->stream,
+                     (const BatchBottom*)d_bottom,
                      (const BatchCoef*)d_bcoef, (const DBox*)d_bbox, (T* const*)d_ptrs,
                      (const T* const*)((void**)d_ptrs + 1), d_scratch, (const long long*)d_soff,
                      (const double*)d_metric, (const int*)d_active, (int*)d_stats, ba);

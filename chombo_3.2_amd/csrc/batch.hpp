@@ -77,6 +77,7 @@ class DeviceBottomSolver : public LinearSolver<LevelData<T>> {
   const void* m_boundOp = nullptr;
   const void* m_ap = nullptr;     // the bound AMRPoissonOp<T>
   BatchCoef m_coef{};             // coefficients uploaded for it
+  long long m_vs = 0;             // elements per Krylov vector of the bound level
   double m_metric = -1.0, m_eps = 1e-6;
   void *d_bottom = nullptr, *d_bcoef = nullptr, *d_bbox = nullptr, *d_ptrs = nullptr, *d_soff = nullptr;
   void *d_active = nullptr, *d_stats = nullptr, *h_stage = nullptr;
