@@ -15,7 +15,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libamrmg.so")
+# another build of the same sources (the sanitizer build) may be named in the environment
+LIB_PATH = os.environ.get("AMRMG_LIB") or os.path.join(_HERE, "libamrmg.so")
 HEADER = os.path.join(os.path.dirname(_HERE), "include", "amrmg.h")
 
 _dp = C.POINTER(C.c_double)

@@ -651,7 +651,7 @@ static Plan plan_exchange_p(const Layout* L, const Box* domain, const int period
             if (keep) plan_push_sh(&p, f, t, r, sh);
           }
   }
-  qsort(p.m, p.n, sizeof(Motion), motion_cmp);
+  if (p.n > 1) qsort(p.m, p.n, sizeof(Motion), motion_cmp);
   return p;
 }
 
@@ -666,12 +666,12 @@ static Plan plan_copy(const Layout* src, const Layout* dst, int dst_ghost) {
       if (!b_empty(&r)) plan_push(&p, f, t, r);
     }
   }
-  qsort(p.m, p.n, sizeof(Motion), motion_cmp);
+  if (p.n > 1) qsort(p.m, p.n, sizeof(Motion), motion_cmp);
   return p;
 }
 static void plan_reverse(Plan* p) {
   for (int q = 0; q < p->n; q++) { int t = p->m[q].from; p->m[q].from = p->m[q].to; p->m[q].to = t; }
-  qsort(p->m, p->n, sizeof(Motion), motion_cmp);
+  if (p->n > 1) qsort(p->m, p->n, sizeof(Motion), motion_cmp);
 }
 static void exec_exchange(LD* x, const Plan* p) {
   if (x->nc > 1) { for (int c = 0; c < x->nc; c++) exec_exchange(COMP(x, c), p); return; }
@@ -2003,7 +2003,7 @@ OHier* oracle_create_r(const int* is, const double* rs, const int* ratios) {
     h->resCopier[l] = plan_copy(&h->cfL[l], &h->grids[l-1], 0);
     h->revCopier[l] = h->resCopier[l];
     h->revCopier[l].m = malloc(sizeof(Motion)*(h->resCopier[l].n ? h->resCopier[l].n : 1));
-    memcpy(h->revCopier[l].m, h->resCopier[l].m, sizeof(Motion)*h->resCopier[l].n);
+    if (h->resCopier[l].n) memcpy(h->revCopier[l].m, h->resCopier[l].m, sizeof(Motion)*h->resCopier[l].n);
     h->revCopier[l].cap = h->revCopier[l].n;
     plan_reverse(&h->revCopier[l]);
   }

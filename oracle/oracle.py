@@ -12,7 +12,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+# another build of the same source (the sanitizer build) may be named in the environment
+_LIB_PATH = os.environ.get("AMRMG_ORACLE_LIB") or os.path.join(_HERE, "_build", "liboracle.so")
 _lib = None
 
 _dp = C.POINTER(C.c_double)
