@@ -175,8 +175,8 @@ class MultiGrid:
     def init(self, e, residual):
         """MultiGrid::init (MultiGrid.H:534-548)."""
         self.close_fields()
-        self.m_residual = [None] + [self.m_op[i - 1].createCoarserField(None) for i in range(1, self.m_depth)]
-        self.m_correction = [None] + [self.m_op[i - 1].createCoarserField(None) for i in range(1, self.m_depth)]
+        self.m_residual = [None] + [self.m_op[i - 1].createCoarserField(residual) for i in range(1, self.m_depth)]
+        self.m_correction = [None] + [self.m_op[i - 1].createCoarserField(e) for i in range(1, self.m_depth)]
 
     def oneCycle(self, e, residual):
         self.cycle(0, e, residual)

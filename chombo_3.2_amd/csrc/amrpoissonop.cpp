@@ -65,6 +65,8 @@ void AMRPoissonOp<Real>::fillGhosts(T& phi, bool doCF, bool homogBC) {
     return;
   }
   need(phi.lay, m_lay, "fillGhosts");
+  if (!homogBC && m_ghost.d_bcv && phi.comp >= m_ncomp)
+    throw std::runtime_error("boundary values were taken for fewer components than the field has");
   ops::fill_ghosts<Real>(*m_lay, m_ghost, phi.d, doCF ? 1 : 0, homogBC ? 1 : 0, m_dx, m_dxCrse, phi.comp);
 }
 template <class Real>

@@ -1572,13 +1572,13 @@ double dot_ordered(const DeviceLayout& L, const T* a, const T* b, int slot, int 
 // memory is allocated or freed (the result goes through the context's scalar
 // slot and its pinned host mirror, both made when the context was created)
 template <class T>
-double dot_ordered_noalloc(const DeviceLayout& L, const T* a, const T* b, int slot) {
+double dot_ordered_noalloc(const DeviceLayout& L, const T* a, const T* b, int slot, int nc, long long cs) {
   if (L.nbox > kNoAllocBoxes)
     throw std::runtime_error("dot without allocation: the level has " + std::to_string(L.nbox) +
                              " boxes, at most " + std::to_string(kNoAllocBoxes) + " are supported");
   double* part = device_scratch(L.ctx->device);
   hipLaunchKernelGGL(k_dot_ordered<T>, dim3(1), dim3(256), 0, L.ctx->stream, L.d_boxes, L.nbox, a, b, part,
-                     L.ctx->d_scalars + slot, 1, 0LL);
+                     L.ctx->d_scalars + slot, nc, cs);
   hip_check(hipGetLastError(), "k_dot_ordered");
   hip_check(hipMemcpyAsync(L.ctx->h_scalars + slot, L.ctx->d_scalars + slot, sizeof(double), hipMemcpyDeviceToHost,
                            L.ctx->stream),
@@ -1586,7 +1586,7 @@ double dot_ordered_noalloc(const DeviceLayout& L, const T* a, const T* b, int sl
   L.ctx->sync();
   return L.ctx->h_scalars[slot];
 }
-template double dot_ordered_noalloc<double>(const DeviceLayout&, const double*, const double*, int);
+template double dot_ordered_noalloc<double>(const DeviceLayout&, const double*, const double*, int, int, long long);
 template <class T, bool ABS>
 static double sum_ordered(const DeviceLayout& L, const T* x, int slot, int nc, long long cs) {
   int W = 0;

@@ -57,7 +57,12 @@ const LevelData<double>& ld(const amrmg_field* f) {
 void same_level(const amrmg_field* a, const amrmg_field* b, const char* what) {
   if (!a || !b) throw std::runtime_error(std::string("null field in ") + what);
   if (a->lev != b->lev) throw std::runtime_error(std::string("fields on different levels in ") + what);
+  if (a->ld.nc != b->ld.nc) throw std::runtime_error(std::string("fields of different component counts in ") + what);
 }
+// component c of a field (its one-component storage)
+double* comp(amrmg_field* f, int c) { return f->ld.d + (long long)c * f->ld.cs; }
+const double* comp(const amrmg_field* f, int c) { return f->ld.d + (long long)c * f->ld.cs; }
+constexpr int kMaxFieldComps = 16;
 // an optional operand: NULL is the reference's undefined LevelData
 LevelData<double>& opt(amrmg_field* f, LevelData<double>& none) { return f ? f->ld : none; }
 AMRPoissonOp<double>& aop(amrmg_op* op) {
@@ -67,19 +72,24 @@ AMRPoissonOp<double>& aop(amrmg_op* op) {
 
 // FArrayBox-order transfers between host buffers (box by box, each box's valid box or
 // FAB in Fortran order) and a level's slab.
-void pack_transfer(const DeviceLayout& L, double* dev, double* host, bool ghosted, bool up) {
-  // the whole slab goes through the host: an upload keeps the cells it does not write
-  std::vector<double> slab((size_t)L.slab, 0.0);
+void pack_transfer(const DeviceLayout& L, double* dev, double* host, bool ghosted, bool up, int nc = 1,
+                   long long cs = 0) {
+  // the whole field goes through the host: an upload keeps the cells it does not write.
+  // Box by box, each box component by component (an FArrayBox's data order).
+  if (nc == 1) cs = L.slab;
+  const size_t n = (size_t)(cs * (nc - 1) + L.slab);
+  std::vector<double> slab(n, 0.0);
   L.ctx->sync();
-  hip_check(hipMemcpy(slab.data(), dev, sizeof(double) * L.slab, hipMemcpyDeviceToHost), "read field");
+  hip_check(hipMemcpy(slab.data(), dev, sizeof(double) * n, hipMemcpyDeviceToHost), "read field");
   long long o = 0;
-  for (int b = 0; b < L.dbl.size(); b++) {
+  for (int b = 0; b < L.dbl.size(); b++)
+  for (int m = 0; m < nc; m++) {
     const Box& v = L.dbl[b];
     const Box r = ghosted ? grow(v, 1) : v;
     for (int k = r.lo[2]; k <= r.hi[2]; k++)
       for (int j = r.lo[1]; j <= r.hi[1]; j++)
         for (int i = r.lo[0]; i <= r.hi[0]; i++, o++) {
-          const long long c = L.index(b, i, j, k);
+          const long long c = L.index(b, i, j, k) + m * cs;
           if (!up) {
             host[o] = slab[(size_t)c];
             continue;
@@ -97,7 +107,7 @@ void pack_transfer(const DeviceLayout& L, double* dev, double* host, bool ghoste
         }
   }
   if (up) {
-    hip_check(hipMemcpyAsync(dev, slab.data(), sizeof(double) * L.slab, hipMemcpyHostToDevice, L.ctx->stream),
+    hip_check(hipMemcpyAsync(dev, slab.data(), sizeof(double) * n, hipMemcpyHostToDevice, L.ctx->stream),
               "write field");
     L.ctx->sync();
   }
@@ -168,16 +178,19 @@ long long amrmg_level_cells(amrmg_level* L, int ghosted) {
 }
 
 // ---------------- fields ----------------
-int amrmg_field_create(amrmg_level* L, amrmg_field** out) {
+int amrmg_field_create(amrmg_level* L, amrmg_field** out) { return amrmg_field_create_ncomp(L, 1, out); }
+int amrmg_field_create_ncomp(amrmg_level* L, int ncomp, amrmg_field** out) {
   API_BEGIN
+  if (ncomp < 1 || ncomp > kMaxFieldComps) throw std::runtime_error("ncomp must be 1..16");
   std::unique_ptr<amrmg_field> f(new amrmg_field());
   f->lev = L;
-  f->ld.define(L->lay.get());
-  hip_check(hipMemsetAsync(f->ld.d, 0, sizeof(double) * L->lay->slab, L->ctx->c->stream), "zero field");
+  f->ld.define(L->lay.get(), ncomp);
+  hip_check(hipMemsetAsync(f->ld.d, 0, sizeof(double) * f->ld.cs * ncomp, L->ctx->c->stream), "zero field");
   L->refs++;
   *out = f.release();
   API_END
 }
+int amrmg_field_ncomp(const amrmg_field* f) { return f ? f->ld.nc : -1; }
 int amrmg_field_destroy(amrmg_field* f) {
   API_BEGIN
   if (!f) return 0;
@@ -188,61 +201,61 @@ int amrmg_field_destroy(amrmg_field* f) {
 }
 int amrmg_field_upload(amrmg_field* f, const double* host, int ghosted) {
   API_BEGIN
-  pack_transfer(lay(f), f->ld.d, const_cast<double*>(host), ghosted != 0, true);
+  pack_transfer(lay(f), f->ld.d, const_cast<double*>(host), ghosted != 0, true, f->ld.nc, f->ld.cs);
   API_END
 }
 int amrmg_field_download(amrmg_field* f, double* host, int ghosted) {
   API_BEGIN
-  pack_transfer(lay(f), f->ld.d, host, ghosted != 0, false);
+  pack_transfer(lay(f), f->ld.d, host, ghosted != 0, false, f->ld.nc, f->ld.cs);
   API_END
 }
 int amrmg_field_assign(amrmg_field* lhs, const amrmg_field* rhs) {
   API_BEGIN
   same_level(lhs, rhs, "assign");
-  ops::copy_all<double>(lay(lhs), lhs->ld.d, rhs->ld.d);
+  for (int c = 0; c < lhs->ld.nc; c++) ops::copy_all<double>(lay(lhs), comp(lhs, c), comp(rhs, c));
   API_END
 }
 int amrmg_field_incr(amrmg_field* lhs, const amrmg_field* x, double scale) {
   API_BEGIN
   same_level(lhs, x, "incr");
-  ops::incr<double>(lay(lhs), lhs->ld.d, x->ld.d, scale);
+  for (int c = 0; c < lhs->ld.nc; c++) ops::incr<double>(lay(lhs), comp(lhs, c), comp(x, c), scale);
   API_END
 }
 int amrmg_field_axby(amrmg_field* lhs, const amrmg_field* x, const amrmg_field* y, double a, double b) {
   API_BEGIN
   same_level(lhs, x, "axby");
   same_level(lhs, y, "axby");
-  ops::axby<double>(lay(lhs), lhs->ld.d, x->ld.d, y->ld.d, a, b);
+  for (int c = 0; c < lhs->ld.nc; c++) ops::axby<double>(lay(lhs), comp(lhs, c), comp(x, c), comp(y, c), a, b);
   API_END
 }
 int amrmg_field_scale(amrmg_field* x, double s) {
   API_BEGIN
-  ops::scale<double>(lay(x), ld(x).d, s);
+  for (int c = 0; c < ld(x).nc; c++) ops::scale<double>(lay(x), comp(x, c), s);
   API_END
 }
 int amrmg_field_setval(amrmg_field* x, double v) {
   API_BEGIN
-  ops::setval<double>(lay(x), ld(x).d, v);
+  for (int c = 0; c < ld(x).nc; c++) ops::setval<double>(lay(x), comp(x, c), v);
   API_END
 }
 int amrmg_field_dot(const amrmg_field* a, const amrmg_field* b, double* out) {
   API_BEGIN
   same_level(a, b, "dot");
-  *out = ops::dot_ordered<double>(lay(a), a->ld.d, b->ld.d, 1);
+  *out = ops::dot_ordered<double>(lay(a), a->ld.d, b->ld.d, 1, a->ld.nc, a->ld.cs);
   API_END
 }
 int amrmg_field_dot_noalloc(const amrmg_field* a, const amrmg_field* b, double* out) {
   API_BEGIN
   same_level(a, b, "dot");
-  *out = ops::dot_ordered_noalloc<double>(lay(a), a->ld.d, b->ld.d, 1);
+  *out = ops::dot_ordered_noalloc<double>(lay(a), a->ld.d, b->ld.d, 1, a->ld.nc, a->ld.cs);
   API_END
 }
 int amrmg_field_norm(const amrmg_field* x, int p, double* out) {
   API_BEGIN
   const DeviceLayout& L = lay(x);
-  if (p == 0) *out = ops::maxnorm<double>(L, x->ld.d, 2);
-  else if (p == 1) *out = ops::norm1_ordered<double>(L, x->ld.d, 3);
-  else if (p == 2) *out = ops::norm2_ordered<double>(L, x->ld.d, 3);
+  if (p == 0) *out = ops::maxnorm<double>(L, x->ld.d, 2, x->ld.nc, x->ld.cs);
+  else if (p == 1) *out = ops::norm1_ordered<double>(L, x->ld.d, 3, x->ld.nc, x->ld.cs);
+  else if (p == 2) *out = ops::norm2_ordered<double>(L, x->ld.d, 3, x->ld.nc, x->ld.cs);
   else throw std::runtime_error("norm: p must be 0, 1 or 2");
   API_END
 }
@@ -314,6 +327,7 @@ int amrmg_op_create_coarser(amrmg_op* op, amrmg_op** out, amrmg_level** level_ou
     h->lev = h->ownLevel.get();
   }
   h->op.reset(new AMRPoissonOp<double>());
+  h->op->m_ncomp = p.m_ncomp;
   // MGnewOp: 2 dx, the AMR level's dxCrse, the same coefficients and BC
   defineLevelOp<double>(h->op.get(), op->ctx->c.get(), h->lev->lay.get(), h->lev->lay->domain, 2 * p.m_dx, p.m_dxCrse,
                         p.m_alpha, p.m_beta, p.m_bc, nullptr);
@@ -489,14 +503,17 @@ int amrmg_op_exchange(amrmg_op* op, amrmg_field* phi) {
   API_BEGIN
   AMRPoissonOp<double>& p = aop(op);
   if (phi->ld.lay != p.m_lay) throw std::runtime_error("exchange: field on another level");
-  ops::fill_ghosts<double>(*p.m_lay, p.m_ghost, phi->ld.d, 3, 1, p.m_dx, p.m_dxCrse);
+  for (int c = 0; c < phi->ld.nc; c++) ops::fill_ghosts<double>(*p.m_lay, p.m_ghost, comp(phi, c), 3, 1, p.m_dx, p.m_dxCrse, c);
   API_END
 }
 int amrmg_op_bc(amrmg_op* op, amrmg_field* phi, int homogeneous) {
   API_BEGIN
   AMRPoissonOp<double>& p = aop(op);
   if (phi->ld.lay != p.m_lay) throw std::runtime_error("bc: field on another level");
-  ops::fill_ghosts<double>(*p.m_lay, p.m_ghost, phi->ld.d, 4, homogeneous != 0, p.m_dx, p.m_dxCrse);
+  if (phi->ld.nc > p.m_ncomp && !homogeneous && p.m_ghost.d_bcv)
+    throw std::runtime_error("bc: boundary values were taken for fewer components (amrmg_op_set_ncomp)");
+  for (int c = 0; c < phi->ld.nc; c++)
+    ops::fill_ghosts<double>(*p.m_lay, p.m_ghost, comp(phi, c), 4, homogeneous != 0, p.m_dx, p.m_dxCrse, c);
   API_END
 }
 int amrmg_op_set_bc_values(amrmg_op* op, amrmg_bc_value_fn fn, void* user) {
@@ -504,11 +521,19 @@ int amrmg_op_set_bc_values(amrmg_op* op, amrmg_bc_value_fn fn, void* user) {
   setLevelOpBCValues<double>(op->op.get(), fn, user);
   API_END
 }
+int amrmg_op_set_ncomp(amrmg_op* op, int ncomp) {
+  API_BEGIN
+  if (ncomp < 1 || ncomp > kMaxFieldComps) throw std::runtime_error("ncomp must be 1..16");
+  AMRPoissonOp<double>& p = aop(op);
+  p.m_ncomp = ncomp;
+  if (p.m_bc.fn) setLevelOpBCValues<double>(&p, p.m_bc.fn, p.m_bc.user);  // one value block per component
+  API_END
+}
 int amrmg_op_cf_homog(amrmg_op* op, amrmg_field* phi) {
   API_BEGIN
   AMRPoissonOp<double>& p = aop(op);
   if (phi->ld.lay != p.m_lay) throw std::runtime_error("homogeneousCFInterp: field on another level");
-  ops::fill_ghosts<double>(*p.m_lay, p.m_ghost, phi->ld.d, 2, 1, p.m_dx, p.m_dxCrse);
+  for (int c = 0; c < phi->ld.nc; c++) ops::fill_ghosts<double>(*p.m_lay, p.m_ghost, comp(phi, c), 2, 1, p.m_dx, p.m_dxCrse, c);
   API_END
 }
 int amrmg_op_cf_inhomog(amrmg_op* op, amrmg_field* phi, const amrmg_field* phi_coarse) {
@@ -522,7 +547,9 @@ int amrmg_op_gsrb(amrmg_op* op, amrmg_field* phi, const amrmg_field* rhs, int co
   AMRPoissonOp<double>& p = aop(op);
   if (phi->ld.lay != p.m_lay || rhs->ld.lay != p.m_lay) throw std::runtime_error("gsrb: field on another level");
   if (color != 0 && color != 1) throw std::runtime_error("gsrb: colour must be 0 or 1");
-  ops::gsrb<double>(*p.m_lay, phi->ld.d, rhs->ld.d, color, p.m_dx, p.m_alpha, p.m_beta);
+  if (phi->ld.nc != rhs->ld.nc) throw std::runtime_error("gsrb: fields of different component counts");
+  for (int c = 0; c < phi->ld.nc; c++)
+    ops::gsrb<double>(*p.m_lay, comp(phi, c), comp(rhs, c), color, p.m_dx, p.m_alpha, p.m_beta);
   API_END
 }
 int amrmg_op_reflux(amrmg_op* op, amrmg_field* phi_fine, amrmg_field* phi, amrmg_field* lphi, amrmg_op* finer) {
@@ -581,9 +608,12 @@ int amrmg_plan_exec(amrmg_plan* p, amrmg_field* dst, const amrmg_field* src, int
   if (!dst || dst->lev != p->dst) throw std::runtime_error("copier: destination field on another level");
   if (op == 0) {
     if (!src || src->lev != p->src) throw std::runtime_error("copier: source field on another level");
-    if (p->P.nitems) ops::motion<double>(p->P, *p->src->lay, *p->dst->lay, dst->ld.d, src->ld.d, 0);
+    if (src->ld.nc != dst->ld.nc) throw std::runtime_error("copier: fields of different component counts");
+    for (int c = 0; c < dst->ld.nc && p->P.nitems; c++)
+      ops::motion<double>(p->P, *p->src->lay, *p->dst->lay, comp(dst, c), comp(src, c), 0);
   } else if (op == 1) {
-    if (p->P.nitems) ops::motion<double>(p->P, *p->src->lay, *p->dst->lay, dst->ld.d, (const double*)nullptr, 1);
+    for (int c = 0; c < dst->ld.nc && p->P.nitems; c++)
+      ops::motion<double>(p->P, *p->src->lay, *p->dst->lay, comp(dst, c), (const double*)nullptr, 1);
   } else {
     throw std::runtime_error("copier op must be 0 (copyTo) or 1 (copyToZero)");
   }

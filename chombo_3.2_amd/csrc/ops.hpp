@@ -153,7 +153,7 @@ double dot_ordered(const DeviceLayout& L, const T* a, const T* b, int slot, int 
 // allocation); levels of at most kNoAllocBoxes boxes
 constexpr int kNoAllocBoxes = 8192;
 template <class T>
-double dot_ordered_noalloc(const DeviceLayout& L, const T* a, const T* b, int slot);
+double dot_ordered_noalloc(const DeviceLayout& L, const T* a, const T* b, int slot, int nc = 1, long long cs = 0);
 template <class T>
 double norm2_ordered(const DeviceLayout& L, const T* x, int slot, int nc = 1, long long cs = 0);
 // norm(.., 1): the same per-box column sums of |x|, no root

@@ -34,6 +34,8 @@ def _lib():
             "amrmg_level_destroy": [_vp],
             "amrmg_level_coarsen": [_vp, _pp],
             "amrmg_field_create": [_vp, _pp],
+            "amrmg_field_create_ncomp": [_vp, C.c_int, _pp],
+            "amrmg_op_set_ncomp": [_vp, C.c_int],
             "amrmg_field_destroy": [_vp],
             "amrmg_field_upload": [_vp, _dp, C.c_int],
             "amrmg_field_download": [_vp, _dp, C.c_int],
@@ -131,23 +133,25 @@ class Level:
 
 
 class Field:
-    """LevelData<FArrayBox>(level, 1 component, one ghost cell), zero on creation."""
+    """LevelData<FArrayBox>(level, ncomp components, one ghost cell), zero on creation;
+    host data box by box, each box component by component."""
 
-    def __init__(self, level: Level):
+    def __init__(self, level: Level, ncomp: int = 1):
         self.level = level
+        self.ncomp = int(ncomp)
         p = C.c_void_p()
-        _chk(_lib().amrmg_field_create(level.p, C.byref(p)))
+        _chk(_lib().amrmg_field_create_ncomp(level.p, self.ncomp, C.byref(p)))
         self.p = p
 
     def upload(self, data, ghosted=False):
         a = np.ascontiguousarray(data, dtype=np.float64)
-        if a.size != self.level.cells(ghosted):
+        if a.size != self.level.cells(ghosted) * self.ncomp:
             raise ValueError("field upload: size does not match the level")
         _chk(_lib().amrmg_field_upload(self.p, a.ctypes.data_as(_dp), 1 if ghosted else 0))
         return self
 
     def download(self, ghosted=False):
-        out = np.zeros(self.level.cells(ghosted))
+        out = np.zeros(self.level.cells(ghosted) * self.ncomp)
         _chk(_lib().amrmg_field_download(self.p, out.ctypes.data_as(_dp), 1 if ghosted else 0))
         return out
 
@@ -248,7 +252,7 @@ class AMRPoissonOp:
 
     # ---- LinearOp ----
     def create(self, like: Field) -> Field:
-        return Field(like.level)
+        return Field(like.level, like.ncomp)
 
     def residual(self, lhs, phi, rhs, homogeneous=False):
         _chk(_lib().amrmg_op_residual(self.p, lhs.p, phi.p, rhs.p, int(homogeneous)))
@@ -286,7 +290,7 @@ class AMRPoissonOp:
 
     # ---- MGLevelOp ----
     def createCoarserField(self, fine: Field) -> Field:
-        return Field(self.mgChild.level)
+        return Field(self.mgChild.level, fine.ncomp if fine is not None else 1)
 
     def relax(self, correction, residual, iterations):
         _chk(_lib().amrmg_op_relax(self.p, correction.p, residual.p, int(iterations)))
@@ -335,7 +339,7 @@ class AMRPoissonOp:
         _chk(_lib().amrmg_op_amr_update_residual(self.p, residual.p, correction.p, _h(coarseCorrection)))
 
     def createCoarsened(self, rhs: Field, refRat=2) -> Field:
-        return Field(self.coarsenedLevel())
+        return Field(self.coarsenedLevel(), rhs.ncomp)
 
     def assignCopier(self, lhs, rhs):
         _chk(_lib().amrmg_op_assign_copier(self.p, lhs.p, rhs.p))
@@ -354,6 +358,10 @@ class AMRPoissonOp:
 
     def applyBC(self, phi, homogeneous):
         _chk(_lib().amrmg_op_bc(self.p, phi.p, int(homogeneous)))
+
+    def setNComp(self, ncomp):
+        """Components of the fields this operator's boundary value function serves."""
+        _chk(_lib().amrmg_op_set_ncomp(self.p, int(ncomp)))
 
     def setBCValues(self, fn):
         """This operator's inhomogeneous BC values from fn(pos, dir, side) at the domain

@@ -349,6 +349,13 @@ long long amrmg_level_cells(amrmg_level* L, int ghosted);
 
 /* LevelData<FArrayBox>(grids, 1, IntVect::Unit), every cell zero */
 int amrmg_field_create(amrmg_level* L, amrmg_field** out);
+/* LevelData<FArrayBox>(grids, ncomp, IntVect::Unit), ncomp 1..16, every cell zero.
+   Transfers move box by box, each box component by component; the LevelDataOps
+   calls act on every component, dot products and norms run over all of them in
+   FArrayBox order; operators and copiers apply to each component (fields of one
+   operation must have the same component count). */
+int amrmg_field_create_ncomp(amrmg_level* L, int ncomp, amrmg_field** out);
+int amrmg_field_ncomp(const amrmg_field* f);
 int amrmg_field_destroy(amrmg_field* f);
 /* FArrayBox-order transfers, box by box in layout order: ghosted = 0 moves each
    box's valid cells (Fortran order), 1 each box's FAB (the box grown by one cell;
@@ -438,6 +445,10 @@ int amrmg_op_bc(amrmg_op* op, amrmg_field* phi, int homogeneous);
 /* this operator's inhomogeneous BC values from fn (as amrmg_solver_set_bc_values; a
    multigrid child made afterwards by amrmg_op_create_coarser inherits fn) */
 int amrmg_op_set_bc_values(amrmg_op* op, amrmg_bc_value_fn fn, void* user);
+/* components of the fields this operator's inhomogeneous boundary values serve
+   (default 1): a value function fills one value per component, taken again here when
+   one is set.  An inhomogeneous ghost fill of a component beyond it is an error. */
+int amrmg_op_set_ncomp(amrmg_op* op, int ncomp);
 int amrmg_op_cf_homog(amrmg_op* op, amrmg_field* phi);
 int amrmg_op_cf_inhomog(amrmg_op* op, amrmg_field* phi, const amrmg_field* phi_coarse);
 int amrmg_op_gsrb(amrmg_op* op, amrmg_field* phi, const amrmg_field* rhs, int color);
