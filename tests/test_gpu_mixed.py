@@ -56,7 +56,9 @@ def test_mixed_precision_solve_matches_fp64_oracle(gpu_ctx, case):
     # the initial residual is computed in FP64 from the same data: identical
     assert hist[0] == hist_o[0]
     assert info["exit_status"] & 1 and hist[-1] <= 1e-10 * hist[0]
-    assert abs(info["iterations"] - info_o["iterations"]) <= 2
+    # the FP32 inner cycles reduce the FP64 residual as fast as FP64 cycles at this
+    # tolerance: the same number of outer iterations (measured on every case here)
+    assert info["iterations"] == info_o["iterations"]
     assert err < MIXED_TOL, err
     g.close()
 
