@@ -18,7 +18,12 @@
   num_pre / num_post / num_bottom; imin is AMRMultiGrid's default, 5.
 * ``gaussians_rhs``: setRHS with the driver's default problem type (three Gaussian
   sources, :169-222).
-* ``run``: runSolver (:588-633): ``main.iterations`` solves from a zero initial guess.
+* ``run``: runSolver (:588-633): ``main.iterations`` solves from a zero initial guess;
+  with ``main.writePlotFiles`` (default true) the plotfile of :640-696, components phi,
+  rhs and res (the composite residual) written by WriteAMRHierarchyHDF5 with dt and time
+  1, as ``poissonOut.3d.hdf5`` (chombo_hdf5.py, no HDF5 library needed).
+
+Run as a program: ``python inputs.py <inputs file> [key=value ...]``.
 """
 from __future__ import annotations
 
@@ -155,10 +160,10 @@ def gaussians_rhs(h: H.Hierarchy) -> np.ndarray:
     return np.concatenate(parts)
 
 
-def run(pp: ParmParse, ctx, rhs: Optional[np.ndarray] = None):
+def run(pp: ParmParse, ctx, rhs: Optional[np.ndarray] = None, plotfile: Optional[str] = None):
     """runSolver: the hierarchy, solver and rhs the inputs describe, then
-    ``main.iterations`` solves from zero.  Returns (hierarchy, phi, history, info) of
-    the last solve."""
+    ``main.iterations`` solves from zero, and the plotfile when ``plotfile`` is given.
+    Returns (hierarchy, phi, history, info) of the last solve."""
     import amrmg
 
     h = setup_grids(pp)
@@ -173,7 +178,36 @@ def run(pp: ParmParse, ctx, rhs: Optional[np.ndarray] = None):
         for _ in range(max(1, pp.query("main.iterations", int, 1))):
             hist, info = s.solve(zero_phi=True)
         phi = s.get_field(s.PHI)
+        if plotfile:
+            import chombo_hdf5 as CH
+
+            res, _ = s.computeAMRResidual()
+            CH.write_amr_hierarchy(plotfile, CH.file_from_hierarchy(h, {"phi": phi, "rhs": rhs, "res": res},
+                                                                    dt=1.0, time=1.0))
     finally:
         s.close()
     return h, phi, hist, info
 
+
+def main(argv: Sequence[str]) -> int:
+    import amrmg
+
+    if len(argv) < 1:
+        print("usage: inputs.py <inputs file> [key=value ...]")
+        return 2
+    pp = ParmParse.from_file(argv[0], argv[1:])
+    plot = "poissonOut.3d.hdf5" if pp.query("main.writePlotFiles", bool, True) else None
+    ctx = amrmg.Context(0)
+    try:
+        h, phi, hist, info = run(pp, ctx, plotfile=plot)
+    finally:
+        ctx.close()
+    print(f"{h.nlev} levels, {h.total_cells()} cells: {info['iterations']} iterations, exit status "
+          f"{info['exit_status']}, residual {hist[0]:.3e} -> {hist[-1]:.3e}" + (f", wrote {plot}" if plot else ""))
+    return 0 if info["exit_status"] & 1 else 1
+
+
+if __name__ == "__main__":
+    import sys
+
+    sys.exit(main(sys.argv[1:]))

@@ -24,3 +24,20 @@ def test_inputs_run_matches_oracle(gpu_ctx, argv):
     assert info["exit_status"] & 1
     np.testing.assert_array_equal(hist, hist_o)
     np.testing.assert_array_equal(phi, phi_o)
+
+
+def test_inputs_plotfile_roundtrip(gpu_ctx, tmp_path):
+    """The driver's plotfile (phi, rhs, res) read back: phi and rhs as solved, res the
+    composite residual, below the tolerance."""
+    import chombo_hdf5 as CH
+
+    pp = I.ParmParse.from_file(FIXTURE, ("grids.read_in_grids = true",))
+    out = str(tmp_path / "poissonOut.3d.hdf5")
+    h, phi, hist, info = I.run(pp, gpu_ctx, plotfile=out)
+    f = CH.read_amr_hierarchy(out)
+    assert f.names == ["phi", "rhs", "res"]
+    np.testing.assert_array_equal(CH.field_from_file(f, 0), phi)
+    np.testing.assert_array_equal(CH.field_from_file(f, 1), I.gaussians_rhs(h))
+    res = CH.field_from_file(f, 2)
+    assert np.abs(res).max() <= 1e-9 * hist[0]
+
