@@ -671,6 +671,22 @@ void AMRPoissonOp<Real>::AMRResidualNF(T& res, T& phi, T& phiCoarse, const T& rh
   if (phiCoarse.isDefined()) coarseFineInterp(phi, phiCoarse);
   residualI(res, phi, rhs, homog);
 }
+template <class Real>
+void AMRPoissonOp<Real>::residualNorm(T& lhs, T& phi, const T& rhs, bool homog, int slot, bool accumulate) {
+  if (!canFuseNorm(lhs)) throw std::runtime_error("residualNorm: one component, constant coefficients");
+  need(lhs.lay, m_lay, "residual"); need(rhs.lay, m_lay, "residual");
+  fillGhosts(phi, homog, homog);
+  ops::residual_norm<Real>(*m_lay, lhs.d, phi.d, rhs.d, m_dx, m_alpha, m_beta, slot, accumulate ? 1 : 0);
+}
+template <class Real>
+void AMRPoissonOp<Real>::AMRResidualNFNorm(T& res, T& phi, T& phiCoarse, const T& rhs, bool homog, int slot,
+                                          bool accumulate) {
+  if (!canFuseNorm(res)) throw std::runtime_error("AMRResidualNFNorm: one component, constant coefficients");
+  if (phiCoarse.isDefined()) coarseFineInterp(phi, phiCoarse);
+  need(res.lay, m_lay, "residualI"); need(rhs.lay, m_lay, "residualI");
+  fillGhosts(phi, false, homog);
+  ops::residual_norm<Real>(*m_lay, res.d, phi.d, rhs.d, m_dx, m_alpha, m_beta, slot, accumulate ? 1 : 0);
+}
 // AMROperator (:790-815)
 template <class Real>
 void AMRPoissonOp<Real>::AMROperator(T& LofPhi, T& phiFine, T& phi, T& phiCoarse, bool homog,
@@ -1249,6 +1265,15 @@ template <class Real>
 double AMRMultiGrid<Real>::computeAMRResidual(std::vector<T*>& resid, std::vector<T*>& phi, std::vector<T*>& rhs,
                                               int l_max, int l_base, bool homog, bool computeNorm) {
   for (int ilev = l_base; ilev <= l_max; ilev++) {
+    // the finest level's residual and its max-norm in one pass (the same values)
+    if (computeNorm && ilev == l_max && m_op[ilev]->canFuseNorm(*resid[ilev])) {
+      if (l_max == l_base && l_base == 0)
+        m_op[ilev]->residualNorm(*resid[ilev], *phi[ilev], *rhs[ilev], homog, 0, ilev != l_base);
+      else
+        m_op[ilev]->AMRResidualNFNorm(*resid[ilev], *phi[ilev], *phi[ilev - 1], *rhs[ilev], homog, 0,
+                                      ilev != l_base);
+      continue;
+    }
     computeAMRResidualLevel(resid, phi, rhs, l_max, l_base, ilev, homog);
     if (computeNorm) {
       if (ilev != l_max) m_op[ilev]->zeroCovered(*resid[ilev]);

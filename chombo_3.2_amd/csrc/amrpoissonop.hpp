@@ -132,6 +132,12 @@ class AMRPoissonOp : public AMRLevelOp<LevelData<Real>> {
   void reflux(T& phiFine, T& phi, T& res, AMRPoissonOp* finerOp);
   void residualFused(T& res, T& phiFine, T& phi, const T& rhs, bool homog, AMRPoissonOp* finerOp);
   void localMaxNormAsync(const T& x, int slot, bool accumulate);
+  // residual (homog CF fill: residual's) or AMRResidualNF with the max-norm of the
+  // result folded into the context's scalar slot, as localMaxNormAsync would add it
+  // (one component, constant coefficients)
+  bool canFuseNorm(const T& res) const { return res.nc == 1 && !m_vc; }
+  void residualNorm(T& lhs, T& phi, const T& rhs, bool homog, int slot, bool accumulate);
+  void AMRResidualNFNorm(T& res, T& phi, T& phiCoarse, const T& rhs, bool homog, int slot, bool accumulate);
   // Autotuner: times the one-launch relaxation's tile shapes on (phi, rhs) of this
   // level and keeps the fastest.  No-op (returns false) when the level relaxes
   // with the four-launch sequence.  times: kFusedShapeCount x 4 ms values.

@@ -959,27 +959,45 @@ __global__ void __launch_bounds__(256) k_residual(const DBox* __restrict__ boxes
   const T lap = (phi[c + 1] + phi[c - 1] + phi[c + sy] + phi[c - sy] + phi[c + sz] + phi[c - sz] - T(6.0) * p) * dxinv;
   res[c] = -(alpha * p) - beta * lap + rhs[c];
 }
-// the same over a merged level's column tiles, marching in z (as k_apply_z)
+// the same over a merged level's column tiles, marching in z (as k_apply_z).
+// nrm (optional): the max-norm of what is written is folded into *nrm as k_maxnorm
+// folds it (per-block max, then an atomic max of the non-negative bit pattern, skipped
+// when the block's max does not exceed the value already there)
 template <class T>
 __global__ void __launch_bounds__(256) k_residual_z(const DBox* __restrict__ boxes, const Tile* __restrict__ ctiles,
                                                     int nct, int kz, T* res, const T* __restrict__ phi, const T* rhs,
-                                                    T dxinv, T alpha, T beta) {
+                                                    T dxinv, T alpha, T beta, double* nrm) {
   const int w = xcd_contiguous_block();
-  if (w >= nct) return;
+  if (w >= nct) return;  // uniform over the block
   const Tile t = ctiles[w];
   const DBox b = boxes[t.box];
   const int i = t.i0 + (threadIdx.x & 63), j = t.j0 + (threadIdx.x >> 6);
-  if (i > b.hi[0] || j > b.hi[1]) return;
-  const int kend = min(t.k0 + kz - 1, b.hi[2]);
-  const long long sy = b.sy, sz = b.sz;
-  long long c = cidx(b, i, j, t.k0);
-  T pm = phi[c - sz], p = phi[c];
-  for (int k = t.k0; k <= kend; k++, c += sz) {
-    const T pz = phi[c + sz];
-    const T lap = (phi[c + 1] + phi[c - 1] + phi[c + sy] + phi[c - sy] + pz + pm - T(6.0) * p) * dxinv;
-    res[c] = -(alpha * p) - beta * lap + rhs[c];
-    pm = p;
-    p = pz;
+  double v = 0.0;
+  if (i <= b.hi[0] && j <= b.hi[1]) {
+    const int kend = min(t.k0 + kz - 1, b.hi[2]);
+    const long long sy = b.sy, sz = b.sz;
+    long long c = cidx(b, i, j, t.k0);
+    T pm = phi[c - sz], p = phi[c];
+    for (int k = t.k0; k <= kend; k++, c += sz) {
+      const T pz = phi[c + sz];
+      const T lap = (phi[c + 1] + phi[c - 1] + phi[c + sy] + phi[c - sy] + pz + pm - T(6.0) * p) * dxinv;
+      const T r = -(alpha * p) - beta * lap + rhs[c];
+      res[c] = r;
+      v = fmax(v, fabs((double)r));
+      pm = p;
+      p = pz;
+    }
+  }
+  if (nrm) {  // uniform over the launch
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    __shared__ double sm[4];
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const double m = fmax(fmax(sm[0], sm[1]), fmax(sm[2], sm[3]));
+      if (m > 0.0 && m > *(volatile double*)nrm)
+        atomicMax((unsigned long long*)nrm, (unsigned long long)__double_as_longlong(m));
+    }
   }
 }
 template <class T>
@@ -987,7 +1005,8 @@ void residual(const DeviceLayout& L, T* res, const T* phi, const T* rhs, double 
   double dxinv = 1.0 / (dx * dx);
   if (L.merged && L.nctiles > 0) {
     LAUNCH_P(KC_RESIDUAL, 3.0 * sizeof(T) * L.valid_cells, k_residual_z<T>, (L.nctiles + 7) / 8 * 8, L.d_boxes,
-             L.d_ctiles, L.nctiles, DeviceLayout::kColumnPlanes, res, phi, rhs, T(dxinv), T(alpha), T(beta));
+             L.d_ctiles, L.nctiles, DeviceLayout::kColumnPlanes, res, phi, rhs, T(dxinv), T(alpha), T(beta),
+             (double*)nullptr);
     return;
   }
   LAUNCH_P(KC_RESIDUAL, 3.0 * sizeof(T) * L.valid_cells, k_residual<T>, L.ntiles, L.d_boxes, L.d_tiles, geom(L), res, phi, rhs, T(dxinv), T(alpha), T(beta));
@@ -1438,6 +1457,21 @@ void maxnorm_async(const DeviceLayout& L, const T* x, int slot, int accumulate) 
            L.ctx->d_scalars + slot);
 }
 template <class T>
+void residual_norm(const DeviceLayout& L, T* res, const T* phi, const T* rhs, double dx, double alpha, double beta,
+                   int slot, int accumulate) {
+  if (!(L.merged && L.nctiles > 0)) {
+    residual<T>(L, res, phi, rhs, dx, alpha, beta);
+    maxnorm_async<T>(L, res, slot, accumulate);
+    return;
+  }
+  if (!accumulate) LAUNCH(k_zero_scalar, 1, L.ctx->d_scalars, slot);
+  const double dxinv = 1.0 / (dx * dx);
+  LAUNCH_P(KC_RESIDUAL, 3.0 * sizeof(T) * L.valid_cells, k_residual_z<T>, (L.nctiles + 7) / 8 * 8, L.d_boxes,
+           L.d_ctiles, L.nctiles, DeviceLayout::kColumnPlanes, res, phi, rhs, T(dxinv), T(alpha), T(beta),
+           L.ctx->d_scalars + slot);
+}
+
+template <class T>
 double maxnorm(const DeviceLayout& L, const T* x, int slot, int nc, long long cs) {
   for (int c = 0; c < nc; c++) maxnorm_async(L, x + c * cs, slot, c > 0 ? 1 : 0);
   hip_check(hipMemcpyAsync(L.ctx->h_scalars + slot, L.ctx->d_scalars + slot, sizeof(double), hipMemcpyDeviceToHost,
@@ -1830,6 +1864,7 @@ void reflux(Context* ctx, const RefluxCell* cells, int ncells, const RefluxCoars
                                                   double, double, double, double*);                          \
   template void apply<T>(const DeviceLayout&, T*, const T*, double, double, double);                           \
   template void residual<T>(const DeviceLayout&, T*, const T*, const T*, double, double, double);              \
+  template void residual_norm<T>(const DeviceLayout&, T*, const T*, const T*, double, double, double, int, int); \
   template void restrict_res<T>(const DeviceLayout&, const DeviceLayout&, T*, const T*, const T*, double, double, \
                                 double);                                                                      \
   template void prolong<T>(const DeviceLayout&, const DeviceLayout&, T*, const T*, int);                       \

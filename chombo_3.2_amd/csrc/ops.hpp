@@ -92,6 +92,11 @@ void apply(const DeviceLayout& L, T* lofphi, const T* phi, double dx, double alp
 template <class T>
 void residual(const DeviceLayout& L, T* res, const T* phi, const T* rhs, double dx, double alpha,
               double beta);
+// residual, with its max-norm folded into ctx->d_scalars[slot] (accumulate 0: the slot
+// is cleared first), as maxnorm_async on the result would
+template <class T>
+void residual_norm(const DeviceLayout& L, T* res, const T* phi, const T* rhs, double dx, double alpha, double beta,
+                   int slot, int accumulate);
 template <class T>
 void restrict_res(const DeviceLayout& Lf, const DeviceLayout& Lc, T* resc, const T* phi, const T* rhs,
                   double dx, double alpha, double beta);
